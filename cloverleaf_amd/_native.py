@@ -1,0 +1,296 @@
+"""ctypes binding of the C ABI in include/cloverleaf_ep.h (libcloverleaf_ep.so).
+
+The library is built in-tree by ``cloverleaf_amd.build_native`` (or
+``__graft_entry__.build()``).  There is no fallback: if the library is missing
+or fails to load, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, Structure, c_char_p, c_double, c_float, c_int, c_int32, c_uint16, c_uint32, c_uint64, c_void_p
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libcloverleaf_ep.so"
+
+EP_SAMPLE_ALL, EP_SAMPLE_FIXED = 0, 1
+EP_LW_NONE, EP_LW_LOG, EP_LW_EXPONENTIAL = 0, 1, 2
+EP_PREC_FP32, EP_PREC_BF16 = 0, 1
+EP_GRAD_FIRST_LEAF, EP_GRAD_EXACT = 0, 1
+EP_SCATTER_SORT, EP_SCATTER_ATOMIC = 0, 1
+
+ERROR_NAMES = {-1: "EP_EINVAL", -2: "EP_ESTATE", -3: "EP_EHIP", -4: "EP_ENOMEM", -5: "EP_ECAPACITY"}
+
+
+class EPConfig(Structure):
+    _fields_ = [
+        ("dims", c_uint32),
+        ("batch_size", c_uint32),
+        ("passes", c_uint32),
+        ("negatives", c_uint32),
+        ("alpha", c_float),
+        ("gamma", c_float),
+        ("valid_pct", c_float),
+        ("max_nodes_kind", c_int32),
+        ("max_nodes", c_uint32),
+        ("max_features_kind", c_int32),
+        ("max_features", c_uint32),
+        ("loss_weighting", c_int32),
+        ("loss_weight_exp", c_float),
+        ("hard_negatives", c_uint32),
+        ("noise", c_float),
+        ("weighted_neighbor_sampling", c_int32),
+        ("weighted_neighbor_averaging", c_int32),
+        ("precision", c_int32),
+        ("grad_mode", c_int32),
+        ("scatter", c_int32),
+        ("use_graph", c_int32),
+        ("seed", c_uint64),
+        ("device", c_int32),
+    ]
+
+
+class EPBenchResult(Structure):
+    _fields_ = [
+        ("ms_total", c_double),
+        ("ms_forward_avg", c_double),
+        ("ms_scatter_avg", c_double),
+        ("examples", c_uint64),
+        ("rows", c_uint64),
+        ("nodes", c_uint64),
+        ("unique_rows", c_uint64),
+        ("steps", c_uint32),
+        ("forward_launches", c_uint32),
+    ]
+
+
+PROGRESS_FN = ctypes.CFUNCTYPE(None, c_void_p, c_uint32, c_uint64, c_float, c_float, c_float)
+
+_u32p = POINTER(c_uint32)
+_u64p = POINTER(c_uint64)
+_f32p = POINTER(c_float)
+_u16p = POINTER(c_uint16)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "ep_config_default": (c_int, [POINTER(EPConfig)]),
+    "ep_create": (c_int, [POINTER(EPConfig), POINTER(c_void_p)]),
+    "ep_destroy": (None, [c_void_p]),
+    "ep_set_graph": (c_int, [c_void_p, c_uint64, _u64p, _u32p]),
+    "ep_set_features": (c_int, [c_void_p, c_uint64, c_uint32, _u64p, _u32p]),
+    "ep_set_table": (c_int, [c_void_p, _f32p]),
+    "ep_get_table": (c_int, [c_void_p, _f32p]),
+    "ep_get_table_bf16": (c_int, [c_void_p, _u16p]),
+    "ep_get_adam": (c_int, [c_void_p, _f32p, _f32p]),
+    "ep_set_split": (c_int, [c_void_p, _u32p, c_uint64, _u32p, c_uint64]),
+    "ep_learn": (c_int, [c_void_p, PROGRESS_FN, c_void_p]),
+    "ep_sample_batch": (c_int, [c_void_p, _u32p, c_uint32, c_uint32, _u32p, _u32p, _u32p]),
+    "ep_step_explicit": (c_int, [c_void_p, _u32p, c_uint32, _u32p, _u32p, _u32p, c_uint32, c_float, c_int,
+                                 _f32p, _f32p, _u32p, _u32p, _f32p]),
+    "ep_bench_steps": (c_int, [c_void_p, c_uint32, c_uint32, c_int, POINTER(EPBenchResult)]),
+    "ep_embed_averaged": (c_int, [c_void_p, c_uint64, _u64p, _u32p, _f32p]),
+    "ep_philox4x32": (c_int, [_u32p, _u32p, _u32p]),
+    "ep_last_error": (c_char_p, []),
+}
+
+_lib = None
+
+
+class NativeError(RuntimeError):
+    """A nonzero return code of the C ABI."""
+
+
+def load(path: os.PathLike | None = None) -> ctypes.CDLL:
+    """Loads libcloverleaf_ep.so (once) and declares every exported signature."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path is not None else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"{p} is missing: build it with `python -m cloverleaf_amd.build_native` "
+            "(or __graft_entry__.build()); there is no CPU fallback")
+    lib = ctypes.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        msg = load().ep_last_error().decode(errors="replace")
+        raise NativeError(f"{what or 'ep call'} failed ({ERROR_NAMES.get(rc, rc)}): {msg}")
+
+
+def ptr(a: np.ndarray, ctype):
+    """Pointer to a C-contiguous numpy array (None passes through)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return a.ctypes.data_as(POINTER(ctype))
+
+
+def default_config() -> EPConfig:
+    cfg = EPConfig()
+    check(load().ep_config_default(ctypes.byref(cfg)), "ep_config_default")
+    return cfg
+
+
+class Context:
+    """Owning handle of an ``ep_ctx`` (device memory + stream)."""
+
+    def __init__(self, cfg: EPConfig):
+        lib = load()
+        h = c_void_p()
+        check(lib.ep_create(ctypes.byref(cfg), ctypes.byref(h)), "ep_create")
+        self._h = h
+        self.cfg = cfg
+        self.n_vocab = 0
+        self.n_nodes = 0
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise RuntimeError("context is closed")
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            load().ep_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- uploads ---------------------------------------------------------------
+    def set_graph(self, indptr: np.ndarray, indices: np.ndarray):
+        indptr = np.ascontiguousarray(indptr, dtype=np.uint64)
+        indices = np.ascontiguousarray(indices, dtype=np.uint32)
+        n = len(indptr) - 1
+        check(load().ep_set_graph(self.handle, n, ptr(indptr, c_uint64), ptr(indices, c_uint32)), "ep_set_graph")
+        self.n_nodes = n
+
+    def set_features(self, f_indptr: np.ndarray, f_ids: np.ndarray, n_vocab: int):
+        f_indptr = np.ascontiguousarray(f_indptr, dtype=np.uint64)
+        f_ids = np.ascontiguousarray(f_ids, dtype=np.uint32)
+        n = len(f_indptr) - 1
+        check(load().ep_set_features(self.handle, n, int(n_vocab), ptr(f_indptr, c_uint64), ptr(f_ids, c_uint32)),
+              "ep_set_features")
+        self.n_vocab = int(n_vocab)
+
+    def set_table(self, rows: np.ndarray | None):
+        if rows is None:
+            check(load().ep_set_table(self.handle, None), "ep_set_table")
+            return
+        rows = np.ascontiguousarray(rows, dtype=np.float32)
+        assert rows.shape == (self.n_vocab, self.cfg.dims), rows.shape
+        check(load().ep_set_table(self.handle, ptr(rows, c_float)), "ep_set_table")
+
+    def get_table(self) -> np.ndarray:
+        out = np.empty((self.n_vocab, self.cfg.dims), dtype=np.float32)
+        check(load().ep_get_table(self.handle, ptr(out, c_float)), "ep_get_table")
+        return out
+
+    def get_table_bf16(self) -> np.ndarray:
+        out = np.empty((self.n_vocab, self.cfg.dims), dtype=np.uint16)
+        check(load().ep_get_table_bf16(self.handle, ptr(out, c_uint16)), "ep_get_table_bf16")
+        return out
+
+    def get_adam(self):
+        m = np.empty((self.n_vocab, self.cfg.dims), dtype=np.float32)
+        v = np.empty_like(m)
+        check(load().ep_get_adam(self.handle, ptr(m, c_float), ptr(v, c_float)), "ep_get_adam")
+        return m, v
+
+    def set_split(self, train: np.ndarray, valid: np.ndarray):
+        train = np.ascontiguousarray(train, dtype=np.uint32)
+        valid = np.ascontiguousarray(valid, dtype=np.uint32)
+        check(load().ep_set_split(self.handle, ptr(train, c_uint32), len(train), ptr(valid, c_uint32), len(valid)),
+              "ep_set_split")
+
+    # -- training ----------------------------------------------------------------
+    def learn(self, progress=None):
+        if progress is None:
+            cb = PROGRESS_FN(0)
+        else:
+            def _cb(_user, p, step, tr, va, lr):
+                progress(int(p), int(step), float(tr), float(va), float(lr))
+            cb = PROGRESS_FN(_cb)
+        check(load().ep_learn(self.handle, cb, None), "ep_learn")
+
+    def sample_batch(self, anchors: np.ndarray, pass_: int):
+        anchors = np.ascontiguousarray(anchors, dtype=np.uint32)
+        n = len(anchors)
+        S, K = self.cfg.max_nodes, self.cfg.negatives
+        cnt = np.zeros(n, dtype=np.uint32)
+        nbr = np.zeros((n, S), dtype=np.uint32)
+        neg = np.zeros((n, K), dtype=np.uint32)
+        check(load().ep_sample_batch(self.handle, ptr(anchors, c_uint32), n, int(pass_), ptr(cnt, c_uint32),
+                                     ptr(nbr, c_uint32), ptr(neg, c_uint32)), "ep_sample_batch")
+        return cnt, nbr, neg
+
+    def step_explicit(self, anchors, nbr_cnt, nbr, neg, pass_: int, lr: float, apply_update: bool = True,
+                      want_cons: bool = False):
+        """One step on explicit samples.  Returns dict(loss, cons, feat, grad)."""
+        anchors = np.ascontiguousarray(anchors, dtype=np.uint32)
+        n = len(anchors)
+        S, K, D = self.cfg.max_nodes, self.cfg.negatives, self.cfg.dims
+        nbr_cnt = np.ascontiguousarray(nbr_cnt, dtype=np.uint32)
+        nbr = np.ascontiguousarray(nbr, dtype=np.uint32).reshape(n, S)
+        neg = np.ascontiguousarray(neg, dtype=np.uint32).reshape(n, K)
+        loss = np.zeros(n, dtype=np.float32)
+        cons = np.zeros((n, K + 2, D), dtype=np.float32) if want_cons else None
+        nu = c_uint32(0)
+        cap = self.n_vocab
+        feat = np.zeros(cap, dtype=np.uint32)
+        grad = np.zeros((cap, D), dtype=np.float32)
+        check(load().ep_step_explicit(self.handle, ptr(anchors, c_uint32), n, ptr(nbr_cnt, c_uint32),
+                                      ptr(nbr, c_uint32), ptr(neg, c_uint32), int(pass_), float(lr),
+                                      1 if apply_update else 0, ptr(loss, c_float), ptr(cons, c_float),
+                                      ctypes.byref(nu), ptr(feat, c_uint32), ptr(grad, c_float)),
+              "ep_step_explicit")
+        k = nu.value
+        return {"loss": loss, "cons": cons, "feat": feat[:k].copy(), "grad": grad[:k].copy()}
+
+    def bench_steps(self, warmup: int, steps: int, instrument: bool = True) -> EPBenchResult:
+        out = EPBenchResult()
+        check(load().ep_bench_steps(self.handle, int(warmup), int(steps), 1 if instrument else 0, ctypes.byref(out)),
+              "ep_bench_steps")
+        return out
+
+    def embed_averaged(self, f_indptr: np.ndarray, f_ids: np.ndarray) -> np.ndarray:
+        f_indptr = np.ascontiguousarray(f_indptr, dtype=np.uint64)
+        f_ids = np.ascontiguousarray(f_ids, dtype=np.uint32)
+        n = len(f_indptr) - 1
+        out = np.empty((n, self.cfg.dims), dtype=np.float32)
+        check(load().ep_embed_averaged(self.handle, n, ptr(f_indptr, c_uint64), ptr(f_ids, c_uint32),
+                                       ptr(out, c_float)), "ep_embed_averaged")
+        return out
+
+
+def philox4x32(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    check(load().ep_philox4x32(ptr(c, c_uint32), ptr(k, c_uint32), ptr(out, c_uint32)), "ep_philox4x32")
+    return out
+
+
+def exported_symbols():
+    return list(SIGNATURES.keys())

@@ -1,0 +1,579 @@
+"""Python facade with the reference's EP API surface (src/lib.rs PyO3 classes).
+
+Class and argument names, defaults and side effects follow the reference:
+
+* ``EmbeddingPropagator`` / ``learn_features``   src/lib.rs:1119-1377
+* ``EPLoss``, ``LossWeighting``                  src/lib.rs:939-1110
+* ``Graph``, ``GraphBuilder``, ``EdgeType``      src/lib.rs:292-502, 836-937
+* ``FeatureSet``                                 src/lib.rs:1432-1691, src/feature_store.rs
+* ``NodeEmbeddings``, ``Distance``               src/lib.rs:2535-2894
+* ``FeatureAggregator``, ``NodeEmbedder``        src/lib.rs:1766-2207 (Averaged only)
+
+The training step itself runs in libcloverleaf_ep.so (HIP kernels on the GPU);
+this module only converts names to integer CSR arrays and back.
+"""
+from __future__ import annotations
+
+import math
+import sys
+from typing import Dict, Iterable, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _native as N
+
+FQNode = Tuple[str, str]
+SEED = 20222022  # src/lib.rs:90
+
+
+# --------------------------------------------------------------------------------------
+# Vocabulary (src/vocab.rs): (node_type, name) <-> dense id, in insertion order
+# --------------------------------------------------------------------------------------
+class Vocab:
+    def __init__(self, names: Optional[Iterable[FQNode]] = None):
+        self.names: List[FQNode] = []
+        self.index: Dict[FQNode, int] = {}
+        for nm in names or ():
+            self.get_or_insert(*nm)
+
+    def get_or_insert(self, node_type: str, name: str) -> int:
+        key = (str(node_type), str(name))
+        i = self.index.get(key)
+        if i is None:
+            i = len(self.names)
+            self.names.append(key)
+            self.index[key] = i
+        return i
+
+    def get(self, node: FQNode) -> Optional[int]:
+        return self.index.get((str(node[0]), str(node[1])))
+
+    def name(self, i: int) -> FQNode:
+        return self.names[i]
+
+    def __len__(self):
+        return len(self.names)
+
+    @classmethod
+    def anonymous(cls, n: int, node_type: str = "node") -> "Vocab":
+        v = cls()
+        v.names = [(node_type, str(i)) for i in range(n)]
+        v.index = {k: i for i, k in enumerate(v.names)}
+        return v
+
+
+def _node_id(vocab: Vocab, node: FQNode) -> int:
+    i = vocab.get(node)
+    if i is None:
+        raise KeyError(f"Node {node} does not exist!")
+    return i
+
+
+class EdgeType:
+    Directed = "Directed"
+    Undirected = "Undirected"
+
+
+class Distance:
+    Cosine = "Cosine"
+    Euclidean = "Euclidean"
+    Dot = "Dot"
+    ALT = "ALT"
+    Hamming = "Hamming"
+    Jaccard = "Jaccard"
+
+
+# --------------------------------------------------------------------------------------
+# Graph
+# --------------------------------------------------------------------------------------
+class Graph:
+    """CSR graph with CDF edge weights (CumCSR, src/graph.rs:399-550)."""
+
+    def __init__(self, indptr: np.ndarray, indices: np.ndarray, weights_cdf: Optional[np.ndarray], vocab: Vocab):
+        self.indptr = np.ascontiguousarray(indptr, dtype=np.uint64)
+        self.indices = np.ascontiguousarray(indices, dtype=np.uint32)
+        self.weights = None if weights_cdf is None else np.ascontiguousarray(weights_cdf, dtype=np.float32)
+        self.vocab_ = vocab
+
+    @classmethod
+    def from_csr(cls, indptr, indices, weights=None, vocab: Optional[Vocab] = None) -> "Graph":
+        """Builds a Graph from integer CSR arrays (node i named ("node", str(i)) by default).
+
+        ``weights`` are raw edge weights; they are stored as per-row CDFs like CumCSR::convert.
+        """
+        n = len(indptr) - 1
+        cdf = None
+        if weights is not None:
+            cdf = _rows_to_cdf(np.asarray(indptr, dtype=np.int64), np.asarray(weights, dtype=np.float32))
+        return cls(indptr, indices, cdf, vocab or Vocab.anonymous(n))
+
+    def nodes(self) -> int:
+        return len(self.indptr) - 1
+
+    def edges(self) -> int:
+        return int(self.indptr[-1])
+
+    def contains_node(self, name: FQNode) -> bool:
+        return self.vocab_.get(name) is not None
+
+    def degree_array(self) -> np.ndarray:
+        return np.diff(self.indptr.astype(np.int64))
+
+    def get_edges(self, node: FQNode, normalized: Optional[bool] = None):
+        i = _node_id(self.vocab_, node)
+        s, e = int(self.indptr[i]), int(self.indptr[i + 1])
+        names = [self.vocab_.name(int(j)) for j in self.indices[s:e]]
+        w = self.weights[s:e] if self.weights is not None else _uniform_cdf(e - s)
+        if normalized:
+            w = np.concatenate([w[:1], np.diff(w)]) if len(w) else w
+        return names, [float(x) for x in w]
+
+    def vocab(self):
+        return iter(self.vocab_.names)
+
+    def __len__(self):
+        return self.nodes()
+
+    def __repr__(self):
+        return f"Graph<Nodes={self.nodes()}, Edges={self.edges()}>"
+
+
+def _uniform_cdf(k: int) -> np.ndarray:
+    if k == 0:
+        return np.zeros(0, dtype=np.float32)
+    w = np.minimum(np.cumsum(np.ones(k, dtype=np.float32)) / np.float32(k), np.float32(1.0))
+    w[-1] = 1.0
+    return w
+
+
+def _rows_to_cdf(indptr: np.ndarray, weights: np.ndarray) -> np.ndarray:
+    """convert_edges_to_cdf per row (src/graph.rs:664-691), float32 like the reference."""
+    out = weights.astype(np.float32).copy()
+    for i in range(len(indptr) - 1):
+        s, e = int(indptr[i]), int(indptr[i + 1])
+        if s == e:
+            continue
+        w = out[s:e]
+        denom = np.float32(w.sum(dtype=np.float32))
+        if denom == 0:
+            w[:] = 1.0
+            denom = np.float32(e - s)
+        acc = np.cumsum(w, dtype=np.float32)
+        w[:] = np.minimum(acc / denom, np.float32(1.0))
+        w[-1] = 1.0
+    return out
+
+
+class GraphBuilder:
+    """GraphBuilder (src/lib.rs:850-937)."""
+
+    def __init__(self):
+        self.vocab = Vocab()
+        self.edges_: List[Tuple[int, int, float]] = []
+
+    def __repr__(self):
+        return f"GraphBuilder<Nodes={len(self.vocab)}, Edges={len(self.edges_)}>"
+
+    def add_edge(self, from_node: FQNode, to_node: FQNode, weight: float, node_type: str):
+        f = self.vocab.get_or_insert(*from_node)
+        t = self.vocab.get_or_insert(*to_node)
+        self.edges_.append((f, t, float(weight)))
+        if node_type == EdgeType.Undirected:
+            self.edges_.append((t, f, float(weight)))
+
+    def build_graph(self, deduplicate: Optional[bool] = None) -> Optional[Graph]:
+        """CSR::construct_from_edges (src/graph.rs:70-142) + CumCSR::convert (414-422)."""
+        if not self.edges_:
+            return None
+        vocab, edges = self.vocab, self.edges_
+        self.vocab, self.edges_ = Vocab(), []
+        dedup = True if deduplicate is None else bool(deduplicate)
+        e = np.asarray(edges, dtype=np.float64)
+        src, dst, w = e[:, 0].astype(np.int64), e[:, 1].astype(np.int64), e[:, 2].astype(np.float32)
+        n = max(int(max(src.max(), dst.max())) + 1, len(vocab))
+        if dedup:
+            order = np.lexsort((dst, src))
+            src, dst, w = src[order], dst[order], w[order]
+            new = np.ones(len(src), dtype=bool)
+            new[1:] = (src[1:] != src[:-1]) | (dst[1:] != dst[:-1])
+            grp = np.cumsum(new) - 1
+            ws = np.zeros(int(grp[-1]) + 1, dtype=np.float32)
+            for gi, wi in zip(grp, w):  # sequential float32 sums, like the reference
+                ws[gi] = np.float32(ws[gi] + wi)
+            src, dst, w = src[new], dst[new], ws
+        else:
+            order = np.argsort(src, kind="stable")
+            src, dst, w = src[order], dst[order], w[order]
+        indptr = np.zeros(n + 1, dtype=np.uint64)
+        np.add.at(indptr, src + 1, 1)
+        indptr = np.cumsum(indptr).astype(np.uint64)
+        cdf = _rows_to_cdf(indptr.astype(np.int64), w)
+        return Graph(indptr, dst.astype(np.uint32), cdf, vocab)
+
+
+# --------------------------------------------------------------------------------------
+# FeatureSet (src/feature_store.rs, src/lib.rs:1432-1691)
+# --------------------------------------------------------------------------------------
+class FeatureNamespace:
+    def __init__(self, kind: str, value: Optional[str] = None):
+        self.kind, self.value = kind, value
+
+    @staticmethod
+    def single(ns: str):
+        return FeatureNamespace("static", ns)
+
+    @staticmethod
+    def node_type():
+        return FeatureNamespace("node_type")
+
+    @staticmethod
+    def prefix(delim: str):
+        return FeatureNamespace("prefix", delim)
+
+    def parse(self, node_type: str, token: str) -> Optional[FQNode]:
+        if self.kind == "static":
+            return (self.value, token)
+        if self.kind == "node_type":
+            return (node_type, token)
+        idx = token.find(self.value)
+        return None if idx < 0 else (token[:idx], token[idx + 1:])
+
+
+class FeatureSet:
+    def __init__(self, vocab: Vocab, n_nodes: int):
+        self.vocab_ = vocab
+        self.features: List[List[int]] = [[] for _ in range(n_nodes)]
+        self.feature_vocab = Vocab()
+        self._csr = None  # (f_indptr, f_ids) fast path for large synthetic sets
+
+    @staticmethod
+    def new_from_graph(graph: Graph) -> "FeatureSet":
+        return FeatureSet(graph.vocab_, graph.nodes())
+
+    @classmethod
+    def from_csr(cls, graph: Graph, f_indptr: np.ndarray, f_ids: np.ndarray, n_vocab: int) -> "FeatureSet":
+        """Integer feature lists (feature k named ("feat", str(k))) for large synthetic inputs."""
+        fs = cls(graph.vocab_, 0)
+        fs._csr = (np.ascontiguousarray(f_indptr, dtype=np.uint64), np.ascontiguousarray(f_ids, dtype=np.uint32))
+        fs._csr_nodes = graph.nodes()
+        fs.feature_vocab = Vocab.anonymous(int(n_vocab), "feat")
+        return fs
+
+    def _materialize(self):
+        if self._csr is not None:
+            ip, ids = self._csr
+            self.features = [ids[int(ip[i]):int(ip[i + 1])].astype(np.int64).tolist() for i in range(self._csr_nodes)]
+            self._csr = None
+
+    def set_features(self, node: FQNode, features: Sequence[FQNode]):
+        self._materialize()
+        i = _node_id(self.vocab_, node)
+        self.features[i] = [self.feature_vocab.get_or_insert(t, n) for t, n in features]
+
+    def get_features(self, node: FQNode) -> List[FQNode]:
+        i = _node_id(self.vocab_, node)
+        if self._csr is not None:
+            ip, ids = self._csr
+            return [self.feature_vocab.name(int(f)) for f in ids[int(ip[i]):int(ip[i + 1])]]
+        return [self.feature_vocab.name(f) for f in self.features[i]]
+
+    def load_into(self, path: str, f_ns: Optional[FeatureNamespace] = None):
+        """FeatureSet.load_into (src/lib.rs:1579-1606): node_type<TAB>name<TAB>f1 f2 ..."""
+        self._materialize()
+        ns = f_ns or FeatureNamespace.single("feat")
+        with _open_text(path) as fh:
+            for i, line in enumerate(fh):
+                pieces = line.rstrip("\n").split("\t")
+                if len(pieces) != 3:
+                    raise ValueError(f"Line {i + 1}:Malformed feature line! Need node_type<TAB>name<TAB>f1 f2 ...")
+                toks = [ns.parse(pieces[0], s) or ("feat", s) for s in pieces[2].split()]
+                nid = self.vocab_.get((pieces[0], pieces[1]))
+                if nid is not None:
+                    self.features[nid] = [self.feature_vocab.get_or_insert(t, n) for t, n in toks]
+
+    def fill_missing_nodes(self):
+        """feature_store.rs:83-89: featureless nodes get the anonymous feature ("node", str(i))."""
+        if self._csr is not None:
+            ip, ids = self._csr
+            empty = np.flatnonzero(np.diff(ip.astype(np.int64)) == 0)
+            if len(empty) == 0:
+                return
+            self._materialize()
+        for i in range(len(self.features)):
+            if not self.features[i]:
+                self.features[i] = [self.feature_vocab.get_or_insert("node", str(i))]
+
+    def to_csr(self) -> Tuple[np.ndarray, np.ndarray]:
+        if self._csr is not None:
+            return self._csr
+        counts = np.fromiter((len(f) for f in self.features), dtype=np.uint64, count=len(self.features))
+        ip = np.zeros(len(self.features) + 1, dtype=np.uint64)
+        np.cumsum(counts, out=ip[1:])
+        ids = np.fromiter((x for f in self.features for x in f), dtype=np.uint32, count=int(ip[-1]))
+        return ip, ids
+
+    def nodes(self) -> int:
+        return len(self.vocab_)
+
+    def num_features(self) -> int:
+        return len(self.feature_vocab)
+
+    def vocab(self):
+        return iter(self.vocab_.names)
+
+    def __getitem__(self, idx: int) -> FQNode:
+        n = self.num_features()
+        if idx < 0:
+            idx += n
+        if idx >= n:
+            raise IndexError("Feature Index larger than FeatureSet!")
+        return self.feature_vocab.name(idx)
+
+    def __repr__(self):
+        return f"FeatureSet<Nodes={len(self.vocab_)},UniqueFeatures={self.num_features()}>"
+
+
+def _open_text(path: str):
+    if path.endswith(".gz"):
+        import gzip
+        return gzip.open(path, "rt")
+    return open(path, "r")
+
+
+# --------------------------------------------------------------------------------------
+# NodeEmbeddings (src/lib.rs:2535-2894)
+# --------------------------------------------------------------------------------------
+class NodeEmbeddings:
+    def __init__(self, vocab: Vocab, embeddings: np.ndarray, distance: str = Distance.Cosine):
+        self.vocab_ = vocab
+        self.embeddings = np.ascontiguousarray(embeddings, dtype=np.float32)
+        self.distance = distance
+
+    def dims(self) -> int:
+        return int(self.embeddings.shape[1]) if self.embeddings.ndim == 2 else 0
+
+    def len(self) -> int:
+        return int(self.embeddings.shape[0])
+
+    def __len__(self):
+        return self.len()
+
+    def contains(self, node: FQNode) -> bool:
+        return self.vocab_.get(node) is not None
+
+    def get_embedding(self, node: FQNode) -> List[float]:
+        return self.embeddings[_node_id(self.vocab_, node)].tolist()
+
+    def set_embedding(self, node: FQNode, embedding: Sequence[float]):
+        self.embeddings[_node_id(self.vocab_, node)] = np.asarray(embedding, dtype=np.float32)
+
+    def vocab(self):
+        return iter(self.vocab_.names)
+
+    def get_distance(self) -> str:
+        return self.distance
+
+    def __repr__(self):
+        return f"NodeEmbeddings<Nodes={self.len()}, Dims={self.dims()}>"
+
+
+# --------------------------------------------------------------------------------------
+# Losses and weighting (src/lib.rs:939-1110)
+# --------------------------------------------------------------------------------------
+class LossWeighting:
+    def __init__(self, kind: int, weight: float = 0.0):
+        self.kind, self.weight = kind, weight
+
+    @staticmethod
+    def Log():
+        return LossWeighting(N.EP_LW_LOG)
+
+    @staticmethod
+    def Exponential(weight: float):
+        return LossWeighting(N.EP_LW_EXPONENTIAL, float(weight))
+
+
+class EPLoss:
+    def __init__(self, kind: str, params: tuple, negatives: int):
+        self.kind, self.params, self.negatives = kind, params, negatives
+
+    @staticmethod
+    def margin(gamma: float, negatives: Optional[int] = None):
+        return EPLoss("margin", (float(gamma),), int(negatives) if negatives is not None else 1)
+
+    @staticmethod
+    def contrastive(positive_margin: float, negative_margin: float, negatives: int):
+        return EPLoss("contrastive", (positive_margin, negative_margin), max(1, negatives))
+
+    @staticmethod
+    def starspace(gamma: float, negatives: int):
+        return EPLoss("starspace", (gamma,), max(1, negatives))
+
+    @staticmethod
+    def rank(tau: float, negatives: int):
+        return EPLoss("rank", (tau,), max(1, negatives))
+
+    @staticmethod
+    def rankspace(tau: float, negatives: int):
+        return EPLoss("rankspace", (tau,), max(1, negatives))
+
+    @staticmethod
+    def ppr(gamma: float, negatives: int, restart_p: float):
+        return EPLoss("ppr", (gamma, restart_p), max(1, negatives))
+
+    def __repr__(self):
+        return f"EPLoss<{self.kind}{self.params}, negatives={self.negatives}>"
+
+
+def _parse_sample(x: Optional[float], what: str) -> Tuple[int, int]:
+    """Sample::new (src/algos/utils.rs:173-184): (0,1) -> Probability, else Fixed(int)."""
+    if x is None:
+        return N.EP_SAMPLE_ALL, 0
+    x = float(x)
+    if x < 0:
+        raise ValueError(f"{what} must be either None, or (0, N)")
+    if 0 < x < 1:
+        raise ValueError(f"{what}: probability sampling is not supported by this build")
+    k = int(x)
+    if k == 0:
+        raise ValueError(f"{what}: Fixed(0) samples nothing")
+    return N.EP_SAMPLE_FIXED, k
+
+
+# --------------------------------------------------------------------------------------
+# EmbeddingPropagator (src/lib.rs:1119-1377)
+# --------------------------------------------------------------------------------------
+class EmbeddingPropagator:
+    """Embedding Propagation trainer; the step runs on the GPU.
+
+    Reference arguments keep their names and defaults.  Extra keyword arguments
+    select the MI355X execution: ``precision`` ("fp32" | "bf16"), ``grad_mode``
+    ("first_leaf", the reference rule, | "exact"), ``scatter`` ("sort" | "atomic"),
+    ``use_graph`` (replay steps from a captured hipGraph), ``device``.
+    """
+
+    def __init__(self, alpha=None, loss=None, batch_size=None, dims=None, passes=None, seed=None, max_nodes=None,
+                 weighted_neighbor_sampling=None, weighted_neighbor_averaging=None, max_features=None,
+                 loss_weighting=None, valid_pct=None, hard_negatives=None, indicator=None, attention=None,
+                 attention_heads=None, context_window=None, noise=None, *, precision="fp32",
+                 grad_mode="first_leaf", scatter="sort", use_graph=False, device=0):
+        loss = loss or EPLoss.margin(1.0, 1)
+        mf_kind, mf = _parse_sample(max_features, "max_features")
+        mn_kind, mn = _parse_sample(max_nodes, "max_nodes")
+        if loss.kind != "margin":
+            raise ValueError(f"loss {loss.kind} is not supported by this build (margin only)")
+        if attention is not None:
+            raise ValueError("the attention aggregator is not supported by this build")
+        if mf_kind != N.EP_SAMPLE_ALL:
+            raise ValueError("max_features sampling is not supported by this build")
+        if weighted_neighbor_sampling or weighted_neighbor_averaging:
+            raise ValueError("weighted neighbour sampling/averaging is not supported by this build")
+        if hard_negatives:
+            raise ValueError("hard negatives are not supported by this build")
+        if noise:
+            raise ValueError("gradient noise is not supported by this build")
+        cfg = N.default_config()
+        cfg.alpha = 0.9 if alpha is None else float(alpha)
+        cfg.batch_size = 50 if batch_size is None else int(batch_size)
+        cfg.dims = 100 if dims is None else int(dims)
+        cfg.passes = 100 if passes is None else int(passes)
+        cfg.negatives = loss.negatives
+        cfg.gamma = loss.params[0]
+        cfg.valid_pct = 0.1 if valid_pct is None else float(valid_pct)
+        cfg.seed = SEED if seed is None else int(seed)
+        cfg.max_nodes_kind, cfg.max_nodes = mn_kind, mn
+        cfg.max_features_kind, cfg.max_features = mf_kind, mf
+        if loss_weighting is not None:
+            cfg.loss_weighting = loss_weighting.kind
+            cfg.loss_weight_exp = loss_weighting.weight
+        cfg.precision = {"fp32": N.EP_PREC_FP32, "bf16": N.EP_PREC_BF16}[precision]
+        cfg.grad_mode = {"first_leaf": N.EP_GRAD_FIRST_LEAF, "exact": N.EP_GRAD_EXACT}[grad_mode]
+        cfg.scatter = {"sort": N.EP_SCATTER_SORT, "atomic": N.EP_SCATTER_ATOMIC}[scatter]
+        cfg.use_graph = 1 if use_graph else 0
+        cfg.device = int(device)
+        self.cfg = cfg
+        self.indicator = True if indicator is None else bool(indicator)
+        self.history: List[dict] = []
+
+    def __repr__(self):
+        c = self.cfg
+        return (f"EmbeddingPropagation {{ alpha: {c.alpha}, loss: MarginLoss({c.gamma}, {c.negatives}), "
+                f"batch_size: {c.batch_size}, d_model: {c.dims}, passes: {c.passes}, seed: {c.seed}, "
+                f"valid_pct: {c.valid_pct} }}")
+
+    def learn_features(self, graph: Graph, features: FeatureSet,
+                       feature_embeddings: Optional[NodeEmbeddings] = None) -> NodeEmbeddings:
+        """lib.rs:1327-1370.  Side effects kept: ``features`` gets fill_missing_nodes;
+        a warm-start ``feature_embeddings`` is moved out (left as an empty store)."""
+        features.fill_missing_nodes()
+        f_indptr, f_ids = features.to_csr()
+        n_vocab = features.num_features()
+        warm = None
+        if feature_embeddings is not None:
+            warm = feature_embeddings.embeddings
+            feature_embeddings.embeddings = np.zeros((len(feature_embeddings.vocab_), 0), dtype=np.float32)
+            if warm.shape != (n_vocab, self.cfg.dims):
+                raise ValueError(f"warm-start table has shape {warm.shape}, expected {(n_vocab, self.cfg.dims)}")
+        with N.Context(self.cfg) as ctx:
+            ctx.set_graph(graph.indptr, graph.indices)
+            ctx.set_features(f_indptr, f_ids, n_vocab)
+            ctx.set_table(warm)
+            self.history = []
+
+            def progress(p, step, tr, va, lr):
+                self.history.append(dict(pass_=p, step=step, train=tr, valid=va, lr=lr))
+                if self.indicator:
+                    print(f"Pass {p}/{self.cfg.passes}, Train: {tr:.5f}, Valid: {va:.5f}, LR: {lr:.5f}",
+                          file=sys.stderr)
+
+            ctx.learn(progress)
+            table = ctx.get_table()
+        return NodeEmbeddings(features.feature_vocab, table, Distance.Cosine)
+
+
+# --------------------------------------------------------------------------------------
+# Inference aggregate (src/lib.rs:1766-2207, aggregator.rs:34-47)
+# --------------------------------------------------------------------------------------
+class FeatureAggregator:
+    def __init__(self, kind: str):
+        self.kind = kind
+
+    @staticmethod
+    def Averaged():
+        return FeatureAggregator("averaged")
+
+
+class NodeEmbedder:
+    def __init__(self, feat_agg: FeatureAggregator):
+        if feat_agg.kind != "averaged":
+            raise ValueError("only FeatureAggregator.Averaged is supported by this build")
+        self.feat_agg = feat_agg
+
+    def embed_feature_set(self, feat_set: FeatureSet, feature_embeddings: NodeEmbeddings,
+                          device: int = 0) -> NodeEmbeddings:
+        """lib.rs:2092-2124: node embedding = mean of its (known) feature rows, on the GPU."""
+        ip, ids = feat_set.to_csr()
+        # translate FeatureSet feature ids into the embedding vocabulary, dropping unknown ones
+        fv, ev = feat_set.feature_vocab, feature_embeddings.vocab_
+        if fv is ev or fv.names == ev.names[:len(fv.names)]:
+            t_ip, t_ids = ip, ids
+        else:
+            lut = np.array([ev.get(nm) if ev.get(nm) is not None else -1 for nm in fv.names], dtype=np.int64)
+            mapped = lut[ids.astype(np.int64)]
+            keep = mapped >= 0
+            counts = np.add.reduceat(keep.astype(np.int64), ip[:-1].astype(np.int64)) if len(ids) else \
+                np.zeros(len(ip) - 1, dtype=np.int64)
+            counts[np.diff(ip.astype(np.int64)) == 0] = 0
+            t_ip = np.zeros(len(ip), dtype=np.uint64)
+            np.cumsum(counts, out=t_ip[1:])
+            t_ids = mapped[keep].astype(np.uint32)
+        table = feature_embeddings.embeddings
+        cfg = N.default_config()
+        cfg.dims = table.shape[1]
+        cfg.device = int(device)
+        with N.Context(cfg) as ctx:
+            # a one-node placeholder feature list: only the table is needed
+            ctx.set_features(np.array([0, 1], dtype=np.uint64), np.array([0], dtype=np.uint32), table.shape[0])
+            ctx.set_table(table)
+            out = ctx.embed_averaged(t_ip, t_ids)
+        return NodeEmbeddings(feat_set.vocab_, out, Distance.Cosine)

@@ -1,0 +1,909 @@
+// Host runtime of the EP training step: context, device memory plan, training loop,
+// hipGraph capture and the C ABI declared in include/cloverleaf_ep.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cloverleaf_ep.h"
+#include "ep_device.h"
+#include "ep_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define EP_HIP(call)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return fail(EP_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));                \
+  } while (0)
+
+// splitmix64: host-side stream for the train/valid split and per-pass shuffles.
+struct HostRng {
+  uint64_t s;
+  explicit HostRng(uint64_t seed) : s(seed) {}
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  uint64_t below(uint64_t n) { return (uint64_t)(((__uint128_t)next() * n) >> 64); }
+};
+
+template <typename T>
+void shuffle(std::vector<T>& v, HostRng& r) {
+  for (size_t i = v.size(); i > 1; --i) std::swap(v[i - 1], v[r.below(i)]);
+}
+
+uint32_t next_pow2(uint64_t x) {
+  uint64_t p = 1;
+  while (p < x) p <<= 1;
+  return (uint32_t)p;
+}
+
+}  // namespace
+
+struct ep_ctx {
+  ep_config cfg{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocs;
+
+  // graph / features
+  uint64_t n_nodes = 0, n_entries = 0;
+  uint32_t deg_max = 0;
+  std::vector<uint32_t> degrees;
+  uint64_t* d_indptr = nullptr;
+  uint32_t* d_indices = nullptr;
+  uint64_t f_nodes = 0, f_total = 0;
+  uint32_t n_vocab = 0, f_max = 0;
+  uint64_t* d_findptr = nullptr;
+  uint32_t* d_fids = nullptr;
+
+  // parameters
+  void* d_table = nullptr;
+  float* d_master = nullptr;
+  float* d_mom = nullptr;
+  float* d_var = nullptr;
+  bool table_ready = false;
+
+  // split
+  std::vector<uint32_t> train, valid;
+  bool split_set = false;
+  uint32_t* d_train = nullptr;  // negative pool (train ids)
+  uint32_t* d_valid = nullptr;
+  uint32_t* d_perm = nullptr;   // pass permutation of train ids
+  uint32_t* h_perm = nullptr;   // pinned staging
+  float* d_pass_loss = nullptr;
+  float* d_valid_loss = nullptr;
+
+  // step buffers
+  uint32_t cap_batch = 0;
+  uint32_t cap_S = 0;
+  uint32_t key_cap = 0;
+  ep::StepConfig base{};
+  ep::StepState* d_state = nullptr;
+  ep::StepState* d_vstate = nullptr;
+  uint32_t* d_explicit = nullptr;  // anchors of explicit / sampled test batches
+  float* d_xout = nullptr;
+
+  // graph capture
+  hipGraphExec_t step_exec = nullptr;
+  bool step_exec_ready = false;
+
+  ~ep_ctx() {
+    if (step_exec) (void)hipGraphExecDestroy(step_exec);
+    for (void* p : allocs) (void)hipFree(p);
+    if (h_perm) (void)hipHostFree(h_perm);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  template <typename T>
+  int alloc(T** p, size_t count) {
+    void* q = nullptr;
+    const size_t bytes = std::max<size_t>(count * sizeof(T), 16);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return fail(EP_ENOMEM, "hipMalloc(" + std::to_string(bytes) + " bytes) failed: " + hipGetErrorString(e));
+    allocs.push_back(q);
+    *p = (T*)q;
+    return EP_OK;
+  }
+
+  template <typename T>
+  void release(T*& p) {
+    if (!p) return;
+    auto it = std::find(allocs.begin(), allocs.end(), (void*)p);
+    if (it != allocs.end()) allocs.erase(it);
+    (void)hipFree((void*)p);
+    p = nullptr;
+  }
+};
+
+#define EP_TRY(x)            \
+  do {                       \
+    int r_ = (x);            \
+    if (r_ != EP_OK) return r_; \
+  } while (0)
+
+namespace {
+
+uint32_t neighbour_cap(const ep_ctx* c) {
+  return c->cfg.max_nodes_kind == EP_SAMPLE_FIXED ? c->cfg.max_nodes : c->deg_max;
+}
+
+// Worst-case feature occurrences of one example: F_max * (1 + S + K).
+uint64_t example_rows_max(const ep_ctx* c) {
+  return (uint64_t)c->f_max * (1ull + neighbour_cap(c) + c->cfg.negatives);
+}
+
+// Sizes every per-step buffer for batches of up to `batch` anchors.
+int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
+  if (batch <= c->cap_batch && c->base.nbr) return EP_OK;
+  if (!c->d_indptr || !c->d_findptr) return fail(EP_ESTATE, "graph and features must be set first");
+  const ep_config& cfg = c->cfg;
+  const uint32_t K = cfg.negatives, NC = K + 2, D = cfg.dims;
+  const uint32_t S = cfg.max_nodes_kind == EP_SAMPLE_FIXED ? cfg.max_nodes : 0;
+  // key capacity: sum of the largest `batch` example sizes
+  uint64_t keycap;
+  if (cfg.max_nodes_kind == EP_SAMPLE_FIXED) {
+    keycap = (uint64_t)batch * example_rows_max(c);
+  } else {
+    std::vector<uint32_t> d = c->degrees;
+    const size_t take = std::min<size_t>(batch, d.size());
+    std::partial_sort(d.begin(), d.begin() + take, d.end(), std::greater<uint32_t>());
+    uint64_t s = 0;
+    for (size_t i = 0; i < take; ++i) s += d[i];
+    keycap = (uint64_t)c->f_max * (s + (uint64_t)batch * (1ull + K));
+  }
+  if (keycap >= (1ull << 31)) return fail(EP_ECAPACITY, "batch too large: key capacity exceeds 2^31");
+  const uint64_t hash = next_pow2(2 * std::max<uint64_t>(example_rows_max(c), 1));
+  if (cfg.grad_mode == EP_GRAD_FIRST_LEAF && hash > 16384)
+    return fail(EP_ECAPACITY, "first_leaf dedup needs <= 8192 feature occurrences per example (F_max*(1+S+K))");
+
+  // drop old buffers
+  ep::StepConfig& b = c->base;
+  c->release(b.nbr_cnt); c->release(b.nbr); c->release(b.neg); c->release(b.rcount); c->release(b.keyoff);
+  c->release(b.adj); c->release(b.coef); c->release(b.elig); c->release(b.loss);
+  c->release(b.keys[0]); c->release(b.keys[1]); c->release(b.slots[0]); c->release(b.slots[1]);
+  c->release(b.ghist); c->release(b.G); c->release(b.ufeat); c->release(b.long_st); c->release(b.long_len);
+  c->release(b.long_pbase); c->release(b.piece_seg); c->release(b.part); c->release(c->d_explicit);
+  c->release(c->d_xout);
+  if (c->step_exec) { (void)hipGraphExecDestroy(c->step_exec); c->step_exec = nullptr; c->step_exec_ready = false; }
+
+  const uint64_t unique_cap = std::min<uint64_t>(c->n_vocab, keycap) + 1;
+  const uint32_t long_cap = (uint32_t)(keycap / 256 + 1);
+  const uint32_t piece_cap = (uint32_t)(keycap / 256 + long_cap + 1);
+  const int sort_grid = 1024;
+  EP_TRY(c->alloc(&b.nbr_cnt, batch));
+  EP_TRY(c->alloc(&b.nbr, (size_t)batch * std::max<uint32_t>(S, 1)));
+  EP_TRY(c->alloc(&b.neg, (size_t)batch * K));
+  EP_TRY(c->alloc(&b.rcount, batch));
+  EP_TRY(c->alloc(&b.keyoff, batch + 1));
+  EP_TRY(c->alloc(&b.adj, (size_t)batch * NC * D));
+  EP_TRY(c->alloc(&b.coef, (size_t)batch * NC));
+  EP_TRY(c->alloc(&b.elig, (size_t)batch * NC));
+  EP_TRY(c->alloc(&b.loss, batch));
+  for (int i = 0; i < 2; ++i) {
+    EP_TRY(c->alloc(&b.keys[i], keycap));
+    EP_TRY(c->alloc(&b.slots[i], keycap));
+  }
+  EP_TRY(c->alloc(&b.ghist, (size_t)256 * sort_grid));
+  EP_TRY(c->alloc(&b.G, (size_t)unique_cap * D));
+  EP_TRY(c->alloc(&b.ufeat, unique_cap));
+  EP_TRY(c->alloc(&b.long_st, long_cap));
+  EP_TRY(c->alloc(&b.long_len, long_cap));
+  EP_TRY(c->alloc(&b.long_pbase, long_cap));
+  EP_TRY(c->alloc(&b.piece_seg, piece_cap));
+  EP_TRY(c->alloc(&b.part, (size_t)piece_cap * D));
+  EP_TRY(c->alloc(&c->d_explicit, batch));
+  if (!b.Gd && cfg.scatter == EP_SCATTER_ATOMIC) {
+    EP_TRY(c->alloc(&b.Gd, (size_t)c->n_vocab * D));
+    EP_TRY(c->alloc(&b.flag, c->n_vocab));
+    EP_TRY(c->alloc(&b.touched, c->n_vocab));
+    EP_HIP(hipMemsetAsync(b.Gd, 0, (size_t)c->n_vocab * D * sizeof(float), c->stream));
+    EP_HIP(hipMemsetAsync(b.flag, 0, (size_t)c->n_vocab * sizeof(uint32_t), c->stream));
+  }
+  b.batch = batch;
+  b.S = S;
+  b.all_nbrs = cfg.max_nodes_kind == EP_SAMPLE_ALL ? 1 : 0;
+  b.key_cap = (uint32_t)keycap;
+  b.sort_grid = sort_grid;
+  b.long_cap = long_cap;
+  b.piece_cap = piece_cap;
+  b.hash_entries = (uint32_t)hash;
+  b.unique_grid = 2048;
+  c->cap_batch = batch;
+  c->cap_S = S;
+  c->key_cap = (uint32_t)keycap;
+  return EP_OK;
+}
+
+// Fills the parts of the base StepConfig that depend on graph / table / config.
+void refresh_base(ep_ctx* c) {
+  ep::StepConfig& b = c->base;
+  const ep_config& cfg = c->cfg;
+  b.indptr = c->d_indptr;
+  b.indices = c->d_indices;
+  b.findptr = c->d_findptr;
+  b.fids = c->d_fids;
+  b.n_nodes = c->n_nodes;
+  b.n_vocab = c->n_vocab;
+  int bits = 1;
+  while ((1ull << bits) - 1ull < (uint64_t)c->n_vocab) ++bits;  // sentinel = 2^bits - 1 >= n_vocab
+  b.sort_bits = bits;
+  b.sentinel = (uint32_t)((1ull << bits) - 1ull);
+  b.table = c->d_table;
+  b.master = c->d_master;
+  b.mom = c->d_mom;
+  b.var = c->d_var;
+  b.bf16 = cfg.precision == EP_PREC_BF16 ? 1 : 0;
+  b.dims = (int)cfg.dims;
+  b.K = cfg.negatives;
+  b.gamma = cfg.gamma;
+  b.lw_kind = cfg.loss_weighting;
+  b.lw_exp = cfg.loss_weight_exp;
+  b.grad_mode = cfg.grad_mode;
+  b.seed = cfg.seed;
+  b.alpha = cfg.alpha;
+  b.scatter = cfg.scatter;
+  b.train = 1;
+  b.explicit_samples = 0;
+  b.pass_loss = nullptr;
+  b.xout = nullptr;
+}
+
+int launch_training_step(ep_ctx* c, const ep::StepConfig& sc, bool update, float lr_override,
+                         hipEvent_t ev_f0 = nullptr, hipEvent_t ev_f1 = nullptr, hipEvent_t ev_s1 = nullptr) {
+  hipStream_t s = c->stream;
+  if (lr_override >= 0.f) EP_HIP(ep::launch_step_begin_lr(sc, c->d_state, lr_override, s));
+  else EP_HIP(ep::launch_step_begin(sc, c->d_state, s));
+  EP_HIP(ep::launch_sample(sc, c->d_state, s));
+  EP_HIP(ep::launch_scan(sc, c->d_state, s));
+  if (ev_f0) EP_HIP(hipEventRecord(ev_f0, s));
+  EP_HIP(ep::launch_forward(sc, c->d_state, s));
+  if (ev_f1) EP_HIP(hipEventRecord(ev_f1, s));
+  EP_HIP(ep::launch_emit(sc, c->d_state, s));
+  if (sc.scatter == EP_SCATTER_SORT) {
+    int buf = 0;
+    EP_HIP(ep::launch_sort(sc, c->d_state, s, &buf));
+    EP_HIP(ep::launch_segreduce(sc, c->d_state, buf, s));
+    if (update) EP_HIP(ep::launch_adam_unique(sc, c->d_state, s));
+  } else {
+    EP_HIP(ep::launch_atomic_scatter(sc, c->d_state, s));
+    if (update) EP_HIP(ep::launch_adam_touched(sc, c->d_state, s));
+  }
+  if (ev_s1) EP_HIP(hipEventRecord(ev_s1, s));
+  return EP_OK;
+}
+
+int check_state_err(ep_ctx* c) {
+  ep::StepState hs;
+  EP_HIP(hipMemcpyAsync(&hs, c->d_state, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
+  EP_HIP(hipStreamSynchronize(c->stream));
+  if (hs.err) return fail(EP_ECAPACITY, "key capacity exceeded in a step");
+  return EP_OK;
+}
+
+int set_pass(ep_ctx* c, ep::StepState* st, uint32_t pass, uint64_t* next_step) {
+  // pass and step_in_pass live at fixed offsets; write them with small async copies
+  const uint32_t vals[2] = {pass, 0xFFFFFFFFu};
+  EP_HIP(hipMemcpyAsync((char*)st + offsetof(ep::StepState, pass), vals, sizeof(vals), hipMemcpyHostToDevice,
+                        c->stream));
+  if (next_step)
+    EP_HIP(hipMemcpyAsync((char*)st + offsetof(ep::StepState, next_step), next_step, sizeof(uint64_t),
+                          hipMemcpyHostToDevice, c->stream));
+  EP_HIP(hipStreamSynchronize(c->stream));
+  return EP_OK;
+}
+
+int ensure_split(ep_ctx* c) {
+  if (c->split_set) return EP_OK;
+  // mod.rs:119-122: shuffle node ids, the last floor(N * valid_pct) are validation
+  HostRng rng(c->cfg.seed);
+  std::vector<uint32_t> ids(c->n_nodes);
+  for (uint64_t i = 0; i < c->n_nodes; ++i) ids[i] = (uint32_t)i;
+  shuffle(ids, rng);
+  const uint64_t nv = (uint64_t)((float)c->n_nodes * c->cfg.valid_pct);
+  c->valid.assign(ids.end() - nv, ids.end());
+  ids.resize(c->n_nodes - nv);
+  c->train.swap(ids);
+  c->split_set = true;
+  return EP_OK;
+}
+
+int upload_split(ep_ctx* c) {
+  c->release(c->d_train);
+  c->release(c->d_valid);
+  c->release(c->d_perm);
+  c->release(c->d_pass_loss);
+  c->release(c->d_valid_loss);
+  if (c->h_perm) { (void)hipHostFree(c->h_perm); c->h_perm = nullptr; }
+  const size_t nt = c->train.size(), nv = c->valid.size();
+  EP_TRY(c->alloc(&c->d_train, nt));
+  EP_TRY(c->alloc(&c->d_valid, nv));
+  EP_TRY(c->alloc(&c->d_perm, nt));
+  EP_TRY(c->alloc(&c->d_pass_loss, nt));
+  EP_TRY(c->alloc(&c->d_valid_loss, nv));
+  EP_HIP(hipHostMalloc((void**)&c->h_perm, std::max<size_t>(nt, 1) * sizeof(uint32_t), hipHostMallocDefault));
+  if (nt) EP_HIP(hipMemcpy(c->d_train, c->train.data(), nt * sizeof(uint32_t), hipMemcpyHostToDevice));
+  if (nv) EP_HIP(hipMemcpy(c->d_valid, c->valid.data(), nv * sizeof(uint32_t), hipMemcpyHostToDevice));
+  if (c->step_exec) { (void)hipGraphExecDestroy(c->step_exec); c->step_exec = nullptr; c->step_exec_ready = false; }
+  return EP_OK;
+}
+
+// Shuffle the train ids for a pass (mod.rs:173) and stage them to the device.
+int upload_pass_perm(ep_ctx* c, HostRng& rng) {
+  EP_HIP(hipStreamSynchronize(c->stream));
+  std::vector<uint32_t>& t = c->train;
+  shuffle(t, rng);
+  std::memcpy(c->h_perm, t.data(), t.size() * sizeof(uint32_t));
+  if (!t.empty())
+    EP_HIP(hipMemcpyAsync(c->d_perm, c->h_perm, t.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+  return EP_OK;
+}
+
+ep::StepConfig train_config(ep_ctx* c) {
+  ep::StepConfig sc = c->base;
+  sc.perm = c->d_perm;
+  sc.negpool = c->d_train;
+  sc.n_negpool = (uint32_t)c->train.size();
+  sc.n_batch_total = (uint32_t)c->train.size();
+  const uint32_t B = c->cfg.batch_size;
+  sc.steps_per_pass = (uint32_t)((c->train.size() + B - 1) / B);
+  sc.total_updates = sc.steps_per_pass * c->cfg.passes;
+  sc.pass_loss = c->d_pass_loss;
+  return sc;
+}
+
+int run_step(ep_ctx* c, const ep::StepConfig& sc) {
+  if (!c->cfg.use_graph) return launch_training_step(c, sc, true, -1.f);
+  if (!c->step_exec_ready) {
+    hipGraph_t g = nullptr;
+    EP_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int r = launch_training_step(c, sc, true, -1.f);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (r != EP_OK) { if (g) (void)hipGraphDestroy(g); return r; }
+    if (e != hipSuccess) return fail(EP_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->step_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(EP_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    c->step_exec_ready = true;
+  }
+  EP_HIP(hipGraphLaunch(c->step_exec, c->stream));
+  return EP_OK;
+}
+
+int validate_config(const ep_config* cfg) {
+  if (!ep::dims_supported((int)cfg->dims)) return fail(EP_EINVAL, "dims must be one of 8,16,32,64,100,128,256");
+  if (cfg->batch_size == 0) return fail(EP_EINVAL, "batch_size must be > 0");
+  if (cfg->negatives == 0 || cfg->negatives > 29) return fail(EP_EINVAL, "negatives must be in [1, 29]");
+  if (cfg->passes == 0) return fail(EP_EINVAL, "passes must be > 0");
+  if (cfg->max_nodes_kind == EP_SAMPLE_FIXED && (cfg->max_nodes == 0 || cfg->max_nodes > 1024))
+    return fail(EP_EINVAL, "max_nodes (Fixed) must be in [1, 1024]");
+  if (cfg->max_nodes_kind != EP_SAMPLE_FIXED && cfg->max_nodes_kind != EP_SAMPLE_ALL)
+    return fail(EP_EINVAL, "max_nodes must be All or Fixed");
+  if (cfg->max_features_kind != EP_SAMPLE_ALL) return fail(EP_EINVAL, "max_features sampling is not supported");
+  if (cfg->hard_negatives != 0) return fail(EP_EINVAL, "hard_negatives is not supported");
+  if (cfg->noise != 0.f) return fail(EP_EINVAL, "gradient noise is not supported");
+  if (cfg->weighted_neighbor_sampling || cfg->weighted_neighbor_averaging)
+    return fail(EP_EINVAL, "weighted neighbour sampling/averaging is not supported");
+  if (cfg->precision != EP_PREC_FP32 && cfg->precision != EP_PREC_BF16) return fail(EP_EINVAL, "precision");
+  if (cfg->grad_mode != EP_GRAD_FIRST_LEAF && cfg->grad_mode != EP_GRAD_EXACT) return fail(EP_EINVAL, "grad_mode");
+  if (cfg->scatter != EP_SCATTER_SORT && cfg->scatter != EP_SCATTER_ATOMIC) return fail(EP_EINVAL, "scatter");
+  if (cfg->loss_weighting < EP_LW_NONE || cfg->loss_weighting > EP_LW_EXPONENTIAL)
+    return fail(EP_EINVAL, "loss_weighting");
+  if (!(cfg->valid_pct >= 0.f && cfg->valid_pct < 1.f)) return fail(EP_EINVAL, "valid_pct must be in [0, 1)");
+  return EP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ep_last_error(void) { return g_err.c_str(); }
+
+int ep_config_default(ep_config* cfg) {
+  if (!cfg) return fail(EP_EINVAL, "cfg is NULL");
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->dims = 100;
+  cfg->batch_size = 50;
+  cfg->passes = 100;
+  cfg->negatives = 1;
+  cfg->alpha = 0.9f;
+  cfg->gamma = 1.0f;
+  cfg->valid_pct = 0.1f;
+  cfg->max_nodes_kind = EP_SAMPLE_ALL;
+  cfg->max_features_kind = EP_SAMPLE_ALL;
+  cfg->loss_weighting = EP_LW_NONE;
+  cfg->precision = EP_PREC_FP32;
+  cfg->grad_mode = EP_GRAD_FIRST_LEAF;
+  cfg->scatter = EP_SCATTER_SORT;
+  cfg->use_graph = 0;
+  cfg->seed = 20222022ull;
+  cfg->device = 0;
+  return EP_OK;
+}
+
+int ep_create(const ep_config* cfg, ep_ctx** out) {
+  if (!cfg || !out) return fail(EP_EINVAL, "NULL argument");
+  *out = nullptr;
+  EP_TRY(validate_config(cfg));
+  int ndev = 0;
+  EP_HIP(hipGetDeviceCount(&ndev));
+  if (cfg->device < 0 || cfg->device >= ndev) return fail(EP_EINVAL, "no such HIP device");
+  EP_HIP(hipSetDevice(cfg->device));
+  ep_ctx* c = new ep_ctx();
+  c->cfg = *cfg;
+  c->device = cfg->device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete c; return fail(EP_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
+  int r = c->alloc(&c->d_state, 1);
+  if (r == EP_OK) r = c->alloc(&c->d_vstate, 1);
+  if (r != EP_OK) { delete c; return r; }
+  ep::StepState zero{};
+  zero.next_step = 1;
+  e = hipMemcpy(c->d_state, &zero, sizeof(zero), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->d_vstate, &zero, sizeof(zero), hipMemcpyHostToDevice);
+  if (e != hipSuccess) { delete c; return fail(EP_EHIP, std::string("hipMemcpy: ") + hipGetErrorString(e)); }
+  *out = c;
+  return EP_OK;
+}
+
+void ep_destroy(ep_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  delete ctx;
+}
+
+int ep_set_graph(ep_ctx* c, uint64_t n_nodes, const uint64_t* indptr, const uint32_t* indices) {
+  if (!c || !indptr || (!indices && indptr[n_nodes] > 0)) return fail(EP_EINVAL, "NULL argument");
+  if (n_nodes == 0 || n_nodes >= (1ull << 32)) return fail(EP_EINVAL, "n_nodes must be in [1, 2^32)");
+  if (indptr[0] != 0) return fail(EP_EINVAL, "indptr[0] must be 0");
+  std::vector<uint32_t> deg(n_nodes);
+  uint32_t dmax = 0;
+  for (uint64_t i = 0; i < n_nodes; ++i) {
+    if (indptr[i + 1] < indptr[i]) return fail(EP_EINVAL, "indptr must be non-decreasing");
+    const uint64_t d = indptr[i + 1] - indptr[i];
+    if (d >= (1ull << 32)) return fail(EP_EINVAL, "degree too large");
+    deg[i] = (uint32_t)d;
+    dmax = std::max(dmax, deg[i]);
+  }
+  const uint64_t ne = indptr[n_nodes];
+  for (uint64_t i = 0; i < ne; ++i)
+    if (indices[i] >= n_nodes) return fail(EP_EINVAL, "edge target out of range");
+  EP_HIP(hipSetDevice(c->device));
+  c->release(c->d_indptr);
+  c->release(c->d_indices);
+  EP_TRY(c->alloc(&c->d_indptr, n_nodes + 1));
+  EP_TRY(c->alloc(&c->d_indices, ne));
+  EP_HIP(hipMemcpy(c->d_indptr, indptr, (n_nodes + 1) * sizeof(uint64_t), hipMemcpyHostToDevice));
+  if (ne) EP_HIP(hipMemcpy(c->d_indices, indices, ne * sizeof(uint32_t), hipMemcpyHostToDevice));
+  c->n_nodes = n_nodes;
+  c->n_entries = ne;
+  c->deg_max = dmax;
+  c->degrees.swap(deg);
+  c->cap_batch = 0;  // capacities depend on degrees
+  c->split_set = false;
+  return EP_OK;
+}
+
+int ep_set_features(ep_ctx* c, uint64_t n_nodes, uint32_t n_vocab, const uint64_t* f_indptr, const uint32_t* f_ids) {
+  if (!c || !f_indptr || !f_ids) return fail(EP_EINVAL, "NULL argument");
+  if (c->n_nodes && n_nodes != c->n_nodes) return fail(EP_EINVAL, "features must cover every graph node");
+  if (n_vocab == 0 || n_vocab >= (1u << 27) - 1) return fail(EP_EINVAL, "n_vocab must be in [1, 2^27-1)");
+  if (f_indptr[0] != 0) return fail(EP_EINVAL, "f_indptr[0] must be 0");
+  uint32_t fmax = 0;
+  for (uint64_t i = 0; i < n_nodes; ++i) {
+    if (f_indptr[i + 1] <= f_indptr[i])
+      return fail(EP_EINVAL, "every node needs at least one feature (fill_missing_nodes)");
+    fmax = (uint32_t)std::max<uint64_t>(fmax, f_indptr[i + 1] - f_indptr[i]);
+  }
+  const uint64_t tot = f_indptr[n_nodes];
+  for (uint64_t i = 0; i < tot; ++i)
+    if (f_ids[i] >= n_vocab) return fail(EP_EINVAL, "feature id out of range");
+  EP_HIP(hipSetDevice(c->device));
+  c->release(c->d_findptr);
+  c->release(c->d_fids);
+  EP_TRY(c->alloc(&c->d_findptr, n_nodes + 1));
+  EP_TRY(c->alloc(&c->d_fids, tot));
+  EP_HIP(hipMemcpy(c->d_findptr, f_indptr, (n_nodes + 1) * sizeof(uint64_t), hipMemcpyHostToDevice));
+  EP_HIP(hipMemcpy(c->d_fids, f_ids, tot * sizeof(uint32_t), hipMemcpyHostToDevice));
+  if (c->n_vocab != n_vocab) {
+    c->release(c->d_table); c->release(c->d_master); c->release(c->d_mom); c->release(c->d_var);
+    c->release(c->base.Gd); c->release(c->base.flag); c->release(c->base.touched);
+    c->table_ready = false;
+  }
+  c->f_nodes = n_nodes;
+  c->f_total = tot;
+  c->n_vocab = n_vocab;
+  c->f_max = fmax;
+  c->cap_batch = 0;
+  return EP_OK;
+}
+
+int ep_set_table(ep_ctx* c, const float* rows) {
+  if (!c) return fail(EP_EINVAL, "NULL ctx");
+  if (!c->n_vocab) return fail(EP_ESTATE, "features must be set first");
+  EP_HIP(hipSetDevice(c->device));
+  const size_t n = (size_t)c->n_vocab * c->cfg.dims;
+  const bool bf16 = c->cfg.precision == EP_PREC_BF16;
+  if (!c->d_table) {
+    if (bf16) {
+      uint16_t* t = nullptr;
+      EP_TRY(c->alloc(&t, n));
+      c->d_table = t;
+      EP_TRY(c->alloc(&c->d_master, n));
+    } else {
+      float* t = nullptr;
+      EP_TRY(c->alloc(&t, n));
+      c->d_table = t;
+    }
+    EP_TRY(c->alloc(&c->d_mom, n));
+    EP_TRY(c->alloc(&c->d_var, n));
+  }
+  EP_HIP(hipMemsetAsync(c->d_mom, 0, n * sizeof(float), c->stream));
+  EP_HIP(hipMemsetAsync(c->d_var, 0, n * sizeof(float), c->stream));
+  refresh_base(c);
+  if (rows) {
+    float* dst = bf16 ? c->d_master : (float*)c->d_table;
+    EP_HIP(hipMemcpyAsync(dst, rows, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    if (bf16) EP_HIP(ep::launch_fill_bf16(c->d_master, (uint16_t*)c->d_table, n, c->stream));
+  } else {
+    EP_HIP(ep::launch_init_table(c->base, c->cfg.seed, c->stream));
+  }
+  EP_HIP(hipStreamSynchronize(c->stream));
+  c->table_ready = true;
+  return EP_OK;
+}
+
+int ep_get_table(ep_ctx* c, float* rows) {
+  if (!c || !rows) return fail(EP_EINVAL, "NULL argument");
+  if (!c->table_ready) return fail(EP_ESTATE, "no table");
+  EP_HIP(hipSetDevice(c->device));
+  const size_t n = (size_t)c->n_vocab * c->cfg.dims;
+  const float* src = c->cfg.precision == EP_PREC_BF16 ? c->d_master : (const float*)c->d_table;
+  EP_HIP(hipStreamSynchronize(c->stream));
+  EP_HIP(hipMemcpy(rows, src, n * sizeof(float), hipMemcpyDeviceToHost));
+  return EP_OK;
+}
+
+int ep_get_table_bf16(ep_ctx* c, uint16_t* rows) {
+  if (!c || !rows) return fail(EP_EINVAL, "NULL argument");
+  if (!c->table_ready || c->cfg.precision != EP_PREC_BF16) return fail(EP_ESTATE, "no bf16 table");
+  EP_HIP(hipSetDevice(c->device));
+  EP_HIP(hipStreamSynchronize(c->stream));
+  EP_HIP(hipMemcpy(rows, c->d_table, (size_t)c->n_vocab * c->cfg.dims * sizeof(uint16_t), hipMemcpyDeviceToHost));
+  return EP_OK;
+}
+
+int ep_get_adam(ep_ctx* c, float* m, float* v) {
+  if (!c || !m || !v) return fail(EP_EINVAL, "NULL argument");
+  if (!c->table_ready) return fail(EP_ESTATE, "no table");
+  EP_HIP(hipSetDevice(c->device));
+  const size_t n = (size_t)c->n_vocab * c->cfg.dims;
+  EP_HIP(hipStreamSynchronize(c->stream));
+  EP_HIP(hipMemcpy(m, c->d_mom, n * sizeof(float), hipMemcpyDeviceToHost));
+  EP_HIP(hipMemcpy(v, c->d_var, n * sizeof(float), hipMemcpyDeviceToHost));
+  return EP_OK;
+}
+
+int ep_set_split(ep_ctx* c, const uint32_t* train, uint64_t n_train, const uint32_t* valid, uint64_t n_valid) {
+  if (!c || (!train && n_train) || (!valid && n_valid)) return fail(EP_EINVAL, "NULL argument");
+  if (!c->n_nodes) return fail(EP_ESTATE, "graph must be set first");
+  for (uint64_t i = 0; i < n_train; ++i) if (train[i] >= c->n_nodes) return fail(EP_EINVAL, "train id out of range");
+  for (uint64_t i = 0; i < n_valid; ++i) if (valid[i] >= c->n_nodes) return fail(EP_EINVAL, "valid id out of range");
+  if (n_train == 0) return fail(EP_EINVAL, "empty train set");
+  c->train.assign(train, train + n_train);
+  c->valid.assign(valid, valid + n_valid);
+  c->split_set = true;
+  EP_HIP(hipSetDevice(c->device));
+  return upload_split(c);
+}
+
+int ep_learn(ep_ctx* c, ep_progress_fn progress, void* user) {
+  if (!c) return fail(EP_EINVAL, "NULL ctx");
+  if (!c->d_indptr || !c->d_findptr) return fail(EP_ESTATE, "graph and features must be set first");
+  if (c->f_nodes != c->n_nodes) return fail(EP_ESTATE, "features must cover every graph node");
+  EP_HIP(hipSetDevice(c->device));
+  if (!c->table_ready) EP_TRY(ep_set_table(c, nullptr));
+  if (!c->split_set) {
+    EP_TRY(ensure_split(c));
+    EP_TRY(upload_split(c));
+  } else if (!c->d_train) {
+    EP_TRY(upload_split(c));
+  }
+  EP_TRY(ensure_step_buffers(c, c->cfg.batch_size));
+  refresh_base(c);
+  ep::StepConfig sc = train_config(c);
+  const uint32_t B = c->cfg.batch_size;
+  const uint32_t spp = sc.steps_per_pass;
+  HostRng rng(c->cfg.seed ^ 0x5851F42D4C957F2Dull);
+  uint64_t next_step = 1;
+  std::vector<float> losses(c->train.size());
+  std::vector<float> vloss(c->valid.size());
+  // validation config (mod.rs:267-283): forward only, negatives from the valid set, fixed seed - 1
+  ep::StepConfig vc = c->base;
+  vc.perm = c->d_valid;
+  vc.negpool = c->d_valid;
+  vc.n_negpool = (uint32_t)c->valid.size();
+  vc.n_batch_total = (uint32_t)c->valid.size();
+  vc.train = 0;
+  vc.seed = c->cfg.seed - 1ull;
+  vc.pass_loss = c->d_valid_loss;
+  vc.steps_per_pass = (uint32_t)((c->valid.size() + B - 1) / B);
+  vc.total_updates = 1;
+  for (uint32_t pass = 1; pass <= c->cfg.passes; ++pass) {
+    EP_TRY(upload_pass_perm(c, rng));
+    EP_TRY(set_pass(c, c->d_state, pass, pass == 1 ? &next_step : nullptr));
+    for (uint32_t s = 0; s < spp; ++s) EP_TRY(run_step(c, sc));
+    EP_TRY(check_state_err(c));
+    EP_HIP(hipMemcpy(losses.data(), c->d_pass_loss, losses.size() * sizeof(float), hipMemcpyDeviceToHost));
+    // per-batch error = sum of contributing losses / batch size; pass error = mean over batches
+    double err = 0.0;
+    for (uint32_t s = 0; s < spp; ++s) {
+      const size_t lo = (size_t)s * B, hi = std::min(losses.size(), lo + B);
+      float e = 0.f;
+      for (size_t i = lo; i < hi; ++i) e += losses[i];
+      float be = e / (float)(hi - lo);
+      if (std::isinf(be)) be = 0.f;
+      err += be;
+    }
+    const float train_err = spp ? (float)(err / spp) : 0.f;
+    float valid_err = NAN;
+    if (!c->valid.empty()) {
+      EP_TRY(set_pass(c, c->d_vstate, 0, nullptr));
+      for (uint32_t s = 0; s < vc.steps_per_pass; ++s) {
+        EP_HIP(ep::launch_step_begin_lr(vc, c->d_vstate, 0.f, c->stream));
+        EP_HIP(ep::launch_sample(vc, c->d_vstate, c->stream));
+        EP_HIP(ep::launch_forward(vc, c->d_vstate, c->stream));
+      }
+      EP_HIP(hipStreamSynchronize(c->stream));
+      EP_HIP(hipMemcpy(vloss.data(), c->d_valid_loss, vloss.size() * sizeof(float), hipMemcpyDeviceToHost));
+      double vs = 0.0;
+      for (float x : vloss) vs += x;
+      valid_err = (float)(vs / vloss.size());
+    }
+    if (progress) {
+      ep::StepState hs;
+      EP_HIP(hipMemcpy(&hs, c->d_state, sizeof(hs), hipMemcpyDeviceToHost));
+      progress(user, pass, hs.next_step, train_err, valid_err, hs.lr);
+    }
+  }
+  EP_HIP(hipStreamSynchronize(c->stream));
+  return EP_OK;
+}
+
+int ep_sample_batch(ep_ctx* c, const uint32_t* anchors, uint32_t n, uint32_t pass, uint32_t* nbr_cnt, uint32_t* nbr,
+                    uint32_t* neg) {
+  if (!c || !anchors || !nbr_cnt || !nbr || !neg || n == 0) return fail(EP_EINVAL, "bad argument");
+  if (c->cfg.max_nodes_kind != EP_SAMPLE_FIXED) return fail(EP_EINVAL, "ep_sample_batch needs max_nodes Fixed");
+  for (uint32_t i = 0; i < n; ++i) if (anchors[i] >= c->n_nodes) return fail(EP_EINVAL, "anchor out of range");
+  EP_HIP(hipSetDevice(c->device));
+  if (!c->split_set) { EP_TRY(ensure_split(c)); EP_TRY(upload_split(c)); }
+  else if (!c->d_train) EP_TRY(upload_split(c));
+  EP_TRY(ensure_step_buffers(c, std::max(n, c->cfg.batch_size)));
+  refresh_base(c);
+  ep::StepConfig sc = c->base;
+  sc.perm = c->d_explicit;
+  sc.negpool = c->d_train;
+  sc.n_negpool = (uint32_t)c->train.size();
+  sc.n_batch_total = n;
+  sc.batch = n;
+  EP_HIP(hipMemcpy(c->d_explicit, anchors, n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  EP_TRY(set_pass(c, c->d_vstate, pass, nullptr));
+  EP_HIP(ep::launch_step_begin_lr(sc, c->d_vstate, 0.f, c->stream));
+  EP_HIP(ep::launch_sample(sc, c->d_vstate, c->stream));
+  EP_HIP(hipStreamSynchronize(c->stream));
+  EP_HIP(hipMemcpy(nbr_cnt, sc.nbr_cnt, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  EP_HIP(hipMemcpy(nbr, sc.nbr, (size_t)n * sc.S * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  EP_HIP(hipMemcpy(neg, sc.neg, (size_t)n * sc.K * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return EP_OK;
+}
+
+int ep_step_explicit(ep_ctx* c, const uint32_t* anchors, uint32_t n, const uint32_t* nbr_cnt, const uint32_t* nbr,
+                     const uint32_t* neg, uint32_t pass, float lr, int apply_update, float* loss, float* cons,
+                     uint32_t* n_unique, uint32_t* feat, float* grad) {
+  if (!c || !anchors || !nbr_cnt || !nbr || !neg || n == 0) return fail(EP_EINVAL, "bad argument");
+  if (c->cfg.max_nodes_kind != EP_SAMPLE_FIXED) return fail(EP_EINVAL, "ep_step_explicit needs max_nodes Fixed");
+  if (!(lr >= 0.f)) return fail(EP_EINVAL, "lr must be >= 0");
+  if (!c->d_indptr || !c->d_findptr) return fail(EP_ESTATE, "graph and features must be set first");
+  const uint32_t S = c->cfg.max_nodes, K = c->cfg.negatives, D = c->cfg.dims, NC = K + 2;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (anchors[i] >= c->n_nodes) return fail(EP_EINVAL, "anchor out of range");
+    if (nbr_cnt[i] > S) return fail(EP_EINVAL, "nbr_cnt exceeds max_nodes");
+    for (uint32_t j = 0; j < nbr_cnt[i]; ++j)
+      if (nbr[(size_t)i * S + j] >= c->n_nodes) return fail(EP_EINVAL, "neighbour out of range");
+    for (uint32_t k = 0; k < K; ++k)
+      if (neg[(size_t)i * K + k] >= c->n_nodes) return fail(EP_EINVAL, "negative out of range");
+  }
+  EP_HIP(hipSetDevice(c->device));
+  if (!c->table_ready) EP_TRY(ep_set_table(c, nullptr));
+  EP_TRY(ensure_step_buffers(c, std::max(n, c->cfg.batch_size)));
+  refresh_base(c);
+  if (cons && !c->d_xout) EP_TRY(c->alloc(&c->d_xout, (size_t)c->cap_batch * NC * D));
+  ep::StepConfig sc = c->base;
+  sc.perm = c->d_explicit;
+  sc.negpool = c->d_explicit;
+  sc.n_negpool = n;
+  sc.n_batch_total = n;
+  sc.batch = n;
+  sc.explicit_samples = 1;
+  sc.xout = cons ? c->d_xout : nullptr;
+  EP_HIP(hipMemcpy(c->d_explicit, anchors, n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  EP_HIP(hipMemcpy(sc.nbr_cnt, nbr_cnt, n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  EP_HIP(hipMemcpy(sc.nbr, nbr, (size_t)n * S * sizeof(uint32_t), hipMemcpyHostToDevice));
+  EP_HIP(hipMemcpy(sc.neg, neg, (size_t)n * K * sizeof(uint32_t), hipMemcpyHostToDevice));
+  EP_TRY(set_pass(c, c->d_state, pass, nullptr));
+  const bool upd = apply_update != 0;
+  // run everything up to (but excluding) the update, read the merged gradient, then update
+  EP_TRY(launch_training_step(c, sc, false, lr));
+  EP_TRY(check_state_err(c));
+  ep::StepState hs;
+  EP_HIP(hipMemcpy(&hs, c->d_state, sizeof(hs), hipMemcpyDeviceToHost));
+  if (loss) EP_HIP(hipMemcpy(loss, sc.loss, n * sizeof(float), hipMemcpyDeviceToHost));
+  if (cons) EP_HIP(hipMemcpy(cons, c->d_xout, (size_t)n * NC * D * sizeof(float), hipMemcpyDeviceToHost));
+  if (sc.scatter == EP_SCATTER_SORT) {
+    if (n_unique) *n_unique = hs.n_unique;
+    if (feat) EP_HIP(hipMemcpy(feat, sc.ufeat, hs.n_unique * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (grad) EP_HIP(hipMemcpy(grad, sc.G, (size_t)hs.n_unique * D * sizeof(float), hipMemcpyDeviceToHost));
+    if (upd) EP_HIP(ep::launch_adam_unique(sc, c->d_state, c->stream));
+  } else {
+    if (n_unique) *n_unique = hs.n_touched;
+    std::vector<uint32_t> tf(hs.n_touched);
+    if (hs.n_touched)
+      EP_HIP(hipMemcpy(tf.data(), sc.touched, hs.n_touched * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (feat && hs.n_touched) std::memcpy(feat, tf.data(), hs.n_touched * sizeof(uint32_t));
+    if (grad)
+      for (uint32_t i = 0; i < hs.n_touched; ++i)
+        EP_HIP(hipMemcpy(grad + (size_t)i * D, sc.Gd + (size_t)tf[i] * D, D * sizeof(float), hipMemcpyDeviceToHost));
+    if (upd) {
+      EP_HIP(ep::launch_adam_touched(sc, c->d_state, c->stream));
+    } else {
+      for (uint32_t i = 0; i < hs.n_touched; ++i) {
+        EP_HIP(hipMemsetAsync(sc.Gd + (size_t)tf[i] * D, 0, D * sizeof(float), c->stream));
+        EP_HIP(hipMemsetAsync(sc.flag + tf[i], 0, sizeof(uint32_t), c->stream));
+      }
+    }
+  }
+  EP_HIP(hipStreamSynchronize(c->stream));
+  return EP_OK;
+}
+
+int ep_bench_steps(ep_ctx* c, uint32_t warmup, uint32_t steps, int instrument, ep_bench_result* out) {
+  if (!c || !out || steps == 0) return fail(EP_EINVAL, "bad argument");
+  std::memset(out, 0, sizeof(*out));
+  EP_HIP(hipSetDevice(c->device));
+  if (!c->table_ready) EP_TRY(ep_set_table(c, nullptr));
+  if (!c->split_set) { EP_TRY(ensure_split(c)); EP_TRY(upload_split(c)); }
+  else if (!c->d_train) EP_TRY(upload_split(c));
+  EP_TRY(ensure_step_buffers(c, c->cfg.batch_size));
+  refresh_base(c);
+  ep::StepConfig sc = train_config(c);
+  sc.pass_loss = c->d_pass_loss;
+  HostRng rng(c->cfg.seed ^ 0x2545F4914F6CDD1Dull);
+  uint32_t pass = 1;
+  uint32_t in_pass = 0;
+  uint64_t next_step = 1;
+  EP_TRY(upload_pass_perm(c, rng));
+  EP_TRY(set_pass(c, c->d_state, pass, &next_step));
+  auto advance = [&]() -> int {
+    if (in_pass == sc.steps_per_pass) {
+      ++pass;
+      in_pass = 0;
+      EP_TRY(upload_pass_perm(c, rng));
+      EP_TRY(set_pass(c, c->d_state, pass, nullptr));
+    }
+    ++in_pass;
+    return EP_OK;
+  };
+  for (uint32_t i = 0; i < warmup; ++i) {
+    EP_TRY(advance());
+    EP_TRY(run_step(c, sc));
+  }
+  EP_HIP(hipStreamSynchronize(c->stream));
+  EP_TRY(check_state_err(c));
+  ep::StepState h0, h1;
+  EP_HIP(hipMemcpy(&h0, c->d_state, sizeof(h0), hipMemcpyDeviceToHost));
+  hipEvent_t e0, e1;
+  EP_HIP(hipEventCreate(&e0));
+  EP_HIP(hipEventCreate(&e1));
+  // timed steps: no host work except at pass boundaries
+  EP_HIP(hipEventRecord(e0, c->stream));
+  for (uint32_t i = 0; i < steps; ++i) {
+    if (in_pass == sc.steps_per_pass) {
+      // a pass boundary inside the timed region: reshuffle (counted in the time)
+      EP_TRY(advance());
+    } else {
+      ++in_pass;
+    }
+    EP_TRY(run_step(c, sc));
+  }
+  EP_HIP(hipEventRecord(e1, c->stream));
+  EP_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  EP_HIP(hipEventElapsedTime(&ms, e0, e1));
+  EP_TRY(check_state_err(c));
+  EP_HIP(hipMemcpy(&h1, c->d_state, sizeof(h1), hipMemcpyDeviceToHost));
+  out->ms_total = ms;
+  out->steps = steps;
+  out->examples = h1.stat_examples - h0.stat_examples;
+  out->rows = h1.stat_rows - h0.stat_rows;
+  out->nodes = h1.stat_nodes - h0.stat_nodes;
+  out->unique_rows = h1.stat_unique - h0.stat_unique;
+  if (instrument) {
+    // per-kernel timing: the same number of steps launched one by one with events
+    std::vector<hipEvent_t> ev(3 * steps);
+    for (auto& e : ev) EP_HIP(hipEventCreate(&e));
+    for (uint32_t i = 0; i < steps; ++i) {
+      if (in_pass == sc.steps_per_pass) EP_TRY(advance());
+      else ++in_pass;
+      EP_TRY(launch_training_step(c, sc, true, -1.f, ev[3 * i], ev[3 * i + 1], ev[3 * i + 2]));
+    }
+    EP_HIP(hipStreamSynchronize(c->stream));
+    double fsum = 0.0, ssum = 0.0;
+    for (uint32_t i = 0; i < steps; ++i) {
+      float a = 0.f, b = 0.f;
+      EP_HIP(hipEventElapsedTime(&a, ev[3 * i], ev[3 * i + 1]));
+      EP_HIP(hipEventElapsedTime(&b, ev[3 * i + 1], ev[3 * i + 2]));
+      fsum += a;
+      ssum += b;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    out->ms_forward_avg = fsum / steps;
+    out->ms_scatter_avg = ssum / steps;
+    out->forward_launches = steps;
+    EP_TRY(check_state_err(c));
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return EP_OK;
+}
+
+int ep_embed_averaged(ep_ctx* c, uint64_t n_nodes, const uint64_t* f_indptr, const uint32_t* f_ids, float* out) {
+  if (!c || !f_indptr || !out) return fail(EP_EINVAL, "NULL argument");
+  if (!c->table_ready) return fail(EP_ESTATE, "no table");
+  const uint64_t tot = f_indptr[n_nodes];
+  for (uint64_t i = 0; i < n_nodes; ++i)
+    if (f_indptr[i + 1] < f_indptr[i]) return fail(EP_EINVAL, "f_indptr must be non-decreasing");
+  for (uint64_t i = 0; i < tot; ++i)
+    if (f_ids[i] >= c->n_vocab) return fail(EP_EINVAL, "feature id out of range");
+  EP_HIP(hipSetDevice(c->device));
+  uint64_t* dp = nullptr;
+  uint32_t* di = nullptr;
+  float* dout = nullptr;
+  EP_TRY(c->alloc(&dp, n_nodes + 1));
+  EP_TRY(c->alloc(&di, tot));
+  EP_TRY(c->alloc(&dout, (size_t)n_nodes * c->cfg.dims));
+  EP_HIP(hipMemcpy(dp, f_indptr, (n_nodes + 1) * sizeof(uint64_t), hipMemcpyHostToDevice));
+  if (tot) EP_HIP(hipMemcpy(di, f_ids, tot * sizeof(uint32_t), hipMemcpyHostToDevice));
+  refresh_base(c);
+  EP_HIP(ep::launch_embed_averaged(c->base, dp, di, n_nodes, dout, c->stream));
+  EP_HIP(hipStreamSynchronize(c->stream));
+  EP_HIP(hipMemcpy(out, dout, (size_t)n_nodes * c->cfg.dims * sizeof(float), hipMemcpyDeviceToHost));
+  c->release(dp);
+  c->release(di);
+  c->release(dout);
+  return EP_OK;
+}
+
+int ep_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  if (!ctr || !key || !out) return fail(EP_EINVAL, "NULL argument");
+  const ep::u32x4 r = ep::philox4x32(ep::u32x4{ctr[0], ctr[1], ctr[2], ctr[3]}, key[0], key[1]);
+  out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = r.w;
+  return EP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1205 @@
+// HIP kernels for one Embedding Propagation training step on gfx950 (MI355X).
+//
+// Step = k_step_begin -> k_sample -> k_scan -> k_forward -> k_emit ->
+//        {radix sort -> k_segreduce -> k_pieces -> k_long_combine -> k_adam_unique}
+//     or {k_atomic_scatter -> k_adam_touched}
+//
+// Every kernel reads the per-step values (batch offset, batch size, learning rate,
+// key count) from the device-resident StepState, so the whole step can be captured
+// once in a hipGraph and replayed for every batch of a pass.  Grids are sized from
+// host-known capacities and every kernel exits early (or grid-strides) against the
+// device-side counts.
+#include "ep_device.h"
+#include "ep_kernels.h"
+
+namespace ep {
+
+static constexpr int NW = 4;             // waves per 256-thread block
+static constexpr uint32_t LONG_SEG = 256;  // segments longer than this are split into pieces
+static constexpr uint32_t PIECE = 256;     // entries per piece of a long segment
+static constexpr uint32_t HASH_EMPTY = 0xFFFFFFFFu;
+static constexpr uint32_t HASH_FBITS = 27;
+static constexpr uint32_t HASH_FMASK = (1u << HASH_FBITS) - 1u;
+
+// ------------------------------------------------------------------------------------
+// Helpers
+// ------------------------------------------------------------------------------------
+
+// Accumulate rows table[fids[fs..fe)] into acc (lane slice), U row loads in flight.
+template <int D, int U, typename T>
+__device__ inline void gather_rows(const T* __restrict__ table, const uint32_t* __restrict__ fids,
+                                   uint64_t fs, uint64_t fe, int lane, float (&acc)[RowShape<D>::PL]) {
+  constexpr int PL = RowShape<D>::PL;
+  for (uint64_t base = fs; base < fe; base += EP_WAVE) {
+    const int n = (int)((fe - base) < (uint64_t)EP_WAVE ? (fe - base) : (uint64_t)EP_WAVE);
+    const uint32_t myf = lane < n ? fids[base + lane] : 0u;
+    for (int j = 0; j < n; j += U) {
+      float r[U][PL];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int t = (j + u < n) ? (j + u) : (n - 1);
+        const uint32_t f = __shfl(myf, t, EP_WAVE);
+        load_slice<D>(table + (size_t)f * D, lane, r[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (j + u < n) {
+#pragma unroll
+          for (int q = 0; q < PL; ++q) acc[q] += r[u][q];
+        }
+      }
+    }
+  }
+}
+
+// 256-thread block exclusive scan of one value per thread; returns the block total in *total.
+__device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t* s_wave, uint32_t* total) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  uint32_t inc = x;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(inc, off, EP_WAVE);
+    if (lane >= off) inc += y;
+  }
+  if (lane == 63) s_wave[w] = inc;
+  __syncthreads();
+  uint32_t pre = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) {
+    const uint32_t sw = s_wave[i];
+    if (i < w) pre += sw;
+    tot += sw;
+  }
+  __syncthreads();
+  *total = tot;
+  return pre + inc - x;
+}
+
+// First index p in [lo, hi) with K[p] != f (K sorted ascending and K[i] == f before p),
+// or hi.  64-ary search: all lanes probe at once, ~log64(span) dependent rounds.
+__device__ inline uint32_t wave_upper_bound(const uint32_t* __restrict__ K, uint32_t lo, uint32_t hi, uint32_t f) {
+  const uint32_t lane = lane_id();
+  while (true) {
+    const uint32_t span = hi - lo;
+    if (span == 0) return lo;
+    const uint32_t step = (span + 63u) / 64u;
+    const uint64_t off = (uint64_t)lane * step;
+    const bool valid = off < span;
+    const uint32_t kv = valid ? K[lo + (uint32_t)off] : f;
+    const uint64_t m = __ballot(valid && kv != f);
+    if (m == 0ull) {
+      const uint32_t lmax = (span - 1u) / step;
+      lo = lo + lmax * step + 1u;
+      if (step == 1u) return lo;
+      continue;
+    }
+    const uint32_t L = (uint32_t)(__ffsll((unsigned long long)m) - 1);
+    if (step == 1u) return lo + L;
+    hi = lo + L * step;
+    lo = L > 0 ? lo + (L - 1u) * step + 1u : lo;
+  }
+}
+
+__device__ inline uint32_t hash_slot(uint32_t f, uint32_t mask) {
+  return (f * 0x9E3779B1u) & mask;
+}
+
+// Insert (f, c) keeping the minimum construction index per feature.  Entries pack
+// (c << 27) | f, so for a fixed f the minimum packed value has the minimum c.
+__device__ inline void hash_insert_min(uint32_t* h, uint32_t mask, uint32_t f, uint32_t c) {
+  const uint32_t kv = (c << HASH_FBITS) | f;
+  uint32_t i = hash_slot(f, mask);
+  while (true) {
+    const uint32_t old = atomicCAS(&h[i], HASH_EMPTY, kv);
+    if (old == HASH_EMPTY) return;
+    if ((old & HASH_FMASK) == f) {
+      atomicMin(&h[i], kv);
+      return;
+    }
+    i = (i + 1u) & mask;
+  }
+}
+
+__device__ inline uint32_t hash_lookup(const uint32_t* h, uint32_t mask, uint32_t f) {
+  uint32_t i = hash_slot(f, mask);
+  while (true) {
+    const uint32_t val = h[i];
+    if (val == HASH_EMPTY) return 0xFFu;
+    if ((val & HASH_FMASK) == f) return val >> HASH_FBITS;
+    i = (i + 1u) & mask;
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// k_step_begin: advance the batch cursor, LR schedule and Adam bias corrections.
+// LRScheduler::cos_decay (scheduler.rs:39-47) as built at mod.rs:132-137, in f32.
+// ------------------------------------------------------------------------------------
+__global__ void k_step_begin(StepConfig c, StepState* st, float lr_override) {
+  uint32_t* dt = &st->dtot[0][0];
+  for (int i = threadIdx.x; i < 4 * 256; i += blockDim.x) dt[i] = 0u;
+  if (threadIdx.x == 0) {
+    const uint32_t sip = st->step_in_pass + 1u;  // host writes 0xFFFFFFFF at pass start
+    st->step_in_pass = sip;
+    const uint64_t base64 = (uint64_t)sip * c.batch;
+    const uint32_t base = (uint32_t)(base64 < c.n_batch_total ? base64 : c.n_batch_total);
+    st->anchor_base = base;
+    const uint32_t rem = c.n_batch_total - base;
+    st->batch_n = rem < c.batch ? rem : c.batch;
+    const uint64_t cur = st->next_step;
+    st->next_step = cur + 1u;
+    float lr;
+    if (lr_override >= 0.f) {
+      lr = lr_override;
+    } else {
+      const float alpha = c.alpha;
+      const float mn = alpha / 100.0f;
+      const uint64_t warm = (uint64_t)((float)c.total_updates / 5.0f);
+      if (cur > warm) {
+        const float ratio = (float)cur / (float)c.total_updates;
+        lr = mn + 0.5f * (alpha - mn) * (1.0f + cosf(3.14159265358979323846f * ratio));
+      } else {
+        const float ratio = (float)cur / (float)warm;
+        lr = mn + ratio * (alpha - mn);
+      }
+    }
+    st->lr = lr;
+    const float t = (float)st->pass + 1.0f;  // optimizer.rs:50: t = pass + 1
+    st->bc1 = 1.0f - powf(0.9f, t);
+    st->bc2 = 1.0f - powf(0.999f, t);
+    st->n_keys = 0u;
+    st->n_unique = 0u;
+    st->n_long = 0u;
+    st->n_pieces = 0u;
+    st->n_touched = 0u;
+    st->stat_examples += st->batch_n;
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// k_sample: neighbour sample (S of deg without replacement, Floyd's algorithm) and K
+// uniform-with-replacement negatives per anchor; R_e = feature occurrences of the example.
+// model.rs:354-377 + utils.rs:193-216 (Fixed(S): all neighbours when deg <= S) and
+// node_sampler.rs:185-188 (negatives uniform from the pool).
+// ------------------------------------------------------------------------------------
+struct Draws {
+  u32x4 r;
+  int left;
+  uint32_t blk;
+};
+
+__device__ inline uint32_t next_draw(Draws& d, uint32_t pos, uint32_t pass, uint32_t purpose, uint32_t k0, uint32_t k1) {
+  if (d.left == 0) {
+    d.r = philox4x32(u32x4{pos, pass, purpose, d.blk++}, k0, k1);
+    d.left = 4;
+  }
+  --d.left;
+  return d.left == 3 ? d.r.x : d.left == 2 ? d.r.y : d.left == 1 ? d.r.z : d.r.w;
+}
+
+__global__ __launch_bounds__(256) void k_sample(StepConfig c, StepState* st) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= c.batch) return;
+  const uint32_t bn = st->batch_n;
+  if (b >= bn) {
+    c.rcount[b] = 0u;
+    c.nbr_cnt[b] = 0u;
+    return;
+  }
+  const uint32_t pos = st->anchor_base + b;
+  const uint32_t pass = st->pass;
+  const uint32_t v = c.perm[pos];
+  const uint64_t s0 = c.indptr[v], s1 = c.indptr[v + 1];
+  const uint32_t deg = (uint32_t)(s1 - s0);
+  uint32_t* out = c.nbr + (size_t)b * c.S;
+  uint32_t* negs = c.neg + (size_t)b * c.K;
+  const uint32_t k0 = (uint32_t)c.seed, k1 = (uint32_t)(c.seed >> 32);
+  uint32_t cnt;
+  if (!c.explicit_samples) {
+    if (c.all_nbrs) {
+      cnt = deg;
+    } else if (deg <= c.S) {
+      cnt = deg;
+      for (uint32_t j = 0; j < deg; ++j) out[j] = c.indices[s0 + j];
+    } else {
+      cnt = c.S;
+      Draws d{u32x4{0, 0, 0, 0}, 0, 0};
+      for (uint32_t i = 0; i < c.S; ++i) {
+        const uint32_t hi = deg - c.S + i;  // draw t in [0, hi]
+        const uint32_t t = uniform_below(next_draw(d, pos, pass, RNG_NEIGHBOURS, k0, k1), hi + 1u);
+        bool dup = false;
+        for (uint32_t q = 0; q < i; ++q) dup |= (out[q] == t);
+        out[i] = dup ? hi : t;
+      }
+      for (uint32_t i = 0; i < c.S; ++i) out[i] = c.indices[s0 + out[i]];
+    }
+    Draws d{u32x4{0, 0, 0, 0}, 0, 0};
+    for (uint32_t k = 0; k < c.K; ++k)
+      negs[k] = c.negpool[uniform_below(next_draw(d, pos, pass, RNG_NEGATIVES, k0, k1), c.n_negpool)];
+    c.nbr_cnt[b] = cnt;
+  } else {
+    cnt = c.nbr_cnt[b];
+  }
+  uint32_t R = 0;
+  if (cnt > 0u) {
+    R += (uint32_t)(c.findptr[v + 1] - c.findptr[v]);
+    const uint32_t* pn = c.all_nbrs ? c.indices + s0 : out;
+    for (uint32_t j = 0; j < cnt; ++j) {
+      const uint32_t u = pn[j];
+      R += (uint32_t)(c.findptr[u + 1] - c.findptr[u]);
+    }
+    for (uint32_t k = 0; k < c.K; ++k) {
+      const uint32_t u = negs[k];
+      R += (uint32_t)(c.findptr[u + 1] - c.findptr[u]);
+    }
+  }
+  c.rcount[b] = R;
+}
+
+// ------------------------------------------------------------------------------------
+// k_scan: exclusive scan of R_e over the batch -> key offsets; total -> st->n_keys.
+// ------------------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_scan(StepConfig c, StepState* st) {
+  __shared__ uint32_t s_w[16];
+  __shared__ uint32_t s_carry;
+  __shared__ unsigned long long s_nodes;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  if (tid == 0) { s_carry = 0u; s_nodes = 0ull; }
+  __syncthreads();
+  uint64_t nodes = 0;
+  for (uint32_t base = 0; base < c.batch; base += 1024) {
+    const uint32_t i = base + tid;
+    const uint32_t x = i < c.batch ? c.rcount[i] : 0u;
+    if (i < c.batch && i < st->batch_n) {
+      const uint32_t cn = c.nbr_cnt[i];
+      if (cn > 0u) nodes += cn + c.K;
+    }
+    uint32_t inc = x;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(inc, off, EP_WAVE);
+      if (lane >= off) inc += y;
+    }
+    if (lane == 63) s_w[w] = inc;
+    __syncthreads();
+    uint32_t pre = s_carry, tot = 0;
+    for (int q = 0; q < 16; ++q) {
+      if (q < w) pre += s_w[q];
+      tot += s_w[q];
+    }
+    if (i < c.batch) c.keyoff[i] = pre + inc - x;
+    __syncthreads();
+    if (tid == 0) s_carry += tot;
+    __syncthreads();
+  }
+  atomicAdd(&s_nodes, (unsigned long long)nodes);
+  __syncthreads();
+  if (tid == 0) {
+    c.keyoff[c.batch] = s_carry;
+    st->n_keys = s_carry;
+    st->stat_rows += s_carry;
+    st->stat_nodes += s_nodes;
+    if (s_carry > c.key_cap) st->err = 1u;  // capacity error (checked by the host)
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// k_forward: one 256-thread block per anchor.
+//   gather + mean of h(v), ~h(v), h(u_1..K)     (model.rs:257-301, 342-416, 448-463)
+//   margin loss (loss.rs:59-73) + loss weighting (mod.rs:190-200)
+//   adjoints dL/dx of every construction, eligibility (finite, in the loss graph)
+// Neighbour features of ~h(v) are split over the 4 waves (fixed-order combine);
+// the 1+K single-node constructions are dealt round-robin to the waves.
+// ------------------------------------------------------------------------------------
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
+  constexpr int PL = RowShape<D>::PL;
+  constexpr int U = 8;
+  const uint32_t b = blockIdx.x;
+  if (b >= st->batch_n) return;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint32_t K = c.K, NC = K + 2u;
+  extern __shared__ float sm[];
+  float* xc = sm;                 // NC*D
+  float* part = xc + NC * D;      // NW*D
+  float* dist = part + NW * D;    // K+1
+  float* sn = dist + K + 1;       // NC occurrence counts
+  __shared__ float s_scale, s_invA;
+  __shared__ int s_contrib;
+  __shared__ uint32_t s_active;
+  const T* table = (const T*)c.table;
+  const uint32_t v = c.perm[st->anchor_base + b];
+  const uint32_t cnt = c.nbr_cnt[b];
+  if (cnt == 0u) {  // no neighbourhood: no example
+    if (tid == 0) {
+      c.loss[b] = 0.f;
+      if (c.pass_loss) c.pass_loss[st->anchor_base + b] = 0.f;
+    }
+    if (tid < (int)NC) c.elig[(size_t)b * NC + tid] = 0;
+    return;
+  }
+  const uint64_t vs = c.indptr[v];
+  const uint32_t deg = (uint32_t)(c.indptr[v + 1] - vs);
+  const uint32_t* pn = c.all_nbrs ? c.indices + vs : c.nbr + (size_t)b * c.S;
+  const int base = lane * PL;
+
+  // ~h(v): neighbour nodes dealt round-robin to the waves.
+  {
+    float acc[PL];
+#pragma unroll
+    for (int q = 0; q < PL; ++q) acc[q] = 0.f;
+    uint32_t np = 0;
+    for (uint32_t j = w; j < cnt; j += NW) {
+      const uint32_t u = pn[j];
+      const uint64_t fs = c.findptr[u], fe = c.findptr[u + 1];
+      gather_rows<D, U>(table, c.fids, fs, fe, lane, acc);
+      np += (uint32_t)(fe - fs);
+    }
+#pragma unroll
+    for (int q = 0; q < PL; ++q)
+      if (base + q < D) part[w * D + base + q] = acc[q];
+    __shared__ uint32_t s_np[NW];
+    if (lane == 0) s_np[w] = np;
+    // single-node constructions: i = 0 -> h(v) (c=0), i >= 1 -> h(u_i) (c=i+1)
+    for (uint32_t i = w; i < K + 1u; i += NW) {
+      const uint32_t node = i == 0u ? v : c.neg[(size_t)b * K + i - 1u];
+      const uint32_t ci = i == 0u ? 0u : i + 1u;
+      float a2[PL];
+#pragma unroll
+      for (int q = 0; q < PL; ++q) a2[q] = 0.f;
+      const uint64_t fs = c.findptr[node], fe = c.findptr[node + 1];
+      gather_rows<D, U>(table, c.fids, fs, fe, lane, a2);
+      const float n = (float)(fe - fs);
+#pragma unroll
+      for (int q = 0; q < PL; ++q)
+        if (base + q < D) xc[ci * D + base + q] = a2[q] / n;
+      if (lane == 0) sn[ci] = n;
+    }
+    __syncthreads();
+    if (tid == 0) sn[1] = (float)(s_np[0] + s_np[1] + s_np[2] + s_np[3]);
+    __syncthreads();
+  }
+  {
+    const float ntot = sn[1];
+    for (int t = tid; t < D; t += blockDim.x)
+      xc[D + t] = (((part[t] + part[D + t]) + part[2 * D + t]) + part[3 * D + t]) / ntot;
+  }
+  __syncthreads();
+  // distances: D0 = |p - a|, D_i = |p - n_i|
+  for (uint32_t i = w; i < K + 1u; i += NW) {
+    const float* other = (i == 0u) ? xc : xc + (i + 1u) * D;
+    float sq = 0.f;
+#pragma unroll
+    for (int q = 0; q < PL; ++q) {
+      if (base + q < D) {
+        const float d = xc[D + base + q] - other[base + q];
+        sq += d * d;
+      }
+    }
+    sq = wave_sum(sq);
+    if (lane == 0) dist[i] = sqrtf(sq);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const float d1 = c.gamma + dist[0];
+    float sum = 0.f;
+    uint32_t nA = 0, act = 0;
+    for (uint32_t i = 1; i <= K; ++i) {
+      const float l = d1 - dist[i];
+      if (l > 0.f) {
+        sum += l;
+        ++nA;
+        act |= 1u << (i - 1u);
+      }
+    }
+    const float L = nA ? sum / (float)nA : 0.f;
+    float div = 1.f;
+    if (c.lw_kind == 1) div = logf(1.f + (float)deg);
+    else if (c.lw_kind == 2) div = powf((float)deg, c.lw_exp);
+    const float Lw = L / div;
+    const int contrib = (Lw > 0.f) && !isnan(Lw);
+    s_contrib = contrib;
+    s_active = act;
+    s_invA = nA ? 1.f / (float)nA : 0.f;
+    s_scale = 1.f / div;
+    c.loss[b] = contrib ? Lw : 0.f;
+    if (c.pass_loss) c.pass_loss[st->anchor_base + b] = contrib ? Lw : 0.f;
+  }
+  __syncthreads();
+  if (c.xout) {
+    float* xo = c.xout + (size_t)b * NC * D;
+    for (uint32_t t = tid; t < NC * D; t += blockDim.x) xo[t] = xc[t];
+  }
+  if (!c.train) return;
+  const int contrib = s_contrib;
+  const uint32_t act = s_active;
+  const float invA = s_invA, scale = s_scale, D0 = dist[0];
+  for (uint32_t ci = w; ci < NC; ci += NW) {
+    bool e = false;
+    float g[PL];
+#pragma unroll
+    for (int q = 0; q < PL; ++q) g[q] = 0.f;
+    if (contrib) {
+      if (ci == 0u) {
+        e = true;
+#pragma unroll
+        for (int q = 0; q < PL; ++q)
+          if (base + q < D) g[q] = (xc[base + q] - xc[D + base + q]) / D0 * scale;
+      } else if (ci == 1u) {
+        e = true;
+#pragma unroll
+        for (int q = 0; q < PL; ++q) {
+          if (base + q < D) {
+            const float p = xc[D + base + q];
+            float gq = (p - xc[base + q]) / D0;
+            for (uint32_t i = 1; i <= K; ++i)
+              if (act & (1u << (i - 1u))) gq -= (p - xc[(i + 1u) * D + base + q]) / dist[i] * invA;
+            g[q] = gq * scale;
+          }
+        }
+      } else {
+        const uint32_t i = ci - 1u;
+        if (act & (1u << (i - 1u))) {
+          e = true;
+#pragma unroll
+          for (int q = 0; q < PL; ++q)
+            if (base + q < D) g[q] = (xc[D + base + q] - xc[ci * D + base + q]) / dist[i] * invA * scale;
+        }
+      }
+      if (e) {
+        bool bad = false;
+#pragma unroll
+        for (int q = 0; q < PL; ++q) bad |= (base + q < D) && !isfinite(g[q]);
+        if (wave_any(bad)) e = false;
+      }
+    }
+    if (e) store_slice<D>(c.adj + ((size_t)b * NC + ci) * D, lane, g);
+    if (lane == 0) {
+      c.elig[(size_t)b * NC + ci] = e ? 1 : 0;
+      c.coef[(size_t)b * NC + ci] = 1.0f / sn[ci];
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// k_emit: one block per anchor.  Writes one (feature, slot) key per feature occurrence,
+// in construction order a, p, n_1..n_K, at keyoff[b].  Occurrences of ineligible
+// constructions, and (first_leaf) occurrences outside the feature's first eligible
+// construction (mod.rs:347-352, 366-367), get the sentinel key.
+// ------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_emit(StepConfig c, StepState* st) {
+  const uint32_t b = blockIdx.x;
+  if (b >= st->batch_n) return;
+  if (st->n_keys > c.key_cap) return;
+  const uint32_t cnt = c.nbr_cnt[b];
+  if (cnt == 0u) return;
+  const int tid = threadIdx.x;
+  const uint32_t K = c.K, NC = K + 2u;
+  extern __shared__ uint32_t hsh[];
+  __shared__ uint32_t s_wave[NW];
+  __shared__ uint8_t s_el[32];
+  __shared__ uint32_t s_off[257];
+  __shared__ uint32_t s_node[256];
+  __shared__ uint8_t s_ci[256];
+  const uint32_t H = c.hash_entries, hmask = H - 1u;
+  const bool first_leaf = c.grad_mode == 0;
+  const uint32_t v = c.perm[st->anchor_base + b];
+  const uint32_t* pn = c.all_nbrs ? c.indices + c.indptr[v] : c.nbr + (size_t)b * c.S;
+  if (tid < (int)NC) s_el[tid] = c.elig[(size_t)b * NC + tid];
+  if (first_leaf)
+    for (uint32_t i = tid; i < H; i += blockDim.x) hsh[i] = HASH_EMPTY;
+  __syncthreads();
+  const uint32_t E = 1u + cnt + K;
+  const uint32_t kbase = c.keyoff[b];
+  const uint32_t slot0 = b * NC;
+  for (int phase = first_leaf ? 0 : 1; phase < 2; ++phase) {
+    uint32_t running = 0;
+    for (uint32_t cs = 0; cs < E; cs += blockDim.x) {
+      const uint32_t e = cs + tid;
+      uint32_t ne = 0, node = 0;
+      uint8_t ci = 0;
+      if (e < E) {
+        if (e == 0u) { node = v; ci = 0; }
+        else if (e <= cnt) { node = pn[e - 1u]; ci = 1; }
+        else { node = c.neg[(size_t)b * K + (e - cnt - 1u)]; ci = (uint8_t)(2u + (e - cnt - 1u)); }
+        ne = (uint32_t)(c.findptr[node + 1] - c.findptr[node]);
+      }
+      uint32_t total;
+      const uint32_t excl = block_exclusive_scan(ne, s_wave, &total);
+      s_off[tid] = excl;
+      s_node[tid] = node;
+      s_ci[tid] = ci;
+      if (tid == 0) s_off[256] = total;
+      __syncthreads();
+      const uint32_t m = (E - cs) < blockDim.x ? (E - cs) : blockDim.x;
+      for (uint32_t o = tid; o < total; o += blockDim.x) {
+        // entry j with s_off[j] <= o < s_off[j+1] (entries beyond m have zero length)
+        uint32_t lo = 0, hi = m;  // find last j with s_off[j] <= o
+        while (hi - lo > 1u) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (s_off[mid] <= o) lo = mid; else hi = mid;
+        }
+        const uint32_t j = lo;
+        const uint32_t nd = s_node[j];
+        const uint32_t cj = s_ci[j];
+        const uint32_t f = c.fids[c.findptr[nd] + (o - s_off[j])];
+        const bool ok = s_el[cj] != 0;
+        if (phase == 0) {
+          if (ok) hash_insert_min(hsh, hmask, f, cj);
+        } else {
+          uint32_t key = c.sentinel;
+          if (ok && (!first_leaf || hash_lookup(hsh, hmask, f) == cj)) key = f;
+          const uint32_t pos = kbase + running + o;
+          c.keys[0][pos] = key;
+          c.slots[0][pos] = slot0 + cj;
+        }
+      }
+      running += total;
+      __syncthreads();
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// LSD radix sort of (key, slot) pairs with the element count read from device memory.
+// Stable: within a round, ranks follow thread order (wave-level match via ballots);
+// rounds and blocks are ordered through per-block digit bases.
+// ------------------------------------------------------------------------------------
+__device__ inline void block_range(uint32_t n, int G, uint32_t blk, uint32_t* lo, uint32_t* hi) {
+  const uint64_t per = (((uint64_t)n + G - 1) / G + 255u) / 256u * 256u;
+  const uint64_t l = (uint64_t)blk * per;
+  const uint64_t h = l + per;
+  *lo = (uint32_t)(l < n ? l : n);
+  *hi = (uint32_t)(h < n ? h : n);
+}
+
+__global__ __launch_bounds__(256) void k_rs_hist(const uint32_t* __restrict__ kin, StepState* st, uint32_t* ghist,
+                                                 int pass, int shift, uint32_t mask, int G) {
+  __shared__ uint32_t h[256];
+  const int tid = threadIdx.x;
+  uint32_t lo, hi;
+  block_range(st->n_keys, G, blockIdx.x, &lo, &hi);
+  h[tid] = 0u;
+  __syncthreads();
+  for (uint32_t i = lo + tid; i < hi; i += 256u) atomicAdd(&h[(kin[i] >> shift) & mask], 1u);
+  __syncthreads();
+  ghist[(size_t)tid * G + blockIdx.x] = h[tid];
+  if (h[tid]) atomicAdd(&st->dtot[pass][tid], h[tid]);
+}
+
+__global__ __launch_bounds__(256) void k_rs_scan(uint32_t* ghist, StepState* st, int pass, int G) {
+  __shared__ uint32_t s_wave[NW];
+  __shared__ uint32_t s_red[256];
+  const int tid = threadIdx.x;
+  const uint32_t d = blockIdx.x;
+  s_red[tid] = (uint32_t)tid < d ? st->dtot[pass][tid] : 0u;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (tid < off) s_red[tid] += s_red[tid + off];
+    __syncthreads();
+  }
+  const uint32_t pre = s_red[0];
+  uint32_t* row = ghist + (size_t)d * G;
+  const int per = G / 256;  // G is a multiple of 256
+  uint32_t loc = 0;
+  for (int i = 0; i < per; ++i) loc += row[tid * per + i];
+  uint32_t total;
+  uint32_t run = pre + block_exclusive_scan(loc, s_wave, &total);
+  for (int i = 0; i < per; ++i) {
+    const uint32_t x = row[tid * per + i];
+    row[tid * per + i] = run;
+    run += x;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                    StepState* st, const uint32_t* __restrict__ ghist,
+                                                    int shift, uint32_t mask, int bits, int G) {
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t wc[NW][256];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  uint32_t lo, hi;
+  block_range(st->n_keys, G, blockIdx.x, &lo, &hi);
+  run[tid] = ghist[(size_t)tid * G + blockIdx.x];
+#pragma unroll
+  for (int q = 0; q < NW; ++q) wc[q][tid] = 0xFFFFFFFFu;
+  __syncthreads();
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint32_t r = 0;
+  for (uint32_t i0 = lo; i0 < hi; i0 += 256u, ++r) {
+    const uint32_t i = i0 + tid;
+    const bool act = i < hi;
+    const uint32_t k = act ? kin[i] : 0u;
+    const uint32_t val = act ? vin[i] : 0u;
+    const uint32_t d = (k >> shift) & mask;
+    uint64_t peers = __ballot(act);
+    for (int bit = 0; bit < bits; ++bit) {
+      const bool set = (d >> bit) & 1u;
+      const uint64_t bb = __ballot(act && set);
+      peers &= set ? bb : ~bb;
+    }
+    if (!act) peers = 0ull;
+    const uint32_t rank = (uint32_t)__popcll(peers & lt);
+    const uint32_t cw = (uint32_t)__popcll(peers);
+    const uint32_t tag = r << 7;
+    if (act && rank == cw - 1u) wc[w][d] = tag | cw;
+    __syncthreads();
+    if (act) {
+      uint32_t off = run[d];
+      for (int q = 0; q < w; ++q) {
+        const uint32_t x = wc[q][d];
+        if ((x >> 7) == r) off += x & 0x7Fu;
+      }
+      kout[off + rank] = k;
+      vout[off + rank] = val;
+    }
+    __syncthreads();
+    {
+      uint32_t tot = 0;
+#pragma unroll
+      for (int q = 0; q < NW; ++q) {
+        const uint32_t x = wc[q][tid];
+        if ((x >> 7) == r) tot += x & 0x7Fu;
+      }
+      run[tid] += tot;
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Segment reduce over the sorted keys.  G_f = sum over entries of coef[slot]*adj[slot]
+// (the batch merge of mod.rs:223-233).  Each wave owns the segments that START in its
+// 64-entry chunk; the last of them may run past the chunk (<= LONG_SEG entries) or be
+// deferred as a long segment split into PIECE-entry pieces.
+// ------------------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(256) void k_segreduce(StepConfig c, StepState* st, const uint32_t* __restrict__ Ks,
+                                                   const uint32_t* __restrict__ Ss) {
+  constexpr int PL = RowShape<D>::PL;
+  constexpr int U = 8;
+  const int lane = threadIdx.x & 63;
+  const uint32_t gw = blockIdx.x * NW + (threadIdx.x >> 6);
+  const uint32_t nwaves = gridDim.x * NW;
+  const uint32_t n = st->n_keys <= c.key_cap ? st->n_keys : 0u;
+  const uint32_t SENT = c.sentinel;
+  for (uint64_t chunk = gw; chunk * 64u < n; chunk += nwaves) {
+    const uint32_t i0 = (uint32_t)(chunk * 64u);
+    const uint32_t j = i0 + lane;
+    const uint32_t f = j < n ? Ks[j] : SENT;
+    if (__shfl(f, 0, EP_WAVE) == SENT) break;  // sorted: the rest is sentinel
+    uint32_t prev = __shfl_up(f, 1, EP_WAVE);
+    if (lane == 0) prev = i0 > 0u ? Ks[i0 - 1u] : ~f;
+    const bool is_start = (f != SENT) && (f != prev);
+    const uint64_t starts = __ballot(is_start);
+    if (starts == 0ull) continue;
+    const int qf = __ffsll((unsigned long long)starts) - 1;
+    const int ql = 63 - __clzll((unsigned long long)starts);
+    const uint32_t fl = __shfl(f, ql, EP_WAVE);
+    const uint64_t diff = __ballot(lane > ql && f != fl);
+    uint32_t end;
+    if (diff) {
+      end = i0 + (uint32_t)(__ffsll((unsigned long long)diff) - 1);
+    } else {
+      const uint32_t st0 = i0 + (uint32_t)ql;
+      const uint64_t lim64 = (uint64_t)st0 + LONG_SEG + 1u;
+      const uint32_t lim = (uint32_t)(lim64 < n ? lim64 : n);
+      const uint32_t from = i0 + 64u < lim ? i0 + 64u : lim;
+      end = wave_upper_bound(Ks, from, lim, fl);
+      if (end - st0 > LONG_SEG) {
+        end = wave_upper_bound(Ks, lim, n, fl);
+        const uint32_t len = end - st0;
+        const uint32_t pcs = (len + PIECE - 1u) / PIECE;
+        uint32_t idx = 0, pb = 0;
+        if (lane == 0) {
+          idx = atomicAdd(&st->n_long, 1u);
+          pb = atomicAdd(&st->n_pieces, pcs);
+          if (idx < c.long_cap) {
+            c.long_st[idx] = st0;
+            c.long_len[idx] = len;
+            c.long_pbase[idx] = pb;
+          }
+        }
+        idx = __shfl(idx, 0, EP_WAVE);
+        pb = __shfl(pb, 0, EP_WAVE);
+        if (idx < c.long_cap)
+          for (uint32_t q = lane; q < pcs; q += 64u)
+            if (pb + q < c.piece_cap) c.piece_seg[pb + q] = idx;
+        end = st0;  // the flat loop stops before the deferred segment
+      }
+    }
+    float acc[PL];
+#pragma unroll
+    for (int q = 0; q < PL; ++q) acc[q] = 0.f;
+    for (uint32_t g0 = i0 + (uint32_t)qf; g0 < end; g0 += 64u) {
+      const uint32_t jj = g0 + lane;
+      const bool valid = jj < end;
+      const uint32_t fj = valid ? Ks[jj] : SENT;
+      const uint32_t sj = valid ? Ss[jj] : 0u;
+      const float cj = valid ? c.coef[sj] : 0.f;
+      uint32_t fn = __shfl_down(fj, 1, EP_WAVE);
+      if (lane == 63) fn = (jj + 1u < end) ? Ks[jj + 1u] : SENT;
+      const bool is_end = valid && (fn != fj);
+      const uint64_t ends = __ballot(is_end);
+      uint32_t ub = 0;
+      if (lane == 0 && ends) ub = atomicAdd(&st->n_unique, (uint32_t)__popcll(ends));
+      ub = __shfl(ub, 0, EP_WAVE);
+      const int m = (int)((end - g0) < 64u ? (end - g0) : 64u);
+      for (int t0 = 0; t0 < m; t0 += U) {
+        float r[U][PL];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int t = (t0 + u < m) ? (t0 + u) : (m - 1);
+          const uint32_t s = __shfl(sj, t, EP_WAVE);
+          load_slice<D>(c.adj + (size_t)s * D, lane, r[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int t = t0 + u;
+          if (t < m) {
+            const float cc = __shfl(cj, t, EP_WAVE);
+#pragma unroll
+            for (int q = 0; q < PL; ++q) acc[q] += cc * r[u][q];
+            if ((ends >> t) & 1ull) {
+              const uint32_t idx = ub + (uint32_t)__popcll(ends & ((t == 0) ? 0ull : (~0ull >> (64 - t))));
+              store_slice<D>(c.G + (size_t)idx * D, lane, acc);
+              const uint32_t ft = __shfl(fj, t, EP_WAVE);
+              if (lane == 0) c.ufeat[idx] = ft;
+#pragma unroll
+              for (int q = 0; q < PL; ++q) acc[q] = 0.f;
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// Partial sums of the pieces of long segments.
+template <int D>
+__global__ __launch_bounds__(256) void k_pieces(StepConfig c, StepState* st, const uint32_t* __restrict__ Ss) {
+  constexpr int PL = RowShape<D>::PL;
+  constexpr int U = 8;
+  const int lane = threadIdx.x & 63;
+  const uint32_t gw = blockIdx.x * NW + (threadIdx.x >> 6);
+  const uint32_t nwaves = gridDim.x * NW;
+  const uint32_t np = st->n_pieces < c.piece_cap ? st->n_pieces : c.piece_cap;
+  for (uint32_t gp = gw; gp < np; gp += nwaves) {
+    const uint32_t idx = c.piece_seg[gp];
+    if (idx >= c.long_cap) continue;
+    const uint32_t s0 = c.long_st[idx] + (gp - c.long_pbase[idx]) * PIECE;
+    const uint32_t segend = c.long_st[idx] + c.long_len[idx];
+    const uint32_t s1 = s0 + PIECE < segend ? s0 + PIECE : segend;
+    float acc[PL];
+#pragma unroll
+    for (int q = 0; q < PL; ++q) acc[q] = 0.f;
+    for (uint32_t g0 = s0; g0 < s1; g0 += 64u) {
+      const uint32_t jj = g0 + lane;
+      const bool valid = jj < s1;
+      const uint32_t sj = valid ? Ss[jj] : 0u;
+      const float cj = valid ? c.coef[sj] : 0.f;
+      const int m = (int)((s1 - g0) < 64u ? (s1 - g0) : 64u);
+      for (int t0 = 0; t0 < m; t0 += U) {
+        float r[U][PL];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int t = (t0 + u < m) ? (t0 + u) : (m - 1);
+          const uint32_t s = __shfl(sj, t, EP_WAVE);
+          load_slice<D>(c.adj + (size_t)s * D, lane, r[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int t = t0 + u;
+          if (t < m) {
+            const float cc = __shfl(cj, t, EP_WAVE);
+#pragma unroll
+            for (int q = 0; q < PL; ++q) acc[q] += cc * r[u][q];
+          }
+        }
+      }
+    }
+    store_slice<D>(c.part + (size_t)gp * D, lane, acc);
+  }
+}
+
+// Combine the pieces of each long segment in piece order (deterministic).
+template <int D>
+__global__ __launch_bounds__(256) void k_long_combine(StepConfig c, StepState* st, const uint32_t* __restrict__ Ks) {
+  constexpr int PL = RowShape<D>::PL;
+  __shared__ float red[NW][D];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int base = lane * PL;
+  const uint32_t nl = st->n_long < c.long_cap ? st->n_long : c.long_cap;
+  for (uint32_t idx = blockIdx.x; idx < nl; idx += gridDim.x) {
+    const uint32_t pb = c.long_pbase[idx];
+    const uint32_t pcs = (c.long_len[idx] + PIECE - 1u) / PIECE;
+    float acc[PL];
+#pragma unroll
+    for (int q = 0; q < PL; ++q) acc[q] = 0.f;
+    for (uint32_t j = w; j < pcs; j += NW) {
+      if (pb + j >= c.piece_cap) break;
+      float r[PL];
+      load_slice<D>(c.part + (size_t)(pb + j) * D, lane, r);
+#pragma unroll
+      for (int q = 0; q < PL; ++q) acc[q] += r[q];
+    }
+#pragma unroll
+    for (int q = 0; q < PL; ++q)
+      if (base + q < D) red[w][base + q] = acc[q];
+    __syncthreads();
+    if (w == 0) {
+      float s[PL];
+#pragma unroll
+      for (int q = 0; q < PL; ++q)
+        s[q] = (base + q < D) ? (((red[0][base + q] + red[1][base + q]) + red[2][base + q]) + red[3][base + q]) : 0.f;
+      uint32_t u = 0;
+      if (lane == 0) u = atomicAdd(&st->n_unique, 1u);
+      u = __shfl(u, 0, EP_WAVE);
+      store_slice<D>(c.G + (size_t)u * D, lane, s);
+      if (lane == 0) c.ufeat[u] = Ks[c.long_st[idx]];
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Sparse lazy Adam on one row (optimizer.rs:43-81): skip if any NaN in g (line 56);
+// m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2; theta -= lr (m/bc1) / (sqrt(v/bc2) + eps).
+// bf16 mode: the FP32 master row is updated and written back to the bf16 table with
+// stochastic rounding.
+// ------------------------------------------------------------------------------------
+template <int D>
+__device__ inline void adam_row(const StepConfig& c, const StepState* st, uint32_t f, const float (&g)[RowShape<D>::PL],
+                                int lane) {
+  constexpr int PL = RowShape<D>::PL;
+  const int base = lane * PL;
+  bool bad = false;
+#pragma unroll
+  for (int q = 0; q < PL; ++q) bad |= (base + q < D) && isnan(g[q]);
+  if (wave_any(bad)) return;
+  const size_t off = (size_t)f * D;
+  float m[PL], v[PL], th[PL];
+  load_slice<D>(c.mom + off, lane, m);
+  load_slice<D>(c.var + off, lane, v);
+  if (c.bf16) load_slice<D>(c.master + off, lane, th);
+  else load_slice<D>((const float*)c.table + off, lane, th);
+  const float lr = st->lr, bc1 = st->bc1, bc2 = st->bc2;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+#pragma unroll
+  for (int q = 0; q < PL; ++q) {
+    m[q] = b1 * m[q] + (1.0f - b1) * g[q];
+    v[q] = b2 * v[q] + (1.0f - b2) * (g[q] * g[q]);
+    const float mh = m[q] / bc1;
+    const float vh = v[q] / bc2;
+    th[q] -= lr * (mh / (sqrtf(vh) + eps));
+  }
+  store_slice<D>(c.mom + off, lane, m);
+  store_slice<D>(c.var + off, lane, v);
+  if (c.bf16) {
+    store_slice<D>(c.master + off, lane, th);
+    const u32x4 r = philox4x32(u32x4{f, (uint32_t)lane, (uint32_t)st->next_step, RNG_ROUNDING},
+                               (uint32_t)c.seed, (uint32_t)(c.seed >> 32));
+    const uint32_t rw[4] = {r.x, r.y, r.z, r.w};
+    uint16_t* row = (uint16_t*)c.table + off;
+#pragma unroll
+    for (int q = 0; q < PL; ++q) {
+      if (base + q < D) {
+        const uint32_t bits16 = (q & 1) ? (rw[(q >> 1) & 3] >> 16) : (rw[(q >> 1) & 3] & 0xffffu);
+        row[base + q] = f32_to_bf16_sr(th[q], bits16);
+      }
+    }
+  } else {
+    store_slice<D>((float*)c.table + off, lane, th);
+  }
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_adam_unique(StepConfig c, StepState* st) {
+  constexpr int PL = RowShape<D>::PL;
+  const int lane = threadIdx.x & 63;
+  const uint32_t gw = blockIdx.x * NW + (threadIdx.x >> 6);
+  const uint32_t nwaves = gridDim.x * NW;
+  const uint32_t nu = st->n_unique;
+  if (gw == 0 && lane == 0) atomicAdd((unsigned long long*)&st->stat_unique, (unsigned long long)nu);
+  for (uint32_t u = gw; u < nu; u += nwaves) {
+    const uint32_t f = c.ufeat[u];
+    float g[PL];
+    load_slice<D>(c.G + (size_t)u * D, lane, g);
+    adam_row<D>(c, st, f, g, lane);
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Atomic fast path: scatter coef*adj straight into a dense FP32 gradient table with
+// float atomics (emission order, no sort), then Adam over the touched rows, which also
+// re-zeroes their gradient rows and flags.  Differs from the sorted path only in the
+// order of the FP32 additions.
+// ------------------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(256) void k_atomic_scatter(StepConfig c, StepState* st) {
+  constexpr int PL = RowShape<D>::PL;
+  constexpr int U = 8;
+  const int lane = threadIdx.x & 63;
+  const int base = lane * PL;
+  const uint32_t gw = blockIdx.x * NW + (threadIdx.x >> 6);
+  const uint32_t nwaves = gridDim.x * NW;
+  const uint32_t n = st->n_keys <= c.key_cap ? st->n_keys : 0u;
+  const uint32_t SENT = c.sentinel;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (uint64_t chunk = gw; chunk * 64u < n; chunk += nwaves) {
+    const uint32_t i0 = (uint32_t)(chunk * 64u);
+    const uint32_t j = i0 + lane;
+    const bool valid = j < n;
+    const uint32_t f = valid ? c.keys[0][j] : SENT;
+    const uint32_t s = valid ? c.slots[0][j] : 0u;
+    const bool real = f != SENT;
+    const float cj = real ? c.coef[s] : 0.f;
+    const bool newt = real && atomicCAS(&c.flag[f], 0u, 1u) == 0u;
+    const uint64_t nm = __ballot(newt);
+    uint32_t tb = 0;
+    if (lane == 0 && nm) tb = atomicAdd(&st->n_touched, (uint32_t)__popcll(nm));
+    tb = __shfl(tb, 0, EP_WAVE);
+    if (newt) c.touched[tb + (uint32_t)__popcll(nm & lt)] = f;
+    const uint64_t rm = __ballot(real);
+    if (!rm) continue;
+    const int m = 64 - __clzll((unsigned long long)rm);  // entries up to the last real one
+    for (int t0 = 0; t0 < m; t0 += U) {
+      float r[U][PL];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int t = (t0 + u < m) ? (t0 + u) : (m - 1);
+        const uint32_t st_ = __shfl(s, t, EP_WAVE);
+        load_slice<D>(c.adj + (size_t)st_ * D, lane, r[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int t = t0 + u;
+        if (t < m && ((rm >> t) & 1ull)) {
+          const float cc = __shfl(cj, t, EP_WAVE);
+          const uint32_t ft = __shfl(f, t, EP_WAVE);
+          float* grow = c.Gd + (size_t)ft * D;
+#pragma unroll
+          for (int q = 0; q < PL; ++q)
+            if (base + q < D) atomicAdd(grow + base + q, cc * r[u][q]);
+        }
+      }
+    }
+  }
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_adam_touched(StepConfig c, StepState* st) {
+  constexpr int PL = RowShape<D>::PL;
+  const int lane = threadIdx.x & 63;
+  const uint32_t gw = blockIdx.x * NW + (threadIdx.x >> 6);
+  const uint32_t nwaves = gridDim.x * NW;
+  const uint32_t nt = st->n_touched;
+  if (gw == 0 && lane == 0) atomicAdd((unsigned long long*)&st->stat_unique, (unsigned long long)nt);
+  for (uint32_t u = gw; u < nt; u += nwaves) {
+    const uint32_t f = c.touched[u];
+    float* grow = c.Gd + (size_t)f * D;
+    float g[PL], z[PL];
+    load_slice<D>(grow, lane, g);
+#pragma unroll
+    for (int q = 0; q < PL; ++q) z[q] = 0.f;
+    store_slice<D>(grow, lane, z);
+    if (lane == 0) c.flag[f] = 0u;
+    adam_row<D>(c, st, f, g, lane);
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Random init (mod.rs:381-392): x = 2u - 1 per value, then each row divided by its L2 norm.
+// ------------------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(256) void k_init_table(StepConfig c, uint64_t seed) {
+  constexpr int PL = RowShape<D>::PL;
+  const int lane = threadIdx.x & 63;
+  const int base = lane * PL;
+  const uint64_t gw = (uint64_t)blockIdx.x * NW + (threadIdx.x >> 6);
+  const uint64_t nwaves = (uint64_t)gridDim.x * NW;
+  for (uint64_t row = gw; row < c.n_vocab; row += nwaves) {
+    float x[PL];
+    const u32x4 r0 = philox4x32(u32x4{(uint32_t)row, (uint32_t)(row >> 32), (uint32_t)lane, RNG_INIT},
+                                (uint32_t)seed, (uint32_t)(seed >> 32));
+    const uint32_t rw[4] = {r0.x, r0.y, r0.z, r0.w};
+    float sq = 0.f;
+#pragma unroll
+    for (int q = 0; q < PL; ++q) {
+      x[q] = (base + q < D) ? (2.0f * uniform01(rw[q & 3]) - 1.0f) : 0.f;
+      sq += x[q] * x[q];
+    }
+    const float nrm = sqrtf(wave_sum(sq));
+#pragma unroll
+    for (int q = 0; q < PL; ++q) x[q] = x[q] / nrm;
+    if (c.bf16) {
+      store_slice<D>(c.master + row * D, lane, x);
+      uint16_t* t = (uint16_t*)c.table + row * D;
+#pragma unroll
+      for (int q = 0; q < PL; ++q)
+        if (base + q < D) t[base + q] = f32_to_bf16_rne(x[q]);
+    } else {
+      store_slice<D>((float*)c.table + row * D, lane, x);
+    }
+  }
+}
+
+__global__ void k_fill_bf16(const float* __restrict__ src, uint16_t* __restrict__ dst, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = f32_to_bf16_rne(src[i]);
+}
+
+// ------------------------------------------------------------------------------------
+// Averaged inference aggregate (aggregator.rs:34-47): node row = mean of its feature rows.
+// ------------------------------------------------------------------------------------
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_embed_avg(const T* __restrict__ table, const uint64_t* __restrict__ findptr,
+                                                   const uint32_t* __restrict__ fids, uint64_t n_nodes,
+                                                   float* __restrict__ out) {
+  constexpr int PL = RowShape<D>::PL;
+  const int lane = threadIdx.x & 63;
+  const uint64_t gw = (uint64_t)blockIdx.x * NW + (threadIdx.x >> 6);
+  const uint64_t nwaves = (uint64_t)gridDim.x * NW;
+  for (uint64_t node = gw; node < n_nodes; node += nwaves) {
+    float acc[PL];
+#pragma unroll
+    for (int q = 0; q < PL; ++q) acc[q] = 0.f;
+    const uint64_t fs = findptr[node], fe = findptr[node + 1];
+    gather_rows<D, 8>(table, fids, fs, fe, lane, acc);
+    const float n = (float)(fe - fs);
+#pragma unroll
+    for (int q = 0; q < PL; ++q) acc[q] = fe > fs ? acc[q] / n : 0.f;
+    store_slice<D>(out + node * D, lane, acc);
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Dispatch
+// ------------------------------------------------------------------------------------
+#define EP_DIMS(X) X(8) X(16) X(32) X(64) X(100) X(128) X(256)
+
+bool dims_supported(int d) {
+#define EP_CASE(DD) if (d == DD) return true;
+  EP_DIMS(EP_CASE)
+#undef EP_CASE
+  return false;
+}
+
+#define EP_DISPATCH(d, BODY)                 \
+  switch (d) {                               \
+    case 8: { constexpr int D = 8; BODY; } break;     \
+    case 16: { constexpr int D = 16; BODY; } break;   \
+    case 32: { constexpr int D = 32; BODY; } break;   \
+    case 64: { constexpr int D = 64; BODY; } break;   \
+    case 100: { constexpr int D = 100; BODY; } break; \
+    case 128: { constexpr int D = 128; BODY; } break; \
+    case 256: { constexpr int D = 256; BODY; } break; \
+    default: return hipErrorInvalidValue;    \
+  }
+
+hipError_t launch_step_begin(const StepConfig& c, StepState* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(256), 0, s, c, st, -1.0f);
+  return hipGetLastError();
+}
+
+hipError_t launch_step_begin_lr(const StepConfig& c, StepState* st, float lr, hipStream_t s) {
+  hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(256), 0, s, c, st, lr);
+  return hipGetLastError();
+}
+
+hipError_t launch_sample(const StepConfig& c, StepState* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_sample, dim3((c.batch + 255) / 256), dim3(256), 0, s, c, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan(const StepConfig& c, StepState* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, c, st);
+  return hipGetLastError();
+}
+
+size_t forward_lds_bytes(const StepConfig& c) {
+  const size_t NC = c.K + 2;
+  return ((NC + NW) * c.dims + (c.K + 1) + NC) * sizeof(float);
+}
+
+hipError_t launch_forward(const StepConfig& c, StepState* st, hipStream_t s) {
+  const size_t lds = forward_lds_bytes(c);
+  EP_DISPATCH(c.dims, {
+    if (c.bf16) hipLaunchKernelGGL((k_forward<D, uint16_t>), dim3(c.batch), dim3(256), lds, s, c, st);
+    else hipLaunchKernelGGL((k_forward<D, float>), dim3(c.batch), dim3(256), lds, s, c, st);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_emit(const StepConfig& c, StepState* st, hipStream_t s) {
+  const size_t lds = c.grad_mode == 0 ? (size_t)c.hash_entries * sizeof(uint32_t) : 0;
+  hipLaunchKernelGGL(k_emit, dim3(c.batch), dim3(256), lds, s, c, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_sort(const StepConfig& c, StepState* st, hipStream_t s, int* out_buf) {
+  int cur = 0;
+  const int G = c.sort_grid;
+  for (int p = 0, shift = 0; shift < c.sort_bits; ++p, shift += 8) {
+    const int bits = (c.sort_bits - shift) < 8 ? (c.sort_bits - shift) : 8;
+    const uint32_t mask = (1u << bits) - 1u;
+    hipLaunchKernelGGL(k_rs_hist, dim3(G), dim3(256), 0, s, c.keys[cur], st, c.ghist, p, shift, mask, G);
+    hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(256), 0, s, c.ghist, st, p, G);
+    hipLaunchKernelGGL(k_rs_scatter, dim3(G), dim3(256), 0, s, c.keys[cur], c.slots[cur], c.keys[1 - cur],
+                       c.slots[1 - cur], st, c.ghist, shift, mask, bits, G);
+    cur = 1 - cur;
+  }
+  *out_buf = cur;
+  return hipGetLastError();
+}
+
+hipError_t launch_segreduce(const StepConfig& c, StepState* st, int buf, hipStream_t s) {
+  const int grid = c.unique_grid;
+  EP_DISPATCH(c.dims, {
+    hipLaunchKernelGGL((k_segreduce<D>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
+    hipLaunchKernelGGL((k_pieces<D>), dim3(grid), dim3(256), 0, s, c, st, c.slots[buf]);
+    hipLaunchKernelGGL((k_long_combine<D>), dim3(256), dim3(256), 0, s, c, st, c.keys[buf]);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_adam_unique(const StepConfig& c, StepState* st, hipStream_t s) {
+  EP_DISPATCH(c.dims, { hipLaunchKernelGGL((k_adam_unique<D>), dim3(c.unique_grid), dim3(256), 0, s, c, st); });
+  return hipGetLastError();
+}
+
+hipError_t launch_atomic_scatter(const StepConfig& c, StepState* st, hipStream_t s) {
+  EP_DISPATCH(c.dims, { hipLaunchKernelGGL((k_atomic_scatter<D>), dim3(c.unique_grid), dim3(256), 0, s, c, st); });
+  return hipGetLastError();
+}
+
+hipError_t launch_adam_touched(const StepConfig& c, StepState* st, hipStream_t s) {
+  EP_DISPATCH(c.dims, { hipLaunchKernelGGL((k_adam_touched<D>), dim3(c.unique_grid), dim3(256), 0, s, c, st); });
+  return hipGetLastError();
+}
+
+hipError_t launch_init_table(const StepConfig& c, uint64_t seed, hipStream_t s) {
+  EP_DISPATCH(c.dims, { hipLaunchKernelGGL((k_init_table<D>), dim3(2048), dim3(256), 0, s, c, seed); });
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_bf16(const float* src, uint16_t* dst, uint64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_bf16, dim3(2048), dim3(256), 0, s, src, dst, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_embed_averaged(const StepConfig& c, const uint64_t* findptr, const uint32_t* fids,
+                                 uint64_t n_nodes, float* out, hipStream_t s) {
+  EP_DISPATCH(c.dims, {
+    if (c.bf16)
+      hipLaunchKernelGGL((k_embed_avg<D, uint16_t>), dim3(2048), dim3(256), 0, s, (const uint16_t*)c.table, findptr,
+                         fids, n_nodes, out);
+    else
+      hipLaunchKernelGGL((k_embed_avg<D, float>), dim3(2048), dim3(256), 0, s, (const float*)c.table, findptr, fids,
+                         n_nodes, out);
+  });
+  return hipGetLastError();
+}
+
+}  // namespace ep

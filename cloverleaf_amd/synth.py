@@ -1,0 +1,148 @@
+"""Seeded synthetic graphs and feature sets (ctypes wrapper of libcloverleaf_synth.so).
+
+All generators are deterministic for a seed (independent of the thread count).
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_double, c_int, c_uint32, c_uint64, c_void_p
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libcloverleaf_synth.so"
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m cloverleaf_amd.build_native`")
+        lib = ctypes.CDLL(str(LIB_PATH))
+        u64pp, u32pp = POINTER(POINTER(c_uint64)), POINTER(POINTER(c_uint32))
+        lib.synth_free.argtypes = [c_void_p]
+        lib.synth_free.restype = None
+        lib.synth_chung_lu.argtypes = [c_uint64, c_uint64, c_double, c_double, c_int, c_uint64, u64pp, u32pp,
+                                       POINTER(c_uint64)]
+        lib.synth_chung_lu.restype = c_int
+        lib.synth_bipartite.argtypes = [c_uint64, c_uint64, c_uint64, c_double, c_double, c_uint64, u64pp, u32pp,
+                                        POINTER(c_uint64)]
+        lib.synth_bipartite.restype = c_int
+        lib.synth_features.argtypes = [c_uint64, c_int, c_double, c_double, c_uint32, c_double, c_uint64, u64pp,
+                                       u32pp]
+        lib.synth_features.restype = c_int
+        _lib = lib
+    return _lib
+
+
+def _take(p, n, dtype):
+    arr = np.ctypeslib.as_array(p, shape=(n,)).astype(dtype, copy=True)
+    _load().synth_free(ctypes.cast(p, c_void_p))
+    return arr
+
+
+def chung_lu(n: int, m: int, beta: float, dmax: float = 0.0, directed: bool = False, seed: int = 1):
+    """CSR (indptr u64[n+1], indices u32) of a Chung-Lu graph with m sampled edges."""
+    ip, ix, ne = POINTER(c_uint64)(), POINTER(c_uint32)(), c_uint64(0)
+    rc = _load().synth_chung_lu(n, m, beta, dmax, 1 if directed else 0, seed, ctypes.byref(ip), ctypes.byref(ix),
+                                ctypes.byref(ne))
+    if rc != 0:
+        raise ValueError("synth_chung_lu: bad arguments")
+    return _take(ip, n + 1, np.uint64), _take(ix, max(ne.value, 1), np.uint32)[:ne.value]
+
+
+def bipartite(nq: int, ni: int, m: int, beta: float, dmax: float = 0.0, seed: int = 1):
+    """CSR of an undirected bipartite Chung-Lu graph (queries [0,nq), items [nq, nq+ni))."""
+    ip, ix, ne = POINTER(c_uint64)(), POINTER(c_uint32)(), c_uint64(0)
+    rc = _load().synth_bipartite(nq, ni, m, beta, dmax, seed, ctypes.byref(ip), ctypes.byref(ix), ctypes.byref(ne))
+    if rc != 0:
+        raise ValueError("synth_bipartite: bad arguments")
+    n = nq + ni
+    return _take(ip, n + 1, np.uint64), _take(ix, max(ne.value, 1), np.uint32)[:ne.value]
+
+
+COUNT_POISSON, COUNT_UNIFORM, COUNT_LOGUNIFORM = 0, 1, 2
+
+
+def features(n: int, count_kind: int, p1: float, p2: float, vocab: int, zipf_s: float, seed: int = 2):
+    """Feature CSR (f_indptr u64[n+1], f_ids u32) with Zipf(zipf_s) ids over `vocab`."""
+    ip, ix = POINTER(c_uint64)(), POINTER(c_uint32)()
+    rc = _load().synth_features(n, count_kind, p1, p2, vocab, zipf_s, seed, ctypes.byref(ip), ctypes.byref(ix))
+    if rc != 0:
+        raise ValueError("synth_features: bad arguments")
+    f_indptr = _take(ip, n + 1, np.uint64)
+    total = int(f_indptr[-1])
+    return f_indptr, _take(ix, max(total, 1), np.uint32)[:total]
+
+
+@dataclass
+class Workload:
+    """One benchmark configuration: graph + features + model hyper-parameters."""
+    name: str
+    description: str
+    indptr: np.ndarray
+    indices: np.ndarray
+    f_indptr: np.ndarray
+    f_ids: np.ndarray
+    vocab: int
+    dims: int
+    batch: int
+    negatives: int
+    max_nodes: int
+    gamma: float
+    alpha: float
+    passes: int
+
+    @property
+    def n_nodes(self) -> int:
+        return len(self.indptr) - 1
+
+
+# Workload table.  Graph seed 1, feature seed 2 (train seed 20222022 lives in the config).
+# dmax is the target largest expected degree of the Chung-Lu weights.
+WORKLOADS = {
+    "ep-cfg1-10k": dict(
+        description="Chung-Lu undirected N=10k, 100k edges, beta=2.5; features 1+Poisson(7), Zipf(1.1) over 50k; "
+                    "d=64, S=10, K=5, margin 0.5, B=128, lr 0.01, 5 passes",
+        graph=("chung_lu", dict(n=10_000, m=100_000, beta=2.5, dmax=1_000, directed=False)),
+        feats=dict(count_kind=COUNT_POISSON, p1=7.0, p2=0.0, vocab=50_000, zipf_s=1.1),
+        dims=64, batch=128, negatives=5, max_nodes=10, gamma=0.5, alpha=0.01, passes=5),
+    "ep-cfg2-1m": dict(
+        description="bipartite query(300k)-item(700k) Chung-Lu, 20M undirected edges, beta=2.2; features U{1..32}, "
+                    "Zipf(1.1) over 1M; d=128, B=1024, K=10, S=20, margin 0.5, lr 0.01",
+        graph=("bipartite", dict(nq=300_000, ni=700_000, m=20_000_000, beta=2.2, dmax=10_000)),
+        feats=dict(count_kind=COUNT_UNIFORM, p1=1.0, p2=32.0, vocab=1_000_000, zipf_s=1.1),
+        dims=128, batch=1024, negatives=10, max_nodes=20, gamma=0.5, alpha=0.01, passes=1),
+    "ep-cfg3-10m": dict(
+        description="Chung-Lu directed N=10M, 200M edges, beta=2.1 (max degree ~1e5); features log-uniform [1,128], "
+                    "Zipf(1.05) over 5M; d=256, B=8192, S=32, K=20",
+        graph=("chung_lu", dict(n=10_000_000, m=200_000_000, beta=2.1, dmax=100_000, directed=True)),
+        feats=dict(count_kind=COUNT_LOGUNIFORM, p1=1.0, p2=128.0, vocab=5_000_000, zipf_s=1.05),
+        dims=256, batch=8192, negatives=20, max_nodes=32, gamma=0.5, alpha=0.01, passes=1),
+}
+
+
+def make_workload(name: str, scale: float = 1.0, seed_graph: int = 1, seed_feat: int = 2) -> Workload:
+    """Generates a named workload; `scale` < 1 shrinks nodes, edges and vocab proportionally."""
+    spec = WORKLOADS[name]
+    kind, g = spec["graph"]
+    g = dict(g)
+    if kind == "chung_lu":
+        n = max(16, int(g["n"] * scale))
+        m = max(16, int(g["m"] * scale))
+        indptr, indices = chung_lu(n, m, g["beta"], g["dmax"] * (scale if scale < 1 else 1), g["directed"], seed_graph)
+    else:
+        nq = max(8, int(g["nq"] * scale))
+        ni = max(8, int(g["ni"] * scale))
+        m = max(16, int(g["m"] * scale))
+        indptr, indices = bipartite(nq, ni, m, g["beta"], g["dmax"] * (scale if scale < 1 else 1), seed_graph)
+        n = nq + ni
+    f = dict(spec["feats"])
+    vocab = max(16, int(f["vocab"] * scale))
+    f_indptr, f_ids = features(n, f["count_kind"], f["p1"], f["p2"], vocab, f["zipf_s"], seed_feat)
+    return Workload(name=name, description=spec["description"], indptr=indptr, indices=indices, f_indptr=f_indptr,
+                    f_ids=f_ids, vocab=vocab, dims=spec["dims"], batch=spec["batch"], negatives=spec["negatives"],
+                    max_nodes=spec["max_nodes"], gamma=spec["gamma"], alpha=spec["alpha"], passes=spec["passes"])

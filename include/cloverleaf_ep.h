@@ -1,0 +1,172 @@
+/*
+ * cloverleaf_ep.h -- C ABI of the MI355X Embedding Propagation (EP) training step.
+ *
+ * This is the drop-in boundary for the path
+ *     EmbeddingPropagator.learn_features(graph, features, feature_embeddings)
+ * of the reference (src/lib.rs:1327-1370), which calls
+ *     EmbeddingPropagation::learn(&G, &FeatureStore, Option<EmbeddingStore>, &Model)
+ * (src/algos/ep/mod.rs:80-89 -> learn_feature_embeddings 92-287).
+ *
+ * Conventions
+ *   - every call returns 0 on success or a negative EP_E* code; the message of the
+ *     last failure on the calling thread is available from ep_last_error();
+ *   - host buffers are caller-owned and copied in or out; device memory is owned by
+ *     the ep_ctx;
+ *   - one host thread per context; calls on one context are not re-entrant;
+ *   - ids are uint32 (node ids < 2^32, feature ids < 2^27 - 1), CSR offsets uint64.
+ */
+#ifndef CLOVERLEAF_EP_H
+#define CLOVERLEAF_EP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  EP_OK = 0,
+  EP_EINVAL = -1,      /* bad argument / unsupported option                    */
+  EP_ESTATE = -2,      /* call out of order (e.g. learn before set_graph)       */
+  EP_EHIP = -3,        /* HIP runtime error                                     */
+  EP_ENOMEM = -4,      /* device allocation failed                              */
+  EP_ECAPACITY = -5,   /* a step exceeded a preallocated device capacity        */
+};
+
+/* Sample kinds of max_nodes / max_features (src/algos/utils.rs:166-191).       */
+enum { EP_SAMPLE_ALL = 0, EP_SAMPLE_FIXED = 1 };
+/* LossWeighting (src/algos/ep/mod.rs:31-36).                                   */
+enum { EP_LW_NONE = 0, EP_LW_LOG = 1, EP_LW_EXPONENTIAL = 2 };
+enum { EP_PREC_FP32 = 0, EP_PREC_BF16 = 1 };
+/* Gradient extraction: the reference's first-leaf rule (mod.rs:347-352,366-367)
+ * or the exact gradient (sum over every construction).                         */
+enum { EP_GRAD_FIRST_LEAF = 0, EP_GRAD_EXACT = 1 };
+/* Batch merge + update: sort + segment-reduce (deterministic) or float atomics. */
+enum { EP_SCATTER_SORT = 0, EP_SCATTER_ATOMIC = 1 };
+
+/* Mirrors EmbeddingPropagation (src/algos/ep/mod.rs:40-75) plus the model options
+ * of EmbeddingPropagator::new (src/lib.rs:1205-1307).                           */
+typedef struct ep_config {
+  uint32_t dims;               /* d_model; 8,16,32,64,100,128,256                  */
+  uint32_t batch_size;         /* default 50                                        */
+  uint32_t passes;             /* default 100                                       */
+  uint32_t negatives;          /* MarginLoss negatives, default 1                   */
+  float alpha;                 /* learning rate, default 0.9                        */
+  float gamma;                 /* MarginLoss margin, default 1.0                    */
+  float valid_pct;             /* default 0.1                                       */
+  int32_t max_nodes_kind;      /* EP_SAMPLE_ALL (default) / EP_SAMPLE_FIXED         */
+  uint32_t max_nodes;          /* S when Fixed                                      */
+  int32_t max_features_kind;   /* EP_SAMPLE_ALL only                                */
+  uint32_t max_features;
+  int32_t loss_weighting;      /* EP_LW_*                                           */
+  float loss_weight_exp;       /* exponent of EP_LW_EXPONENTIAL                     */
+  uint32_t hard_negatives;     /* must be 0                                         */
+  float noise;                 /* must be 0                                         */
+  int32_t weighted_neighbor_sampling;   /* must be 0                                */
+  int32_t weighted_neighbor_averaging;  /* must be 0                                */
+  int32_t precision;           /* EP_PREC_*                                         */
+  int32_t grad_mode;           /* EP_GRAD_*                                         */
+  int32_t scatter;             /* EP_SCATTER_*                                      */
+  int32_t use_graph;           /* 1: replay each step from a captured hipGraph      */
+  uint64_t seed;               /* default 20222022 (src/lib.rs:90)                  */
+  int32_t device;              /* HIP device ordinal                                */
+} ep_config;
+
+typedef struct ep_ctx ep_ctx;
+
+/* Progress callback, once per pass: pass (1-based), global step, mean train
+ * error (mod.rs:256-265), mean validation loss (or NaN), current LR.            */
+typedef void (*ep_progress_fn)(void* user, uint32_t pass, uint64_t step, float train_err, float valid_err,
+                               float lr);
+
+/* Per-run measurement of ep_bench_steps. */
+typedef struct ep_bench_result {
+  double ms_total;             /* wall time of the timed steps (HIP events)        */
+  double ms_forward_avg;       /* average duration of the gather/loss kernel        */
+  double ms_scatter_avg;       /* average duration of scatter+update (all kernels)  */
+  uint64_t examples;           /* anchors processed in the timed steps              */
+  uint64_t rows;               /* feature-row reads (sum of R_e) in the timed steps */
+  uint64_t nodes;              /* sampled neighbour + negative ids read              */
+  uint64_t unique_rows;        /* rows updated by Adam in the timed steps            */
+  uint32_t steps;
+  uint32_t forward_launches;
+} ep_bench_result;
+
+/* Fills the reference defaults (EmbeddingPropagator::new, src/lib.rs:1262-1274). */
+int ep_config_default(ep_config* cfg);
+
+/* Creates a context (device memory, stream).  Replaces the construction of
+ * EmbeddingPropagation + AveragedFeatureModel (src/lib.rs:1262-1306).          */
+int ep_create(const ep_config* cfg, ep_ctx** out);
+void ep_destroy(ep_ctx* ctx);
+
+/* Uploads the CumCSR adjacency (src/graph.rs:399-550; only offsets and columns
+ * are used by the unweighted path).  indptr[n_nodes+1], indices[indptr[n]].    */
+int ep_set_graph(ep_ctx* ctx, uint64_t n_nodes, const uint64_t* indptr, const uint32_t* indices);
+
+/* Uploads the FeatureStore (src/feature_store.rs:19-25) as a CSR: node i has
+ * features f_ids[f_indptr[i]..f_indptr[i+1]).  Every node must have >= 1 feature
+ * (FeatureStore::fill_missing_nodes, feature_store.rs:83-89).                  */
+int ep_set_features(ep_ctx* ctx, uint64_t n_nodes, uint32_t n_vocab, const uint64_t* f_indptr,
+                    const uint32_t* f_ids);
+
+/* Sets the feature table (n_vocab x dims, row-major FP32); rows == NULL draws the
+ * random init of randomize_embedding_store (src/algos/ep/mod.rs:381-392) on the
+ * device.  Resets the Adam state (optimizer.rs:34-38).                          */
+int ep_set_table(ep_ctx* ctx, const float* rows);
+
+/* Copies the FP32 table (FP32 master rows in BF16 mode) to rows[n_vocab*dims].  */
+int ep_get_table(ep_ctx* ctx, float* rows);
+/* Copies the BF16 working table (BF16 mode only) to rows[n_vocab*dims].         */
+int ep_get_table_bf16(ep_ctx* ctx, uint16_t* rows);
+/* Copies the Adam moments (optimizer.rs:29-30) to m, v [n_vocab*dims].          */
+int ep_get_adam(ep_ctx* ctx, float* m, float* v);
+
+/* Sets the train / validation node ids.  ep_learn draws its own split
+ * (mod.rs:119-122) when this has not been called.                               */
+int ep_set_split(ep_ctx* ctx, const uint32_t* train, uint64_t n_train, const uint32_t* valid, uint64_t n_valid);
+
+/* The training loop of learn_feature_embeddings (mod.rs:92-287): passes x
+ * ceil(n_train/B) steps of sample -> forward -> backward -> merge -> Adam, with
+ * the cosine LR schedule and per-pass validation.                               */
+int ep_learn(ep_ctx* ctx, ep_progress_fn progress, void* user);
+
+/* Test hook: the sampler of one batch (model.rs:354-377, node_sampler.rs:185-188)
+ * for the given anchors at (pass, position base).  nbr_cnt[n], nbr[n*S],
+ * neg[n*K].  Requires max_nodes Fixed.                                          */
+int ep_sample_batch(ep_ctx* ctx, const uint32_t* anchors, uint32_t n, uint32_t pass, uint32_t* nbr_cnt,
+                    uint32_t* nbr, uint32_t* neg);
+
+/* Test hook: one training step on explicit samples (the body of the per-batch
+ * closure, mod.rs:176-252).  anchors[n], nbr_cnt[n], nbr[n*S] (S = max_nodes),
+ * neg[n*K].  lr and pass are given.  Outputs (each nullable):
+ *   loss[n]                per-example weighted loss (0 when no gradient)
+ *   cons[n*(K+2)*dims]     constructions h(v), ~h(v), h(u_1..K)
+ *   n_unique, feat[], grad[]  merged batch gradient (mod.rs:223-233); feat and
+ *                          grad must hold up to n_vocab entries / rows
+ * apply_update != 0 runs the Adam update (optimizer.rs:43-81).                  */
+int ep_step_explicit(ep_ctx* ctx, const uint32_t* anchors, uint32_t n, const uint32_t* nbr_cnt,
+                     const uint32_t* nbr, const uint32_t* neg, uint32_t pass, float lr, int apply_update,
+                     float* loss, float* cons, uint32_t* n_unique, uint32_t* feat, float* grad);
+
+/* Measurement hook: runs `warmup` then `steps` training steps from the current
+ * split with everything resident in HBM, timed with HIP events on the context
+ * stream.  Per-kernel timings are taken in a separate instrumented run of the
+ * same number of steps when instrument != 0.                                    */
+int ep_bench_steps(ep_ctx* ctx, uint32_t warmup, uint32_t steps, int instrument, ep_bench_result* out);
+
+/* NodeEmbedder(FeatureAggregator.Averaged).embed_feature_set (src/lib.rs:2092-2124,
+ * aggregator.rs:34-47): out[n_nodes*dims] = mean of each node's feature rows.   */
+int ep_embed_averaged(ep_ctx* ctx, uint64_t n_nodes, const uint64_t* f_indptr, const uint32_t* f_ids, float* out);
+
+/* Philox4x32-10 on the host (the device RNG), for known-answer tests.           */
+int ep_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* Thread-local message of the last failure ("" if none).                        */
+const char* ep_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLOVERLEAF_EP_H */

@@ -1,0 +1,530 @@
+"""FP64 CPU oracle for one Embedding Propagation (EP) training step.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the shipped package imports this module;
+only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` use it, and only as the checker.
+
+It restates, in float64 numpy with an *explicit-samples* interface, the
+algorithm of the Rust reference (paths relative to the reference checkout):
+
+* graph construction          src/graph.rs:70-142, 414-422, 630-691
+* ``Sample::sample``           src/algos/utils.rs:193-216
+* feature collection / mean    src/algos/ep/model.rs:257-301, 342-416, 448-463
+* margin loss                  src/algos/ep/loss.rs:59-73, 241-243
+* loss weighting               src/algos/ep/mod.rs:190-200
+* gradient extraction          src/algos/ep/mod.rs:329-378
+* batch merge                  src/algos/ep/mod.rs:223-233
+* cosine LR schedule           src/algos/grad_utils/scheduler.rs:25-54, mod.rs:132-137
+* sparse lazy Adam             src/algos/grad_utils/optimizer.rs:43-81
+* random init                  src/algos/ep/mod.rs:381-392
+* multi-head attention         src/algos/ep/attention.rs:100-296, model.rs:306-335, 419-446
+
+Random streams (rand 0.7 XorShift, ``choose_multiple``, reservoir sampling)
+are *not* restated: callers pass the sampled neighbour and negative lists
+explicitly, so the oracle and the GPU path see identical samples.
+
+Pinning (what checks each function; see tests/test_oracle.py):
+  csr_from_edges / cdf_convert   reference KATs graph.rs:707-776 (+ dedup order)
+  euclidean                      reference KAT loss.rs:250-255 (sqrt 8)
+  construct_mean                 closed forms on hand-made inputs, invariants
+  margin_loss_adjoints           central finite differences, closed form
+  example_grads(first_leaf)      brute force with per-construction autograd leaves (torch fp64)
+  example_grads(exact)           central finite differences of the loss w.r.t. the table
+  adam_update                    closed form of the first update (m=v=0)
+  cos_decay_lr                   closed form at step 1, warm-up end, warm-up+1, T
+  attention_forward              reference KATs attention.rs:316-425
+  attention_backward             central finite differences, torch fp64 autograd
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+# --------------------------------------------------------------------------------------
+# Graph construction (src/graph.rs)
+# --------------------------------------------------------------------------------------
+
+
+def csr_from_edges(edges: Sequence[Tuple[int, int, float]], deduplicate: bool):
+    """CSR::construct_from_edges (src/graph.rs:70-109) with dedup (120-142).
+
+    Returns (rows int64[N+1], columns int64[E], weights float64[E]).  With
+    ``deduplicate`` the edge list is first sorted by (from, to) and repeated
+    edges are merged by summing weights; without it, columns keep insertion
+    order within a row.
+    """
+    edges = [(int(f), int(t), float(w)) for f, t, w in edges]
+    if deduplicate and edges:
+        edges.sort(key=lambda e: (e[0], e[1]))
+        merged = [list(edges[0])]
+        for f, t, w in edges[1:]:
+            if f == merged[-1][0] and t == merged[-1][1]:
+                merged[-1][2] += w
+            else:
+                merged.append([f, t, w])
+        edges = [tuple(e) for e in merged]
+    max_node = max((max(f, t) for f, t, _ in edges), default=0)
+    rows = np.zeros(max_node + 2, dtype=np.int64)
+    for f, _, _ in edges:
+        rows[f + 1] += 1
+    rows = np.cumsum(rows)
+    counts = np.zeros(max_node + 1, dtype=np.int64)
+    cols = np.zeros(len(edges), dtype=np.int64)
+    wts = np.zeros(len(edges), dtype=np.float64)
+    for f, t, w in edges:
+        idx = rows[f] + counts[f]
+        cols[idx] = t
+        wts[idx] = w
+        counts[f] += 1
+    return rows, cols, wts
+
+
+def cdf_convert(weights: np.ndarray) -> np.ndarray:
+    """convert_edges_to_cdf (src/graph.rs:664-691) for one row of weights."""
+    w = np.asarray(weights, dtype=np.float64).copy()
+    if w.size == 0:
+        return w
+    denom = w.sum()
+    if denom == 0.0:
+        w[:] = 1.0
+        denom = float(w.size)
+    w = np.minimum(np.cumsum(w) / denom, 1.0)
+    w[-1] = 1.0
+    return w
+
+
+def cumcsr_convert(rows: np.ndarray, weights: np.ndarray) -> np.ndarray:
+    """CumCSR::convert (src/graph.rs:414-422): per-row CDF of the weights."""
+    out = np.asarray(weights, dtype=np.float64).copy()
+    for i in range(len(rows) - 1):
+        s, e = rows[i], rows[i + 1]
+        if s < e:
+            out[s:e] = cdf_convert(out[s:e])
+    return out
+
+
+def cdf_to_p(cdf: np.ndarray) -> np.ndarray:
+    """CDFtoP (src/graph.rs:630-661)."""
+    cdf = np.asarray(cdf, dtype=np.float64)
+    if cdf.size == 0:
+        return cdf
+    return np.concatenate([cdf[:1], np.diff(cdf)])
+
+
+# --------------------------------------------------------------------------------------
+# Sampling specification (src/algos/utils.rs:166-216)
+# --------------------------------------------------------------------------------------
+
+
+def sample_spec(kind: str, k: float, n: int) -> Tuple[int, float]:
+    """Deterministic part of ``Sample::sample`` for All and Fixed(k).
+
+    Returns (number to draw, unbiasing scale).  ``Fixed(k)``: (min(k, n), n/r).
+    ``All``: (n, 1).  (Probability(p) draws a Binomial and is not restated.)
+    """
+    if kind == "all":
+        return n, 1.0
+    if kind == "fixed":
+        r = min(int(k), n)
+        return r, (float(n) / float(r)) if r > 0 else float("nan")
+    raise ValueError(kind)
+
+
+# --------------------------------------------------------------------------------------
+# Constructions (src/algos/ep/model.rs)
+# --------------------------------------------------------------------------------------
+
+
+@dataclass
+class Construction:
+    """One node construction: ``feature -> accumulated count`` in first-seen order.
+
+    Mirrors the ``NodeCounts`` map built by ``collect_embeddings_from_node``
+    (model.rs:257-278): the first occurrence of a feature creates its leaf with
+    the occurrence weight; later occurrences add their weight to the count.
+    """
+
+    counts: Dict[int, float] = field(default_factory=dict)
+
+    def add(self, feat: int, weight: float):
+        if feat in self.counts:
+            self.counts[feat] += weight
+        else:
+            self.counts[feat] = weight
+
+    @property
+    def total(self) -> float:
+        return float(sum(self.counts.values()))
+
+
+def collect_node(con: Construction, feats: Sequence[int], weight: float, feat_scale: float = 1.0):
+    """collect_embeddings_from_node (model.rs:257-278) with ``max_features`` = All.
+
+    ``weight += scalar`` (model.rs:268) is additive: each occurrence adds
+    ``weight + feat_scale`` to its feature's count (feat_scale = 1 for All).
+    """
+    w = weight + feat_scale
+    for f in feats:
+        con.add(int(f), w)
+
+
+def node_features(f_indptr, f_ids, node) -> np.ndarray:
+    return np.asarray(f_ids[int(f_indptr[node]):int(f_indptr[node + 1])], dtype=np.int64)
+
+
+def build_anchor_construction(f_indptr, f_ids, v) -> Construction:
+    """h(v): construct_node_embedding (model.rs:283-301) called with weight 1 (mod.rs:300-301)."""
+    c = Construction()
+    collect_node(c, node_features(f_indptr, f_ids, v), 1.0)
+    return c
+
+
+def build_neighbour_construction(f_indptr, f_ids, nbrs: Sequence[int], scale_weight: float) -> Construction:
+    """~h(v): reconstruct_node_embedding (model.rs:342-386) -> construct_from_multiple_nodes (388-416).
+
+    ``nbrs`` is the explicit sampled neighbour list; ``scale_weight`` is the
+    unbiasing scale of ``Sample::sample`` (deg/S when subsampled, 1 otherwise),
+    used as each neighbour's weight (model.rs:360).
+    """
+    c = Construction()
+    for u in nbrs:
+        collect_node(c, node_features(f_indptr, f_ids, u), scale_weight)
+    return c
+
+
+def construct_mean(table: np.ndarray, con: Construction) -> np.ndarray:
+    """mean_embeddings (model.rs:448-463): sum_f c_f E_f / sum_f c_f."""
+    acc = np.zeros(table.shape[1], dtype=np.float64)
+    n = 0.0
+    for f, c in con.counts.items():
+        acc += c * table[f].astype(np.float64)
+        n += c
+    return acc / n
+
+
+# --------------------------------------------------------------------------------------
+# Margin loss (src/algos/ep/loss.rs:59-73, 241-243)
+# --------------------------------------------------------------------------------------
+
+
+def euclidean(x: np.ndarray, y: np.ndarray) -> float:
+    """euclidean_distance (loss.rs:241-243): sqrt(sum (x-y)^2)."""
+    d = np.asarray(x, dtype=np.float64) - np.asarray(y, dtype=np.float64)
+    return float(np.sqrt(np.sum(d * d)))
+
+
+def margin_loss_adjoints(a: np.ndarray, p: np.ndarray, negs: Sequence[np.ndarray], gamma: float):
+    """Margin loss and its adjoints.
+
+    a = h(v), p = ~h(v), n_i = h(u_i).  D0 = |p-a|, D_i = |p-n_i|,
+    l_i = gamma + D0 - D_i, A = {i: l_i > 0}; L = mean_{i in A} l_i (0 if A empty).
+    Returns (L, active mask, g_a, g_p, [g_n_i]) with g = dL/dx; inactive
+    negatives get None (they are not in the loss graph).  Division by a zero
+    distance yields NaN exactly as the reference's tape does.
+    """
+    a = np.asarray(a, dtype=np.float64)
+    p = np.asarray(p, dtype=np.float64)
+    D0 = euclidean(p, a)
+    Ds = [euclidean(p, n) for n in negs]
+    ls = [gamma + D0 - d for d in Ds]
+    active = np.array([l > 0.0 for l in ls], dtype=bool)
+    nA = int(active.sum())
+    if nA == 0:
+        return 0.0, active, None, None, [None] * len(negs)
+    L = float(sum(l for l, act in zip(ls, active) if act) / nA)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        g_a = (a - p) / D0
+        g_p = (p - a) / D0
+        g_n = []
+        for n, d, act in zip(negs, Ds, active):
+            if act:
+                gn = (p - np.asarray(n, dtype=np.float64)) / d / nA
+                g_p = g_p - gn
+                g_n.append(gn)
+            else:
+                g_n.append(None)
+    return L, active, g_a, g_p, g_n
+
+
+def loss_weight_divisor(kind: str, degree: int, weight: float = 0.0) -> float:
+    """LossWeighting (mod.rs:190-200): loss is divided by this value."""
+    if kind == "none":
+        return 1.0
+    if kind == "log":
+        return math.log(1.0 + degree)
+    if kind == "exponential":
+        return float(degree) ** weight
+    raise ValueError(kind)
+
+
+# --------------------------------------------------------------------------------------
+# One example and one batch (src/algos/ep/mod.rs:174-233, 289-378)
+# --------------------------------------------------------------------------------------
+
+
+@dataclass
+class ExampleResult:
+    loss: float                      # weighted loss (0 when no gradient)
+    contributes: bool                # loss > 0 and not NaN (mod.rs:206-214)
+    grads: Dict[int, np.ndarray]     # feature -> gradient (after first_leaf/exact rule)
+    vectors: List[np.ndarray]        # [a, p, n_1..n_K]
+    adjoints: List[Optional[np.ndarray]]
+
+
+def example_grads(table, f_indptr, f_ids, anchor, nbrs, nbr_scale, negs, gamma,
+                  grad_mode="first_leaf", loss_div=1.0) -> ExampleResult:
+    """Forward + adjoint + gradient extraction for one anchor with explicit samples.
+
+    Constructions are ordered a=h(v), p=~h(v), n_1..n_K (mod.rs:300-320).
+    A construction is *eligible* for gradient extraction when it is part of
+    the loss graph (a and p always, negatives only when active) and its
+    adjoint is finite (mod.rs:370-375 drops NaN/Inf feature gradients).  The
+    gradient of feature f through construction x is (c_f/C_x) dL/dx.
+
+    grad_mode:
+      "first_leaf"  the reference rule (mod.rs:347-352, 366-367): the first
+                    eligible construction containing f, in the order above.
+      "exact"       the true gradient: sum over every eligible construction.
+    """
+    cons = [build_anchor_construction(f_indptr, f_ids, anchor),
+            build_neighbour_construction(f_indptr, f_ids, nbrs, nbr_scale)]
+    for u in negs:
+        cons.append(build_anchor_construction(f_indptr, f_ids, u))
+    vecs = [construct_mean(table, c) for c in cons]
+    L, active, g_a, g_p, g_n = margin_loss_adjoints(vecs[0], vecs[1], vecs[2:], gamma)
+    L = L / loss_div
+    adjs = [g_a, g_p] + list(g_n)
+    adjs = [None if g is None else g / loss_div for g in adjs]
+    if not (L > 0.0) or math.isnan(L):
+        return ExampleResult(L if not math.isnan(L) else float("nan"), False, {}, vecs, adjs)
+    grads: Dict[int, np.ndarray] = {}
+    for con, adj in zip(cons, adjs):
+        if adj is None or not np.all(np.isfinite(adj)):
+            continue
+        C = con.total
+        for f, c in con.counts.items():
+            g = (c / C) * adj
+            if not np.all(np.isfinite(g)):
+                continue
+            if grad_mode == "first_leaf":
+                if f in grads:
+                    continue
+                grads[f] = g
+            elif grad_mode == "exact":
+                grads[f] = grads[f] + g if f in grads else g
+            else:
+                raise ValueError(grad_mode)
+    return ExampleResult(L, True, grads, vecs, adjs)
+
+
+def merge_batch(results: Sequence[ExampleResult], dims: int) -> Dict[int, np.ndarray]:
+    """Batch merge (mod.rs:223-233): sum of per-example gradients per feature."""
+    out: Dict[int, np.ndarray] = {}
+    for r in results:
+        if not r.contributes:
+            continue
+        for f, g in r.grads.items():
+            out[f] = out[f] + g if f in out else g.copy()
+    return out
+
+
+# --------------------------------------------------------------------------------------
+# Schedule and optimizer
+# --------------------------------------------------------------------------------------
+
+
+def cos_decay_lr(step: int, alpha: float, total_updates: int) -> float:
+    """LRScheduler::cos_decay as built at mod.rs:132-137 (scheduler.rs:39-47).
+
+    warm = floor(T/5), min = alpha/100; step <= warm: linear warm-up from min;
+    otherwise min + (alpha-min)/2 (1 + cos(pi step/T)).  Steps are numbered from 1.
+    """
+    warm = int(total_updates / 5.0)
+    lo = alpha / 100.0
+    if step > warm:
+        return lo + 0.5 * (alpha - lo) * (1.0 + math.cos(math.pi * step / total_updates))
+    return lo + (step / warm) * (alpha - lo) if warm > 0 else float("nan")
+
+
+ADAM_B1 = 0.9
+ADAM_B2 = 0.999
+ADAM_EPS = 1e-8
+
+
+def adam_update(table, mom, var, grads: Dict[int, np.ndarray], lr: float, pass_: int):
+    """AdamOptimizer::update (optimizer.rs:43-81), in place, float64.
+
+    Bias-correction time step t = pass + 1 (pass is 1-based, optimizer.rs:50).
+    A row whose gradient has any NaN is skipped (optimizer.rs:56).
+    """
+    t = float(pass_) + 1.0
+    for f, g in grads.items():
+        if np.any(np.isnan(g)):
+            continue
+        mom[f] = ADAM_B1 * mom[f] + (1.0 - ADAM_B1) * g
+        var[f] = ADAM_B2 * var[f] + (1.0 - ADAM_B2) * g * g
+        mh = mom[f] / (1.0 - ADAM_B1 ** t)
+        vh = var[f] / (1.0 - ADAM_B2 ** t)
+        table[f] = table[f] - lr * (mh / (np.sqrt(vh) + ADAM_EPS))
+
+
+def ep_batch_step(table, mom, var, f_indptr, f_ids, anchors, nbr_lists, nbr_scales, neg_lists,
+                  gamma, lr, pass_, grad_mode="first_leaf", loss_divs=None):
+    """One batch: per-example forward/backward, merge, Adam (mod.rs:174-252).
+
+    Updates table/mom/var in place (float64 arrays).  Returns (results, merged grads).
+    The Adam update runs only when at least one example contributes (mod.rs:237).
+    """
+    results = []
+    for i, v in enumerate(anchors):
+        div = 1.0 if loss_divs is None else loss_divs[i]
+        results.append(example_grads(table, f_indptr, f_ids, v, nbr_lists[i], nbr_scales[i],
+                                     neg_lists[i], gamma, grad_mode, div))
+    merged = merge_batch(results, table.shape[1])
+    if any(r.contributes for r in results):
+        adam_update(table, mom, var, merged, lr, pass_)
+    return results, merged
+
+
+def batch_error(results: Sequence[ExampleResult]) -> float:
+    """Per-batch train error (mod.rs:227-260): sum of contributing losses / batch size."""
+    cnt = sum(1 for r in results if r.contributes)
+    if cnt == 0:
+        return 0.0
+    return sum(r.loss for r in results if r.contributes) / len(results)
+
+
+# --------------------------------------------------------------------------------------
+# Initialization (mod.rs:381-392)
+# --------------------------------------------------------------------------------------
+
+
+def normalize_rows(raw_uniform01: np.ndarray) -> np.ndarray:
+    """x = 2u - 1 then divide each row by its L2 norm (mod.rs:385-390)."""
+    x = 2.0 * np.asarray(raw_uniform01, dtype=np.float64) - 1.0
+    return x / np.linalg.norm(x, axis=1, keepdims=True)
+
+
+# --------------------------------------------------------------------------------------
+# Attention aggregator (src/algos/ep/attention.rs, model.rs:306-335, 419-446)
+# --------------------------------------------------------------------------------------
+
+
+def attention_split(rows: np.ndarray, heads: int, d_k: int):
+    """Row layout [Q_1..Q_H | K_1..K_H | V_1..V_H] (attention.rs:38-66)."""
+    d_model = (rows.shape[1] - 2 * heads * d_k) // heads
+    Q = [rows[:, h * d_k:(h + 1) * d_k] for h in range(heads)]
+    K = [rows[:, heads * d_k + h * d_k: heads * d_k + (h + 1) * d_k] for h in range(heads)]
+    base = 2 * heads * d_k
+    V = [rows[:, base + h * d_model: base + (h + 1) * d_model] for h in range(heads)]
+    return Q, K, V, d_model
+
+
+def softmax_exact(x: np.ndarray) -> np.ndarray:
+    """softmax (attention.rs:283-296) with the exact exponential."""
+    z = np.exp(x - np.max(x))
+    return z / z.sum()
+
+
+def attention_matrix_full(Q: np.ndarray, K: np.ndarray, counts: np.ndarray) -> np.ndarray:
+    """compute_full_attention_matrix (attention.rs:171-191): S_ij = Q_i.K_j * (c_i c_j if >= 1)."""
+    S = Q @ K.T
+    num = np.outer(counts, counts)
+    return np.where(num >= 1.0, S * num, S)
+
+
+def attention_mean(rows: np.ndarray, counts: np.ndarray, heads: int, d_k: int) -> np.ndarray:
+    """attention_mean with Full attention (attention.rs:100-131), float64.
+
+    rows: n x W feature rows, counts: n multiplicities.  For n == 1 the value
+    vector of head 0 is returned (attention.rs:113-115).  Otherwise, per head,
+    A = softmax_row(S / sqrt(d_k)), out_h = (1/n) sum_i sum_j A_ij V_j, and
+    the result is the mean over heads.
+    """
+    rows = np.asarray(rows, dtype=np.float64)
+    Q, K, V, d_model = attention_split(rows, heads, d_k)
+    n = rows.shape[0]
+    if n == 1:
+        return V[0][0].copy()
+    out = np.zeros(d_model)
+    for h in range(heads):
+        S = attention_matrix_full(Q[h], K[h], np.asarray(counts, dtype=np.float64)) / math.sqrt(d_k)
+        A = np.stack([softmax_exact(S[i]) for i in range(n)])
+        out += (A.sum(axis=0) @ V[h]) / n
+    return out / heads
+
+
+def attention_mean_backward(rows: np.ndarray, counts: np.ndarray, heads: int, d_k: int,
+                            g_out: np.ndarray) -> np.ndarray:
+    """Adjoint of ``attention_mean`` w.r.t. the feature rows (float64).
+
+    With w_j = sum_i A_ij:  dV_j = w_j g / (n H);
+    dA_ij = (V_j . g)/(n H);  dS_ij = A_ij (dA_ij - sum_k A_ik dA_ik);
+    dT = dS * m / sqrt(d_k) with m_ij = c_i c_j if >= 1 else 1;
+    dQ_i = sum_j dT_ij K_j;  dK_j = sum_i dT_ij Q_i.
+    """
+    rows = np.asarray(rows, dtype=np.float64)
+    counts = np.asarray(counts, dtype=np.float64)
+    Q, K, V, d_model = attention_split(rows, heads, d_k)
+    n = rows.shape[0]
+    grad = np.zeros_like(rows)
+    base = 2 * heads * d_k
+    if n == 1:
+        grad[0, base:base + d_model] = g_out
+        return grad
+    num = np.outer(counts, counts)
+    m = np.where(num >= 1.0, num, 1.0)
+    for h in range(heads):
+        S = (Q[h] @ K[h].T) * m / math.sqrt(d_k)
+        A = np.stack([softmax_exact(S[i]) for i in range(n)])
+        w = A.sum(axis=0)
+        scale = 1.0 / (n * heads)
+        grad[:, base + h * d_model: base + (h + 1) * d_model] += np.outer(w, g_out) * scale
+        dA = np.tile((V[h] @ g_out) * scale, (n, 1))
+        dS = A * (dA - np.sum(A * dA, axis=1, keepdims=True))
+        dT = dS * m / math.sqrt(d_k)
+        grad[:, h * d_k:(h + 1) * d_k] += dT @ K[h]
+        grad[:, heads * d_k + h * d_k: heads * d_k + (h + 1) * d_k] += dT.T @ Q[h]
+    return grad
+
+
+# --------------------------------------------------------------------------------------
+# Inference aggregate and holdout AUC (src/algos/aggregator.rs:34-47)
+# --------------------------------------------------------------------------------------
+
+
+def averaged_node_embeddings(table, f_indptr, f_ids) -> np.ndarray:
+    """AvgAggregator::construct (aggregator.rs:34-47): plain mean of the node's feature rows.
+
+    Nodes without features keep a zero row (lib.rs:2116-2118 only writes when
+    the node has features).
+    """
+    n = len(f_indptr) - 1
+    out = np.zeros((n, table.shape[1]), dtype=np.float64)
+    for i in range(n):
+        fs = f_ids[f_indptr[i]:f_indptr[i + 1]]
+        if len(fs):
+            out[i] = table[np.asarray(fs, dtype=np.int64)].astype(np.float64).mean(axis=0)
+    return out
+
+
+def roc_auc(pos_scores: np.ndarray, neg_scores: np.ndarray) -> float:
+    """Mann-Whitney ROC-AUC with average ranks for ties."""
+    pos = np.asarray(pos_scores, dtype=np.float64)
+    neg = np.asarray(neg_scores, dtype=np.float64)
+    allv = np.concatenate([pos, neg])
+    order = np.argsort(allv, kind="mergesort")
+    ranks = np.empty(len(allv), dtype=np.float64)
+    sv = allv[order]
+    i = 0
+    while i < len(sv):
+        j = i
+        while j + 1 < len(sv) and sv[j + 1] == sv[i]:
+            j += 1
+        ranks[order[i:j + 1]] = 0.5 * (i + j) + 1.0
+        i = j + 1
+    rp = ranks[:len(pos)].sum()
+    return float((rp - len(pos) * (len(pos) + 1) / 2.0) / (len(pos) * len(neg)))

@@ -1,0 +1,259 @@
+"""GPU parity of the EP step (HIP kernels through the C ABI) against the FP64 oracle.
+
+Samples are drawn by the GPU sampler and the same lists are fed to the oracle.
+Bars (normwise relative, FP32 path): loss <= 1e-5, merged gradient <= 1e-5.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from cloverleaf_amd import _native as N
+from oracle import ep_oracle as O
+from tests.problems import dense_grad, make_problem, oracle_step
+
+pytestmark = pytest.mark.gpu
+
+LOSS_TOL = 1e-5
+GRAD_TOL = 1e-5
+
+
+def make_ctx(prob, S, K, gamma=0.5, B=32, grad_mode=N.EP_GRAD_FIRST_LEAF, scatter=N.EP_SCATTER_SORT,
+             precision=N.EP_PREC_FP32, lw=N.EP_LW_NONE, lw_exp=0.0, use_graph=0, seed=20222022):
+    cfg = N.default_config()
+    cfg.dims, cfg.batch_size, cfg.negatives, cfg.gamma = prob.D, B, K, gamma
+    cfg.max_nodes_kind, cfg.max_nodes = N.EP_SAMPLE_FIXED, S
+    cfg.grad_mode, cfg.scatter, cfg.precision = grad_mode, scatter, precision
+    cfg.loss_weighting, cfg.loss_weight_exp = lw, lw_exp
+    cfg.use_graph = use_graph
+    cfg.seed = seed
+    cfg.valid_pct = 0.0
+    ctx = N.Context(cfg)
+    ctx.set_graph(prob.indptr, prob.indices)
+    ctx.set_features(prob.f_indptr, prob.f_ids, prob.V)
+    ctx.set_table(prob.table)
+    return ctx
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+def check_step(prob, ctx, anchors, gamma, grad_mode, pass_=1, lr=0.01, loss_divs=None, tol=GRAD_TOL):
+    print(f"[parity] D={prob.D} n={prob.n} V={prob.V} B={len(anchors)} mode={grad_mode}", flush=True)
+    cnt, nbr, neg = ctx.sample_batch(anchors, pass_)
+    out = ctx.step_explicit(anchors, cnt, nbr, neg, pass_=pass_, lr=lr, apply_update=True, want_cons=True)
+    mode = "first_leaf" if grad_mode == N.EP_GRAD_FIRST_LEAF else "exact"
+    res, merged, t64, m64, v64 = oracle_step(prob, anchors, cnt, nbr, neg, gamma, lr, pass_, mode,
+                                             loss_divs=loss_divs)
+    ref_loss = np.array([r.loss if (r is not None and r.contributes) else 0.0 for r in res])
+    assert rel(out["loss"], ref_loss) <= LOSS_TOL
+    for i, r in enumerate(res):
+        if r is None:
+            continue
+        cons64 = np.stack(r.vectors)
+        assert rel(out["cons"][i], cons64) <= 1e-5
+    G64 = dense_grad(merged, prob.V, prob.D)
+    Gg = np.zeros_like(G64)
+    assert len(set(out["feat"].tolist())) == len(out["feat"])
+    Gg[out["feat"]] = out["grad"]
+    assert set(out["feat"].tolist()) == set(merged.keys())
+    assert rel(Gg, G64) <= tol
+    return out, res, merged, t64, m64, v64
+
+
+@pytest.mark.parametrize("D", [32, 64, 128])
+@pytest.mark.parametrize("grad_mode", [N.EP_GRAD_FIRST_LEAF, N.EP_GRAD_EXACT])
+@pytest.mark.parametrize("scatter", [N.EP_SCATTER_SORT, N.EP_SCATTER_ATOMIC])
+def test_step_matches_oracle(D, grad_mode, scatter):
+    prob = make_problem(seed=D, n=150, V=400, D=D)
+    S, K, gamma = 6, 5, 0.5
+    ctx = make_ctx(prob, S, K, gamma, grad_mode=grad_mode, scatter=scatter)
+    rng = np.random.default_rng(1)
+    anchors = rng.choice(prob.n, size=48, replace=False).astype(np.uint32)
+    out, res, merged, t64, m64, v64 = check_step(prob, ctx, anchors, gamma, grad_mode)
+    # Adam update on the touched rows (elements whose gradient is above FP32 noise)
+    new = ctx.get_table()
+    m, v = ctx.get_adam()
+    G64 = dense_grad(merged, prob.V, prob.D)
+    rows = sorted(merged)
+    assert rel(m[rows], m64[rows]) <= 1e-5
+    assert rel(v[rows], v64[rows]) <= 1e-4
+    mask = np.abs(G64[rows]) > 1e-3 * np.abs(G64).max()
+    assert np.max(np.abs(new[rows] - t64[rows])[mask]) < 1e-5
+    untouched = np.setdiff1d(np.arange(prob.V), rows)
+    assert np.array_equal(new[untouched], prob.table[untouched])
+    ctx.close()
+
+
+def test_dims_100_and_tiny_dims():
+    for D in (8, 16, 100):
+        prob = make_problem(seed=D + 1, n=80, V=150, D=D)
+        ctx = make_ctx(prob, 5, 3, 0.5)
+        anchors = np.arange(0, 80, 3, dtype=np.uint32)
+        check_step(prob, ctx, anchors, 0.5, N.EP_GRAD_FIRST_LEAF)
+        ctx.close()
+
+
+def test_hub_sampling_and_long_segments():
+    # few features, heavy Zipf -> very long sorted segments (long-segment path), hubs with deg >> S
+    prob = make_problem(seed=11, n=400, V=20, D=64, avg_deg=40, fmax=6, zipf=1.5)
+    ctx = make_ctx(prob, 8, 10, 1.0, B=400)
+    anchors = np.arange(400, dtype=np.uint32)
+    check_step(prob, ctx, anchors, 1.0, N.EP_GRAD_FIRST_LEAF)
+    ctx.close()
+
+
+def test_zero_degree_anchor_is_skipped():
+    prob = make_problem(seed=5, n=60, V=100, D=32, zero_deg=5)
+    ctx = make_ctx(prob, 4, 3, 0.5)
+    anchors = np.arange(50, 60, dtype=np.uint32)  # last 5 have no neighbours
+    out, res, *_ = check_step(prob, ctx, anchors, 0.5, N.EP_GRAD_FIRST_LEAF)
+    assert np.all(out["loss"][5:] == 0)
+    ctx.close()
+
+
+def test_zero_distance_drops_nan_constructions():
+    # every node carries the same single feature except the negatives' extra one: a == p exactly
+    n = 30
+    indptr = np.arange(0, 2 * n + 1, 2, dtype=np.uint64)
+    indices = np.array([[(i + 1) % n, (i + 2) % n] for i in range(n)], dtype=np.uint32).ravel()
+    f_indptr = np.concatenate([[0], np.cumsum([1] * 20 + [2] * 10)]).astype(np.uint64)
+    f_ids = np.array([0] * 20 + [x for i in range(10) for x in (0, 1 + i)], dtype=np.uint32)
+    from tests.problems import Problem
+    rng = np.random.default_rng(0)
+    prob = Problem(indptr, indices, f_indptr, f_ids, 11, 32, rng.uniform(-1, 1, (11, 32)).astype(np.float32))
+    ctx = make_ctx(prob, 2, 3, 10.0)
+    anchors = np.arange(0, 18, dtype=np.uint32)  # anchors and neighbours all single-feature {0}
+    cnt, nbr, _ = ctx.sample_batch(anchors, 1)
+    neg = np.tile(np.array([20, 25, 29], dtype=np.uint32), (len(anchors), 1))
+    out = ctx.step_explicit(anchors, cnt, nbr, neg, 1, 0.01, apply_update=False)
+    res, merged, *_ = oracle_step(prob, anchors, cnt, nbr, neg, 10.0, 0.01, 1, "first_leaf")
+    assert all(r.contributes for r in res)
+    assert all(not np.all(np.isfinite(r.adjoints[0])) for r in res)
+    G64 = dense_grad(merged, prob.V, prob.D)
+    Gg = np.zeros_like(G64)
+    Gg[out["feat"]] = out["grad"]
+    assert set(out["feat"].tolist()) == set(merged)
+    assert rel(Gg, G64) <= GRAD_TOL
+    ctx.close()
+
+
+def test_all_satisfied_margin_gives_no_update():
+    prob = make_problem(seed=8, n=50, V=80, D=32, table_scale=1.0)
+    ctx = make_ctx(prob, 4, 3, gamma=-100.0)  # l_i = gamma + D0 - D_i < 0 always
+    anchors = np.arange(20, dtype=np.uint32)
+    cnt, nbr, neg = ctx.sample_batch(anchors, 1)
+    out = ctx.step_explicit(anchors, cnt, nbr, neg, 1, 0.05, apply_update=True)
+    assert np.all(out["loss"] == 0) and len(out["feat"]) == 0
+    assert np.array_equal(ctx.get_table(), prob.table)
+    ctx.close()
+
+
+@pytest.mark.parametrize("lw,lw_exp", [(N.EP_LW_LOG, 0.0), (N.EP_LW_EXPONENTIAL, 0.7)])
+def test_loss_weighting(lw, lw_exp):
+    prob = make_problem(seed=9, n=100, V=200, D=32)
+    ctx = make_ctx(prob, 5, 4, 0.5, lw=lw, lw_exp=lw_exp)
+    anchors = np.arange(0, 100, 2, dtype=np.uint32)
+    deg = np.diff(prob.indptr.astype(np.int64))[anchors]
+    kind = "log" if lw == N.EP_LW_LOG else "exponential"
+    divs = [O.loss_weight_divisor(kind, int(d), lw_exp) for d in deg]
+    check_step(prob, ctx, anchors, 0.5, N.EP_GRAD_FIRST_LEAF, loss_divs=divs)
+    ctx.close()
+
+
+def test_sampler_properties():
+    prob = make_problem(seed=3, n=200, V=100, D=16, avg_deg=30)
+    S, K = 6, 7
+    ctx = make_ctx(prob, S, K, B=200)
+    ip = prob.indptr.astype(np.int64)
+    anchors = np.arange(200, dtype=np.uint32)
+    cnt, nbr, neg = ctx.sample_batch(anchors, 1)
+    for i, v in enumerate(anchors):
+        row = prob.indices[ip[v]:ip[v + 1]]
+        assert cnt[i] == min(len(row), S)
+        s = nbr[i, :cnt[i]]
+        assert len(set(s.tolist())) == cnt[i] and set(s.tolist()) <= set(row.tolist())
+    # a different pass draws different samples; the same pass is reproducible
+    c2, n2, g2 = ctx.sample_batch(anchors, 1)
+    assert np.array_equal(n2, nbr) and np.array_equal(g2, neg)
+    c3, n3, g3 = ctx.sample_batch(anchors, 2)
+    assert not np.array_equal(g3, neg)
+    ctx.close()
+
+
+def test_sampler_uniformity():
+    # one hub of degree 40, S = 5: every neighbour picked with probability S/deg; negatives uniform
+    n = 41
+    indptr = np.zeros(n + 1, dtype=np.uint64)
+    indptr[1:] = 40
+    indptr[2:] = 40 + np.arange(1, n)
+    indices = np.concatenate([np.arange(1, n), np.zeros(n - 1)]).astype(np.uint32)
+    from tests.problems import Problem
+    f_indptr = np.arange(n + 1, dtype=np.uint64)
+    prob = Problem(indptr, indices, f_indptr, np.arange(n, dtype=np.uint32), n, 8,
+                   np.zeros((n, 8), dtype=np.float32))
+    S, K = 5, 4
+    ctx = make_ctx(prob, S, K, B=4096)
+    anchors = np.zeros(4096, dtype=np.uint32)
+    cnt, nbr, neg = ctx.sample_batch(anchors, 1)
+    assert np.all(cnt == S)
+    counts = np.bincount(nbr.ravel(), minlength=n)[1:]
+    exp = 4096 * S / 40
+    chi2 = float(((counts - exp) ** 2 / exp).sum())
+    assert chi2 < 90, chi2  # 39 dof: p ~ 1e-5
+    negc = np.bincount(neg.ravel(), minlength=n)
+    expn = 4096 * K / n  # the train pool is every node (no split set)
+    assert float(((negc - expn) ** 2 / expn).sum()) < 95
+    ctx.close()
+
+
+def test_bf16_step_error_bar():
+    prob = make_problem(seed=21, n=150, V=400, D=64)
+    ctx = make_ctx(prob, 6, 5, 0.5, precision=N.EP_PREC_BF16)
+    anchors = np.arange(0, 150, 3, dtype=np.uint32)
+    cnt, nbr, neg = ctx.sample_batch(anchors, 1)
+    out = ctx.step_explicit(anchors, cnt, nbr, neg, 1, 0.01, apply_update=True)
+    # oracle on the bf16-rounded table (what the forward reads)
+    tb = prob.table.astype(np.float32).view(np.uint32)
+    tb = ((tb + 0x7FFF + ((tb >> 16) & 1)) & 0xFFFF0000).view(np.float32)
+    from tests.problems import Problem
+    pb = Problem(prob.indptr, prob.indices, prob.f_indptr, prob.f_ids, prob.V, prob.D, tb)
+    res, merged, *_ = oracle_step(pb, anchors, cnt, nbr, neg, 0.5, 0.01, 1, "first_leaf")
+    G64 = dense_grad(merged, prob.V, prob.D)
+    Gg = np.zeros_like(G64)
+    Gg[out["feat"]] = out["grad"]
+    assert rel(Gg, G64) <= 1e-3
+    ctx.close()
+
+
+def test_graph_replay_matches_eager():
+    prob = make_problem(seed=30, n=300, V=500, D=64)
+    tables = []
+    for use_graph in (0, 1):
+        ctx = make_ctx(prob, 6, 5, 0.5, B=64, use_graph=use_graph)
+        ctx.set_split(np.arange(300, dtype=np.uint32), np.zeros(0, dtype=np.uint32))
+        r = ctx.bench_steps(warmup=0, steps=7, instrument=False)
+        assert r.examples == 4 * 64 + 44 + 2 * 64  # 5 batches per pass (300 = 4*64 + 44)
+        tables.append(ctx.get_table())
+        ctx.close()
+    assert np.array_equal(tables[0], tables[1])
+
+
+def test_learn_reduces_loss_and_embed_averaged():
+    import cloverleaf_amd as C
+    from cloverleaf_amd import synth
+
+    w = synth.make_workload("ep-cfg1-10k", scale=0.2)
+    g = C.Graph.from_csr(w.indptr, w.indices)
+    fs = C.FeatureSet.from_csr(g, w.f_indptr, w.f_ids, w.vocab)
+    ep = C.EmbeddingPropagator(alpha=0.05, loss=C.EPLoss.margin(0.5, 5), batch_size=128, dims=32, passes=3,
+                               max_nodes=10, valid_pct=0.1, indicator=False)
+    emb = ep.learn_features(g, fs)
+    hist = ep.history
+    assert len(hist) == 3 and hist[-1]["train"] < hist[0]["train"]
+    assert math.isfinite(hist[-1]["valid"])
+    node = C.NodeEmbedder(C.FeatureAggregator.Averaged()).embed_feature_set(fs, emb)
+    ref = O.averaged_node_embeddings(emb.embeddings.astype(np.float64), w.f_indptr.astype(np.int64),
+                                     w.f_ids.astype(np.int64))
+    assert rel(node.embeddings, ref) < 1e-5

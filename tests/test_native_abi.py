@@ -1,0 +1,71 @@
+"""CPU-only checks of the native boundary: the library loads, exports every symbol the
+public header declares, validates configurations, and its host Philox matches the
+published Random123 known answers.  No GPU compute is called."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from cloverleaf_amd import _native as N
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def header_functions():
+    text = (ROOT / "include" / "cloverleaf_ep.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ep_[a-z0-9_]+)\s*\(", text)) - {"ep_progress_fn"})
+
+
+def test_library_exports_every_header_symbol(native):
+    funcs = header_functions()
+    assert len(funcs) >= 15
+    for name in funcs:
+        assert hasattr(native, name), name
+    assert set(funcs) == set(N.SIGNATURES), set(funcs) ^ set(N.SIGNATURES)
+
+
+def test_config_defaults_match_reference():
+    # EmbeddingPropagator::new defaults (src/lib.rs:1262-1274, SEED lib.rs:90)
+    c = N.default_config()
+    assert (c.dims, c.batch_size, c.passes, c.negatives) == (100, 50, 100, 1)
+    assert abs(c.alpha - 0.9) < 1e-7 and c.gamma == 1.0 and abs(c.valid_pct - 0.1) < 1e-7
+    assert c.seed == 20222022
+    assert c.max_nodes_kind == N.EP_SAMPLE_ALL and c.max_features_kind == N.EP_SAMPLE_ALL
+
+
+@pytest.mark.parametrize("field,value", [("dims", 7), ("negatives", 0), ("batch_size", 0), ("hard_negatives", 2),
+                                         ("noise", 0.5), ("max_features_kind", 1), ("valid_pct", 1.5)])
+def test_invalid_config_rejected(field, value):
+    c = N.default_config()
+    setattr(c, field, value)
+    h = ctypes.c_void_p()
+    rc = N.load().ep_create(ctypes.byref(c), ctypes.byref(h))
+    assert rc == -1 and not h.value
+    assert N.load().ep_last_error()
+
+
+# Random123 kat_vectors for philox4x32 with 10 rounds
+PHILOX_KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF), (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,exp", PHILOX_KAT)
+def test_philox_known_answers(ctr, key, exp):
+    assert tuple(int(x) for x in N.philox4x32(ctr, key)) == exp
+
+
+def test_null_arguments_rejected(native):
+    assert native.ep_create(None, None) == -1
+    assert native.ep_set_graph(None, 0, None, None) == -1
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        N.load(tmp_path / "nope.so")

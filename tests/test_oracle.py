@@ -1,0 +1,449 @@
+"""Pins the FP64 oracle (oracle/ep_oracle.py) before anything is compared against it.
+
+Every check is independent of the oracle's own formulas: reference known-answer
+tests, closed forms worked by hand, central finite differences, brute force with
+per-construction autograd leaves (the reference's extraction mechanism), and
+library routines.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import ep_oracle as O
+
+# --------------------------------------------------------------------------------------
+# Graph construction: reference KATs src/graph.rs:696-776
+# --------------------------------------------------------------------------------------
+KAT_EDGES = [(0, 1, 1.0), (1, 1, 3.0), (1, 2, 2.0), (2, 0, 2.5), (1, 0, 10.0)]  # graph.rs:697-705
+
+
+def test_csr_kat_insertion_order():
+    # graph.rs:707-713 expects rows [0,1,4,5], columns [1,1,2,0,0], weights [1,3,2,10,2.5];
+    # those are the insertion-order results (deduplicate = false)
+    rows, cols, w = O.csr_from_edges(KAT_EDGES, deduplicate=False)
+    assert rows.tolist() == [0, 1, 4, 5]
+    assert cols.tolist() == [1, 1, 2, 0, 0]
+    assert w.tolist() == [1.0, 3.0, 2.0, 10.0, 2.5]
+
+
+def test_csr_dedup_sorts_and_merges():
+    # with deduplicate=true the edges are sorted by (from, to) first (graph.rs:120-142)
+    rows, cols, w = O.csr_from_edges(KAT_EDGES, deduplicate=True)
+    assert rows.tolist() == [0, 1, 4, 5]
+    assert cols.tolist() == [1, 0, 1, 2, 0]
+    assert w.tolist() == [1.0, 10.0, 3.0, 2.0, 2.5]
+    rows, cols, w = O.csr_from_edges([(0, 1, 1.0), (0, 1, 2.0), (1, 0, 4.0), (0, 1, 0.5)], deduplicate=True)
+    assert rows.tolist() == [0, 1, 2] and cols.tolist() == [1, 0] and w.tolist() == [3.5, 4.0]
+
+
+def test_cumcsr_cdf_kat():
+    # graph.rs:744-752: weights [1, 3/15, 5/15, 15/15, 1]
+    rows, cols, w = O.csr_from_edges(KAT_EDGES, deduplicate=False)
+    cdf = O.cumcsr_convert(rows, w)
+    assert np.allclose(cdf, [1.0, 3 / 15, 5 / 15, 1.0, 1.0], atol=1e-12)
+
+
+def test_cdf_to_p_kat():
+    # graph.rs:755-768
+    rows, cols, w = O.csr_from_edges(KAT_EDGES, deduplicate=False)
+    cdf = O.cumcsr_convert(rows, w)
+    assert np.allclose(O.cdf_to_p(cdf[1:4]), [3 / 15, 2 / 15, 10 / 15], atol=1e-7)
+
+
+def test_cdf_edge_cases():
+    assert O.cdf_convert(np.zeros(0)).size == 0  # dead-end node (graph.rs:668-670)
+    assert np.allclose(O.cdf_convert(np.zeros(4)), [0.25, 0.5, 0.75, 1.0])  # zero weights -> uniform
+    assert O.cdf_convert(np.array([1.0, 1.0, 1.0]))[-1] == 1.0
+
+
+# --------------------------------------------------------------------------------------
+# Sample, distance, constructions
+# --------------------------------------------------------------------------------------
+def test_sample_spec():
+    assert O.sample_spec("fixed", 10, 25) == (10, 2.5)
+    assert O.sample_spec("fixed", 10, 4) == (4, 1.0)
+    assert O.sample_spec("all", 0, 7) == (7, 1.0)
+
+
+def test_euclidean_kat():
+    # loss.rs:250-255: distance([1,3],[3,5]) = sqrt(8)
+    assert O.euclidean(np.array([1.0, 3.0]), np.array([3.0, 5.0])) == math.sqrt(8.0)
+
+
+def test_construct_mean_closed_form():
+    table = np.array([[1.0, 0.0], [0.0, 2.0], [4.0, 4.0]])
+    # node with features [0, 1, 1]: counts 2,4 (weight 1 + scale 1 per occurrence) -> (2*[1,0] + 4*[0,2]) / 6
+    f_indptr = np.array([0, 3, 4])
+    f_ids = np.array([0, 1, 1, 2])
+    c = O.build_anchor_construction(f_indptr, f_ids, 0)
+    assert c.counts == {0: 2.0, 1: 4.0}
+    assert np.allclose(O.construct_mean(table, c), [1 / 3, 4 / 3])
+    # order of features does not matter
+    c2 = O.build_anchor_construction(np.array([0, 3]), np.array([1, 0, 1]), 0)
+    assert np.allclose(O.construct_mean(table, c2), O.construct_mean(table, c))
+
+
+def test_neighbour_construction_is_feature_pooled():
+    # ~h(v) pools all neighbour features (model.rs:398-414), not a mean of per-node means
+    table = np.array([[0.0], [10.0], [1.0]])
+    f_indptr = np.array([0, 1, 4])  # node 0: [0]; node 1: [1, 2, 2]
+    f_ids = np.array([0, 1, 2, 2])
+    c = O.build_neighbour_construction(f_indptr, f_ids, [0, 1], scale_weight=3.0)
+    pooled = (0.0 + 10.0 + 1.0 + 1.0) / 4.0
+    assert np.allclose(O.construct_mean(table, c), [pooled])
+    assert not np.isclose(pooled, (0.0 + (10 + 1 + 1) / 3) / 2)
+    # the per-neighbour weight (scale + 1) is uniform and cancels
+    c1 = O.build_neighbour_construction(f_indptr, f_ids, [0, 1], scale_weight=1.0)
+    assert np.allclose(O.construct_mean(table, c1), O.construct_mean(table, c))
+
+
+def test_single_feature_complete_graph_gives_zero_distance():
+    # every node has the same single feature: h(v) == ~h(v)
+    table = np.array([[0.3, -0.4]])
+    f_indptr = np.array([0, 1, 2, 3])
+    f_ids = np.array([0, 0, 0])
+    a = O.construct_mean(table, O.build_anchor_construction(f_indptr, f_ids, 0))
+    p = O.construct_mean(table, O.build_neighbour_construction(f_indptr, f_ids, [1, 2], 1.0))
+    assert O.euclidean(a, p) == 0.0
+
+
+# --------------------------------------------------------------------------------------
+# Margin loss
+# --------------------------------------------------------------------------------------
+def test_margin_loss_closed_form():
+    a = np.array([0.0, 0.0])
+    p = np.array([3.0, 4.0])          # D0 = 5
+    n1 = np.array([3.0, 0.0])         # D1 = 4 -> l = 1 + 5 - 4 = 2 (active)
+    n2 = np.array([3.0, 14.0])        # D2 = 10 -> l = -4 (inactive)
+    L, act, ga, gp, gn = O.margin_loss_adjoints(a, p, [n1, n2], gamma=1.0)
+    assert L == 2.0 and act.tolist() == [True, False]
+    assert np.allclose(ga, (a - p) / 5.0)
+    assert np.allclose(gn[0], (p - n1) / 4.0) and gn[1] is None
+    assert np.allclose(gp, (p - a) / 5.0 - (p - n1) / 4.0)
+    L0, act0, *_ = O.margin_loss_adjoints(a, p, [n2], gamma=1.0)
+    assert L0 == 0.0 and not act0.any()
+
+
+def test_margin_loss_translation_invariant():
+    rng = np.random.default_rng(0)
+    a, p, n1, n2 = rng.normal(size=(4, 5))
+    t = rng.normal(size=5)
+    L1 = O.margin_loss_adjoints(a, p, [n1, n2], 2.0)[0]
+    L2 = O.margin_loss_adjoints(a + t, p + t, [n1 + t, n2 + t], 2.0)[0]
+    assert abs(L1 - L2) < 1e-12 and L1 >= 0
+
+
+def test_margin_adjoints_central_fd():
+    rng = np.random.default_rng(1)
+    d, K = 6, 4
+    a, p = rng.normal(size=(2, d))
+    ns = list(rng.normal(size=(K, d)))
+    gaps = sorted(O.euclidean(p, n) - O.euclidean(p, a) for n in ns)
+    gamma = 0.5 * (gaps[K // 2 - 1] + gaps[K // 2])  # half the negatives active, away from the kink
+    L, act, ga, gp, gn = O.margin_loss_adjoints(a, p, ns, gamma)
+    assert act.any() and not act.all()
+
+    def f(a_, p_, ns_):
+        return O.margin_loss_adjoints(a_, p_, ns_, gamma)[0]
+
+    eps = 1e-6
+    for vec, g in [("a", ga), ("p", gp)] + [(i, gn[i]) for i in range(K)]:
+        for j in range(d):
+            def pert(s):
+                a_, p_, ns_ = a.copy(), p.copy(), [x.copy() for x in ns]
+                if vec == "a":
+                    a_[j] += s
+                elif vec == "p":
+                    p_[j] += s
+                else:
+                    ns_[vec][j] += s
+                return f(a_, p_, ns_)
+            fd = (pert(eps) - pert(-eps)) / (2 * eps)
+            exp = 0.0 if g is None else g[j]
+            assert abs(fd - exp) < 1e-7, (vec, j, fd, exp)
+
+
+# --------------------------------------------------------------------------------------
+# Gradients of one example: exact mode vs FD, first-leaf mode vs per-leaf autograd
+# --------------------------------------------------------------------------------------
+def _toy_example(seed=3, V=12, D=5, n_nodes=10, K=4):
+    rng = np.random.default_rng(seed)
+    counts = rng.integers(1, 5, size=n_nodes)
+    f_indptr = np.concatenate([[0], np.cumsum(counts)])
+    f_ids = rng.integers(0, V, size=f_indptr[-1])  # heavy feature sharing
+    table = rng.normal(size=(V, D))
+    nbrs = [1, 2, 3]
+    negs = list(rng.integers(0, n_nodes, size=K))
+    return table, f_indptr, f_ids, 0, nbrs, negs
+
+
+def _example_loss(table, f_indptr, f_ids, v, nbrs, negs, gamma):
+    return O.example_grads(table, f_indptr, f_ids, v, nbrs, 1.0, negs, gamma, "exact").loss
+
+
+@pytest.mark.parametrize("seed", [3, 4, 5])
+def test_exact_grads_central_fd(seed):
+    table, f_indptr, f_ids, v, nbrs, negs = _toy_example(seed)
+    gamma = 2.0
+    r = O.example_grads(table, f_indptr, f_ids, v, nbrs, 1.0, negs, gamma, "exact")
+    assert r.contributes
+    eps = 1e-6
+    for f in range(table.shape[0]):
+        for j in range(table.shape[1]):
+            tp, tm = table.copy(), table.copy()
+            tp[f, j] += eps
+            tm[f, j] -= eps
+            fd = (_example_loss(tp, f_indptr, f_ids, v, nbrs, negs, gamma)
+                  - _example_loss(tm, f_indptr, f_ids, v, nbrs, negs, gamma)) / (2 * eps)
+            g = r.grads[f][j] if f in r.grads else 0.0
+            assert abs(fd - g) < 1e-6, (f, j, fd, g)
+
+
+def torch_first_leaf(table, f_indptr, f_ids, v, nbrs, negs, gamma):
+    """The reference's mechanism (model.rs:257-278, mod.rs:329-378), in torch fp64:
+    one autograd leaf per (construction, feature); keep the first leaf's gradient per
+    feature in the order a, p, n_1..K; leaves the loss does not reach have no grad."""
+    def spec(nodes, weight):
+        out = []
+        for u in nodes:
+            for f in f_ids[f_indptr[u]:f_indptr[u + 1]]:
+                out.append((int(f), weight + 1.0))
+        return out
+
+    specs = [spec([v], 1.0), spec(nbrs, 1.0)] + [spec([u], 1.0) for u in negs]
+    leaves, vecs = [], []
+    for sp in specs:
+        d = {}
+        for f, w in sp:
+            if f in d:
+                d[f][1] += w
+            else:
+                d[f] = [torch.tensor(table[f], dtype=torch.float64, requires_grad=True), w]
+        num = sum(leaf * c for leaf, c in d.values())
+        den = sum(c for _, c in d.values())
+        vecs.append(num / den)
+        leaves.append(d)
+    a, p, ns = vecs[0], vecs[1], vecs[2:]
+    d1 = gamma + torch.sqrt(((p - a) ** 2).sum())
+    ls = [d1 - torch.sqrt(((p - n) ** 2).sum()) for n in ns]
+    pos = [l for l in ls if l.item() > 0]
+    if not pos:
+        return 0.0, {}
+    L = sum(pos) / len(pos)
+    L.backward()
+    grads = {}
+    for d in leaves:
+        for f, (leaf, _) in d.items():
+            if f in grads or leaf.grad is None:
+                continue
+            g = leaf.grad.numpy()
+            if np.all(np.isfinite(g)):
+                grads[f] = g.copy()
+    return L.item(), grads
+
+
+@pytest.mark.parametrize("seed", [3, 4, 5, 6, 7])
+def test_first_leaf_matches_per_leaf_autograd(seed):
+    table, f_indptr, f_ids, v, nbrs, negs = _toy_example(seed)
+    gamma = 2.0
+    r = O.example_grads(table, f_indptr, f_ids, v, nbrs, 1.0, negs, gamma, "first_leaf")
+    L, g = torch_first_leaf(table, f_indptr, f_ids, v, nbrs, negs, gamma)
+    assert abs(r.loss - L) < 1e-12
+    assert set(r.grads) == set(g)
+    for f in g:
+        assert np.allclose(r.grads[f], g[f], atol=1e-12)
+
+
+def test_first_leaf_differs_from_exact_only_on_shared_features():
+    table, f_indptr, f_ids, v, nbrs, negs = _toy_example(3)
+    fl = O.example_grads(table, f_indptr, f_ids, v, nbrs, 1.0, negs, 2.0, "first_leaf").grads
+    ex = O.example_grads(table, f_indptr, f_ids, v, nbrs, 1.0, negs, 2.0, "exact").grads
+    assert set(fl) == set(ex)
+    assert any(not np.allclose(fl[f], ex[f]) for f in fl)  # the toy shares features
+
+
+def test_first_leaf_equals_exact_when_constructions_are_disjoint():
+    V, D = 9, 3
+    rng = np.random.default_rng(2)
+    table = rng.normal(size=(V, D))
+    f_indptr = np.array([0, 2, 4, 6, 8])
+    f_ids = np.arange(8)
+    fl = O.example_grads(table, f_indptr, f_ids, 0, [1], 1.0, [2, 3], 5.0, "first_leaf").grads
+    ex = O.example_grads(table, f_indptr, f_ids, 0, [1], 1.0, [2, 3], 5.0, "exact").grads
+    assert set(fl) == set(ex)
+    for f in fl:
+        assert np.allclose(fl[f], ex[f])
+
+
+def test_zero_distance_drops_anchor_and_positive():
+    # a == p (D0 = 0): their adjoints are NaN and dropped (mod.rs:370-375); the negative keeps its
+    # gradient, and a feature shared with the negative is taken from the negative (first finite leaf)
+    table = np.array([[1.0, 0.0], [0.0, 1.0], [0.5, 0.5]])
+    f_indptr = np.array([0, 1, 2, 4])  # node0: [0], node1: [0], node2: [1, 0]
+    f_ids = np.array([0, 0, 1, 0])
+    r = O.example_grads(table, f_indptr, f_ids, 0, [1], 1.0, [2], 1.0, "first_leaf")
+    assert r.contributes
+    assert not np.all(np.isfinite(r.adjoints[0])) and not np.all(np.isfinite(r.adjoints[1]))
+    L, g = torch_first_leaf(table, f_indptr, f_ids, 0, [1], [2], 1.0)
+    assert set(r.grads) == set(g) == {0, 1}
+    for f in g:
+        assert np.allclose(r.grads[f], g[f])
+
+
+def test_batch_merge_is_dense_scatter_sum():
+    table, f_indptr, f_ids, v, nbrs, negs = _toy_example(5)
+    rs = [O.example_grads(table, f_indptr, f_ids, v, nbrs, 1.0, negs, 2.0, "first_leaf"),
+          O.example_grads(table, f_indptr, f_ids, 4, [5, 6], 1.0, negs[::-1], 2.0, "first_leaf")]
+    merged = O.merge_batch(rs, table.shape[1])
+    dense = np.zeros_like(table)
+    for r in rs:
+        for f, g in r.grads.items():
+            dense[f] += g
+    for f in range(table.shape[0]):
+        assert np.allclose(dense[f], merged.get(f, np.zeros(table.shape[1])))
+
+
+# --------------------------------------------------------------------------------------
+# Adam and the LR schedule
+# --------------------------------------------------------------------------------------
+def test_adam_first_update_closed_form():
+    V, D = 3, 4
+    rng = np.random.default_rng(4)
+    table = rng.normal(size=(V, D))
+    t0 = table.copy()
+    m, v = np.zeros((V, D)), np.zeros((V, D))
+    g = rng.normal(size=D)
+    lr = 0.05
+    O.adam_update(table, m, v, {1: g}, lr, pass_=1)  # t = 2
+    b1, b2 = 0.9, 0.999
+    step = lr * (g * (1 - b1) / (1 - b1 ** 2)) / (np.abs(g) * math.sqrt((1 - b2) / (1 - b2 ** 2)) + 1e-8)
+    assert np.allclose(table[1], t0[1] - step, atol=1e-14)
+    assert np.array_equal(table[0], t0[0]) and np.array_equal(table[2], t0[2])  # lazy: untouched rows stay
+
+
+def test_adam_skips_nan_rows():
+    table = np.ones((2, 2))
+    m, v = np.zeros((2, 2)), np.zeros((2, 2))
+    O.adam_update(table, m, v, {0: np.array([np.nan, 1.0]), 1: np.array([1.0, 1.0])}, 0.1, 1)
+    assert np.array_equal(table[0], [1.0, 1.0]) and not np.array_equal(table[1], [1.0, 1.0])
+    assert np.array_equal(m[0], [0.0, 0.0])
+
+
+def test_cos_decay_schedule_closed_form():
+    alpha, T = 0.01, 100
+    lo = alpha / 100
+    warm = 20
+    assert math.isclose(O.cos_decay_lr(1, alpha, T), lo + (1 / warm) * (alpha - lo))
+    assert math.isclose(O.cos_decay_lr(warm, alpha, T), alpha)
+    # no warm-up offset after warm-up: jump to the cosine at step/T (scheduler.rs:39-43)
+    assert math.isclose(O.cos_decay_lr(warm + 1, alpha, T), lo + 0.5 * (alpha - lo) * (1 + math.cos(math.pi * 21 / 100)))
+    assert math.isclose(O.cos_decay_lr(T, alpha, T), lo)
+
+
+def test_init_rows_unit_norm():
+    rng = np.random.default_rng(0)
+    x = O.normalize_rows(rng.uniform(size=(100, 16)))
+    assert np.allclose(np.linalg.norm(x, axis=1), 1.0)
+    assert abs(x.mean()) < 0.05
+
+
+# --------------------------------------------------------------------------------------
+# Attention: reference KATs src/algos/ep/attention.rs:303-425
+# --------------------------------------------------------------------------------------
+ATT_ROWS = np.array([[-1.0, -1.0, 1.0, 1.0], [0.0, 0.0, 2.0, 2.0], [1.0, 1.0, -1.0, -1.0]])
+
+
+def test_attention_matrix_kat():
+    Q, K, V, dm = O.attention_split(ATT_ROWS, heads=1, d_k=1)
+    S = O.attention_matrix_full(Q[0], K[0], np.ones(3))
+    assert S.tolist() == [[1.0, -0.0, -1.0], [-0.0, 0.0, 0.0], [-1.0, 0.0, 1.0]]
+
+
+def test_attention_softmax_kat():
+    Q, K, V, dm = O.attention_split(ATT_ROWS, heads=1, d_k=1)
+    S = O.attention_matrix_full(Q[0], K[0], np.ones(3))
+    A = np.stack([O.softmax_exact(S[i]) for i in range(3)])
+    exp = [[0.66524096, 0.24472847, 0.09003057], [1 / 3, 1 / 3, 1 / 3], [0.09003057, 0.24472847, 0.66524096]]
+    assert np.allclose(A, exp, atol=1e-7)
+
+
+def test_attention_reweighted_kat():
+    Q, K, V, dm = O.attention_split(ATT_ROWS, heads=1, d_k=1)
+    S = O.attention_matrix_full(Q[0], K[0], np.ones(3))
+    A = np.stack([O.softmax_exact(S[i]) for i in range(3)])
+    rows = A @ V[0]
+    exp = [[1.0647, 1.0647], [0.6667, 0.6667], [-0.0858, -0.0858]]
+    assert np.all(np.abs(rows - exp) < 1e-4)
+    # attention_mean = (1/n) sum_i rows_i, mean over heads (attention.rs:122-130)
+    assert np.allclose(O.attention_mean(ATT_ROWS, np.ones(3), 1, 1), rows.sum(axis=0) / 3)
+
+
+def test_attention_single_feature_returns_head0_value():
+    rows = np.array([[1.0, 2.0, 3.0, 4.0, 5.0, 6.0, 7.0, 8.0]])  # H=2, d_k=1, d_model=2
+    assert O.attention_mean(rows, np.ones(1), 2, 1).tolist() == [5.0, 6.0]
+
+
+def test_attention_count_scaling_kat():
+    # logits multiplied by c_i c_j when >= 1 (attention.rs:183-186)
+    Q, K, V, dm = O.attention_split(ATT_ROWS, heads=1, d_k=1)
+    S = O.attention_matrix_full(Q[0], K[0], np.array([2.0, 2.0, 2.0]))
+    assert np.allclose(S, 4.0 * (Q[0] @ K[0].T))
+    S2 = O.attention_matrix_full(Q[0], K[0], np.array([0.5, 0.5, 0.5]))
+    assert np.allclose(S2, Q[0] @ K[0].T)
+
+
+@pytest.mark.parametrize("n,heads,dk,dm", [(5, 2, 3, 4), (3, 1, 2, 2), (7, 4, 2, 3)])
+def test_attention_backward_vs_fd_and_torch(n, heads, dk, dm):
+    rng = np.random.default_rng(n * 10 + heads)
+    W = heads * (2 * dk + dm)
+    rows = rng.normal(size=(n, W))
+    counts = rng.integers(1, 3, size=n).astype(np.float64)
+    g = rng.normal(size=dm)
+    grad = O.attention_mean_backward(rows, counts, heads, dk, g)
+    # torch fp64 autograd on an independent formulation
+    R = torch.tensor(rows, dtype=torch.float64, requires_grad=True)
+    c = torch.tensor(counts, dtype=torch.float64)
+    out = 0
+    for h in range(heads):
+        Qh = R[:, h * dk:(h + 1) * dk]
+        Kh = R[:, heads * dk + h * dk: heads * dk + (h + 1) * dk]
+        Vh = R[:, 2 * heads * dk + h * dm: 2 * heads * dk + (h + 1) * dm]
+        num = torch.outer(c, c)
+        S = (Qh @ Kh.T) * torch.where(num >= 1, num, torch.ones_like(num)) / math.sqrt(dk)
+        A = torch.softmax(S, dim=1)
+        out = out + (A.sum(0) @ Vh) / n
+    out = out / heads
+    (out @ torch.tensor(g)).backward()
+    assert np.allclose(grad, R.grad.numpy(), atol=1e-10)
+    assert np.allclose(O.attention_mean(rows, counts, heads, dk), out.detach().numpy(), atol=1e-12)
+    eps = 1e-6
+    for i in range(n):
+        for j in range(W):
+            rp, rm = rows.copy(), rows.copy()
+            rp[i, j] += eps
+            rm[i, j] -= eps
+            fd = (O.attention_mean(rp, counts, heads, dk) @ g - O.attention_mean(rm, counts, heads, dk) @ g) / (2 * eps)
+            assert abs(fd - grad[i, j]) < 1e-7
+
+
+# --------------------------------------------------------------------------------------
+# AUC
+# --------------------------------------------------------------------------------------
+def test_roc_auc_matches_sklearn():
+    from sklearn.metrics import roc_auc_score
+
+    rng = np.random.default_rng(0)
+    pos = np.round(rng.normal(1.0, 1.0, 300), 1)
+    neg = np.round(rng.normal(0.0, 1.0, 400), 1)  # rounding creates ties
+    y = np.concatenate([np.ones(300), np.zeros(400)])
+    assert abs(O.roc_auc(pos, neg) - roc_auc_score(y, np.concatenate([pos, neg]))) < 1e-12
+    assert O.roc_auc(np.array([2.0, 3.0]), np.array([0.0, 1.0])) == 1.0
+    assert O.roc_auc(np.array([1.0]), np.array([1.0])) == 0.5
+
+
+def test_averaged_node_embeddings_closed_form():
+    table = np.array([[1.0, 1.0], [3.0, -1.0]])
+    out = O.averaged_node_embeddings(table, np.array([0, 2, 2, 3]), np.array([0, 1, 1]))
+    assert np.allclose(out, [[2.0, 0.0], [0.0, 0.0], [3.0, -1.0]])
