@@ -183,9 +183,11 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   if (c->step_exec) { (void)hipGraphExecDestroy(c->step_exec); c->step_exec = nullptr; c->step_exec_ready = false; }
 
   const uint64_t unique_cap = std::min<uint64_t>(c->n_vocab, keycap) + 1;
-  const uint32_t long_cap = (uint32_t)(keycap / 256 + 1);
-  const uint32_t piece_cap = (uint32_t)(keycap / 256 + long_cap + 1);
-  const int sort_grid = 1024;
+  // long segments are > 128 entries; pieces hold 128 entries (see LONG_SEG / PIECE)
+  const uint32_t long_cap = (uint32_t)(keycap / 128 + 1);
+  const uint32_t piece_cap = (uint32_t)(keycap / 128 + long_cap + 1);
+  // radix-sort grid: ~8K keys per block at capacity, a multiple of 256 in [256, 2048]
+  const int sort_grid = (int)std::min<uint64_t>(2048, std::max<uint64_t>(256, (keycap / 8192 + 255) / 256 * 256));
   EP_TRY(c->alloc(&b.nbr_cnt, batch));
   EP_TRY(c->alloc(&b.nbr, (size_t)batch * std::max<uint32_t>(S, 1)));
   EP_TRY(c->alloc(&b.neg, (size_t)batch * K));
@@ -199,7 +201,7 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
     EP_TRY(c->alloc(&b.keys[i], keycap));
     EP_TRY(c->alloc(&b.slots[i], keycap));
   }
-  EP_TRY(c->alloc(&b.ghist, (size_t)256 * sort_grid));
+  EP_TRY(c->alloc(&b.ghist, (size_t)2048 * sort_grid));
   EP_TRY(c->alloc(&b.G, (size_t)unique_cap * D));
   EP_TRY(c->alloc(&b.ufeat, unique_cap));
   EP_TRY(c->alloc(&b.long_st, long_cap));
@@ -223,6 +225,8 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   b.long_cap = long_cap;
   b.piece_cap = piece_cap;
   b.hash_entries = (uint32_t)hash;
+  b.ent_cap = (uint32_t)std::min<uint64_t>(256, 1ull + neighbour_cap(c) + K);
+  b.ids_cap = (uint32_t)std::min<uint64_t>(8192, example_rows_max(c));
   b.unique_grid = 2048;
   c->cap_batch = batch;
   c->cap_S = S;
@@ -264,8 +268,12 @@ void refresh_base(ep_ctx* c) {
   b.xout = nullptr;
 }
 
+// Launches one step.  update == false runs the sorted path in gradient-output mode (the
+// merged gradient lands in G/ufeat, nothing is updated); *sorted_buf receives the key
+// buffer holding the sorted keys.
 int launch_training_step(ep_ctx* c, const ep::StepConfig& sc, bool update, float lr_override,
-                         hipEvent_t ev_f0 = nullptr, hipEvent_t ev_f1 = nullptr, hipEvent_t ev_s1 = nullptr) {
+                         hipEvent_t ev_f0 = nullptr, hipEvent_t ev_f1 = nullptr, hipEvent_t ev_s1 = nullptr,
+                         int* sorted_buf = nullptr) {
   hipStream_t s = c->stream;
   if (lr_override >= 0.f) EP_HIP(ep::launch_step_begin_lr(sc, c->d_state, lr_override, s));
   else EP_HIP(ep::launch_step_begin(sc, c->d_state, s));
@@ -278,8 +286,14 @@ int launch_training_step(ep_ctx* c, const ep::StepConfig& sc, bool update, float
   if (sc.scatter == EP_SCATTER_SORT) {
     int buf = 0;
     EP_HIP(ep::launch_sort(sc, c->d_state, s, &buf));
-    EP_HIP(ep::launch_segreduce(sc, c->d_state, buf, s));
-    if (update) EP_HIP(ep::launch_adam_unique(sc, c->d_state, s));
+    if (sorted_buf) *sorted_buf = buf;
+    if (update) {
+      EP_HIP(ep::launch_scatter_sorted(sc, c->d_state, buf, s));
+    } else {
+      ep::StepConfig g = sc;
+      g.grad_out = 1;
+      EP_HIP(ep::launch_scatter_sorted(g, c->d_state, buf, s));
+    }
   } else {
     EP_HIP(ep::launch_atomic_scatter(sc, c->d_state, s));
     if (update) EP_HIP(ep::launch_adam_touched(sc, c->d_state, s));
@@ -290,7 +304,7 @@ int launch_training_step(ep_ctx* c, const ep::StepConfig& sc, bool update, float
 
 int check_state_err(ep_ctx* c) {
   ep::StepState hs;
-  EP_HIP(hipMemcpyAsync(&hs, c->d_state, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
+  EP_HIP(hipMemcpyAsync(&hs, c->d_state, offsetof(ep::StepState, dtot), hipMemcpyDeviceToHost, c->stream));
   EP_HIP(hipStreamSynchronize(c->stream));
   if (hs.err) return fail(EP_ECAPACITY, "key capacity exceeded in a step");
   return EP_OK;
@@ -749,17 +763,24 @@ int ep_step_explicit(ep_ctx* c, const uint32_t* anchors, uint32_t n, const uint3
   EP_TRY(set_pass(c, c->d_state, pass, nullptr));
   const bool upd = apply_update != 0;
   // run everything up to (but excluding) the update, read the merged gradient, then update
-  EP_TRY(launch_training_step(c, sc, false, lr));
+  int buf = 0;
+  EP_TRY(launch_training_step(c, sc, false, lr, nullptr, nullptr, nullptr, &buf));
   EP_TRY(check_state_err(c));
   ep::StepState hs;
-  EP_HIP(hipMemcpy(&hs, c->d_state, sizeof(hs), hipMemcpyDeviceToHost));
+  EP_HIP(hipMemcpy(&hs, c->d_state, offsetof(ep::StepState, dtot), hipMemcpyDeviceToHost));
   if (loss) EP_HIP(hipMemcpy(loss, sc.loss, n * sizeof(float), hipMemcpyDeviceToHost));
   if (cons) EP_HIP(hipMemcpy(cons, c->d_xout, (size_t)n * NC * D * sizeof(float), hipMemcpyDeviceToHost));
   if (sc.scatter == EP_SCATTER_SORT) {
     if (n_unique) *n_unique = hs.n_unique;
     if (feat) EP_HIP(hipMemcpy(feat, sc.ufeat, hs.n_unique * sizeof(uint32_t), hipMemcpyDeviceToHost));
     if (grad) EP_HIP(hipMemcpy(grad, sc.G, (size_t)hs.n_unique * D * sizeof(float), hipMemcpyDeviceToHost));
-    if (upd) EP_HIP(ep::launch_adam_unique(sc, c->d_state, c->stream));
+    if (upd) {
+      // the update pass re-reads the sorted keys; its long-segment lists start empty
+      const uint32_t zero4[4] = {0u, 0u, 0u, 0u};
+      EP_HIP(hipMemcpy((char*)c->d_state + offsetof(ep::StepState, n_unique), zero4, sizeof(zero4),
+                       hipMemcpyHostToDevice));
+      EP_HIP(ep::launch_scatter_sorted(sc, c->d_state, buf, c->stream));
+    }
   } else {
     if (n_unique) *n_unique = hs.n_touched;
     std::vector<uint32_t> tf(hs.n_touched);

@@ -171,6 +171,12 @@ __device__ inline double wave_sum(double x) {
   return x;
 }
 
+__device__ inline uint32_t wave_sum_u32(uint32_t x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, EP_WAVE);
+  return x;
+}
+
 __device__ inline bool wave_any(bool p) { return __ballot(p) != 0ull; }
 
 __device__ inline uint32_t lane_id() { return __lane_id(); }

@@ -28,7 +28,7 @@ struct StepState {
   float lr;                // learning rate of this step
   float bc1, bc2;          // Adam bias corrections 1 - beta^t, t = pass + 1
   float pad1;
-  uint32_t dtot[4][256];   // radix-sort digit totals, one row per sort pass
+  uint32_t dtot[4][2048];  // radix-sort digit totals, one row per sort pass (<= 11-bit digits)
 };
 
 // Static description of the step (fixed for the life of a captured graph).
@@ -82,11 +82,13 @@ struct StepConfig {
   float* pass_loss;        // [n_batch_total] or nullptr
   float* xout;             // [B*NC*D] constructions (test hook) or nullptr
   uint32_t hash_entries;   // first-leaf LDS hash size (power of two)
+  uint32_t ent_cap;        // forward LDS stage: max entries (nodes) per example, <= 256
+  uint32_t ids_cap;        // forward LDS stage: max feature occurrences per example
   // keys
   uint32_t* keys[2];
   uint32_t* slots[2];
   uint32_t key_cap;
-  uint32_t* ghist;         // [256 * sort_grid]
+  uint32_t* ghist;         // [2048 * sort_grid]
   int sort_grid;
   // scatter (sort path)
   float* G;                // [key_cap*D] compact gradient rows (bounded by unique count)
@@ -102,6 +104,7 @@ struct StepConfig {
   uint32_t* flag;          // [V]
   uint32_t* touched;       // [V]
   int scatter;             // 0 sort, 1 atomic
+  int grad_out;            // test hook: write merged gradients (G/ufeat) instead of updating
   int unique_grid;         // grid size for the persistent row kernels
 };
 
@@ -115,8 +118,8 @@ hipError_t launch_forward(const StepConfig& c, StepState* st, hipStream_t s);
 hipError_t launch_emit(const StepConfig& c, StepState* st, hipStream_t s);
 // Sort keys/slots; returns which buffer (0/1) holds the result in *out_buf.
 hipError_t launch_sort(const StepConfig& c, StepState* st, hipStream_t s, int* out_buf);
-hipError_t launch_segreduce(const StepConfig& c, StepState* st, int buf, hipStream_t s);
-hipError_t launch_adam_unique(const StepConfig& c, StepState* st, hipStream_t s);
+// Segment reduce + Adam over the sorted keys (grad_out: write merged gradients instead).
+hipError_t launch_scatter_sorted(const StepConfig& c, StepState* st, int buf, hipStream_t s);
 hipError_t launch_atomic_scatter(const StepConfig& c, StepState* st, hipStream_t s);
 hipError_t launch_adam_touched(const StepConfig& c, StepState* st, hipStream_t s);
 // One-off kernels.

@@ -1,7 +1,7 @@
 // HIP kernels for one Embedding Propagation training step on gfx950 (MI355X).
 //
 // Step = k_step_begin -> k_sample -> k_scan -> k_forward -> k_emit ->
-//        {radix sort -> k_segreduce -> k_pieces -> k_long_combine -> k_adam_unique}
+//        {radix sort -> k_seg_adam -> k_pieces -> k_long_combine}
 //     or {k_atomic_scatter -> k_adam_touched}
 //
 // Every kernel reads the per-step values (batch offset, batch size, learning rate,
@@ -15,8 +15,9 @@
 namespace ep {
 
 static constexpr int NW = 4;             // waves per 256-thread block
-static constexpr uint32_t LONG_SEG = 256;  // segments longer than this are split into pieces
-static constexpr uint32_t PIECE = 256;     // entries per piece of a long segment
+static constexpr uint32_t LONG_SEG = 128;  // segments longer than this are split into pieces
+static constexpr uint32_t PIECE = 128;     // entries per piece of a long segment
+static constexpr int SR_UNROLL = 16;       // adjoint rows in flight per wave in the scatter
 static constexpr uint32_t HASH_EMPTY = 0xFFFFFFFFu;
 static constexpr uint32_t HASH_FBITS = 27;
 static constexpr uint32_t HASH_FMASK = (1u << HASH_FBITS) - 1u;
@@ -136,7 +137,7 @@ __device__ inline uint32_t hash_lookup(const uint32_t* h, uint32_t mask, uint32_
 // ------------------------------------------------------------------------------------
 __global__ void k_step_begin(StepConfig c, StepState* st, float lr_override) {
   uint32_t* dt = &st->dtot[0][0];
-  for (int i = threadIdx.x; i < 4 * 256; i += blockDim.x) dt[i] = 0u;
+  for (int i = threadIdx.x; i < 4 * 2048; i += blockDim.x) dt[i] = 0u;
   if (threadIdx.x == 0) {
     const uint32_t sip = st->step_in_pass + 1u;  // host writes 0xFFFFFFFF at pass start
     st->step_in_pass = sip;
@@ -176,33 +177,29 @@ __global__ void k_step_begin(StepConfig c, StepState* st, float lr_override) {
 }
 
 // ------------------------------------------------------------------------------------
-// k_sample: neighbour sample (S of deg without replacement, Floyd's algorithm) and K
-// uniform-with-replacement negatives per anchor; R_e = feature occurrences of the example.
-// model.rs:354-377 + utils.rs:193-216 (Fixed(S): all neighbours when deg <= S) and
-// node_sampler.rs:185-188 (negatives uniform from the pool).
+// k_sample: one wavefront per anchor.  Neighbour sample: S of deg without replacement
+// (Floyd's algorithm; picks in LDS, the duplicate test is one ballot), all
+// neighbours when deg <= S; K negatives uniform with replacement from the pool; R_e =
+// feature occurrences of the example.  model.rs:354-377 + utils.rs:193-216 (Fixed(S))
+// and node_sampler.rs:185-188.  Draw i of a purpose is word i%4 of Philox block i/4.
 // ------------------------------------------------------------------------------------
-struct Draws {
-  u32x4 r;
-  int left;
-  uint32_t blk;
-};
-
-__device__ inline uint32_t next_draw(Draws& d, uint32_t pos, uint32_t pass, uint32_t purpose, uint32_t k0, uint32_t k1) {
-  if (d.left == 0) {
-    d.r = philox4x32(u32x4{pos, pass, purpose, d.blk++}, k0, k1);
-    d.left = 4;
-  }
-  --d.left;
-  return d.left == 3 ? d.r.x : d.left == 2 ? d.r.y : d.left == 1 ? d.r.z : d.r.w;
+__device__ inline uint32_t draw_word(uint32_t pos, uint32_t pass, uint32_t purpose, uint32_t i, uint32_t k0,
+                                     uint32_t k1) {
+  const u32x4 r = philox4x32(u32x4{pos, pass, purpose, i >> 2}, k0, k1);
+  const uint32_t q = i & 3u;
+  return q == 0 ? r.x : q == 1 ? r.y : q == 2 ? r.z : r.w;
 }
 
 __global__ __launch_bounds__(256) void k_sample(StepConfig c, StepState* st) {
-  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const uint32_t b = blockIdx.x * NW + (threadIdx.x >> 6);
   if (b >= c.batch) return;
   const uint32_t bn = st->batch_n;
   if (b >= bn) {
-    c.rcount[b] = 0u;
-    c.nbr_cnt[b] = 0u;
+    if (lane == 0) {
+      c.rcount[b] = 0u;
+      c.nbr_cnt[b] = 0u;
+    }
     return;
   }
   const uint32_t pos = st->anchor_base + b;
@@ -219,40 +216,41 @@ __global__ __launch_bounds__(256) void k_sample(StepConfig c, StepState* st) {
       cnt = deg;
     } else if (deg <= c.S) {
       cnt = deg;
-      for (uint32_t j = 0; j < deg; ++j) out[j] = c.indices[s0 + j];
+      for (uint32_t j = lane; j < deg; j += 64u) out[j] = c.indices[s0 + j];
     } else {
       cnt = c.S;
-      Draws d{u32x4{0, 0, 0, 0}, 0, 0};
+      // Floyd: for i = 0..S-1 draw t in [0, deg-S+i]; take deg-S+i instead when t was
+      // already picked.  Picks live in LDS; the membership test is one ballot.
+      extern __shared__ uint32_t s_pick[];
+      uint32_t* pk = s_pick + (threadIdx.x >> 6) * c.S;
       for (uint32_t i = 0; i < c.S; ++i) {
-        const uint32_t hi = deg - c.S + i;  // draw t in [0, hi]
-        const uint32_t t = uniform_below(next_draw(d, pos, pass, RNG_NEIGHBOURS, k0, k1), hi + 1u);
+        const uint32_t hi = deg - c.S + i;
+        const uint32_t t = uniform_below(draw_word(pos, pass, RNG_NEIGHBOURS, i, k0, k1), hi + 1u);
         bool dup = false;
-        for (uint32_t q = 0; q < i; ++q) dup |= (out[q] == t);
-        out[i] = dup ? hi : t;
+        for (uint32_t q = lane; q < i; q += 64u) dup |= (pk[q] == t);
+        const bool any = wave_any(dup);
+        if (lane == 0) pk[i] = any ? hi : t;
+        __builtin_amdgcn_wave_barrier();
       }
-      for (uint32_t i = 0; i < c.S; ++i) out[i] = c.indices[s0 + out[i]];
+      for (uint32_t j = lane; j < c.S; j += 64u) out[j] = c.indices[s0 + pk[j]];
     }
-    Draws d{u32x4{0, 0, 0, 0}, 0, 0};
-    for (uint32_t k = 0; k < c.K; ++k)
-      negs[k] = c.negpool[uniform_below(next_draw(d, pos, pass, RNG_NEGATIVES, k0, k1), c.n_negpool)];
-    c.nbr_cnt[b] = cnt;
+    for (uint32_t k = lane; k < c.K; k += 64u)
+      negs[k] = c.negpool[uniform_below(draw_word(pos, pass, RNG_NEGATIVES, k, k0, k1), c.n_negpool)];
+    if (lane == 0) c.nbr_cnt[b] = cnt;
   } else {
     cnt = c.nbr_cnt[b];
   }
   uint32_t R = 0;
   if (cnt > 0u) {
-    R += (uint32_t)(c.findptr[v + 1] - c.findptr[v]);
     const uint32_t* pn = c.all_nbrs ? c.indices + s0 : out;
-    for (uint32_t j = 0; j < cnt; ++j) {
-      const uint32_t u = pn[j];
+    const uint32_t E = 1u + cnt + c.K;  // v, neighbours, negatives
+    for (uint32_t e = lane; e < E; e += 64u) {
+      const uint32_t u = e == 0u ? v : (e <= cnt ? pn[e - 1u] : negs[e - 1u - cnt]);
       R += (uint32_t)(c.findptr[u + 1] - c.findptr[u]);
     }
-    for (uint32_t k = 0; k < c.K; ++k) {
-      const uint32_t u = negs[k];
-      R += (uint32_t)(c.findptr[u + 1] - c.findptr[u]);
-    }
+    R = wave_sum_u32(R);
   }
-  c.rcount[b] = R;
+  if (lane == 0) c.rcount[b] = R;
 }
 
 // ------------------------------------------------------------------------------------
@@ -302,6 +300,35 @@ __global__ __launch_bounds__(1024) void k_scan(StepConfig c, StepState* st) {
   }
 }
 
+// Accumulate rows table[ids[a..b)] (ids in LDS) into acc, U row loads in flight.
+template <int D, int U, typename T>
+__device__ inline void gather_lds(const T* __restrict__ table, const uint32_t* ids, uint32_t a, uint32_t b, int lane,
+                                  float (&acc)[RowShape<D>::PL]) {
+  constexpr int PL = RowShape<D>::PL;
+  for (uint32_t j = a; j < b; j += U) {
+    float r[U][PL];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t t = (j + u < b) ? (j + u) : (b - 1u);
+      load_slice<D>(table + (size_t)ids[t] * D, lane, r[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (j + u < b) {
+#pragma unroll
+        for (int q = 0; q < PL; ++q) acc[q] += r[u][q];
+      }
+    }
+  }
+}
+
+// 32-bit words of the float part of the forward kernel's LDS (xc, part, dist, sn),
+// rounded up to an even count so the 64-bit entry array after it is aligned.
+__host__ __device__ inline uint32_t forward_float_words(uint32_t NC, uint32_t K, uint32_t D) {
+  const uint32_t w = (NC + NW) * D + (K + 1u) + NC;
+  return (w + 1u) & ~1u;
+}
+
 // ------------------------------------------------------------------------------------
 // k_forward: one 256-thread block per anchor.
 //   gather + mean of h(v), ~h(v), h(u_1..K)     (model.rs:257-301, 342-416, 448-463)
@@ -314,6 +341,7 @@ template <int D, typename T>
 __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
   constexpr int PL = RowShape<D>::PL;
   constexpr int U = 8;
+  constexpr int GU = PL <= 2 ? 16 : 8;  // rows in flight per wave on the staged path
   const uint32_t b = blockIdx.x;
   if (b >= st->batch_n) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -342,8 +370,69 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
   const uint32_t* pn = c.all_nbrs ? c.indices + vs : c.nbr + (size_t)b * c.S;
   const int base = lane * PL;
 
-  // ~h(v): neighbour nodes dealt round-robin to the waves.
-  {
+  __shared__ uint32_t s_wave[NW];
+  __shared__ uint32_t s_np[NW];
+  // Staged path: the example's occurrence list is copied to LDS first (node ranges, then
+  // feature ids, one block-wide round trip each), so every wave streams rows from a
+  // contiguous id range with GU loads in flight.  Entries: e = 0 (v), 1..cnt
+  // (neighbours), cnt+1..cnt+K (negatives).
+  const uint32_t E = 1u + cnt + K;
+  const uint32_t Re = c.rcount[b];
+  const bool staged = (E <= c.ent_cap) && (Re <= c.ids_cap);
+  uint64_t* ent_fs = (uint64_t*)(sm + forward_float_words(NC, K, D));
+  uint32_t* ent_off = (uint32_t*)(ent_fs + c.ent_cap);
+  uint32_t* ids = ent_off + c.ent_cap + 1;
+  if (staged) {
+    uint32_t ne = 0;
+    if ((uint32_t)tid < E) {
+      const uint32_t node = tid == 0 ? v : ((uint32_t)tid <= cnt ? pn[tid - 1] : c.neg[(size_t)b * K + tid - 1 - cnt]);
+      const uint64_t fs = c.findptr[node];
+      ne = (uint32_t)(c.findptr[node + 1] - fs);
+      ent_fs[tid] = fs;
+    }
+    uint32_t tot;
+    const uint32_t excl = block_exclusive_scan(ne, s_wave, &tot);
+    if ((uint32_t)tid < E) ent_off[tid] = excl;
+    if (tid == 0) ent_off[E] = tot;
+    __syncthreads();
+    for (uint32_t o = tid; o < tot; o += blockDim.x) {
+      uint32_t lo = 0, hi = E;  // last entry e with ent_off[e] <= o
+      while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (ent_off[mid] <= o) lo = mid; else hi = mid;
+      }
+      ids[o] = c.fids[ent_fs[lo] + (o - ent_off[lo])];
+    }
+    __syncthreads();
+    // ~h(v): the pooled neighbour occurrences split evenly over the waves
+    const uint32_t p0 = ent_off[1], p1 = ent_off[1 + cnt];
+    const uint32_t per = (p1 - p0 + NW - 1) / NW;
+    const uint32_t a0 = min(p1, p0 + w * per), a1 = min(p1, a0 + per);
+    float acc[PL];
+#pragma unroll
+    for (int q = 0; q < PL; ++q) acc[q] = 0.f;
+    gather_lds<D, GU>(table, ids, a0, a1, lane, acc);
+#pragma unroll
+    for (int q = 0; q < PL; ++q)
+      if (base + q < D) part[w * D + base + q] = acc[q];
+    if (tid == 0) sn[1] = (float)(p1 - p0);
+    // single-node constructions: i = 0 -> h(v) (c=0), i >= 1 -> h(u_i) (c=i+1)
+    for (uint32_t i = w; i < K + 1u; i += NW) {
+      const uint32_t e = i == 0u ? 0u : cnt + i;
+      const uint32_t ci = i == 0u ? 0u : i + 1u;
+      float a2[PL];
+#pragma unroll
+      for (int q = 0; q < PL; ++q) a2[q] = 0.f;
+      gather_lds<D, GU>(table, ids, ent_off[e], ent_off[e + 1], lane, a2);
+      const float n = (float)(ent_off[e + 1] - ent_off[e]);
+#pragma unroll
+      for (int q = 0; q < PL; ++q)
+        if (base + q < D) xc[ci * D + base + q] = a2[q] / n;
+      if (lane == 0) sn[ci] = n;
+    }
+    __syncthreads();
+  } else {
+    // Fallback for examples larger than the LDS stage: neighbour nodes dealt round-robin.
     float acc[PL];
 #pragma unroll
     for (int q = 0; q < PL; ++q) acc[q] = 0.f;
@@ -357,9 +446,7 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
 #pragma unroll
     for (int q = 0; q < PL; ++q)
       if (base + q < D) part[w * D + base + q] = acc[q];
-    __shared__ uint32_t s_np[NW];
     if (lane == 0) s_np[w] = np;
-    // single-node constructions: i = 0 -> h(v) (c=0), i >= 1 -> h(u_i) (c=i+1)
     for (uint32_t i = w; i < K + 1u; i += NW) {
       const uint32_t node = i == 0u ? v : c.neg[(size_t)b * K + i - 1u];
       const uint32_t ci = i == 0u ? 0u : i + 1u;
@@ -472,10 +559,17 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
         if (wave_any(bad)) e = false;
       }
     }
-    if (e) store_slice<D>(c.adj + ((size_t)b * NC + ci) * D, lane, g);
+    // stored pre-scaled by the occurrence coefficient 1/n_c (count/total of the mean),
+    // so a feature's gradient through this construction is (multiplicity) x adj
+    const float co = 1.0f / sn[ci];
+    if (e) {
+#pragma unroll
+      for (int q = 0; q < PL; ++q) g[q] *= co;
+      store_slice<D>(c.adj + ((size_t)b * NC + ci) * D, lane, g);
+    }
     if (lane == 0) {
       c.elig[(size_t)b * NC + ci] = e ? 1 : 0;
-      c.coef[(size_t)b * NC + ci] = 1.0f / sn[ci];
+      c.coef[(size_t)b * NC + ci] = co;
     }
   }
 }
@@ -562,9 +656,13 @@ __global__ __launch_bounds__(256) void k_emit(StepConfig c, StepState* st) {
 
 // ------------------------------------------------------------------------------------
 // LSD radix sort of (key, slot) pairs with the element count read from device memory.
-// Stable: within a round, ranks follow thread order (wave-level match via ballots);
-// rounds and blocks are ordered through per-block digit bases.
+// RB-bit digits (RB <= 11).  Each block owns a contiguous range, processed in tiles of
+// 256 x RS_ITEMS elements loaded up front.  Stable: within a round of 256 elements the
+// ranks follow thread order (wave-level match via RB ballots); rounds, tiles and blocks
+// are ordered through per-block digit bases.
 // ------------------------------------------------------------------------------------
+static constexpr int RS_ITEMS = 8;
+
 __device__ inline void block_range(uint32_t n, int G, uint32_t blk, uint32_t* lo, uint32_t* hi) {
   const uint64_t per = (((uint64_t)n + G - 1) / G + 255u) / 256u * 256u;
   const uint64_t l = (uint64_t)blk * per;
@@ -573,214 +671,364 @@ __device__ inline void block_range(uint32_t n, int G, uint32_t blk, uint32_t* lo
   *hi = (uint32_t)(h < n ? h : n);
 }
 
-__global__ __launch_bounds__(256) void k_rs_hist(const uint32_t* __restrict__ kin, StepState* st, uint32_t* ghist,
-                                                 int pass, int shift, uint32_t mask, int G) {
-  __shared__ uint32_t h[256];
-  const int tid = threadIdx.x;
-  uint32_t lo, hi;
-  block_range(st->n_keys, G, blockIdx.x, &lo, &hi);
-  h[tid] = 0u;
-  __syncthreads();
-  for (uint32_t i = lo + tid; i < hi; i += 256u) atomicAdd(&h[(kin[i] >> shift) & mask], 1u);
-  __syncthreads();
-  ghist[(size_t)tid * G + blockIdx.x] = h[tid];
-  if (h[tid]) atomicAdd(&st->dtot[pass][tid], h[tid]);
+// Lanes of the wave holding the same digit as this lane (active lanes only).
+template <int RB>
+__device__ inline uint64_t digit_peers(bool act, uint32_t d) {
+  uint64_t peers = __ballot(act);
+#pragma unroll
+  for (int bit = 0; bit < RB; ++bit) {
+    const bool set = (d >> bit) & 1u;
+    const uint64_t bb = __ballot(act && set);
+    peers &= set ? bb : ~bb;
+  }
+  return act ? peers : 0ull;
 }
 
-__global__ __launch_bounds__(256) void k_rs_scan(uint32_t* ghist, StepState* st, int pass, int G) {
-  __shared__ uint32_t s_wave[NW];
-  __shared__ uint32_t s_red[256];
-  const int tid = threadIdx.x;
-  const uint32_t d = blockIdx.x;
-  s_red[tid] = (uint32_t)tid < d ? st->dtot[pass][tid] : 0u;
+template <int RB>
+__global__ __launch_bounds__(256) void k_rs_hist(const uint32_t* __restrict__ kin, StepState* st, uint32_t* ghist,
+                                                 int pass, int shift, int G) {
+  constexpr uint32_t BINS = 1u << RB, MASK = BINS - 1u;
+  __shared__ uint32_t h[BINS];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint32_t lo, hi;
+  block_range(st->n_keys, G, blockIdx.x, &lo, &hi);
+  for (uint32_t d = tid; d < BINS; d += 256u) h[d] = 0u;
   __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if (tid < off) s_red[tid] += s_red[tid + off];
-    __syncthreads();
+  for (uint32_t t0 = lo; t0 < hi; t0 += 256u * RS_ITEMS) {
+    uint32_t k[RS_ITEMS];
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+      const uint32_t i = t0 + r * 256u + tid;
+      k[r] = i < hi ? kin[i] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+      const uint32_t i = t0 + r * 256u + tid;
+      const bool act = i < hi;
+      const uint32_t d = (k[r] >> shift) & MASK;
+      const uint64_t peers = digit_peers<RB>(act, d);
+      if (act && (peers & lt) == 0ull) atomicAdd(&h[d], (uint32_t)__popcll(peers));  // lowest lane of the group
+    }
   }
-  const uint32_t pre = s_red[0];
-  uint32_t* row = ghist + (size_t)d * G;
-  const int per = G / 256;  // G is a multiple of 256
+  __syncthreads();
+  for (uint32_t d = tid; d < BINS; d += 256u) {
+    ghist[(size_t)blockIdx.x * BINS + d] = h[d];  // block-major: coalesced
+    if (h[d]) atomicAdd(&st->dtot[pass][d], h[d]);
+  }
+}
+
+// Turns the block-major counts into scatter bases: base(blk, d) = sum_{d' < d} total(d')
+// + sum_{b < blk} count(b, d).  One wave per digit; lane l owns G/64 consecutive source
+// blocks (their loads are independent), then a wave scan.
+template <int RB>
+__global__ __launch_bounds__(256) void k_rs_scan(uint32_t* ghist, StepState* st, int pass, int G) {
+  constexpr uint32_t BINS = 1u << RB;
+  const int lane = threadIdx.x & 63;
+  const uint32_t d = blockIdx.x * NW + (threadIdx.x >> 6);
+  if (d >= BINS) return;
+  uint32_t pre = 0;
+  for (uint32_t q = lane; q < d; q += 64u) pre += st->dtot[pass][q];
+  pre = wave_sum_u32(pre);
+  const uint32_t per = (uint32_t)G / 64u;  // G is a multiple of 256
+  const uint32_t b0 = (uint32_t)lane * per;
   uint32_t loc = 0;
-  for (int i = 0; i < per; ++i) loc += row[tid * per + i];
-  uint32_t total;
-  uint32_t run = pre + block_exclusive_scan(loc, s_wave, &total);
-  for (int i = 0; i < per; ++i) {
-    const uint32_t x = row[tid * per + i];
-    row[tid * per + i] = run;
+  for (uint32_t i = 0; i < per; ++i) loc += ghist[(size_t)(b0 + i) * BINS + d];
+  uint32_t inc = loc;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(inc, off, EP_WAVE);
+    if (lane >= off) inc += y;
+  }
+  uint32_t run = pre + inc - loc;
+  for (uint32_t i = 0; i < per; ++i) {
+    const size_t idx = (size_t)(b0 + i) * BINS + d;
+    const uint32_t x = ghist[idx];
+    ghist[idx] = run;
     run += x;
   }
 }
 
+template <int RB>
 __global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                     uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                    StepState* st, const uint32_t* __restrict__ ghist,
-                                                    int shift, uint32_t mask, int bits, int G) {
-  __shared__ uint32_t run[256];
-  __shared__ uint32_t wc[NW][256];
+                                                    StepState* st, const uint32_t* __restrict__ ghist, int shift,
+                                                    int G) {
+  constexpr uint32_t BINS = 1u << RB, MASK = BINS - 1u;
+  __shared__ uint32_t run[BINS];
+  __shared__ uint32_t wc[NW][BINS];  // (round << 7) | count of wave w for digit d in that round
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   uint32_t lo, hi;
   block_range(st->n_keys, G, blockIdx.x, &lo, &hi);
-  run[tid] = ghist[(size_t)tid * G + blockIdx.x];
+  for (uint32_t d = tid; d < BINS; d += 256u) {
+    run[d] = ghist[(size_t)blockIdx.x * BINS + d];
 #pragma unroll
-  for (int q = 0; q < NW; ++q) wc[q][tid] = 0xFFFFFFFFu;
+    for (int q = 0; q < NW; ++q) wc[q][d] = 0xFFFFFFFFu;
+  }
   __syncthreads();
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint32_t r = 0;
-  for (uint32_t i0 = lo; i0 < hi; i0 += 256u, ++r) {
-    const uint32_t i = i0 + tid;
-    const bool act = i < hi;
-    const uint32_t k = act ? kin[i] : 0u;
-    const uint32_t val = act ? vin[i] : 0u;
-    const uint32_t d = (k >> shift) & mask;
-    uint64_t peers = __ballot(act);
-    for (int bit = 0; bit < bits; ++bit) {
-      const bool set = (d >> bit) & 1u;
-      const uint64_t bb = __ballot(act && set);
-      peers &= set ? bb : ~bb;
-    }
-    if (!act) peers = 0ull;
-    const uint32_t rank = (uint32_t)__popcll(peers & lt);
-    const uint32_t cw = (uint32_t)__popcll(peers);
-    const uint32_t tag = r << 7;
-    if (act && rank == cw - 1u) wc[w][d] = tag | cw;
-    __syncthreads();
-    if (act) {
-      uint32_t off = run[d];
-      for (int q = 0; q < w; ++q) {
-        const uint32_t x = wc[q][d];
-        if ((x >> 7) == r) off += x & 0x7Fu;
-      }
-      kout[off + rank] = k;
-      vout[off + rank] = val;
-    }
-    __syncthreads();
-    {
-      uint32_t tot = 0;
+  for (uint32_t t0 = lo; t0 < hi; t0 += 256u * RS_ITEMS) {
+    uint32_t k[RS_ITEMS], val[RS_ITEMS];
 #pragma unroll
-      for (int q = 0; q < NW; ++q) {
-        const uint32_t x = wc[q][tid];
-        if ((x >> 7) == r) tot += x & 0x7Fu;
-      }
-      run[tid] += tot;
+    for (int q = 0; q < RS_ITEMS; ++q) {
+      const uint32_t i = t0 + q * 256u + tid;
+      k[q] = i < hi ? kin[i] : 0u;
+      val[q] = i < hi ? vin[i] : 0u;
     }
-    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < RS_ITEMS; ++q, ++r) {
+      const uint32_t i = t0 + q * 256u + tid;
+      const bool act = i < hi;
+      const uint32_t d = (k[q] >> shift) & MASK;
+      const uint64_t peers = digit_peers<RB>(act, d);
+      const uint32_t rank = (uint32_t)__popcll(peers & lt);
+      const uint32_t cw = (uint32_t)__popcll(peers);
+      if (act && rank == cw - 1u) wc[w][d] = (r << 7) | cw;
+      __syncthreads();
+      if (act) {
+        uint32_t off = run[d];
+        for (int q2 = 0; q2 < w; ++q2) {
+          const uint32_t x = wc[q2][d];
+          if ((x >> 7) == r) off += x & 0x7Fu;
+        }
+        kout[off + rank] = k[q];
+        vout[off + rank] = val[q];
+      }
+      __syncthreads();
+      for (uint32_t dd = tid; dd < BINS; dd += 256u) {
+        uint32_t tot = 0;
+#pragma unroll
+        for (int q2 = 0; q2 < NW; ++q2) {
+          const uint32_t x = wc[q2][dd];
+          if ((x >> 7) == r) tot += x & 0x7Fu;
+        }
+        run[dd] += tot;
+      }
+      __syncthreads();
+    }
   }
 }
 
 // ------------------------------------------------------------------------------------
-// Segment reduce over the sorted keys.  G_f = sum over entries of coef[slot]*adj[slot]
-// (the batch merge of mod.rs:223-233).  Each wave owns the segments that START in its
-// 64-entry chunk; the last of them may run past the chunk (<= LONG_SEG entries) or be
-// deferred as a long segment split into PIECE-entry pieces.
+// Sparse lazy Adam (optimizer.rs:43-81): skip the row if its gradient has any NaN
+// (line 56); m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2;
+// theta -= lr (m/bc1) / (sqrt(v/bc2) + eps).  bf16 mode: the FP32 master row is updated
+// and written back to the bf16 table with stochastic rounding (Philox bits keyed by
+// row, column and step).
+// ------------------------------------------------------------------------------------
+__device__ inline void adam_math(float& th, float& m, float& v, float g, float lr, float bc1, float bc2) {
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  m = b1 * m + (1.0f - b1) * g;
+  v = b2 * v + (1.0f - b2) * (g * g);
+  const float mh = m / bc1;
+  const float vh = v / bc2;
+  th -= lr * (mh / (sqrtf(vh) + eps));
+}
+
+__device__ inline uint32_t sr_bits(const StepConfig& c, const StepState* st, uint32_t f, uint32_t col4, int q) {
+  const u32x4 r = philox4x32(u32x4{f, col4, (uint32_t)st->next_step, RNG_ROUNDING}, (uint32_t)c.seed,
+                             (uint32_t)(c.seed >> 32));
+  const uint32_t w = (q >> 1) == 0 ? r.x : r.y;
+  return (q & 1) ? (w >> 16) : (w & 0xffffu);
+}
+
+// Whole-wave row layout (lane owns RowShape<D>::PL values).
+template <int D>
+__device__ inline void adam_row(const StepConfig& c, const StepState* st, uint32_t f, const float (&g)[RowShape<D>::PL],
+                                int lane) {
+  constexpr int PL = RowShape<D>::PL;
+  const int base = lane * PL;
+  bool bad = false;
+#pragma unroll
+  for (int q = 0; q < PL; ++q) bad |= (base + q < D) && isnan(g[q]);
+  if (wave_any(bad)) return;
+  const size_t off = (size_t)f * D;
+  float m[PL], v[PL], th[PL];
+  load_slice<D>(c.mom + off, lane, m);
+  load_slice<D>(c.var + off, lane, v);
+  if (c.bf16) load_slice<D>(c.master + off, lane, th);
+  else load_slice<D>((const float*)c.table + off, lane, th);
+  const float lr = st->lr, bc1 = st->bc1, bc2 = st->bc2;
+#pragma unroll
+  for (int q = 0; q < PL; ++q) adam_math(th[q], m[q], v[q], g[q], lr, bc1, bc2);
+  store_slice<D>(c.mom + off, lane, m);
+  store_slice<D>(c.var + off, lane, v);
+  if (c.bf16) {
+    store_slice<D>(c.master + off, lane, th);
+    uint16_t* row = (uint16_t*)c.table + off;
+#pragma unroll
+    for (int q = 0; q < PL; ++q)
+      if (base + q < D) row[base + q] = f32_to_bf16_sr(th[q], sr_bits(c, st, f, (uint32_t)((base + q) >> 2), (base + q) & 3));
+  } else {
+    store_slice<D>((float*)c.table + off, lane, th);
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Fused segment reduce + Adam over the sorted keys (the batch merge of mod.rs:223-233
+// followed by the update).  A wave takes a 64-entry chunk and owns every segment that
+// starts in it.  Lane groups of L = D/4 lanes (4 values per lane) each take one segment
+// at a time, G = 64/L segments in parallel: the adjoint rows of the segment and the
+// row's theta/m/v are loaded together, then the Adam update is applied.  A chunk's last
+// segment that does not end within the next 64 entries is deferred as a long segment
+// (split into PIECE-entry pieces, combined in order).  grad_out != 0 writes the merged
+// gradient rows to G/ufeat instead of updating (test hook).
 // ------------------------------------------------------------------------------------
 template <int D>
-__global__ __launch_bounds__(256) void k_segreduce(StepConfig c, StepState* st, const uint32_t* __restrict__ Ks,
-                                                   const uint32_t* __restrict__ Ss) {
-  constexpr int PL = RowShape<D>::PL;
-  constexpr int U = 8;
-  const int lane = threadIdx.x & 63;
-  const uint32_t gw = blockIdx.x * NW + (threadIdx.x >> 6);
+__global__ __launch_bounds__(256) void k_seg_adam(StepConfig c, StepState* st, const uint32_t* __restrict__ Ks,
+                                                  const uint32_t* __restrict__ Ss) {
+  static_assert(D % 4 == 0 && D <= 256, "4-value lane slices");
+  constexpr int L = D / 4;        // lanes per row
+  constexpr int G = 64 / L;       // segments in flight per wave
+  constexpr int U = 8;            // adjoint rows in flight per group
+  __shared__ uint32_t s_pos[NW][66];
+  __shared__ unsigned long long s_cnt;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int grp = lane / L, lg = lane % L;
+  const bool gact = grp < G;
+  const uint32_t col = (uint32_t)lg * 4u;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const uint64_t gmask = (L == 64) ? ~0ull : (((1ull << L) - 1ull) << (grp * L));
+  const uint32_t gw = blockIdx.x * NW + w;
   const uint32_t nwaves = gridDim.x * NW;
   const uint32_t n = st->n_keys <= c.key_cap ? st->n_keys : 0u;
   const uint32_t SENT = c.sentinel;
+  const float lr = st->lr, bc1 = st->bc1, bc2 = st->bc2;
+  if (tid == 0) s_cnt = 0ull;
+  __syncthreads();
+  uint32_t processed = 0;
   for (uint64_t chunk = gw; chunk * 64u < n; chunk += nwaves) {
     const uint32_t i0 = (uint32_t)(chunk * 64u);
-    const uint32_t j = i0 + lane;
-    const uint32_t f = j < n ? Ks[j] : SENT;
-    if (__shfl(f, 0, EP_WAVE) == SENT) break;  // sorted: the rest is sentinel
-    uint32_t prev = __shfl_up(f, 1, EP_WAVE);
-    if (lane == 0) prev = i0 > 0u ? Ks[i0 - 1u] : ~f;
-    const bool is_start = (f != SENT) && (f != prev);
+    const uint32_t j0 = i0 + lane, j1 = i0 + 64u + lane;
+    const uint32_t k0 = j0 < n ? Ks[j0] : SENT;
+    const uint32_t k1 = j1 < n ? Ks[j1] : SENT;
+    const uint32_t s0 = j0 < n ? Ss[j0] : 0u;
+    const uint32_t s1 = j1 < n ? Ss[j1] : 0u;
+    if (__shfl(k0, 0, EP_WAVE) == SENT) break;  // sorted: the rest is sentinel
+    uint32_t prev = __shfl_up(k0, 1, EP_WAVE);
+    if (lane == 0) prev = i0 > 0u ? Ks[i0 - 1u] : ~k0;
+    const bool is_start = (k0 != SENT) && (k0 != prev);
     const uint64_t starts = __ballot(is_start);
     if (starts == 0ull) continue;
-    const int qf = __ffsll((unsigned long long)starts) - 1;
+    const uint32_t ns = (uint32_t)__popcll(starts);
+    if (is_start) s_pos[w][__popcll(starts & lt)] = (uint32_t)lane;
     const int ql = 63 - __clzll((unsigned long long)starts);
-    const uint32_t fl = __shfl(f, ql, EP_WAVE);
-    const uint64_t diff = __ballot(lane > ql && f != fl);
-    uint32_t end;
-    if (diff) {
-      end = i0 + (uint32_t)(__ffsll((unsigned long long)diff) - 1);
+    const uint32_t fl = __shfl(k0, ql, EP_WAVE);
+    const uint64_t m0 = __ballot(lane > ql && k0 != fl);
+    const uint64_t m1 = __ballot(k1 != fl);
+    uint32_t nproc = ns;
+    if (m0) {
+      if (lane == 0) s_pos[w][ns] = (uint32_t)(__ffsll((unsigned long long)m0) - 1);
+    } else if (m1) {
+      if (lane == 0) s_pos[w][ns] = 64u + (uint32_t)(__ffsll((unsigned long long)m1) - 1);
     } else {
+      // the last segment runs past the 128-entry window: defer it as a long segment
+      nproc = ns - 1u;
       const uint32_t st0 = i0 + (uint32_t)ql;
-      const uint64_t lim64 = (uint64_t)st0 + LONG_SEG + 1u;
-      const uint32_t lim = (uint32_t)(lim64 < n ? lim64 : n);
-      const uint32_t from = i0 + 64u < lim ? i0 + 64u : lim;
-      end = wave_upper_bound(Ks, from, lim, fl);
-      if (end - st0 > LONG_SEG) {
-        end = wave_upper_bound(Ks, lim, n, fl);
-        const uint32_t len = end - st0;
-        const uint32_t pcs = (len + PIECE - 1u) / PIECE;
-        uint32_t idx = 0, pb = 0;
-        if (lane == 0) {
-          idx = atomicAdd(&st->n_long, 1u);
-          pb = atomicAdd(&st->n_pieces, pcs);
-          if (idx < c.long_cap) {
-            c.long_st[idx] = st0;
-            c.long_len[idx] = len;
-            c.long_pbase[idx] = pb;
-          }
+      const uint32_t end = wave_upper_bound(Ks, i0 + 128u, n, fl);
+      const uint32_t len = end - st0;
+      const uint32_t pcs = (len + PIECE - 1u) / PIECE;
+      uint32_t idx = 0, pb = 0;
+      if (lane == 0) {
+        idx = atomicAdd(&st->n_long, 1u);
+        pb = atomicAdd(&st->n_pieces, pcs);
+        if (idx < c.long_cap) {
+          c.long_st[idx] = st0;
+          c.long_len[idx] = len;
+          c.long_pbase[idx] = pb;
         }
-        idx = __shfl(idx, 0, EP_WAVE);
-        pb = __shfl(pb, 0, EP_WAVE);
-        if (idx < c.long_cap)
-          for (uint32_t q = lane; q < pcs; q += 64u)
-            if (pb + q < c.piece_cap) c.piece_seg[pb + q] = idx;
-        end = st0;  // the flat loop stops before the deferred segment
       }
+      idx = __shfl(idx, 0, EP_WAVE);
+      pb = __shfl(pb, 0, EP_WAVE);
+      if (idx < c.long_cap)
+        for (uint32_t q = lane; q < pcs; q += 64u)
+          if (pb + q < c.piece_cap) c.piece_seg[pb + q] = idx;
     }
-    float acc[PL];
+    __builtin_amdgcn_wave_barrier();
+    processed += nproc;
+    for (uint32_t r0 = 0; r0 < nproc; r0 += (uint32_t)G) {
+      const uint32_t j = r0 + (uint32_t)grp;
+      const bool has = gact && j < nproc;
+      const uint32_t sa = has ? s_pos[w][j] : 0u;
+      const uint32_t sb = has ? s_pos[w][j + 1] : 0u;
+      const uint32_t f = __shfl(k0, (int)sa, EP_WAVE);
+      const uint32_t len = sb - sa;
+      // Adam operands first: independent of the reduction, so their latency overlaps it
+      float4 m4 = make_float4(0.f, 0.f, 0.f, 0.f), v4 = m4, t4 = m4;
+      const size_t roff = (size_t)f * D + col;
+      if (has && !c.grad_out) {
+        m4 = *reinterpret_cast<const float4*>(c.mom + roff);
+        v4 = *reinterpret_cast<const float4*>(c.var + roff);
+        t4 = c.bf16 ? *reinterpret_cast<const float4*>(c.master + roff)
+                    : *reinterpret_cast<const float4*>((const float*)c.table + roff);
+      }
+      uint32_t maxlen = len;
 #pragma unroll
-    for (int q = 0; q < PL; ++q) acc[q] = 0.f;
-    for (uint32_t g0 = i0 + (uint32_t)qf; g0 < end; g0 += 64u) {
-      const uint32_t jj = g0 + lane;
-      const bool valid = jj < end;
-      const uint32_t fj = valid ? Ks[jj] : SENT;
-      const uint32_t sj = valid ? Ss[jj] : 0u;
-      const float cj = valid ? c.coef[sj] : 0.f;
-      uint32_t fn = __shfl_down(fj, 1, EP_WAVE);
-      if (lane == 63) fn = (jj + 1u < end) ? Ks[jj + 1u] : SENT;
-      const bool is_end = valid && (fn != fj);
-      const uint64_t ends = __ballot(is_end);
-      uint32_t ub = 0;
-      if (lane == 0 && ends) ub = atomicAdd(&st->n_unique, (uint32_t)__popcll(ends));
-      ub = __shfl(ub, 0, EP_WAVE);
-      const int m = (int)((end - g0) < 64u ? (end - g0) : 64u);
-      for (int t0 = 0; t0 < m; t0 += U) {
-        float r[U][PL];
+      for (int off = 32; off > 0; off >>= 1) maxlen = max(maxlen, (uint32_t)__shfl_xor(maxlen, off, EP_WAVE));
+      float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+      for (uint32_t u0 = 0; u0 < maxlen; u0 += U) {
+        float4 r[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          const int t = (t0 + u < m) ? (t0 + u) : (m - 1);
-          const uint32_t s = __shfl(sj, t, EP_WAVE);
-          load_slice<D>(c.adj + (size_t)s * D, lane, r[u]);
+          const uint32_t t = sa + u0 + (uint32_t)u;  // chunk-relative entry, < 128
+          const bool valid = has && (u0 + (uint32_t)u < len);
+          const uint32_t src = t & 63u;
+          const uint32_t sl0 = __shfl(s0, (int)src, EP_WAVE);
+          const uint32_t sl1 = __shfl(s1, (int)src, EP_WAVE);
+          const uint32_t sl = t < 64u ? sl0 : sl1;
+          r[u] = valid ? *reinterpret_cast<const float4*>(c.adj + (size_t)sl * D + col) : make_float4(0.f, 0.f, 0.f, 0.f);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          const int t = t0 + u;
-          if (t < m) {
-            const float cc = __shfl(cj, t, EP_WAVE);
-#pragma unroll
-            for (int q = 0; q < PL; ++q) acc[q] += cc * r[u][q];
-            if ((ends >> t) & 1ull) {
-              const uint32_t idx = ub + (uint32_t)__popcll(ends & ((t == 0) ? 0ull : (~0ull >> (64 - t))));
-              store_slice<D>(c.G + (size_t)idx * D, lane, acc);
-              const uint32_t ft = __shfl(fj, t, EP_WAVE);
-              if (lane == 0) c.ufeat[idx] = ft;
-#pragma unroll
-              for (int q = 0; q < PL; ++q) acc[q] = 0.f;
-            }
-          }
+          acc.x += r[u].x; acc.y += r[u].y; acc.z += r[u].z; acc.w += r[u].w;
+        }
+      }
+      if (c.grad_out) {
+        uint32_t ub = 0;
+        const uint64_t heads = __ballot(has && lg == 0);
+        if (lane == 0 && heads) ub = atomicAdd(&st->n_unique, (uint32_t)__popcll(heads));
+        ub = __shfl(ub, 0, EP_WAVE);
+        const uint32_t u = ub + (uint32_t)__popcll(heads & ((grp * L == 0) ? 0ull : (~0ull >> (64 - grp * L))));
+        if (has) {
+          *reinterpret_cast<float4*>(c.G + (size_t)u * D + col) = acc;
+          if (lg == 0) c.ufeat[u] = f;
+        }
+        continue;
+      }
+      const bool bad = has && (isnan(acc.x) || isnan(acc.y) || isnan(acc.z) || isnan(acc.w));
+      const bool skip = (__ballot(bad) & gmask) != 0ull;
+      if (has && !skip) {
+        adam_math(t4.x, m4.x, v4.x, acc.x, lr, bc1, bc2);
+        adam_math(t4.y, m4.y, v4.y, acc.y, lr, bc1, bc2);
+        adam_math(t4.z, m4.z, v4.z, acc.z, lr, bc1, bc2);
+        adam_math(t4.w, m4.w, v4.w, acc.w, lr, bc1, bc2);
+        *reinterpret_cast<float4*>(c.mom + roff) = m4;
+        *reinterpret_cast<float4*>(c.var + roff) = v4;
+        if (c.bf16) {
+          *reinterpret_cast<float4*>(c.master + roff) = t4;
+          const u32x4 rb = philox4x32(u32x4{f, col >> 2, (uint32_t)st->next_step, RNG_ROUNDING},
+                                      (uint32_t)c.seed, (uint32_t)(c.seed >> 32));
+          uint2 pk;
+          pk.x = (uint32_t)f32_to_bf16_sr(t4.x, rb.x & 0xffffu) | ((uint32_t)f32_to_bf16_sr(t4.y, rb.x >> 16) << 16);
+          pk.y = (uint32_t)f32_to_bf16_sr(t4.z, rb.y & 0xffffu) | ((uint32_t)f32_to_bf16_sr(t4.w, rb.y >> 16) << 16);
+          *reinterpret_cast<uint2*>((uint16_t*)c.table + roff) = pk;
+        } else {
+          *reinterpret_cast<float4*>((float*)c.table + roff) = t4;
         }
       }
     }
   }
+  if (lane == 0 && processed) atomicAdd(&s_cnt, (unsigned long long)processed);
+  __syncthreads();
+  if (tid == 0 && s_cnt) atomicAdd((unsigned long long*)&st->stat_unique, s_cnt);
 }
 
 // Partial sums of the pieces of long segments.
 template <int D>
 __global__ __launch_bounds__(256) void k_pieces(StepConfig c, StepState* st, const uint32_t* __restrict__ Ss) {
   constexpr int PL = RowShape<D>::PL;
-  constexpr int U = 8;
+  constexpr int U = SR_UNROLL;
   const int lane = threadIdx.x & 63;
   const uint32_t gw = blockIdx.x * NW + (threadIdx.x >> 6);
   const uint32_t nwaves = gridDim.x * NW;
@@ -796,9 +1044,7 @@ __global__ __launch_bounds__(256) void k_pieces(StepConfig c, StepState* st, con
     for (int q = 0; q < PL; ++q) acc[q] = 0.f;
     for (uint32_t g0 = s0; g0 < s1; g0 += 64u) {
       const uint32_t jj = g0 + lane;
-      const bool valid = jj < s1;
-      const uint32_t sj = valid ? Ss[jj] : 0u;
-      const float cj = valid ? c.coef[sj] : 0.f;
+      const uint32_t sj = jj < s1 ? Ss[jj] : 0u;
       const int m = (int)((s1 - g0) < 64u ? (s1 - g0) : 64u);
       for (int t0 = 0; t0 < m; t0 += U) {
         float r[U][PL];
@@ -810,11 +1056,9 @@ __global__ __launch_bounds__(256) void k_pieces(StepConfig c, StepState* st, con
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          const int t = t0 + u;
-          if (t < m) {
-            const float cc = __shfl(cj, t, EP_WAVE);
+          if (t0 + u < m) {
 #pragma unroll
-            for (int q = 0; q < PL; ++q) acc[q] += cc * r[u][q];
+            for (int q = 0; q < PL; ++q) acc[q] += r[u][q];
           }
         }
       }
@@ -823,7 +1067,7 @@ __global__ __launch_bounds__(256) void k_pieces(StepConfig c, StepState* st, con
   }
 }
 
-// Combine the pieces of each long segment in piece order (deterministic).
+// Combine the pieces of each long segment in piece order (deterministic), then update.
 template <int D>
 __global__ __launch_bounds__(256) void k_long_combine(StepConfig c, StepState* st, const uint32_t* __restrict__ Ks) {
   constexpr int PL = RowShape<D>::PL;
@@ -849,89 +1093,28 @@ __global__ __launch_bounds__(256) void k_long_combine(StepConfig c, StepState* s
       if (base + q < D) red[w][base + q] = acc[q];
     __syncthreads();
     if (w == 0) {
-      float s[PL];
+      float g[PL];
 #pragma unroll
       for (int q = 0; q < PL; ++q)
-        s[q] = (base + q < D) ? (((red[0][base + q] + red[1][base + q]) + red[2][base + q]) + red[3][base + q]) : 0.f;
-      uint32_t u = 0;
-      if (lane == 0) u = atomicAdd(&st->n_unique, 1u);
-      u = __shfl(u, 0, EP_WAVE);
-      store_slice<D>(c.G + (size_t)u * D, lane, s);
-      if (lane == 0) c.ufeat[u] = Ks[c.long_st[idx]];
+        g[q] = (base + q < D) ? (((red[0][base + q] + red[1][base + q]) + red[2][base + q]) + red[3][base + q]) : 0.f;
+      const uint32_t f = Ks[c.long_st[idx]];
+      if (c.grad_out) {
+        uint32_t u = 0;
+        if (lane == 0) u = atomicAdd(&st->n_unique, 1u);
+        u = __shfl(u, 0, EP_WAVE);
+        store_slice<D>(c.G + (size_t)u * D, lane, g);
+        if (lane == 0) c.ufeat[u] = f;
+      } else {
+        adam_row<D>(c, st, f, g, lane);
+        if (lane == 0) atomicAdd((unsigned long long*)&st->stat_unique, 1ull);
+      }
     }
     __syncthreads();
   }
 }
 
 // ------------------------------------------------------------------------------------
-// Sparse lazy Adam on one row (optimizer.rs:43-81): skip if any NaN in g (line 56);
-// m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2; theta -= lr (m/bc1) / (sqrt(v/bc2) + eps).
-// bf16 mode: the FP32 master row is updated and written back to the bf16 table with
-// stochastic rounding.
-// ------------------------------------------------------------------------------------
-template <int D>
-__device__ inline void adam_row(const StepConfig& c, const StepState* st, uint32_t f, const float (&g)[RowShape<D>::PL],
-                                int lane) {
-  constexpr int PL = RowShape<D>::PL;
-  const int base = lane * PL;
-  bool bad = false;
-#pragma unroll
-  for (int q = 0; q < PL; ++q) bad |= (base + q < D) && isnan(g[q]);
-  if (wave_any(bad)) return;
-  const size_t off = (size_t)f * D;
-  float m[PL], v[PL], th[PL];
-  load_slice<D>(c.mom + off, lane, m);
-  load_slice<D>(c.var + off, lane, v);
-  if (c.bf16) load_slice<D>(c.master + off, lane, th);
-  else load_slice<D>((const float*)c.table + off, lane, th);
-  const float lr = st->lr, bc1 = st->bc1, bc2 = st->bc2;
-  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-#pragma unroll
-  for (int q = 0; q < PL; ++q) {
-    m[q] = b1 * m[q] + (1.0f - b1) * g[q];
-    v[q] = b2 * v[q] + (1.0f - b2) * (g[q] * g[q]);
-    const float mh = m[q] / bc1;
-    const float vh = v[q] / bc2;
-    th[q] -= lr * (mh / (sqrtf(vh) + eps));
-  }
-  store_slice<D>(c.mom + off, lane, m);
-  store_slice<D>(c.var + off, lane, v);
-  if (c.bf16) {
-    store_slice<D>(c.master + off, lane, th);
-    const u32x4 r = philox4x32(u32x4{f, (uint32_t)lane, (uint32_t)st->next_step, RNG_ROUNDING},
-                               (uint32_t)c.seed, (uint32_t)(c.seed >> 32));
-    const uint32_t rw[4] = {r.x, r.y, r.z, r.w};
-    uint16_t* row = (uint16_t*)c.table + off;
-#pragma unroll
-    for (int q = 0; q < PL; ++q) {
-      if (base + q < D) {
-        const uint32_t bits16 = (q & 1) ? (rw[(q >> 1) & 3] >> 16) : (rw[(q >> 1) & 3] & 0xffffu);
-        row[base + q] = f32_to_bf16_sr(th[q], bits16);
-      }
-    }
-  } else {
-    store_slice<D>((float*)c.table + off, lane, th);
-  }
-}
-
-template <int D>
-__global__ __launch_bounds__(256) void k_adam_unique(StepConfig c, StepState* st) {
-  constexpr int PL = RowShape<D>::PL;
-  const int lane = threadIdx.x & 63;
-  const uint32_t gw = blockIdx.x * NW + (threadIdx.x >> 6);
-  const uint32_t nwaves = gridDim.x * NW;
-  const uint32_t nu = st->n_unique;
-  if (gw == 0 && lane == 0) atomicAdd((unsigned long long*)&st->stat_unique, (unsigned long long)nu);
-  for (uint32_t u = gw; u < nu; u += nwaves) {
-    const uint32_t f = c.ufeat[u];
-    float g[PL];
-    load_slice<D>(c.G + (size_t)u * D, lane, g);
-    adam_row<D>(c, st, f, g, lane);
-  }
-}
-
-// ------------------------------------------------------------------------------------
-// Atomic fast path: scatter coef*adj straight into a dense FP32 gradient table with
+// Atomic fast path: scatter the (pre-scaled) adjoint rows straight into a dense FP32 gradient table with
 // float atomics (emission order, no sort), then Adam over the touched rows, which also
 // re-zeroes their gradient rows and flags.  Differs from the sorted path only in the
 // order of the FP32 additions.
@@ -954,7 +1137,6 @@ __global__ __launch_bounds__(256) void k_atomic_scatter(StepConfig c, StepState*
     const uint32_t f = valid ? c.keys[0][j] : SENT;
     const uint32_t s = valid ? c.slots[0][j] : 0u;
     const bool real = f != SENT;
-    const float cj = real ? c.coef[s] : 0.f;
     const bool newt = real && atomicCAS(&c.flag[f], 0u, 1u) == 0u;
     const uint64_t nm = __ballot(newt);
     uint32_t tb = 0;
@@ -976,12 +1158,11 @@ __global__ __launch_bounds__(256) void k_atomic_scatter(StepConfig c, StepState*
       for (int u = 0; u < U; ++u) {
         const int t = t0 + u;
         if (t < m && ((rm >> t) & 1ull)) {
-          const float cc = __shfl(cj, t, EP_WAVE);
           const uint32_t ft = __shfl(f, t, EP_WAVE);
           float* grow = c.Gd + (size_t)ft * D;
 #pragma unroll
           for (int q = 0; q < PL; ++q)
-            if (base + q < D) atomicAdd(grow + base + q, cc * r[u][q]);
+            if (base + q < D) atomicAdd(grow + base + q, r[u][q]);
         }
       }
     }
@@ -1109,7 +1290,7 @@ hipError_t launch_step_begin_lr(const StepConfig& c, StepState* st, float lr, hi
 }
 
 hipError_t launch_sample(const StepConfig& c, StepState* st, hipStream_t s) {
-  hipLaunchKernelGGL(k_sample, dim3((c.batch + 255) / 256), dim3(256), 0, s, c, st);
+  hipLaunchKernelGGL(k_sample, dim3((c.batch + NW - 1) / NW), dim3(256), (size_t)NW * c.S * sizeof(uint32_t), s, c, st);
   return hipGetLastError();
 }
 
@@ -1119,8 +1300,9 @@ hipError_t launch_scan(const StepConfig& c, StepState* st, hipStream_t s) {
 }
 
 size_t forward_lds_bytes(const StepConfig& c) {
-  const size_t NC = c.K + 2;
-  return ((NC + NW) * c.dims + (c.K + 1) + NC) * sizeof(float);
+  const uint32_t NC = c.K + 2;
+  return (size_t)forward_float_words(NC, c.K, c.dims) * 4 + (size_t)c.ent_cap * 8 + (size_t)(c.ent_cap + 1) * 4 +
+         (size_t)c.ids_cap * 4;
 }
 
 hipError_t launch_forward(const StepConfig& c, StepState* st, hipStream_t s) {
@@ -1138,34 +1320,53 @@ hipError_t launch_emit(const StepConfig& c, StepState* st, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_sort(const StepConfig& c, StepState* st, hipStream_t s, int* out_buf) {
-  int cur = 0;
+// Digit plan: passes = ceil(bits / 11), RB = ceil(bits / passes).
+static void radix_plan(int bits, int* passes, int* rb) {
+  *passes = (bits + 10) / 11;
+  *rb = (bits + *passes - 1) / *passes;
+}
+
+template <int RB>
+static void launch_sort_rb(const StepConfig& c, StepState* st, hipStream_t s, int passes, int* cur) {
   const int G = c.sort_grid;
-  for (int p = 0, shift = 0; shift < c.sort_bits; ++p, shift += 8) {
-    const int bits = (c.sort_bits - shift) < 8 ? (c.sort_bits - shift) : 8;
-    const uint32_t mask = (1u << bits) - 1u;
-    hipLaunchKernelGGL(k_rs_hist, dim3(G), dim3(256), 0, s, c.keys[cur], st, c.ghist, p, shift, mask, G);
-    hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(256), 0, s, c.ghist, st, p, G);
-    hipLaunchKernelGGL(k_rs_scatter, dim3(G), dim3(256), 0, s, c.keys[cur], c.slots[cur], c.keys[1 - cur],
-                       c.slots[1 - cur], st, c.ghist, shift, mask, bits, G);
-    cur = 1 - cur;
+  for (int p = 0; p < passes; ++p) {
+    const int shift = p * RB;
+    hipLaunchKernelGGL((k_rs_hist<RB>), dim3(G), dim3(256), 0, s, c.keys[*cur], st, c.ghist, p, shift, G);
+    hipLaunchKernelGGL((k_rs_scan<RB>), dim3(((1u << RB) + NW - 1u) / NW), dim3(256), 0, s, c.ghist, st, p, G);
+    hipLaunchKernelGGL((k_rs_scatter<RB>), dim3(G), dim3(256), 0, s, c.keys[*cur], c.slots[*cur], c.keys[1 - *cur],
+                       c.slots[1 - *cur], st, c.ghist, shift, G);
+    *cur = 1 - *cur;
+  }
+}
+
+hipError_t launch_sort(const StepConfig& c, StepState* st, hipStream_t s, int* out_buf) {
+  int passes, rb, cur = 0;
+  radix_plan(c.sort_bits, &passes, &rb);
+  switch (rb) {
+    case 1: launch_sort_rb<1>(c, st, s, passes, &cur); break;
+    case 2: launch_sort_rb<2>(c, st, s, passes, &cur); break;
+    case 3: launch_sort_rb<3>(c, st, s, passes, &cur); break;
+    case 4: launch_sort_rb<4>(c, st, s, passes, &cur); break;
+    case 5: launch_sort_rb<5>(c, st, s, passes, &cur); break;
+    case 6: launch_sort_rb<6>(c, st, s, passes, &cur); break;
+    case 7: launch_sort_rb<7>(c, st, s, passes, &cur); break;
+    case 8: launch_sort_rb<8>(c, st, s, passes, &cur); break;
+    case 9: launch_sort_rb<9>(c, st, s, passes, &cur); break;
+    case 10: launch_sort_rb<10>(c, st, s, passes, &cur); break;
+    case 11: launch_sort_rb<11>(c, st, s, passes, &cur); break;
+    default: return hipErrorInvalidValue;
   }
   *out_buf = cur;
   return hipGetLastError();
 }
 
-hipError_t launch_segreduce(const StepConfig& c, StepState* st, int buf, hipStream_t s) {
+hipError_t launch_scatter_sorted(const StepConfig& c, StepState* st, int buf, hipStream_t s) {
   const int grid = c.unique_grid;
   EP_DISPATCH(c.dims, {
-    hipLaunchKernelGGL((k_segreduce<D>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
+    hipLaunchKernelGGL((k_seg_adam<D>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
     hipLaunchKernelGGL((k_pieces<D>), dim3(grid), dim3(256), 0, s, c, st, c.slots[buf]);
     hipLaunchKernelGGL((k_long_combine<D>), dim3(256), dim3(256), 0, s, c, st, c.keys[buf]);
   });
-  return hipGetLastError();
-}
-
-hipError_t launch_adam_unique(const StepConfig& c, StepState* st, hipStream_t s) {
-  EP_DISPATCH(c.dims, { hipLaunchKernelGGL((k_adam_unique<D>), dim3(c.unique_grid), dim3(256), 0, s, c, st); });
   return hipGetLastError();
 }
 

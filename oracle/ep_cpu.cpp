@@ -1,0 +1,346 @@
+// CPU restatement of the reference's EP training loop, FP32, multithreaded.
+//
+// TEST INFRASTRUCTURE ONLY: the CPU baseline timed by bench.py and the CPU side of the
+// holdout-AUC comparison.  Never linked into the shipped library.
+//
+// Follows src/algos/ep/mod.rs:92-287 with the reference's parallel structure:
+//   * batches run concurrently on a thread pool (mod.rs:174, rayon par_iter over chunks);
+//   * per node: h(v), ~h(v), K negatives, margin loss, gradients (mod.rs:183-215);
+//   * per-node RNG XorShift seeded with seed + batch_idx + node_id (mod.rs:185);
+//   * ~h(v) neighbours by reservoir sampling over the whole row (utils.rs:128-145,
+//     model.rs:354-377): O(deg) like the reference;
+//   * first-leaf gradient extraction with a per-example hash map (mod.rs:329-378);
+//   * per-batch HashMap merge (mod.rs:223-233);
+//   * Hogwild sparse Adam, t = pass + 1, unsynchronised row writes (optimizer.rs:43-81);
+//   * cosine LR with 20% warm-up (scheduler.rs:39-47, mod.rs:132-137).
+// The mean aggregator with max_features = All and MarginLoss (loss.rs:59-73).
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+struct XorShift {  // xorshift128 (the reference uses rand_xorshift's XorShiftRng)
+  uint32_t x, y, z, w;
+  explicit XorShift(uint64_t seed) {
+    uint64_t s = seed;
+    auto mix = [&]() {
+      uint64_t q = (s += 0x9E3779B97F4A7C15ull);
+      q = (q ^ (q >> 30)) * 0xBF58476D1CE4E5B9ull;
+      q = (q ^ (q >> 27)) * 0x94D049BB133111EBull;
+      return q ^ (q >> 31);
+    };
+    uint64_t a = mix(), b = mix();
+    x = (uint32_t)a; y = (uint32_t)(a >> 32); z = (uint32_t)b; w = (uint32_t)(b >> 32) | 1u;
+  }
+  uint32_t next() {
+    uint32_t t = x ^ (x << 11);
+    x = y; y = z; z = w;
+    w = w ^ (w >> 19) ^ (t ^ (t >> 8));
+    return w;
+  }
+  uint32_t below(uint32_t n) { return (uint32_t)(((uint64_t)next() * n) >> 32); }
+  float uniform() { return (float)(next() >> 8) * (1.0f / 16777216.0f); }
+};
+
+struct Args {
+  std::string dir;
+  uint64_t nodes = 0;
+  uint32_t vocab = 0, dims = 64, batch = 128, negatives = 5, max_nodes = 10, threads = 1, passes = 1;
+  float gamma = 0.5f, alpha = 0.01f, budget_s = 0.f;
+  uint64_t seed = 20222022ull;
+  bool json = false;
+  std::string table_out, train_file;
+};
+
+template <typename T>
+std::vector<T> read_bin(const std::string& path) {
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) { std::fprintf(stderr, "cannot open %s\n", path.c_str()); std::exit(2); }
+  std::fseek(f, 0, SEEK_END);
+  const long n = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  std::vector<T> v(n / sizeof(T));
+  if (n && std::fread(v.data(), 1, n, f) != (size_t)n) { std::fprintf(stderr, "short read %s\n", path.c_str()); std::exit(2); }
+  std::fclose(f);
+  return v;
+}
+
+struct Model {
+  const Args& a;
+  const std::vector<uint64_t>& indptr;
+  const std::vector<uint32_t>& indices;
+  const std::vector<uint64_t>& findptr;
+  const std::vector<uint32_t>& fids;
+  std::vector<float> table, mom, var;
+  std::vector<uint32_t> train;
+
+  // NodeCounts: feature -> count in first-seen order (model.rs:254-278)
+  struct Cons {
+    std::vector<uint32_t> feats;
+    std::vector<float> counts;
+    std::unordered_map<uint32_t, uint32_t> pos;
+    void add(uint32_t f, float w) {
+      auto it = pos.find(f);
+      if (it == pos.end()) { pos.emplace(f, (uint32_t)feats.size()); feats.push_back(f); counts.push_back(w); }
+      else counts[it->second] += w;
+    }
+  };
+
+  void collect(Cons& c, uint32_t node, float weight) const {
+    // Sample::All for features: scale 1; weight += scale (model.rs:267-268)
+    const float w = weight + 1.0f;
+    for (uint64_t q = findptr[node]; q < findptr[node + 1]; ++q) c.add(fids[q], w);
+  }
+
+  void mean(const Cons& c, float* out) const {  // mean_embeddings (model.rs:448-463)
+    const uint32_t D = a.dims;
+    std::fill(out, out + D, 0.f);
+    float n = 0.f;
+    for (size_t i = 0; i < c.feats.size(); ++i) {
+      const float* e = &table[(size_t)c.feats[i] * D];
+      const float cnt = c.counts[i];
+      for (uint32_t d = 0; d < D; ++d) out[d] += e[d] * cnt;
+      n += cnt;
+    }
+    for (uint32_t d = 0; d < D; ++d) out[d] /= n;
+  }
+
+  // One example; returns loss (<= 0 means no gradient) and fills grads (first-leaf rule).
+  float example(uint32_t v, XorShift& rng, std::unordered_map<uint32_t, std::vector<float>>& grads) const {
+    const uint32_t D = a.dims, K = a.negatives;
+    const uint64_t s0 = indptr[v], s1 = indptr[v + 1];
+    const uint32_t deg = (uint32_t)(s1 - s0);
+    if (deg == 0) return 0.f;
+    std::vector<Cons> cons(2 + K);
+    collect(cons[0], v, 1.0f);
+    // ~h(v): Fixed(S).sample(deg) = (min(S,deg), deg/r); reservoir when deg > S (utils.rs:128-145)
+    const uint32_t S = a.max_nodes;
+    const uint32_t r = std::min(S, deg);
+    const float scale = (float)deg / (float)r;
+    if (deg <= S) {
+      for (uint64_t j = s0; j < s1; ++j) collect(cons[1], indices[j], scale);
+    } else {
+      std::vector<uint32_t> res(indices.begin() + s0, indices.begin() + s0 + S);
+      for (uint32_t i = S; i < deg; ++i) {
+        const uint32_t idx = rng.below(i);
+        if (idx < S) res[idx] = indices[s0 + i];
+      }
+      for (uint32_t u : res) collect(cons[1], u, scale);
+    }
+    for (uint32_t k = 0; k < K; ++k) collect(cons[2 + k], train[rng.below((uint32_t)train.size())], 1.0f);
+    std::vector<float> x((2 + K) * (size_t)D);
+    for (uint32_t c = 0; c < 2 + K; ++c) mean(cons[c], &x[(size_t)c * D]);
+    const float* av = &x[0];
+    const float* pv = &x[D];
+    auto dist = [&](const float* p, const float* q) {
+      float s = 0.f;
+      for (uint32_t d = 0; d < D; ++d) { const float t = p[d] - q[d]; s += t * t; }
+      return std::sqrt(s);
+    };
+    const float D0 = dist(pv, av);
+    const float d1 = a.gamma + D0;
+    std::vector<float> Di(K);
+    std::vector<char> act(K, 0);
+    float sum = 0.f;
+    uint32_t nA = 0;
+    for (uint32_t k = 0; k < K; ++k) {
+      Di[k] = dist(pv, &x[(size_t)(2 + k) * D]);
+      const float l = d1 - Di[k];
+      if (l > 0.f) { act[k] = 1; sum += l; ++nA; }
+    }
+    if (nA == 0) return 0.f;
+    const float L = sum / (float)nA;
+    if (!(L > 0.f)) return 0.f;
+    // adjoints
+    std::vector<float> g((2 + K) * (size_t)D, 0.f);
+    std::vector<char> elig(2 + K, 0);
+    elig[0] = elig[1] = 1;
+    for (uint32_t d = 0; d < D; ++d) {
+      g[d] = (av[d] - pv[d]) / D0;
+      g[D + d] = (pv[d] - av[d]) / D0;
+    }
+    for (uint32_t k = 0; k < K; ++k) {
+      if (!act[k]) continue;
+      elig[2 + k] = 1;
+      const float* n = &x[(size_t)(2 + k) * D];
+      for (uint32_t d = 0; d < D; ++d) {
+        const float t = (pv[d] - n[d]) / Di[k] / (float)nA;
+        g[D + d] -= t;
+        g[(size_t)(2 + k) * D + d] = t;
+      }
+    }
+    // first-leaf extraction (mod.rs:347-378)
+    for (uint32_t c = 0; c < 2 + K; ++c) {
+      if (!elig[c]) continue;
+      const float* gc = &g[(size_t)c * D];
+      bool finite = true;
+      for (uint32_t d = 0; d < D; ++d) finite &= std::isfinite(gc[d]);
+      if (!finite) continue;
+      float C = 0.f;
+      for (float q : cons[c].counts) C += q;
+      for (size_t i = 0; i < cons[c].feats.size(); ++i) {
+        const uint32_t f = cons[c].feats[i];
+        if (grads.count(f)) continue;
+        std::vector<float> gr(D);
+        const float w = cons[c].counts[i] / C;
+        for (uint32_t d = 0; d < D; ++d) gr[d] = w * gc[d];
+        grads.emplace(f, std::move(gr));
+      }
+    }
+    return L;
+  }
+
+  void adam(const std::unordered_map<uint32_t, std::vector<float>>& all, float lr, float pass) {
+    const uint32_t D = a.dims;
+    const float t = pass + 1.f, b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float bc1 = 1.f - std::pow(b1, t), bc2 = 1.f - std::pow(b2, t);
+    for (const auto& kv : all) {
+      const std::vector<float>& g = kv.second;
+      bool nan = false;
+      for (float q : g) nan |= std::isnan(q);
+      if (nan) continue;
+      float* m = &mom[(size_t)kv.first * D];
+      float* v = &var[(size_t)kv.first * D];
+      float* e = &table[(size_t)kv.first * D];
+      for (uint32_t d = 0; d < D; ++d) {
+        m[d] = b1 * m[d] + (1.f - b1) * g[d];
+        v[d] = b2 * v[d] + (1.f - b2) * g[d] * g[d];
+        e[d] -= lr * ((m[d] / bc1) / (std::sqrt(v[d] / bc2) + eps));
+      }
+    }
+  }
+};
+
+float cos_decay(uint64_t step, float alpha, uint64_t total) {
+  const uint64_t warm = (uint64_t)((float)total / 5.f);
+  const float mn = alpha / 100.f;
+  if (step > warm) return mn + 0.5f * (alpha - mn) * (1.f + std::cos(3.14159265f * (float)step / (float)total));
+  return mn + ((float)step / (float)warm) * (alpha - mn);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  Args a;
+  for (int i = 1; i < argc; ++i) {
+    std::string k = argv[i];
+    auto nx = [&]() { return std::string(argv[++i]); };
+    if (k == "--dir") a.dir = nx();
+    else if (k == "--nodes") a.nodes = std::stoull(nx());
+    else if (k == "--vocab") a.vocab = (uint32_t)std::stoul(nx());
+    else if (k == "--dims") a.dims = (uint32_t)std::stoul(nx());
+    else if (k == "--batch") a.batch = (uint32_t)std::stoul(nx());
+    else if (k == "--negatives") a.negatives = (uint32_t)std::stoul(nx());
+    else if (k == "--max-nodes") a.max_nodes = (uint32_t)std::stoul(nx());
+    else if (k == "--gamma") a.gamma = std::stof(nx());
+    else if (k == "--alpha") a.alpha = std::stof(nx());
+    else if (k == "--threads") a.threads = (uint32_t)std::stoul(nx());
+    else if (k == "--passes") a.passes = (uint32_t)std::stoul(nx());
+    else if (k == "--budget-s") a.budget_s = std::stof(nx());
+    else if (k == "--seed") a.seed = std::stoull(nx());
+    else if (k == "--table-out") a.table_out = nx();
+    else if (k == "--train") a.train_file = nx();
+    else if (k == "--json") a.json = true;
+    else { std::fprintf(stderr, "unknown argument %s\n", k.c_str()); return 2; }
+  }
+  const auto indptr = read_bin<uint64_t>(a.dir + "/indptr.bin");
+  const auto indices = read_bin<uint32_t>(a.dir + "/indices.bin");
+  const auto findptr = read_bin<uint64_t>(a.dir + "/findptr.bin");
+  const auto fids = read_bin<uint32_t>(a.dir + "/fids.bin");
+  if (indptr.size() != a.nodes + 1 || findptr.size() != a.nodes + 1) { std::fprintf(stderr, "size mismatch\n"); return 2; }
+  Model m{a, indptr, indices, findptr, fids, {}, {}, {}, {}};
+  const uint32_t D = a.dims;
+  // randomize_embedding_store (mod.rs:381-392)
+  m.table.resize((size_t)a.vocab * D);
+  m.mom.assign(m.table.size(), 0.f);
+  m.var.assign(m.table.size(), 0.f);
+  XorShift init(a.seed);
+  for (uint32_t r = 0; r < a.vocab; ++r) {
+    float* e = &m.table[(size_t)r * D];
+    float n = 0.f;
+    for (uint32_t d = 0; d < D; ++d) { e[d] = 2.f * init.uniform() - 1.f; n += e[d] * e[d]; }
+    n = std::sqrt(n);
+    for (uint32_t d = 0; d < D; ++d) e[d] /= n;
+  }
+  if (!a.train_file.empty()) m.train = read_bin<uint32_t>(a.train_file);
+  else { m.train.resize(a.nodes); for (uint64_t i = 0; i < a.nodes; ++i) m.train[i] = (uint32_t)i; }
+  std::vector<uint32_t> order = m.train;
+  const uint64_t steps_per_pass = (order.size() + a.batch - 1) / a.batch;
+  const uint64_t total = steps_per_pass * a.passes;
+  std::atomic<uint64_t> step{1};
+  std::atomic<uint64_t> examples{0}, batches_done{0};
+  XorShift shuf(a.seed ^ 0xABCDEF);
+  const auto t0 = std::chrono::steady_clock::now();
+  bool stop = false;
+  double err_sum = 0.0;
+  for (uint32_t pass = 1; pass <= a.passes && !stop; ++pass) {
+    for (size_t i = order.size(); i > 1; --i) std::swap(order[i - 1], order[shuf.below((uint32_t)i)]);
+    std::atomic<uint64_t> next_batch{0};
+    std::atomic<bool> timeout{false};
+    std::vector<double> errs(steps_per_pass, 0.0);
+    auto worker = [&]() {
+      for (;;) {
+        if (timeout.load()) return;
+        const uint64_t bi = next_batch.fetch_add(1);
+        if (bi >= steps_per_pass) return;
+        const size_t lo = bi * a.batch, hi = std::min(order.size(), lo + a.batch);
+        std::unordered_map<uint32_t, std::vector<float>> all;
+        double err = 0.0;
+        uint32_t cnt = 0;
+        for (size_t q = lo; q < hi; ++q) {
+          const uint32_t v = order[q];
+          XorShift rng(a.seed + bi + v);
+          std::unordered_map<uint32_t, std::vector<float>> g;
+          const float L = m.example(v, rng, g);
+          if (L > 0.f) {
+            ++cnt;
+            err += L;
+            for (auto& kv : g) {
+              auto it = all.find(kv.first);
+              if (it == all.end()) all.emplace(kv.first, std::move(kv.second));
+              else for (uint32_t d = 0; d < D; ++d) it->second[d] += kv.second[d];
+            }
+          }
+        }
+        const uint64_t cur = step.fetch_add(1);
+        if (cnt > 0) m.adam(all, cos_decay(cur, a.alpha, total), (float)pass);
+        errs[bi] = cnt ? err / (double)(hi - lo) : 0.0;
+        examples.fetch_add(hi - lo);
+        batches_done.fetch_add(1);
+        if (a.budget_s > 0) {
+          const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+          if (el > a.budget_s) timeout.store(true);
+        }
+      }
+    };
+    std::vector<std::thread> ts;
+    for (uint32_t t = 0; t < a.threads; ++t) ts.emplace_back(worker);
+    for (auto& t : ts) t.join();
+    if (timeout.load()) stop = true;
+    double e = 0.0;
+    for (double x : errs) e += x;
+    err_sum = e / (double)steps_per_pass;
+    if (!a.json) std::fprintf(stderr, "pass %u train error %.5f\n", pass, err_sum);
+  }
+  const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (!a.table_out.empty()) {
+    FILE* f = std::fopen(a.table_out.c_str(), "wb");
+    std::fwrite(m.table.data(), sizeof(float), m.table.size(), f);
+    std::fclose(f);
+  }
+  std::printf("{\"examples\": %llu, \"batches\": %llu, \"seconds\": %.4f, \"examples_per_s\": %.2f, "
+              "\"threads\": %u, \"train_error\": %.6f}\n",
+              (unsigned long long)examples.load(), (unsigned long long)batches_done.load(), el,
+              (double)examples.load() / el, a.threads, err_sum);
+  return 0;
+}
