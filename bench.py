@@ -122,6 +122,8 @@ def main():
     ap.add_argument("--forms", default="fp32-plain,bf16-batched")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (HBM-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -129,6 +131,19 @@ def main():
         sys.exit(2)
 
     from cloverleaf_amd import synth
+
+    if args.copy_probe_mib:
+        import torch
+
+        n = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] copy probe: 3 x {args.copy_probe_mib} MiB device copies (elementwise copy kernel)",
+              file=sys.stderr, flush=True)
+        del src, dst
 
     t0 = time.time()
     w = synth.make_workload(args.workload, scale=args.scale)

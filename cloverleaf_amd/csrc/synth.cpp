@@ -207,9 +207,67 @@ int synth_bipartite(uint64_t nq, uint64_t ni, uint64_t m, double beta, double dm
   return 0;
 }
 
+// Community variant (quality workloads): node i is in community i % C.  Each edge picks
+// u ~ w; with probability p_in the other endpoint is drawn ~ w within u's community,
+// otherwise ~ w over all nodes.  Undirected.
+int synth_chung_lu_communities(uint64_t n, uint64_t m, double beta, double dmax, uint32_t C, double p_in,
+                               uint64_t seed, uint64_t** indptr, uint32_t** indices, uint64_t* n_entries) {
+  if (n == 0 || n >= (1ull << 32) || beta <= 1.0 || C == 0) return -1;
+  const double alpha = 1.0 / (beta - 1.0);
+  const double i0 = solve_i0(n, alpha, 2.0 * (double)m, dmax);
+  const std::vector<double> cdf = powerlaw_cdf(n, alpha, i0);
+  // per-community CDFs over members i = c, c + C, ...
+  std::vector<std::vector<double>> ccdf(C);
+  for (uint32_t c = 0; c < C; ++c) {
+    double acc = 0.0;
+    for (uint64_t i = c; i < n; i += C) { acc += std::pow((double)i + i0, -alpha); ccdf[c].push_back(acc); }
+    for (auto& x : ccdf[c]) x /= acc;
+    if (!ccdf[c].empty()) ccdf[c].back() = 1.0;
+  }
+  const uint64_t CH = 1 << 20;
+  std::vector<Edge> edges(2 * m);
+  std::vector<uint8_t> keep(m, 1);
+  parallel_chunks((m + CH - 1) / CH, [&](uint64_t ch) {
+    Rng r(stream_seed(seed, 5, ch));
+    const uint64_t lo = ch * CH, hi = std::min(m, lo + CH);
+    for (uint64_t k = lo; k < hi; ++k) {
+      const uint32_t a = draw_cdf(cdf, r.uniform());
+      uint32_t b;
+      if (r.uniform() < p_in) {
+        const uint32_t c = a % C;
+        b = c + C * draw_cdf(ccdf[c], r.uniform());
+      } else {
+        b = draw_cdf(cdf, r.uniform());
+      }
+      if (a == b) keep[k] = 0;
+      edges[2 * k] = Edge{a, b};
+      edges[2 * k + 1] = Edge{b, a};
+    }
+  });
+  size_t wi = 0;
+  for (uint64_t k = 0; k < m; ++k) {
+    if (!keep[k]) continue;
+    edges[wi++] = edges[2 * k];
+    edges[wi++] = edges[2 * k + 1];
+  }
+  edges.resize(wi);
+  build_csr(n, edges, indptr, indices, n_entries);
+  return 0;
+}
+
 // Feature lists.  count_kind: 0 -> 1 + Poisson(p1); 1 -> U{p1..p2}; 2 -> log-uniform [1, p2].
+// C > 0: node i draws Zipf ranks shifted by its community's offset (i % C), so each
+// community has its own hot features.
+int synth_features_ex(uint64_t n, int count_kind, double p1, double p2, uint32_t vocab, double zipf_s, uint64_t seed,
+                      uint32_t C, uint64_t** f_indptr, uint32_t** f_ids);
+
 int synth_features(uint64_t n, int count_kind, double p1, double p2, uint32_t vocab, double zipf_s, uint64_t seed,
                    uint64_t** f_indptr, uint32_t** f_ids) {
+  return synth_features_ex(n, count_kind, p1, p2, vocab, zipf_s, seed, 0, f_indptr, f_ids);
+}
+
+int synth_features_ex(uint64_t n, int count_kind, double p1, double p2, uint32_t vocab, double zipf_s, uint64_t seed,
+                      uint32_t C, uint64_t** f_indptr, uint32_t** f_ids) {
   if (n == 0 || vocab == 0) return -1;
   const uint64_t CH = 1 << 16;
   const uint64_t nch = (n + CH - 1) / CH;
@@ -244,8 +302,10 @@ int synth_features(uint64_t n, int count_kind, double p1, double p2, uint32_t vo
   parallel_chunks(nch, [&](uint64_t c) {
     Rng r(stream_seed(seed, 4, c));
     const uint64_t lo = c * CH, hi = std::min(n, lo + CH);
-    for (uint64_t i = lo; i < hi; ++i)
-      for (uint64_t q = ip[i]; q < ip[i + 1]; ++q) ids[q] = draw_cdf(cdf, r.uniform());
+    for (uint64_t i = lo; i < hi; ++i) {
+      const uint64_t off = C ? (stream_seed(seed, 6, i % C) % vocab) : 0;
+      for (uint64_t q = ip[i]; q < ip[i + 1]; ++q) ids[q] = (uint32_t)((draw_cdf(cdf, r.uniform()) + off) % vocab);
+    }
   });
   *f_indptr = ip;
   *f_ids = ids;

@@ -34,6 +34,12 @@ def _load():
         lib.synth_features.argtypes = [c_uint64, c_int, c_double, c_double, c_uint32, c_double, c_uint64, u64pp,
                                        u32pp]
         lib.synth_features.restype = c_int
+        lib.synth_chung_lu_communities.argtypes = [c_uint64, c_uint64, c_double, c_double, c_uint32, c_double,
+                                                   c_uint64, u64pp, u32pp, POINTER(c_uint64)]
+        lib.synth_chung_lu_communities.restype = c_int
+        lib.synth_features_ex.argtypes = [c_uint64, c_int, c_double, c_double, c_uint32, c_double, c_uint64,
+                                          c_uint32, u64pp, u32pp]
+        lib.synth_features_ex.restype = c_int
         _lib = lib
     return _lib
 
@@ -67,10 +73,24 @@ def bipartite(nq: int, ni: int, m: int, beta: float, dmax: float = 0.0, seed: in
 COUNT_POISSON, COUNT_UNIFORM, COUNT_LOGUNIFORM = 0, 1, 2
 
 
-def features(n: int, count_kind: int, p1: float, p2: float, vocab: int, zipf_s: float, seed: int = 2):
-    """Feature CSR (f_indptr u64[n+1], f_ids u32) with Zipf(zipf_s) ids over `vocab`."""
+def chung_lu_communities(n: int, m: int, beta: float, dmax: float, communities: int, p_in: float, seed: int = 1):
+    """Undirected Chung-Lu graph where a fraction p_in of edges stays inside community i % C."""
+    ip, ix, ne = POINTER(c_uint64)(), POINTER(c_uint32)(), c_uint64(0)
+    rc = _load().synth_chung_lu_communities(n, m, beta, dmax, communities, p_in, seed, ctypes.byref(ip),
+                                            ctypes.byref(ix), ctypes.byref(ne))
+    if rc != 0:
+        raise ValueError("synth_chung_lu_communities: bad arguments")
+    return _take(ip, n + 1, np.uint64), _take(ix, max(ne.value, 1), np.uint32)[:ne.value]
+
+
+def features(n: int, count_kind: int, p1: float, p2: float, vocab: int, zipf_s: float, seed: int = 2,
+             communities: int = 0):
+    """Feature CSR (f_indptr u64[n+1], f_ids u32) with Zipf(zipf_s) ids over `vocab`.
+
+    communities > 0 rotates each community's (i % C) Zipf ranks by its own offset."""
     ip, ix = POINTER(c_uint64)(), POINTER(c_uint32)()
-    rc = _load().synth_features(n, count_kind, p1, p2, vocab, zipf_s, seed, ctypes.byref(ip), ctypes.byref(ix))
+    rc = _load().synth_features_ex(n, count_kind, p1, p2, vocab, zipf_s, seed, communities, ctypes.byref(ip),
+                                   ctypes.byref(ix))
     if rc != 0:
         raise ValueError("synth_features: bad arguments")
     f_indptr = _take(ip, n + 1, np.uint64)
@@ -110,6 +130,14 @@ WORKLOADS = {
         graph=("chung_lu", dict(n=10_000, m=100_000, beta=2.5, dmax=1_000, directed=False)),
         feats=dict(count_kind=COUNT_POISSON, p1=7.0, p2=0.0, vocab=50_000, zipf_s=1.1),
         dims=64, batch=128, negatives=5, max_nodes=10, gamma=0.5, alpha=0.01, passes=5),
+    # quality variant of the 10k config: same sizes, 50 communities (80% of edges and a
+    # community-specific rotation of the Zipf feature ranks), so held-out links are learnable
+    "ep-cfg1-10k-comm": dict(
+        description="cfg1 sizes with 50 communities: 80% intra-community edges, per-community Zipf rank rotation; "
+                    "d=64, S=10, K=5, margin 0.5, B=128, lr 0.01, 5 passes",
+        graph=("communities", dict(n=10_000, m=100_000, beta=2.5, dmax=1_000, communities=50, p_in=0.8)),
+        feats=dict(count_kind=COUNT_POISSON, p1=7.0, p2=0.0, vocab=50_000, zipf_s=1.1),
+        dims=64, batch=128, negatives=5, max_nodes=10, gamma=0.5, alpha=0.01, passes=5),
     "ep-cfg2-1m": dict(
         description="bipartite query(300k)-item(700k) Chung-Lu, 20M undirected edges, beta=2.2; features U{1..32}, "
                     "Zipf(1.1) over 1M; d=128, B=1024, K=10, S=20, margin 0.5, lr 0.01",
@@ -130,7 +158,14 @@ def make_workload(name: str, scale: float = 1.0, seed_graph: int = 1, seed_feat:
     spec = WORKLOADS[name]
     kind, g = spec["graph"]
     g = dict(g)
-    if kind == "chung_lu":
+    comm = 0
+    if kind == "communities":
+        n = max(16, int(g["n"] * scale))
+        m = max(16, int(g["m"] * scale))
+        comm = g["communities"]
+        indptr, indices = chung_lu_communities(n, m, g["beta"], g["dmax"] * (scale if scale < 1 else 1), comm,
+                                               g["p_in"], seed_graph)
+    elif kind == "chung_lu":
         n = max(16, int(g["n"] * scale))
         m = max(16, int(g["m"] * scale))
         indptr, indices = chung_lu(n, m, g["beta"], g["dmax"] * (scale if scale < 1 else 1), g["directed"], seed_graph)
@@ -142,7 +177,7 @@ def make_workload(name: str, scale: float = 1.0, seed_graph: int = 1, seed_feat:
         n = nq + ni
     f = dict(spec["feats"])
     vocab = max(16, int(f["vocab"] * scale))
-    f_indptr, f_ids = features(n, f["count_kind"], f["p1"], f["p2"], vocab, f["zipf_s"], seed_feat)
+    f_indptr, f_ids = features(n, f["count_kind"], f["p1"], f["p2"], vocab, f["zipf_s"], seed_feat, comm)
     return Workload(name=name, description=spec["description"], indptr=indptr, indices=indices, f_indptr=f_indptr,
                     f_ids=f_ids, vocab=vocab, dims=spec["dims"], batch=spec["batch"], negatives=spec["negatives"],
                     max_nodes=spec["max_nodes"], gamma=spec["gamma"], alpha=spec["alpha"], passes=spec["passes"])
