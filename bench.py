@@ -58,6 +58,8 @@ def run_form(w, form, steps, warmup, scatter=None):
     elif form == "bf16-batched":
         cfg.precision, cfg.use_graph = N.EP_PREC_BF16, 1
         cfg.scatter = N.EP_SCATTER_ATOMIC if scatter == "atomic" else N.EP_SCATTER_SORT
+    elif form == "bf16-plain":
+        cfg.precision, cfg.use_graph, cfg.scatter = N.EP_PREC_BF16, 0, N.EP_SCATTER_SORT
     else:
         raise ValueError(form)
     with N.Context(cfg) as ctx:
@@ -73,6 +75,7 @@ def run_form(w, form, steps, warmup, scatter=None):
     return {
         "form": form,
         "scatter": scatter or "sort",
+        "seg_vpl": int(r.seg_vpl),
         "examples_per_s": r.examples / (r.ms_total / 1e3),
         "ms_per_step": r.ms_total / r.steps,
         "examples": int(r.examples),
@@ -121,6 +124,7 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--forms", default="fp32-plain,bf16-batched")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--scatters", default="sort,atomic", help="scatter paths tried for the batched forms")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (HBM-counter calibration)")
@@ -153,12 +157,12 @@ def main():
     results = {}
     for form in args.forms.split(","):
         best = None
-        scatters = ["sort"] if form == "fp32-plain" else ["sort", "atomic"]
+        scatters = ["sort"] if form.endswith("-plain") else [s for s in args.scatters.split(",")]
         for sc in scatters:
             r = run_form(w, form, args.steps, args.warmup, scatter=sc)
             print(f"[bench] {form}/{sc}: {r['examples_per_s']:.0f} ex/s, {r['ms_per_step']:.3f} ms/step, "
                   f"fwd {r['forward_ms_avg']:.3f} ms ({r['forward_gbs']:.0f} GB/s), "
-                  f"scatter+update {r['scatter_update_ms_avg']:.3f} ms", file=sys.stderr, flush=True)
+                  f"scatter+update {r['scatter_update_ms_avg']:.3f} ms (seg_vpl {r['seg_vpl']})", file=sys.stderr, flush=True)
             if best is None or r["examples_per_s"] > best["examples_per_s"]:
                 best = r
         results[form] = best

@@ -50,6 +50,7 @@ class EPConfig(Structure):
         ("use_graph", c_int32),
         ("seed", c_uint64),
         ("device", c_int32),
+        ("seg_vpl", c_int32),
     ]
 
 
@@ -64,6 +65,7 @@ class EPBenchResult(Structure):
         ("unique_rows", c_uint64),
         ("steps", c_uint32),
         ("forward_launches", c_uint32),
+        ("seg_vpl", c_int32),
     ]
 
 

@@ -74,9 +74,7 @@ struct ep_ctx {
 
   // parameters
   void* d_table = nullptr;
-  float* d_master = nullptr;
-  float* d_mom = nullptr;
-  float* d_var = nullptr;
+  float* d_ostate = nullptr;  // optimizer records: [master |] m | v per row
   bool table_ready = false;
 
   // split
@@ -103,7 +101,15 @@ struct ep_ctx {
   hipGraphExec_t step_exec = nullptr;
   bool step_exec_ready = false;
 
+  // segment-Adam variant tuner: the first training steps rotate through the supported
+  // values-per-lane variants (identical results) and time the scatter with events
+  int seg_vpl = 0;                 // chosen variant, 0 while tuning
+  std::vector<int> tune_cands;
+  std::vector<hipEvent_t> tune_ev;  // two per tuning step
+  uint32_t tune_i = 0;
+
   ~ep_ctx() {
+    for (hipEvent_t e : tune_ev) (void)hipEventDestroy(e);
     if (step_exec) (void)hipGraphExecDestroy(step_exec);
     for (void* p : allocs) (void)hipFree(p);
     if (h_perm) (void)hipHostFree(h_perm);
@@ -227,6 +233,8 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   b.hash_entries = (uint32_t)hash;
   b.ent_cap = (uint32_t)std::min<uint64_t>(256, 1ull + neighbour_cap(c) + K);
   b.ids_cap = (uint32_t)std::min<uint64_t>(8192, example_rows_max(c));
+  // fuse emission into the forward while the first-leaf hash is small (<= 16 KiB of LDS)
+  b.fuse_emit = (cfg.grad_mode == EP_GRAD_EXACT || hash <= 4096) ? 1 : 0;
   b.unique_grid = 2048;
   c->cap_batch = batch;
   c->cap_S = S;
@@ -249,9 +257,14 @@ void refresh_base(ep_ctx* c) {
   b.sort_bits = bits;
   b.sentinel = (uint32_t)((1ull << bits) - 1ull);
   b.table = c->d_table;
-  b.master = c->d_master;
-  b.mom = c->d_mom;
-  b.var = c->d_var;
+  {
+    const uint32_t D = cfg.dims;
+    const bool bf = cfg.precision == EP_PREC_BF16;
+    b.ostride = bf ? 3 * D : 2 * D;
+    b.master = (bf && c->d_ostate) ? c->d_ostate : nullptr;
+    b.mom = c->d_ostate ? c->d_ostate + (bf ? D : 0) : nullptr;
+    b.var = c->d_ostate ? c->d_ostate + (bf ? 2 * D : D) : nullptr;
+  }
   b.bf16 = cfg.precision == EP_PREC_BF16 ? 1 : 0;
   b.dims = (int)cfg.dims;
   b.K = cfg.negatives;
@@ -271,10 +284,12 @@ void refresh_base(ep_ctx* c) {
 // Launches one step.  update == false runs the sorted path in gradient-output mode (the
 // merged gradient lands in G/ufeat, nothing is updated); *sorted_buf receives the key
 // buffer holding the sorted keys.
-int launch_training_step(ep_ctx* c, const ep::StepConfig& sc, bool update, float lr_override,
+int launch_training_step(ep_ctx* c, const ep::StepConfig& sc0, bool update, float lr_override,
                          hipEvent_t ev_f0 = nullptr, hipEvent_t ev_f1 = nullptr, hipEvent_t ev_s1 = nullptr,
-                         int* sorted_buf = nullptr) {
+                         int* sorted_buf = nullptr, hipEvent_t ev_s0 = nullptr) {
   hipStream_t s = c->stream;
+  ep::StepConfig sc = sc0;
+  if (sc.seg_vpl == 0) sc.seg_vpl = c->seg_vpl ? c->seg_vpl : 4;
   if (lr_override >= 0.f) EP_HIP(ep::launch_step_begin_lr(sc, c->d_state, lr_override, s));
   else EP_HIP(ep::launch_step_begin(sc, c->d_state, s));
   EP_HIP(ep::launch_sample(sc, c->d_state, s));
@@ -287,6 +302,7 @@ int launch_training_step(ep_ctx* c, const ep::StepConfig& sc, bool update, float
     int buf = 0;
     EP_HIP(ep::launch_sort(sc, c->d_state, s, &buf));
     if (sorted_buf) *sorted_buf = buf;
+    if (ev_s0) EP_HIP(hipEventRecord(ev_s0, s));
     if (update) {
       EP_HIP(ep::launch_scatter_sorted(sc, c->d_state, buf, s));
     } else {
@@ -381,7 +397,52 @@ ep::StepConfig train_config(ep_ctx* c) {
   return sc;
 }
 
+// Number of tuning rounds; the first round is a warm-up and is not scored.
+constexpr uint32_t kTuneRounds = 4;
+
+bool tuning(const ep_ctx* c) { return c->cfg.scatter == EP_SCATTER_SORT && c->seg_vpl == 0; }
+
+// One eager training step with the next candidate variant; after the last tuning step
+// the fastest variant (median scatter time over the scored rounds) is fixed.
+int tune_step(ep_ctx* c, const ep::StepConfig& sc) {
+  if (c->tune_cands.empty()) {
+    for (int v : {16, 8, 4})
+      if (ep::seg_vpl_supported((int)c->cfg.dims, v)) c->tune_cands.push_back(v);
+    if (c->tune_cands.size() <= 1) {
+      c->seg_vpl = c->tune_cands.empty() ? 4 : c->tune_cands[0];
+      return launch_training_step(c, sc, true, -1.f);
+    }
+    const size_t ne = 2 * c->tune_cands.size() * kTuneRounds;
+    c->tune_ev.resize(ne, nullptr);
+    for (auto& e : c->tune_ev) EP_HIP(hipEventCreate(&e));
+  }
+  const uint32_t n = (uint32_t)c->tune_cands.size();
+  const uint32_t i = c->tune_i++;
+  ep::StepConfig t = sc;
+  t.seg_vpl = c->tune_cands[i % n];
+  EP_TRY(launch_training_step(c, t, true, -1.f, nullptr, nullptr, c->tune_ev[2 * i + 1], nullptr, c->tune_ev[2 * i]));
+  if (c->tune_i < n * kTuneRounds) return EP_OK;
+  EP_HIP(hipEventSynchronize(c->tune_ev.back()));
+  float best = INFINITY;
+  int pick = c->tune_cands.back();
+  for (uint32_t k = 0; k < n; ++k) {
+    std::vector<float> ms;
+    for (uint32_t r = 1; r < kTuneRounds; ++r) {
+      const uint32_t j = r * n + k;
+      float x = 0.f;
+      EP_HIP(hipEventElapsedTime(&x, c->tune_ev[2 * j], c->tune_ev[2 * j + 1]));
+      ms.push_back(x);
+    }
+    std::sort(ms.begin(), ms.end());
+    const float med = ms[ms.size() / 2];
+    if (med < best) { best = med; pick = c->tune_cands[k]; }
+  }
+  c->seg_vpl = pick;
+  return EP_OK;
+}
+
 int run_step(ep_ctx* c, const ep::StepConfig& sc) {
+  if (tuning(c)) return tune_step(c, sc);
   if (!c->cfg.use_graph) return launch_training_step(c, sc, true, -1.f);
   if (!c->step_exec_ready) {
     hipGraph_t g = nullptr;
@@ -419,6 +480,8 @@ int validate_config(const ep_config* cfg) {
   if (cfg->loss_weighting < EP_LW_NONE || cfg->loss_weighting > EP_LW_EXPONENTIAL)
     return fail(EP_EINVAL, "loss_weighting");
   if (!(cfg->valid_pct >= 0.f && cfg->valid_pct < 1.f)) return fail(EP_EINVAL, "valid_pct must be in [0, 1)");
+  if (cfg->seg_vpl != 0 && !ep::seg_vpl_supported((int)cfg->dims, cfg->seg_vpl))
+    return fail(EP_EINVAL, "seg_vpl must be 0 or a values-per-lane variant (4, 8, 16) dividing dims");
   return EP_OK;
 }
 
@@ -460,6 +523,7 @@ int ep_create(const ep_config* cfg, ep_ctx** out) {
   EP_HIP(hipSetDevice(cfg->device));
   ep_ctx* c = new ep_ctx();
   c->cfg = *cfg;
+  c->seg_vpl = cfg->seg_vpl;  // 0: tuned during the first training steps
   c->device = cfg->device;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete c; return fail(EP_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
@@ -536,7 +600,7 @@ int ep_set_features(ep_ctx* c, uint64_t n_nodes, uint32_t n_vocab, const uint64_
   EP_HIP(hipMemcpy(c->d_findptr, f_indptr, (n_nodes + 1) * sizeof(uint64_t), hipMemcpyHostToDevice));
   EP_HIP(hipMemcpy(c->d_fids, f_ids, tot * sizeof(uint32_t), hipMemcpyHostToDevice));
   if (c->n_vocab != n_vocab) {
-    c->release(c->d_table); c->release(c->d_master); c->release(c->d_mom); c->release(c->d_var);
+    c->release(c->d_table); c->release(c->d_ostate);
     c->release(c->base.Gd); c->release(c->base.flag); c->release(c->base.touched);
     c->table_ready = false;
   }
@@ -554,27 +618,30 @@ int ep_set_table(ep_ctx* c, const float* rows) {
   EP_HIP(hipSetDevice(c->device));
   const size_t n = (size_t)c->n_vocab * c->cfg.dims;
   const bool bf16 = c->cfg.precision == EP_PREC_BF16;
+  const uint32_t D = c->cfg.dims;
+  const size_t ostride = bf16 ? 3 * (size_t)D : 2 * (size_t)D;
   if (!c->d_table) {
     if (bf16) {
       uint16_t* t = nullptr;
       EP_TRY(c->alloc(&t, n));
       c->d_table = t;
-      EP_TRY(c->alloc(&c->d_master, n));
     } else {
       float* t = nullptr;
       EP_TRY(c->alloc(&t, n));
       c->d_table = t;
     }
-    EP_TRY(c->alloc(&c->d_mom, n));
-    EP_TRY(c->alloc(&c->d_var, n));
+    EP_TRY(c->alloc(&c->d_ostate, (size_t)c->n_vocab * ostride));
   }
-  EP_HIP(hipMemsetAsync(c->d_mom, 0, n * sizeof(float), c->stream));
-  EP_HIP(hipMemsetAsync(c->d_var, 0, n * sizeof(float), c->stream));
+  EP_HIP(hipMemsetAsync(c->d_ostate, 0, (size_t)c->n_vocab * ostride * sizeof(float), c->stream));
   refresh_base(c);
   if (rows) {
-    float* dst = bf16 ? c->d_master : (float*)c->d_table;
-    EP_HIP(hipMemcpyAsync(dst, rows, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    if (bf16) EP_HIP(ep::launch_fill_bf16(c->d_master, (uint16_t*)c->d_table, n, c->stream));
+    if (bf16) {
+      EP_HIP(hipMemcpy2DAsync(c->d_ostate, ostride * sizeof(float), rows, D * sizeof(float), D * sizeof(float),
+                              c->n_vocab, hipMemcpyHostToDevice, c->stream));
+      EP_HIP(ep::launch_fill_bf16(c->d_ostate, (uint16_t*)c->d_table, n, D, (uint32_t)ostride, c->stream));
+    } else {
+      EP_HIP(hipMemcpyAsync(c->d_table, rows, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    }
   } else {
     EP_HIP(ep::launch_init_table(c->base, c->cfg.seed, c->stream));
   }
@@ -588,9 +655,13 @@ int ep_get_table(ep_ctx* c, float* rows) {
   if (!c->table_ready) return fail(EP_ESTATE, "no table");
   EP_HIP(hipSetDevice(c->device));
   const size_t n = (size_t)c->n_vocab * c->cfg.dims;
-  const float* src = c->cfg.precision == EP_PREC_BF16 ? c->d_master : (const float*)c->d_table;
+  const uint32_t D = c->cfg.dims;
   EP_HIP(hipStreamSynchronize(c->stream));
-  EP_HIP(hipMemcpy(rows, src, n * sizeof(float), hipMemcpyDeviceToHost));
+  if (c->cfg.precision == EP_PREC_BF16)
+    EP_HIP(hipMemcpy2D(rows, D * sizeof(float), c->d_ostate, 3 * D * sizeof(float), D * sizeof(float), c->n_vocab,
+                       hipMemcpyDeviceToHost));
+  else
+    EP_HIP(hipMemcpy(rows, c->d_table, n * sizeof(float), hipMemcpyDeviceToHost));
   return EP_OK;
 }
 
@@ -607,10 +678,14 @@ int ep_get_adam(ep_ctx* c, float* m, float* v) {
   if (!c || !m || !v) return fail(EP_EINVAL, "NULL argument");
   if (!c->table_ready) return fail(EP_ESTATE, "no table");
   EP_HIP(hipSetDevice(c->device));
-  const size_t n = (size_t)c->n_vocab * c->cfg.dims;
+  const uint32_t D = c->cfg.dims;
+  const bool bf = c->cfg.precision == EP_PREC_BF16;
+  const size_t ostride = (bf ? 3 : 2) * (size_t)D;
   EP_HIP(hipStreamSynchronize(c->stream));
-  EP_HIP(hipMemcpy(m, c->d_mom, n * sizeof(float), hipMemcpyDeviceToHost));
-  EP_HIP(hipMemcpy(v, c->d_var, n * sizeof(float), hipMemcpyDeviceToHost));
+  EP_HIP(hipMemcpy2D(m, D * sizeof(float), c->d_ostate + (bf ? D : 0), ostride * sizeof(float), D * sizeof(float),
+                     c->n_vocab, hipMemcpyDeviceToHost));
+  EP_HIP(hipMemcpy2D(v, D * sizeof(float), c->d_ostate + (bf ? 2 * D : D), ostride * sizeof(float),
+                     D * sizeof(float), c->n_vocab, hipMemcpyDeviceToHost));
   return EP_OK;
 }
 
@@ -830,7 +905,8 @@ int ep_bench_steps(ep_ctx* c, uint32_t warmup, uint32_t steps, int instrument, e
     ++in_pass;
     return EP_OK;
   };
-  for (uint32_t i = 0; i < warmup; ++i) {
+  // warm-up steps; extended while the scatter-variant tuner is still choosing
+  for (uint32_t i = 0; i < warmup || tuning(c); ++i) {
     EP_TRY(advance());
     EP_TRY(run_step(c, sc));
   }
@@ -864,6 +940,7 @@ int ep_bench_steps(ep_ctx* c, uint32_t warmup, uint32_t steps, int instrument, e
   out->rows = h1.stat_rows - h0.stat_rows;
   out->nodes = h1.stat_nodes - h0.stat_nodes;
   out->unique_rows = h1.stat_unique - h0.stat_unique;
+  out->seg_vpl = c->seg_vpl;
   if (instrument) {
     // per-kernel timing: the same number of steps launched one by one with events
     std::vector<hipEvent_t> ev(3 * steps);

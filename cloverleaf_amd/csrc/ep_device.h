@@ -61,6 +61,22 @@ __host__ __device__ inline float uniform01(uint32_t r) {
   return (float)(r >> 8) * (1.0f / 16777216.0f);
 }
 
+// 32-bit integer hash (C. Wellons' "lowbias32" finalizer: xorshift-multiply, two rounds).
+// Used as a cheap counter-based generator for stochastic-rounding bits, keyed by
+// (row, column block, step); Philox stays on the sampling paths.
+__host__ __device__ inline uint32_t hash32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+__host__ __device__ inline uint32_t sr_key(uint32_t row, uint32_t col4, uint32_t step, uint32_t seed) {
+  return hash32(row ^ hash32(col4 * 0x9E3779B9u ^ step * 0x85EBCA77u ^ seed));
+}
+
 // ------------------------------------------------------------------------------------
 // bfloat16 storage
 // ------------------------------------------------------------------------------------

@@ -27,7 +27,7 @@ struct StepState {
   uint32_t err;            // nonzero: key capacity exceeded this step
   float lr;                // learning rate of this step
   float bc1, bc2;          // Adam bias corrections 1 - beta^t, t = pass + 1
-  float pad1;
+  uint32_t n_real;         // non-sentinel keys (the sorted key count)
   uint32_t dtot[4][2048];  // radix-sort digit totals, one row per sort pass (<= 11-bit digits)
 };
 
@@ -44,9 +44,12 @@ struct StepConfig {
   int sort_bits;
   // parameters
   void* table;             // float* (fp32) or uint16_t* (bf16)
-  float* master;           // fp32 master rows (bf16 mode) else nullptr
-  float* mom;
-  float* var;
+  // Optimizer state: one record of `ostride` floats per row, [master |] m | v, so an
+  // update touches one contiguous record (master only in bf16 mode).
+  float* master;           // record + 0 (bf16 mode) else nullptr
+  float* mom;              // record + 0 (fp32) / + D (bf16)
+  float* var;              // record + D (fp32) / + 2D (bf16)
+  uint32_t ostride;        // 2D (fp32) or 3D (bf16)
   int bf16;
   int dims;
   // batch
@@ -84,6 +87,7 @@ struct StepConfig {
   uint32_t hash_entries;   // first-leaf LDS hash size (power of two)
   uint32_t ent_cap;        // forward LDS stage: max entries (nodes) per example, <= 256
   uint32_t ids_cap;        // forward LDS stage: max feature occurrences per example
+  int fuse_emit;           // k_forward emits the keys of LDS-staged examples
   // keys
   uint32_t* keys[2];
   uint32_t* slots[2];
@@ -106,6 +110,7 @@ struct StepConfig {
   int scatter;             // 0 sort, 1 atomic
   int grad_out;            // test hook: write merged gradients (G/ufeat) instead of updating
   int unique_grid;         // grid size for the persistent row kernels
+  int seg_vpl;             // k_seg_adam values per lane (4, 8, 16; chosen by the tuner)
 };
 
 // Launchers (all asynchronous on `stream`).
@@ -124,10 +129,11 @@ hipError_t launch_atomic_scatter(const StepConfig& c, StepState* st, hipStream_t
 hipError_t launch_adam_touched(const StepConfig& c, StepState* st, hipStream_t s);
 // One-off kernels.
 hipError_t launch_init_table(const StepConfig& c, uint64_t seed, hipStream_t s);
-hipError_t launch_fill_bf16(const float* src, uint16_t* dst, uint64_t n, hipStream_t s);
+hipError_t launch_fill_bf16(const float* src, uint16_t* dst, uint64_t n, uint32_t D, uint32_t stride, hipStream_t s);
 hipError_t launch_embed_averaged(const StepConfig& c, const uint64_t* findptr, const uint32_t* fids,
                                  uint64_t n_nodes, float* out, hipStream_t s);
 
 bool dims_supported(int d);
+bool seg_vpl_supported(int d, int vpl);
 
 }  // namespace ep

@@ -168,6 +168,7 @@ __global__ void k_step_begin(StepConfig c, StepState* st, float lr_override) {
     st->bc1 = 1.0f - powf(0.9f, t);
     st->bc2 = 1.0f - powf(0.999f, t);
     st->n_keys = 0u;
+    st->n_real = 0u;
     st->n_unique = 0u;
     st->n_long = 0u;
     st->n_pieces = 0u;
@@ -354,6 +355,7 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
   __shared__ float s_scale, s_invA;
   __shared__ int s_contrib;
   __shared__ uint32_t s_active;
+  __shared__ uint8_t s_el[32];
   const T* table = (const T*)c.table;
   const uint32_t v = c.perm[st->anchor_base + b];
   const uint32_t cnt = c.nbr_cnt[b];
@@ -570,7 +572,44 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
     if (lane == 0) {
       c.elig[(size_t)b * NC + ci] = e ? 1 : 0;
       c.coef[(size_t)b * NC + ci] = co;
+      s_el[ci] = e ? 1 : 0;
     }
+  }
+  // Key emission for staged examples (k_emit handles the rest): one (feature, slot) key
+  // per occurrence, sentinel unless the construction is eligible and, under first_leaf,
+  // the feature's first eligible construction (mod.rs:347-352, 366-367).
+  if (!(staged && c.fuse_emit)) return;
+  if (st->n_keys > c.key_cap) return;
+  __syncthreads();
+  const uint32_t tot = ent_off[E];
+  const uint32_t kb = c.keyoff[b];
+  const bool first_leaf = c.grad_mode == 0;
+  uint32_t* hsh = ids + c.ids_cap;
+  const uint32_t hmask = c.hash_entries - 1u;
+  auto cons_of = [&](uint32_t o) -> uint32_t {
+    uint32_t lo = 0, hi = E;  // last entry e with ent_off[e] <= o
+    while (hi - lo > 1u) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (ent_off[mid] <= o) lo = mid; else hi = mid;
+    }
+    return lo == 0u ? 0u : (lo <= cnt ? 1u : 2u + (lo - 1u - cnt));
+  };
+  if (first_leaf) {
+    for (uint32_t i = tid; i < c.hash_entries; i += blockDim.x) hsh[i] = HASH_EMPTY;
+    __syncthreads();
+    for (uint32_t o = tid; o < tot; o += blockDim.x) {
+      const uint32_t ci = cons_of(o);
+      if (s_el[ci]) hash_insert_min(hsh, hmask, ids[o], ci);
+    }
+    __syncthreads();
+  }
+  for (uint32_t o = tid; o < tot; o += blockDim.x) {
+    const uint32_t ci = cons_of(o);
+    const uint32_t f = ids[o];
+    uint32_t key = c.sentinel;
+    if (s_el[ci] && (!first_leaf || hash_lookup(hsh, hmask, f) == ci)) key = f;
+    c.keys[0][kb + o] = key;
+    c.slots[0][kb + o] = b * NC + ci;
   }
 }
 
@@ -588,6 +627,7 @@ __global__ __launch_bounds__(256) void k_emit(StepConfig c, StepState* st) {
   if (cnt == 0u) return;
   const int tid = threadIdx.x;
   const uint32_t K = c.K, NC = K + 2u;
+  if (c.fuse_emit && 1u + cnt + K <= c.ent_cap && c.rcount[b] <= c.ids_cap) return;  // emitted by k_forward
   extern __shared__ uint32_t hsh[];
   __shared__ uint32_t s_wave[NW];
   __shared__ uint8_t s_el[32];
@@ -684,15 +724,17 @@ __device__ inline uint64_t digit_peers(bool act, uint32_t d) {
   return act ? peers : 0ull;
 }
 
+// Pass 0 reads the emitted keys (st->n_keys, sentinels included) and drops the sentinels;
+// later passes read the st->n_real compacted keys.
 template <int RB>
 __global__ __launch_bounds__(256) void k_rs_hist(const uint32_t* __restrict__ kin, StepState* st, uint32_t* ghist,
-                                                 int pass, int shift, int G) {
+                                                 int pass, int shift, int G, uint32_t sent) {
   constexpr uint32_t BINS = 1u << RB, MASK = BINS - 1u;
   __shared__ uint32_t h[BINS];
   const int tid = threadIdx.x, lane = tid & 63;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint32_t lo, hi;
-  block_range(st->n_keys, G, blockIdx.x, &lo, &hi);
+  block_range(pass == 0 ? st->n_keys : st->n_real, G, blockIdx.x, &lo, &hi);
   for (uint32_t d = tid; d < BINS; d += 256u) h[d] = 0u;
   __syncthreads();
   for (uint32_t t0 = lo; t0 < hi; t0 += 256u * RS_ITEMS) {
@@ -705,7 +747,7 @@ __global__ __launch_bounds__(256) void k_rs_hist(const uint32_t* __restrict__ ki
 #pragma unroll
     for (int r = 0; r < RS_ITEMS; ++r) {
       const uint32_t i = t0 + r * 256u + tid;
-      const bool act = i < hi;
+      const bool act = i < hi && k[r] != sent;
       const uint32_t d = (k[r] >> shift) & MASK;
       const uint64_t peers = digit_peers<RB>(act, d);
       if (act && (peers & lt) == 0ull) atomicAdd(&h[d], (uint32_t)__popcll(peers));  // lowest lane of the group
@@ -730,6 +772,7 @@ __global__ __launch_bounds__(256) void k_rs_scan(uint32_t* ghist, StepState* st,
   uint32_t pre = 0;
   for (uint32_t q = lane; q < d; q += 64u) pre += st->dtot[pass][q];
   pre = wave_sum_u32(pre);
+  if (d == BINS - 1u && lane == 0) st->n_real = pre + st->dtot[pass][d];
   const uint32_t per = (uint32_t)G / 64u;  // G is a multiple of 256
   const uint32_t b0 = (uint32_t)lane * per;
   uint32_t loc = 0;
@@ -752,14 +795,14 @@ __global__ __launch_bounds__(256) void k_rs_scan(uint32_t* ghist, StepState* st,
 template <int RB>
 __global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                     uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                    StepState* st, const uint32_t* __restrict__ ghist, int shift,
-                                                    int G) {
+                                                    StepState* st, const uint32_t* __restrict__ ghist, int pass,
+                                                    int shift, int G, uint32_t sent) {
   constexpr uint32_t BINS = 1u << RB, MASK = BINS - 1u;
   __shared__ uint32_t run[BINS];
   __shared__ uint32_t wc[NW][BINS];  // (round << 7) | count of wave w for digit d in that round
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   uint32_t lo, hi;
-  block_range(st->n_keys, G, blockIdx.x, &lo, &hi);
+  block_range(pass == 0 ? st->n_keys : st->n_real, G, blockIdx.x, &lo, &hi);
   for (uint32_t d = tid; d < BINS; d += 256u) {
     run[d] = ghist[(size_t)blockIdx.x * BINS + d];
 #pragma unroll
@@ -779,7 +822,7 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t* __restrict__
 #pragma unroll
     for (int q = 0; q < RS_ITEMS; ++q, ++r) {
       const uint32_t i = t0 + q * 256u + tid;
-      const bool act = i < hi;
+      const bool act = i < hi && k[q] != sent;
       const uint32_t d = (k[q] >> shift) & MASK;
       const uint64_t peers = digit_peers<RB>(act, d);
       const uint32_t rank = (uint32_t)__popcll(peers & lt);
@@ -826,10 +869,10 @@ __device__ inline void adam_math(float& th, float& m, float& v, float g, float l
   th -= lr * (mh / (sqrtf(vh) + eps));
 }
 
+// 16 stochastic-rounding bits for column 4*col4 + q of row f at this step.
 __device__ inline uint32_t sr_bits(const StepConfig& c, const StepState* st, uint32_t f, uint32_t col4, int q) {
-  const u32x4 r = philox4x32(u32x4{f, col4, (uint32_t)st->next_step, RNG_ROUNDING}, (uint32_t)c.seed,
-                             (uint32_t)(c.seed >> 32));
-  const uint32_t w = (q >> 1) == 0 ? r.x : r.y;
+  const uint32_t r0 = sr_key(f, col4, (uint32_t)st->next_step, (uint32_t)c.seed);
+  const uint32_t w = (q >> 1) == 0 ? r0 : hash32(r0 ^ 0x27D4EB2Fu);
   return (q & 1) ? (w >> 16) : (w & 0xffffu);
 }
 
@@ -844,18 +887,19 @@ __device__ inline void adam_row(const StepConfig& c, const StepState* st, uint32
   for (int q = 0; q < PL; ++q) bad |= (base + q < D) && isnan(g[q]);
   if (wave_any(bad)) return;
   const size_t off = (size_t)f * D;
+  const size_t ooff = (size_t)f * c.ostride;  // optimizer record [master |] m | v
   float m[PL], v[PL], th[PL];
-  load_slice<D>(c.mom + off, lane, m);
-  load_slice<D>(c.var + off, lane, v);
-  if (c.bf16) load_slice<D>(c.master + off, lane, th);
+  load_slice<D>(c.mom + ooff, lane, m);
+  load_slice<D>(c.var + ooff, lane, v);
+  if (c.bf16) load_slice<D>(c.master + ooff, lane, th);
   else load_slice<D>((const float*)c.table + off, lane, th);
   const float lr = st->lr, bc1 = st->bc1, bc2 = st->bc2;
 #pragma unroll
   for (int q = 0; q < PL; ++q) adam_math(th[q], m[q], v[q], g[q], lr, bc1, bc2);
-  store_slice<D>(c.mom + off, lane, m);
-  store_slice<D>(c.var + off, lane, v);
+  store_slice<D>(c.mom + ooff, lane, m);
+  store_slice<D>(c.var + ooff, lane, v);
   if (c.bf16) {
-    store_slice<D>(c.master + off, lane, th);
+    store_slice<D>(c.master + ooff, lane, th);
     uint16_t* row = (uint16_t*)c.table + off;
 #pragma unroll
     for (int q = 0; q < PL; ++q)
@@ -868,31 +912,33 @@ __device__ inline void adam_row(const StepConfig& c, const StepState* st, uint32
 // ------------------------------------------------------------------------------------
 // Fused segment reduce + Adam over the sorted keys (the batch merge of mod.rs:223-233
 // followed by the update).  A wave takes a 64-entry chunk and owns every segment that
-// starts in it.  Lane groups of L = D/4 lanes (4 values per lane) each take one segment
-// at a time, G = 64/L segments in parallel: the adjoint rows of the segment and the
-// row's theta/m/v are loaded together, then the Adam update is applied.  A chunk's last
-// segment that does not end within the next 64 entries is deferred as a long segment
-// (split into PIECE-entry pieces, combined in order).  grad_out != 0 writes the merged
-// gradient rows to G/ufeat instead of updating (test hook).
+// starts in it.  Lane groups of L = D/VPL lanes (VPL values per lane, as VPL/4 float4
+// columns strided by L) each take one segment at a time, G = 64/L segments in parallel:
+// the adjoint rows of the segment and the row's theta/m/v are loaded together, then the
+// Adam update is applied.  Larger VPL keeps more segments (and loads) in flight per wave;
+// every VPL adds in the same order, so the variants give identical bits and the tuner
+// picks the fastest.  A chunk's last segment that does not end within the next 64
+// entries is deferred as a long segment (split into PIECE-entry pieces, combined in
+// order).  grad_out != 0 writes the merged gradient rows to G/ufeat instead (test hook).
 // ------------------------------------------------------------------------------------
-template <int D>
+template <int D, int VPL>
 __global__ __launch_bounds__(256) void k_seg_adam(StepConfig c, StepState* st, const uint32_t* __restrict__ Ks,
                                                   const uint32_t* __restrict__ Ss) {
-  static_assert(D % 4 == 0 && D <= 256, "4-value lane slices");
-  constexpr int L = D / 4;        // lanes per row
+  static_assert(D % VPL == 0 && VPL % 4 == 0 && D / VPL <= 64, "float4 lane slices");
+  constexpr int L = D / VPL;      // lanes per row
   constexpr int G = 64 / L;       // segments in flight per wave
-  constexpr int U = 8;            // adjoint rows in flight per group
+  constexpr int NQ = VPL / 4;     // float4 columns per lane
+  constexpr int U = NQ >= 8 ? 1 : 8 / NQ;  // adjoint rows in flight per group
   __shared__ uint32_t s_pos[NW][66];
   __shared__ unsigned long long s_cnt;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int grp = lane / L, lg = lane % L;
   const bool gact = grp < G;
-  const uint32_t col = (uint32_t)lg * 4u;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const uint64_t gmask = (L == 64) ? ~0ull : (((1ull << L) - 1ull) << (grp * L));
+  const uint64_t gmask = (L == 64) ? ~0ull : (((1ull << L) - 1ull) << ((grp * L) & 63));
   const uint32_t gw = blockIdx.x * NW + w;
   const uint32_t nwaves = gridDim.x * NW;
-  const uint32_t n = st->n_keys <= c.key_cap ? st->n_keys : 0u;
+  const uint32_t n = st->n_keys <= c.key_cap ? st->n_real : 0u;  // sorted, sentinels dropped
   const uint32_t SENT = c.sentinel;
   const float lr = st->lr, bc1 = st->bc1, bc2 = st->bc2;
   if (tid == 0) s_cnt = 0ull;
@@ -955,20 +1001,26 @@ __global__ __launch_bounds__(256) void k_seg_adam(StepConfig c, StepState* st, c
       const uint32_t f = __shfl(k0, (int)sa, EP_WAVE);
       const uint32_t len = sb - sa;
       // Adam operands first: independent of the reduction, so their latency overlaps it
-      float4 m4 = make_float4(0.f, 0.f, 0.f, 0.f), v4 = m4, t4 = m4;
-      const size_t roff = (size_t)f * D + col;
-      if (has && !c.grad_out) {
-        m4 = *reinterpret_cast<const float4*>(c.mom + roff);
-        v4 = *reinterpret_cast<const float4*>(c.var + roff);
-        t4 = c.bf16 ? *reinterpret_cast<const float4*>(c.master + roff)
-                    : *reinterpret_cast<const float4*>((const float*)c.table + roff);
+      float4 m4[NQ], v4[NQ], t4[NQ], acc[NQ];
+      const size_t roff = (size_t)f * D;
+      const size_t ooff = (size_t)f * c.ostride;  // optimizer record [master |] m | v
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const uint32_t col = (uint32_t)(q * L + lg) * 4u;
+        acc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+        m4[q] = v4[q] = t4[q] = acc[q];
+        if (has && !c.grad_out) {
+          m4[q] = *reinterpret_cast<const float4*>(c.mom + ooff + col);
+          v4[q] = *reinterpret_cast<const float4*>(c.var + ooff + col);
+          t4[q] = c.bf16 ? *reinterpret_cast<const float4*>(c.master + ooff + col)
+                         : *reinterpret_cast<const float4*>((const float*)c.table + roff + col);
+        }
       }
       uint32_t maxlen = len;
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) maxlen = max(maxlen, (uint32_t)__shfl_xor(maxlen, off, EP_WAVE));
-      float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
       for (uint32_t u0 = 0; u0 < maxlen; u0 += U) {
-        float4 r[U];
+        float4 r[U][NQ];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const uint32_t t = sa + u0 + (uint32_t)u;  // chunk-relative entry, < 128
@@ -977,11 +1029,19 @@ __global__ __launch_bounds__(256) void k_seg_adam(StepConfig c, StepState* st, c
           const uint32_t sl0 = __shfl(s0, (int)src, EP_WAVE);
           const uint32_t sl1 = __shfl(s1, (int)src, EP_WAVE);
           const uint32_t sl = t < 64u ? sl0 : sl1;
-          r[u] = valid ? *reinterpret_cast<const float4*>(c.adj + (size_t)sl * D + col) : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            const uint32_t col = (uint32_t)(q * L + lg) * 4u;
+            r[u][q] = valid ? *reinterpret_cast<const float4*>(c.adj + (size_t)sl * D + col)
+                            : make_float4(0.f, 0.f, 0.f, 0.f);
+          }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          acc.x += r[u].x; acc.y += r[u].y; acc.z += r[u].z; acc.w += r[u].w;
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            acc[q].x += r[u][q].x; acc[q].y += r[u][q].y; acc[q].z += r[u][q].z; acc[q].w += r[u][q].w;
+          }
         }
       }
       if (c.grad_out) {
@@ -991,30 +1051,38 @@ __global__ __launch_bounds__(256) void k_seg_adam(StepConfig c, StepState* st, c
         ub = __shfl(ub, 0, EP_WAVE);
         const uint32_t u = ub + (uint32_t)__popcll(heads & ((grp * L == 0) ? 0ull : (~0ull >> (64 - grp * L))));
         if (has) {
-          *reinterpret_cast<float4*>(c.G + (size_t)u * D + col) = acc;
+#pragma unroll
+          for (int q = 0; q < NQ; ++q)
+            *reinterpret_cast<float4*>(c.G + (size_t)u * D + (q * L + lg) * 4) = acc[q];
           if (lg == 0) c.ufeat[u] = f;
         }
         continue;
       }
-      const bool bad = has && (isnan(acc.x) || isnan(acc.y) || isnan(acc.z) || isnan(acc.w));
-      const bool skip = (__ballot(bad) & gmask) != 0ull;
+      bool bad = false;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) bad |= isnan(acc[q].x) || isnan(acc[q].y) || isnan(acc[q].z) || isnan(acc[q].w);
+      const bool skip = (__ballot(has && bad) & gmask) != 0ull;
       if (has && !skip) {
-        adam_math(t4.x, m4.x, v4.x, acc.x, lr, bc1, bc2);
-        adam_math(t4.y, m4.y, v4.y, acc.y, lr, bc1, bc2);
-        adam_math(t4.z, m4.z, v4.z, acc.z, lr, bc1, bc2);
-        adam_math(t4.w, m4.w, v4.w, acc.w, lr, bc1, bc2);
-        *reinterpret_cast<float4*>(c.mom + roff) = m4;
-        *reinterpret_cast<float4*>(c.var + roff) = v4;
-        if (c.bf16) {
-          *reinterpret_cast<float4*>(c.master + roff) = t4;
-          const u32x4 rb = philox4x32(u32x4{f, col >> 2, (uint32_t)st->next_step, RNG_ROUNDING},
-                                      (uint32_t)c.seed, (uint32_t)(c.seed >> 32));
-          uint2 pk;
-          pk.x = (uint32_t)f32_to_bf16_sr(t4.x, rb.x & 0xffffu) | ((uint32_t)f32_to_bf16_sr(t4.y, rb.x >> 16) << 16);
-          pk.y = (uint32_t)f32_to_bf16_sr(t4.z, rb.y & 0xffffu) | ((uint32_t)f32_to_bf16_sr(t4.w, rb.y >> 16) << 16);
-          *reinterpret_cast<uint2*>((uint16_t*)c.table + roff) = pk;
-        } else {
-          *reinterpret_cast<float4*>((float*)c.table + roff) = t4;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          const uint32_t col = (uint32_t)(q * L + lg) * 4u;
+          adam_math(t4[q].x, m4[q].x, v4[q].x, acc[q].x, lr, bc1, bc2);
+          adam_math(t4[q].y, m4[q].y, v4[q].y, acc[q].y, lr, bc1, bc2);
+          adam_math(t4[q].z, m4[q].z, v4[q].z, acc[q].z, lr, bc1, bc2);
+          adam_math(t4[q].w, m4[q].w, v4[q].w, acc[q].w, lr, bc1, bc2);
+          *reinterpret_cast<float4*>(c.mom + ooff + col) = m4[q];
+          *reinterpret_cast<float4*>(c.var + ooff + col) = v4[q];
+          if (c.bf16) {
+            *reinterpret_cast<float4*>(c.master + ooff + col) = t4[q];
+            const uint32_t r0 = sr_key(f, col >> 2, (uint32_t)st->next_step, (uint32_t)c.seed);
+            const uint32_t r1 = hash32(r0 ^ 0x27D4EB2Fu);
+            uint2 pk;
+            pk.x = (uint32_t)f32_to_bf16_sr(t4[q].x, r0 & 0xffffu) | ((uint32_t)f32_to_bf16_sr(t4[q].y, r0 >> 16) << 16);
+            pk.y = (uint32_t)f32_to_bf16_sr(t4[q].z, r1 & 0xffffu) | ((uint32_t)f32_to_bf16_sr(t4[q].w, r1 >> 16) << 16);
+            *reinterpret_cast<uint2*>((uint16_t*)c.table + roff + col) = pk;
+          } else {
+            *reinterpret_cast<float4*>((float*)c.table + roff + col) = t4[q];
+          }
         }
       }
     }
@@ -1215,7 +1283,7 @@ __global__ __launch_bounds__(256) void k_init_table(StepConfig c, uint64_t seed)
 #pragma unroll
     for (int q = 0; q < PL; ++q) x[q] = x[q] / nrm;
     if (c.bf16) {
-      store_slice<D>(c.master + row * D, lane, x);
+      store_slice<D>(c.master + row * c.ostride, lane, x);
       uint16_t* t = (uint16_t*)c.table + row * D;
 #pragma unroll
       for (int q = 0; q < PL; ++q)
@@ -1226,9 +1294,11 @@ __global__ __launch_bounds__(256) void k_init_table(StepConfig c, uint64_t seed)
   }
 }
 
-__global__ void k_fill_bf16(const float* __restrict__ src, uint16_t* __restrict__ dst, uint64_t n) {
+// dst[r*D + j] = bf16(src[r*stride + j]) for rows r < n/D.
+__global__ void k_fill_bf16(const float* __restrict__ src, uint16_t* __restrict__ dst, uint64_t n, uint32_t D,
+                            uint32_t stride) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-    dst[i] = f32_to_bf16_rne(src[i]);
+    dst[i] = f32_to_bf16_rne(src[(i / D) * stride + i % D]);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1301,8 +1371,9 @@ hipError_t launch_scan(const StepConfig& c, StepState* st, hipStream_t s) {
 
 size_t forward_lds_bytes(const StepConfig& c) {
   const uint32_t NC = c.K + 2;
+  const size_t hash = (c.train && c.fuse_emit && c.grad_mode == 0) ? (size_t)c.hash_entries * 4 : 0;
   return (size_t)forward_float_words(NC, c.K, c.dims) * 4 + (size_t)c.ent_cap * 8 + (size_t)(c.ent_cap + 1) * 4 +
-         (size_t)c.ids_cap * 4;
+         (size_t)c.ids_cap * 4 + hash;
 }
 
 hipError_t launch_forward(const StepConfig& c, StepState* st, hipStream_t s) {
@@ -1331,10 +1402,11 @@ static void launch_sort_rb(const StepConfig& c, StepState* st, hipStream_t s, in
   const int G = c.sort_grid;
   for (int p = 0; p < passes; ++p) {
     const int shift = p * RB;
-    hipLaunchKernelGGL((k_rs_hist<RB>), dim3(G), dim3(256), 0, s, c.keys[*cur], st, c.ghist, p, shift, G);
+    hipLaunchKernelGGL((k_rs_hist<RB>), dim3(G), dim3(256), 0, s, c.keys[*cur], st, c.ghist, p, shift, G,
+                       c.sentinel);
     hipLaunchKernelGGL((k_rs_scan<RB>), dim3(((1u << RB) + NW - 1u) / NW), dim3(256), 0, s, c.ghist, st, p, G);
     hipLaunchKernelGGL((k_rs_scatter<RB>), dim3(G), dim3(256), 0, s, c.keys[*cur], c.slots[*cur], c.keys[1 - *cur],
-                       c.slots[1 - *cur], st, c.ghist, shift, G);
+                       c.slots[1 - *cur], st, c.ghist, p, shift, G, c.sentinel);
     *cur = 1 - *cur;
   }
 }
@@ -1360,10 +1432,34 @@ hipError_t launch_sort(const StepConfig& c, StepState* st, hipStream_t s, int* o
   return hipGetLastError();
 }
 
+// Values-per-lane variants of k_seg_adam available for a row width (VPL | D, D/VPL <= 64).
+bool seg_vpl_supported(int d, int vpl) {
+  if (vpl != 4 && vpl != 8 && vpl != 16) return false;
+  return d % vpl == 0 && d / vpl <= 64 && (d % 4 == 0);
+}
+
+template <int D>
+static void launch_seg_adam_vpl(const StepConfig& c, StepState* st, int buf, hipStream_t s, int grid) {
+  int vpl = c.seg_vpl;
+  if constexpr (D % 16 == 0 && D / 16 <= 64) {
+    if (vpl == 16) {
+      hipLaunchKernelGGL((k_seg_adam<D, 16>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
+      return;
+    }
+  }
+  if constexpr (D % 8 == 0 && D / 8 <= 64) {
+    if (vpl == 8) {
+      hipLaunchKernelGGL((k_seg_adam<D, 8>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_seg_adam<D, 4>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
+}
+
 hipError_t launch_scatter_sorted(const StepConfig& c, StepState* st, int buf, hipStream_t s) {
   const int grid = c.unique_grid;
   EP_DISPATCH(c.dims, {
-    hipLaunchKernelGGL((k_seg_adam<D>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
+    launch_seg_adam_vpl<D>(c, st, buf, s, grid);
     hipLaunchKernelGGL((k_pieces<D>), dim3(grid), dim3(256), 0, s, c, st, c.slots[buf]);
     hipLaunchKernelGGL((k_long_combine<D>), dim3(256), dim3(256), 0, s, c, st, c.keys[buf]);
   });
@@ -1385,8 +1481,8 @@ hipError_t launch_init_table(const StepConfig& c, uint64_t seed, hipStream_t s) 
   return hipGetLastError();
 }
 
-hipError_t launch_fill_bf16(const float* src, uint16_t* dst, uint64_t n, hipStream_t s) {
-  hipLaunchKernelGGL(k_fill_bf16, dim3(2048), dim3(256), 0, s, src, dst, n);
+hipError_t launch_fill_bf16(const float* src, uint16_t* dst, uint64_t n, uint32_t D, uint32_t stride, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_bf16, dim3(2048), dim3(256), 0, s, src, dst, n, D, stride);
   return hipGetLastError();
 }
 

@@ -70,6 +70,8 @@ typedef struct ep_config {
   int32_t use_graph;           /* 1: replay each step from a captured hipGraph      */
   uint64_t seed;               /* default 20222022 (src/lib.rs:90)                  */
   int32_t device;              /* HIP device ordinal                                */
+  int32_t seg_vpl;             /* segment-Adam values per lane: 0 = autotune over   */
+                               /* the first training steps, else 4, 8 or 16         */
 } ep_config;
 
 typedef struct ep_ctx ep_ctx;
@@ -90,6 +92,7 @@ typedef struct ep_bench_result {
   uint64_t unique_rows;        /* rows updated by Adam in the timed steps            */
   uint32_t steps;
   uint32_t forward_launches;
+  int32_t seg_vpl;             /* segment-Adam variant in use (after tuning)        */
 } ep_bench_result;
 
 /* Fills the reference defaults (EmbeddingPropagator::new, src/lib.rs:1262-1274). */
