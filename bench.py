@@ -42,10 +42,12 @@ def algorithmic_bytes(rows, nodes, examples, dims, bytes_per_value):
     return rows * dims * bytes_per_value + 4 * rows + 4 * nodes + 16 * (examples + nodes)
 
 
-def run_form(w, form, steps, warmup, scatter=None):
+def run_form(w, form, steps, warmup, scatter=None, concurrent=1):
     from cloverleaf_amd import _native as N
 
     cfg = N.default_config()
+    if form.endswith("-batched"):
+        cfg.concurrent_batches = concurrent
     cfg.dims, cfg.batch_size, cfg.negatives, cfg.gamma, cfg.alpha = w.dims, w.batch, w.negatives, w.gamma, w.alpha
     cfg.max_nodes_kind, cfg.max_nodes = N.EP_SAMPLE_FIXED, w.max_nodes
     cfg.valid_pct = 0.0
@@ -57,6 +59,9 @@ def run_form(w, form, steps, warmup, scatter=None):
         cfg.scatter = N.EP_SCATTER_ATOMIC if scatter == "atomic" else N.EP_SCATTER_SORT
     elif form == "bf16-batched":
         cfg.precision, cfg.use_graph = N.EP_PREC_BF16, 1
+        cfg.scatter = N.EP_SCATTER_ATOMIC if scatter == "atomic" else N.EP_SCATTER_SORT
+    elif form == "bf16-master-batched":
+        cfg.precision, cfg.use_graph = N.EP_PREC_BF16_MASTER, 1
         cfg.scatter = N.EP_SCATTER_ATOMIC if scatter == "atomic" else N.EP_SCATTER_SORT
     elif form == "bf16-plain":
         cfg.precision, cfg.use_graph, cfg.scatter = N.EP_PREC_BF16, 0, N.EP_SCATTER_SORT
@@ -75,7 +80,8 @@ def run_form(w, form, steps, warmup, scatter=None):
     return {
         "form": form,
         "scatter": scatter or "sort",
-        "seg_vpl": int(r.seg_vpl),
+        "concurrent_batches": int(cfg.concurrent_batches),
+        "seg_tile": [int(r.seg_vpl), int(r.seg_unroll)],
         "examples_per_s": r.examples / (r.ms_total / 1e3),
         "ms_per_step": r.ms_total / r.steps,
         "examples": int(r.examples),
@@ -126,6 +132,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--scatters", default="sort,atomic", help="scatter paths tried for the batched forms")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--concurrent", type=int, default=4,
+                    help="reference batches per step in the batched forms (the plain forms use 1)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (HBM-counter calibration)")
     args = ap.parse_args()
@@ -159,10 +167,13 @@ def main():
         best = None
         scatters = ["sort"] if form.endswith("-plain") else [s for s in args.scatters.split(",")]
         for sc in scatters:
-            r = run_form(w, form, args.steps, args.warmup, scatter=sc)
+            if sc == "atomic" and args.concurrent > 1:
+                continue  # the atomic path runs one batch per step
+            r = run_form(w, form, args.steps, args.warmup, scatter=sc, concurrent=args.concurrent)
             print(f"[bench] {form}/{sc}: {r['examples_per_s']:.0f} ex/s, {r['ms_per_step']:.3f} ms/step, "
                   f"fwd {r['forward_ms_avg']:.3f} ms ({r['forward_gbs']:.0f} GB/s), "
-                  f"scatter+update {r['scatter_update_ms_avg']:.3f} ms (seg_vpl {r['seg_vpl']})", file=sys.stderr, flush=True)
+                  f"scatter+update {r['scatter_update_ms_avg']:.3f} ms (seg tile {r['seg_tile']}, P {r['concurrent_batches']}, "
+                  f"{r['unique_rows_per_step']:.0f} rows/step)", file=sys.stderr, flush=True)
             if best is None or r["examples_per_s"] > best["examples_per_s"]:
                 best = r
         results[form] = best

@@ -18,7 +18,7 @@ LIB_PATH = PKG / "libcloverleaf_ep.so"
 
 EP_SAMPLE_ALL, EP_SAMPLE_FIXED = 0, 1
 EP_LW_NONE, EP_LW_LOG, EP_LW_EXPONENTIAL = 0, 1, 2
-EP_PREC_FP32, EP_PREC_BF16 = 0, 1
+EP_PREC_FP32, EP_PREC_BF16, EP_PREC_BF16_MASTER = 0, 1, 2
 EP_GRAD_FIRST_LEAF, EP_GRAD_EXACT = 0, 1
 EP_SCATTER_SORT, EP_SCATTER_ATOMIC = 0, 1
 
@@ -51,6 +51,8 @@ class EPConfig(Structure):
         ("seed", c_uint64),
         ("device", c_int32),
         ("seg_vpl", c_int32),
+        ("seg_unroll", c_int32),
+        ("concurrent_batches", c_uint32),
     ]
 
 
@@ -66,6 +68,7 @@ class EPBenchResult(Structure):
         ("steps", c_uint32),
         ("forward_launches", c_uint32),
         ("seg_vpl", c_int32),
+        ("seg_unroll", c_int32),
     ]
 
 
@@ -259,7 +262,7 @@ class Context:
         loss = np.zeros(n, dtype=np.float32)
         cons = np.zeros((n, K + 2, D), dtype=np.float32) if want_cons else None
         nu = c_uint32(0)
-        cap = self.n_vocab
+        cap = self.n_vocab * max(1, int(self.cfg.concurrent_batches))  # one record per (row, batch)
         feat = np.zeros(cap, dtype=np.uint32)
         grad = np.zeros((cap, D), dtype=np.float32)
         check(load().ep_step_explicit(self.handle, ptr(anchors, c_uint32), n, ptr(nbr_cnt, c_uint32),

@@ -447,16 +447,18 @@ class EmbeddingPropagator:
     """Embedding Propagation trainer; the step runs on the GPU.
 
     Reference arguments keep their names and defaults.  Extra keyword arguments
-    select the MI355X execution: ``precision`` ("fp32" | "bf16"), ``grad_mode``
+    select the MI355X execution: ``precision`` ("fp32" | "bf16" | "bf16-master"), ``grad_mode``
     ("first_leaf", the reference rule, | "exact"), ``scatter`` ("sort" | "atomic"),
-    ``use_graph`` (replay steps from a captured hipGraph), ``device``.
+    ``use_graph`` (replay steps from a captured hipGraph), ``concurrent_batches`` (1, 2, 4
+    or 8 batches per step, reading the same parameters and updating in batch order, like
+    the reference's concurrently running batches; 1 = strictly sequential), ``device``.
     """
 
     def __init__(self, alpha=None, loss=None, batch_size=None, dims=None, passes=None, seed=None, max_nodes=None,
                  weighted_neighbor_sampling=None, weighted_neighbor_averaging=None, max_features=None,
                  loss_weighting=None, valid_pct=None, hard_negatives=None, indicator=None, attention=None,
                  attention_heads=None, context_window=None, noise=None, *, precision="fp32",
-                 grad_mode="first_leaf", scatter="sort", use_graph=False, device=0):
+                 grad_mode="first_leaf", scatter="sort", use_graph=False, concurrent_batches=1, device=0):
         loss = loss or EPLoss.margin(1.0, 1)
         mf_kind, mf = _parse_sample(max_features, "max_features")
         mn_kind, mn = _parse_sample(max_nodes, "max_nodes")
@@ -486,10 +488,12 @@ class EmbeddingPropagator:
         if loss_weighting is not None:
             cfg.loss_weighting = loss_weighting.kind
             cfg.loss_weight_exp = loss_weighting.weight
-        cfg.precision = {"fp32": N.EP_PREC_FP32, "bf16": N.EP_PREC_BF16}[precision]
+        cfg.precision = {"fp32": N.EP_PREC_FP32, "bf16": N.EP_PREC_BF16,
+                         "bf16-master": N.EP_PREC_BF16_MASTER}[precision]
         cfg.grad_mode = {"first_leaf": N.EP_GRAD_FIRST_LEAF, "exact": N.EP_GRAD_EXACT}[grad_mode]
         cfg.scatter = {"sort": N.EP_SCATTER_SORT, "atomic": N.EP_SCATTER_ATOMIC}[scatter]
         cfg.use_graph = 1 if use_graph else 0
+        cfg.concurrent_batches = int(concurrent_batches)
         cfg.device = int(device)
         self.cfg = cfg
         self.indicator = True if indicator is None else bool(indicator)

@@ -103,8 +103,9 @@ struct ep_ctx {
 
   // segment-Adam variant tuner: the first training steps rotate through the supported
   // values-per-lane variants (identical results) and time the scatter with events
-  int seg_vpl = 0;                 // chosen variant, 0 while tuning
-  std::vector<int> tune_cands;
+  int seg_vpl = 0;                 // chosen variant (values per lane, unroll), 0 while tuning
+  int seg_unroll = 0;
+  std::vector<std::pair<int, int>> tune_cands;
   std::vector<hipEvent_t> tune_ev;  // two per tuning step
   uint32_t tune_i = 0;
 
@@ -149,6 +150,9 @@ uint32_t neighbour_cap(const ep_ctx* c) {
   return c->cfg.max_nodes_kind == EP_SAMPLE_FIXED ? c->cfg.max_nodes : c->deg_max;
 }
 
+// Anchors per training step: concurrent_batches reference batches.
+uint32_t group_anchors(const ep_ctx* c) { return c->cfg.batch_size * std::max<uint32_t>(1u, c->cfg.concurrent_batches); }
+
 // Worst-case feature occurrences of one example: F_max * (1 + S + K).
 uint64_t example_rows_max(const ep_ctx* c) {
   return (uint64_t)c->f_max * (1ull + neighbour_cap(c) + c->cfg.negatives);
@@ -184,14 +188,17 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   c->release(b.adj); c->release(b.coef); c->release(b.elig); c->release(b.loss);
   c->release(b.keys[0]); c->release(b.keys[1]); c->release(b.slots[0]); c->release(b.slots[1]);
   c->release(b.ghist); c->release(b.G); c->release(b.ufeat); c->release(b.long_st); c->release(b.long_len);
-  c->release(b.long_pbase); c->release(b.piece_seg); c->release(b.part); c->release(c->d_explicit);
+  c->release(b.long_pbase); c->release(b.piece_st); c->release(b.piece_info); c->release(b.part);
+  c->release(c->d_explicit);
   c->release(c->d_xout);
   if (c->step_exec) { (void)hipGraphExecDestroy(c->step_exec); c->step_exec = nullptr; c->step_exec_ready = false; }
 
-  const uint64_t unique_cap = std::min<uint64_t>(c->n_vocab, keycap) + 1;
-  // long segments are > 128 entries; pieces hold 128 entries (see LONG_SEG / PIECE)
-  const uint32_t long_cap = (uint32_t)(keycap / 128 + 1);
-  const uint32_t piece_cap = (uint32_t)(keycap / 128 + long_cap + 1);
+  const uint32_t P = std::max<uint32_t>(1u, cfg.concurrent_batches);
+  // merged-gradient records (test hook): one per (row, batch) run
+  const uint64_t unique_cap = std::min<uint64_t>((uint64_t)c->n_vocab * P, keycap) + 1;
+  // long rows have > kSegShort entries; each run of one is cut into pieces of kPieceEntries
+  const uint32_t long_cap = (uint32_t)(keycap / (ep::kSegShort + 1) + 1);
+  const uint32_t piece_cap = (uint32_t)(keycap / ep::kPieceEntries + (uint64_t)long_cap * P + 1);
   // radix-sort grid: ~8K keys per block at capacity, a multiple of 256 in [256, 2048]
   const int sort_grid = (int)std::min<uint64_t>(2048, std::max<uint64_t>(256, (keycap / 8192 + 255) / 256 * 256));
   EP_TRY(c->alloc(&b.nbr_cnt, batch));
@@ -213,7 +220,8 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   EP_TRY(c->alloc(&b.long_st, long_cap));
   EP_TRY(c->alloc(&b.long_len, long_cap));
   EP_TRY(c->alloc(&b.long_pbase, long_cap));
-  EP_TRY(c->alloc(&b.piece_seg, piece_cap));
+  EP_TRY(c->alloc(&b.piece_st, piece_cap));
+  EP_TRY(c->alloc(&b.piece_info, piece_cap));
   EP_TRY(c->alloc(&b.part, (size_t)piece_cap * D));
   EP_TRY(c->alloc(&c->d_explicit, batch));
   if (!b.Gd && cfg.scatter == EP_SCATTER_ATOMIC) {
@@ -252,20 +260,43 @@ void refresh_base(ep_ctx* c) {
   b.fids = c->d_fids;
   b.n_nodes = c->n_nodes;
   b.n_vocab = c->n_vocab;
+  // row part: a permutation of [0, 2^bits - 1) (> every row id); batch part: pbits
   int bits = 1;
-  while ((1ull << bits) - 1ull < (uint64_t)c->n_vocab) ++bits;  // sentinel = 2^bits - 1 >= n_vocab
-  b.sort_bits = bits;
-  b.sentinel = (uint32_t)((1ull << bits) - 1ull);
+  while ((1ull << bits) - 1ull < (uint64_t)c->n_vocab) ++bits;
+  const uint32_t P = std::max<uint32_t>(1u, cfg.concurrent_batches);
+  uint32_t pbits = 0;
+  while ((1u << pbits) < P) ++pbits;
+  b.pbits = pbits;
+  b.pmask = (1u << pbits) - 1u;
+  b.sort_bits = bits + (int)pbits;
+  b.sentinel = (uint32_t)((1ull << b.sort_bits) - 1ull);
+  const uint64_t m = (1ull << bits) - 1ull;
+  b.key_m = (uint32_t)m;
+  b.key_bits = (uint32_t)bits;
+  uint64_t a = m > 2 ? 0x9E3779B1ull % m : 1ull;  // multiplier coprime with m
+  auto gcd = [](uint64_t x, uint64_t y) { while (y) { const uint64_t t = x % y; x = y; y = t; } return x; };
+  while (m > 2 && (a < 2 || gcd(a, m) != 1)) a = (a + 1) % m;
+  // inverse of a modulo m (extended Euclid)
+  int64_t t0 = 0, t1 = 1, r0 = (int64_t)m, r1 = (int64_t)(a % std::max<uint64_t>(m, 1));
+  while (r1 != 0) {
+    const int64_t q = r0 / r1;
+    int64_t tmp = t0 - q * t1; t0 = t1; t1 = tmp;
+    tmp = r0 - q * r1; r0 = r1; r1 = tmp;
+  }
+  if (t0 < 0) t0 += (int64_t)m;
+  b.key_a = (uint32_t)a;
+  b.key_ainv = m > 1 ? (uint32_t)t0 : 1u;
+  b.rbatch = cfg.batch_size;
   b.table = c->d_table;
   {
     const uint32_t D = cfg.dims;
-    const bool bf = cfg.precision == EP_PREC_BF16;
-    b.ostride = bf ? 3 * D : 2 * D;
-    b.master = (bf && c->d_ostate) ? c->d_ostate : nullptr;
-    b.mom = c->d_ostate ? c->d_ostate + (bf ? D : 0) : nullptr;
-    b.var = c->d_ostate ? c->d_ostate + (bf ? 2 * D : D) : nullptr;
+    const bool ms = cfg.precision == EP_PREC_BF16_MASTER;
+    b.ostride = ms ? 3 * D : 2 * D;
+    b.master = (ms && c->d_ostate) ? c->d_ostate : nullptr;
+    b.mom = c->d_ostate ? c->d_ostate + (ms ? D : 0) : nullptr;
+    b.var = c->d_ostate ? c->d_ostate + (ms ? 2 * D : D) : nullptr;
   }
-  b.bf16 = cfg.precision == EP_PREC_BF16 ? 1 : 0;
+  b.bf16 = cfg.precision != EP_PREC_FP32 ? 1 : 0;
   b.dims = (int)cfg.dims;
   b.K = cfg.negatives;
   b.gamma = cfg.gamma;
@@ -289,7 +320,10 @@ int launch_training_step(ep_ctx* c, const ep::StepConfig& sc0, bool update, floa
                          int* sorted_buf = nullptr, hipEvent_t ev_s0 = nullptr) {
   hipStream_t s = c->stream;
   ep::StepConfig sc = sc0;
-  if (sc.seg_vpl == 0) sc.seg_vpl = c->seg_vpl ? c->seg_vpl : 4;
+  if (sc.seg_vpl == 0) {
+    sc.seg_vpl = c->seg_vpl ? c->seg_vpl : 4;
+    sc.seg_unroll = c->seg_vpl ? c->seg_unroll : 8;
+  }
   if (lr_override >= 0.f) EP_HIP(ep::launch_step_begin_lr(sc, c->d_state, lr_override, s));
   else EP_HIP(ep::launch_step_begin(sc, c->d_state, s));
   EP_HIP(ep::launch_sample(sc, c->d_state, s));
@@ -391,8 +425,9 @@ ep::StepConfig train_config(ep_ctx* c) {
   sc.n_negpool = (uint32_t)c->train.size();
   sc.n_batch_total = (uint32_t)c->train.size();
   const uint32_t B = c->cfg.batch_size;
-  sc.steps_per_pass = (uint32_t)((c->train.size() + B - 1) / B);
-  sc.total_updates = sc.steps_per_pass * c->cfg.passes;
+  const uint32_t GA = group_anchors(c);
+  sc.steps_per_pass = (uint32_t)((c->train.size() + GA - 1) / GA);  // launches per pass
+  sc.total_updates = (uint32_t)((c->train.size() + B - 1) / B) * c->cfg.passes;  // reference batches
   sc.pass_loss = c->d_pass_loss;
   return sc;
 }
@@ -407,9 +442,12 @@ bool tuning(const ep_ctx* c) { return c->cfg.scatter == EP_SCATTER_SORT && c->se
 int tune_step(ep_ctx* c, const ep::StepConfig& sc) {
   if (c->tune_cands.empty()) {
     for (int v : {16, 8, 4})
-      if (ep::seg_vpl_supported((int)c->cfg.dims, v)) c->tune_cands.push_back(v);
+      for (int u : {8 / (v / 4), 16 / (v / 4)})
+        if (ep::seg_variant_supported((int)c->cfg.dims, v, u)) c->tune_cands.push_back({v, u});
     if (c->tune_cands.size() <= 1) {
-      c->seg_vpl = c->tune_cands.empty() ? 4 : c->tune_cands[0];
+      const std::pair<int, int> only = c->tune_cands.empty() ? std::make_pair(4, 8) : c->tune_cands[0];
+      c->seg_vpl = only.first;
+      c->seg_unroll = only.second;
       return launch_training_step(c, sc, true, -1.f);
     }
     const size_t ne = 2 * c->tune_cands.size() * kTuneRounds;
@@ -419,12 +457,13 @@ int tune_step(ep_ctx* c, const ep::StepConfig& sc) {
   const uint32_t n = (uint32_t)c->tune_cands.size();
   const uint32_t i = c->tune_i++;
   ep::StepConfig t = sc;
-  t.seg_vpl = c->tune_cands[i % n];
+  t.seg_vpl = c->tune_cands[i % n].first;
+  t.seg_unroll = c->tune_cands[i % n].second;
   EP_TRY(launch_training_step(c, t, true, -1.f, nullptr, nullptr, c->tune_ev[2 * i + 1], nullptr, c->tune_ev[2 * i]));
   if (c->tune_i < n * kTuneRounds) return EP_OK;
   EP_HIP(hipEventSynchronize(c->tune_ev.back()));
   float best = INFINITY;
-  int pick = c->tune_cands.back();
+  std::pair<int, int> pick = c->tune_cands.back();
   for (uint32_t k = 0; k < n; ++k) {
     std::vector<float> ms;
     for (uint32_t r = 1; r < kTuneRounds; ++r) {
@@ -437,7 +476,8 @@ int tune_step(ep_ctx* c, const ep::StepConfig& sc) {
     const float med = ms[ms.size() / 2];
     if (med < best) { best = med; pick = c->tune_cands[k]; }
   }
-  c->seg_vpl = pick;
+  c->seg_vpl = pick.first;
+  c->seg_unroll = pick.second;
   return EP_OK;
 }
 
@@ -474,14 +514,22 @@ int validate_config(const ep_config* cfg) {
   if (cfg->noise != 0.f) return fail(EP_EINVAL, "gradient noise is not supported");
   if (cfg->weighted_neighbor_sampling || cfg->weighted_neighbor_averaging)
     return fail(EP_EINVAL, "weighted neighbour sampling/averaging is not supported");
-  if (cfg->precision != EP_PREC_FP32 && cfg->precision != EP_PREC_BF16) return fail(EP_EINVAL, "precision");
+  if (cfg->precision != EP_PREC_FP32 && cfg->precision != EP_PREC_BF16 && cfg->precision != EP_PREC_BF16_MASTER)
+    return fail(EP_EINVAL, "precision");
   if (cfg->grad_mode != EP_GRAD_FIRST_LEAF && cfg->grad_mode != EP_GRAD_EXACT) return fail(EP_EINVAL, "grad_mode");
   if (cfg->scatter != EP_SCATTER_SORT && cfg->scatter != EP_SCATTER_ATOMIC) return fail(EP_EINVAL, "scatter");
   if (cfg->loss_weighting < EP_LW_NONE || cfg->loss_weighting > EP_LW_EXPONENTIAL)
     return fail(EP_EINVAL, "loss_weighting");
   if (!(cfg->valid_pct >= 0.f && cfg->valid_pct < 1.f)) return fail(EP_EINVAL, "valid_pct must be in [0, 1)");
-  if (cfg->seg_vpl != 0 && !ep::seg_vpl_supported((int)cfg->dims, cfg->seg_vpl))
-    return fail(EP_EINVAL, "seg_vpl must be 0 or a values-per-lane variant (4, 8, 16) dividing dims");
+  if (cfg->concurrent_batches != 1 && cfg->concurrent_batches != 2 && cfg->concurrent_batches != 4 &&
+      cfg->concurrent_batches != 8)
+    return fail(EP_EINVAL, "concurrent_batches must be 1, 2, 4 or 8");
+  if (cfg->concurrent_batches > 1 && cfg->scatter != EP_SCATTER_SORT)
+    return fail(EP_EINVAL, "concurrent_batches > 1 needs the sort scatter");
+  if ((cfg->seg_vpl == 0) != (cfg->seg_unroll == 0) ||
+      (cfg->seg_vpl != 0 && !ep::seg_variant_supported((int)cfg->dims, cfg->seg_vpl, cfg->seg_unroll)))
+    return fail(EP_EINVAL, "seg_vpl/seg_unroll must both be 0 (tuned) or a built variant: vpl in {4,8,16} "
+                           "dividing dims, unroll in {8,16}/(vpl/4)");
   return EP_OK;
 }
 
@@ -510,6 +558,7 @@ int ep_config_default(ep_config* cfg) {
   cfg->use_graph = 0;
   cfg->seed = 20222022ull;
   cfg->device = 0;
+  cfg->concurrent_batches = 1;
   return EP_OK;
 }
 
@@ -524,6 +573,7 @@ int ep_create(const ep_config* cfg, ep_ctx** out) {
   ep_ctx* c = new ep_ctx();
   c->cfg = *cfg;
   c->seg_vpl = cfg->seg_vpl;  // 0: tuned during the first training steps
+  c->seg_unroll = cfg->seg_unroll;
   c->device = cfg->device;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete c; return fail(EP_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
@@ -617,9 +667,10 @@ int ep_set_table(ep_ctx* c, const float* rows) {
   if (!c->n_vocab) return fail(EP_ESTATE, "features must be set first");
   EP_HIP(hipSetDevice(c->device));
   const size_t n = (size_t)c->n_vocab * c->cfg.dims;
-  const bool bf16 = c->cfg.precision == EP_PREC_BF16;
+  const bool bf16 = c->cfg.precision != EP_PREC_FP32;
+  const bool master = c->cfg.precision == EP_PREC_BF16_MASTER;
   const uint32_t D = c->cfg.dims;
-  const size_t ostride = bf16 ? 3 * (size_t)D : 2 * (size_t)D;
+  const size_t ostride = master ? 3 * (size_t)D : 2 * (size_t)D;
   if (!c->d_table) {
     if (bf16) {
       uint16_t* t = nullptr;
@@ -632,18 +683,22 @@ int ep_set_table(ep_ctx* c, const float* rows) {
     }
     EP_TRY(c->alloc(&c->d_ostate, (size_t)c->n_vocab * ostride));
   }
-  EP_HIP(hipMemsetAsync(c->d_ostate, 0, (size_t)c->n_vocab * ostride * sizeof(float), c->stream));
   refresh_base(c);
-  if (rows) {
-    if (bf16) {
-      EP_HIP(hipMemcpy2DAsync(c->d_ostate, ostride * sizeof(float), rows, D * sizeof(float), D * sizeof(float),
-                              c->n_vocab, hipMemcpyHostToDevice, c->stream));
-      EP_HIP(ep::launch_fill_bf16(c->d_ostate, (uint16_t*)c->d_table, n, D, (uint32_t)ostride, c->stream));
-    } else {
-      EP_HIP(hipMemcpyAsync(c->d_table, rows, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    }
+  if (rows && master) {
+    // FP32 rows into the master slots of the records, bf16 copy rounded from them
+    EP_HIP(hipMemsetAsync(c->d_ostate, 0, (size_t)c->n_vocab * ostride * sizeof(float), c->stream));
+    EP_HIP(hipMemcpy2DAsync(c->d_ostate, ostride * sizeof(float), rows, D * sizeof(float), D * sizeof(float),
+                            c->n_vocab, hipMemcpyHostToDevice, c->stream));
+    EP_HIP(ep::launch_fill_bf16(c->d_ostate, (uint16_t*)c->d_table, n, D, (uint32_t)ostride, c->stream));
+  } else if (rows && bf16) {
+    // stage the FP32 rows in the (not yet used) optimizer records, round them, then clear
+    EP_HIP(hipMemcpyAsync(c->d_ostate, rows, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    EP_HIP(ep::launch_fill_bf16(c->d_ostate, (uint16_t*)c->d_table, n, D, D, c->stream));
+    EP_HIP(hipMemsetAsync(c->d_ostate, 0, (size_t)c->n_vocab * ostride * sizeof(float), c->stream));
   } else {
-    EP_HIP(ep::launch_init_table(c->base, c->cfg.seed, c->stream));
+    EP_HIP(hipMemsetAsync(c->d_ostate, 0, (size_t)c->n_vocab * ostride * sizeof(float), c->stream));
+    if (rows) EP_HIP(hipMemcpyAsync(c->d_table, rows, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    else EP_HIP(ep::launch_init_table(c->base, c->cfg.seed, c->stream));
   }
   EP_HIP(hipStreamSynchronize(c->stream));
   c->table_ready = true;
@@ -657,17 +712,26 @@ int ep_get_table(ep_ctx* c, float* rows) {
   const size_t n = (size_t)c->n_vocab * c->cfg.dims;
   const uint32_t D = c->cfg.dims;
   EP_HIP(hipStreamSynchronize(c->stream));
-  if (c->cfg.precision == EP_PREC_BF16)
+  if (c->cfg.precision == EP_PREC_BF16_MASTER) {
     EP_HIP(hipMemcpy2D(rows, D * sizeof(float), c->d_ostate, 3 * D * sizeof(float), D * sizeof(float), c->n_vocab,
                        hipMemcpyDeviceToHost));
-  else
+  } else if (c->cfg.precision == EP_PREC_BF16) {
+    // the bf16 table is the parameter: widen it (exact)
+    std::vector<uint16_t> h(n);
+    EP_HIP(hipMemcpy(h.data(), c->d_table, n * sizeof(uint16_t), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) {
+      const uint32_t u = (uint32_t)h[i] << 16;
+      std::memcpy(rows + i, &u, sizeof(float));
+    }
+  } else {
     EP_HIP(hipMemcpy(rows, c->d_table, n * sizeof(float), hipMemcpyDeviceToHost));
+  }
   return EP_OK;
 }
 
 int ep_get_table_bf16(ep_ctx* c, uint16_t* rows) {
   if (!c || !rows) return fail(EP_EINVAL, "NULL argument");
-  if (!c->table_ready || c->cfg.precision != EP_PREC_BF16) return fail(EP_ESTATE, "no bf16 table");
+  if (!c->table_ready || c->cfg.precision == EP_PREC_FP32) return fail(EP_ESTATE, "no bf16 table");
   EP_HIP(hipSetDevice(c->device));
   EP_HIP(hipStreamSynchronize(c->stream));
   EP_HIP(hipMemcpy(rows, c->d_table, (size_t)c->n_vocab * c->cfg.dims * sizeof(uint16_t), hipMemcpyDeviceToHost));
@@ -679,12 +743,12 @@ int ep_get_adam(ep_ctx* c, float* m, float* v) {
   if (!c->table_ready) return fail(EP_ESTATE, "no table");
   EP_HIP(hipSetDevice(c->device));
   const uint32_t D = c->cfg.dims;
-  const bool bf = c->cfg.precision == EP_PREC_BF16;
-  const size_t ostride = (bf ? 3 : 2) * (size_t)D;
+  const bool ms = c->cfg.precision == EP_PREC_BF16_MASTER;
+  const size_t ostride = (ms ? 3 : 2) * (size_t)D;
   EP_HIP(hipStreamSynchronize(c->stream));
-  EP_HIP(hipMemcpy2D(m, D * sizeof(float), c->d_ostate + (bf ? D : 0), ostride * sizeof(float), D * sizeof(float),
+  EP_HIP(hipMemcpy2D(m, D * sizeof(float), c->d_ostate + (ms ? D : 0), ostride * sizeof(float), D * sizeof(float),
                      c->n_vocab, hipMemcpyDeviceToHost));
-  EP_HIP(hipMemcpy2D(v, D * sizeof(float), c->d_ostate + (bf ? 2 * D : D), ostride * sizeof(float),
+  EP_HIP(hipMemcpy2D(v, D * sizeof(float), c->d_ostate + (ms ? 2 * D : D), ostride * sizeof(float),
                      D * sizeof(float), c->n_vocab, hipMemcpyDeviceToHost));
   return EP_OK;
 }
@@ -714,11 +778,12 @@ int ep_learn(ep_ctx* c, ep_progress_fn progress, void* user) {
   } else if (!c->d_train) {
     EP_TRY(upload_split(c));
   }
-  EP_TRY(ensure_step_buffers(c, c->cfg.batch_size));
+  EP_TRY(ensure_step_buffers(c, group_anchors(c)));
   refresh_base(c);
   ep::StepConfig sc = train_config(c);
   const uint32_t B = c->cfg.batch_size;
   const uint32_t spp = sc.steps_per_pass;
+  const uint32_t nbatch = (uint32_t)((c->train.size() + B - 1) / B);  // reference batches per pass
   HostRng rng(c->cfg.seed ^ 0x5851F42D4C957F2Dull);
   uint64_t next_step = 1;
   std::vector<float> losses(c->train.size());
@@ -732,7 +797,7 @@ int ep_learn(ep_ctx* c, ep_progress_fn progress, void* user) {
   vc.train = 0;
   vc.seed = c->cfg.seed - 1ull;
   vc.pass_loss = c->d_valid_loss;
-  vc.steps_per_pass = (uint32_t)((c->valid.size() + B - 1) / B);
+  vc.steps_per_pass = (uint32_t)((c->valid.size() + vc.batch - 1) / vc.batch);
   vc.total_updates = 1;
   for (uint32_t pass = 1; pass <= c->cfg.passes; ++pass) {
     EP_TRY(upload_pass_perm(c, rng));
@@ -742,7 +807,7 @@ int ep_learn(ep_ctx* c, ep_progress_fn progress, void* user) {
     EP_HIP(hipMemcpy(losses.data(), c->d_pass_loss, losses.size() * sizeof(float), hipMemcpyDeviceToHost));
     // per-batch error = sum of contributing losses / batch size; pass error = mean over batches
     double err = 0.0;
-    for (uint32_t s = 0; s < spp; ++s) {
+    for (uint32_t s = 0; s < nbatch; ++s) {
       const size_t lo = (size_t)s * B, hi = std::min(losses.size(), lo + B);
       float e = 0.f;
       for (size_t i = lo; i < hi; ++i) e += losses[i];
@@ -750,7 +815,7 @@ int ep_learn(ep_ctx* c, ep_progress_fn progress, void* user) {
       if (std::isinf(be)) be = 0.f;
       err += be;
     }
-    const float train_err = spp ? (float)(err / spp) : 0.f;
+    const float train_err = nbatch ? (float)(err / nbatch) : 0.f;
     float valid_err = NAN;
     if (!c->valid.empty()) {
       EP_TRY(set_pass(c, c->d_vstate, 0, nullptr));
@@ -768,7 +833,7 @@ int ep_learn(ep_ctx* c, ep_progress_fn progress, void* user) {
     if (progress) {
       ep::StepState hs;
       EP_HIP(hipMemcpy(&hs, c->d_state, sizeof(hs), hipMemcpyDeviceToHost));
-      progress(user, pass, hs.next_step, train_err, valid_err, hs.lr);
+      progress(user, pass, hs.next_step, train_err, valid_err, hs.lr[hs.n_batches ? hs.n_batches - 1 : 0]);
     }
   }
   EP_HIP(hipStreamSynchronize(c->stream));
@@ -783,7 +848,7 @@ int ep_sample_batch(ep_ctx* c, const uint32_t* anchors, uint32_t n, uint32_t pas
   EP_HIP(hipSetDevice(c->device));
   if (!c->split_set) { EP_TRY(ensure_split(c)); EP_TRY(upload_split(c)); }
   else if (!c->d_train) EP_TRY(upload_split(c));
-  EP_TRY(ensure_step_buffers(c, std::max(n, c->cfg.batch_size)));
+  EP_TRY(ensure_step_buffers(c, std::max(n, group_anchors(c))));
   refresh_base(c);
   ep::StepConfig sc = c->base;
   sc.perm = c->d_explicit;
@@ -791,6 +856,7 @@ int ep_sample_batch(ep_ctx* c, const uint32_t* anchors, uint32_t n, uint32_t pas
   sc.n_negpool = (uint32_t)c->train.size();
   sc.n_batch_total = n;
   sc.batch = n;
+  sc.rbatch = n;
   EP_HIP(hipMemcpy(c->d_explicit, anchors, n * sizeof(uint32_t), hipMemcpyHostToDevice));
   EP_TRY(set_pass(c, c->d_vstate, pass, nullptr));
   EP_HIP(ep::launch_step_begin_lr(sc, c->d_vstate, 0.f, c->stream));
@@ -820,7 +886,12 @@ int ep_step_explicit(ep_ctx* c, const uint32_t* anchors, uint32_t n, const uint3
   }
   EP_HIP(hipSetDevice(c->device));
   if (!c->table_ready) EP_TRY(ep_set_table(c, nullptr));
-  EP_TRY(ensure_step_buffers(c, std::max(n, c->cfg.batch_size)));
+  // concurrent_batches == 1: the n anchors are one batch; else batches of batch_size,
+  // at most concurrent_batches of them
+  const uint32_t P = std::max<uint32_t>(1u, c->cfg.concurrent_batches);
+  if (P > 1 && (n + c->cfg.batch_size - 1) / c->cfg.batch_size > P)
+    return fail(EP_EINVAL, "ep_step_explicit: more than concurrent_batches batches of batch_size anchors");
+  EP_TRY(ensure_step_buffers(c, std::max(n, group_anchors(c))));
   refresh_base(c);
   if (cons && !c->d_xout) EP_TRY(c->alloc(&c->d_xout, (size_t)c->cap_batch * NC * D));
   ep::StepConfig sc = c->base;
@@ -829,6 +900,7 @@ int ep_step_explicit(ep_ctx* c, const uint32_t* anchors, uint32_t n, const uint3
   sc.n_negpool = n;
   sc.n_batch_total = n;
   sc.batch = n;
+  sc.rbatch = P > 1 ? c->cfg.batch_size : n;
   sc.explicit_samples = 1;
   sc.xout = cons ? c->d_xout : nullptr;
   EP_HIP(hipMemcpy(c->d_explicit, anchors, n * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -885,7 +957,7 @@ int ep_bench_steps(ep_ctx* c, uint32_t warmup, uint32_t steps, int instrument, e
   if (!c->table_ready) EP_TRY(ep_set_table(c, nullptr));
   if (!c->split_set) { EP_TRY(ensure_split(c)); EP_TRY(upload_split(c)); }
   else if (!c->d_train) EP_TRY(upload_split(c));
-  EP_TRY(ensure_step_buffers(c, c->cfg.batch_size));
+  EP_TRY(ensure_step_buffers(c, group_anchors(c)));
   refresh_base(c);
   ep::StepConfig sc = train_config(c);
   sc.pass_loss = c->d_pass_loss;
@@ -941,6 +1013,7 @@ int ep_bench_steps(ep_ctx* c, uint32_t warmup, uint32_t steps, int instrument, e
   out->nodes = h1.stat_nodes - h0.stat_nodes;
   out->unique_rows = h1.stat_unique - h0.stat_unique;
   out->seg_vpl = c->seg_vpl;
+  out->seg_unroll = c->seg_unroll;
   if (instrument) {
     // per-kernel timing: the same number of steps launched one by one with events
     std::vector<hipEvent_t> ev(3 * steps);

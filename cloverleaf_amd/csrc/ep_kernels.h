@@ -6,6 +6,11 @@
 
 namespace ep {
 
+// A step processes a group of up to kMaxGroup reference batches that read the same
+// parameters (the reference's concurrently running Hogwild batches, mod.rs:174 par_iter
+// over chunks); their updates are applied one after the other in batch order.
+constexpr uint32_t kMaxGroup = 8;
+
 // Device-resident per-step state.  Written by k_step_begin / k_scan / the scatter
 // kernels; read by every kernel of the step, so one captured step graph can be
 // replayed for every batch of a pass.
@@ -25,7 +30,8 @@ struct StepState {
   uint32_t n_pieces;       // pieces of long segments this step
   uint32_t n_touched;      // rows touched by the atomic path this step
   uint32_t err;            // nonzero: key capacity exceeded this step
-  float lr;                // learning rate of this step
+  uint32_t n_batches;      // reference batches in this step's group (<= kMaxGroup)
+  float lr[8];             // learning rate of each batch of the group (kMaxGroup)
   float bc1, bc2;          // Adam bias corrections 1 - beta^t, t = pass + 1
   uint32_t n_real;         // non-sentinel keys (the sorted key count)
   uint32_t dtot[4][2048];  // radix-sort digit totals, one row per sort pass (<= 11-bit digits)
@@ -40,13 +46,21 @@ struct StepConfig {
   const uint32_t* fids;
   uint64_t n_nodes;
   uint32_t n_vocab;
-  uint32_t sentinel;       // key for "no gradient" (>= n_vocab, all ones in sort_bits)
+  // Gradient key = (perm(f) << pbits) | batch, with perm(f) = f * key_a mod key_m
+  // (key_m = 2^(sort_bits - pbits) - 1 > every row id, gcd(key_a, key_m) = 1, key_ainv
+  // the inverse of key_a mod key_m) and batch the index of the example's batch in the
+  // step's group.  The permutation interleaves hot and cold rows in the sorted order, so
+  // 64-entry chunks of the sorted keys hold similar numbers of segments; a row's batches
+  // stay adjacent and in batch order.  sentinel = all ones in sort_bits (perm < key_m).
+  uint32_t sentinel;       // key for "no gradient", above every real key
   int sort_bits;
+  uint32_t key_m, key_a, key_ainv, key_bits;  // key_m = 2^key_bits - 1
+  uint32_t pbits, pmask;   // batch bits of a key: log2(group size), group size - 1
   // parameters
   void* table;             // float* (fp32) or uint16_t* (bf16)
   // Optimizer state: one record of `ostride` floats per row, [master |] m | v, so an
-  // update touches one contiguous record (master only in bf16 mode).
-  float* master;           // record + 0 (bf16 mode) else nullptr
+  // update touches one contiguous record (master only with a persistent FP32 master).
+  float* master;           // record + 0 (persistent-master bf16 mode) else nullptr
   float* mom;              // record + 0 (fp32) / + D (bf16)
   float* var;              // record + D (fp32) / + 2D (bf16)
   uint32_t ostride;        // 2D (fp32) or 3D (bf16)
@@ -57,7 +71,8 @@ struct StepConfig {
   const uint32_t* negpool; // negatives drawn uniformly from this pool
   uint32_t n_negpool;
   uint32_t n_batch_total;  // anchors in the pass
-  uint32_t batch;          // B
+  uint32_t batch;          // anchors per step: group size x rbatch (buffer capacity)
+  uint32_t rbatch;         // B, the reference batch size
   uint32_t S;              // neighbour cap (Fixed) ; 0 with all_nbrs
   int all_nbrs;            // 1: use every neighbour (Sample::All)
   uint32_t K;              // negatives
@@ -97,10 +112,14 @@ struct StepConfig {
   // scatter (sort path)
   float* G;                // [key_cap*D] compact gradient rows (bounded by unique count)
   uint32_t* ufeat;         // [key_cap]
+  // Long rows: first sorted entry, first piece and piece count.  A piece is at most
+  // kPieceEntries entries of one (row, batch) run: start entry and info = len |
+  // batch << 8 | (last piece of the run) << 16.
   uint32_t* long_st;       // [long_cap]
-  uint32_t* long_len;
+  uint32_t* long_len;      // pieces of the row
   uint32_t* long_pbase;
-  uint32_t* piece_seg;     // [piece_cap]
+  uint32_t* piece_st;      // [piece_cap]
+  uint32_t* piece_info;    // [piece_cap]
   float* part;             // [piece_cap*D]
   uint32_t long_cap, piece_cap;
   // scatter (atomic path)
@@ -108,9 +127,11 @@ struct StepConfig {
   uint32_t* flag;          // [V]
   uint32_t* touched;       // [V]
   int scatter;             // 0 sort, 1 atomic
-  int grad_out;            // test hook: write merged gradients (G/ufeat) instead of updating
+  int grad_out;            // test hook: write merged gradients (G/ufeat) instead of updating;
+                           // ufeat = row | batch << 27
   int unique_grid;         // grid size for the persistent row kernels
   int seg_vpl;             // k_seg_adam values per lane (4, 8, 16; chosen by the tuner)
+  int seg_unroll;          // k_seg_adam adjoint rows in flight per lane group (tuned)
 };
 
 // Launchers (all asynchronous on `stream`).
@@ -134,6 +155,12 @@ hipError_t launch_embed_averaged(const StepConfig& c, const uint64_t* findptr, c
                                  uint64_t n_nodes, float* out, hipStream_t s);
 
 bool dims_supported(int d);
+// Segments of the sorted keys longer than kSegShort entries are summed as pieces of
+// kPieceEntries entries by k_pieces and combined in piece order by k_long_combine.
+constexpr uint32_t kSegShort = 32;
+constexpr uint32_t kPieceEntries = 32;
 bool seg_vpl_supported(int d, int vpl);
+// Whether k_seg_adam<D, VPL, U> is built (U in {8, 16} / (VPL/4)).
+bool seg_variant_supported(int d, int vpl, int unroll);
 
 }  // namespace ep

@@ -15,8 +15,7 @@
 namespace ep {
 
 static constexpr int NW = 4;             // waves per 256-thread block
-static constexpr uint32_t LONG_SEG = 128;  // segments longer than this are split into pieces
-static constexpr uint32_t PIECE = 128;     // entries per piece of a long segment
+static constexpr uint32_t PIECE = kPieceEntries;  // entries per piece of a long segment
 static constexpr int SR_UNROLL = 16;       // adjoint rows in flight per wave in the scatter
 static constexpr uint32_t HASH_EMPTY = 0xFFFFFFFFu;
 static constexpr uint32_t HASH_FBITS = 27;
@@ -25,6 +24,45 @@ static constexpr uint32_t HASH_FMASK = (1u << HASH_FBITS) - 1u;
 // ------------------------------------------------------------------------------------
 // Helpers
 // ------------------------------------------------------------------------------------
+
+// Gradient key of row f for batch `bidx` of the group, and the row of a key (see
+// StepConfig: the row part is a multiplicative permutation modulo key_m).
+// x * y mod m for x, y < m = 2^b - 1 (x * y < 2^2b): two folds and one subtraction.
+__device__ inline uint32_t mul_mod_mersenne(uint32_t x, uint32_t y, uint32_t m, uint32_t b) {
+  uint64_t p = (uint64_t)x * y;
+  p = (p & m) + (p >> b);  // < 2^(b+1)
+  p = (p & m) + (p >> b);  // <= m + 1
+  return (uint32_t)(p >= m ? p - m : p);
+}
+__device__ inline uint32_t key_of_row(const StepConfig& c, uint32_t f, uint32_t bidx) {
+  return (mul_mod_mersenne(f, c.key_a, c.key_m, c.key_bits) << c.pbits) | bidx;
+}
+__device__ inline uint32_t row_of_key(const StepConfig& c, uint32_t k) {
+  return mul_mod_mersenne(k >> c.pbits, c.key_ainv, c.key_m, c.key_bits);
+}
+
+// First index i in [lo, hi) with K[i] >= x (hi if none), K sorted; 64-ary search by
+// the whole wave.
+__device__ inline uint32_t wave_lower_bound(const uint32_t* __restrict__ K, uint32_t lo, uint32_t hi, uint32_t x) {
+  const uint32_t lane = lane_id();
+  while (hi > lo) {
+    const uint32_t span = hi - lo;
+    const uint32_t step = (span + 63u) / 64u;
+    const uint64_t off = (uint64_t)lane * step;
+    const bool valid = off < span;
+    const uint32_t kv = valid ? K[lo + (uint32_t)off] : 0u;
+    const uint64_t m = __ballot(valid && kv >= x);
+    if (step == 1u) return m ? lo + (uint32_t)(__ffsll((unsigned long long)m) - 1) : hi;
+    if (m == 0ull) {
+      lo = lo + ((span - 1u) / step) * step + 1u;  // past the last sample (all < x)
+      continue;
+    }
+    const uint32_t f = (uint32_t)(__ffsll((unsigned long long)m) - 1);
+    hi = lo + f * step;                        // this sample is >= x
+    lo = f > 0u ? lo + (f - 1u) * step + 1u : lo;  // the one before is < x
+  }
+  return lo;
+}
 
 // Accumulate rows table[fids[fs..fe)] into acc (lane slice), U row loads in flight.
 template <int D, int U, typename T>
@@ -135,6 +173,21 @@ __device__ inline uint32_t hash_lookup(const uint32_t* h, uint32_t mask, uint32_
 // k_step_begin: advance the batch cursor, LR schedule and Adam bias corrections.
 // LRScheduler::cos_decay (scheduler.rs:39-47) as built at mod.rs:132-137, in f32.
 // ------------------------------------------------------------------------------------
+__device__ inline float scheduled_lr(const StepConfig& c, uint64_t cur) {
+  const float alpha = c.alpha;
+  const float mn = alpha / 100.0f;
+  const uint64_t warm = (uint64_t)((float)c.total_updates / 5.0f);
+  if (cur > warm) {
+    const float ratio = (float)cur / (float)c.total_updates;
+    return mn + 0.5f * (alpha - mn) * (1.0f + cosf(3.14159265358979323846f * ratio));
+  }
+  const float ratio = (float)cur / (float)warm;
+  return mn + ratio * (alpha - mn);
+}
+
+// The group of a step covers up to c.batch anchors, cut into reference batches of
+// c.rbatch; batch k of the group takes step number next_step + k (mod.rs:234 fetch_add
+// per batch) and its scheduled learning rate.
 __global__ void k_step_begin(StepConfig c, StepState* st, float lr_override) {
   uint32_t* dt = &st->dtot[0][0];
   for (int i = threadIdx.x; i < 4 * 2048; i += blockDim.x) dt[i] = 0u;
@@ -145,25 +198,15 @@ __global__ void k_step_begin(StepConfig c, StepState* st, float lr_override) {
     const uint32_t base = (uint32_t)(base64 < c.n_batch_total ? base64 : c.n_batch_total);
     st->anchor_base = base;
     const uint32_t rem = c.n_batch_total - base;
-    st->batch_n = rem < c.batch ? rem : c.batch;
+    const uint32_t bn = rem < c.batch ? rem : c.batch;
+    st->batch_n = bn;
+    uint32_t nb = (bn + c.rbatch - 1u) / c.rbatch;
+    if (nb > kMaxGroup) nb = kMaxGroup;
+    st->n_batches = nb;
     const uint64_t cur = st->next_step;
-    st->next_step = cur + 1u;
-    float lr;
-    if (lr_override >= 0.f) {
-      lr = lr_override;
-    } else {
-      const float alpha = c.alpha;
-      const float mn = alpha / 100.0f;
-      const uint64_t warm = (uint64_t)((float)c.total_updates / 5.0f);
-      if (cur > warm) {
-        const float ratio = (float)cur / (float)c.total_updates;
-        lr = mn + 0.5f * (alpha - mn) * (1.0f + cosf(3.14159265358979323846f * ratio));
-      } else {
-        const float ratio = (float)cur / (float)warm;
-        lr = mn + ratio * (alpha - mn);
-      }
-    }
-    st->lr = lr;
+    st->next_step = cur + nb;
+    for (uint32_t k = 0; k < kMaxGroup; ++k)
+      st->lr[k] = k < nb ? (lr_override >= 0.f ? lr_override : scheduled_lr(c, cur + k)) : 0.f;
     const float t = (float)st->pass + 1.0f;  // optimizer.rs:50: t = pass + 1
     st->bc1 = 1.0f - powf(0.9f, t);
     st->bc2 = 1.0f - powf(0.999f, t);
@@ -173,7 +216,7 @@ __global__ void k_step_begin(StepConfig c, StepState* st, float lr_override) {
     st->n_long = 0u;
     st->n_pieces = 0u;
     st->n_touched = 0u;
-    st->stat_examples += st->batch_n;
+    st->stat_examples += bn;
   }
 }
 
@@ -607,7 +650,7 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
     const uint32_t ci = cons_of(o);
     const uint32_t f = ids[o];
     uint32_t key = c.sentinel;
-    if (s_el[ci] && (!first_leaf || hash_lookup(hsh, hmask, f) == ci)) key = f;
+    if (s_el[ci] && (!first_leaf || hash_lookup(hsh, hmask, f) == ci)) key = key_of_row(c, f, b / c.rbatch);
     c.keys[0][kb + o] = key;
     c.slots[0][kb + o] = b * NC + ci;
   }
@@ -681,7 +724,7 @@ __global__ __launch_bounds__(256) void k_emit(StepConfig c, StepState* st) {
           if (ok) hash_insert_min(hsh, hmask, f, cj);
         } else {
           uint32_t key = c.sentinel;
-          if (ok && (!first_leaf || hash_lookup(hsh, hmask, f) == cj)) key = f;
+          if (ok && (!first_leaf || hash_lookup(hsh, hmask, f) == cj)) key = key_of_row(c, f, b / c.rbatch);
           const uint32_t pos = kbase + running + o;
           c.keys[0][pos] = key;
           c.slots[0][pos] = slot0 + cj;
@@ -856,9 +899,10 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t* __restrict__
 // ------------------------------------------------------------------------------------
 // Sparse lazy Adam (optimizer.rs:43-81): skip the row if its gradient has any NaN
 // (line 56); m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2;
-// theta -= lr (m/bc1) / (sqrt(v/bc2) + eps).  bf16 mode: the FP32 master row is updated
-// and written back to the bf16 table with stochastic rounding (Philox bits keyed by
-// row, column and step).
+// theta -= lr (m/bc1) / (sqrt(v/bc2) + eps).  bf16 modes: theta is taken to FP32 (from
+// the persistent master when there is one, else from the bf16 row itself), updated, and
+// written back to the bf16 table with stochastic rounding (hash bits keyed by row,
+// column and step).
 // ------------------------------------------------------------------------------------
 __device__ inline void adam_math(float& th, float& m, float& v, float g, float lr, float bc1, float bc2) {
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
@@ -869,17 +913,28 @@ __device__ inline void adam_math(float& th, float& m, float& v, float g, float l
   th -= lr * (mh / (sqrtf(vh) + eps));
 }
 
+// Per-step scalars of the update, read once per kernel (before any store, so the
+// compiler keeps them in registers instead of reloading them after each store).
+struct StepScalars {
+  float lr0;         // learning rate of the group's first batch
+  float bc1, bc2;
+  uint32_t sr_step;  // step number keying the stochastic-rounding bits
+};
+__device__ inline StepScalars step_scalars(const StepState* st) {
+  return StepScalars{st->lr[0], st->bc1, st->bc2, (uint32_t)st->next_step};
+}
+
 // 16 stochastic-rounding bits for column 4*col4 + q of row f at this step.
-__device__ inline uint32_t sr_bits(const StepConfig& c, const StepState* st, uint32_t f, uint32_t col4, int q) {
-  const uint32_t r0 = sr_key(f, col4, (uint32_t)st->next_step, (uint32_t)c.seed);
+__device__ inline uint32_t sr_bits(const StepConfig& c, const StepScalars& sv, uint32_t f, uint32_t col4, int q) {
+  const uint32_t r0 = sr_key(f, col4, sv.sr_step, (uint32_t)c.seed);
   const uint32_t w = (q >> 1) == 0 ? r0 : hash32(r0 ^ 0x27D4EB2Fu);
   return (q & 1) ? (w >> 16) : (w & 0xffffu);
 }
 
 // Whole-wave row layout (lane owns RowShape<D>::PL values).
 template <int D>
-__device__ inline void adam_row(const StepConfig& c, const StepState* st, uint32_t f, const float (&g)[RowShape<D>::PL],
-                                int lane) {
+__device__ inline void adam_row(const StepConfig& c, const StepScalars& sv, uint32_t f,
+                                const float (&g)[RowShape<D>::PL], int lane) {
   constexpr int PL = RowShape<D>::PL;
   const int base = lane * PL;
   bool bad = false;
@@ -891,19 +946,19 @@ __device__ inline void adam_row(const StepConfig& c, const StepState* st, uint32
   float m[PL], v[PL], th[PL];
   load_slice<D>(c.mom + ooff, lane, m);
   load_slice<D>(c.var + ooff, lane, v);
-  if (c.bf16) load_slice<D>(c.master + ooff, lane, th);
+  if (c.master) load_slice<D>(c.master + ooff, lane, th);
+  else if (c.bf16) load_slice<D>((const uint16_t*)c.table + off, lane, th);
   else load_slice<D>((const float*)c.table + off, lane, th);
-  const float lr = st->lr, bc1 = st->bc1, bc2 = st->bc2;
 #pragma unroll
-  for (int q = 0; q < PL; ++q) adam_math(th[q], m[q], v[q], g[q], lr, bc1, bc2);
+  for (int q = 0; q < PL; ++q) adam_math(th[q], m[q], v[q], g[q], sv.lr0, sv.bc1, sv.bc2);
   store_slice<D>(c.mom + ooff, lane, m);
   store_slice<D>(c.var + ooff, lane, v);
   if (c.bf16) {
-    store_slice<D>(c.master + ooff, lane, th);
+    if (c.master) store_slice<D>(c.master + ooff, lane, th);
     uint16_t* row = (uint16_t*)c.table + off;
 #pragma unroll
     for (int q = 0; q < PL; ++q)
-      if (base + q < D) row[base + q] = f32_to_bf16_sr(th[q], sr_bits(c, st, f, (uint32_t)((base + q) >> 2), (base + q) & 3));
+      if (base + q < D) row[base + q] = f32_to_bf16_sr(th[q], sr_bits(c, sv, f, (uint32_t)((base + q) >> 2), (base + q) & 3));
   } else {
     store_slice<D>((float*)c.table + off, lane, th);
   }
@@ -911,25 +966,55 @@ __device__ inline void adam_row(const StepConfig& c, const StepState* st, uint32
 
 // ------------------------------------------------------------------------------------
 // Fused segment reduce + Adam over the sorted keys (the batch merge of mod.rs:223-233
-// followed by the update).  A wave takes a 64-entry chunk and owns every segment that
-// starts in it.  Lane groups of L = D/VPL lanes (VPL values per lane, as VPL/4 float4
-// columns strided by L) each take one segment at a time, G = 64/L segments in parallel:
-// the adjoint rows of the segment and the row's theta/m/v are loaded together, then the
-// Adam update is applied.  Larger VPL keeps more segments (and loads) in flight per wave;
-// every VPL adds in the same order, so the variants give identical bits and the tuner
-// picks the fastest.  A chunk's last segment that does not end within the next 64
-// entries is deferred as a long segment (split into PIECE-entry pieces, combined in
-// order).  grad_out != 0 writes the merged gradient rows to G/ufeat instead (test hook).
+// followed by the update of each batch of the group, in batch order).  The sorted keys
+// form rows (same row id) made of runs (same row and batch).  A wave takes a 64-entry
+// chunk and owns every row that starts in it.  Rows of at most kSegShort entries are
+// handled here: lane groups of L = D/VPL lanes (VPL values per lane, as VPL/4 float4
+// columns strided by L) each take one row, G = 64/L rows in parallel: the row's
+// theta/m/v are loaded once, each run is reduced (U adjoint rows in flight per group)
+// and applied as one Adam step with its batch's learning rate, and the row is stored
+// once.  Longer rows (and a last row running past the 128-entry window) are registered
+// as long rows, cut into pieces of at most kPieceEntries entries of one run: k_pieces
+// sums the pieces in parallel and k_long_combine adds them in order and updates.  Every
+// (VPL, U) adds each run's rows in entry order, so the variants give identical bits and
+// the tuner picks the fastest.  grad_out != 0 writes each run's merged gradient to
+// G/ufeat (ufeat = row | batch << 27) instead of updating (test hook).
 // ------------------------------------------------------------------------------------
-template <int D, int VPL>
+
+// Registers long row `idx` with first piece `pb`: one piece record per kPieceEntries
+// entries of each run.  run_at(j) gives run j's [start, end) (absolute) and batch.
+template <typename RunAt>
+__device__ inline void write_pieces(const StepConfig& c, uint32_t pb, uint32_t nruns, RunAt run_at, uint32_t lane0,
+                                    uint32_t stride) {
+  uint32_t p = pb;
+  for (uint32_t j = 0; j < nruns; ++j) {
+    uint32_t a, z, bq;
+    run_at(j, &a, &z, &bq);
+    const uint32_t np = (z - a + PIECE - 1u) / PIECE;
+    for (uint32_t q = lane0; q < np; q += stride) {
+      const uint32_t ps = a + q * PIECE;
+      const uint32_t len = z - ps < PIECE ? z - ps : PIECE;
+      if (p + q < c.piece_cap) {
+        c.piece_st[p + q] = ps;
+        c.piece_info[p + q] = len | (bq << 8) | ((q + 1u == np) ? (1u << 16) : 0u);
+      }
+    }
+    p += np;
+  }
+}
+
+template <int D, int VPL, int U>
 __global__ __launch_bounds__(256) void k_seg_adam(StepConfig c, StepState* st, const uint32_t* __restrict__ Ks,
                                                   const uint32_t* __restrict__ Ss) {
   static_assert(D % VPL == 0 && VPL % 4 == 0 && D / VPL <= 64, "float4 lane slices");
   constexpr int L = D / VPL;      // lanes per row
-  constexpr int G = 64 / L;       // segments in flight per wave
+  constexpr int G = 64 / L;       // rows in flight per wave
   constexpr int NQ = VPL / 4;     // float4 columns per lane
-  constexpr int U = NQ >= 8 ? 1 : 8 / NQ;  // adjoint rows in flight per group
-  __shared__ uint32_t s_pos[NW][66];
+  __shared__ uint32_t s_run[NW][132];  // run starts (chunk-relative); s_run[nrun] = window end
+  __shared__ uint8_t s_bat[NW][132];   // batch of each run
+  __shared__ uint32_t s_row[NW][66];   // first run of each owned row; s_row[nrow] = nrun
+  __shared__ uint32_t s_sr[NW][64];    // short rows: first run | end run << 8
+  __shared__ float s_lr[kMaxGroup];    // learning rate of each batch of the group
   __shared__ unsigned long long s_cnt;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int grp = lane / L, lg = lane % L;
@@ -939,9 +1024,10 @@ __global__ __launch_bounds__(256) void k_seg_adam(StepConfig c, StepState* st, c
   const uint32_t gw = blockIdx.x * NW + w;
   const uint32_t nwaves = gridDim.x * NW;
   const uint32_t n = st->n_keys <= c.key_cap ? st->n_real : 0u;  // sorted, sentinels dropped
-  const uint32_t SENT = c.sentinel;
-  const float lr = st->lr, bc1 = st->bc1, bc2 = st->bc2;
+  const uint32_t SENT = c.sentinel, PB = c.pbits, PM = c.pmask;
+  const StepScalars sv = step_scalars(st);
   if (tid == 0) s_cnt = 0ull;
+  if (tid < (int)kMaxGroup) s_lr[tid] = st->lr[tid];
   __syncthreads();
   uint32_t processed = 0;
   for (uint64_t chunk = gw; chunk * 64u < n; chunk += nwaves) {
@@ -954,132 +1040,239 @@ __global__ __launch_bounds__(256) void k_seg_adam(StepConfig c, StepState* st, c
     if (__shfl(k0, 0, EP_WAVE) == SENT) break;  // sorted: the rest is sentinel
     uint32_t prev = __shfl_up(k0, 1, EP_WAVE);
     if (lane == 0) prev = i0 > 0u ? Ks[i0 - 1u] : ~k0;
-    const bool is_start = (k0 != SENT) && (k0 != prev);
+    const uint32_t last0 = __shfl(k0, 63, EP_WAVE);
+    uint32_t prev1 = __shfl_up(k1, 1, EP_WAVE);
+    if (lane == 0) prev1 = last0;
+    const uint32_t r0 = k0 >> PB, r1 = k1 >> PB;  // row parts (SENT's is above every row)
+    const bool is_start = (k0 != SENT) && (r0 != (prev >> PB));
     const uint64_t starts = __ballot(is_start);
     if (starts == 0ull) continue;
-    const uint32_t ns = (uint32_t)__popcll(starts);
-    if (is_start) s_pos[w][__popcll(starts & lt)] = (uint32_t)lane;
+    const int first = __ffsll((unsigned long long)starts) - 1;
     const int ql = 63 - __clzll((unsigned long long)starts);
-    const uint32_t fl = __shfl(k0, ql, EP_WAVE);
-    const uint64_t m0 = __ballot(lane > ql && k0 != fl);
-    const uint64_t m1 = __ballot(k1 != fl);
-    uint32_t nproc = ns;
-    if (m0) {
-      if (lane == 0) s_pos[w][ns] = (uint32_t)(__ffsll((unsigned long long)m0) - 1);
-    } else if (m1) {
-      if (lane == 0) s_pos[w][ns] = 64u + (uint32_t)(__ffsll((unsigned long long)m1) - 1);
-    } else {
-      // the last segment runs past the 128-entry window: defer it as a long segment
-      nproc = ns - 1u;
-      const uint32_t st0 = i0 + (uint32_t)ql;
-      const uint32_t end = wave_upper_bound(Ks, i0 + 128u, n, fl);
-      const uint32_t len = end - st0;
-      const uint32_t pcs = (len + PIECE - 1u) / PIECE;
+    const uint32_t fl = __shfl(r0, ql, EP_WAVE);
+    const uint64_t m0 = __ballot(lane > ql && r0 != fl);
+    const uint64_t m1 = __ballot(r1 != fl);
+    const bool deferred = (m0 == 0ull) && (m1 == 0ull);
+    const uint32_t wend = m0 ? (uint32_t)(__ffsll((unsigned long long)m0) - 1)
+                             : (m1 ? 64u + (uint32_t)(__ffsll((unsigned long long)m1) - 1) : (uint32_t)ql);
+    // runs of the rows handled in the window: [first, wend)
+    const uint64_t fs0 = __ballot(lane >= first && (uint32_t)lane < wend && k0 != prev);
+    const uint64_t fs1 = __ballot(64u + (uint32_t)lane < wend && k1 != prev1);
+    const uint32_t n0 = (uint32_t)__popcll(fs0);
+    const uint32_t nrun = n0 + (uint32_t)__popcll(fs1);
+    if ((fs0 >> lane) & 1ull) {
+      const uint32_t i = (uint32_t)__popcll(fs0 & lt);
+      s_run[w][i] = (uint32_t)lane;
+      s_bat[w][i] = (uint8_t)(k0 & PM);
+    }
+    if ((fs1 >> lane) & 1ull) {
+      const uint32_t i = n0 + (uint32_t)__popcll(fs1 & lt);
+      s_run[w][i] = 64u + (uint32_t)lane;
+      s_bat[w][i] = (uint8_t)(k1 & PM);
+    }
+    const uint64_t rowm = deferred ? (starts & ~(1ull << ql)) : starts;
+    const uint32_t nrow = (uint32_t)__popcll(rowm);
+    if ((rowm >> lane) & 1ull) s_row[w][__popcll(rowm & lt)] = (uint32_t)__popcll(fs0 & lt);
+    if (lane == 0) {
+      s_run[w][nrun] = wend;
+      s_row[w][nrow] = nrun;
+    }
+    if (deferred) {
+      // the last row runs past the window: find its runs by search, register it as long
+      const uint32_t lo = i0 + (uint32_t)ql;
+      uint32_t bnd[kMaxGroup + 1];
+      bnd[0] = lo;
+      for (uint32_t b = 0; b <= PM; ++b) bnd[b + 1] = wave_lower_bound(Ks, bnd[b], n, ((fl << PB) | b) + 1u);
+      uint32_t pcs = 0, nr = 0;
+      for (uint32_t b = 0; b <= PM; ++b) {
+        if (bnd[b + 1] > bnd[b]) ++nr;
+        pcs += (bnd[b + 1] - bnd[b] + PIECE - 1u) / PIECE;
+      }
       uint32_t idx = 0, pb = 0;
       if (lane == 0) {
         idx = atomicAdd(&st->n_long, 1u);
         pb = atomicAdd(&st->n_pieces, pcs);
         if (idx < c.long_cap) {
-          c.long_st[idx] = st0;
-          c.long_len[idx] = len;
+          c.long_st[idx] = lo;
+          c.long_len[idx] = pcs;
           c.long_pbase[idx] = pb;
         }
       }
       idx = __shfl(idx, 0, EP_WAVE);
       pb = __shfl(pb, 0, EP_WAVE);
-      if (idx < c.long_cap)
-        for (uint32_t q = lane; q < pcs; q += 64u)
-          if (pb + q < c.piece_cap) c.piece_seg[pb + q] = idx;
+      if (idx < c.long_cap) {
+        auto run_at = [&](uint32_t j, uint32_t* a, uint32_t* z, uint32_t* bq) {
+          uint32_t k = 0, seen = 0;  // j-th nonempty batch run
+          for (uint32_t b = 0; b <= PM; ++b)
+            if (bnd[b + 1] > bnd[b]) { if (seen == j) k = b; ++seen; }
+          *a = bnd[k];
+          *z = bnd[k + 1];
+          *bq = k;
+        };
+        write_pieces(c, pb, nr, run_at, (uint32_t)lane, 64u);
+      }
     }
     __builtin_amdgcn_wave_barrier();
-    processed += nproc;
-    for (uint32_t r0 = 0; r0 < nproc; r0 += (uint32_t)G) {
-      const uint32_t j = r0 + (uint32_t)grp;
-      const bool has = gact && j < nproc;
-      const uint32_t sa = has ? s_pos[w][j] : 0u;
-      const uint32_t sb = has ? s_pos[w][j + 1] : 0u;
-      const uint32_t f = __shfl(k0, (int)sa, EP_WAVE);
-      const uint32_t len = sb - sa;
+    // classify the owned rows: short ones stay here, longer ones become long rows
+    const bool own = (uint32_t)lane < nrow;
+    const uint32_t ra = own ? s_row[w][lane] : 0u;
+    const uint32_t rz = own ? s_row[w][lane + 1] : 0u;
+    const uint32_t ea = own ? s_run[w][ra] : 0u;
+    const uint32_t ez = own ? s_run[w][rz] : 0u;
+    const bool lng = own && (ez - ea) > kSegShort;
+    const uint64_t lmask = __ballot(lng);
+    const uint64_t smask = __ballot(own && !lng);
+    if (own && !lng) s_sr[w][__popcll(smask & lt)] = ra | (rz << 8);
+    if (lmask) {
+      uint32_t pcs = 0;
+      if (lng)
+        for (uint32_t j = ra; j < rz; ++j) pcs += (s_run[w][j + 1] - s_run[w][j] + PIECE - 1u) / PIECE;
+      uint32_t incl = pcs;  // inclusive prefix sum of the piece counts
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(incl, off, EP_WAVE);
+        if (lane >= off) incl += t;
+      }
+      const uint32_t tot = __shfl(incl, 63, EP_WAVE);
+      uint32_t idx0 = 0, pb0 = 0;
+      if (lane == 0) {
+        idx0 = atomicAdd(&st->n_long, (uint32_t)__popcll(lmask));
+        pb0 = atomicAdd(&st->n_pieces, tot);
+      }
+      idx0 = __shfl(idx0, 0, EP_WAVE);
+      pb0 = __shfl(pb0, 0, EP_WAVE);
+      if (lng) {
+        const uint32_t idx = idx0 + (uint32_t)__popcll(lmask & lt);
+        const uint32_t pb = pb0 + incl - pcs;
+        if (idx < c.long_cap) {
+          c.long_st[idx] = i0 + ea;
+          c.long_len[idx] = pcs;
+          c.long_pbase[idx] = pb;
+          auto run_at = [&](uint32_t j, uint32_t* a, uint32_t* z, uint32_t* bq) {
+            *a = i0 + s_run[w][ra + j];
+            *z = i0 + s_run[w][ra + j + 1];
+            *bq = s_bat[w][ra + j];
+          };
+          write_pieces(c, pb, rz - ra, run_at, 0u, 1u);
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t nsr = (uint32_t)__popcll(smask);
+    processed += nsr;
+    for (uint32_t q0 = 0; q0 < nsr; q0 += (uint32_t)G) {
+      const uint32_t j = q0 + (uint32_t)grp;
+      const bool has = gact && j < nsr;
+      const uint32_t pk = has ? s_sr[w][j] : 0u;
+      const uint32_t fa = pk & 0xffu, fz = pk >> 8;  // runs [fa, fz)
+      const uint32_t f = row_of_key(c, __shfl(k0, (int)(has ? s_run[w][fa] : 0u), EP_WAVE));
       // Adam operands first: independent of the reduction, so their latency overlaps it
-      float4 m4[NQ], v4[NQ], t4[NQ], acc[NQ];
+      float4 m4[NQ], v4[NQ], t4[NQ];
       const size_t roff = (size_t)f * D;
       const size_t ooff = (size_t)f * c.ostride;  // optimizer record [master |] m | v
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         const uint32_t col = (uint32_t)(q * L + lg) * 4u;
-        acc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
-        m4[q] = v4[q] = t4[q] = acc[q];
+        m4[q] = v4[q] = t4[q] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (has && !c.grad_out) {
           m4[q] = *reinterpret_cast<const float4*>(c.mom + ooff + col);
           v4[q] = *reinterpret_cast<const float4*>(c.var + ooff + col);
-          t4[q] = c.bf16 ? *reinterpret_cast<const float4*>(c.master + ooff + col)
-                         : *reinterpret_cast<const float4*>((const float*)c.table + roff + col);
-        }
-      }
-      uint32_t maxlen = len;
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) maxlen = max(maxlen, (uint32_t)__shfl_xor(maxlen, off, EP_WAVE));
-      for (uint32_t u0 = 0; u0 < maxlen; u0 += U) {
-        float4 r[U][NQ];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const uint32_t t = sa + u0 + (uint32_t)u;  // chunk-relative entry, < 128
-          const bool valid = has && (u0 + (uint32_t)u < len);
-          const uint32_t src = t & 63u;
-          const uint32_t sl0 = __shfl(s0, (int)src, EP_WAVE);
-          const uint32_t sl1 = __shfl(s1, (int)src, EP_WAVE);
-          const uint32_t sl = t < 64u ? sl0 : sl1;
-#pragma unroll
-          for (int q = 0; q < NQ; ++q) {
-            const uint32_t col = (uint32_t)(q * L + lg) * 4u;
-            r[u][q] = valid ? *reinterpret_cast<const float4*>(c.adj + (size_t)sl * D + col)
-                            : make_float4(0.f, 0.f, 0.f, 0.f);
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-#pragma unroll
-          for (int q = 0; q < NQ; ++q) {
-            acc[q].x += r[u][q].x; acc[q].y += r[u][q].y; acc[q].z += r[u][q].z; acc[q].w += r[u][q].w;
+          if (c.master) {
+            t4[q] = *reinterpret_cast<const float4*>(c.master + ooff + col);
+          } else if (c.bf16) {
+            const uint2 b = *reinterpret_cast<const uint2*>((const uint16_t*)c.table + roff + col);
+            t4[q] = make_float4(__uint_as_float(b.x << 16), __uint_as_float(b.x & 0xffff0000u),
+                                __uint_as_float(b.y << 16), __uint_as_float(b.y & 0xffff0000u));
+          } else {
+            t4[q] = *reinterpret_cast<const float4*>((const float*)c.table + roff + col);
           }
         }
       }
-      if (c.grad_out) {
-        uint32_t ub = 0;
-        const uint64_t heads = __ballot(has && lg == 0);
-        if (lane == 0 && heads) ub = atomicAdd(&st->n_unique, (uint32_t)__popcll(heads));
-        ub = __shfl(ub, 0, EP_WAVE);
-        const uint32_t u = ub + (uint32_t)__popcll(heads & ((grp * L == 0) ? 0ull : (~0ull >> (64 - grp * L))));
-        if (has) {
+      uint32_t maxruns = fz - fa;
 #pragma unroll
-          for (int q = 0; q < NQ; ++q)
-            *reinterpret_cast<float4*>(c.G + (size_t)u * D + (q * L + lg) * 4) = acc[q];
-          if (lg == 0) c.ufeat[u] = f;
+      for (int off = 32; off > 0; off >>= 1) maxruns = max(maxruns, (uint32_t)__shfl_xor(maxruns, off, EP_WAVE));
+      bool upd = false;
+      for (uint32_t rq = 0; rq < maxruns; ++rq) {
+        const bool hr = has && rq < fz - fa;
+        const uint32_t sa = hr ? s_run[w][fa + rq] : 0u;
+        const uint32_t sb = hr ? s_run[w][fa + rq + 1] : 0u;
+        const uint32_t bq = hr ? s_bat[w][fa + rq] : 0u;
+        const uint32_t len = sb - sa;
+        float4 acc[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) acc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+        uint32_t maxlen = len;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) maxlen = max(maxlen, (uint32_t)__shfl_xor(maxlen, off, EP_WAVE));
+        for (uint32_t u0 = 0; u0 < maxlen; u0 += U) {
+          float4 r[U][NQ];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const uint32_t t = sa + u0 + (uint32_t)u;  // chunk-relative entry
+            const bool valid = hr && (u0 + (uint32_t)u < len);
+            const uint32_t src = t & 63u;
+            const uint32_t sl0 = __shfl(s0, (int)src, EP_WAVE);
+            const uint32_t sl1 = __shfl(s1, (int)src, EP_WAVE);
+            const uint32_t sl = t < 64u ? sl0 : sl1;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+              const uint32_t col = (uint32_t)(q * L + lg) * 4u;
+              r[u][q] = valid ? *reinterpret_cast<const float4*>(c.adj + (size_t)sl * D + col)
+                              : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+              acc[q].x += r[u][q].x; acc[q].y += r[u][q].y; acc[q].z += r[u][q].z; acc[q].w += r[u][q].w;
+            }
+          }
         }
-        continue;
-      }
-      bool bad = false;
+        if (c.grad_out) {
+          uint32_t ub = 0;
+          const uint64_t heads = __ballot(hr && lg == 0);
+          if (lane == 0 && heads) ub = atomicAdd(&st->n_unique, (uint32_t)__popcll(heads));
+          ub = __shfl(ub, 0, EP_WAVE);
+          const uint32_t u = ub + (uint32_t)__popcll(heads & ((grp * L == 0) ? 0ull : (~0ull >> (64 - grp * L))));
+          if (hr) {
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) bad |= isnan(acc[q].x) || isnan(acc[q].y) || isnan(acc[q].z) || isnan(acc[q].w);
-      const bool skip = (__ballot(has && bad) & gmask) != 0ull;
-      if (has && !skip) {
+            for (int q = 0; q < NQ; ++q)
+              *reinterpret_cast<float4*>(c.G + (size_t)u * D + (q * L + lg) * 4) = acc[q];
+            if (lg == 0) c.ufeat[u] = f | (bq << 27);
+          }
+          continue;
+        }
+        bool bad = false;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) bad |= isnan(acc[q].x) || isnan(acc[q].y) || isnan(acc[q].z) || isnan(acc[q].w);
+        const bool skip = (__ballot(hr && bad) & gmask) != 0ull;  // optimizer.rs:56, per batch
+        if (hr && !skip) {
+          const float lr = s_lr[bq];
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            adam_math(t4[q].x, m4[q].x, v4[q].x, acc[q].x, lr, sv.bc1, sv.bc2);
+            adam_math(t4[q].y, m4[q].y, v4[q].y, acc[q].y, lr, sv.bc1, sv.bc2);
+            adam_math(t4[q].z, m4[q].z, v4[q].z, acc[q].z, lr, sv.bc1, sv.bc2);
+            adam_math(t4[q].w, m4[q].w, v4[q].w, acc[q].w, lr, sv.bc1, sv.bc2);
+          }
+          upd = true;
+        }
+      }
+      if (upd) {
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
           const uint32_t col = (uint32_t)(q * L + lg) * 4u;
-          adam_math(t4[q].x, m4[q].x, v4[q].x, acc[q].x, lr, bc1, bc2);
-          adam_math(t4[q].y, m4[q].y, v4[q].y, acc[q].y, lr, bc1, bc2);
-          adam_math(t4[q].z, m4[q].z, v4[q].z, acc[q].z, lr, bc1, bc2);
-          adam_math(t4[q].w, m4[q].w, v4[q].w, acc[q].w, lr, bc1, bc2);
           *reinterpret_cast<float4*>(c.mom + ooff + col) = m4[q];
           *reinterpret_cast<float4*>(c.var + ooff + col) = v4[q];
           if (c.bf16) {
-            *reinterpret_cast<float4*>(c.master + ooff + col) = t4[q];
-            const uint32_t r0 = sr_key(f, col >> 2, (uint32_t)st->next_step, (uint32_t)c.seed);
-            const uint32_t r1 = hash32(r0 ^ 0x27D4EB2Fu);
-            uint2 pk;
-            pk.x = (uint32_t)f32_to_bf16_sr(t4[q].x, r0 & 0xffffu) | ((uint32_t)f32_to_bf16_sr(t4[q].y, r0 >> 16) << 16);
-            pk.y = (uint32_t)f32_to_bf16_sr(t4[q].z, r1 & 0xffffu) | ((uint32_t)f32_to_bf16_sr(t4[q].w, r1 >> 16) << 16);
-            *reinterpret_cast<uint2*>((uint16_t*)c.table + roff + col) = pk;
+            if (c.master) *reinterpret_cast<float4*>(c.master + ooff + col) = t4[q];
+            const uint32_t h0 = sr_key(f, col >> 2, sv.sr_step, (uint32_t)c.seed);
+            const uint32_t h1 = hash32(h0 ^ 0x27D4EB2Fu);
+            uint2 pk2;
+            pk2.x = (uint32_t)f32_to_bf16_sr(t4[q].x, h0 & 0xffffu) | ((uint32_t)f32_to_bf16_sr(t4[q].y, h0 >> 16) << 16);
+            pk2.y = (uint32_t)f32_to_bf16_sr(t4[q].z, h1 & 0xffffu) | ((uint32_t)f32_to_bf16_sr(t4[q].w, h1 >> 16) << 16);
+            *reinterpret_cast<uint2*>((uint16_t*)c.table + roff + col) = pk2;
           } else {
             *reinterpret_cast<float4*>((float*)c.table + roff + col) = t4[q];
           }
@@ -1092,7 +1285,7 @@ __global__ __launch_bounds__(256) void k_seg_adam(StepConfig c, StepState* st, c
   if (tid == 0 && s_cnt) atomicAdd((unsigned long long*)&st->stat_unique, s_cnt);
 }
 
-// Partial sums of the pieces of long segments.
+// Partial sums of the pieces of long rows (each piece: <= kPieceEntries entries of one run).
 template <int D>
 __global__ __launch_bounds__(256) void k_pieces(StepConfig c, StepState* st, const uint32_t* __restrict__ Ss) {
   constexpr int PL = RowShape<D>::PL;
@@ -1102,32 +1295,25 @@ __global__ __launch_bounds__(256) void k_pieces(StepConfig c, StepState* st, con
   const uint32_t nwaves = gridDim.x * NW;
   const uint32_t np = st->n_pieces < c.piece_cap ? st->n_pieces : c.piece_cap;
   for (uint32_t gp = gw; gp < np; gp += nwaves) {
-    const uint32_t idx = c.piece_seg[gp];
-    if (idx >= c.long_cap) continue;
-    const uint32_t s0 = c.long_st[idx] + (gp - c.long_pbase[idx]) * PIECE;
-    const uint32_t segend = c.long_st[idx] + c.long_len[idx];
-    const uint32_t s1 = s0 + PIECE < segend ? s0 + PIECE : segend;
+    const uint32_t s0 = c.piece_st[gp];
+    const uint32_t m = c.piece_info[gp] & 0xffu;  // <= PIECE <= 64
+    const uint32_t sj = (uint32_t)lane < m ? Ss[s0 + lane] : 0u;
     float acc[PL];
 #pragma unroll
     for (int q = 0; q < PL; ++q) acc[q] = 0.f;
-    for (uint32_t g0 = s0; g0 < s1; g0 += 64u) {
-      const uint32_t jj = g0 + lane;
-      const uint32_t sj = jj < s1 ? Ss[jj] : 0u;
-      const int m = (int)((s1 - g0) < 64u ? (s1 - g0) : 64u);
-      for (int t0 = 0; t0 < m; t0 += U) {
-        float r[U][PL];
+    for (uint32_t t0 = 0; t0 < m; t0 += U) {
+      float r[U][PL];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int t = (t0 + u < m) ? (t0 + u) : (m - 1);
-          const uint32_t s = __shfl(sj, t, EP_WAVE);
-          load_slice<D>(c.adj + (size_t)s * D, lane, r[u]);
-        }
+      for (int u = 0; u < U; ++u) {
+        const uint32_t t = (t0 + u < m) ? (t0 + u) : (m - 1u);
+        const uint32_t s = __shfl(sj, (int)t, EP_WAVE);
+        load_slice<D>(c.adj + (size_t)s * D, lane, r[u]);
+      }
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          if (t0 + u < m) {
+      for (int u = 0; u < U; ++u) {
+        if (t0 + u < m) {
 #pragma unroll
-            for (int q = 0; q < PL; ++q) acc[q] += r[u][q];
-          }
+          for (int q = 0; q < PL; ++q) acc[q] += r[u][q];
         }
       }
     }
@@ -1135,50 +1321,98 @@ __global__ __launch_bounds__(256) void k_pieces(StepConfig c, StepState* st, con
   }
 }
 
-// Combine the pieces of each long segment in piece order (deterministic), then update.
+// One wave per long row: loads theta/m/v once, adds the pieces of each run in order and
+// applies one Adam step per run (its batch's learning rate), stores the row once.
 template <int D>
 __global__ __launch_bounds__(256) void k_long_combine(StepConfig c, StepState* st, const uint32_t* __restrict__ Ks) {
   constexpr int PL = RowShape<D>::PL;
-  __shared__ float red[NW][D];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  constexpr int U = SR_UNROLL;
+  const int lane = threadIdx.x & 63;
   const int base = lane * PL;
+  const uint32_t gw = blockIdx.x * NW + (threadIdx.x >> 6);
+  const uint32_t nwaves = gridDim.x * NW;
   const uint32_t nl = st->n_long < c.long_cap ? st->n_long : c.long_cap;
-  for (uint32_t idx = blockIdx.x; idx < nl; idx += gridDim.x) {
+  const StepScalars sv = step_scalars(st);
+  __shared__ float s_lr[kMaxGroup];
+  if (threadIdx.x < kMaxGroup) s_lr[threadIdx.x] = st->lr[threadIdx.x];
+  __syncthreads();
+  uint32_t done = 0;
+  for (uint32_t idx = gw; idx < nl; idx += nwaves) {
     const uint32_t pb = c.long_pbase[idx];
-    const uint32_t pcs = (c.long_len[idx] + PIECE - 1u) / PIECE;
+    uint32_t pcs = c.long_len[idx];
+    if (pb >= c.piece_cap) pcs = 0u;
+    else if (pcs > c.piece_cap - pb) pcs = c.piece_cap - pb;
+    const uint32_t f = row_of_key(c, Ks[c.long_st[idx]]);
+    const size_t off = (size_t)f * D;
+    const size_t ooff = (size_t)f * c.ostride;
+    float m[PL], v[PL], th[PL];
+    if (!c.grad_out) {
+      load_slice<D>(c.mom + ooff, lane, m);
+      load_slice<D>(c.var + ooff, lane, v);
+      if (c.master) load_slice<D>(c.master + ooff, lane, th);
+      else if (c.bf16) load_slice<D>((const uint16_t*)c.table + off, lane, th);
+      else load_slice<D>((const float*)c.table + off, lane, th);
+    }
     float acc[PL];
 #pragma unroll
     for (int q = 0; q < PL; ++q) acc[q] = 0.f;
-    for (uint32_t j = w; j < pcs; j += NW) {
-      if (pb + j >= c.piece_cap) break;
-      float r[PL];
-      load_slice<D>(c.part + (size_t)(pb + j) * D, lane, r);
+    bool upd = false;
+    for (uint32_t j0 = 0; j0 < pcs; j0 += U) {
+      float r[U][PL];
+      uint32_t inf[U];
 #pragma unroll
-      for (int q = 0; q < PL; ++q) acc[q] += r[q];
-    }
+      for (int u = 0; u < U; ++u) {
+        const uint32_t j = (j0 + u < pcs) ? j0 + u : pcs - 1u;
+        load_slice<D>(c.part + (size_t)(pb + j) * D, lane, r[u]);
+        inf[u] = c.piece_info[pb + j];
+      }
 #pragma unroll
-    for (int q = 0; q < PL; ++q)
-      if (base + q < D) red[w][base + q] = acc[q];
-    __syncthreads();
-    if (w == 0) {
-      float g[PL];
+      for (int u = 0; u < U; ++u) {
+        if (j0 + u < pcs) {
 #pragma unroll
-      for (int q = 0; q < PL; ++q)
-        g[q] = (base + q < D) ? (((red[0][base + q] + red[1][base + q]) + red[2][base + q]) + red[3][base + q]) : 0.f;
-      const uint32_t f = Ks[c.long_st[idx]];
-      if (c.grad_out) {
-        uint32_t u = 0;
-        if (lane == 0) u = atomicAdd(&st->n_unique, 1u);
-        u = __shfl(u, 0, EP_WAVE);
-        store_slice<D>(c.G + (size_t)u * D, lane, g);
-        if (lane == 0) c.ufeat[u] = f;
-      } else {
-        adam_row<D>(c, st, f, g, lane);
-        if (lane == 0) atomicAdd((unsigned long long*)&st->stat_unique, 1ull);
+          for (int q = 0; q < PL; ++q) acc[q] += r[u][q];
+          if ((inf[u] >> 16) & 1u) {  // last piece of its run: the batch's update
+            const uint32_t bq = (inf[u] >> 8) & 0xffu;
+            if (c.grad_out) {
+              uint32_t ui = 0;
+              if (lane == 0) ui = atomicAdd(&st->n_unique, 1u);
+              ui = __shfl(ui, 0, EP_WAVE);
+              store_slice<D>(c.G + (size_t)ui * D, lane, acc);
+              if (lane == 0) c.ufeat[ui] = f | (bq << 27);
+            } else {
+              bool bad = false;
+#pragma unroll
+              for (int q = 0; q < PL; ++q) bad |= (base + q < D) && isnan(acc[q]);
+              if (!wave_any(bad)) {  // optimizer.rs:56, per batch
+                const float lr = s_lr[bq];
+#pragma unroll
+                for (int q = 0; q < PL; ++q) adam_math(th[q], m[q], v[q], acc[q], lr, sv.bc1, sv.bc2);
+                upd = true;
+              }
+            }
+#pragma unroll
+            for (int q = 0; q < PL; ++q) acc[q] = 0.f;
+          }
+        }
       }
     }
-    __syncthreads();
+    if (upd) {
+      store_slice<D>(c.mom + ooff, lane, m);
+      store_slice<D>(c.var + ooff, lane, v);
+      if (c.bf16) {
+        if (c.master) store_slice<D>(c.master + ooff, lane, th);
+        uint16_t* row = (uint16_t*)c.table + off;
+#pragma unroll
+        for (int q = 0; q < PL; ++q)
+          if (base + q < D)
+            row[base + q] = f32_to_bf16_sr(th[q], sr_bits(c, sv, f, (uint32_t)((base + q) >> 2), (base + q) & 3));
+      } else {
+        store_slice<D>((float*)c.table + off, lane, th);
+      }
+      ++done;
+    }
   }
+  if (lane == 0 && done) atomicAdd((unsigned long long*)&st->stat_unique, (unsigned long long)done);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1202,9 +1436,10 @@ __global__ __launch_bounds__(256) void k_atomic_scatter(StepConfig c, StepState*
     const uint32_t i0 = (uint32_t)(chunk * 64u);
     const uint32_t j = i0 + lane;
     const bool valid = j < n;
-    const uint32_t f = valid ? c.keys[0][j] : SENT;
+    const uint32_t kf = valid ? c.keys[0][j] : SENT;
     const uint32_t s = valid ? c.slots[0][j] : 0u;
-    const bool real = f != SENT;
+    const bool real = kf != SENT;
+    const uint32_t f = real ? row_of_key(c, kf) : 0u;
     const bool newt = real && atomicCAS(&c.flag[f], 0u, 1u) == 0u;
     const uint64_t nm = __ballot(newt);
     uint32_t tb = 0;
@@ -1244,6 +1479,7 @@ __global__ __launch_bounds__(256) void k_adam_touched(StepConfig c, StepState* s
   const uint32_t gw = blockIdx.x * NW + (threadIdx.x >> 6);
   const uint32_t nwaves = gridDim.x * NW;
   const uint32_t nt = st->n_touched;
+  const StepScalars sv = step_scalars(st);
   if (gw == 0 && lane == 0) atomicAdd((unsigned long long*)&st->stat_unique, (unsigned long long)nt);
   for (uint32_t u = gw; u < nt; u += nwaves) {
     const uint32_t f = c.touched[u];
@@ -1254,7 +1490,7 @@ __global__ __launch_bounds__(256) void k_adam_touched(StepConfig c, StepState* s
     for (int q = 0; q < PL; ++q) z[q] = 0.f;
     store_slice<D>(grow, lane, z);
     if (lane == 0) c.flag[f] = 0u;
-    adam_row<D>(c, st, f, g, lane);
+    adam_row<D>(c, sv, f, g, lane);
   }
 }
 
@@ -1283,7 +1519,7 @@ __global__ __launch_bounds__(256) void k_init_table(StepConfig c, uint64_t seed)
 #pragma unroll
     for (int q = 0; q < PL; ++q) x[q] = x[q] / nrm;
     if (c.bf16) {
-      store_slice<D>(c.master + row * c.ostride, lane, x);
+      if (c.master) store_slice<D>(c.master + row * c.ostride, lane, x);
       uint16_t* t = (uint16_t*)c.table + row * D;
 #pragma unroll
       for (int q = 0; q < PL; ++q)
@@ -1438,30 +1674,38 @@ bool seg_vpl_supported(int d, int vpl) {
   return d % vpl == 0 && d / vpl <= 64 && (d % 4 == 0);
 }
 
+bool seg_variant_supported(int d, int vpl, int unroll) {
+  if (!seg_vpl_supported(d, vpl)) return false;
+  const int nq = vpl / 4;
+  return unroll == 8 / nq || unroll == 16 / nq;
+}
+
+template <int D, int VPL, int U>
+static bool try_seg_adam(const StepConfig& c, StepState* st, int buf, hipStream_t s, int grid) {
+  if constexpr (D % VPL == 0 && D / VPL <= 64) {
+    if (c.seg_vpl == VPL && c.seg_unroll == U) {
+      hipLaunchKernelGGL((k_seg_adam<D, VPL, U>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
+      return true;
+    }
+  }
+  return false;
+}
+
 template <int D>
-static void launch_seg_adam_vpl(const StepConfig& c, StepState* st, int buf, hipStream_t s, int grid) {
-  int vpl = c.seg_vpl;
-  if constexpr (D % 16 == 0 && D / 16 <= 64) {
-    if (vpl == 16) {
-      hipLaunchKernelGGL((k_seg_adam<D, 16>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
-      return;
-    }
-  }
-  if constexpr (D % 8 == 0 && D / 8 <= 64) {
-    if (vpl == 8) {
-      hipLaunchKernelGGL((k_seg_adam<D, 8>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
-      return;
-    }
-  }
-  hipLaunchKernelGGL((k_seg_adam<D, 4>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
+static void launch_seg_adam_variant(const StepConfig& c, StepState* st, int buf, hipStream_t s, int grid) {
+  if (try_seg_adam<D, 16, 2>(c, st, buf, s, grid) || try_seg_adam<D, 16, 4>(c, st, buf, s, grid) ||
+      try_seg_adam<D, 8, 4>(c, st, buf, s, grid) || try_seg_adam<D, 8, 8>(c, st, buf, s, grid) ||
+      try_seg_adam<D, 4, 16>(c, st, buf, s, grid))
+    return;
+  hipLaunchKernelGGL((k_seg_adam<D, 4, 8>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
 }
 
 hipError_t launch_scatter_sorted(const StepConfig& c, StepState* st, int buf, hipStream_t s) {
   const int grid = c.unique_grid;
   EP_DISPATCH(c.dims, {
-    launch_seg_adam_vpl<D>(c, st, buf, s, grid);
+    launch_seg_adam_variant<D>(c, st, buf, s, grid);
     hipLaunchKernelGGL((k_pieces<D>), dim3(grid), dim3(256), 0, s, c, st, c.slots[buf]);
-    hipLaunchKernelGGL((k_long_combine<D>), dim3(256), dim3(256), 0, s, c, st, c.keys[buf]);
+    hipLaunchKernelGGL((k_long_combine<D>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf]);
   });
   return hipGetLastError();
 }

@@ -85,7 +85,7 @@ def holdout_auc(node_emb: np.ndarray, h: Holdout) -> float:
 
 
 def train_gpu_table(graph_indptr, graph_indices, f_indptr, f_ids, vocab, dims, batch, negatives, max_nodes, gamma,
-                    alpha, passes, seed, precision="fp32", use_graph=False, scatter="sort"):
+                    alpha, passes, seed, precision="fp32", use_graph=False, scatter="sort", concurrent=1):
     """Trains with the GPU path (ep_learn) and returns the FP32 feature table."""
     from . import _native as N
 
@@ -93,9 +93,10 @@ def train_gpu_table(graph_indptr, graph_indices, f_indptr, f_ids, vocab, dims, b
     cfg.dims, cfg.batch_size, cfg.negatives, cfg.gamma, cfg.alpha = dims, batch, negatives, gamma, alpha
     cfg.passes, cfg.valid_pct, cfg.seed = passes, 0.0, seed
     cfg.max_nodes_kind, cfg.max_nodes = N.EP_SAMPLE_FIXED, max_nodes
-    cfg.precision = N.EP_PREC_BF16 if precision == "bf16" else N.EP_PREC_FP32
+    cfg.precision = {"fp32": N.EP_PREC_FP32, "bf16": N.EP_PREC_BF16, "bf16-master": N.EP_PREC_BF16_MASTER}[precision]
     cfg.use_graph = 1 if use_graph else 0
     cfg.scatter = N.EP_SCATTER_ATOMIC if scatter == "atomic" else N.EP_SCATTER_SORT
+    cfg.concurrent_batches = concurrent
     with N.Context(cfg) as ctx:
         ctx.set_graph(graph_indptr, graph_indices)
         ctx.set_features(f_indptr, f_ids, vocab)

@@ -37,7 +37,12 @@ enum {
 enum { EP_SAMPLE_ALL = 0, EP_SAMPLE_FIXED = 1 };
 /* LossWeighting (src/algos/ep/mod.rs:31-36).                                   */
 enum { EP_LW_NONE = 0, EP_LW_LOG = 1, EP_LW_EXPONENTIAL = 2 };
-enum { EP_PREC_FP32 = 0, EP_PREC_BF16 = 1 };
+/* Table precision.  FP32: FP32 table.  BF16: BF16 table; each touched row is taken
+ * to FP32 for the update (FP32 master copy of the touched rows) and written back with
+ * stochastic rounding.  BF16_MASTER: BF16 working table plus a persistent FP32 master
+ * of every row (the update reads and writes the master, the BF16 copy is rounded
+ * from it stochastically).  Adam moments are FP32 in every mode.                  */
+enum { EP_PREC_FP32 = 0, EP_PREC_BF16 = 1, EP_PREC_BF16_MASTER = 2 };
 /* Gradient extraction: the reference's first-leaf rule (mod.rs:347-352,366-367)
  * or the exact gradient (sum over every construction).                         */
 enum { EP_GRAD_FIRST_LEAF = 0, EP_GRAD_EXACT = 1 };
@@ -70,8 +75,13 @@ typedef struct ep_config {
   int32_t use_graph;           /* 1: replay each step from a captured hipGraph      */
   uint64_t seed;               /* default 20222022 (src/lib.rs:90)                  */
   int32_t device;              /* HIP device ordinal                                */
-  int32_t seg_vpl;             /* segment-Adam values per lane: 0 = autotune over   */
-                               /* the first training steps, else 4, 8 or 16         */
+  int32_t seg_vpl;             /* segment-Adam tile: values per lane (4, 8, 16) and */
+  int32_t seg_unroll;          /* adjoint rows in flight ({8,16}/(vpl/4)); both 0 = */
+                               /* autotuned over the first training steps           */
+  uint32_t concurrent_batches; /* batches per step (1, 2, 4, 8): they read the same  */
+                               /* parameters and update in batch order, as the      */
+                               /* reference's concurrently running batches          */
+                               /* (mod.rs:174); 1 = strictly sequential (default)   */
 } ep_config;
 
 typedef struct ep_ctx ep_ctx;
@@ -93,6 +103,7 @@ typedef struct ep_bench_result {
   uint32_t steps;
   uint32_t forward_launches;
   int32_t seg_vpl;             /* segment-Adam variant in use (after tuning)        */
+  int32_t seg_unroll;
 } ep_bench_result;
 
 /* Fills the reference defaults (EmbeddingPropagator::new, src/lib.rs:1262-1274). */

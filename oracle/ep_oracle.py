@@ -389,6 +389,34 @@ def ep_batch_step(table, mom, var, f_indptr, f_ids, anchors, nbr_lists, nbr_scal
     return results, merged
 
 
+def ep_group_step(table, mom, var, f_indptr, f_ids, anchors, nbr_lists, nbr_scales, neg_lists,
+                  gamma, lrs, batch_of, pass_, grad_mode="first_leaf", loss_divs=None):
+    """A group of concurrently running batches (mod.rs:174: par_iter over the batch
+    chunks, Hogwild updates of the shared store), serialised: every batch of the group
+    computes its gradients from the parameters as they were at the group's start, then
+    the batches' updates are applied in batch order, batch k with lrs[k] (its own step's
+    learning rate, mod.rs:234-248).  batch_of[i] is example i's batch (0 .. len(lrs)-1);
+    an int B means consecutive batches of B examples.  Updates table/mom/var in place;
+    returns the per-example results and the list of per-batch merged gradients.
+    """
+    if isinstance(batch_of, int):
+        batch_of = [i // batch_of for i in range(len(anchors))]
+    snap = table.copy()
+    results = []
+    for i, v in enumerate(anchors):
+        div = 1.0 if loss_divs is None else loss_divs[i]
+        results.append(example_grads(snap, f_indptr, f_ids, v, nbr_lists[i], nbr_scales[i],
+                                     neg_lists[i], gamma, grad_mode, div))
+    merged = []
+    for k in range(len(lrs)):
+        part = [r for r, b in zip(results, batch_of) if b == k]
+        mk = merge_batch(part, table.shape[1])
+        if any(r.contributes for r in part):
+            adam_update(table, mom, var, mk, lrs[k], pass_)
+        merged.append(mk)
+    return results, merged
+
+
 def batch_error(results: Sequence[ExampleResult]) -> float:
     """Per-batch train error (mod.rs:227-260): sum of contributing losses / batch size."""
     cnt = sum(1 for r in results if r.contributes)

@@ -68,6 +68,23 @@ def oracle_step(prob: Problem, anchors, cnt, nbr, neg, gamma, lr, pass_, grad_mo
     return results, merged, t, m, v
 
 
+def oracle_group_step(prob: Problem, anchors, cnt, nbr, neg, gamma, lr, pass_, grad_mode, batch_size):
+    """FP64 oracle of one explicit-sample step of consecutive batches of `batch_size`
+    anchors run as one concurrent group (every batch reads the group-start table, updates
+    in batch order, all with learning rate lr).  Returns (per-batch merged, table, mom, var)."""
+    t = prob.table.astype(np.float64).copy()
+    m = np.zeros_like(t)
+    v = np.zeros_like(t)
+    B = len(anchors)
+    keep = [i for i in range(B) if cnt[i] > 0]  # anchors with no neighbourhood give no example
+    nb = (B + batch_size - 1) // batch_size
+    _, merged = O.ep_group_step(t, m, v, prob.f_indptr, prob.f_ids, [int(anchors[i]) for i in keep],
+                                [list(map(int, nbr[i, :cnt[i]])) for i in keep], [1.0] * len(keep),
+                                [list(map(int, neg[i])) for i in keep], gamma, [lr] * nb,
+                                [i // batch_size for i in keep], pass_, grad_mode)
+    return merged, t, m, v
+
+
 def dense_grad(merged, V, D):
     G = np.zeros((V, D))
     for f, g in merged.items():

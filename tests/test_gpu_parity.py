@@ -10,7 +10,7 @@ import pytest
 
 from cloverleaf_amd import _native as N
 from oracle import ep_oracle as O
-from tests.problems import dense_grad, make_problem, oracle_step
+from tests.problems import dense_grad, make_problem, oracle_group_step, oracle_step
 
 pytestmark = pytest.mark.gpu
 
@@ -19,7 +19,8 @@ GRAD_TOL = 1e-5
 
 
 def make_ctx(prob, S, K, gamma=0.5, B=32, grad_mode=N.EP_GRAD_FIRST_LEAF, scatter=N.EP_SCATTER_SORT,
-             precision=N.EP_PREC_FP32, lw=N.EP_LW_NONE, lw_exp=0.0, use_graph=0, seed=20222022, seg_vpl=0):
+             precision=N.EP_PREC_FP32, lw=N.EP_LW_NONE, lw_exp=0.0, use_graph=0, seed=20222022, seg_tile=(0, 0),
+             concurrent=1):
     cfg = N.default_config()
     cfg.dims, cfg.batch_size, cfg.negatives, cfg.gamma = prob.D, B, K, gamma
     cfg.max_nodes_kind, cfg.max_nodes = N.EP_SAMPLE_FIXED, S
@@ -27,7 +28,8 @@ def make_ctx(prob, S, K, gamma=0.5, B=32, grad_mode=N.EP_GRAD_FIRST_LEAF, scatte
     cfg.loss_weighting, cfg.loss_weight_exp = lw, lw_exp
     cfg.use_graph = use_graph
     cfg.seed = seed
-    cfg.seg_vpl = seg_vpl
+    cfg.seg_vpl, cfg.seg_unroll = seg_tile
+    cfg.concurrent_batches = concurrent
     cfg.valid_pct = 0.0
     ctx = N.Context(cfg)
     ctx.set_graph(prob.indptr, prob.indices)
@@ -209,18 +211,23 @@ def test_sampler_uniformity():
     ctx.close()
 
 
-def test_bf16_step_error_bar():
+def bf16_round_nearest(x):
+    u = np.asarray(x, dtype=np.float32).view(np.uint32)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).view(np.float32)
+
+
+@pytest.mark.parametrize("precision", [N.EP_PREC_BF16, N.EP_PREC_BF16_MASTER])
+def test_bf16_step_error_bar(precision):
     prob = make_problem(seed=21, n=150, V=400, D=64)
-    ctx = make_ctx(prob, 6, 5, 0.5, precision=N.EP_PREC_BF16)
+    ctx = make_ctx(prob, 6, 5, 0.5, precision=precision)
     anchors = np.arange(0, 150, 3, dtype=np.uint32)
     cnt, nbr, neg = ctx.sample_batch(anchors, 1)
     out = ctx.step_explicit(anchors, cnt, nbr, neg, 1, 0.01, apply_update=True)
     # oracle on the bf16-rounded table (what the forward reads)
-    tb = prob.table.astype(np.float32).view(np.uint32)
-    tb = ((tb + 0x7FFF + ((tb >> 16) & 1)) & 0xFFFF0000).view(np.float32)
+    tb = bf16_round_nearest(prob.table)
     from tests.problems import Problem
     pb = Problem(prob.indptr, prob.indices, prob.f_indptr, prob.f_ids, prob.V, prob.D, tb)
-    res, merged, *_ = oracle_step(pb, anchors, cnt, nbr, neg, 0.5, 0.01, 1, "first_leaf")
+    res, merged, t64, *_ = oracle_step(pb, anchors, cnt, nbr, neg, 0.5, 0.01, 1, "first_leaf")
     G64 = dense_grad(merged, prob.V, prob.D)
     Gg = np.zeros_like(G64)
     Gg[out["feat"]] = out["grad"]
@@ -230,23 +237,49 @@ def test_bf16_step_error_bar():
     _, merged32, *_ = oracle_step(prob, anchors, cnt, nbr, neg, 0.5, 0.01, 1, "first_leaf")
     e32 = rel(Gg, dense_grad(merged32, prob.V, prob.D))
     assert 1e-5 < e32 <= 1e-2, e32
-    # the working table is the round-to-nearest bf16 copy of the master rows, updated by SR
-    tbf = ctx.get_table_bf16().astype(np.uint32) << 16
-    master = ctx.get_table()
+    tbf = (ctx.get_table_bf16().astype(np.uint32) << 16).view(np.float32)
     rows = sorted(merged)
     untouched = np.setdiff1d(np.arange(prob.V), rows)
-    assert np.array_equal(tbf[untouched].view(np.float32), tb[untouched])
-    diff = np.abs(tbf[rows].view(np.float32) - master[rows])
-    assert np.all(diff <= np.abs(master[rows]) * 2.0 ** -7 + 1e-30)  # within one bf16 ulp of the master
+    assert np.array_equal(tbf[untouched], tb[untouched])
+    if precision == N.EP_PREC_BF16_MASTER:
+        # the working table is the stochastically rounded copy of the FP32 master rows
+        master = ctx.get_table()
+        diff = np.abs(tbf[rows] - master[rows])
+        assert np.all(diff <= np.abs(master[rows]) * 2.0 ** -7 + 1e-30)  # within one bf16 ulp of the master
+        mask = np.abs(G64[rows]) > 1e-3 * np.abs(G64).max()
+        # the master starts from the FP32 rows; the first Adam step adds a delta that depends
+        # only on the gradient (taken on the bf16 rows)
+        expect = prob.table.astype(np.float64) + (t64 - tb.astype(np.float64))
+        assert np.max(np.abs(master[rows] - expect[rows])[mask]) < 1e-5
+    else:
+        # the bf16 table is the parameter; get_table widens it exactly
+        assert np.array_equal(ctx.get_table(), tbf)
+        # touched rows: the FP32 update of the bf16 row, stochastically rounded to one of the
+        # two bf16 neighbours of the exact (FP64) update, rounding up with probability equal
+        # to the position between them
+        mask = np.abs(G64[rows]) > 1e-3 * np.abs(G64).max()
+        x = t64[rows][mask].astype(np.float64)
+        got = tbf[rows][mask].astype(np.float64)
+        lo = (np.float32(x).view(np.uint32) & 0xFFFF0000).view(np.float32).astype(np.float64)  # toward zero
+        ulp = np.abs((((np.abs(lo).astype(np.float32).view(np.uint32) + 0x10000)).view(np.float32)).astype(np.float64)
+                     - np.abs(lo))
+        away = lo + np.sign(x) * ulp
+        assert np.all((got == lo) | (got == away)), "not a neighbour of the exact update"
+        pr = np.abs(x - lo) / ulp  # probability of rounding away from zero
+        up = (got == away).astype(np.float64)
+        sd = np.sqrt(np.sum(pr * (1 - pr))) + 1e-9
+        assert abs(np.sum(up - pr)) <= 4 * sd + 2, (np.sum(up), np.sum(pr), sd)
+        assert len(x) > 500
     ctx.close()
 
 
-@pytest.mark.parametrize("D,vpl", [(64, 8), (64, 16), (128, 8), (128, 16), (16, 16)])
-def test_seg_vpl_variant_matches_oracle(D, vpl):
-    # every values-per-lane variant of the segment-Adam kernel against the oracle
-    prob = make_problem(seed=D + vpl, n=150, V=300, D=D)
+@pytest.mark.parametrize("D,tile", [(64, (8, 4)), (64, (16, 2)), (128, (8, 8)), (128, (16, 4)), (16, (16, 2)),
+                                    (128, (4, 16))])
+def test_seg_tile_variant_matches_oracle(D, tile):
+    # segment-Adam tiles (values per lane, unroll) against the oracle
+    prob = make_problem(seed=D + tile[0], n=150, V=300, D=D)
     S, K, gamma = 6, 5, 0.5
-    ctx = make_ctx(prob, S, K, gamma, seg_vpl=vpl)
+    ctx = make_ctx(prob, S, K, gamma, seg_tile=tile)
     rng = np.random.default_rng(2)
     anchors = rng.choice(prob.n, size=64, replace=False).astype(np.uint32)
     out, res, merged, t64, m64, v64 = check_step(prob, ctx, anchors, gamma, N.EP_GRAD_FIRST_LEAF)
@@ -262,19 +295,84 @@ def test_seg_vpl_variant_matches_oracle(D, vpl):
 
 
 @pytest.mark.parametrize("precision", [N.EP_PREC_FP32, N.EP_PREC_BF16])
-def test_seg_vpl_variants_bit_identical(precision):
-    # the variants add every segment in the same order: tables and moments agree bit for bit,
-    # so the tuner's choice never changes results; seg_vpl=0 (tuned) included
-    prob = make_problem(seed=41, n=400, V=300, D=128)
+def test_seg_tile_variants_bit_identical(precision):
+    # the tiles add every segment in the same order: tables and moments agree bit for bit,
+    # so the tuner's choice never changes results; (0, 0) (tuned) included
+    prob = make_problem(seed=41, n=400, V=300, D=128, avg_deg=30, zipf=1.3)
     outs = []
-    for vpl in (4, 8, 16, 0):
-        ctx = make_ctx(prob, 8, 5, 0.5, B=64, precision=precision, seg_vpl=vpl)
+    for tile in ((4, 8), (4, 16), (8, 4), (8, 8), (16, 2), (16, 4), (0, 0)):
+        ctx = make_ctx(prob, 8, 5, 0.5, B=64, precision=precision, seg_tile=tile)
         ctx.set_split(np.arange(400, dtype=np.uint32), np.zeros(0, dtype=np.uint32))
-        # seg_vpl=0 tunes over its first 3 variants x 4 rounds = 12 steps (warm-up is
-        # extended to cover them); the fixed variants run the same 12 + 14 steps
-        ctx.bench_steps(warmup=0 if vpl == 0 else 12, steps=14, instrument=False)
+        # (0, 0) tunes over its 6 candidates x 4 rounds = 24 steps (warm-up is extended
+        # to cover them); the fixed tiles run the same 24 + 14 steps
+        ctx.bench_steps(warmup=0 if tile == (0, 0) else 24, steps=14, instrument=False)
         t = ctx.get_table_bf16() if precision == N.EP_PREC_BF16 else ctx.get_table()
         outs.append((t, *ctx.get_adam()))
+        ctx.close()
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("P,D,S,tile", [(2, 64, 6, (0, 0)), (4, 128, 6, (8, 4)), (4, 128, 12, (16, 2)),
+                                         (8, 32, 4, (4, 8)), (4, 100, 6, (0, 0))])
+def test_concurrent_batches_match_oracle(P, D, S, tile):
+    # P batches of B anchors in one step: every batch reads the group-start table, the
+    # per-(row, batch) gradients are merged separately and applied in batch order
+    B = 24
+    prob = make_problem(seed=60 + P + D, n=260, V=90, D=D, zipf=1.3, avg_deg=12)
+    ctx = make_ctx(prob, S, 5, 0.5, B=B, concurrent=P, seg_tile=tile)
+    rng = np.random.default_rng(P)
+    anchors = rng.choice(prob.n, size=P * B - 5, replace=False).astype(np.uint32)  # last batch partial
+    cnt, nbr, neg = ctx.sample_batch(anchors, 1)
+    print(f"[parity] concurrent P={P} D={D} S={S} tile={tile}", flush=True)
+    out = ctx.step_explicit(anchors, cnt, nbr, neg, 1, 0.01, apply_update=True)
+    merged, t64, m64, v64 = oracle_group_step(prob, anchors, cnt, nbr, neg, 0.5, 0.01, 1, "first_leaf", B)
+    rows_g = out["feat"] & ((1 << 27) - 1)
+    bat_g = out["feat"] >> 27
+    assert len(set(zip(rows_g.tolist(), bat_g.tolist()))) == len(rows_g)
+    shared = 0
+    for k in range(P):
+        Gk = dense_grad(merged[k], prob.V, prob.D)
+        sel = bat_g == k
+        assert set(rows_g[sel].tolist()) == set(merged[k])
+        Gg = np.zeros_like(Gk)
+        Gg[rows_g[sel]] = out["grad"][sel]
+        assert rel(Gg, Gk) <= GRAD_TOL
+        if k:
+            shared += len(set(merged[k]) & set(merged[k - 1]))
+    assert shared > 0  # rows updated by several batches of the group
+    new = ctx.get_table()
+    m, v = ctx.get_adam()
+    rows = sorted(set().union(*[set(mk) for mk in merged]))
+    assert rel(m[rows], m64[rows]) <= 1e-5
+    assert rel(v[rows], v64[rows]) <= 1e-4
+    # compare theta where every batch gradient of the element is clearly above FP32 noise
+    gmax = max(np.abs(dense_grad(mk, prob.V, prob.D)).max() for mk in merged)
+    mask = np.ones((len(rows), prob.D), dtype=bool)
+    for mk in merged:
+        Gk = dense_grad(mk, prob.V, prob.D)[rows]
+        present = np.array([r in mk for r in rows])[:, None]
+        mask &= ~present | (np.abs(Gk) > 1e-3 * gmax)
+    assert mask.sum() > 100
+    assert np.max(np.abs(new[rows] - t64[rows])[mask]) < 1e-5
+    untouched = np.setdiff1d(np.arange(prob.V), rows)
+    assert np.array_equal(new[untouched], prob.table[untouched])
+    ctx.close()
+
+
+@pytest.mark.parametrize("precision", [N.EP_PREC_FP32, N.EP_PREC_BF16])
+def test_concurrent_batches_tiles_bit_identical_and_learn(precision):
+    # training steps with P=4: every segment tile gives the same bits; learn() runs passes
+    prob = make_problem(seed=77, n=500, V=200, D=64, avg_deg=20, zipf=1.2)
+    outs = []
+    for tile in ((4, 8), (8, 4), (16, 4), (0, 0)):
+        ctx = make_ctx(prob, 8, 5, 0.5, B=32, precision=precision, seg_tile=tile, concurrent=4)
+        ctx.set_split(np.arange(500, dtype=np.uint32), np.zeros(0, dtype=np.uint32))
+        r = ctx.bench_steps(warmup=0 if tile == (0, 0) else 24, steps=9, instrument=False)
+        # 500 anchors = 4 steps of 128, 128, 128, 116; steps 25-33 = two passes + 128
+        assert r.examples == 2 * 500 + 128
+        outs.append((ctx.get_table(), *ctx.get_adam()))
         ctx.close()
     for o in outs[1:]:
         for a, b in zip(outs[0], o):
@@ -285,7 +383,8 @@ def test_graph_replay_matches_eager():
     prob = make_problem(seed=30, n=300, V=500, D=64)
     tables = []
     for use_graph in (0, 1):
-        ctx = make_ctx(prob, 6, 5, 0.5, B=64, use_graph=use_graph)
+        # a fixed tile: no tuning steps ahead of the timed ones
+        ctx = make_ctx(prob, 6, 5, 0.5, B=64, use_graph=use_graph, seg_tile=(8, 4))
         ctx.set_split(np.arange(300, dtype=np.uint32), np.zeros(0, dtype=np.uint32))
         r = ctx.bench_steps(warmup=0, steps=7, instrument=False)
         assert r.examples == 4 * 64 + 44 + 2 * 64  # 5 batches per pass (300 = 4*64 + 44)

@@ -34,18 +34,30 @@ def cpu_tables(w, h):
             yield np.fromfile(out, dtype=np.float32).reshape(w.vocab, w.dims)
 
 
-@pytest.mark.parametrize("precision,use_graph", [("fp32", False), ("bf16", True)])
-def test_holdout_auc_matches_cpu_restatement(precision, use_graph):
+@pytest.fixture(scope="module")
+def setup():
     w = synth.make_workload("ep-cfg1-10k-comm")
     h = Q.holdout_edges(w.indptr, w.indices, 0.05, seed=7)
-    print(f"[quality] {precision} graph={use_graph} holdout pairs {len(h.pos)}", flush=True)
 
     def auc(tab):
         return Q.holdout_auc(Q.node_embeddings_averaged(tab, w.f_indptr, w.f_ids), h)
 
-    gpu = [auc(Q.train_gpu_table(h.indptr, h.indices, w.f_indptr, w.f_ids, w.vocab, w.dims, w.batch, w.negatives,
-                                 w.max_nodes, w.gamma, w.alpha, w.passes, s, precision, use_graph)) for s in SEEDS]
     cpu = [auc(t) for t in cpu_tables(w, h)]
+    return w, h, auc, cpu
+
+
+# (precision, hipGraph replay, reference batches per step)
+FORMS = [("fp32", False, 1), ("bf16", True, 1), ("bf16-master", True, 1), ("fp32", True, 4), ("bf16", True, 4),
+         ("bf16", True, 8)]
+
+
+@pytest.mark.parametrize("precision,use_graph,concurrent", FORMS)
+def test_holdout_auc_matches_cpu_restatement(setup, precision, use_graph, concurrent):
+    w, h, auc, cpu = setup
+    print(f"[quality] {precision} graph={use_graph} P={concurrent} holdout pairs {len(h.pos)}", flush=True)
+    gpu = [auc(Q.train_gpu_table(h.indptr, h.indices, w.f_indptr, w.f_ids, w.vocab, w.dims, w.batch, w.negatives,
+                                 w.max_nodes, w.gamma, w.alpha, w.passes, s, precision, use_graph,
+                                 concurrent=concurrent)) for s in SEEDS]
     print(f"[quality] gpu {np.mean(gpu):.4f} +- {np.std(gpu):.4f}  cpu {np.mean(cpu):.4f} +- {np.std(cpu):.4f}")
     assert np.mean(cpu) > 0.8  # the workload is learnable
     assert abs(np.mean(gpu) - np.mean(cpu)) <= 0.005

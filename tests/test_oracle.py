@@ -331,6 +331,77 @@ def test_adam_skips_nan_rows():
     assert np.array_equal(m[0], [0.0, 0.0])
 
 
+# --------------------------------------------------------------------------------------
+# Concurrent batches (mod.rs:174 par_iter over chunks): stale reads, updates in batch order
+# --------------------------------------------------------------------------------------
+def _group_problem(seed, V, disjoint):
+    rng = np.random.default_rng(seed)
+    n_nodes, D = 12, 4
+    if disjoint:  # nodes 0-5 use features [0, V/2), nodes 6-11 features [V/2, V)
+        f_ids, f_indptr = [], [0]
+        for i in range(n_nodes):
+            lo, hi = (0, V // 2) if i < 6 else (V // 2, V)
+            k = int(rng.integers(1, 4))
+            f_ids += list(rng.integers(lo, hi, size=k))
+            f_indptr.append(len(f_ids))
+        f_ids, f_indptr = np.array(f_ids), np.array(f_indptr)
+    else:
+        counts = rng.integers(1, 4, size=n_nodes)
+        f_indptr = np.concatenate([[0], np.cumsum(counts)])
+        f_ids = rng.integers(0, V, size=f_indptr[-1])
+    table = rng.normal(size=(V, D))
+    # batch 0: anchors 0, 1 (graph part 0-5); batch 1: anchors 6, 7 (graph part 6-11)
+    anchors = [0, 1, 6, 7]
+    nbrs = [[2, 3], [4, 5], [8, 9], [10, 11]]
+    negs = [[3, 4], [5, 2], [9, 10], [11, 8]]
+    return table, f_indptr, f_ids, anchors, nbrs, negs
+
+
+def test_group_of_one_batch_is_the_batch_step():
+    table, f_indptr, f_ids, anchors, nbrs, negs = _group_problem(1, 10, False)
+    t1, m1, v1 = table.copy(), np.zeros_like(table), np.zeros_like(table)
+    t2, m2, v2 = table.copy(), np.zeros_like(table), np.zeros_like(table)
+    O.ep_batch_step(t1, m1, v1, f_indptr, f_ids, anchors, nbrs, [1.0] * 4, negs, 3.0, 0.05, 1)
+    O.ep_group_step(t2, m2, v2, f_indptr, f_ids, anchors, nbrs, [1.0] * 4, negs, 3.0, [0.05], [0, 0, 0, 0], 1)
+    assert np.array_equal(t1, t2) and np.array_equal(m1, m2) and np.array_equal(v1, v2)
+
+
+def test_group_on_disjoint_rows_equals_sequential_batches():
+    # no row read by batch 1 is written by batch 0: stale and fresh reads coincide
+    table, f_indptr, f_ids, anchors, nbrs, negs = _group_problem(2, 10, True)
+    t1, m1, v1 = table.copy(), np.zeros_like(table), np.zeros_like(table)
+    O.ep_batch_step(t1, m1, v1, f_indptr, f_ids, anchors[:2], nbrs[:2], [1.0] * 2, negs[:2], 3.0, 0.05, 1)
+    O.ep_batch_step(t1, m1, v1, f_indptr, f_ids, anchors[2:], nbrs[2:], [1.0] * 2, negs[2:], 3.0, 0.02, 1)
+    t2, m2, v2 = table.copy(), np.zeros_like(table), np.zeros_like(table)
+    O.ep_group_step(t2, m2, v2, f_indptr, f_ids, anchors, nbrs, [1.0] * 4, negs, 3.0, [0.05, 0.02], 2, 1)
+    assert np.allclose(t1, t2, atol=1e-15) and np.allclose(m1, m2, atol=1e-15)
+    assert not np.array_equal(t1, table)
+
+
+def test_group_shared_row_gets_two_adam_steps_from_stale_gradients():
+    table, f_indptr, f_ids, anchors, nbrs, negs = _group_problem(3, 6, False)
+    snap = table.copy()
+    # per-batch merged gradients, both from the group-start parameters
+    g = []
+    for lo in (0, 2):
+        rs = [O.example_grads(snap, f_indptr, f_ids, anchors[i], nbrs[i], 1.0, negs[i], 3.0, "first_leaf")
+              for i in (lo, lo + 1)]
+        g.append(O.merge_batch(rs, 4))
+    shared = sorted(set(g[0]) & set(g[1]))
+    assert shared
+    t, m, v = table.copy(), np.zeros_like(table), np.zeros_like(table)
+    O.ep_group_step(t, m, v, f_indptr, f_ids, anchors, nbrs, [1.0] * 4, negs, 3.0, [0.05, 0.02], 2, 2)
+    b1, b2, eps = 0.9, 0.999, 1e-8
+    bc1, bc2 = 1 - b1 ** 3, 1 - b2 ** 3  # t = pass + 1 = 3
+    for f in shared:
+        g1, g2 = g[0][f], g[1][f]
+        mm, vv = (1 - b1) * g1, (1 - b2) * g1 ** 2
+        th = snap[f] - 0.05 * (mm / bc1) / (np.sqrt(vv / bc2) + eps)
+        mm, vv = b1 * mm + (1 - b1) * g2, b2 * vv + (1 - b2) * g2 ** 2
+        th = th - 0.02 * (mm / bc1) / (np.sqrt(vv / bc2) + eps)
+        assert np.allclose(t[f], th, atol=1e-14) and np.allclose(m[f], mm, atol=1e-15)
+
+
 def test_cos_decay_schedule_closed_form():
     alpha, T = 0.01, 100
     lo = alpha / 100
