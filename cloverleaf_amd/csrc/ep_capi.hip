@@ -188,7 +188,8 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   c->release(b.adj); c->release(b.coef); c->release(b.elig); c->release(b.loss);
   c->release(b.keys[0]); c->release(b.keys[1]); c->release(b.slots[0]); c->release(b.slots[1]);
   c->release(b.ghist); c->release(b.G); c->release(b.ufeat); c->release(b.long_st); c->release(b.long_len);
-  c->release(b.long_pbase); c->release(b.piece_st); c->release(b.piece_info); c->release(b.part);
+  c->release(b.long_pbase); c->release(b.long_runs); c->release(b.piece_st); c->release(b.piece_info);
+  c->release(b.part); c->release(b.srec); c->release(b.mrec); c->release(b.scount); c->release(b.mcount);
   c->release(c->d_explicit);
   c->release(c->d_xout);
   if (c->step_exec) { (void)hipGraphExecDestroy(c->step_exec); c->step_exec = nullptr; c->step_exec_ready = false; }
@@ -199,6 +200,8 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   // long rows have > kSegShort entries; each run of one is cut into pieces of kPieceEntries
   const uint32_t long_cap = (uint32_t)(keycap / (ep::kSegShort + 1) + 1);
   const uint32_t piece_cap = (uint32_t)(keycap / ep::kPieceEntries + (uint64_t)long_cap * P + 1);
+  // persistent row kernels: 2048 blocks of kBlockWaves waves
+  const int unique_grid = 2048;
   // radix-sort grid: ~8K keys per block at capacity, a multiple of 256 in [256, 2048]
   const int sort_grid = (int)std::min<uint64_t>(2048, std::max<uint64_t>(256, (keycap / 8192 + 255) / 256 * 256));
   EP_TRY(c->alloc(&b.nbr_cnt, batch));
@@ -220,6 +223,17 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   EP_TRY(c->alloc(&b.long_st, long_cap));
   EP_TRY(c->alloc(&b.long_len, long_cap));
   EP_TRY(c->alloc(&b.long_pbase, long_cap));
+  EP_TRY(c->alloc(&b.long_runs, (size_t)long_cap * ep::kMaxGroup));
+  // plan regions: wave w of the row kernels covers cpw chunks of 64 keys; up to 64 short
+  // and 2 medium rows start in a chunk
+  const uint64_t nwaves = (uint64_t)unique_grid * ep::kBlockWaves;
+  const uint64_t cpw_max = ((keycap + 63) / 64 + nwaves - 1) / nwaves;
+  EP_TRY(c->alloc(&b.srec, nwaves * cpw_max * 64));
+  EP_TRY(c->alloc(&b.mrec, nwaves * cpw_max * 2));
+  EP_TRY(c->alloc(&b.scount, nwaves));
+  EP_TRY(c->alloc(&b.mcount, nwaves));
+  b.srec_cap = (uint32_t)(nwaves * cpw_max * 64);
+  b.mrec_cap = (uint32_t)(nwaves * cpw_max * 2);
   EP_TRY(c->alloc(&b.piece_st, piece_cap));
   EP_TRY(c->alloc(&b.piece_info, piece_cap));
   EP_TRY(c->alloc(&b.part, (size_t)piece_cap * D));
@@ -243,7 +257,7 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   b.ids_cap = (uint32_t)std::min<uint64_t>(8192, example_rows_max(c));
   // fuse emission into the forward while the first-leaf hash is small (<= 16 KiB of LDS)
   b.fuse_emit = (cfg.grad_mode == EP_GRAD_EXACT || hash <= 4096) ? 1 : 0;
-  b.unique_grid = 2048;
+  b.unique_grid = unique_grid;
   c->cap_batch = batch;
   c->cap_S = S;
   c->key_cap = (uint32_t)keycap;
@@ -923,7 +937,9 @@ int ep_step_explicit(ep_ctx* c, const uint32_t* anchors, uint32_t n, const uint3
     if (grad) EP_HIP(hipMemcpy(grad, sc.G, (size_t)hs.n_unique * D * sizeof(float), hipMemcpyDeviceToHost));
     if (upd) {
       // the update pass re-reads the sorted keys; its long-segment lists start empty
-      const uint32_t zero4[4] = {0u, 0u, 0u, 0u};
+      const uint32_t zero4[5] = {0u, 0u, 0u, 0u, 0u};  // n_unique, n_long, n_pieces, n_touched, n_short
+      static_assert(offsetof(ep::StepState, n_short) == offsetof(ep::StepState, n_unique) + 4 * sizeof(uint32_t),
+                    "counter layout");
       EP_HIP(hipMemcpy((char*)c->d_state + offsetof(ep::StepState, n_unique), zero4, sizeof(zero4),
                        hipMemcpyHostToDevice));
       EP_HIP(ep::launch_scatter_sorted(sc, c->d_state, buf, c->stream));

@@ -10,6 +10,8 @@ namespace ep {
 // parameters (the reference's concurrently running Hogwild batches, mod.rs:174 par_iter
 // over chunks); their updates are applied one after the other in batch order.
 constexpr uint32_t kMaxGroup = 8;
+// Waves per 256-thread block of every kernel.
+constexpr int kBlockWaves = 4;
 
 // Device-resident per-step state.  Written by k_step_begin / k_scan / the scatter
 // kernels; read by every kernel of the step, so one captured step graph can be
@@ -29,6 +31,7 @@ struct StepState {
   uint32_t n_long;         // long segments deferred this step
   uint32_t n_pieces;       // pieces of long segments this step
   uint32_t n_touched;      // rows touched by the atomic path this step
+  uint32_t n_short;        // short rows planned for k_seg_update this step
   uint32_t err;            // nonzero: key capacity exceeded this step
   uint32_t n_batches;      // reference batches in this step's group (<= kMaxGroup)
   float lr[8];             // learning rate of each batch of the group (kMaxGroup)
@@ -112,14 +115,28 @@ struct StepConfig {
   // scatter (sort path)
   float* G;                // [key_cap*D] compact gradient rows (bounded by unique count)
   uint32_t* ufeat;         // [key_cap]
-  // Long rows: first sorted entry, first piece and piece count.  A piece is at most
-  // kPieceEntries entries of one (row, batch) run: start entry and info = len |
-  // batch << 8 | (last piece of the run) << 16.
+  // k_seg_plan wave w covers chunks [w*cpw, (w+1)*cpw) of 64 sorted entries (cpw from
+  // the step's key count and the grid) and writes, without atomics, into its own
+  // regions: short rows (<= kSegShort entries) at srec[w*cpw*64 ...] with count
+  // scount[w], one record {first entry, row id, run-start mask (bit i: entry i starts
+  // a (row, batch) run), len | batch of run j << (6 + 3j)}; medium rows (more entries,
+  // inside the 128-entry window, at most 2 per chunk) at mrec[w*cpw*2 ...] with count
+  // mcount[w], {first entry, len}.
+  uint4* srec;             // [srec_cap]
+  uint32_t srec_cap;
+  uint2* mrec;             // [mrec_cap]
+  uint32_t mrec_cap;
+  uint32_t* scount;        // [unique_grid * NW]
+  uint32_t* mcount;        // [unique_grid * NW]
+  // Long rows (running past the window): first sorted entry, first piece, run count and
+  // per run (piece count | batch << 24); the pieces of a row follow each other run by
+  // run.  A piece is at most kPieceEntries entries of one run: start entry and length.
   uint32_t* long_st;       // [long_cap]
-  uint32_t* long_len;      // pieces of the row
+  uint32_t* long_len;      // runs of the row
   uint32_t* long_pbase;
+  uint32_t* long_runs;     // [long_cap * kMaxGroup]
   uint32_t* piece_st;      // [piece_cap]
-  uint32_t* piece_info;    // [piece_cap]
+  uint32_t* piece_info;    // [piece_cap] entries of the piece
   float* part;             // [piece_cap*D]
   uint32_t long_cap, piece_cap;
   // scatter (atomic path)

@@ -14,7 +14,7 @@
 
 namespace ep {
 
-static constexpr int NW = 4;             // waves per 256-thread block
+static constexpr int NW = kBlockWaves;   // waves per 256-thread block
 static constexpr uint32_t PIECE = kPieceEntries;  // entries per piece of a long segment
 static constexpr int SR_UNROLL = 16;       // adjoint rows in flight per wave in the scatter
 static constexpr uint32_t HASH_EMPTY = 0xFFFFFFFFu;
@@ -216,6 +216,7 @@ __global__ void k_step_begin(StepConfig c, StepState* st, float lr_override) {
     st->n_long = 0u;
     st->n_pieces = 0u;
     st->n_touched = 0u;
+    st->n_short = 0u;
     st->stat_examples += bn;
   }
 }
@@ -965,79 +966,70 @@ __device__ inline void adam_row(const StepConfig& c, const StepScalars& sv, uint
 }
 
 // ------------------------------------------------------------------------------------
-// Fused segment reduce + Adam over the sorted keys (the batch merge of mod.rs:223-233
-// followed by the update of each batch of the group, in batch order).  The sorted keys
-// form rows (same row id) made of runs (same row and batch).  A wave takes a 64-entry
-// chunk and owns every row that starts in it.  Rows of at most kSegShort entries are
-// handled here: lane groups of L = D/VPL lanes (VPL values per lane, as VPL/4 float4
-// columns strided by L) each take one row, G = 64/L rows in parallel: the row's
-// theta/m/v are loaded once, each run is reduced (U adjoint rows in flight per group)
-// and applied as one Adam step with its batch's learning rate, and the row is stored
-// once.  Longer rows (and a last row running past the 128-entry window) are registered
-// as long rows, cut into pieces of at most kPieceEntries entries of one run: k_pieces
-// sums the pieces in parallel and k_long_combine adds them in order and updates.  Every
-// (VPL, U) adds each run's rows in entry order, so the variants give identical bits and
-// the tuner picks the fastest.  grad_out != 0 writes each run's merged gradient to
-// G/ufeat (ufeat = row | batch << 27) instead of updating (test hook).
+// Batch merge + update over the sorted keys (mod.rs:223-233 merge, then the update of
+// each batch of the group in batch order).  The sorted keys form rows (same row id)
+// made of runs (same row and batch).
+//   k_seg_plan    one wave per 64-entry chunk; owns the rows that start in it.  Rows of
+//                 at most kSegShort entries become short-row records (first entry, row,
+//                 run-start mask, run batches); longer rows, and a last row running past
+//                 the 128-entry window, become long rows cut into pieces of at most
+//                 kPieceEntries entries of one run.
+//   k_seg_update  short rows, balanced over lane groups: theta/m/v loaded once, each run
+//                 reduced (adjoint rows in entry order) and applied as one Adam step
+//                 with its batch's learning rate, the row stored once.
+//   k_pieces      partial sums of the pieces.
+//   k_long_combine  one wave per long row: pieces added in order, one Adam step per run.
+// grad_out != 0 writes each run's merged gradient to G/ufeat (ufeat = row | batch << 27)
+// instead of updating (test hook).
 // ------------------------------------------------------------------------------------
 
-// Registers long row `idx` with first piece `pb`: one piece record per kPieceEntries
-// entries of each run.  run_at(j) gives run j's [start, end) (absolute) and batch.
+// Pieces of the runs of one long row, starting at piece pb; run_at(j) gives run j's
+// absolute [start, end) and batch.  Lanes lane0, lane0 + stride, ... write the pieces.
 template <typename RunAt>
-__device__ inline void write_pieces(const StepConfig& c, uint32_t pb, uint32_t nruns, RunAt run_at, uint32_t lane0,
-                                    uint32_t stride) {
+__device__ inline void write_long_row(const StepConfig& c, uint32_t idx, uint32_t st0, uint32_t pb, uint32_t nruns,
+                                      RunAt run_at, uint32_t lane0, uint32_t stride) {
+  if (lane0 == 0u) {
+    c.long_st[idx] = st0;
+    c.long_pbase[idx] = pb;
+    c.long_len[idx] = nruns;
+  }
   uint32_t p = pb;
   for (uint32_t j = 0; j < nruns; ++j) {
     uint32_t a, z, bq;
     run_at(j, &a, &z, &bq);
     const uint32_t np = (z - a + PIECE - 1u) / PIECE;
+    if (lane0 == 0u) c.long_runs[(size_t)idx * kMaxGroup + j] = np | (bq << 24);
     for (uint32_t q = lane0; q < np; q += stride) {
       const uint32_t ps = a + q * PIECE;
-      const uint32_t len = z - ps < PIECE ? z - ps : PIECE;
       if (p + q < c.piece_cap) {
         c.piece_st[p + q] = ps;
-        c.piece_info[p + q] = len | (bq << 8) | ((q + 1u == np) ? (1u << 16) : 0u);
+        c.piece_info[p + q] = z - ps < PIECE ? z - ps : PIECE;
       }
     }
     p += np;
   }
 }
 
-template <int D, int VPL, int U>
-__global__ __launch_bounds__(256) void k_seg_adam(StepConfig c, StepState* st, const uint32_t* __restrict__ Ks,
-                                                  const uint32_t* __restrict__ Ss) {
-  static_assert(D % VPL == 0 && VPL % 4 == 0 && D / VPL <= 64, "float4 lane slices");
-  constexpr int L = D / VPL;      // lanes per row
-  constexpr int G = 64 / L;       // rows in flight per wave
-  constexpr int NQ = VPL / 4;     // float4 columns per lane
+__global__ __launch_bounds__(256) void k_seg_plan(StepConfig c, StepState* st, const uint32_t* __restrict__ Ks) {
   __shared__ uint32_t s_run[NW][132];  // run starts (chunk-relative); s_run[nrun] = window end
   __shared__ uint8_t s_bat[NW][132];   // batch of each run
   __shared__ uint32_t s_row[NW][66];   // first run of each owned row; s_row[nrow] = nrun
-  __shared__ uint32_t s_sr[NW][64];    // short rows: first run | end run << 8
-  __shared__ float s_lr[kMaxGroup];    // learning rate of each batch of the group
-  __shared__ unsigned long long s_cnt;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int grp = lane / L, lg = lane % L;
-  const bool gact = grp < G;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const uint64_t gmask = (L == 64) ? ~0ull : (((1ull << L) - 1ull) << ((grp * L) & 63));
   const uint32_t gw = blockIdx.x * NW + w;
   const uint32_t nwaves = gridDim.x * NW;
   const uint32_t n = st->n_keys <= c.key_cap ? st->n_real : 0u;  // sorted, sentinels dropped
   const uint32_t SENT = c.sentinel, PB = c.pbits, PM = c.pmask;
-  const StepScalars sv = step_scalars(st);
-  if (tid == 0) s_cnt = 0ull;
-  if (tid < (int)kMaxGroup) s_lr[tid] = st->lr[tid];
-  __syncthreads();
-  uint32_t processed = 0;
-  for (uint64_t chunk = gw; chunk * 64u < n; chunk += nwaves) {
-    const uint32_t i0 = (uint32_t)(chunk * 64u);
+  const uint32_t nchunks = (n + 63u) / 64u;
+  const uint32_t cpw = (nchunks + nwaves - 1u) / nwaves;
+  const uint32_t cbeg = gw * cpw, cend = min(nchunks, cbeg + cpw);
+  const size_t sbase = (size_t)gw * cpw * 64u, mbase = (size_t)gw * cpw * 2u;
+  uint32_t ns = 0, nm = 0;  // records written to this wave's regions
+  for (uint32_t chunk = cbeg; chunk < cend; ++chunk) {
+    const uint32_t i0 = chunk * 64u;
     const uint32_t j0 = i0 + lane, j1 = i0 + 64u + lane;
     const uint32_t k0 = j0 < n ? Ks[j0] : SENT;
     const uint32_t k1 = j1 < n ? Ks[j1] : SENT;
-    const uint32_t s0 = j0 < n ? Ss[j0] : 0u;
-    const uint32_t s1 = j1 < n ? Ss[j1] : 0u;
-    if (__shfl(k0, 0, EP_WAVE) == SENT) break;  // sorted: the rest is sentinel
     uint32_t prev = __shfl_up(k0, 1, EP_WAVE);
     if (lane == 0) prev = i0 > 0u ? Ks[i0 - 1u] : ~k0;
     const uint32_t last0 = __shfl(k0, 63, EP_WAVE);
@@ -1092,11 +1084,6 @@ __global__ __launch_bounds__(256) void k_seg_adam(StepConfig c, StepState* st, c
       if (lane == 0) {
         idx = atomicAdd(&st->n_long, 1u);
         pb = atomicAdd(&st->n_pieces, pcs);
-        if (idx < c.long_cap) {
-          c.long_st[idx] = lo;
-          c.long_len[idx] = pcs;
-          c.long_pbase[idx] = pb;
-        }
       }
       idx = __shfl(idx, 0, EP_WAVE);
       pb = __shfl(pb, 0, EP_WAVE);
@@ -1109,194 +1096,277 @@ __global__ __launch_bounds__(256) void k_seg_adam(StepConfig c, StepState* st, c
           *z = bnd[k + 1];
           *bq = k;
         };
-        write_pieces(c, pb, nr, run_at, (uint32_t)lane, 64u);
+        write_long_row(c, idx, lo, pb, nr, run_at, (uint32_t)lane, 64u);
       }
     }
     __builtin_amdgcn_wave_barrier();
-    // classify the owned rows: short ones stay here, longer ones become long rows
+    // classify the owned rows of the window: short records or medium records
     const bool own = (uint32_t)lane < nrow;
     const uint32_t ra = own ? s_row[w][lane] : 0u;
     const uint32_t rz = own ? s_row[w][lane + 1] : 0u;
     const uint32_t ea = own ? s_run[w][ra] : 0u;
     const uint32_t ez = own ? s_run[w][rz] : 0u;
-    const bool lng = own && (ez - ea) > kSegShort;
-    const uint64_t lmask = __ballot(lng);
-    const uint64_t smask = __ballot(own && !lng);
-    if (own && !lng) s_sr[w][__popcll(smask & lt)] = ra | (rz << 8);
-    if (lmask) {
-      uint32_t pcs = 0;
-      if (lng)
-        for (uint32_t j = ra; j < rz; ++j) pcs += (s_run[w][j + 1] - s_run[w][j] + PIECE - 1u) / PIECE;
-      uint32_t incl = pcs;  // inclusive prefix sum of the piece counts
+    const bool med = own && (ez - ea) > kSegShort;
+    const uint64_t mmask = __ballot(med);
+    const uint64_t smask = __ballot(own && !med);
+    const uint32_t f = row_of_key(c, __shfl(k0, (int)ea, EP_WAVE));  // row starts lie in [0, 64)
+    if (own && !med) {
+      uint32_t mask = 0u, meta = ez - ea;
+      for (uint32_t j = ra; j < rz; ++j) {
+        mask |= 1u << (s_run[w][j] - ea);
+        meta |= (uint32_t)s_bat[w][j] << (6u + 3u * (j - ra));
+      }
+      c.srec[sbase + ns + (uint32_t)__popcll(smask & lt)] = make_uint4(i0 + ea, f, mask, meta);
+    }
+    if (med) c.mrec[mbase + nm + (uint32_t)__popcll(mmask & lt)] = make_uint2(i0 + ea, ez - ea);
+    ns += (uint32_t)__popcll(smask);
+    nm += (uint32_t)__popcll(mmask);
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (lane == 0) {
+    c.scount[gw] = ns;
+    c.mcount[gw] = nm;
+  }
+}
+
+// Short rows: lane groups of L = D/VPL lanes (VPL values per lane as VPL/4 float4
+// columns strided by L), G = 64/L rows per wave, U adjoint rows in flight per group.
+// Every (VPL, U) adds each run's rows in entry order: identical bits, so the tuner picks
+// the fastest.
+template <int D, int VPL, int U>
+__global__ __launch_bounds__(256) void k_seg_update(StepConfig c, StepState* st, const uint32_t* __restrict__ Ss) {
+  static_assert(D % VPL == 0 && VPL % 4 == 0 && D / VPL <= 64, "float4 lane slices");
+  constexpr int L = D / VPL;      // lanes per row
+  constexpr int G = 64 / L;       // rows in flight per wave
+  constexpr int NQ = VPL / 4;     // float4 columns per lane
+  constexpr int NS = ((int)kSegShort + L - 1) / L;  // slots held per lane
+  __shared__ float s_lr[kMaxGroup];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int grp = lane / L, lg = lane % L;
+  const bool gact = grp < G;
+  const uint64_t gmask = (L == 64) ? ~0ull : (((1ull << L) - 1ull) << ((grp * L) & 63));
+  const uint32_t gw = blockIdx.x * NW + (tid >> 6);
+  const uint32_t nwaves = gridDim.x * NW;
+  // this wave's region of short-row records (written by k_seg_plan wave gw)
+  const uint32_t n = st->n_keys <= c.key_cap ? st->n_real : 0u;
+  const uint32_t nchunks = (n + 63u) / 64u;
+  const uint32_t cpw = (nchunks + nwaves - 1u) / nwaves;
+  const uint32_t nrec = gw * cpw < nchunks ? c.scount[gw] : 0u;
+  const uint4* recs = c.srec + (size_t)gw * cpw * 64u;
+  const StepScalars sv = step_scalars(st);
+  if (tid < (int)kMaxGroup) s_lr[tid] = st->lr[tid];
+  __syncthreads();
+  uint32_t done = 0;
+  for (uint32_t j0 = 0; j0 < nrec; j0 += (uint32_t)G) {
+    const uint32_t j = j0 + (uint32_t)grp;
+    const bool has = gact && j < nrec;
+    const uint4 rec = has ? recs[j] : make_uint4(0u, 0u, 0u, 0u);
+    const uint32_t start = rec.x, f = rec.y, rmask = rec.z;
+    const uint32_t len = rec.w & 63u;
+    // the row's slots, spread over the group's lanes (entry e at lane e % L, register e / L)
+    uint32_t sl[NS];
 #pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t t = __shfl_up(incl, off, EP_WAVE);
-        if (lane >= off) incl += t;
-      }
-      const uint32_t tot = __shfl(incl, 63, EP_WAVE);
-      uint32_t idx0 = 0, pb0 = 0;
-      if (lane == 0) {
-        idx0 = atomicAdd(&st->n_long, (uint32_t)__popcll(lmask));
-        pb0 = atomicAdd(&st->n_pieces, tot);
-      }
-      idx0 = __shfl(idx0, 0, EP_WAVE);
-      pb0 = __shfl(pb0, 0, EP_WAVE);
-      if (lng) {
-        const uint32_t idx = idx0 + (uint32_t)__popcll(lmask & lt);
-        const uint32_t pb = pb0 + incl - pcs;
-        if (idx < c.long_cap) {
-          c.long_st[idx] = i0 + ea;
-          c.long_len[idx] = pcs;
-          c.long_pbase[idx] = pb;
-          auto run_at = [&](uint32_t j, uint32_t* a, uint32_t* z, uint32_t* bq) {
-            *a = i0 + s_run[w][ra + j];
-            *z = i0 + s_run[w][ra + j + 1];
-            *bq = s_bat[w][ra + j];
-          };
-          write_pieces(c, pb, rz - ra, run_at, 0u, 1u);
+    for (int i = 0; i < NS; ++i) {
+      const uint32_t e = (uint32_t)(lg + i * L);
+      sl[i] = (has && e < len) ? Ss[start + e] : 0u;
+    }
+    float4 m4[NQ], v4[NQ], t4[NQ];
+    const size_t roff = (size_t)f * D;
+    const size_t ooff = (size_t)f * c.ostride;  // optimizer record [master |] m | v
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const uint32_t col = (uint32_t)(q * L + lg) * 4u;
+      m4[q] = v4[q] = t4[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (has && !c.grad_out) {
+        m4[q] = *reinterpret_cast<const float4*>(c.mom + ooff + col);
+        v4[q] = *reinterpret_cast<const float4*>(c.var + ooff + col);
+        if (c.master) {
+          t4[q] = *reinterpret_cast<const float4*>(c.master + ooff + col);
+        } else if (c.bf16) {
+          const uint2 b = *reinterpret_cast<const uint2*>((const uint16_t*)c.table + roff + col);
+          t4[q] = make_float4(__uint_as_float(b.x << 16), __uint_as_float(b.x & 0xffff0000u),
+                              __uint_as_float(b.y << 16), __uint_as_float(b.y & 0xffff0000u));
+        } else {
+          t4[q] = *reinterpret_cast<const float4*>((const float*)c.table + roff + col);
         }
       }
     }
-    __builtin_amdgcn_wave_barrier();
-    const uint32_t nsr = (uint32_t)__popcll(smask);
-    processed += nsr;
-    for (uint32_t q0 = 0; q0 < nsr; q0 += (uint32_t)G) {
-      const uint32_t j = q0 + (uint32_t)grp;
-      const bool has = gact && j < nsr;
-      const uint32_t pk = has ? s_sr[w][j] : 0u;
-      const uint32_t fa = pk & 0xffu, fz = pk >> 8;  // runs [fa, fz)
-      const uint32_t f = row_of_key(c, __shfl(k0, (int)(has ? s_run[w][fa] : 0u), EP_WAVE));
-      // Adam operands first: independent of the reduction, so their latency overlaps it
-      float4 m4[NQ], v4[NQ], t4[NQ];
-      const size_t roff = (size_t)f * D;
-      const size_t ooff = (size_t)f * c.ostride;  // optimizer record [master |] m | v
+    const uint32_t nruns = has ? (uint32_t)__popc(rmask) : 0u;
+    uint32_t maxruns = nruns;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) maxruns = max(maxruns, (uint32_t)__shfl_xor(maxruns, off, EP_WAVE));
+    bool upd = false;
+    uint32_t rest = rmask;  // run starts not yet consumed
+    for (uint32_t rq = 0; rq < maxruns; ++rq) {
+      const bool hr = rq < nruns;
+      const uint32_t a = hr ? (uint32_t)__builtin_ctz(rest) : 0u;
+      rest = hr ? (rest & (rest - 1u)) : rest;
+      const uint32_t z = hr ? (rest ? (uint32_t)__builtin_ctz(rest) : len) : 0u;
+      const uint32_t bq = (rec.w >> (6u + 3u * rq)) & 7u;
+      uint32_t maxlen = z - a;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) maxlen = max(maxlen, (uint32_t)__shfl_xor(maxlen, off, EP_WAVE));
+      float4 acc[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) acc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+      for (uint32_t u0 = 0; u0 < maxlen; u0 += U) {
+        float4 r[U][NQ];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const uint32_t e = a + u0 + (uint32_t)u;
+          const bool valid = hr && e < z;
+          const uint32_t ei = e / (uint32_t)L;
+          uint32_t pick = sl[0];
+#pragma unroll
+          for (int i = 1; i < NS; ++i) pick = (ei == (uint32_t)i) ? sl[i] : pick;
+          const uint32_t slot = __shfl(pick, grp * L + (int)(e % (uint32_t)L), EP_WAVE);
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            const uint32_t col = (uint32_t)(q * L + lg) * 4u;
+            r[u][q] = valid ? *reinterpret_cast<const float4*>(c.adj + (size_t)slot * D + col)
+                            : make_float4(0.f, 0.f, 0.f, 0.f);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            acc[q].x += r[u][q].x; acc[q].y += r[u][q].y; acc[q].z += r[u][q].z; acc[q].w += r[u][q].w;
+          }
+        }
+      }
+      if (c.grad_out) {
+        uint32_t ub = 0;
+        const uint64_t heads = __ballot(hr && lg == 0);
+        if (lane == 0 && heads) ub = atomicAdd(&st->n_unique, (uint32_t)__popcll(heads));
+        ub = __shfl(ub, 0, EP_WAVE);
+        const uint32_t u = ub + (uint32_t)__popcll(heads & ((grp * L == 0) ? 0ull : (~0ull >> (64 - grp * L))));
+        if (hr) {
+#pragma unroll
+          for (int q = 0; q < NQ; ++q)
+            *reinterpret_cast<float4*>(c.G + (size_t)u * D + (q * L + lg) * 4) = acc[q];
+          if (lg == 0) c.ufeat[u] = f | (bq << 27);
+        }
+        continue;
+      }
+      bool bad = false;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) bad |= isnan(acc[q].x) || isnan(acc[q].y) || isnan(acc[q].z) || isnan(acc[q].w);
+      const bool skip = (__ballot(hr && bad) & gmask) != 0ull;  // optimizer.rs:56, per batch
+      if (hr && !skip) {
+        const float lr = s_lr[bq];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          adam_math(t4[q].x, m4[q].x, v4[q].x, acc[q].x, lr, sv.bc1, sv.bc2);
+          adam_math(t4[q].y, m4[q].y, v4[q].y, acc[q].y, lr, sv.bc1, sv.bc2);
+          adam_math(t4[q].z, m4[q].z, v4[q].z, acc[q].z, lr, sv.bc1, sv.bc2);
+          adam_math(t4[q].w, m4[q].w, v4[q].w, acc[q].w, lr, sv.bc1, sv.bc2);
+        }
+        upd = true;
+      }
+    }
+    if (upd) {
+      if (lg == 0) ++done;
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         const uint32_t col = (uint32_t)(q * L + lg) * 4u;
-        m4[q] = v4[q] = t4[q] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (has && !c.grad_out) {
-          m4[q] = *reinterpret_cast<const float4*>(c.mom + ooff + col);
-          v4[q] = *reinterpret_cast<const float4*>(c.var + ooff + col);
-          if (c.master) {
-            t4[q] = *reinterpret_cast<const float4*>(c.master + ooff + col);
-          } else if (c.bf16) {
-            const uint2 b = *reinterpret_cast<const uint2*>((const uint16_t*)c.table + roff + col);
-            t4[q] = make_float4(__uint_as_float(b.x << 16), __uint_as_float(b.x & 0xffff0000u),
-                                __uint_as_float(b.y << 16), __uint_as_float(b.y & 0xffff0000u));
-          } else {
-            t4[q] = *reinterpret_cast<const float4*>((const float*)c.table + roff + col);
-          }
-        }
-      }
-      uint32_t maxruns = fz - fa;
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) maxruns = max(maxruns, (uint32_t)__shfl_xor(maxruns, off, EP_WAVE));
-      bool upd = false;
-      for (uint32_t rq = 0; rq < maxruns; ++rq) {
-        const bool hr = has && rq < fz - fa;
-        const uint32_t sa = hr ? s_run[w][fa + rq] : 0u;
-        const uint32_t sb = hr ? s_run[w][fa + rq + 1] : 0u;
-        const uint32_t bq = hr ? s_bat[w][fa + rq] : 0u;
-        const uint32_t len = sb - sa;
-        float4 acc[NQ];
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) acc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
-        uint32_t maxlen = len;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) maxlen = max(maxlen, (uint32_t)__shfl_xor(maxlen, off, EP_WAVE));
-        for (uint32_t u0 = 0; u0 < maxlen; u0 += U) {
-          float4 r[U][NQ];
-#pragma unroll
-          for (int u = 0; u < U; ++u) {
-            const uint32_t t = sa + u0 + (uint32_t)u;  // chunk-relative entry
-            const bool valid = hr && (u0 + (uint32_t)u < len);
-            const uint32_t src = t & 63u;
-            const uint32_t sl0 = __shfl(s0, (int)src, EP_WAVE);
-            const uint32_t sl1 = __shfl(s1, (int)src, EP_WAVE);
-            const uint32_t sl = t < 64u ? sl0 : sl1;
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-              const uint32_t col = (uint32_t)(q * L + lg) * 4u;
-              r[u][q] = valid ? *reinterpret_cast<const float4*>(c.adj + (size_t)sl * D + col)
-                              : make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-          }
-#pragma unroll
-          for (int u = 0; u < U; ++u) {
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-              acc[q].x += r[u][q].x; acc[q].y += r[u][q].y; acc[q].z += r[u][q].z; acc[q].w += r[u][q].w;
-            }
-          }
-        }
-        if (c.grad_out) {
-          uint32_t ub = 0;
-          const uint64_t heads = __ballot(hr && lg == 0);
-          if (lane == 0 && heads) ub = atomicAdd(&st->n_unique, (uint32_t)__popcll(heads));
-          ub = __shfl(ub, 0, EP_WAVE);
-          const uint32_t u = ub + (uint32_t)__popcll(heads & ((grp * L == 0) ? 0ull : (~0ull >> (64 - grp * L))));
-          if (hr) {
-#pragma unroll
-            for (int q = 0; q < NQ; ++q)
-              *reinterpret_cast<float4*>(c.G + (size_t)u * D + (q * L + lg) * 4) = acc[q];
-            if (lg == 0) c.ufeat[u] = f | (bq << 27);
-          }
-          continue;
-        }
-        bool bad = false;
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) bad |= isnan(acc[q].x) || isnan(acc[q].y) || isnan(acc[q].z) || isnan(acc[q].w);
-        const bool skip = (__ballot(hr && bad) & gmask) != 0ull;  // optimizer.rs:56, per batch
-        if (hr && !skip) {
-          const float lr = s_lr[bq];
-#pragma unroll
-          for (int q = 0; q < NQ; ++q) {
-            adam_math(t4[q].x, m4[q].x, v4[q].x, acc[q].x, lr, sv.bc1, sv.bc2);
-            adam_math(t4[q].y, m4[q].y, v4[q].y, acc[q].y, lr, sv.bc1, sv.bc2);
-            adam_math(t4[q].z, m4[q].z, v4[q].z, acc[q].z, lr, sv.bc1, sv.bc2);
-            adam_math(t4[q].w, m4[q].w, v4[q].w, acc[q].w, lr, sv.bc1, sv.bc2);
-          }
-          upd = true;
-        }
-      }
-      if (upd) {
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-          const uint32_t col = (uint32_t)(q * L + lg) * 4u;
-          *reinterpret_cast<float4*>(c.mom + ooff + col) = m4[q];
-          *reinterpret_cast<float4*>(c.var + ooff + col) = v4[q];
-          if (c.bf16) {
-            if (c.master) *reinterpret_cast<float4*>(c.master + ooff + col) = t4[q];
-            const uint32_t h0 = sr_key(f, col >> 2, sv.sr_step, (uint32_t)c.seed);
-            const uint32_t h1 = hash32(h0 ^ 0x27D4EB2Fu);
-            uint2 pk2;
-            pk2.x = (uint32_t)f32_to_bf16_sr(t4[q].x, h0 & 0xffffu) | ((uint32_t)f32_to_bf16_sr(t4[q].y, h0 >> 16) << 16);
-            pk2.y = (uint32_t)f32_to_bf16_sr(t4[q].z, h1 & 0xffffu) | ((uint32_t)f32_to_bf16_sr(t4[q].w, h1 >> 16) << 16);
-            *reinterpret_cast<uint2*>((uint16_t*)c.table + roff + col) = pk2;
-          } else {
-            *reinterpret_cast<float4*>((float*)c.table + roff + col) = t4[q];
-          }
+        *reinterpret_cast<float4*>(c.mom + ooff + col) = m4[q];
+        *reinterpret_cast<float4*>(c.var + ooff + col) = v4[q];
+        if (c.bf16) {
+          if (c.master) *reinterpret_cast<float4*>(c.master + ooff + col) = t4[q];
+          const uint32_t h0 = sr_key(f, col >> 2, sv.sr_step, (uint32_t)c.seed);
+          const uint32_t h1 = hash32(h0 ^ 0x27D4EB2Fu);
+          uint2 pk2;
+          pk2.x = (uint32_t)f32_to_bf16_sr(t4[q].x, h0 & 0xffffu) | ((uint32_t)f32_to_bf16_sr(t4[q].y, h0 >> 16) << 16);
+          pk2.y = (uint32_t)f32_to_bf16_sr(t4[q].z, h1 & 0xffffu) | ((uint32_t)f32_to_bf16_sr(t4[q].w, h1 >> 16) << 16);
+          *reinterpret_cast<uint2*>((uint16_t*)c.table + roff + col) = pk2;
+        } else {
+          *reinterpret_cast<float4*>((float*)c.table + roff + col) = t4[q];
         }
       }
     }
   }
-  if (lane == 0 && processed) atomicAdd(&s_cnt, (unsigned long long)processed);
-  __syncthreads();
-  if (tid == 0 && s_cnt) atomicAdd((unsigned long long*)&st->stat_unique, s_cnt);
+  done = wave_sum_u32(done);
+  if (lane == 0 && done) atomicAdd((unsigned long long*)&st->stat_unique, (unsigned long long)done);
 }
 
-// Partial sums of the pieces of long rows (each piece: <= kPieceEntries entries of one run).
+// Loads row f's optimizer operands (whole-wave slice layout).
 template <int D>
-__global__ __launch_bounds__(256) void k_pieces(StepConfig c, StepState* st, const uint32_t* __restrict__ Ss) {
+__device__ inline void load_row_state(const StepConfig& c, uint32_t f, int lane, float (&th)[RowShape<D>::PL],
+                                      float (&m)[RowShape<D>::PL], float (&v)[RowShape<D>::PL]) {
+  const size_t off = (size_t)f * D, ooff = (size_t)f * c.ostride;
+  load_slice<D>(c.mom + ooff, lane, m);
+  load_slice<D>(c.var + ooff, lane, v);
+  if (c.master) load_slice<D>(c.master + ooff, lane, th);
+  else if (c.bf16) load_slice<D>((const uint16_t*)c.table + off, lane, th);
+  else load_slice<D>((const float*)c.table + off, lane, th);
+}
+
+// Stores row f's optimizer operands; bf16 tables get the stochastically rounded theta.
+template <int D>
+__device__ inline void store_row_state(const StepConfig& c, const StepScalars& sv, uint32_t f, int lane,
+                                       const float (&th)[RowShape<D>::PL], const float (&m)[RowShape<D>::PL],
+                                       const float (&v)[RowShape<D>::PL]) {
+  constexpr int PL = RowShape<D>::PL;
+  const int base = lane * PL;
+  const size_t off = (size_t)f * D, ooff = (size_t)f * c.ostride;
+  store_slice<D>(c.mom + ooff, lane, m);
+  store_slice<D>(c.var + ooff, lane, v);
+  if (c.bf16) {
+    if (c.master) store_slice<D>(c.master + ooff, lane, th);
+    uint16_t* row = (uint16_t*)c.table + off;
+#pragma unroll
+    for (int q = 0; q < PL; ++q)
+      if (base + q < D)
+        row[base + q] = f32_to_bf16_sr(th[q], sr_bits(c, sv, f, (uint32_t)((base + q) >> 2), (base + q) & 3));
+  } else {
+    store_slice<D>((float*)c.table + off, lane, th);
+  }
+}
+
+// Applies one run's merged gradient (NaN rows skipped, optimizer.rs:56) or, in grad_out
+// mode, writes it as a (row, batch) record.  Returns whether theta changed.
+template <int D>
+__device__ inline bool apply_run(const StepConfig& c, StepState* st, const StepScalars& sv, float lr, uint32_t f,
+                                 uint32_t bq, int lane, const float (&acc)[RowShape<D>::PL], float (&th)[RowShape<D>::PL],
+                                 float (&m)[RowShape<D>::PL], float (&v)[RowShape<D>::PL]) {
+  constexpr int PL = RowShape<D>::PL;
+  const int base = lane * PL;
+  if (c.grad_out) {
+    uint32_t ui = 0;
+    if (lane == 0) ui = atomicAdd(&st->n_unique, 1u);
+    ui = __shfl(ui, 0, EP_WAVE);
+    store_slice<D>(c.G + (size_t)ui * D, lane, acc);
+    if (lane == 0) c.ufeat[ui] = f | (bq << 27);
+    return false;
+  }
+  bool bad = false;
+#pragma unroll
+  for (int q = 0; q < PL; ++q) bad |= (base + q < D) && isnan(acc[q]);
+  if (wave_any(bad)) return false;
+#pragma unroll
+  for (int q = 0; q < PL; ++q) adam_math(th[q], m[q], v[q], acc[q], lr, sv.bc1, sv.bc2);
+  return true;
+}
+
+// Partial sums of the pieces of long rows (each piece: <= kPieceEntries entries of one
+// run), then the medium rows of this wave's plan region: one wave per row (<= 128
+// entries), its runs reduced from the adjoint rows and applied in batch order.
+template <int D>
+__global__ __launch_bounds__(256) void k_pieces(StepConfig c, StepState* st, const uint32_t* __restrict__ Ks,
+                                                const uint32_t* __restrict__ Ss) {
   constexpr int PL = RowShape<D>::PL;
   constexpr int U = SR_UNROLL;
+  __shared__ float s_lr[kMaxGroup];
   const int lane = threadIdx.x & 63;
   const uint32_t gw = blockIdx.x * NW + (threadIdx.x >> 6);
   const uint32_t nwaves = gridDim.x * NW;
+  if (threadIdx.x < kMaxGroup) s_lr[threadIdx.x] = st->lr[threadIdx.x];
+  __syncthreads();
   const uint32_t np = st->n_pieces < c.piece_cap ? st->n_pieces : c.piece_cap;
   for (uint32_t gp = gw; gp < np; gp += nwaves) {
     const uint32_t s0 = c.piece_st[gp];
-    const uint32_t m = c.piece_info[gp] & 0xffu;  // <= PIECE <= 64
+    const uint32_t m = c.piece_info[gp];  // <= PIECE <= 64
     const uint32_t sj = (uint32_t)lane < m ? Ss[s0 + lane] : 0u;
     float acc[PL];
 #pragma unroll
@@ -1319,6 +1389,68 @@ __global__ __launch_bounds__(256) void k_pieces(StepConfig c, StepState* st, con
     }
     store_slice<D>(c.part + (size_t)gp * D, lane, acc);
   }
+  // medium rows of plan region gw
+  const uint32_t n = st->n_keys <= c.key_cap ? st->n_real : 0u;
+  const uint32_t nchunks = (n + 63u) / 64u;
+  const uint32_t cpw = (nchunks + nwaves - 1u) / nwaves;
+  const uint32_t nmed = gw * cpw < nchunks ? c.mcount[gw] : 0u;
+  const uint2* recs = c.mrec + (size_t)gw * cpw * 2u;
+  const StepScalars sv = step_scalars(st);
+  uint32_t done = 0;
+  for (uint32_t k = 0; k < nmed; ++k) {
+    const uint2 rec = recs[k];
+    const uint32_t start = rec.x, len = rec.y;  // 32 < len <= 128
+    const uint32_t e0 = (uint32_t)lane, e1 = 64u + (uint32_t)lane;
+    const uint32_t kk0 = e0 < len ? Ks[start + e0] : 0u;
+    const uint32_t kk1 = e1 < len ? Ks[start + e1] : 0u;
+    const uint32_t ss0 = e0 < len ? Ss[start + e0] : 0u;
+    const uint32_t ss1 = e1 < len ? Ss[start + e1] : 0u;
+    const uint32_t key0 = __shfl(kk0, 0, EP_WAVE);
+    const uint32_t f = row_of_key(c, key0);
+    float th[PL], mo[PL], va[PL];
+    if (!c.grad_out) load_row_state<D>(c, f, lane, th, mo, va);
+    const uint32_t last0 = __shfl(kk0, 63, EP_WAVE);
+    uint32_t p0 = __shfl_up(kk0, 1, EP_WAVE);
+    uint32_t p1 = __shfl_up(kk1, 1, EP_WAVE);
+    if (lane == 0) { p0 = ~kk0; p1 = last0; }
+    uint64_t rs0 = __ballot(e0 < len && kk0 != p0);  // run starts
+    uint64_t rs1 = __ballot(e1 < len && kk1 != p1);
+    bool upd = false;
+    while (rs0 | rs1) {
+      const uint32_t a = rs0 ? (uint32_t)(__ffsll((unsigned long long)rs0) - 1) : 64u + (uint32_t)(__ffsll((unsigned long long)rs1) - 1);
+      if (rs0) rs0 &= rs0 - 1ull; else rs1 &= rs1 - 1ull;
+      const uint32_t z = rs0 ? (uint32_t)(__ffsll((unsigned long long)rs0) - 1)
+                             : (rs1 ? 64u + (uint32_t)(__ffsll((unsigned long long)rs1) - 1) : len);
+      const uint32_t ka = a < 64u ? __shfl(kk0, (int)a, EP_WAVE) : __shfl(kk1, (int)(a - 64u), EP_WAVE);
+      const uint32_t bq = ka & c.pmask;
+      float acc[PL];
+#pragma unroll
+      for (int q = 0; q < PL; ++q) acc[q] = 0.f;
+      for (uint32_t t0 = a; t0 < z; t0 += U) {
+        float r[U][PL];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const uint32_t t = (t0 + u < z) ? (t0 + u) : (z - 1u);
+          const uint32_t sa = __shfl(ss0, (int)(t & 63u), EP_WAVE);
+          const uint32_t sb = __shfl(ss1, (int)(t & 63u), EP_WAVE);
+          load_slice<D>(c.adj + (size_t)(t < 64u ? sa : sb) * D, lane, r[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (t0 + u < z) {
+#pragma unroll
+            for (int q = 0; q < PL; ++q) acc[q] += r[u][q];
+          }
+        }
+      }
+      upd |= apply_run<D>(c, st, sv, s_lr[bq], f, bq, lane, acc, th, mo, va);
+    }
+    if (upd) {
+      store_row_state<D>(c, sv, f, lane, th, mo, va);
+      ++done;
+    }
+  }
+  if (lane == 0 && done) atomicAdd((unsigned long long*)&st->stat_unique, (unsigned long long)done);
 }
 
 // One wave per long row: loads theta/m/v once, adds the pieces of each run in order and
@@ -1328,7 +1460,6 @@ __global__ __launch_bounds__(256) void k_long_combine(StepConfig c, StepState* s
   constexpr int PL = RowShape<D>::PL;
   constexpr int U = SR_UNROLL;
   const int lane = threadIdx.x & 63;
-  const int base = lane * PL;
   const uint32_t gw = blockIdx.x * NW + (threadIdx.x >> 6);
   const uint32_t nwaves = gridDim.x * NW;
   const uint32_t nl = st->n_long < c.long_cap ? st->n_long : c.long_cap;
@@ -1338,77 +1469,42 @@ __global__ __launch_bounds__(256) void k_long_combine(StepConfig c, StepState* s
   __syncthreads();
   uint32_t done = 0;
   for (uint32_t idx = gw; idx < nl; idx += nwaves) {
-    const uint32_t pb = c.long_pbase[idx];
-    uint32_t pcs = c.long_len[idx];
-    if (pb >= c.piece_cap) pcs = 0u;
-    else if (pcs > c.piece_cap - pb) pcs = c.piece_cap - pb;
+    const uint32_t pb0 = c.long_pbase[idx];
+    const uint32_t nruns = c.long_len[idx];
     const uint32_t f = row_of_key(c, Ks[c.long_st[idx]]);
-    const size_t off = (size_t)f * D;
-    const size_t ooff = (size_t)f * c.ostride;
-    float m[PL], v[PL], th[PL];
-    if (!c.grad_out) {
-      load_slice<D>(c.mom + ooff, lane, m);
-      load_slice<D>(c.var + ooff, lane, v);
-      if (c.master) load_slice<D>(c.master + ooff, lane, th);
-      else if (c.bf16) load_slice<D>((const uint16_t*)c.table + off, lane, th);
-      else load_slice<D>((const float*)c.table + off, lane, th);
-    }
-    float acc[PL];
-#pragma unroll
-    for (int q = 0; q < PL; ++q) acc[q] = 0.f;
+    float th[PL], m[PL], v[PL];
+    if (!c.grad_out) load_row_state<D>(c, f, lane, th, m, v);
     bool upd = false;
-    for (uint32_t j0 = 0; j0 < pcs; j0 += U) {
-      float r[U][PL];
-      uint32_t inf[U];
+    uint32_t pb = pb0;
+    for (uint32_t jr = 0; jr < nruns && jr < kMaxGroup; ++jr) {
+      const uint32_t rinfo = c.long_runs[(size_t)idx * kMaxGroup + jr];
+      const uint32_t bq = (rinfo >> 24) & 7u;
+      uint32_t pcs = rinfo & 0xffffffu;
+      if (pb >= c.piece_cap) pcs = 0u;
+      else if (pcs > c.piece_cap - pb) pcs = c.piece_cap - pb;
+      float acc[PL];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const uint32_t j = (j0 + u < pcs) ? j0 + u : pcs - 1u;
-        load_slice<D>(c.part + (size_t)(pb + j) * D, lane, r[u]);
-        inf[u] = c.piece_info[pb + j];
-      }
+      for (int q = 0; q < PL; ++q) acc[q] = 0.f;
+      for (uint32_t j0 = 0; j0 < pcs; j0 += U) {
+        float r[U][PL];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        if (j0 + u < pcs) {
+        for (int u = 0; u < U; ++u) {
+          const uint32_t jj = (j0 + u < pcs) ? j0 + u : pcs - 1u;
+          load_slice<D>(c.part + (size_t)(pb + jj) * D, lane, r[u]);
+        }
 #pragma unroll
-          for (int q = 0; q < PL; ++q) acc[q] += r[u][q];
-          if ((inf[u] >> 16) & 1u) {  // last piece of its run: the batch's update
-            const uint32_t bq = (inf[u] >> 8) & 0xffu;
-            if (c.grad_out) {
-              uint32_t ui = 0;
-              if (lane == 0) ui = atomicAdd(&st->n_unique, 1u);
-              ui = __shfl(ui, 0, EP_WAVE);
-              store_slice<D>(c.G + (size_t)ui * D, lane, acc);
-              if (lane == 0) c.ufeat[ui] = f | (bq << 27);
-            } else {
-              bool bad = false;
+        for (int u = 0; u < U; ++u) {
+          if (j0 + u < pcs) {
 #pragma unroll
-              for (int q = 0; q < PL; ++q) bad |= (base + q < D) && isnan(acc[q]);
-              if (!wave_any(bad)) {  // optimizer.rs:56, per batch
-                const float lr = s_lr[bq];
-#pragma unroll
-                for (int q = 0; q < PL; ++q) adam_math(th[q], m[q], v[q], acc[q], lr, sv.bc1, sv.bc2);
-                upd = true;
-              }
-            }
-#pragma unroll
-            for (int q = 0; q < PL; ++q) acc[q] = 0.f;
+            for (int q = 0; q < PL; ++q) acc[q] += r[u][q];
           }
         }
       }
+      pb += rinfo & 0xffffffu;
+      upd |= apply_run<D>(c, st, sv, s_lr[bq], f, bq, lane, acc, th, m, v);
     }
     if (upd) {
-      store_slice<D>(c.mom + ooff, lane, m);
-      store_slice<D>(c.var + ooff, lane, v);
-      if (c.bf16) {
-        if (c.master) store_slice<D>(c.master + ooff, lane, th);
-        uint16_t* row = (uint16_t*)c.table + off;
-#pragma unroll
-        for (int q = 0; q < PL; ++q)
-          if (base + q < D)
-            row[base + q] = f32_to_bf16_sr(th[q], sr_bits(c, sv, f, (uint32_t)((base + q) >> 2), (base + q) & 3));
-      } else {
-        store_slice<D>((float*)c.table + off, lane, th);
-      }
+      store_row_state<D>(c, sv, f, lane, th, m, v);
       ++done;
     }
   }
@@ -1681,10 +1777,10 @@ bool seg_variant_supported(int d, int vpl, int unroll) {
 }
 
 template <int D, int VPL, int U>
-static bool try_seg_adam(const StepConfig& c, StepState* st, int buf, hipStream_t s, int grid) {
+static bool try_seg_update(const StepConfig& c, StepState* st, int buf, hipStream_t s, int grid) {
   if constexpr (D % VPL == 0 && D / VPL <= 64) {
     if (c.seg_vpl == VPL && c.seg_unroll == U) {
-      hipLaunchKernelGGL((k_seg_adam<D, VPL, U>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
+      hipLaunchKernelGGL((k_seg_update<D, VPL, U>), dim3(grid), dim3(256), 0, s, c, st, c.slots[buf]);
       return true;
     }
   }
@@ -1692,19 +1788,20 @@ static bool try_seg_adam(const StepConfig& c, StepState* st, int buf, hipStream_
 }
 
 template <int D>
-static void launch_seg_adam_variant(const StepConfig& c, StepState* st, int buf, hipStream_t s, int grid) {
-  if (try_seg_adam<D, 16, 2>(c, st, buf, s, grid) || try_seg_adam<D, 16, 4>(c, st, buf, s, grid) ||
-      try_seg_adam<D, 8, 4>(c, st, buf, s, grid) || try_seg_adam<D, 8, 8>(c, st, buf, s, grid) ||
-      try_seg_adam<D, 4, 16>(c, st, buf, s, grid))
+static void launch_seg_update_variant(const StepConfig& c, StepState* st, int buf, hipStream_t s, int grid) {
+  if (try_seg_update<D, 16, 2>(c, st, buf, s, grid) || try_seg_update<D, 16, 4>(c, st, buf, s, grid) ||
+      try_seg_update<D, 8, 4>(c, st, buf, s, grid) || try_seg_update<D, 8, 8>(c, st, buf, s, grid) ||
+      try_seg_update<D, 4, 16>(c, st, buf, s, grid))
     return;
-  hipLaunchKernelGGL((k_seg_adam<D, 4, 8>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
+  hipLaunchKernelGGL((k_seg_update<D, 4, 8>), dim3(grid), dim3(256), 0, s, c, st, c.slots[buf]);
 }
 
 hipError_t launch_scatter_sorted(const StepConfig& c, StepState* st, int buf, hipStream_t s) {
   const int grid = c.unique_grid;
+  hipLaunchKernelGGL(k_seg_plan, dim3(grid), dim3(256), 0, s, c, st, c.keys[buf]);
   EP_DISPATCH(c.dims, {
-    launch_seg_adam_variant<D>(c, st, buf, s, grid);
-    hipLaunchKernelGGL((k_pieces<D>), dim3(grid), dim3(256), 0, s, c, st, c.slots[buf]);
+    launch_seg_update_variant<D>(c, st, buf, s, grid);
+    hipLaunchKernelGGL((k_pieces<D>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
     hipLaunchKernelGGL((k_long_combine<D>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf]);
   });
   return hipGetLastError();
