@@ -6,19 +6,25 @@ Workload (default): the 1M-node bipartite query-item graph (300k queries, 700k i
 20M undirected edges, Chung-Lu beta 2.2; features U{1..32} Zipf(1.1) over a 1M vocab;
 d=128, B=1024, K=10 negatives, S=20 neighbours, margin 0.5).  Synthetic, seeded inputs.
 
-Forms timed on the same graph and the same batches:
-  fp32-plain    FP32 table, one launch per kernel, sort + segment-reduce scatter
-  bf16-batched  BF16 table + FP32 master rows + stochastic rounding, steps replayed
-                from a captured hipGraph, autotuned scatter
-The JSON `value` is the examples/s of the headline form (bf16-batched).  Every
-example is one anchor taken through h(v), ~h(v), K negatives, the margin loss, the
-hand-written backward and its share of the sparse Adam update.
+Forms timed on the same graph:
+  fp32-plain    FP32 table, one batch per step, one launch per kernel (no graph), the
+                default segment tile (no tuning), sort + segment-reduce scatter
+  fp32-batched  FP32 table, --concurrent batches per step (they read the same table and
+                update in batch order, like the reference's concurrently running
+                batches), steps replayed from a captured hipGraph, tuned segment tile
+  bf16-batched  the same with the BF16 table (FP32 master of the touched rows during the
+                update, stochastic rounding on write-back)
+The JSON `value` is the examples/s of the headline form (bf16-batched).  Every example
+is one anchor taken through h(v), ~h(v), K negatives, the margin loss, the hand-written
+backward and its batch's sparse Adam update.  A "step" in `ms_per_step` is one launch
+sequence of the headline form, i.e. --concurrent reference batches.
 
-`roofline` is the HBM roofline of the gather kernel (k_forward), its dominant
-kernel: algorithmic bytes per launch (from the sampled lists of the timed steps)
-over its average launch duration, timed with HIP events on the context stream.
-`cpu_baseline` times the C++ restatement of the reference's rayon loop on the
-host cores (bounded sample of the same workload).
+`roofline` is the HBM roofline of the dominant kernel of the headline form (the
+short-row update, or the forward gather when that takes longer): its algorithmic bytes
+per launch over its average launch duration, timed with HIP events on the context's
+stream.  `roofline_step` applies the same algorithmic-byte model to the whole step.
+`cpu_baseline` times the C++ restatement of the reference's rayon loop on the host
+cores (bounded sample of the same workload).
 """
 from __future__ import annotations
 
@@ -35,38 +41,35 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PHASES = ["begin+sample+scan", "forward", "emit", "sort", "plan", "short_update", "pieces+medium", "long_rows"]
 
 
-def algorithmic_bytes(rows, nodes, examples, dims, bytes_per_value):
+def gather_bytes(rows, nodes, examples, dims, bytes_per_value):
     """Sum over examples of R_e*d*b + 4 R_e + 4 (S_e + K) + 16 (1 + S_e + K)."""
     return rows * dims * bytes_per_value + 4 * rows + 4 * nodes + 16 * (examples + nodes)
 
 
-def run_form(w, form, steps, warmup, scatter=None, concurrent=1):
+def update_bytes(urows, dims, bytes_per_value, master=False):
+    """Per updated row: theta read+write (2b), m and v read+write (16), FP32 master (8)."""
+    return urows * dims * (2 * bytes_per_value + 16 + (8 if master else 0))
+
+
+def run_form(w, form, steps, warmup, scatter=None, concurrent=1, graph=True):
     from cloverleaf_amd import _native as N
 
     cfg = N.default_config()
-    if form.endswith("-batched"):
-        cfg.concurrent_batches = concurrent
     cfg.dims, cfg.batch_size, cfg.negatives, cfg.gamma, cfg.alpha = w.dims, w.batch, w.negatives, w.gamma, w.alpha
     cfg.max_nodes_kind, cfg.max_nodes = N.EP_SAMPLE_FIXED, w.max_nodes
     cfg.valid_pct = 0.0
     cfg.passes = max(1, w.passes)
-    if form == "fp32-plain":
-        cfg.precision, cfg.use_graph, cfg.scatter = N.EP_PREC_FP32, 0, N.EP_SCATTER_SORT
-    elif form == "fp32-batched":
-        cfg.precision, cfg.use_graph = N.EP_PREC_FP32, 1
-        cfg.scatter = N.EP_SCATTER_ATOMIC if scatter == "atomic" else N.EP_SCATTER_SORT
-    elif form == "bf16-batched":
-        cfg.precision, cfg.use_graph = N.EP_PREC_BF16, 1
-        cfg.scatter = N.EP_SCATTER_ATOMIC if scatter == "atomic" else N.EP_SCATTER_SORT
-    elif form == "bf16-master-batched":
-        cfg.precision, cfg.use_graph = N.EP_PREC_BF16_MASTER, 1
-        cfg.scatter = N.EP_SCATTER_ATOMIC if scatter == "atomic" else N.EP_SCATTER_SORT
-    elif form == "bf16-plain":
-        cfg.precision, cfg.use_graph, cfg.scatter = N.EP_PREC_BF16, 0, N.EP_SCATTER_SORT
+    cfg.scatter = N.EP_SCATTER_ATOMIC if scatter == "atomic" else N.EP_SCATTER_SORT
+    if form.endswith("-plain"):
+        cfg.use_graph, cfg.concurrent_batches = 0, 1
+        cfg.seg_vpl, cfg.seg_unroll = 4, 8  # default tile, no tuning
     else:
-        raise ValueError(form)
+        cfg.use_graph, cfg.concurrent_batches = (1 if graph else 0), concurrent
+    prec = form.rsplit("-", 1)[0]
+    cfg.precision = {"fp32": N.EP_PREC_FP32, "bf16": N.EP_PREC_BF16, "bf16-master": N.EP_PREC_BF16_MASTER}[prec]
     with N.Context(cfg) as ctx:
         ctx.set_graph(w.indptr, w.indices)
         ctx.set_features(w.f_indptr, w.f_ids, w.vocab)
@@ -74,23 +77,31 @@ def run_form(w, form, steps, warmup, scatter=None, concurrent=1):
         train = np.arange(w.n_nodes, dtype=np.uint32)
         ctx.set_split(train, np.zeros(0, dtype=np.uint32))
         r = ctx.bench_steps(warmup, steps, instrument=True)
-    b = 2 if form.startswith("bf16") else 4
-    per_launch = algorithmic_bytes(r.rows, r.nodes, r.examples, w.dims, b) / max(r.steps, 1)
-    fwd_s = r.ms_forward_avg / 1e3
+    b = 2 if prec.startswith("bf16") else 4
+    S = max(r.steps, 1)
+    ph = [float(x) for x in r.ms_phase]
+    g_launch = gather_bytes(r.rows, r.nodes, r.examples, w.dims, b) / S
+    u_launch = update_bytes(r.short_rows, w.dims, b, prec == "bf16-master") / S
+    step_bytes = (gather_bytes(r.rows, r.nodes, r.examples, w.dims, b)
+                  + update_bytes(r.unique_rows, w.dims, b, prec == "bf16-master")) / S
     return {
         "form": form,
         "scatter": scatter or "sort",
         "concurrent_batches": int(cfg.concurrent_batches),
         "seg_tile": [int(r.seg_vpl), int(r.seg_unroll)],
         "examples_per_s": r.examples / (r.ms_total / 1e3),
-        "ms_per_step": r.ms_total / r.steps,
+        "ms_per_step": r.ms_total / S,
         "examples": int(r.examples),
         "rows_per_example": r.rows / max(r.examples, 1),
-        "unique_rows_per_step": r.unique_rows / max(r.steps, 1),
-        "forward_ms_avg": r.ms_forward_avg,
-        "scatter_update_ms_avg": r.ms_scatter_avg,
-        "forward_bytes_per_launch": per_launch,
-        "forward_gbs": per_launch / fwd_s / 1e9 if fwd_s > 0 else 0.0,
+        "unique_rows_per_step": r.unique_rows / S,
+        "short_rows_per_step": r.short_rows / S,
+        "phase_ms": {k: round(v, 4) for k, v in zip(PHASES, ph)},
+        "forward_bytes_per_launch": g_launch,
+        "forward_gbs": g_launch / (ph[1] / 1e3) / 1e9 if ph[1] > 0 else 0.0,
+        "update_bytes_per_launch": u_launch,
+        "update_gbs": u_launch / (ph[5] / 1e3) / 1e9 if ph[5] > 0 else 0.0,
+        "step_bytes": step_bytes,
+        "step_gbs": step_bytes / (r.ms_total / S / 1e3) / 1e9,
     }
 
 
@@ -121,19 +132,35 @@ def cpu_baseline(w, budget_s=15.0):
                       f"workload, hogwild sparse Adam, FP32, C++ restatement of the reference's rayon loop"}
 
 
+def pmc_traffic(kernel, workload, form):
+    """Calibrated HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/hbm_traffic_<workload>.json, written from two rocprofv3 --pmc runs)."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        data = json.load(fh)
+    ent = data.get("kernels", {}).get(f"{form}:{kernel}")
+    if not ent:
+        return None
+    return ent["fetch_bytes"] + ent["write_bytes"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="ep-cfg2-1m")
     ap.add_argument("--scale", type=float, default=1.0)
-    ap.add_argument("--forms", default="fp32-plain,bf16-batched")
+    ap.add_argument("--forms", default="fp32-plain,fp32-batched,bf16-batched")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--scatters", default="sort,atomic", help="scatter paths tried for the batched forms")
+    ap.add_argument("--scatters", default="sort", help="scatter paths tried for the batched forms (sort,atomic)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--concurrent", type=int, default=4,
+    ap.add_argument("--concurrent", type=int, default=8,
                     help="reference batches per step in the batched forms (the plain forms use 1)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the batched forms eagerly (for hardware-counter runs)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (HBM-counter calibration)")
     args = ap.parse_args()
@@ -151,9 +178,9 @@ def main():
         src = torch.ones(n, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(3):
-            dst.copy_(src)
+            torch.mul(src, 2.0, out=dst)  # an elementwise kernel: reads and writes N MiB once
         torch.cuda.synchronize()
-        print(f"[bench] copy probe: 3 x {args.copy_probe_mib} MiB device copies (elementwise copy kernel)",
+        print(f"[bench] copy probe: 3 x {args.copy_probe_mib} MiB elementwise scale (read N MiB, write N MiB)",
               file=sys.stderr, flush=True)
         del src, dst
 
@@ -169,18 +196,29 @@ def main():
         for sc in scatters:
             if sc == "atomic" and args.concurrent > 1:
                 continue  # the atomic path runs one batch per step
-            r = run_form(w, form, args.steps, args.warmup, scatter=sc, concurrent=args.concurrent)
-            print(f"[bench] {form}/{sc}: {r['examples_per_s']:.0f} ex/s, {r['ms_per_step']:.3f} ms/step, "
-                  f"fwd {r['forward_ms_avg']:.3f} ms ({r['forward_gbs']:.0f} GB/s), "
-                  f"scatter+update {r['scatter_update_ms_avg']:.3f} ms (seg tile {r['seg_tile']}, P {r['concurrent_batches']}, "
-                  f"{r['unique_rows_per_step']:.0f} rows/step)", file=sys.stderr, flush=True)
+            r = run_form(w, form, args.steps, args.warmup, scatter=sc, concurrent=args.concurrent,
+                         graph=not args.no_graph)
+            ph = " ".join(f"{k}={v * 1e3:.0f}" for k, v in r["phase_ms"].items())
+            print(f"[bench] {form}/{sc}: {r['examples_per_s']:.0f} ex/s, {r['ms_per_step']:.3f} ms/step "
+                  f"(P {r['concurrent_batches']}, tile {r['seg_tile']}, {r['unique_rows_per_step']:.0f} rows/step); "
+                  f"phases us: {ph}; fwd {r['forward_gbs']:.0f} GB/s, update {r['update_gbs']:.0f} GB/s, "
+                  f"step {r['step_gbs']:.0f} GB/s", file=sys.stderr, flush=True)
             if best is None or r["examples_per_s"] > best["examples_per_s"]:
                 best = r
-        results[form] = best
+        if best is not None:
+            results[form] = best
 
     head_form = "bf16-batched" if "bf16-batched" in results else list(results)[0]
     head = results[head_form]
     cpu = None if args.no_cpu else cpu_baseline(w, args.cpu_budget)
+    ph = head["phase_ms"]
+    if ph["short_update"] >= ph["forward"]:
+        kname, kdesc, kbytes, kgbs = ("k_seg_update", "short-row update: reduce runs of adjoint rows + Adam "
+                                      "(theta/m/v read and written once per row)",
+                                      head["update_bytes_per_launch"], head["update_gbs"])
+    else:
+        kname, kdesc, kbytes, kgbs = ("k_forward", "gather + mean + margin loss + adjoint",
+                                      head["forward_bytes_per_launch"], head["forward_gbs"])
     line = {
         "metric": "EP train examples/sec (fwd+bwd+update), BF16-batched vs FP32 plain, matched AUC",
         "value": round(head["examples_per_s"], 1),
@@ -199,23 +237,35 @@ def main():
             "description": w.description,
             "nodes": w.n_nodes, "csr_entries": int(len(w.indices)), "vocab": w.vocab,
             "dims": w.dims, "batch": w.batch, "negatives": w.negatives, "max_nodes": w.max_nodes,
-            "form": head_form, "scatter": head["scatter"], "accumulate": "fp32",
+            "form": head_form, "concurrent_batches": head["concurrent_batches"], "accumulate": "fp32",
             "forms": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                       for k, v in results.items()},
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_forward (gather + mean + margin loss + adjoint)",
-            "achieved": round(head["forward_gbs"], 1),
+            "kernel": f"{kname} ({kdesc})",
+            "achieved": round(kgbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(head["forward_gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "frac": round(kgbs / HBM_PEAK_GBS, 4),
+            "traffic": pmc_traffic(kname, w.name, head_form),
+            "algorithmic_bytes_per_launch": round(kbytes),
+        },
+        "roofline_step": {
+            "bound": "hbm",
+            "achieved": round(head["step_gbs"], 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(head["step_gbs"] / HBM_PEAK_GBS, 4),
+            "bytes_per_step": round(head["step_bytes"]),
         },
         "cpu_baseline": cpu,
     }
     if "fp32-plain" in results and head_form != "fp32-plain":
         line["config"]["speedup_vs_fp32_plain"] = round(head["examples_per_s"] / results["fp32-plain"]["examples_per_s"], 3)
+    if "fp32-batched" in results and head_form != "fp32-batched":
+        line["config"]["speedup_vs_fp32_batched"] = round(
+            head["examples_per_s"] / results["fp32-batched"]["examples_per_s"], 3)
     print(json.dumps(line))
 
 

@@ -69,6 +69,8 @@ class EPBenchResult(Structure):
         ("forward_launches", c_uint32),
         ("seg_vpl", c_int32),
         ("seg_unroll", c_int32),
+        ("ms_phase", c_double * 8),
+        ("short_rows", c_uint64),
     ]
 
 

@@ -329,40 +329,42 @@ void refresh_base(ep_ctx* c) {
 // Launches one step.  update == false runs the sorted path in gradient-output mode (the
 // merged gradient lands in G/ufeat, nothing is updated); *sorted_buf receives the key
 // buffer holding the sorted keys.
+// ev (nullable): 9 events recorded at the step start and after the scan, forward, emit,
+// sort, plan, short-row, pieces and long-row phases (atomic path: after its scatter and
+// its update, the remaining ones at the same point).
 int launch_training_step(ep_ctx* c, const ep::StepConfig& sc0, bool update, float lr_override,
-                         hipEvent_t ev_f0 = nullptr, hipEvent_t ev_f1 = nullptr, hipEvent_t ev_s1 = nullptr,
-                         int* sorted_buf = nullptr, hipEvent_t ev_s0 = nullptr) {
+                         hipEvent_t* ev = nullptr, int* sorted_buf = nullptr) {
   hipStream_t s = c->stream;
   ep::StepConfig sc = sc0;
   if (sc.seg_vpl == 0) {
     sc.seg_vpl = c->seg_vpl ? c->seg_vpl : 4;
     sc.seg_unroll = c->seg_vpl ? c->seg_unroll : 8;
   }
+  auto mark = [&](int i) -> hipError_t { return ev ? hipEventRecord(ev[i], s) : hipSuccess; };
+  EP_HIP(mark(0));
   if (lr_override >= 0.f) EP_HIP(ep::launch_step_begin_lr(sc, c->d_state, lr_override, s));
   else EP_HIP(ep::launch_step_begin(sc, c->d_state, s));
   EP_HIP(ep::launch_sample(sc, c->d_state, s));
   EP_HIP(ep::launch_scan(sc, c->d_state, s));
-  if (ev_f0) EP_HIP(hipEventRecord(ev_f0, s));
+  EP_HIP(mark(1));
   EP_HIP(ep::launch_forward(sc, c->d_state, s));
-  if (ev_f1) EP_HIP(hipEventRecord(ev_f1, s));
+  EP_HIP(mark(2));
   EP_HIP(ep::launch_emit(sc, c->d_state, s));
+  EP_HIP(mark(3));
   if (sc.scatter == EP_SCATTER_SORT) {
     int buf = 0;
     EP_HIP(ep::launch_sort(sc, c->d_state, s, &buf));
     if (sorted_buf) *sorted_buf = buf;
-    if (ev_s0) EP_HIP(hipEventRecord(ev_s0, s));
-    if (update) {
-      EP_HIP(ep::launch_scatter_sorted(sc, c->d_state, buf, s));
-    } else {
-      ep::StepConfig g = sc;
-      g.grad_out = 1;
-      EP_HIP(ep::launch_scatter_sorted(g, c->d_state, buf, s));
-    }
+    EP_HIP(mark(4));
+    ep::StepConfig g = sc;
+    g.grad_out = update ? 0 : 1;
+    EP_HIP(ep::launch_scatter_sorted(g, c->d_state, buf, s, ev ? ev + 5 : nullptr));
   } else {
     EP_HIP(ep::launch_atomic_scatter(sc, c->d_state, s));
+    EP_HIP(mark(4));
     if (update) EP_HIP(ep::launch_adam_touched(sc, c->d_state, s));
+    for (int i = 5; i < 9; ++i) EP_HIP(mark(i));
   }
-  if (ev_s1) EP_HIP(hipEventRecord(ev_s1, s));
   return EP_OK;
 }
 
@@ -464,7 +466,7 @@ int tune_step(ep_ctx* c, const ep::StepConfig& sc) {
       c->seg_unroll = only.second;
       return launch_training_step(c, sc, true, -1.f);
     }
-    const size_t ne = 2 * c->tune_cands.size() * kTuneRounds;
+    const size_t ne = 9 * c->tune_cands.size() * kTuneRounds;
     c->tune_ev.resize(ne, nullptr);
     for (auto& e : c->tune_ev) EP_HIP(hipEventCreate(&e));
   }
@@ -473,7 +475,9 @@ int tune_step(ep_ctx* c, const ep::StepConfig& sc) {
   ep::StepConfig t = sc;
   t.seg_vpl = c->tune_cands[i % n].first;
   t.seg_unroll = c->tune_cands[i % n].second;
-  EP_TRY(launch_training_step(c, t, true, -1.f, nullptr, nullptr, c->tune_ev[2 * i + 1], nullptr, c->tune_ev[2 * i]));
+  hipEvent_t evs[9];
+  for (int k = 0; k < 9; ++k) evs[k] = c->tune_ev[9 * i + k];
+  EP_TRY(launch_training_step(c, t, true, -1.f, evs));
   if (c->tune_i < n * kTuneRounds) return EP_OK;
   EP_HIP(hipEventSynchronize(c->tune_ev.back()));
   float best = INFINITY;
@@ -483,7 +487,7 @@ int tune_step(ep_ctx* c, const ep::StepConfig& sc) {
     for (uint32_t r = 1; r < kTuneRounds; ++r) {
       const uint32_t j = r * n + k;
       float x = 0.f;
-      EP_HIP(hipEventElapsedTime(&x, c->tune_ev[2 * j], c->tune_ev[2 * j + 1]));
+      EP_HIP(hipEventElapsedTime(&x, c->tune_ev[9 * j + 5], c->tune_ev[9 * j + 6]));  // short-row kernel
       ms.push_back(x);
     }
     std::sort(ms.begin(), ms.end());
@@ -925,7 +929,7 @@ int ep_step_explicit(ep_ctx* c, const uint32_t* anchors, uint32_t n, const uint3
   const bool upd = apply_update != 0;
   // run everything up to (but excluding) the update, read the merged gradient, then update
   int buf = 0;
-  EP_TRY(launch_training_step(c, sc, false, lr, nullptr, nullptr, nullptr, &buf));
+  EP_TRY(launch_training_step(c, sc, false, lr, nullptr, &buf));
   EP_TRY(check_state_err(c));
   ep::StepState hs;
   EP_HIP(hipMemcpy(&hs, c->d_state, offsetof(ep::StepState, dtot), hipMemcpyDeviceToHost));
@@ -1031,26 +1035,30 @@ int ep_bench_steps(ep_ctx* c, uint32_t warmup, uint32_t steps, int instrument, e
   out->seg_vpl = c->seg_vpl;
   out->seg_unroll = c->seg_unroll;
   if (instrument) {
-    // per-kernel timing: the same number of steps launched one by one with events
-    std::vector<hipEvent_t> ev(3 * steps);
+    // per-phase timing: the same number of steps launched one by one with events
+    std::vector<hipEvent_t> ev(9 * (size_t)steps);
     for (auto& e : ev) EP_HIP(hipEventCreate(&e));
+    ep::StepState g0, g1;
+    EP_HIP(hipMemcpy(&g0, c->d_state, sizeof(g0), hipMemcpyDeviceToHost));
     for (uint32_t i = 0; i < steps; ++i) {
       if (in_pass == sc.steps_per_pass) EP_TRY(advance());
       else ++in_pass;
-      EP_TRY(launch_training_step(c, sc, true, -1.f, ev[3 * i], ev[3 * i + 1], ev[3 * i + 2]));
+      EP_TRY(launch_training_step(c, sc, true, -1.f, &ev[9 * (size_t)i]));
     }
     EP_HIP(hipStreamSynchronize(c->stream));
-    double fsum = 0.0, ssum = 0.0;
-    for (uint32_t i = 0; i < steps; ++i) {
-      float a = 0.f, b = 0.f;
-      EP_HIP(hipEventElapsedTime(&a, ev[3 * i], ev[3 * i + 1]));
-      EP_HIP(hipEventElapsedTime(&b, ev[3 * i + 1], ev[3 * i + 2]));
-      fsum += a;
-      ssum += b;
-    }
+    EP_HIP(hipMemcpy(&g1, c->d_state, sizeof(g1), hipMemcpyDeviceToHost));
+    double ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t i = 0; i < steps; ++i)
+      for (int k = 0; k < 8; ++k) {
+        float x = 0.f;
+        EP_HIP(hipEventElapsedTime(&x, ev[9 * (size_t)i + k], ev[9 * (size_t)i + k + 1]));
+        ph[k] += x;
+      }
     for (auto& e : ev) (void)hipEventDestroy(e);
-    out->ms_forward_avg = fsum / steps;
-    out->ms_scatter_avg = ssum / steps;
+    for (int k = 0; k < 8; ++k) out->ms_phase[k] = ph[k] / steps;
+    out->ms_forward_avg = out->ms_phase[1];
+    out->ms_scatter_avg = out->ms_phase[3] + out->ms_phase[4] + out->ms_phase[5] + out->ms_phase[6] + out->ms_phase[7];
+    out->short_rows = g1.stat_short - g0.stat_short;
     out->forward_launches = steps;
     EP_TRY(check_state_err(c));
   }

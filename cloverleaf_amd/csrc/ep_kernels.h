@@ -22,6 +22,7 @@ struct StepState {
   uint64_t stat_nodes;     // cumulative node-id reads (sampled neighbours + negatives)
   uint64_t stat_unique;    // cumulative unique rows updated
   uint64_t stat_examples;  // cumulative anchors processed
+  uint64_t stat_short;     // cumulative rows updated by the short-row kernel
   uint32_t pass;           // 1-based pass
   uint32_t step_in_pass;   // batch index within the pass (0-based)
   uint32_t batch_n;        // anchors in the current batch
@@ -162,7 +163,9 @@ hipError_t launch_emit(const StepConfig& c, StepState* st, hipStream_t s);
 // Sort keys/slots; returns which buffer (0/1) holds the result in *out_buf.
 hipError_t launch_sort(const StepConfig& c, StepState* st, hipStream_t s, int* out_buf);
 // Segment reduce + Adam over the sorted keys (grad_out: write merged gradients instead).
-hipError_t launch_scatter_sorted(const StepConfig& c, StepState* st, int buf, hipStream_t s);
+// ev (nullable): events recorded after the plan, short-row, pieces and long-row kernels.
+hipError_t launch_scatter_sorted(const StepConfig& c, StepState* st, int buf, hipStream_t s,
+                                 hipEvent_t* ev = nullptr);
 hipError_t launch_atomic_scatter(const StepConfig& c, StepState* st, hipStream_t s);
 hipError_t launch_adam_touched(const StepConfig& c, StepState* st, hipStream_t s);
 // One-off kernels.

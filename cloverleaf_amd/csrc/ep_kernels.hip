@@ -1287,7 +1287,10 @@ __global__ __launch_bounds__(256) void k_seg_update(StepConfig c, StepState* st,
     }
   }
   done = wave_sum_u32(done);
-  if (lane == 0 && done) atomicAdd((unsigned long long*)&st->stat_unique, (unsigned long long)done);
+  if (lane == 0 && done) {
+    atomicAdd((unsigned long long*)&st->stat_unique, (unsigned long long)done);
+    atomicAdd((unsigned long long*)&st->stat_short, (unsigned long long)done);
+  }
 }
 
 // Loads row f's optimizer operands (whole-wave slice layout).
@@ -1796,13 +1799,17 @@ static void launch_seg_update_variant(const StepConfig& c, StepState* st, int bu
   hipLaunchKernelGGL((k_seg_update<D, 4, 8>), dim3(grid), dim3(256), 0, s, c, st, c.slots[buf]);
 }
 
-hipError_t launch_scatter_sorted(const StepConfig& c, StepState* st, int buf, hipStream_t s) {
+hipError_t launch_scatter_sorted(const StepConfig& c, StepState* st, int buf, hipStream_t s, hipEvent_t* ev) {
   const int grid = c.unique_grid;
   hipLaunchKernelGGL(k_seg_plan, dim3(grid), dim3(256), 0, s, c, st, c.keys[buf]);
+  if (ev) (void)hipEventRecord(ev[0], s);
   EP_DISPATCH(c.dims, {
     launch_seg_update_variant<D>(c, st, buf, s, grid);
+    if (ev) (void)hipEventRecord(ev[1], s);
     hipLaunchKernelGGL((k_pieces<D>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
+    if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL((k_long_combine<D>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf]);
+    if (ev) (void)hipEventRecord(ev[3], s);
   });
   return hipGetLastError();
 }

@@ -1,0 +1,115 @@
+"""Summaries of rocprofv3 output databases (kernel trace and hardware counters).
+
+python -m cloverleaf_amd.profiling stats <results.db>
+    per-kernel dispatch count and duration statistics (markdown table)
+python -m cloverleaf_amd.profiling traffic <fetch.db> <write.db> --probe-mib N --out file.json --form F
+    calibrated HBM bytes per dispatch of every kernel, from one FETCH_SIZE and one
+    WRITE_SIZE run of the same command; the counters of the copy probe kernel (an
+    elementwise copy of N MiB launched first by `bench.py --copy-probe-mib N`) give the
+    bytes-per-counter-unit factors.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sqlite3
+import sys
+from collections import defaultdict
+
+
+def kernel_stats(db_path: str):
+    db = sqlite3.connect(db_path)
+    rows = db.execute("select name, count(*), avg(end-start), min(end-start), max(end-start), sum(end-start) "
+                      "from kernels group by name order by sum(end-start) desc").fetchall()
+    return [dict(name=r[0], count=r[1], avg_us=r[2] / 1e3, min_us=r[3] / 1e3, max_us=r[4] / 1e3,
+                 total_us=r[5] / 1e3) for r in rows]
+
+
+def counter_per_kernel(db_path: str, counter: str):
+    """(average, count, max) of the counter per dispatch, per kernel name (summed over the
+    counter's instances)."""
+    db = sqlite3.connect(db_path)
+    rows = db.execute("select dispatch_id, kernel_name, sum(value) from counters_collection where counter_name = ? "
+                      "group by dispatch_id", (counter,)).fetchall()
+    acc = defaultdict(list)
+    for _, name, v in rows:
+        acc[name].append(v)
+    return {k: (sum(v) / len(v), len(v), max(v)) for k, v in acc.items()}
+
+
+def short_name(name: str) -> str:
+    base = name.split("(")[0]
+    return base.replace("void ", "").replace("ep::", "").strip()
+
+
+def traffic(fetch_db: str, write_db: str, probe_mib: int):
+    fetch = counter_per_kernel(fetch_db, "FETCH_SIZE")
+    write = counter_per_kernel(write_db, "WRITE_SIZE")
+    # the probe: the elementwise kernel that reads and writes probe_mib MiB (largest
+    # single-dispatch FETCH among elementwise kernels)
+    probe = [k for k in fetch if "elementwise" in k.lower()]
+    if not probe:
+        raise SystemExit("no elementwise probe kernel in the FETCH_SIZE run")
+    pk = max(probe, key=lambda k: fetch[k][2])
+    true_bytes = probe_mib * (1 << 20)
+    f_fac = true_bytes / (fetch[pk][2] * 1024.0)  # counters are in KiB
+    w_fac = true_bytes / (write[pk][2] * 1024.0) if pk in write and write[pk][2] > 0 else 1.0
+    out = {}
+    for k in fetch:
+        if k == pk:
+            continue
+        fb = fetch[k][0] * 1024.0 * f_fac
+        wb = write.get(k, (0.0, 0))[0] * 1024.0 * w_fac
+        out[short_name(k)] = dict(fetch_bytes=round(fb), write_bytes=round(wb), dispatches=fetch[k][1])
+    return out, dict(probe_kernel=short_name(pk), fetch_factor=f_fac, write_factor=w_fac, probe_mib=probe_mib)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s1 = sub.add_parser("stats")
+    s1.add_argument("db")
+    s1.add_argument("--top", type=int, default=25)
+    s2 = sub.add_parser("traffic")
+    s2.add_argument("fetch_db")
+    s2.add_argument("write_db")
+    s2.add_argument("--probe-mib", type=int, required=True)
+    s2.add_argument("--form", required=True)
+    s2.add_argument("--out", required=True)
+    s2.add_argument("--command", default="")
+    a = ap.parse_args(argv)
+    if a.cmd == "stats":
+        print("| kernel | dispatches | avg us | min us | max us | total us |")
+        print("|---|---|---|---|---|---|")
+        for r in kernel_stats(a.db)[:a.top]:
+            print(f"| `{short_name(r['name'])}` | {r['count']} | {r['avg_us']:.2f} | {r['min_us']:.2f} | "
+                  f"{r['max_us']:.2f} | {r['total_us']:.0f} |")
+        return 0
+    ker, cal = traffic(a.fetch_db, a.write_db, a.probe_mib)
+    try:
+        with open(a.out) as fh:
+            data = json.load(fh)
+    except FileNotFoundError:
+        data = {"kernels": {}}
+    data.setdefault("kernels", {})
+    data["calibration"] = cal
+    if a.command:
+        data.setdefault("commands", {})[a.form] = a.command
+    best = {}  # base name -> the template instance with the most dispatches (the tuned one)
+    for k, v in ker.items():
+        data["kernels"][f"{a.form}:{k}"] = v
+        b = k.split("<")[0]
+        if b not in best or v["dispatches"] > best[b]["dispatches"]:
+            best[b] = v
+    for b, v in best.items():
+        data["kernels"][f"{a.form}:{b}"] = v
+    with open(a.out, "w") as fh:
+        json.dump(data, fh, indent=1, sort_keys=True)
+    for k, v in sorted(ker.items(), key=lambda kv: -(kv[1]["fetch_bytes"] + kv[1]["write_bytes"]))[:12]:
+        print(f"{k}: fetch {v['fetch_bytes'] / 1e6:.1f} MB, write {v['write_bytes'] / 1e6:.1f} MB per dispatch "
+              f"({v['dispatches']} dispatches)")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -104,6 +104,11 @@ typedef struct ep_bench_result {
   uint32_t forward_launches;
   int32_t seg_vpl;             /* segment-Adam variant in use (after tuning)        */
   int32_t seg_unroll;
+  /* instrumented run: average duration per step of each phase, in ms:             */
+  /* 0 step begin + sample + scan, 1 forward, 2 key emission, 3 radix sort,        */
+  /* 4 row plan, 5 short-row update, 6 pieces + medium rows, 7 long rows           */
+  double ms_phase[8];
+  uint64_t short_rows;         /* rows updated by the short-row kernel (timed run)  */
 } ep_bench_result;
 
 /* Fills the reference defaults (EmbeddingPropagator::new, src/lib.rs:1262-1274). */
