@@ -95,6 +95,8 @@ def run_form(w, form, steps, warmup, scatter=None, concurrent=1, graph=True):
         "rows_per_example": r.rows / max(r.examples, 1),
         "unique_rows_per_step": r.unique_rows / S,
         "short_rows_per_step": r.short_rows / S,
+        "long_rows_last_step": int(r.last_long_rows),
+        "pieces_last_step": int(r.last_pieces),
         "phase_ms": {k: round(v, 4) for k, v in zip(PHASES, ph)},
         "forward_bytes_per_launch": g_launch,
         "forward_gbs": g_launch / (ph[1] / 1e3) / 1e9 if ph[1] > 0 else 0.0,
@@ -200,7 +202,9 @@ def main():
                          graph=not args.no_graph)
             ph = " ".join(f"{k}={v * 1e3:.0f}" for k, v in r["phase_ms"].items())
             print(f"[bench] {form}/{sc}: {r['examples_per_s']:.0f} ex/s, {r['ms_per_step']:.3f} ms/step "
-                  f"(P {r['concurrent_batches']}, tile {r['seg_tile']}, {r['unique_rows_per_step']:.0f} rows/step); "
+                  f"(P {r['concurrent_batches']}, tile {r['seg_tile']}, {r['unique_rows_per_step']:.0f} rows/step, "
+                  f"{r['short_rows_per_step']:.0f} short, {r['long_rows_last_step']} long, "
+                  f"{r['pieces_last_step']} pieces); "
                   f"phases us: {ph}; fwd {r['forward_gbs']:.0f} GB/s, update {r['update_gbs']:.0f} GB/s, "
                   f"step {r['step_gbs']:.0f} GB/s", file=sys.stderr, flush=True)
             if best is None or r["examples_per_s"] > best["examples_per_s"]:

@@ -71,6 +71,8 @@ class EPBenchResult(Structure):
         ("seg_unroll", c_int32),
         ("ms_phase", c_double * 8),
         ("short_rows", c_uint64),
+        ("last_long_rows", c_uint32),
+        ("last_pieces", c_uint32),
     ]
 
 

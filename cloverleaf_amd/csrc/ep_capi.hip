@@ -1059,6 +1059,8 @@ int ep_bench_steps(ep_ctx* c, uint32_t warmup, uint32_t steps, int instrument, e
     out->ms_forward_avg = out->ms_phase[1];
     out->ms_scatter_avg = out->ms_phase[3] + out->ms_phase[4] + out->ms_phase[5] + out->ms_phase[6] + out->ms_phase[7];
     out->short_rows = g1.stat_short - g0.stat_short;
+    out->last_long_rows = g1.n_long;
+    out->last_pieces = g1.n_pieces;
     out->forward_launches = steps;
     EP_TRY(check_state_err(c));
   }

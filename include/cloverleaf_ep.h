@@ -109,6 +109,8 @@ typedef struct ep_bench_result {
   /* 4 row plan, 5 short-row update, 6 pieces + medium rows, 7 long rows           */
   double ms_phase[8];
   uint64_t short_rows;         /* rows updated by the short-row kernel (timed run)  */
+  uint32_t last_long_rows;     /* long rows and their pieces in the last step        */
+  uint32_t last_pieces;
 } ep_bench_result;
 
 /* Fills the reference defaults (EmbeddingPropagator::new, src/lib.rs:1262-1274). */
