@@ -472,8 +472,6 @@ class EmbeddingPropagator:
             raise ValueError("weighted neighbour sampling/averaging is not supported by this build")
         if hard_negatives:
             raise ValueError("hard negatives are not supported by this build")
-        if noise:
-            raise ValueError("gradient noise is not supported by this build")
         cfg = N.default_config()
         cfg.alpha = 0.9 if alpha is None else float(alpha)
         cfg.batch_size = 50 if batch_size is None else int(batch_size)
@@ -494,6 +492,7 @@ class EmbeddingPropagator:
         cfg.scatter = {"sort": N.EP_SCATTER_SORT, "atomic": N.EP_SCATTER_ATOMIC}[scatter]
         cfg.use_graph = 1 if use_graph else 0
         cfg.concurrent_batches = int(concurrent_batches)
+        cfg.noise = 0.0 if noise is None else float(noise)
         cfg.device = int(device)
         self.cfg = cfg
         self.indicator = True if indicator is None else bool(indicator)

@@ -301,6 +301,8 @@ void refresh_base(ep_ctx* c) {
   b.key_a = (uint32_t)a;
   b.key_ainv = m > 1 ? (uint32_t)t0 : 1u;
   b.rbatch = cfg.batch_size;
+  b.noise = cfg.noise;
+  b.noise_decay = 1.0f;
   b.table = c->d_table;
   {
     const uint32_t D = cfg.dims;
@@ -444,6 +446,9 @@ ep::StepConfig train_config(ep_ctx* c) {
   const uint32_t GA = group_anchors(c);
   sc.steps_per_pass = (uint32_t)((c->train.size() + GA - 1) / GA);  // launches per pass
   sc.total_updates = (uint32_t)((c->train.size() + B - 1) / B) * c->cfg.passes;  // reference batches
+  // mod.rs:140-147: sd(step) = noise * decay^step, decay = exp((ln 1e-9 - ln noise) / T), in f32
+  if (c->cfg.noise > 1e-9f)
+    sc.noise_decay = std::exp((std::log(1e-9f) - std::log(c->cfg.noise)) / (float)sc.total_updates);
   sc.pass_loss = c->d_pass_loss;
   return sc;
 }
@@ -529,7 +534,7 @@ int validate_config(const ep_config* cfg) {
     return fail(EP_EINVAL, "max_nodes must be All or Fixed");
   if (cfg->max_features_kind != EP_SAMPLE_ALL) return fail(EP_EINVAL, "max_features sampling is not supported");
   if (cfg->hard_negatives != 0) return fail(EP_EINVAL, "hard_negatives is not supported");
-  if (cfg->noise != 0.f) return fail(EP_EINVAL, "gradient noise is not supported");
+  if (!(cfg->noise >= 0.f) || !std::isfinite(cfg->noise)) return fail(EP_EINVAL, "noise must be finite and >= 0");
   if (cfg->weighted_neighbor_sampling || cfg->weighted_neighbor_averaging)
     return fail(EP_EINVAL, "weighted neighbour sampling/averaging is not supported");
   if (cfg->precision != EP_PREC_FP32 && cfg->precision != EP_PREC_BF16 && cfg->precision != EP_PREC_BF16_MASTER)

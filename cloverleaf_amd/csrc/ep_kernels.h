@@ -36,6 +36,8 @@ struct StepState {
   uint32_t err;            // nonzero: key capacity exceeded this step
   uint32_t n_batches;      // reference batches in this step's group (<= kMaxGroup)
   float lr[8];             // learning rate of each batch of the group (kMaxGroup)
+  float noise_sd[8];       // gradient-noise sd of each batch of the group
+  uint64_t step0;          // step number of the group's first batch
   float bc1, bc2;          // Adam bias corrections 1 - beta^t, t = pass + 1
   uint32_t n_real;         // non-sentinel keys (the sorted key count)
   uint32_t dtot[4][2048];  // radix-sort digit totals, one row per sort pass (<= 11-bit digits)
@@ -88,6 +90,8 @@ struct StepConfig {
   int explicit_samples;    // 1: neighbour / negative lists are provided by the host
   uint64_t seed;
   // schedule
+  float noise;             // > 0: gaussian gradient noise, sd = noise * noise_decay^step
+  float noise_decay;
   float alpha;
   uint32_t total_updates;
   uint32_t steps_per_pass;

@@ -205,8 +205,16 @@ __global__ void k_step_begin(StepConfig c, StepState* st, float lr_override) {
     st->n_batches = nb;
     const uint64_t cur = st->next_step;
     st->next_step = cur + nb;
-    for (uint32_t k = 0; k < kMaxGroup; ++k)
+    st->step0 = cur;
+    for (uint32_t k = 0; k < kMaxGroup; ++k) {
       st->lr[k] = k < nb ? (lr_override >= 0.f ? lr_override : scheduled_lr(c, cur + k)) : 0.f;
+      // LRScheduler::exp_decay(0, noise, decay) (mod.rs:140-147, scheduler.rs:49-51); with no
+      // schedule length (explicit test steps) the sd stays at `noise`
+      float sd = 0.f;
+      if (c.noise > 1e-9f && k < nb)
+        sd = c.total_updates ? fmaxf(c.noise * powf(c.noise_decay, (float)(cur + k)), 0.f) : c.noise;
+      st->noise_sd[k] = sd;
+    }
     const float t = (float)st->pass + 1.0f;  // optimizer.rs:50: t = pass + 1
     st->bc1 = 1.0f - powf(0.9f, t);
     st->bc2 = 1.0f - powf(0.999f, t);
@@ -920,9 +928,11 @@ struct StepScalars {
   float lr0;         // learning rate of the group's first batch
   float bc1, bc2;
   uint32_t sr_step;  // step number keying the stochastic-rounding bits
+  uint32_t step0;    // step number of the group's first batch (noise key)
+  float sd0;         // noise sd of the group's first batch
 };
 __device__ inline StepScalars step_scalars(const StepState* st) {
-  return StepScalars{st->lr[0], st->bc1, st->bc2, (uint32_t)st->next_step};
+  return StepScalars{st->lr[0], st->bc1, st->bc2, (uint32_t)st->next_step, (uint32_t)st->step0, st->noise_sd[0]};
 }
 
 // 16 stochastic-rounding bits for column 4*col4 + q of row f at this step.
@@ -930,6 +940,42 @@ __device__ inline uint32_t sr_bits(const StepConfig& c, const StepScalars& sv, u
   const uint32_t r0 = sr_key(f, col4, sv.sr_step, (uint32_t)c.seed);
   const uint32_t w = (q >> 1) == 0 ? r0 : hash32(r0 ^ 0x27D4EB2Fu);
   return (q & 1) ? (w >> 16) : (w & 0xffffu);
+}
+
+// Gaussian gradient noise (mod.rs:238-247): N(0,1) for columns 4*col4 .. 4*col4+3 of row
+// f at step `step`, two Box-Muller pairs from one Philox4x32 block keyed by (row, column
+// block, step).  The reference draws from XorShift seeded with seed + batch + feature;
+// the stream differs, the distribution (independent standard normals) is the same.
+__device__ inline void noise4(const StepConfig& c, uint32_t f, uint32_t col4, uint32_t step, float (&z)[4]) {
+  const u32x4 r = philox4x32(u32x4{f, col4, step, RNG_NOISE}, (uint32_t)c.seed, (uint32_t)(c.seed >> 32));
+  const float two_pi = 6.28318530717958647692f;
+  const float ra = sqrtf(-2.0f * logf(((float)(r.x >> 8) + 1.0f) * (1.0f / 16777216.0f)));
+  const float rb = sqrtf(-2.0f * logf(((float)(r.z >> 8) + 1.0f) * (1.0f / 16777216.0f)));
+  const float ta = two_pi * ((float)(r.y >> 8) * (1.0f / 16777216.0f));
+  const float tb = two_pi * ((float)(r.w >> 8) * (1.0f / 16777216.0f));
+  z[0] = ra * cosf(ta);
+  z[1] = ra * sinf(ta);
+  z[2] = rb * cosf(tb);
+  z[3] = rb * sinf(tb);
+}
+
+// Adds sd * N(0,1) to a whole-wave slice (lane owns PL consecutive columns).
+template <int D>
+__device__ inline void add_noise_slice(const StepConfig& c, uint32_t f, uint32_t step, float sd, int lane,
+                                       float (&g)[RowShape<D>::PL]) {
+  constexpr int PL = RowShape<D>::PL;
+  const int base = lane * PL;
+#pragma unroll
+  for (int q = 0; q < PL; ++q) {
+    const int col = base + q;
+    if (col < D && (q == 0 || (col & 3) == 0)) {
+      float z[4];
+      noise4(c, f, (uint32_t)col >> 2, step, z);
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (q + k < PL && ((col + k) >> 2) == (col >> 2) && col + k < D) g[q + k] += sd * z[(col + k) & 3];
+    }
+  }
 }
 
 // Whole-wave row layout (lane owns RowShape<D>::PL values).
@@ -1154,8 +1200,13 @@ __global__ __launch_bounds__(256) void k_seg_update(StepConfig c, StepState* st,
   const uint32_t nrec = gw * cpw < nchunks ? c.scount[gw] : 0u;
   const uint4* recs = c.srec + (size_t)gw * cpw * 64u;
   const StepScalars sv = step_scalars(st);
-  if (tid < (int)kMaxGroup) s_lr[tid] = st->lr[tid];
+  __shared__ float s_sd[kMaxGroup];
+  if (tid < (int)kMaxGroup) {
+    s_lr[tid] = st->lr[tid];
+    s_sd[tid] = st->noise_sd[tid];
+  }
   __syncthreads();
+  const bool noisy = c.noise > 1e-9f;
   uint32_t done = 0;
   for (uint32_t j0 = 0; j0 < nrec; j0 += (uint32_t)G) {
     const uint32_t j = j0 + (uint32_t)grp;
@@ -1233,6 +1284,15 @@ __global__ __launch_bounds__(256) void k_seg_update(StepConfig c, StepState* st,
           for (int q = 0; q < NQ; ++q) {
             acc[q].x += r[u][q].x; acc[q].y += r[u][q].y; acc[q].z += r[u][q].z; acc[q].w += r[u][q].w;
           }
+        }
+      }
+      if (noisy && hr) {  // mod.rs:238-247: noise on the batch's merged gradient row
+        const float sd = s_sd[bq];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          float z[4];
+          noise4(c, f, (uint32_t)(q * L + lg), sv.step0 + bq, z);
+          acc[q].x += sd * z[0]; acc[q].y += sd * z[1]; acc[q].z += sd * z[2]; acc[q].w += sd * z[3];
         }
       }
       if (c.grad_out) {
@@ -1330,11 +1390,12 @@ __device__ inline void store_row_state(const StepConfig& c, const StepScalars& s
 // Applies one run's merged gradient (NaN rows skipped, optimizer.rs:56) or, in grad_out
 // mode, writes it as a (row, batch) record.  Returns whether theta changed.
 template <int D>
-__device__ inline bool apply_run(const StepConfig& c, StepState* st, const StepScalars& sv, float lr, uint32_t f,
-                                 uint32_t bq, int lane, const float (&acc)[RowShape<D>::PL], float (&th)[RowShape<D>::PL],
-                                 float (&m)[RowShape<D>::PL], float (&v)[RowShape<D>::PL]) {
+__device__ inline bool apply_run(const StepConfig& c, StepState* st, const StepScalars& sv, float lr, float sd,
+                                 uint32_t f, uint32_t bq, int lane, float (&acc)[RowShape<D>::PL],
+                                 float (&th)[RowShape<D>::PL], float (&m)[RowShape<D>::PL], float (&v)[RowShape<D>::PL]) {
   constexpr int PL = RowShape<D>::PL;
   const int base = lane * PL;
+  if (c.noise > 1e-9f) add_noise_slice<D>(c, f, sv.step0 + bq, sd, lane, acc);  // mod.rs:238-247
   if (c.grad_out) {
     uint32_t ui = 0;
     if (lane == 0) ui = atomicAdd(&st->n_unique, 1u);
@@ -1360,11 +1421,14 @@ __global__ __launch_bounds__(256) void k_pieces(StepConfig c, StepState* st, con
                                                 const uint32_t* __restrict__ Ss) {
   constexpr int PL = RowShape<D>::PL;
   constexpr int U = SR_UNROLL;
-  __shared__ float s_lr[kMaxGroup];
+  __shared__ float s_lr[kMaxGroup], s_sd[kMaxGroup];
   const int lane = threadIdx.x & 63;
   const uint32_t gw = blockIdx.x * NW + (threadIdx.x >> 6);
   const uint32_t nwaves = gridDim.x * NW;
-  if (threadIdx.x < kMaxGroup) s_lr[threadIdx.x] = st->lr[threadIdx.x];
+  if (threadIdx.x < kMaxGroup) {
+    s_lr[threadIdx.x] = st->lr[threadIdx.x];
+    s_sd[threadIdx.x] = st->noise_sd[threadIdx.x];
+  }
   __syncthreads();
   const uint32_t np = st->n_pieces < c.piece_cap ? st->n_pieces : c.piece_cap;
   for (uint32_t gp = gw; gp < np; gp += nwaves) {
@@ -1446,7 +1510,7 @@ __global__ __launch_bounds__(256) void k_pieces(StepConfig c, StepState* st, con
           }
         }
       }
-      upd |= apply_run<D>(c, st, sv, s_lr[bq], f, bq, lane, acc, th, mo, va);
+      upd |= apply_run<D>(c, st, sv, s_lr[bq], s_sd[bq], f, bq, lane, acc, th, mo, va);
     }
     if (upd) {
       store_row_state<D>(c, sv, f, lane, th, mo, va);
@@ -1467,8 +1531,11 @@ __global__ __launch_bounds__(256) void k_long_combine(StepConfig c, StepState* s
   const uint32_t nwaves = gridDim.x * NW;
   const uint32_t nl = st->n_long < c.long_cap ? st->n_long : c.long_cap;
   const StepScalars sv = step_scalars(st);
-  __shared__ float s_lr[kMaxGroup];
-  if (threadIdx.x < kMaxGroup) s_lr[threadIdx.x] = st->lr[threadIdx.x];
+  __shared__ float s_lr[kMaxGroup], s_sd[kMaxGroup];
+  if (threadIdx.x < kMaxGroup) {
+    s_lr[threadIdx.x] = st->lr[threadIdx.x];
+    s_sd[threadIdx.x] = st->noise_sd[threadIdx.x];
+  }
   __syncthreads();
   uint32_t done = 0;
   for (uint32_t idx = gw; idx < nl; idx += nwaves) {
@@ -1504,7 +1571,7 @@ __global__ __launch_bounds__(256) void k_long_combine(StepConfig c, StepState* s
         }
       }
       pb += rinfo & 0xffffffu;
-      upd |= apply_run<D>(c, st, sv, s_lr[bq], f, bq, lane, acc, th, m, v);
+      upd |= apply_run<D>(c, st, sv, s_lr[bq], s_sd[bq], f, bq, lane, acc, th, m, v);
     }
     if (upd) {
       store_row_state<D>(c, sv, f, lane, th, m, v);
@@ -1589,6 +1656,7 @@ __global__ __launch_bounds__(256) void k_adam_touched(StepConfig c, StepState* s
     for (int q = 0; q < PL; ++q) z[q] = 0.f;
     store_slice<D>(grow, lane, z);
     if (lane == 0) c.flag[f] = 0u;
+    if (c.noise > 1e-9f) add_noise_slice<D>(c, f, sv.step0, sv.sd0, lane, g);  // mod.rs:238-247
     adam_row<D>(c, sv, f, g, lane);
   }
 }

@@ -66,7 +66,8 @@ typedef struct ep_config {
   int32_t loss_weighting;      /* EP_LW_*                                           */
   float loss_weight_exp;       /* exponent of EP_LW_EXPONENTIAL                     */
   uint32_t hard_negatives;     /* must be 0                                         */
-  float noise;                 /* must be 0                                         */
+  float noise;                 /* gradient noise sd at step 0, decaying to 1e-9 over */
+                               /* the run (mod.rs:139-147, 238-247); 0 = off       */
   int32_t weighted_neighbor_sampling;   /* must be 0                                */
   int32_t weighted_neighbor_averaging;  /* must be 0                                */
   int32_t precision;           /* EP_PREC_*                                         */

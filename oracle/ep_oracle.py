@@ -349,6 +349,17 @@ def cos_decay_lr(step: int, alpha: float, total_updates: int) -> float:
     return lo + (step / warm) * (alpha - lo) if warm > 0 else float("nan")
 
 
+def noise_sd(step: int, noise: float, total_updates: int) -> float:
+    """Gradient-noise standard deviation at `step` (mod.rs:139-147 + scheduler.rs:49-51):
+    for noise > 1e-9, exp_decay(min 0, noise, decay) with decay = exp((ln 1e-9 - ln noise) / T),
+    i.e. noise * decay^step, which reaches 1e-9 at step T; otherwise the no-op scheduler (0).
+    """
+    if noise <= 1e-9:
+        return 0.0
+    decay = math.exp((math.log(1e-9) - math.log(noise)) / total_updates)
+    return max(noise * decay ** step, 0.0)
+
+
 ADAM_B1 = 0.9
 ADAM_B2 = 0.999
 ADAM_EPS = 1e-8

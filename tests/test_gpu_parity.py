@@ -20,7 +20,7 @@ GRAD_TOL = 1e-5
 
 def make_ctx(prob, S, K, gamma=0.5, B=32, grad_mode=N.EP_GRAD_FIRST_LEAF, scatter=N.EP_SCATTER_SORT,
              precision=N.EP_PREC_FP32, lw=N.EP_LW_NONE, lw_exp=0.0, use_graph=0, seed=20222022, seg_tile=(0, 0),
-             concurrent=1):
+             concurrent=1, noise=0.0):
     cfg = N.default_config()
     cfg.dims, cfg.batch_size, cfg.negatives, cfg.gamma = prob.D, B, K, gamma
     cfg.max_nodes_kind, cfg.max_nodes = N.EP_SAMPLE_FIXED, S
@@ -30,6 +30,7 @@ def make_ctx(prob, S, K, gamma=0.5, B=32, grad_mode=N.EP_GRAD_FIRST_LEAF, scatte
     cfg.seed = seed
     cfg.seg_vpl, cfg.seg_unroll = seg_tile
     cfg.concurrent_batches = concurrent
+    cfg.noise = noise
     cfg.valid_pct = 0.0
     ctx = N.Context(cfg)
     ctx.set_graph(prob.indptr, prob.indices)
@@ -377,6 +378,37 @@ def test_concurrent_batches_tiles_bit_identical_and_learn(precision):
     for o in outs[1:]:
         for a, b in zip(outs[0], o):
             assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("D,P", [(64, 1), (128, 4), (100, 1)])
+def test_gradient_noise_is_gaussian_and_reproducible(D, P):
+    # mod.rs:238-247: each batch's merged gradient row gets sd * N(0,1) per value.  The
+    # explicit hook has no schedule length, so sd = noise.  Same samples without noise
+    # give the clean gradient; the difference must be iid N(0, sd^2) and reproducible.
+    sd = 0.05
+    prob = make_problem(seed=90 + D, n=300, V=200, D=D, zipf=1.2)
+    B = 32
+    anchors = np.arange(0, P * B, dtype=np.uint32)
+    outs = []
+    for noise in (0.0, sd, sd):
+        ctx = make_ctx(prob, 6, 5, 0.5, B=B, concurrent=P, noise=noise)
+        cnt, nbr, neg = ctx.sample_batch(anchors, 1)
+        out = ctx.step_explicit(anchors, cnt, nbr, neg, 1, 0.01, apply_update=False)
+        order = np.lexsort((out["feat"] >> 27, out["feat"] & ((1 << 27) - 1)))
+        outs.append((out["feat"][order], out["grad"][order]))
+        ctx.close()
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[1][0], outs[2][0])
+    assert np.array_equal(outs[1][1], outs[2][1])  # same seed, same noise
+    z = ((outs[1][1] - outs[0][1]) / sd).astype(np.float64)
+    n = z.size
+    print(f"[parity] noise D={D} P={P}: {n} values, mean {z.mean():.4f}, std {z.std():.4f}", flush=True)
+    assert n > 2000
+    assert abs(z.mean()) < 5.0 / np.sqrt(n)
+    assert abs(z.std() - 1.0) < 0.05
+    assert abs(np.mean(z ** 4) - 3.0) < 0.3  # gaussian kurtosis
+    # neighbouring columns and rows uncorrelated
+    assert abs(np.corrcoef(z[:, :-1].ravel(), z[:, 1:].ravel())[0, 1]) < 0.05
+    assert abs(np.corrcoef(z[:-1].ravel(), z[1:].ravel())[0, 1]) < 0.05
 
 
 def test_graph_replay_matches_eager():

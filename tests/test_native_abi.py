@@ -37,7 +37,8 @@ def test_config_defaults_match_reference():
 
 
 @pytest.mark.parametrize("field,value", [("dims", 7), ("negatives", 0), ("batch_size", 0), ("hard_negatives", 2),
-                                         ("noise", 0.5), ("max_features_kind", 1), ("valid_pct", 1.5)])
+                                         ("noise", -0.5), ("noise", float("nan")), ("max_features_kind", 1),
+                                         ("valid_pct", 1.5), ("concurrent_batches", 3), ("seg_vpl", 5)])
 def test_invalid_config_rejected(field, value):
     c = N.default_config()
     setattr(c, field, value)

@@ -402,6 +402,16 @@ def test_group_shared_row_gets_two_adam_steps_from_stale_gradients():
         assert np.allclose(t[f], th, atol=1e-14) and np.allclose(m[f], mm, atol=1e-15)
 
 
+def test_noise_schedule_closed_form():
+    # sd(0) = noise, geometric decay, 1e-9 at the last step; no-op at or below 1e-9
+    noise, T = 0.1, 50
+    assert O.noise_sd(0, noise, T) == pytest.approx(noise, rel=1e-12)
+    assert O.noise_sd(T, noise, T) == pytest.approx(1e-9, rel=1e-9)
+    r = [O.noise_sd(k + 1, noise, T) / O.noise_sd(k, noise, T) for k in range(5)]
+    assert max(r) - min(r) < 1e-12 and r[0] == pytest.approx((1e-9 / noise) ** (1.0 / T))
+    assert O.noise_sd(3, 1e-9, T) == 0.0 and O.noise_sd(3, 0.0, T) == 0.0
+
+
 def test_cos_decay_schedule_closed_form():
     alpha, T = 0.01, 100
     lo = alpha / 100
