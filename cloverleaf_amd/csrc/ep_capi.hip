@@ -178,6 +178,7 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
     keycap = (uint64_t)c->f_max * (s + (uint64_t)batch * (1ull + K));
   }
   if (keycap >= (1ull << 31)) return fail(EP_ECAPACITY, "batch too large: key capacity exceeds 2^31");
+  if ((uint64_t)batch * NC > ep::kSlotMask) return fail(EP_ECAPACITY, "batch too large: slots exceed 2^29");
   const uint64_t hash = next_pow2(2 * std::max<uint64_t>(example_rows_max(c), 1));
   if (cfg.grad_mode == EP_GRAD_FIRST_LEAF && hash > 16384)
     return fail(EP_ECAPACITY, "first_leaf dedup needs <= 8192 feature occurrences per example (F_max*(1+S+K))");
@@ -274,7 +275,7 @@ void refresh_base(ep_ctx* c) {
   b.fids = c->d_fids;
   b.n_nodes = c->n_nodes;
   b.n_vocab = c->n_vocab;
-  // row part: a permutation of [0, 2^bits - 1) (> every row id); batch part: pbits
+  // keys: a permutation of [0, 2^bits - 1) (> every row id)
   int bits = 1;
   while ((1ull << bits) - 1ull < (uint64_t)c->n_vocab) ++bits;
   const uint32_t P = std::max<uint32_t>(1u, cfg.concurrent_batches);
@@ -282,8 +283,8 @@ void refresh_base(ep_ctx* c) {
   while ((1u << pbits) < P) ++pbits;
   b.pbits = pbits;
   b.pmask = (1u << pbits) - 1u;
-  b.sort_bits = bits + (int)pbits;
-  b.sentinel = (uint32_t)((1ull << b.sort_bits) - 1ull);
+  b.sort_bits = bits;  // the batch rides in the sorted value, not in the key
+  b.sentinel = (uint32_t)((1ull << bits) - 1ull);
   const uint64_t m = (1ull << bits) - 1ull;
   b.key_m = (uint32_t)m;
   b.key_bits = (uint32_t)bits;

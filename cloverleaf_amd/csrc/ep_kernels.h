@@ -12,6 +12,9 @@ namespace ep {
 constexpr uint32_t kMaxGroup = 8;
 // Waves per 256-thread block of every kernel.
 constexpr int kBlockWaves = 4;
+// Sorted values: slot in the low bits, batch of the group above.
+constexpr uint32_t kSlotBatchShift = 29;
+constexpr uint32_t kSlotMask = (1u << kSlotBatchShift) - 1u;
 
 // Device-resident per-step state.  Written by k_step_begin / k_scan / the scatter
 // kernels; read by every kernel of the step, so one captured step graph can be
@@ -52,16 +55,18 @@ struct StepConfig {
   const uint32_t* fids;
   uint64_t n_nodes;
   uint32_t n_vocab;
-  // Gradient key = (perm(f) << pbits) | batch, with perm(f) = f * key_a mod key_m
-  // (key_m = 2^(sort_bits - pbits) - 1 > every row id, gcd(key_a, key_m) = 1, key_ainv
-  // the inverse of key_a mod key_m) and batch the index of the example's batch in the
-  // step's group.  The permutation interleaves hot and cold rows in the sorted order, so
-  // 64-entry chunks of the sorted keys hold similar numbers of segments; a row's batches
-  // stay adjacent and in batch order.  sentinel = all ones in sort_bits (perm < key_m).
+  // Gradient key = perm(f) = f * key_a mod key_m (key_m = 2^sort_bits - 1 > every row
+  // id, gcd(key_a, key_m) = 1, key_ainv the inverse of key_a mod key_m): the permutation
+  // interleaves hot and cold rows in the sorted order, so 64-entry chunks of the sorted
+  // keys hold similar numbers of rows.  sentinel = key_m (no real key reaches it).
+  // The value sorted with a key is the slot of its adjoint row with the example's batch
+  // (index in the step's group) in the top bits: (batch << kSlotBatchShift) | slot.  The
+  // sort is stable and keys are emitted in example order, so within a row the values
+  // rise, batch by batch.
   uint32_t sentinel;       // key for "no gradient", above every real key
   int sort_bits;
   uint32_t key_m, key_a, key_ainv, key_bits;  // key_m = 2^key_bits - 1
-  uint32_t pbits, pmask;   // batch bits of a key: log2(group size), group size - 1
+  uint32_t pbits, pmask;   // log2(group size), group size - 1
   // parameters
   void* table;             // float* (fp32) or uint16_t* (bf16)
   // Optimizer state: one record of `ostride` floats per row, [master |] m | v, so an

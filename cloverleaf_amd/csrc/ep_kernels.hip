@@ -25,8 +25,8 @@ static constexpr uint32_t HASH_FMASK = (1u << HASH_FBITS) - 1u;
 // Helpers
 // ------------------------------------------------------------------------------------
 
-// Gradient key of row f for batch `bidx` of the group, and the row of a key (see
-// StepConfig: the row part is a multiplicative permutation modulo key_m).
+// Gradient key of row f and the row of a key (see StepConfig: a multiplicative
+// permutation modulo key_m).
 // x * y mod m for x, y < m = 2^b - 1 (x * y < 2^2b): two folds and one subtraction.
 __device__ inline uint32_t mul_mod_mersenne(uint32_t x, uint32_t y, uint32_t m, uint32_t b) {
   uint64_t p = (uint64_t)x * y;
@@ -34,11 +34,11 @@ __device__ inline uint32_t mul_mod_mersenne(uint32_t x, uint32_t y, uint32_t m, 
   p = (p & m) + (p >> b);  // <= m + 1
   return (uint32_t)(p >= m ? p - m : p);
 }
-__device__ inline uint32_t key_of_row(const StepConfig& c, uint32_t f, uint32_t bidx) {
-  return (mul_mod_mersenne(f, c.key_a, c.key_m, c.key_bits) << c.pbits) | bidx;
+__device__ inline uint32_t key_of_row(const StepConfig& c, uint32_t f) {
+  return mul_mod_mersenne(f, c.key_a, c.key_m, c.key_bits);
 }
 __device__ inline uint32_t row_of_key(const StepConfig& c, uint32_t k) {
-  return mul_mod_mersenne(k >> c.pbits, c.key_ainv, c.key_m, c.key_bits);
+  return mul_mod_mersenne(k, c.key_ainv, c.key_m, c.key_bits);
 }
 
 // First index i in [lo, hi) with K[i] >= x (hi if none), K sorted; 64-ary search by
@@ -659,9 +659,9 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
     const uint32_t ci = cons_of(o);
     const uint32_t f = ids[o];
     uint32_t key = c.sentinel;
-    if (s_el[ci] && (!first_leaf || hash_lookup(hsh, hmask, f) == ci)) key = key_of_row(c, f, b / c.rbatch);
+    if (s_el[ci] && (!first_leaf || hash_lookup(hsh, hmask, f) == ci)) key = key_of_row(c, f);
     c.keys[0][kb + o] = key;
-    c.slots[0][kb + o] = b * NC + ci;
+    c.slots[0][kb + o] = ((b / c.rbatch) << kSlotBatchShift) | (b * NC + ci);
   }
 }
 
@@ -733,10 +733,10 @@ __global__ __launch_bounds__(256) void k_emit(StepConfig c, StepState* st) {
           if (ok) hash_insert_min(hsh, hmask, f, cj);
         } else {
           uint32_t key = c.sentinel;
-          if (ok && (!first_leaf || hash_lookup(hsh, hmask, f) == cj)) key = key_of_row(c, f, b / c.rbatch);
+          if (ok && (!first_leaf || hash_lookup(hsh, hmask, f) == cj)) key = key_of_row(c, f);
           const uint32_t pos = kbase + running + o;
           c.keys[0][pos] = key;
-          c.slots[0][pos] = slot0 + cj;
+          c.slots[0][pos] = ((b / c.rbatch) << kSlotBatchShift) | (slot0 + cj);
         }
       }
       running += total;
@@ -1056,7 +1056,8 @@ __device__ inline void write_long_row(const StepConfig& c, uint32_t idx, uint32_
   }
 }
 
-__global__ __launch_bounds__(256) void k_seg_plan(StepConfig c, StepState* st, const uint32_t* __restrict__ Ks) {
+__global__ __launch_bounds__(256) void k_seg_plan(StepConfig c, StepState* st, const uint32_t* __restrict__ Ks,
+                                                  const uint32_t* __restrict__ Ss) {
   __shared__ uint32_t s_run[NW][132];  // run starts (chunk-relative); s_run[nrun] = window end
   __shared__ uint8_t s_bat[NW][132];   // batch of each run
   __shared__ uint32_t s_row[NW][66];   // first run of each owned row; s_row[nrow] = nrun
@@ -1065,7 +1066,7 @@ __global__ __launch_bounds__(256) void k_seg_plan(StepConfig c, StepState* st, c
   const uint32_t gw = blockIdx.x * NW + w;
   const uint32_t nwaves = gridDim.x * NW;
   const uint32_t n = st->n_keys <= c.key_cap ? st->n_real : 0u;  // sorted, sentinels dropped
-  const uint32_t SENT = c.sentinel, PB = c.pbits, PM = c.pmask;
+  const uint32_t SENT = c.sentinel, PM = c.pmask;
   const uint32_t nchunks = (n + 63u) / 64u;
   const uint32_t cpw = (nchunks + nwaves - 1u) / nwaves;
   const uint32_t cbeg = gw * cpw, cend = min(nchunks, cbeg + cpw);
@@ -1074,15 +1075,22 @@ __global__ __launch_bounds__(256) void k_seg_plan(StepConfig c, StepState* st, c
   for (uint32_t chunk = cbeg; chunk < cend; ++chunk) {
     const uint32_t i0 = chunk * 64u;
     const uint32_t j0 = i0 + lane, j1 = i0 + 64u + lane;
-    const uint32_t k0 = j0 < n ? Ks[j0] : SENT;
-    const uint32_t k1 = j1 < n ? Ks[j1] : SENT;
-    uint32_t prev = __shfl_up(k0, 1, EP_WAVE);
-    if (lane == 0) prev = i0 > 0u ? Ks[i0 - 1u] : ~k0;
-    const uint32_t last0 = __shfl(k0, 63, EP_WAVE);
-    uint32_t prev1 = __shfl_up(k1, 1, EP_WAVE);
-    if (lane == 0) prev1 = last0;
-    const uint32_t r0 = k0 >> PB, r1 = k1 >> PB;  // row parts (SENT's is above every row)
-    const bool is_start = (k0 != SENT) && (r0 != (prev >> PB));
+    const uint32_t r0 = j0 < n ? Ks[j0] : SENT;  // rows (permuted ids); SENT is above every row
+    const uint32_t r1 = j1 < n ? Ks[j1] : SENT;
+    const uint32_t b0 = j0 < n ? Ss[j0] >> kSlotBatchShift : 0u;  // batches
+    const uint32_t b1 = j1 < n ? Ss[j1] >> kSlotBatchShift : 0u;
+    uint32_t prev = __shfl_up(r0, 1, EP_WAVE), pb0 = __shfl_up(b0, 1, EP_WAVE);
+    if (lane == 0) {
+      prev = i0 > 0u ? Ks[i0 - 1u] : ~r0;
+      pb0 = i0 > 0u ? Ss[i0 - 1u] >> kSlotBatchShift : 0u;
+    }
+    const uint32_t last0 = __shfl(r0, 63, EP_WAVE), lastb0 = __shfl(b0, 63, EP_WAVE);
+    uint32_t prev1 = __shfl_up(r1, 1, EP_WAVE), pb1 = __shfl_up(b1, 1, EP_WAVE);
+    if (lane == 0) {
+      prev1 = last0;
+      pb1 = lastb0;
+    }
+    const bool is_start = (r0 != SENT) && (r0 != prev);
     const uint64_t starts = __ballot(is_start);
     if (starts == 0ull) continue;
     const int first = __ffsll((unsigned long long)starts) - 1;
@@ -1094,19 +1102,19 @@ __global__ __launch_bounds__(256) void k_seg_plan(StepConfig c, StepState* st, c
     const uint32_t wend = m0 ? (uint32_t)(__ffsll((unsigned long long)m0) - 1)
                              : (m1 ? 64u + (uint32_t)(__ffsll((unsigned long long)m1) - 1) : (uint32_t)ql);
     // runs of the rows handled in the window: [first, wend)
-    const uint64_t fs0 = __ballot(lane >= first && (uint32_t)lane < wend && k0 != prev);
-    const uint64_t fs1 = __ballot(64u + (uint32_t)lane < wend && k1 != prev1);
+    const uint64_t fs0 = __ballot(lane >= first && (uint32_t)lane < wend && (r0 != prev || b0 != pb0));
+    const uint64_t fs1 = __ballot(64u + (uint32_t)lane < wend && (r1 != prev1 || b1 != pb1));
     const uint32_t n0 = (uint32_t)__popcll(fs0);
     const uint32_t nrun = n0 + (uint32_t)__popcll(fs1);
     if ((fs0 >> lane) & 1ull) {
       const uint32_t i = (uint32_t)__popcll(fs0 & lt);
       s_run[w][i] = (uint32_t)lane;
-      s_bat[w][i] = (uint8_t)(k0 & PM);
+      s_bat[w][i] = (uint8_t)b0;
     }
     if ((fs1 >> lane) & 1ull) {
       const uint32_t i = n0 + (uint32_t)__popcll(fs1 & lt);
       s_run[w][i] = 64u + (uint32_t)lane;
-      s_bat[w][i] = (uint8_t)(k1 & PM);
+      s_bat[w][i] = (uint8_t)b1;
     }
     const uint64_t rowm = deferred ? (starts & ~(1ull << ql)) : starts;
     const uint32_t nrow = (uint32_t)__popcll(rowm);
@@ -1118,9 +1126,11 @@ __global__ __launch_bounds__(256) void k_seg_plan(StepConfig c, StepState* st, c
     if (deferred) {
       // the last row runs past the window: find its runs by search, register it as long
       const uint32_t lo = i0 + (uint32_t)ql;
-      uint32_t bnd[kMaxGroup + 1];
+      const uint32_t hi = wave_lower_bound(Ks, lo + 128u - (uint32_t)ql, n, fl + 1u);  // row end
+      uint32_t bnd[kMaxGroup + 1];  // batch boundaries: the row's values rise batch by batch
       bnd[0] = lo;
-      for (uint32_t b = 0; b <= PM; ++b) bnd[b + 1] = wave_lower_bound(Ks, bnd[b], n, ((fl << PB) | b) + 1u);
+      for (uint32_t b = 0; b < PM; ++b) bnd[b + 1] = wave_lower_bound(Ss, bnd[b], hi, (b + 1u) << kSlotBatchShift);
+      bnd[PM + 1] = hi;
       uint32_t pcs = 0, nr = 0;
       for (uint32_t b = 0; b <= PM; ++b) {
         if (bnd[b + 1] > bnd[b]) ++nr;
@@ -1155,7 +1165,7 @@ __global__ __launch_bounds__(256) void k_seg_plan(StepConfig c, StepState* st, c
     const bool med = own && (ez - ea) > kSegShort;
     const uint64_t mmask = __ballot(med);
     const uint64_t smask = __ballot(own && !med);
-    const uint32_t f = row_of_key(c, __shfl(k0, (int)ea, EP_WAVE));  // row starts lie in [0, 64)
+    const uint32_t f = row_of_key(c, __shfl(r0, (int)ea, EP_WAVE));  // row starts lie in [0, 64)
     if (own && !med) {
       uint32_t mask = 0u, meta = ez - ea;
       for (uint32_t j = ra; j < rz; ++j) {
@@ -1219,7 +1229,7 @@ __global__ __launch_bounds__(256) void k_seg_update(StepConfig c, StepState* st,
 #pragma unroll
     for (int i = 0; i < NS; ++i) {
       const uint32_t e = (uint32_t)(lg + i * L);
-      sl[i] = (has && e < len) ? Ss[start + e] : 0u;
+      sl[i] = (has && e < len) ? (Ss[start + e] & kSlotMask) : 0u;
     }
     float4 m4[NQ], v4[NQ], t4[NQ];
     const size_t roff = (size_t)f * D;
@@ -1434,7 +1444,7 @@ __global__ __launch_bounds__(256) void k_pieces(StepConfig c, StepState* st, con
   for (uint32_t gp = gw; gp < np; gp += nwaves) {
     const uint32_t s0 = c.piece_st[gp];
     const uint32_t m = c.piece_info[gp];  // <= PIECE <= 64
-    const uint32_t sj = (uint32_t)lane < m ? Ss[s0 + lane] : 0u;
+    const uint32_t sj = (uint32_t)lane < m ? (Ss[s0 + lane] & kSlotMask) : 0u;
     float acc[PL];
 #pragma unroll
     for (int q = 0; q < PL; ++q) acc[q] = 0.f;
@@ -1468,12 +1478,11 @@ __global__ __launch_bounds__(256) void k_pieces(StepConfig c, StepState* st, con
     const uint2 rec = recs[k];
     const uint32_t start = rec.x, len = rec.y;  // 32 < len <= 128
     const uint32_t e0 = (uint32_t)lane, e1 = 64u + (uint32_t)lane;
-    const uint32_t kk0 = e0 < len ? Ks[start + e0] : 0u;
-    const uint32_t kk1 = e1 < len ? Ks[start + e1] : 0u;
-    const uint32_t ss0 = e0 < len ? Ss[start + e0] : 0u;
-    const uint32_t ss1 = e1 < len ? Ss[start + e1] : 0u;
-    const uint32_t key0 = __shfl(kk0, 0, EP_WAVE);
-    const uint32_t f = row_of_key(c, key0);
+    const uint32_t kk0 = e0 < len ? Ss[start + e0] >> kSlotBatchShift : 0u;  // batches
+    const uint32_t kk1 = e1 < len ? Ss[start + e1] >> kSlotBatchShift : 0u;
+    const uint32_t ss0 = e0 < len ? (Ss[start + e0] & kSlotMask) : 0u;
+    const uint32_t ss1 = e1 < len ? (Ss[start + e1] & kSlotMask) : 0u;
+    const uint32_t f = row_of_key(c, Ks[start]);
     float th[PL], mo[PL], va[PL];
     if (!c.grad_out) load_row_state<D>(c, f, lane, th, mo, va);
     const uint32_t last0 = __shfl(kk0, 63, EP_WAVE);
@@ -1488,8 +1497,9 @@ __global__ __launch_bounds__(256) void k_pieces(StepConfig c, StepState* st, con
       if (rs0) rs0 &= rs0 - 1ull; else rs1 &= rs1 - 1ull;
       const uint32_t z = rs0 ? (uint32_t)(__ffsll((unsigned long long)rs0) - 1)
                              : (rs1 ? 64u + (uint32_t)(__ffsll((unsigned long long)rs1) - 1) : len);
-      const uint32_t ka = a < 64u ? __shfl(kk0, (int)a, EP_WAVE) : __shfl(kk1, (int)(a - 64u), EP_WAVE);
-      const uint32_t bq = ka & c.pmask;
+      const uint32_t ka = __shfl(kk0, (int)(a & 63u), EP_WAVE);
+      const uint32_t kb = __shfl(kk1, (int)(a & 63u), EP_WAVE);
+      const uint32_t bq = a < 64u ? ka : kb;
       float acc[PL];
 #pragma unroll
       for (int q = 0; q < PL; ++q) acc[q] = 0.f;
@@ -1603,7 +1613,7 @@ __global__ __launch_bounds__(256) void k_atomic_scatter(StepConfig c, StepState*
     const uint32_t j = i0 + lane;
     const bool valid = j < n;
     const uint32_t kf = valid ? c.keys[0][j] : SENT;
-    const uint32_t s = valid ? c.slots[0][j] : 0u;
+    const uint32_t s = valid ? (c.slots[0][j] & kSlotMask) : 0u;
     const bool real = kf != SENT;
     const uint32_t f = real ? row_of_key(c, kf) : 0u;
     const bool newt = real && atomicCAS(&c.flag[f], 0u, 1u) == 0u;
@@ -1869,7 +1879,7 @@ static void launch_seg_update_variant(const StepConfig& c, StepState* st, int bu
 
 hipError_t launch_scatter_sorted(const StepConfig& c, StepState* st, int buf, hipStream_t s, hipEvent_t* ev) {
   const int grid = c.unique_grid;
-  hipLaunchKernelGGL(k_seg_plan, dim3(grid), dim3(256), 0, s, c, st, c.keys[buf]);
+  hipLaunchKernelGGL(k_seg_plan, dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
   if (ev) (void)hipEventRecord(ev[0], s);
   EP_DISPATCH(c.dims, {
     launch_seg_update_variant<D>(c, st, buf, s, grid);
