@@ -375,6 +375,76 @@ __device__ inline void gather_lds(const T* __restrict__ table, const uint32_t* i
   }
 }
 
+// Wide gather: lane groups of LW = D / VW lanes each load one row with 16-byte loads
+// (VW = 4 FP32 or 8 BF16 values per lane), RPW = 64 / LW rows per wave instruction and U
+// rows in flight per group; the group partial sums are added across groups and lanes of
+// group 0 write scale * sum to dst[0..D) (LDS).  Used where LW is a power of two.
+template <typename T> struct WideVec;
+template <> struct WideVec<float> { static constexpr int VW = 4; };
+template <> struct WideVec<uint16_t> { static constexpr int VW = 8; };
+
+template <int D, typename T>
+struct WideShape {
+  static constexpr int VW = WideVec<T>::VW;
+  static constexpr int LW = D / VW;
+  static constexpr bool OK = (D % VW == 0) && LW >= 1 && LW <= 64 && ((LW & (LW - 1)) == 0);
+  static constexpr int RPW = OK ? 64 / LW : 1;
+};
+
+template <typename T>
+__device__ inline void load_wide(const T* __restrict__ p, float (&v)[WideVec<T>::VW]);
+template <>
+__device__ inline void load_wide<float>(const float* __restrict__ p, float (&v)[4]) {
+  const float4 t = *reinterpret_cast<const float4*>(p);
+  v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+}
+template <>
+__device__ inline void load_wide<uint16_t>(const uint16_t* __restrict__ p, float (&v)[8]) {
+  const uint4 t = *reinterpret_cast<const uint4*>(p);
+  const uint32_t w[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[2 * k] = __uint_as_float(w[k] << 16);
+    v[2 * k + 1] = __uint_as_float(w[k] & 0xffff0000u);
+  }
+}
+
+template <int D, int U, typename T>
+__device__ inline void gather_lds_wide(const T* __restrict__ table, const uint32_t* ids, uint32_t a, uint32_t b,
+                                       int lane, float* dst, float scale) {
+  using WS = WideShape<D, T>;
+  constexpr int VW = WS::VW, LW = WS::LW, RPW = WS::RPW;
+  const int g = lane / LW, lg = lane % LW;
+  float acc[VW];
+#pragma unroll
+  for (int k = 0; k < VW; ++k) acc[k] = 0.f;
+  for (uint32_t j = a + (uint32_t)g; j < b; j += (uint32_t)(RPW * U)) {
+    float r[U][VW];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t t = j + (uint32_t)(u * RPW);
+      const uint32_t tt = t < b ? t : j;  // j < b: a valid row to load (result unused)
+      load_wide<T>(table + (size_t)ids[tt] * D + lg * VW, r[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (j + (uint32_t)(u * RPW) < b) {
+#pragma unroll
+        for (int k = 0; k < VW; ++k) acc[k] += r[u][k];
+      }
+    }
+  }
+#pragma unroll
+  for (int off = LW; off < 64; off <<= 1) {
+#pragma unroll
+    for (int k = 0; k < VW; ++k) acc[k] += __shfl_xor(acc[k], off, EP_WAVE);
+  }
+  if (g == 0) {
+#pragma unroll
+    for (int k = 0; k < VW; ++k) dst[lg * VW + k] = acc[k] * scale;
+  }
+}
+
 // 32-bit words of the float part of the forward kernel's LDS (xc, part, dist, sn),
 // rounded up to an even count so the 64-bit entry array after it is aligned.
 __host__ __device__ inline uint32_t forward_float_words(uint32_t NC, uint32_t K, uint32_t D) {
@@ -395,6 +465,7 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
   constexpr int PL = RowShape<D>::PL;
   constexpr int U = 8;
   constexpr int GU = PL <= 2 ? 16 : 8;  // rows in flight per wave on the staged path
+  constexpr int WU = WideShape<D, T>::RPW >= 4 ? 4 : 8;  // rows in flight per lane group (wide)
   const uint32_t b = blockIdx.x;
   if (b >= st->batch_n) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -462,26 +533,34 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
     const uint32_t p0 = ent_off[1], p1 = ent_off[1 + cnt];
     const uint32_t per = (p1 - p0 + NW - 1) / NW;
     const uint32_t a0 = min(p1, p0 + w * per), a1 = min(p1, a0 + per);
-    float acc[PL];
+    if constexpr (WideShape<D, T>::OK) {
+      gather_lds_wide<D, WU, T>(table, ids, a0, a1, lane, part + w * D, 1.0f);
+    } else {
+      float acc[PL];
 #pragma unroll
-    for (int q = 0; q < PL; ++q) acc[q] = 0.f;
-    gather_lds<D, GU>(table, ids, a0, a1, lane, acc);
+      for (int q = 0; q < PL; ++q) acc[q] = 0.f;
+      gather_lds<D, GU>(table, ids, a0, a1, lane, acc);
 #pragma unroll
-    for (int q = 0; q < PL; ++q)
-      if (base + q < D) part[w * D + base + q] = acc[q];
+      for (int q = 0; q < PL; ++q)
+        if (base + q < D) part[w * D + base + q] = acc[q];
+    }
     if (tid == 0) sn[1] = (float)(p1 - p0);
     // single-node constructions: i = 0 -> h(v) (c=0), i >= 1 -> h(u_i) (c=i+1)
     for (uint32_t i = w; i < K + 1u; i += NW) {
       const uint32_t e = i == 0u ? 0u : cnt + i;
       const uint32_t ci = i == 0u ? 0u : i + 1u;
-      float a2[PL];
-#pragma unroll
-      for (int q = 0; q < PL; ++q) a2[q] = 0.f;
-      gather_lds<D, GU>(table, ids, ent_off[e], ent_off[e + 1], lane, a2);
       const float n = (float)(ent_off[e + 1] - ent_off[e]);
+      if constexpr (WideShape<D, T>::OK) {
+        gather_lds_wide<D, WU, T>(table, ids, ent_off[e], ent_off[e + 1], lane, xc + ci * D, 1.0f / n);
+      } else {
+        float a2[PL];
 #pragma unroll
-      for (int q = 0; q < PL; ++q)
-        if (base + q < D) xc[ci * D + base + q] = a2[q] / n;
+        for (int q = 0; q < PL; ++q) a2[q] = 0.f;
+        gather_lds<D, GU>(table, ids, ent_off[e], ent_off[e + 1], lane, a2);
+#pragma unroll
+        for (int q = 0; q < PL; ++q)
+          if (base + q < D) xc[ci * D + base + q] = a2[q] / n;
+      }
       if (lane == 0) sn[ci] = n;
     }
     __syncthreads();
