@@ -17,12 +17,29 @@ import sys
 from collections import defaultdict
 
 
-def kernel_stats(db_path: str):
+def kernel_stats(db_path: str, split_at: str = "", labels=()):
+    """Per-kernel duration statistics.  With `split_at`, the dispatches are cut into
+    consecutive segments, each starting at a dispatch of a kernel whose name contains
+    `split_at` (bench.py initialises a fresh table once per form, so k_init_table
+    separates the forms; a segment ends where the next one's split kernel runs, so set-up
+    copies and fills just before it count in the previous segment); the result is a list of (label, rows) per segment."""
     db = sqlite3.connect(db_path)
-    rows = db.execute("select name, count(*), avg(end-start), min(end-start), max(end-start), sum(end-start) "
-                      "from kernels group by name order by sum(end-start) desc").fetchall()
-    return [dict(name=r[0], count=r[1], avg_us=r[2] / 1e3, min_us=r[3] / 1e3, max_us=r[4] / 1e3,
-                 total_us=r[5] / 1e3) for r in rows]
+    disp = db.execute("select name, start, end from kernels order by start").fetchall()
+    segs = [[]]
+    for name, t0, t1 in disp:
+        if split_at and split_at in name and any(split_at in n for n, _ in segs[-1]):
+            segs.append([])
+        segs[-1].append((name, t1 - t0))
+    out = []
+    for i, seg in enumerate(segs if split_at else [sum(segs, [])]):
+        acc = defaultdict(list)
+        for name, dt in seg:
+            acc[name].append(dt)
+        rows = [dict(name=k, count=len(v), avg_us=sum(v) / len(v) / 1e3, min_us=min(v) / 1e3, max_us=max(v) / 1e3,
+                     total_us=sum(v) / 1e3) for k, v in acc.items()]
+        rows.sort(key=lambda r: -r["total_us"])
+        out.append((labels[i] if i < len(labels) else f"segment {i}", rows))
+    return out if split_at else out[0][1]
 
 
 def counter_per_kernel(db_path: str, counter: str):
@@ -70,6 +87,8 @@ def main(argv=None):
     s1 = sub.add_parser("stats")
     s1.add_argument("db")
     s1.add_argument("--top", type=int, default=25)
+    s1.add_argument("--split-at", default="", help="start a new table at each dispatch of this kernel")
+    s1.add_argument("--labels", default="", help="comma-separated titles of the split tables")
     s2 = sub.add_parser("traffic")
     s2.add_argument("fetch_db")
     s2.add_argument("write_db")
@@ -79,11 +98,16 @@ def main(argv=None):
     s2.add_argument("--command", default="")
     a = ap.parse_args(argv)
     if a.cmd == "stats":
-        print("| kernel | dispatches | avg us | min us | max us | total us |")
-        print("|---|---|---|---|---|---|")
-        for r in kernel_stats(a.db)[:a.top]:
-            print(f"| `{short_name(r['name'])}` | {r['count']} | {r['avg_us']:.2f} | {r['min_us']:.2f} | "
-                  f"{r['max_us']:.2f} | {r['total_us']:.0f} |")
+        tables = (kernel_stats(a.db, a.split_at, [x for x in a.labels.split(",") if x]) if a.split_at
+                  else [("", kernel_stats(a.db))])
+        for label, rows in tables:
+            if label:
+                print(f"\n### {label}\n")
+            print("| kernel | dispatches | avg us | min us | max us | total us |")
+            print("|---|---|---|---|---|---|")
+            for r in rows[:a.top]:
+                print(f"| `{short_name(r['name'])}` | {r['count']} | {r['avg_us']:.2f} | {r['min_us']:.2f} | "
+                      f"{r['max_us']:.2f} | {r['total_us']:.0f} |")
         return 0
     ker, cal = traffic(a.fetch_db, a.write_db, a.probe_mib)
     try:
