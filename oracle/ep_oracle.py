@@ -34,6 +34,10 @@ Pinning (what checks each function; see tests/test_oracle.py):
   cos_decay_lr                   closed form at step 1, warm-up end, warm-up+1, T
   attention_forward              reference KATs attention.rs:316-425
   attention_backward             central finite differences, torch fp64 autograd
+  attention_example_grads        central finite differences of the example loss w.r.t. the
+                                 table (exact); torch fp64 autograd with one leaf per
+                                 (construction, feature) (first_leaf); item rules checked
+                                 on duplicate features and single-feature nodes
 """
 from __future__ import annotations
 
@@ -382,18 +386,26 @@ def adam_update(table, mom, var, grads: Dict[int, np.ndarray], lr: float, pass_:
         table[f] = table[f] - lr * (mh / (np.sqrt(vh) + ADAM_EPS))
 
 
+def _example(table, f_indptr, f_ids, v, nbrs, nbr_scale, negs, gamma, grad_mode, div, attention):
+    if attention is None:
+        return example_grads(table, f_indptr, f_ids, v, nbrs, nbr_scale, negs, gamma, grad_mode, div)
+    heads, d_k = attention
+    return attention_example_grads(table, f_indptr, f_ids, v, nbrs, negs, gamma, heads, d_k, grad_mode, div)
+
+
 def ep_batch_step(table, mom, var, f_indptr, f_ids, anchors, nbr_lists, nbr_scales, neg_lists,
-                  gamma, lr, pass_, grad_mode="first_leaf", loss_divs=None):
+                  gamma, lr, pass_, grad_mode="first_leaf", loss_divs=None, attention=None):
     """One batch: per-example forward/backward, merge, Adam (mod.rs:174-252).
 
     Updates table/mom/var in place (float64 arrays).  Returns (results, merged grads).
     The Adam update runs only when at least one example contributes (mod.rs:237).
+    attention = (heads, d_k) selects the attention model (attention_example_grads).
     """
     results = []
     for i, v in enumerate(anchors):
         div = 1.0 if loss_divs is None else loss_divs[i]
-        results.append(example_grads(table, f_indptr, f_ids, v, nbr_lists[i], nbr_scales[i],
-                                     neg_lists[i], gamma, grad_mode, div))
+        results.append(_example(table, f_indptr, f_ids, v, nbr_lists[i], nbr_scales[i], neg_lists[i], gamma,
+                                grad_mode, div, attention))
     merged = merge_batch(results, table.shape[1])
     if any(r.contributes for r in results):
         adam_update(table, mom, var, merged, lr, pass_)
@@ -401,7 +413,7 @@ def ep_batch_step(table, mom, var, f_indptr, f_ids, anchors, nbr_lists, nbr_scal
 
 
 def ep_group_step(table, mom, var, f_indptr, f_ids, anchors, nbr_lists, nbr_scales, neg_lists,
-                  gamma, lrs, batch_of, pass_, grad_mode="first_leaf", loss_divs=None):
+                  gamma, lrs, batch_of, pass_, grad_mode="first_leaf", loss_divs=None, attention=None):
     """A group of concurrently running batches (mod.rs:174: par_iter over the batch
     chunks, Hogwild updates of the shared store), serialised: every batch of the group
     computes its gradients from the parameters as they were at the group's start, then
@@ -416,8 +428,8 @@ def ep_group_step(table, mom, var, f_indptr, f_ids, anchors, nbr_lists, nbr_scal
     results = []
     for i, v in enumerate(anchors):
         div = 1.0 if loss_divs is None else loss_divs[i]
-        results.append(example_grads(snap, f_indptr, f_ids, v, nbr_lists[i], nbr_scales[i],
-                                     neg_lists[i], gamma, grad_mode, div))
+        results.append(_example(snap, f_indptr, f_ids, v, nbr_lists[i], nbr_scales[i], neg_lists[i], gamma,
+                                grad_mode, div, attention))
     merged = []
     for k in range(len(lrs)):
         part = [r for r, b in zip(results, batch_of) if b == k]
@@ -528,6 +540,126 @@ def attention_mean_backward(rows: np.ndarray, counts: np.ndarray, heads: int, d_
         grad[:, h * d_k:(h + 1) * d_k] += dT @ K[h]
         grad[:, heads * d_k + h * d_k: heads * d_k + (h + 1) * d_k] += dT.T @ Q[h]
     return grad
+
+
+def attention_node_items(f_indptr, f_ids, node):
+    """Items of h(node) under the attention model.
+
+    attention_construct_node_embedding (model.rs:306-335) collects the node's features
+    with weight 1 (each occurrence adds 1 + 1, model.rs:267-268) into a map and runs
+    attention over the map's entries: one item per distinct feature, count = 2 x its
+    multiplicity.  Items are in first-seen order; Full attention is permutation
+    invariant, so the map's iteration order does not change the result.
+    """
+    con = Construction()
+    collect_node(con, node_features(f_indptr, f_ids, node), 1.0)
+    feats = list(con.counts.keys())
+    return feats, [con.counts[f] for f in feats]
+
+
+def attention_neighbour_items(f_indptr, f_ids, node):
+    """Items of one neighbour's attention inside ~h(v) (attention_multiple, model.rs:419-446).
+
+    The neighbour's feature list as listed (a repeated feature is repeated), each
+    occurrence with the feature's multiplicity within that neighbour as its count.
+    """
+    feats = [int(f) for f in node_features(f_indptr, f_ids, node)]
+    mult: Dict[int, int] = {}
+    for f in feats:
+        mult[f] = mult.get(f, 0) + 1
+    return feats, [float(mult[f]) for f in feats]
+
+
+def attention_inference_items(feats):
+    """Items of AttentionAggregator::construct (aggregator.rs:137-153): every listed
+    feature with count 1."""
+    return [int(f) for f in feats], [1.0] * len(feats)
+
+
+def _scatter_rows(out: Dict[int, np.ndarray], feats, rows: np.ndarray):
+    for f, r in zip(feats, rows):
+        out[f] = out[f] + r if f in out else r.copy()
+
+
+def attention_example_grads(table, f_indptr, f_ids, anchor, nbrs, negs, gamma, heads, d_k,
+                            grad_mode="first_leaf", loss_div=1.0) -> ExampleResult:
+    """Forward + adjoint + gradient extraction for one anchor under the attention model.
+
+    AttentionFeatureModel (model.rs:158-249):
+      a   = h(v)   attention over the anchor's distinct features (attention_node_items)
+      p   = ~h(v)  mean over the sampled neighbours of each neighbour's attention
+                   (attention_neighbour_items; construct_from_multiple_nodes with mha,
+                   model.rs:388-416, then mean_embeddings with count 1 per neighbour)
+      n_k = h(u_k) as a
+    Margin loss and weighting as the mean model.  Each construction has one autodiff
+    leaf per distinct feature (the NodeCounts map); inside ~h(v) every occurrence of a
+    feature in every neighbour reads the same leaf, so the leaf's gradient sums them.
+    Leaf gradients are full table rows [Q | K | V] of width heads * (2 d_k + d_model).
+    The first_leaf / exact rules and the eligibility of constructions are those of
+    example_grads.
+    """
+    table = np.asarray(table)
+    W = table.shape[1]
+    node_sets = [attention_node_items(f_indptr, f_ids, anchor)]
+    nb_sets = [attention_neighbour_items(f_indptr, f_ids, u) for u in nbrs]
+    neg_sets = [attention_node_items(f_indptr, f_ids, u) for u in negs]
+
+    def fwd(items):
+        feats, counts = items
+        return attention_mean(table[np.asarray(feats, dtype=np.int64)], np.asarray(counts), heads, d_k)
+
+    a = fwd(node_sets[0])
+    p = sum(fwd(it) for it in nb_sets) / float(len(nb_sets))
+    nv = [fwd(it) for it in neg_sets]
+    vecs = [a, p] + nv
+    L, active, g_a, g_p, g_n = margin_loss_adjoints(a, p, nv, gamma)
+    L = L / loss_div
+    adjs = [g_a, g_p] + list(g_n)
+    adjs = [None if g is None else g / loss_div for g in adjs]
+    if not (L > 0.0) or math.isnan(L):
+        return ExampleResult(L if not math.isnan(L) else float("nan"), False, {}, vecs, adjs)
+
+    def bwd(items, g):
+        feats, counts = items
+        rows = table[np.asarray(feats, dtype=np.int64)]
+        return attention_mean_backward(rows, np.asarray(counts), heads, d_k, g)
+
+    # leaf gradients per construction, in construction order a, p, n_1..n_K
+    leaves: List[Optional[Dict[int, np.ndarray]]] = []
+    for ci, adj in enumerate(adjs):
+        if adj is None or not np.all(np.isfinite(adj)):
+            leaves.append(None)
+            continue
+        out: Dict[int, np.ndarray] = {}
+        if ci == 1:
+            for it in nb_sets:
+                _scatter_rows(out, it[0], bwd(it, adj / float(len(nb_sets))))
+        else:
+            it = node_sets[0] if ci == 0 else neg_sets[ci - 2]
+            _scatter_rows(out, it[0], bwd(it, adj))
+        leaves.append(out)
+    grads: Dict[int, np.ndarray] = {}
+    for out in leaves:
+        if out is None:
+            continue
+        for f, g in out.items():
+            if g.shape != (W,) or not np.all(np.isfinite(g)):
+                continue
+            if grad_mode == "first_leaf":
+                if f not in grads:
+                    grads[f] = g
+            elif grad_mode == "exact":
+                grads[f] = grads[f] + g if f in grads else g
+            else:
+                raise ValueError(grad_mode)
+    return ExampleResult(L, True, grads, vecs, adjs)
+
+
+def attention_embed(table, feats, heads, d_k) -> np.ndarray:
+    """AttentionAggregator::construct (aggregator.rs:137-153): Full attention over the
+    listed features with count 1 each (float64)."""
+    items = attention_inference_items(feats)
+    return attention_mean(np.asarray(table)[np.asarray(items[0], dtype=np.int64)], np.asarray(items[1]), heads, d_k)
 
 
 # --------------------------------------------------------------------------------------

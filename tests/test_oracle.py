@@ -510,6 +510,143 @@ def test_attention_backward_vs_fd_and_torch(n, heads, dk, dm):
 
 
 # --------------------------------------------------------------------------------------
+# Attention model example: model.rs:158-249, 306-335, 419-446
+# --------------------------------------------------------------------------------------
+def _att_toy(seed, V=10, heads=2, dk=2, dm=3, n_nodes=10, K=3):
+    rng = np.random.default_rng(seed)
+    counts = rng.integers(1, 5, size=n_nodes)
+    counts[4] = 1  # a single-feature node (attention.rs:113-115)
+    f_indptr = np.concatenate([[0], np.cumsum(counts)])
+    f_ids = rng.integers(0, V, size=f_indptr[-1])  # repeated features inside nodes and across them
+    W = heads * (2 * dk + dm)
+    table = 0.7 * rng.normal(size=(V, W))
+    negs = [4] + list(rng.integers(0, n_nodes, size=K - 1))
+    return table, f_indptr, f_ids, 0, [1, 2, 3], negs
+
+
+def test_attention_item_rules():
+    f_indptr = np.array([0, 3])
+    f_ids = np.array([5, 7, 5])
+    # h(v): one item per distinct feature, each occurrence adds 1 + 1 (model.rs:267-268)
+    assert O.attention_node_items(f_indptr, f_ids, 0) == ([5, 7], [4.0, 2.0])
+    # ~h(v) block: the list as listed, count = multiplicity in the neighbour (model.rs:426-443)
+    assert O.attention_neighbour_items(f_indptr, f_ids, 0) == ([5, 7, 5], [2.0, 1.0, 2.0])
+    # inference: every listed feature, count 1 (aggregator.rs:141-146)
+    assert O.attention_inference_items([5, 7, 5]) == ([5, 7, 5], [1.0, 1.0, 1.0])
+
+
+def _torch_attention(leaf_rows, counts, heads, dk):
+    n = len(leaf_rows)
+    R = torch.stack(leaf_rows)
+    dm = (R.shape[1] - 2 * heads * dk) // heads
+    if n == 1:
+        return R[0, 2 * heads * dk: 2 * heads * dk + dm]
+    c = torch.tensor(counts, dtype=torch.float64)
+    num = torch.outer(c, c)
+    m = torch.where(num >= 1, num, torch.ones_like(num))
+    out = 0
+    for h in range(heads):
+        Qh = R[:, h * dk:(h + 1) * dk]
+        Kh = R[:, heads * dk + h * dk: heads * dk + (h + 1) * dk]
+        Vh = R[:, 2 * heads * dk + h * dm: 2 * heads * dk + (h + 1) * dm]
+        A = torch.softmax((Qh @ Kh.T) * m / math.sqrt(dk), dim=1)
+        out = out + (A @ Vh).sum(0) / n
+    return out / heads
+
+
+def torch_attention_first_leaf(table, f_indptr, f_ids, v, nbrs, negs, gamma, heads, dk):
+    """The reference's mechanism in torch fp64: one leaf per (construction, distinct
+    feature); h(v)/h(u) attend over the map entries with count 2 x multiplicity; ~h(v)
+    runs one attention per neighbour over its listed features (counts = multiplicity
+    in that neighbour), all reading the construction's leaves, then the plain mean."""
+    def leaf(f):
+        return torch.tensor(table[f], dtype=torch.float64, requires_grad=True)
+
+    def node_con(u):
+        d = {}
+        for f in f_ids[f_indptr[u]:f_indptr[u + 1]]:
+            f = int(f)
+            if f in d:
+                d[f][1] += 2.0
+            else:
+                d[f] = [leaf(f), 2.0]
+        return d, _torch_attention([x[0] for x in d.values()], [x[1] for x in d.values()], heads, dk)
+
+    a_map, a = node_con(v)
+    p_map, outs = {}, []
+    for u in nbrs:
+        fs = [int(f) for f in f_ids[f_indptr[u]:f_indptr[u + 1]]]
+        for f in fs:
+            if f not in p_map:
+                p_map[f] = [leaf(f), 0.0]
+        mult = {f: fs.count(f) for f in fs}
+        outs.append(_torch_attention([p_map[f][0] for f in fs], [float(mult[f]) for f in fs], heads, dk))
+    p = sum(outs) / len(outs)
+    neg = [node_con(u) for u in negs]
+    d1 = gamma + torch.sqrt(((p - a) ** 2).sum())
+    ls = [d1 - torch.sqrt(((p - x) ** 2).sum()) for _, x in neg]
+    pos = [l for l in ls if l.item() > 0]
+    if not pos:
+        return 0.0, {}
+    L = sum(pos) / len(pos)
+    L.backward()
+    grads = {}
+    for d in [a_map, p_map] + [m for m, _ in neg]:
+        for f, (lf, _) in d.items():
+            if f in grads or lf.grad is None:
+                continue
+            g = lf.grad.numpy()
+            if np.all(np.isfinite(g)):
+                grads[f] = g.copy()
+    return L.item(), grads
+
+
+@pytest.mark.parametrize("seed,heads,dk,dm,gamma", [(1, 2, 2, 3, 1.0), (2, 1, 3, 2, 0.5), (3, 3, 2, 2, 1.0),
+                                                     (4, 2, 2, 3, 0.2)])
+def test_attention_first_leaf_matches_per_leaf_autograd(seed, heads, dk, dm, gamma):
+    table, f_indptr, f_ids, v, nbrs, negs = _att_toy(seed, heads=heads, dk=dk, dm=dm)
+    r = O.attention_example_grads(table, f_indptr, f_ids, v, nbrs, negs, gamma, heads, dk, "first_leaf")
+    L, g = torch_attention_first_leaf(table, f_indptr, f_ids, v, nbrs, negs, gamma, heads, dk)
+    assert abs(r.loss - L) < 1e-12
+    assert set(r.grads) == set(g)
+    for f in g:
+        assert np.allclose(r.grads[f], g[f], atol=1e-12)
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_attention_exact_grads_central_fd(seed):
+    heads, dk, dm, gamma = 2, 2, 3, 6.0  # a wide margin keeps every negative active under +-eps
+    table, f_indptr, f_ids, v, nbrs, negs = _att_toy(seed, heads=heads, dk=dk, dm=dm)
+    r = O.attention_example_grads(table, f_indptr, f_ids, v, nbrs, negs, gamma, heads, dk, "exact")
+    assert r.contributes and all(x is not None for x in r.adjoints)
+    eps = 1e-6
+    for f in range(table.shape[0]):
+        for j in range(table.shape[1]):
+            tp, tm = table.copy(), table.copy()
+            tp[f, j] += eps
+            tm[f, j] -= eps
+            lp = O.attention_example_grads(tp, f_indptr, f_ids, v, nbrs, negs, gamma, heads, dk, "exact").loss
+            lm = O.attention_example_grads(tm, f_indptr, f_ids, v, nbrs, negs, gamma, heads, dk, "exact").loss
+            got = r.grads[f][j] if f in r.grads else 0.0
+            assert abs((lp - lm) / (2 * eps) - got) < 1e-7
+
+
+def test_attention_single_feature_node_and_permutation():
+    heads, dk, dm = 2, 2, 3
+    table, f_indptr, f_ids, v, nbrs, negs = _att_toy(7, heads=heads, dk=dk, dm=dm)
+    r = O.attention_example_grads(table, f_indptr, f_ids, v, nbrs, negs, 0.5, heads, dk, "exact")
+    # negative 0 is node 4 with one feature: its construction is that row's head-0 value
+    f4 = int(f_ids[f_indptr[4]])
+    assert np.array_equal(r.vectors[2], table[f4, 2 * heads * dk: 2 * heads * dk + dm])
+    # Full attention over the anchor's distinct features does not depend on their order
+    fs = f_ids[f_indptr[0]:f_indptr[1]][::-1].copy()
+    f2 = f_ids.copy()
+    f2[f_indptr[0]:f_indptr[1]] = fs
+    r2 = O.attention_example_grads(table, f_indptr, f2, v, nbrs, negs, 0.5, heads, dk, "exact")
+    assert np.allclose(r.vectors[0], r2.vectors[0], atol=1e-14)
+
+
+# --------------------------------------------------------------------------------------
 # AUC
 # --------------------------------------------------------------------------------------
 def test_roc_auc_matches_sklearn():
