@@ -41,6 +41,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 (matrix; packed vector math reaches the same)
 PHASES = ["begin+sample+scan", "forward", "emit", "sort", "plan", "short_update", "pieces+medium", "long_rows"]
 
 
@@ -54,18 +55,29 @@ def update_bytes(urows, dims, bytes_per_value, master=False):
     return urows * dims * (2 * bytes_per_value + 16 + (8 if master else 0))
 
 
+def attention_flops(att_items_sq, att_items, heads, dk, dm):
+    """Attention FLOP model over the timed steps: per block of n items and per head,
+    forward 2 n^2 d_k (scores) + n^2 (softmax) + 2 n d_model (weighted values); the
+    backward about twice the forward.  Returns (forward, forward + backward)."""
+    fwd = heads * (2 * dk * att_items_sq + att_items_sq + 2 * dm * att_items)
+    return fwd, 3 * fwd
+
+
 def run_form(w, form, steps, warmup, scatter=None, concurrent=1, graph=True):
     from cloverleaf_amd import _native as N
 
     cfg = N.default_config()
     cfg.dims, cfg.batch_size, cfg.negatives, cfg.gamma, cfg.alpha = w.dims, w.batch, w.negatives, w.gamma, w.alpha
+    if w.att_dk:
+        cfg.attention_dk, cfg.attention_heads = w.att_dk, w.att_heads
     cfg.max_nodes_kind, cfg.max_nodes = N.EP_SAMPLE_FIXED, w.max_nodes
     cfg.valid_pct = 0.0
     cfg.passes = max(1, w.passes)
     cfg.scatter = N.EP_SCATTER_ATOMIC if scatter == "atomic" else N.EP_SCATTER_SORT
     if form.endswith("-plain"):
         cfg.use_graph, cfg.concurrent_batches = 0, 1
-        cfg.seg_vpl, cfg.seg_unroll = 4, 8  # default tile, no tuning
+        # default tile, no tuning: 4 values per lane, or 16 where a row needs more than 64 lanes
+        cfg.seg_vpl, cfg.seg_unroll = (4, 8) if N.row_width(cfg) <= 256 else (16, 2)
     else:
         cfg.use_graph, cfg.concurrent_batches = (1 if graph else 0), concurrent
     prec = form.rsplit("-", 1)[0]
@@ -80,10 +92,17 @@ def run_form(w, form, steps, warmup, scatter=None, concurrent=1, graph=True):
     b = 2 if prec.startswith("bf16") else 4
     S = max(r.steps, 1)
     ph = [float(x) for x in r.ms_phase]
-    g_launch = gather_bytes(r.rows, r.nodes, r.examples, w.dims, b) / S
-    u_launch = update_bytes(r.short_rows, w.dims, b, prec == "bf16-master") / S
-    step_bytes = (gather_bytes(r.rows, r.nodes, r.examples, w.dims, b)
-                  + update_bytes(r.unique_rows, w.dims, b, prec == "bf16-master")) / S
+    W = N.row_width(cfg)  # table row width (attention rows hold Q, K and V of every head)
+    g_launch = gather_bytes(r.rows, r.nodes, r.examples, W, b) / S
+    u_launch = update_bytes(r.short_rows, W, b, prec == "bf16-master") / S
+    step_bytes = (gather_bytes(r.rows, r.nodes, r.examples, W, b)
+                  + update_bytes(r.unique_rows, W, b, prec == "bf16-master")) / S
+    extra = {}
+    if w.att_dk:
+        _, fb = attention_flops(r.att_items_sq, r.att_items, w.att_heads, w.att_dk, w.dims)
+        extra = {"attention_blocks_items_sq_per_step": r.att_items_sq / S,
+                 "attention_flops_per_step": fb / S,
+                 "attention_tflops": fb / S / (ph[1] / 1e3) / 1e12 if ph[1] > 0 else 0.0}
     return {
         "form": form,
         "scatter": scatter or "sort",
@@ -104,6 +123,7 @@ def run_form(w, form, steps, warmup, scatter=None, concurrent=1, graph=True):
         "update_gbs": u_launch / (ph[5] / 1e3) / 1e9 if ph[5] > 0 else 0.0,
         "step_bytes": step_bytes,
         "step_gbs": step_bytes / (r.ms_total / S / 1e3) / 1e9,
+        **extra,
     }
 
 
@@ -159,8 +179,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--scatters", default="sort", help="scatter paths tried for the batched forms (sort,atomic)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--concurrent", type=int, default=8,
-                    help="reference batches per step in the batched forms (the plain forms use 1)")
+    ap.add_argument("--concurrent", type=int, default=0,
+                    help="reference batches per step in the batched forms (the plain forms use 1); default 8, "
+                         "2 for attention workloads (one gradient row per feature occurrence)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the batched forms eagerly (for hardware-counter runs)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -191,6 +212,8 @@ def main():
     print(f"[bench] workload {w.name}: {w.n_nodes} nodes, {len(w.indices)} CSR entries, vocab {w.vocab} "
           f"(generated in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
 
+    if not args.concurrent:
+        args.concurrent = 2 if w.att_dk else 8
     results = {}
     for form in args.forms.split(","):
         best = None
@@ -206,7 +229,9 @@ def main():
                   f"{r['short_rows_per_step']:.0f} short, {r['long_rows_last_step']} long, "
                   f"{r['pieces_last_step']} pieces); "
                   f"phases us: {ph}; fwd {r['forward_gbs']:.0f} GB/s, update {r['update_gbs']:.0f} GB/s, "
-                  f"step {r['step_gbs']:.0f} GB/s", file=sys.stderr, flush=True)
+                  f"step {r['step_gbs']:.0f} GB/s"
+                  + (f", attention {r['attention_tflops']:.1f} TFLOP/s" if w.att_dk else ""),
+                  file=sys.stderr, flush=True)
             if best is None or r["examples_per_s"] > best["examples_per_s"]:
                 best = r
         if best is not None:
@@ -214,8 +239,19 @@ def main():
 
     head_form = "bf16-batched" if "bf16-batched" in results else list(results)[0]
     head = results[head_form]
-    cpu = None if args.no_cpu else cpu_baseline(w, args.cpu_budget)
+    # the CPU restatement covers the mean model
+    cpu = None if (args.no_cpu or w.att_dk) else cpu_baseline(w, args.cpu_budget)
     ph = head["phase_ms"]
+    roof = None
+    if w.att_dk and ph["forward"] >= ph["short_update"]:
+        # the attention forward + backward (scores, softmax, weighted values, their
+        # adjoints) on the FP32 vector ALUs, against the FP32 dense peak
+        tf = head["attention_tflops"]
+        roof = {"bound": "mfma", "kernel": "k_att_forward + k_att_loss + k_att_backward (attention blocks: "
+                                           "scores, softmax, values and their adjoints, FP32)",
+                "achieved": round(tf, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(tf / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "algorithmic_flops_per_launch": round(head["attention_flops_per_step"])}
     if ph["short_update"] >= ph["forward"]:
         kname, kdesc, kbytes, kgbs = ("k_seg_update", "short-row update: reduce runs of adjoint rows + Adam "
                                       "(theta/m/v read and written once per row)",
@@ -241,11 +277,12 @@ def main():
             "description": w.description,
             "nodes": w.n_nodes, "csr_entries": int(len(w.indices)), "vocab": w.vocab,
             "dims": w.dims, "batch": w.batch, "negatives": w.negatives, "max_nodes": w.max_nodes,
+            **({"attention_heads": w.att_heads, "attention_dk": w.att_dk} if w.att_dk else {}),
             "form": head_form, "concurrent_batches": head["concurrent_batches"], "accumulate": "fp32",
             "forms": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                       for k, v in results.items()},
         },
-        "roofline": {
+        "roofline": roof or {
             "bound": "hbm",
             "kernel": f"{kname} ({kdesc})",
             "achieved": round(kgbs, 1),

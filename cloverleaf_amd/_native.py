@@ -53,7 +53,17 @@ class EPConfig(Structure):
         ("seg_vpl", c_int32),
         ("seg_unroll", c_int32),
         ("concurrent_batches", c_uint32),
+        ("attention_dk", c_uint32),
+        ("attention_heads", c_uint32),
+        ("context_window", c_uint32),
     ]
+
+
+def row_width(cfg: "EPConfig") -> int:
+    """Table row width: d_model (mean model) or H (2 d_k + d_model) (attention)."""
+    if cfg.attention_dk:
+        return int(cfg.attention_heads) * (2 * int(cfg.attention_dk) + int(cfg.dims))
+    return int(cfg.dims)
 
 
 class EPBenchResult(Structure):
@@ -73,6 +83,8 @@ class EPBenchResult(Structure):
         ("short_rows", c_uint64),
         ("last_long_rows", c_uint32),
         ("last_pieces", c_uint32),
+        ("att_items_sq", c_uint64),
+        ("att_items", c_uint64),
     ]
 
 
@@ -208,21 +220,21 @@ class Context:
             check(load().ep_set_table(self.handle, None), "ep_set_table")
             return
         rows = np.ascontiguousarray(rows, dtype=np.float32)
-        assert rows.shape == (self.n_vocab, self.cfg.dims), rows.shape
+        assert rows.shape == (self.n_vocab, row_width(self.cfg)), rows.shape
         check(load().ep_set_table(self.handle, ptr(rows, c_float)), "ep_set_table")
 
     def get_table(self) -> np.ndarray:
-        out = np.empty((self.n_vocab, self.cfg.dims), dtype=np.float32)
+        out = np.empty((self.n_vocab, row_width(self.cfg)), dtype=np.float32)
         check(load().ep_get_table(self.handle, ptr(out, c_float)), "ep_get_table")
         return out
 
     def get_table_bf16(self) -> np.ndarray:
-        out = np.empty((self.n_vocab, self.cfg.dims), dtype=np.uint16)
+        out = np.empty((self.n_vocab, row_width(self.cfg)), dtype=np.uint16)
         check(load().ep_get_table_bf16(self.handle, ptr(out, c_uint16)), "ep_get_table_bf16")
         return out
 
     def get_adam(self):
-        m = np.empty((self.n_vocab, self.cfg.dims), dtype=np.float32)
+        m = np.empty((self.n_vocab, row_width(self.cfg)), dtype=np.float32)
         v = np.empty_like(m)
         check(load().ep_get_adam(self.handle, ptr(m, c_float), ptr(v, c_float)), "ep_get_adam")
         return m, v
@@ -259,7 +271,7 @@ class Context:
         """One step on explicit samples.  Returns dict(loss, cons, feat, grad)."""
         anchors = np.ascontiguousarray(anchors, dtype=np.uint32)
         n = len(anchors)
-        S, K, D = self.cfg.max_nodes, self.cfg.negatives, self.cfg.dims
+        S, K, D, W = self.cfg.max_nodes, self.cfg.negatives, self.cfg.dims, row_width(self.cfg)
         nbr_cnt = np.ascontiguousarray(nbr_cnt, dtype=np.uint32)
         nbr = np.ascontiguousarray(nbr, dtype=np.uint32).reshape(n, S)
         neg = np.ascontiguousarray(neg, dtype=np.uint32).reshape(n, K)
@@ -268,7 +280,7 @@ class Context:
         nu = c_uint32(0)
         cap = self.n_vocab * max(1, int(self.cfg.concurrent_batches))  # one record per (row, batch)
         feat = np.zeros(cap, dtype=np.uint32)
-        grad = np.zeros((cap, D), dtype=np.float32)
+        grad = np.zeros((cap, W), dtype=np.float32)
         check(load().ep_step_explicit(self.handle, ptr(anchors, c_uint32), n, ptr(nbr_cnt, c_uint32),
                                       ptr(nbr, c_uint32), ptr(neg, c_uint32), int(pass_), float(lr),
                                       1 if apply_update else 0, ptr(loss, c_float), ptr(cons, c_float),
@@ -287,7 +299,7 @@ class Context:
         f_indptr = np.ascontiguousarray(f_indptr, dtype=np.uint64)
         f_ids = np.ascontiguousarray(f_ids, dtype=np.uint32)
         n = len(f_indptr) - 1
-        out = np.empty((n, self.cfg.dims), dtype=np.float32)
+        out = np.empty((n, row_width(self.cfg)), dtype=np.float32)
         check(load().ep_embed_averaged(self.handle, n, ptr(f_indptr, c_uint64), ptr(f_ids, c_uint32),
                                        ptr(out, c_float)), "ep_embed_averaged")
         return out

@@ -452,6 +452,9 @@ class EmbeddingPropagator:
     ``use_graph`` (replay steps from a captured hipGraph), ``concurrent_batches`` (1, 2, 4
     or 8 batches per step, reading the same parameters and updating in batch order, like
     the reference's concurrently running batches; 1 = strictly sequential), ``device``.
+    ``attention=d_k`` with ``attention_heads`` trains the attention aggregator (Full
+    attention, rows of attention_heads * (2 d_k + dims) values, at most 64 features per
+    node); ``context_window`` (Sliding attention) is not supported.
     """
 
     def __init__(self, alpha=None, loss=None, batch_size=None, dims=None, passes=None, seed=None, max_nodes=None,
@@ -464,8 +467,8 @@ class EmbeddingPropagator:
         mn_kind, mn = _parse_sample(max_nodes, "max_nodes")
         if loss.kind != "margin":
             raise ValueError(f"loss {loss.kind} is not supported by this build (margin only)")
-        if attention is not None:
-            raise ValueError("the attention aggregator is not supported by this build")
+        if attention is not None and context_window is not None:
+            raise ValueError("context_window (Sliding attention) is not supported by this build")
         if mf_kind != N.EP_SAMPLE_ALL:
             raise ValueError("max_features sampling is not supported by this build")
         if weighted_neighbor_sampling or weighted_neighbor_averaging:
@@ -494,6 +497,10 @@ class EmbeddingPropagator:
         cfg.concurrent_batches = int(concurrent_batches)
         cfg.noise = 0.0 if noise is None else float(noise)
         cfg.device = int(device)
+        if attention is not None:
+            # lib.rs:1288-1299: AttentionFeatureModel with Full attention (max_features All)
+            cfg.attention_dk = int(attention)
+            cfg.attention_heads = 1 if attention_heads is None else int(attention_heads)
         self.cfg = cfg
         self.indicator = True if indicator is None else bool(indicator)
         self.history: List[dict] = []
@@ -515,8 +522,9 @@ class EmbeddingPropagator:
         if feature_embeddings is not None:
             warm = feature_embeddings.embeddings
             feature_embeddings.embeddings = np.zeros((len(feature_embeddings.vocab_), 0), dtype=np.float32)
-            if warm.shape != (n_vocab, self.cfg.dims):
-                raise ValueError(f"warm-start table has shape {warm.shape}, expected {(n_vocab, self.cfg.dims)}")
+            width = N.row_width(self.cfg)
+            if warm.shape != (n_vocab, width):
+                raise ValueError(f"warm-start table has shape {warm.shape}, expected {(n_vocab, width)}")
         with N.Context(self.cfg) as ctx:
             ctx.set_graph(graph.indptr, graph.indices)
             ctx.set_features(f_indptr, f_ids, n_vocab)

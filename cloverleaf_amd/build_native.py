@@ -20,7 +20,7 @@ CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 ROOT = PKG.parent
 
-HIP_SOURCES = ["ep_kernels.hip", "ep_capi.hip"]
+HIP_SOURCES = ["ep_kernels.hip", "ep_attention.hip", "ep_capi.hip"]
 SYNTH_SOURCES = ["synth.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -146,6 +146,12 @@ struct ep_ctx {
 
 namespace {
 
+// Table row width: d_model for the mean model, H (2 d_k + d_model) for attention
+// (model.rs:241-243).
+uint32_t row_width(const ep_config& cfg) {
+  return cfg.attention_dk ? cfg.attention_heads * (2u * cfg.attention_dk + cfg.dims) : cfg.dims;
+}
+
 uint32_t neighbour_cap(const ep_ctx* c) {
   return c->cfg.max_nodes_kind == EP_SAMPLE_FIXED ? c->cfg.max_nodes : c->deg_max;
 }
@@ -163,7 +169,8 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   if (batch <= c->cap_batch && c->base.nbr) return EP_OK;
   if (!c->d_indptr || !c->d_findptr) return fail(EP_ESTATE, "graph and features must be set first");
   const ep_config& cfg = c->cfg;
-  const uint32_t K = cfg.negatives, NC = K + 2, D = cfg.dims;
+  const uint32_t K = cfg.negatives, NC = K + 2, D = row_width(cfg);
+  const bool att = cfg.attention_dk != 0;
   const uint32_t S = cfg.max_nodes_kind == EP_SAMPLE_FIXED ? cfg.max_nodes : 0;
   // key capacity: sum of the largest `batch` example sizes
   uint64_t keycap;
@@ -179,6 +186,11 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   }
   if (keycap >= (1ull << 31)) return fail(EP_ECAPACITY, "batch too large: key capacity exceeds 2^31");
   if ((uint64_t)batch * NC > ep::kSlotMask) return fail(EP_ECAPACITY, "batch too large: slots exceed 2^29");
+  // attention: one gradient row per feature occurrence, slot = key position
+  if (att && keycap > ep::kSlotMask) return fail(EP_ECAPACITY, "batch too large: attention slots exceed 2^29");
+  if (att && c->f_max > 64) return fail(EP_EINVAL, "attention supports at most 64 features per node");
+  const uint64_t blk_cap = att ? (uint64_t)batch * (1ull + neighbour_cap(c) + K) : 0;
+  if (blk_cap >= (1ull << 31)) return fail(EP_ECAPACITY, "batch too large: attention blocks exceed 2^31");
   const uint64_t hash = next_pow2(2 * std::max<uint64_t>(example_rows_max(c), 1));
   if (cfg.grad_mode == EP_GRAD_FIRST_LEAF && hash > 16384)
     return fail(EP_ECAPACITY, "first_leaf dedup needs <= 8192 feature occurrences per example (F_max*(1+S+K))");
@@ -191,6 +203,7 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   c->release(b.ghist); c->release(b.G); c->release(b.ufeat); c->release(b.long_st); c->release(b.long_len);
   c->release(b.long_pbase); c->release(b.long_runs); c->release(b.piece_st); c->release(b.piece_info);
   c->release(b.part); c->release(b.srec); c->release(b.mrec); c->release(b.scount); c->release(b.mcount);
+  c->release(b.blkoff); c->release(b.bitems); c->release(b.bout); c->release(b.badj);
   c->release(c->d_explicit);
   c->release(c->d_xout);
   if (c->step_exec) { (void)hipGraphExecDestroy(c->step_exec); c->step_exec = nullptr; c->step_exec_ready = false; }
@@ -210,7 +223,17 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   EP_TRY(c->alloc(&b.neg, (size_t)batch * K));
   EP_TRY(c->alloc(&b.rcount, batch));
   EP_TRY(c->alloc(&b.keyoff, batch + 1));
-  EP_TRY(c->alloc(&b.adj, (size_t)batch * NC * D));
+  // adjoint rows: mean model one per construction (D wide); attention one gradient row
+  // per feature occurrence
+  EP_TRY(c->alloc(&b.adj, att ? (size_t)keycap * D : (size_t)batch * NC * D));
+  if (att) {
+    const uint32_t dm = cfg.dims;
+    EP_TRY(c->alloc(&b.blkoff, batch + 1));
+    EP_TRY(c->alloc(&b.bitems, blk_cap));
+    EP_TRY(c->alloc(&b.bout, (size_t)blk_cap * dm));
+    EP_TRY(c->alloc(&b.badj, (size_t)blk_cap * dm));
+  }
+  b.blk_cap = (uint32_t)blk_cap;
   EP_TRY(c->alloc(&b.coef, (size_t)batch * NC));
   EP_TRY(c->alloc(&b.elig, (size_t)batch * NC));
   EP_TRY(c->alloc(&b.loss, batch));
@@ -257,7 +280,7 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   b.ent_cap = (uint32_t)std::min<uint64_t>(256, 1ull + neighbour_cap(c) + K);
   b.ids_cap = (uint32_t)std::min<uint64_t>(8192, example_rows_max(c));
   // fuse emission into the forward while the first-leaf hash is small (<= 16 KiB of LDS)
-  b.fuse_emit = (cfg.grad_mode == EP_GRAD_EXACT || hash <= 4096) ? 1 : 0;
+  b.fuse_emit = (!att && (cfg.grad_mode == EP_GRAD_EXACT || hash <= 4096)) ? 1 : 0;
   b.unique_grid = unique_grid;
   c->cap_batch = batch;
   c->cap_S = S;
@@ -306,7 +329,7 @@ void refresh_base(ep_ctx* c) {
   b.noise_decay = 1.0f;
   b.table = c->d_table;
   {
-    const uint32_t D = cfg.dims;
+    const uint32_t D = row_width(cfg);
     const bool ms = cfg.precision == EP_PREC_BF16_MASTER;
     b.ostride = ms ? 3 * D : 2 * D;
     b.master = (ms && c->d_ostate) ? c->d_ostate : nullptr;
@@ -314,7 +337,11 @@ void refresh_base(ep_ctx* c) {
     b.var = c->d_ostate ? c->d_ostate + (ms ? 2 * D : D) : nullptr;
   }
   b.bf16 = cfg.precision != EP_PREC_FP32 ? 1 : 0;
-  b.dims = (int)cfg.dims;
+  b.dims = (int)row_width(cfg);
+  b.model = cfg.attention_dk ? 1 : 0;
+  b.att_heads = (int)cfg.attention_heads;
+  b.att_dk = (int)cfg.attention_dk;
+  b.att_dm = (int)cfg.dims;
   b.K = cfg.negatives;
   b.gamma = cfg.gamma;
   b.lw_kind = cfg.loss_weighting;
@@ -350,7 +377,12 @@ int launch_training_step(ep_ctx* c, const ep::StepConfig& sc0, bool update, floa
   EP_HIP(ep::launch_sample(sc, c->d_state, s));
   EP_HIP(ep::launch_scan(sc, c->d_state, s));
   EP_HIP(mark(1));
-  EP_HIP(ep::launch_forward(sc, c->d_state, s));
+  if (sc.model == 1) {
+    EP_HIP(ep::launch_att_forward(sc, c->d_state, s));
+    EP_HIP(ep::launch_att_backward(sc, c->d_state, s));
+  } else {
+    EP_HIP(ep::launch_forward(sc, c->d_state, s));
+  }
   EP_HIP(mark(2));
   EP_HIP(ep::launch_emit(sc, c->d_state, s));
   EP_HIP(mark(3));
@@ -463,9 +495,9 @@ bool tuning(const ep_ctx* c) { return c->cfg.scatter == EP_SCATTER_SORT && c->se
 // the fastest variant (median scatter time over the scored rounds) is fixed.
 int tune_step(ep_ctx* c, const ep::StepConfig& sc) {
   if (c->tune_cands.empty()) {
-    for (int v : {16, 8, 4})
-      for (int u : {8 / (v / 4), 16 / (v / 4)})
-        if (ep::seg_variant_supported((int)c->cfg.dims, v, u)) c->tune_cands.push_back({v, u});
+    for (int v : {16, 12, 8, 4})
+      for (int u : {v == 12 ? 2 : 8 / (v / 4), v == 12 ? 4 : 16 / (v / 4)})
+        if (ep::seg_variant_supported((int)row_width(c->cfg), v, u)) c->tune_cands.push_back({v, u});
     if (c->tune_cands.size() <= 1) {
       const std::pair<int, int> only = c->tune_cands.empty() ? std::make_pair(4, 8) : c->tune_cands[0];
       c->seg_vpl = only.first;
@@ -525,7 +557,17 @@ int run_step(ep_ctx* c, const ep::StepConfig& sc) {
 }
 
 int validate_config(const ep_config* cfg) {
-  if (!ep::dims_supported((int)cfg->dims)) return fail(EP_EINVAL, "dims must be one of 8,16,32,64,100,128,256");
+  if (cfg->attention_dk == 0) {
+    if (!ep::dims_supported((int)cfg->dims)) return fail(EP_EINVAL, "dims must be one of 8,16,32,64,100,128,256");
+  } else {
+    if (cfg->context_window != 0) return fail(EP_EINVAL, "context_window (Sliding attention) is not supported");
+    if (!ep::att_shape_supported((int)cfg->attention_heads, (int)cfg->attention_dk, (int)cfg->dims))
+      return fail(EP_EINVAL, "attention: d_k must be 4, 8, 16 or 32, heads 1..16, dims a multiple of 4 in [4, 256]");
+    if (!ep::row_dims_supported((int)row_width(*cfg)))
+      return fail(EP_EINVAL, "attention: row width heads*(2*d_k+dims) must be one of 8,16,32,64,100,128,256,384,"
+                             "512,768");
+    if (cfg->scatter != EP_SCATTER_SORT) return fail(EP_EINVAL, "attention needs the sort scatter");
+  }
   if (cfg->batch_size == 0) return fail(EP_EINVAL, "batch_size must be > 0");
   if (cfg->negatives == 0 || cfg->negatives > 29) return fail(EP_EINVAL, "negatives must be in [1, 29]");
   if (cfg->passes == 0) return fail(EP_EINVAL, "passes must be > 0");
@@ -551,9 +593,9 @@ int validate_config(const ep_config* cfg) {
   if (cfg->concurrent_batches > 1 && cfg->scatter != EP_SCATTER_SORT)
     return fail(EP_EINVAL, "concurrent_batches > 1 needs the sort scatter");
   if ((cfg->seg_vpl == 0) != (cfg->seg_unroll == 0) ||
-      (cfg->seg_vpl != 0 && !ep::seg_variant_supported((int)cfg->dims, cfg->seg_vpl, cfg->seg_unroll)))
-    return fail(EP_EINVAL, "seg_vpl/seg_unroll must both be 0 (tuned) or a built variant: vpl in {4,8,16} "
-                           "dividing dims, unroll in {8,16}/(vpl/4)");
+      (cfg->seg_vpl != 0 && !ep::seg_variant_supported((int)row_width(*cfg), cfg->seg_vpl, cfg->seg_unroll)))
+    return fail(EP_EINVAL, "seg_vpl/seg_unroll must both be 0 (tuned) or a built variant: vpl in {4,8,12,16} "
+                           "dividing the row width, unroll in {8,16}/(vpl/4) ({2,4} for vpl 12)");
   return EP_OK;
 }
 
@@ -583,6 +625,7 @@ int ep_config_default(ep_config* cfg) {
   cfg->seed = 20222022ull;
   cfg->device = 0;
   cfg->concurrent_batches = 1;
+  cfg->attention_heads = 1;
   return EP_OK;
 }
 
@@ -690,10 +733,10 @@ int ep_set_table(ep_ctx* c, const float* rows) {
   if (!c) return fail(EP_EINVAL, "NULL ctx");
   if (!c->n_vocab) return fail(EP_ESTATE, "features must be set first");
   EP_HIP(hipSetDevice(c->device));
-  const size_t n = (size_t)c->n_vocab * c->cfg.dims;
+  const size_t n = (size_t)c->n_vocab * row_width(c->cfg);
   const bool bf16 = c->cfg.precision != EP_PREC_FP32;
   const bool master = c->cfg.precision == EP_PREC_BF16_MASTER;
-  const uint32_t D = c->cfg.dims;
+  const uint32_t D = row_width(c->cfg);
   const size_t ostride = master ? 3 * (size_t)D : 2 * (size_t)D;
   if (!c->d_table) {
     if (bf16) {
@@ -733,8 +776,8 @@ int ep_get_table(ep_ctx* c, float* rows) {
   if (!c || !rows) return fail(EP_EINVAL, "NULL argument");
   if (!c->table_ready) return fail(EP_ESTATE, "no table");
   EP_HIP(hipSetDevice(c->device));
-  const size_t n = (size_t)c->n_vocab * c->cfg.dims;
-  const uint32_t D = c->cfg.dims;
+  const size_t n = (size_t)c->n_vocab * row_width(c->cfg);
+  const uint32_t D = row_width(c->cfg);
   EP_HIP(hipStreamSynchronize(c->stream));
   if (c->cfg.precision == EP_PREC_BF16_MASTER) {
     EP_HIP(hipMemcpy2D(rows, D * sizeof(float), c->d_ostate, 3 * D * sizeof(float), D * sizeof(float), c->n_vocab,
@@ -758,7 +801,7 @@ int ep_get_table_bf16(ep_ctx* c, uint16_t* rows) {
   if (!c->table_ready || c->cfg.precision == EP_PREC_FP32) return fail(EP_ESTATE, "no bf16 table");
   EP_HIP(hipSetDevice(c->device));
   EP_HIP(hipStreamSynchronize(c->stream));
-  EP_HIP(hipMemcpy(rows, c->d_table, (size_t)c->n_vocab * c->cfg.dims * sizeof(uint16_t), hipMemcpyDeviceToHost));
+  EP_HIP(hipMemcpy(rows, c->d_table, (size_t)c->n_vocab * row_width(c->cfg) * sizeof(uint16_t), hipMemcpyDeviceToHost));
   return EP_OK;
 }
 
@@ -766,7 +809,7 @@ int ep_get_adam(ep_ctx* c, float* m, float* v) {
   if (!c || !m || !v) return fail(EP_EINVAL, "NULL argument");
   if (!c->table_ready) return fail(EP_ESTATE, "no table");
   EP_HIP(hipSetDevice(c->device));
-  const uint32_t D = c->cfg.dims;
+  const uint32_t D = row_width(c->cfg);
   const bool ms = c->cfg.precision == EP_PREC_BF16_MASTER;
   const size_t ostride = (ms ? 3 : 2) * (size_t)D;
   EP_HIP(hipStreamSynchronize(c->stream));
@@ -846,7 +889,12 @@ int ep_learn(ep_ctx* c, ep_progress_fn progress, void* user) {
       for (uint32_t s = 0; s < vc.steps_per_pass; ++s) {
         EP_HIP(ep::launch_step_begin_lr(vc, c->d_vstate, 0.f, c->stream));
         EP_HIP(ep::launch_sample(vc, c->d_vstate, c->stream));
-        EP_HIP(ep::launch_forward(vc, c->d_vstate, c->stream));
+        if (vc.model == 1) {
+          EP_HIP(ep::launch_scan(vc, c->d_vstate, c->stream));
+          EP_HIP(ep::launch_att_forward(vc, c->d_vstate, c->stream));
+        } else {
+          EP_HIP(ep::launch_forward(vc, c->d_vstate, c->stream));
+        }
       }
       EP_HIP(hipStreamSynchronize(c->stream));
       EP_HIP(hipMemcpy(vloss.data(), c->d_valid_loss, vloss.size() * sizeof(float), hipMemcpyDeviceToHost));
@@ -899,7 +947,7 @@ int ep_step_explicit(ep_ctx* c, const uint32_t* anchors, uint32_t n, const uint3
   if (c->cfg.max_nodes_kind != EP_SAMPLE_FIXED) return fail(EP_EINVAL, "ep_step_explicit needs max_nodes Fixed");
   if (!(lr >= 0.f)) return fail(EP_EINVAL, "lr must be >= 0");
   if (!c->d_indptr || !c->d_findptr) return fail(EP_ESTATE, "graph and features must be set first");
-  const uint32_t S = c->cfg.max_nodes, K = c->cfg.negatives, D = c->cfg.dims, NC = K + 2;
+  const uint32_t S = c->cfg.max_nodes, K = c->cfg.negatives, D = row_width(c->cfg), NC = K + 2;
   for (uint32_t i = 0; i < n; ++i) {
     if (anchors[i] >= c->n_nodes) return fail(EP_EINVAL, "anchor out of range");
     if (nbr_cnt[i] > S) return fail(EP_EINVAL, "nbr_cnt exceeds max_nodes");
@@ -940,7 +988,8 @@ int ep_step_explicit(ep_ctx* c, const uint32_t* anchors, uint32_t n, const uint3
   ep::StepState hs;
   EP_HIP(hipMemcpy(&hs, c->d_state, offsetof(ep::StepState, dtot), hipMemcpyDeviceToHost));
   if (loss) EP_HIP(hipMemcpy(loss, sc.loss, n * sizeof(float), hipMemcpyDeviceToHost));
-  if (cons) EP_HIP(hipMemcpy(cons, c->d_xout, (size_t)n * NC * D * sizeof(float), hipMemcpyDeviceToHost));
+  if (cons)
+    EP_HIP(hipMemcpy(cons, c->d_xout, (size_t)n * NC * c->cfg.dims * sizeof(float), hipMemcpyDeviceToHost));
   if (sc.scatter == EP_SCATTER_SORT) {
     if (n_unique) *n_unique = hs.n_unique;
     if (feat) EP_HIP(hipMemcpy(feat, sc.ufeat, hs.n_unique * sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -1038,6 +1087,8 @@ int ep_bench_steps(ep_ctx* c, uint32_t warmup, uint32_t steps, int instrument, e
   out->rows = h1.stat_rows - h0.stat_rows;
   out->nodes = h1.stat_nodes - h0.stat_nodes;
   out->unique_rows = h1.stat_unique - h0.stat_unique;
+  out->att_items_sq = h1.stat_att_n2 - h0.stat_att_n2;
+  out->att_items = h1.stat_att_n - h0.stat_att_n;
   out->seg_vpl = c->seg_vpl;
   out->seg_unroll = c->seg_unroll;
   if (instrument) {
@@ -1089,13 +1140,13 @@ int ep_embed_averaged(ep_ctx* c, uint64_t n_nodes, const uint64_t* f_indptr, con
   float* dout = nullptr;
   EP_TRY(c->alloc(&dp, n_nodes + 1));
   EP_TRY(c->alloc(&di, tot));
-  EP_TRY(c->alloc(&dout, (size_t)n_nodes * c->cfg.dims));
+  EP_TRY(c->alloc(&dout, (size_t)n_nodes * row_width(c->cfg)));
   EP_HIP(hipMemcpy(dp, f_indptr, (n_nodes + 1) * sizeof(uint64_t), hipMemcpyHostToDevice));
   if (tot) EP_HIP(hipMemcpy(di, f_ids, tot * sizeof(uint32_t), hipMemcpyHostToDevice));
   refresh_base(c);
   EP_HIP(ep::launch_embed_averaged(c->base, dp, di, n_nodes, dout, c->stream));
   EP_HIP(hipStreamSynchronize(c->stream));
-  EP_HIP(hipMemcpy(out, dout, (size_t)n_nodes * c->cfg.dims * sizeof(float), hipMemcpyDeviceToHost));
+  EP_HIP(hipMemcpy(out, dout, (size_t)n_nodes * row_width(c->cfg) * sizeof(float), hipMemcpyDeviceToHost));
   c->release(dp);
   c->release(di);
   c->release(dout);

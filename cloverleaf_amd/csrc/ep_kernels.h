@@ -26,6 +26,8 @@ struct StepState {
   uint64_t stat_unique;    // cumulative unique rows updated
   uint64_t stat_examples;  // cumulative anchors processed
   uint64_t stat_short;     // cumulative rows updated by the short-row kernel
+  uint64_t stat_att_n2;    // attention: cumulative sum over blocks of items^2
+  uint64_t stat_att_n;     // attention: cumulative sum over blocks of items
   uint32_t pass;           // 1-based pass
   uint32_t step_in_pass;   // batch index within the pass (0-based)
   uint32_t batch_n;        // anchors in the current batch
@@ -159,6 +161,18 @@ struct StepConfig {
   int unique_grid;         // grid size for the persistent row kernels
   int seg_vpl;             // k_seg_adam values per lane (4, 8, 16; chosen by the tuner)
   int seg_unroll;          // k_seg_adam adjoint rows in flight per lane group (tuned)
+  // Attention model (model = 1): rows are [Q_1..Q_H | K_1..K_H | V_1..V_H], dims =
+  // att_heads * (2 att_dk + att_dm).  An example's attention blocks (h(v), one per
+  // sampled neighbour, one per negative) start at blkoff[b]; bout/badj hold each
+  // block's output and adjoint (att_dm values).  The gradient rows are written per
+  // feature occurrence: adj[slot * dims] with slot = the occurrence's key position.
+  int model;               // 0 mean, 1 attention
+  int att_heads, att_dk, att_dm;
+  uint32_t* blkoff;        // [batch + 1]
+  uint32_t* bitems;        // [blk_cap] items of each block (written by the forward)
+  float* bout;             // [blk_cap * att_dm]
+  float* badj;             // [blk_cap * att_dm]
+  uint32_t blk_cap;
 };
 
 // Launchers (all asynchronous on `stream`).
@@ -184,6 +198,14 @@ hipError_t launch_embed_averaged(const StepConfig& c, const uint64_t* findptr, c
                                  uint64_t n_nodes, float* out, hipStream_t s);
 
 bool dims_supported(int d);
+// Row widths of the table-row kernels (init, update, averaged embedding): the mean
+// model's widths and the attention row widths.
+bool row_dims_supported(int d);
+// Attention model kernels (ep_attention.hip): forward runs the block outputs and the
+// loss/adjoints; backward writes the per-occurrence gradient rows (train steps).
+bool att_shape_supported(int heads, int dk, int dm);
+hipError_t launch_att_forward(const StepConfig& c, StepState* st, hipStream_t s);
+hipError_t launch_att_backward(const StepConfig& c, StepState* st, hipStream_t s);
 // Segments of the sorted keys longer than kSegShort entries are summed as pieces of
 // kPieceEntries entries by k_pieces and combined in piece order by k_long_combine.
 constexpr uint32_t kSegShort = 32;

@@ -308,38 +308,50 @@ __global__ __launch_bounds__(256) void k_sample(StepConfig c, StepState* st) {
 
 // ------------------------------------------------------------------------------------
 // k_scan: exclusive scan of R_e over the batch -> key offsets; total -> st->n_keys.
+// Attention model: also the scan of the attention blocks per example (1 + cnt + K for
+// an example with neighbours) -> blkoff.
 // ------------------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void k_scan(StepConfig c, StepState* st) {
-  __shared__ uint32_t s_w[16];
-  __shared__ uint32_t s_carry;
+  __shared__ uint32_t s_w[16], s_w2[16];
+  __shared__ uint32_t s_carry, s_carry2;
   __shared__ unsigned long long s_nodes;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  if (tid == 0) { s_carry = 0u; s_nodes = 0ull; }
+  const bool att = c.model == 1;
+  if (tid == 0) { s_carry = 0u; s_carry2 = 0u; s_nodes = 0ull; }
   __syncthreads();
   uint64_t nodes = 0;
   for (uint32_t base = 0; base < c.batch; base += 1024) {
     const uint32_t i = base + tid;
     const uint32_t x = i < c.batch ? c.rcount[i] : 0u;
+    uint32_t y = 0;
     if (i < c.batch && i < st->batch_n) {
       const uint32_t cn = c.nbr_cnt[i];
-      if (cn > 0u) nodes += cn + c.K;
+      if (cn > 0u) {
+        nodes += cn + c.K;
+        y = 1u + cn + c.K;
+      }
     }
-    uint32_t inc = x;
+    uint32_t inc = x, inc2 = y;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-      const uint32_t y = __shfl_up(inc, off, EP_WAVE);
-      if (lane >= off) inc += y;
+      const uint32_t t1 = __shfl_up(inc, off, EP_WAVE);
+      const uint32_t t2 = __shfl_up(inc2, off, EP_WAVE);
+      if (lane >= off) { inc += t1; inc2 += t2; }
     }
-    if (lane == 63) s_w[w] = inc;
+    if (lane == 63) { s_w[w] = inc; s_w2[w] = inc2; }
     __syncthreads();
-    uint32_t pre = s_carry, tot = 0;
+    uint32_t pre = s_carry, tot = 0, pre2 = s_carry2, tot2 = 0;
     for (int q = 0; q < 16; ++q) {
-      if (q < w) pre += s_w[q];
+      if (q < w) { pre += s_w[q]; pre2 += s_w2[q]; }
       tot += s_w[q];
+      tot2 += s_w2[q];
     }
-    if (i < c.batch) c.keyoff[i] = pre + inc - x;
+    if (i < c.batch) {
+      c.keyoff[i] = pre + inc - x;
+      if (att) c.blkoff[i] = pre2 + inc2 - y;
+    }
     __syncthreads();
-    if (tid == 0) s_carry += tot;
+    if (tid == 0) { s_carry += tot; s_carry2 += tot2; }
     __syncthreads();
   }
   atomicAdd(&s_nodes, (unsigned long long)nodes);
@@ -350,6 +362,11 @@ __global__ __launch_bounds__(1024) void k_scan(StepConfig c, StepState* st) {
     st->stat_rows += s_carry;
     st->stat_nodes += s_nodes;
     if (s_carry > c.key_cap) st->err = 1u;  // capacity error (checked by the host)
+    if (att) {
+      // blocks past the capacity are not run (and their keys not emitted)
+      c.blkoff[c.batch] = s_carry2 <= c.blk_cap ? s_carry2 : 0u;
+      if (s_carry2 > c.blk_cap) { st->err = 1u; st->n_keys = c.key_cap + 1u; }
+    }
   }
 }
 
@@ -748,7 +765,9 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
 // k_emit: one block per anchor.  Writes one (feature, slot) key per feature occurrence,
 // in construction order a, p, n_1..n_K, at keyoff[b].  Occurrences of ineligible
 // constructions, and (first_leaf) occurrences outside the feature's first eligible
-// construction (mod.rs:347-352, 366-367), get the sentinel key.
+// construction (mod.rs:347-352, 366-367), get the sentinel key.  Attention model: the
+// slot is the occurrence's key position (its gradient row), and in h(v) / h(u_k) only
+// the first occurrence of a feature carries one.
 // ------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_emit(StepConfig c, StepState* st) {
   const uint32_t b = blockIdx.x;
@@ -767,6 +786,7 @@ __global__ __launch_bounds__(256) void k_emit(StepConfig c, StepState* st) {
   __shared__ uint8_t s_ci[256];
   const uint32_t H = c.hash_entries, hmask = H - 1u;
   const bool first_leaf = c.grad_mode == 0;
+  const bool att = c.model == 1;
   const uint32_t v = c.perm[st->anchor_base + b];
   const uint32_t* pn = c.all_nbrs ? c.indices + c.indptr[v] : c.nbr + (size_t)b * c.S;
   if (tid < (int)NC) s_el[tid] = c.elig[(size_t)b * NC + tid];
@@ -806,16 +826,24 @@ __global__ __launch_bounds__(256) void k_emit(StepConfig c, StepState* st) {
         const uint32_t j = lo;
         const uint32_t nd = s_node[j];
         const uint32_t cj = s_ci[j];
-        const uint32_t f = c.fids[c.findptr[nd] + (o - s_off[j])];
+        const uint64_t nfs = c.findptr[nd];
+        const uint32_t qpos = o - s_off[j];
+        const uint32_t f = c.fids[nfs + qpos];
         const bool ok = s_el[cj] != 0;
         if (phase == 0) {
           if (ok) hash_insert_min(hsh, hmask, f, cj);
         } else {
-          uint32_t key = c.sentinel;
-          if (ok && (!first_leaf || hash_lookup(hsh, hmask, f) == cj)) key = key_of_row(c, f);
+          bool emit = ok && (!first_leaf || hash_lookup(hsh, hmask, f) == cj);
           const uint32_t pos = kbase + running + o;
-          c.keys[0][pos] = key;
-          c.slots[0][pos] = ((b / c.rbatch) << kSlotBatchShift) | (slot0 + cj);
+          if (att && emit && cj != 1u) {
+            // h(v), h(u_k): one attention item (and gradient row) per distinct feature,
+            // at the feature's first position in the node's list
+            for (uint32_t t = 0; t < qpos && emit; ++t) emit = c.fids[nfs + t] != f;
+          }
+          c.keys[0][pos] = emit ? key_of_row(c, f) : c.sentinel;
+          // mean model: the slot of the construction's adjoint; attention: the
+          // occurrence's own gradient row
+          c.slots[0][pos] = ((b / c.rbatch) << kSlotBatchShift) | (att ? pos : slot0 + cj);
         }
       }
       running += total;
@@ -1762,14 +1790,19 @@ __global__ __launch_bounds__(256) void k_init_table(StepConfig c, uint64_t seed)
   const uint64_t nwaves = (uint64_t)gridDim.x * NW;
   for (uint64_t row = gw; row < c.n_vocab; row += nwaves) {
     float x[PL];
-    const u32x4 r0 = philox4x32(u32x4{(uint32_t)row, (uint32_t)(row >> 32), (uint32_t)lane, RNG_INIT},
-                                (uint32_t)seed, (uint32_t)(seed >> 32));
-    const uint32_t rw[4] = {r0.x, r0.y, r0.z, r0.w};
     float sq = 0.f;
 #pragma unroll
-    for (int q = 0; q < PL; ++q) {
-      x[q] = (base + q < D) ? (2.0f * uniform01(rw[q & 3]) - 1.0f) : 0.f;
-      sq += x[q] * x[q];
+    for (int q0 = 0; q0 < PL; q0 += 4) {
+      // one Philox block per 4 values of the lane's slice
+      const u32x4 r0 = philox4x32(u32x4{(uint32_t)row, (uint32_t)(row >> 32), (uint32_t)(lane + 64 * (q0 >> 2)),
+                                        RNG_INIT},
+                                  (uint32_t)seed, (uint32_t)(seed >> 32));
+      const uint32_t rw[4] = {r0.x, r0.y, r0.z, r0.w};
+#pragma unroll
+      for (int q = q0; q < PL && q < q0 + 4; ++q) {
+        x[q] = (base + q < D) ? (2.0f * uniform01(rw[q & 3]) - 1.0f) : 0.f;
+        sq += x[q] * x[q];
+      }
     }
     const float nrm = sqrtf(wave_sum(sq));
 #pragma unroll
@@ -1828,6 +1861,31 @@ bool dims_supported(int d) {
 #undef EP_CASE
   return false;
 }
+
+#define EP_ROW_DIMS(X) EP_DIMS(X) X(384) X(512) X(768)
+
+bool row_dims_supported(int d) {
+#define EP_CASE(DD) if (d == DD) return true;
+  EP_ROW_DIMS(EP_CASE)
+#undef EP_CASE
+  return false;
+}
+
+// Table-row kernels: the mean model's widths and the attention row widths.
+#define EP_ROW_DISPATCH(d, BODY)             \
+  switch (d) {                               \
+    case 8: { constexpr int D = 8; BODY; } break;     \
+    case 16: { constexpr int D = 16; BODY; } break;   \
+    case 32: { constexpr int D = 32; BODY; } break;   \
+    case 64: { constexpr int D = 64; BODY; } break;   \
+    case 100: { constexpr int D = 100; BODY; } break; \
+    case 128: { constexpr int D = 128; BODY; } break; \
+    case 256: { constexpr int D = 256; BODY; } break; \
+    case 384: { constexpr int D = 384; BODY; } break; \
+    case 512: { constexpr int D = 512; BODY; } break; \
+    case 768: { constexpr int D = 768; BODY; } break; \
+    default: return hipErrorInvalidValue;    \
+  }
 
 #define EP_DISPATCH(d, BODY)                 \
   switch (d) {                               \
@@ -1924,14 +1982,15 @@ hipError_t launch_sort(const StepConfig& c, StepState* st, hipStream_t s, int* o
   return hipGetLastError();
 }
 
-// Values-per-lane variants of k_seg_adam available for a row width (VPL | D, D/VPL <= 64).
+// Values-per-lane variants of k_seg_update available for a row width (VPL | D, D/VPL <= 64).
 bool seg_vpl_supported(int d, int vpl) {
-  if (vpl != 4 && vpl != 8 && vpl != 16) return false;
+  if (vpl != 4 && vpl != 8 && vpl != 12 && vpl != 16) return false;
   return d % vpl == 0 && d / vpl <= 64 && (d % 4 == 0);
 }
 
 bool seg_variant_supported(int d, int vpl, int unroll) {
   if (!seg_vpl_supported(d, vpl)) return false;
+  if (vpl == 12) return unroll == 2 || unroll == 4;
   const int nq = vpl / 4;
   return unroll == 8 / nq || unroll == 16 / nq;
 }
@@ -1950,17 +2009,22 @@ static bool try_seg_update(const StepConfig& c, StepState* st, int buf, hipStrea
 template <int D>
 static void launch_seg_update_variant(const StepConfig& c, StepState* st, int buf, hipStream_t s, int grid) {
   if (try_seg_update<D, 16, 2>(c, st, buf, s, grid) || try_seg_update<D, 16, 4>(c, st, buf, s, grid) ||
+      try_seg_update<D, 12, 2>(c, st, buf, s, grid) || try_seg_update<D, 12, 4>(c, st, buf, s, grid) ||
       try_seg_update<D, 8, 4>(c, st, buf, s, grid) || try_seg_update<D, 8, 8>(c, st, buf, s, grid) ||
-      try_seg_update<D, 4, 16>(c, st, buf, s, grid))
+      try_seg_update<D, 4, 16>(c, st, buf, s, grid) || try_seg_update<D, 4, 8>(c, st, buf, s, grid))
     return;
-  hipLaunchKernelGGL((k_seg_update<D, 4, 8>), dim3(grid), dim3(256), 0, s, c, st, c.slots[buf]);
+  // a width no variant divides: the widest built form
+  if constexpr (D % 16 == 0 && D / 16 <= 64)
+    hipLaunchKernelGGL((k_seg_update<D, 16, 2>), dim3(grid), dim3(256), 0, s, c, st, c.slots[buf]);
+  else
+    hipLaunchKernelGGL((k_seg_update<D, 4, 8>), dim3(grid), dim3(256), 0, s, c, st, c.slots[buf]);
 }
 
 hipError_t launch_scatter_sorted(const StepConfig& c, StepState* st, int buf, hipStream_t s, hipEvent_t* ev) {
   const int grid = c.unique_grid;
   hipLaunchKernelGGL(k_seg_plan, dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
   if (ev) (void)hipEventRecord(ev[0], s);
-  EP_DISPATCH(c.dims, {
+  EP_ROW_DISPATCH(c.dims, {
     launch_seg_update_variant<D>(c, st, buf, s, grid);
     if (ev) (void)hipEventRecord(ev[1], s);
     hipLaunchKernelGGL((k_pieces<D>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
@@ -1982,7 +2046,7 @@ hipError_t launch_adam_touched(const StepConfig& c, StepState* st, hipStream_t s
 }
 
 hipError_t launch_init_table(const StepConfig& c, uint64_t seed, hipStream_t s) {
-  EP_DISPATCH(c.dims, { hipLaunchKernelGGL((k_init_table<D>), dim3(2048), dim3(256), 0, s, c, seed); });
+  EP_ROW_DISPATCH(c.dims, { hipLaunchKernelGGL((k_init_table<D>), dim3(2048), dim3(256), 0, s, c, seed); });
   return hipGetLastError();
 }
 
@@ -1993,7 +2057,7 @@ hipError_t launch_fill_bf16(const float* src, uint16_t* dst, uint64_t n, uint32_
 
 hipError_t launch_embed_averaged(const StepConfig& c, const uint64_t* findptr, const uint32_t* fids,
                                  uint64_t n_nodes, float* out, hipStream_t s) {
-  EP_DISPATCH(c.dims, {
+  EP_ROW_DISPATCH(c.dims, {
     if (c.bf16)
       hipLaunchKernelGGL((k_embed_avg<D, uint16_t>), dim3(2048), dim3(256), 0, s, (const uint16_t*)c.table, findptr,
                          fids, n_nodes, out);

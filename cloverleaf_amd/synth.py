@@ -115,6 +115,8 @@ class Workload:
     gamma: float
     alpha: float
     passes: int
+    att_heads: int = 0   # attention model (0: mean aggregation)
+    att_dk: int = 0
 
     @property
     def n_nodes(self) -> int:
@@ -150,6 +152,12 @@ WORKLOADS = {
         graph=("chung_lu", dict(n=10_000_000, m=200_000_000, beta=2.1, dmax=100_000, directed=True)),
         feats=dict(count_kind=COUNT_LOGUNIFORM, p1=1.0, p2=128.0, vocab=5_000_000, zipf_s=1.05),
         dims=256, batch=8192, negatives=20, max_nodes=32, gamma=0.5, alpha=0.01, passes=1),
+    "ep-cfg5-5m": dict(
+        description="Chung-Lu undirected N=5M, 100M edges, beta=2.3; features U{1..64}, Zipf(1.1) over 2M; "
+                    "attention H=4, d_k=32, d_model=128 (rows of 768), B=4096, K=10, S=20, margin 0.5, lr 0.01",
+        graph=("chung_lu", dict(n=5_000_000, m=100_000_000, beta=2.3, dmax=20_000, directed=False)),
+        feats=dict(count_kind=COUNT_UNIFORM, p1=1.0, p2=64.0, vocab=2_000_000, zipf_s=1.1),
+        dims=128, batch=4096, negatives=10, max_nodes=20, gamma=0.5, alpha=0.01, passes=1, att_heads=4, att_dk=32),
 }
 
 
@@ -180,4 +188,5 @@ def make_workload(name: str, scale: float = 1.0, seed_graph: int = 1, seed_feat:
     f_indptr, f_ids = features(n, f["count_kind"], f["p1"], f["p2"], vocab, f["zipf_s"], seed_feat, comm)
     return Workload(name=name, description=spec["description"], indptr=indptr, indices=indices, f_indptr=f_indptr,
                     f_ids=f_ids, vocab=vocab, dims=spec["dims"], batch=spec["batch"], negatives=spec["negatives"],
-                    max_nodes=spec["max_nodes"], gamma=spec["gamma"], alpha=spec["alpha"], passes=spec["passes"])
+                    max_nodes=spec["max_nodes"], gamma=spec["gamma"], alpha=spec["alpha"], passes=spec["passes"],
+                    att_heads=spec.get("att_heads", 0), att_dk=spec.get("att_dk", 0))

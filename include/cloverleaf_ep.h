@@ -52,7 +52,10 @@ enum { EP_SCATTER_SORT = 0, EP_SCATTER_ATOMIC = 1 };
 /* Mirrors EmbeddingPropagation (src/algos/ep/mod.rs:40-75) plus the model options
  * of EmbeddingPropagator::new (src/lib.rs:1205-1307).                           */
 typedef struct ep_config {
-  uint32_t dims;               /* d_model; 8,16,32,64,100,128,256                  */
+  uint32_t dims;               /* d_model; mean model: 8,16,32,64,100,128,256;      */
+                               /* attention: 4..256, a multiple of 4, with a table  */
+                               /* row width heads*(2 d_k + dims) of 8..256, 384,    */
+                               /* 512 or 768                                        */
   uint32_t batch_size;         /* default 50                                        */
   uint32_t passes;             /* default 100                                       */
   uint32_t negatives;          /* MarginLoss negatives, default 1                   */
@@ -83,6 +86,14 @@ typedef struct ep_config {
                                /* parameters and update in batch order, as the      */
                                /* reference's concurrently running batches          */
                                /* (mod.rs:174); 1 = strictly sequential (default)   */
+  uint32_t attention_dk;       /* 0: mean aggregation (AveragedFeatureModel);        */
+                               /* d_k > 0: AttentionFeatureModel with Full attention  */
+                               /* (lib.rs:1288-1299, `attention`); d_k in 4,8,16,32, */
+                               /* at most 64 features per node                       */
+  uint32_t attention_heads;    /* `attention_heads` (default 1); table rows are      */
+                               /* heads * (2 d_k + dims) wide (model.rs:241-243)     */
+  uint32_t context_window;     /* `context_window`: must be 0 (Sliding attention is  */
+                               /* not built)                                         */
 } ep_config;
 
 typedef struct ep_ctx ep_ctx;
@@ -112,6 +123,8 @@ typedef struct ep_bench_result {
   uint64_t short_rows;         /* rows updated by the short-row kernel (timed run)  */
   uint32_t last_long_rows;     /* long rows and their pieces in the last step        */
   uint32_t last_pieces;
+  uint64_t att_items_sq;       /* attention model, timed steps: sum over attention    */
+  uint64_t att_items;          /* blocks of items^2 and of items (FLOP model inputs)  */
 } ep_bench_result;
 
 /* Fills the reference defaults (EmbeddingPropagator::new, src/lib.rs:1262-1274). */
@@ -132,16 +145,19 @@ int ep_set_graph(ep_ctx* ctx, uint64_t n_nodes, const uint64_t* indptr, const ui
 int ep_set_features(ep_ctx* ctx, uint64_t n_nodes, uint32_t n_vocab, const uint64_t* f_indptr,
                     const uint32_t* f_ids);
 
-/* Sets the feature table (n_vocab x dims, row-major FP32); rows == NULL draws the
+/* Table rows are `dims` wide for the mean model and attention_heads * (2
+ * attention_dk + dims) wide for the attention model ("row width" below).        */
+
+/* Sets the feature table (n_vocab x row width, row-major FP32); rows == NULL draws the
  * random init of randomize_embedding_store (src/algos/ep/mod.rs:381-392) on the
  * device.  Resets the Adam state (optimizer.rs:34-38).                          */
 int ep_set_table(ep_ctx* ctx, const float* rows);
 
-/* Copies the FP32 table (FP32 master rows in BF16 mode) to rows[n_vocab*dims].  */
+/* Copies the FP32 table (FP32 master rows in BF16 mode) to rows[n_vocab*width]. */
 int ep_get_table(ep_ctx* ctx, float* rows);
-/* Copies the BF16 working table (BF16 mode only) to rows[n_vocab*dims].         */
+/* Copies the BF16 working table (BF16 mode only) to rows[n_vocab*width].        */
 int ep_get_table_bf16(ep_ctx* ctx, uint16_t* rows);
-/* Copies the Adam moments (optimizer.rs:29-30) to m, v [n_vocab*dims].          */
+/* Copies the Adam moments (optimizer.rs:29-30) to m, v [n_vocab*width].         */
 int ep_get_adam(ep_ctx* ctx, float* m, float* v);
 
 /* Sets the train / validation node ids.  ep_learn draws its own split
@@ -165,7 +181,8 @@ int ep_sample_batch(ep_ctx* ctx, const uint32_t* anchors, uint32_t n, uint32_t p
  *   loss[n]                per-example weighted loss (0 when no gradient)
  *   cons[n*(K+2)*dims]     constructions h(v), ~h(v), h(u_1..K)
  *   n_unique, feat[], grad[]  merged batch gradient (mod.rs:223-233); feat and
- *                          grad must hold up to n_vocab entries / rows
+ *                          grad must hold up to n_vocab entries / rows of the
+ *                          row width
  * apply_update != 0 runs the Adam update (optimizer.rs:43-81).                  */
 int ep_step_explicit(ep_ctx* ctx, const uint32_t* anchors, uint32_t n, const uint32_t* nbr_cnt,
                      const uint32_t* nbr, const uint32_t* neg, uint32_t pass, float lr, int apply_update,
@@ -178,7 +195,7 @@ int ep_step_explicit(ep_ctx* ctx, const uint32_t* anchors, uint32_t n, const uin
 int ep_bench_steps(ep_ctx* ctx, uint32_t warmup, uint32_t steps, int instrument, ep_bench_result* out);
 
 /* NodeEmbedder(FeatureAggregator.Averaged).embed_feature_set (src/lib.rs:2092-2124,
- * aggregator.rs:34-47): out[n_nodes*dims] = mean of each node's feature rows.   */
+ * aggregator.rs:34-47): out[n_nodes*width] = mean of each node's feature rows.  */
 int ep_embed_averaged(ep_ctx* ctx, uint64_t n_nodes, const uint64_t* f_indptr, const uint32_t* f_ids, float* out);
 
 /* Philox4x32-10 on the host (the device RNG), for known-answer tests.           */

@@ -49,8 +49,9 @@ def make_problem(seed=0, n=120, V=300, D=32, avg_deg=6, fmax=8, zipf=1.1, zero_d
 
 
 def oracle_step(prob: Problem, anchors, cnt, nbr, neg, gamma, lr, pass_, grad_mode, table=None, mom=None, var=None,
-                loss_divs=None):
-    """FP64 oracle of one explicit-sample step; returns (results, merged, table, mom, var)."""
+                loss_divs=None, attention=None):
+    """FP64 oracle of one explicit-sample step; returns (results, merged, table, mom, var).
+    attention = (heads, d_k) selects the attention model."""
     t = (prob.table if table is None else table).astype(np.float64).copy()
     m = np.zeros_like(t) if mom is None else mom.astype(np.float64).copy()
     v = np.zeros_like(t) if var is None else var.astype(np.float64).copy()
@@ -62,13 +63,14 @@ def oracle_step(prob: Problem, anchors, cnt, nbr, neg, gamma, lr, pass_, grad_mo
     res_k, merged = O.ep_batch_step(t, m, v, prob.f_indptr, prob.f_ids, [int(anchors[i]) for i in keep],
                                     [nbr_lists[i] for i in keep], [1.0] * len(keep),
                                     [list(map(int, neg[i])) for i in keep], gamma, lr, pass_, grad_mode,
-                                    None if loss_divs is None else [loss_divs[i] for i in keep])
+                                    None if loss_divs is None else [loss_divs[i] for i in keep], attention)
     it = iter(res_k)
     results = [next(it) if i in keep else None for i in range(B)]
     return results, merged, t, m, v
 
 
-def oracle_group_step(prob: Problem, anchors, cnt, nbr, neg, gamma, lr, pass_, grad_mode, batch_size):
+def oracle_group_step(prob: Problem, anchors, cnt, nbr, neg, gamma, lr, pass_, grad_mode, batch_size,
+                      attention=None):
     """FP64 oracle of one explicit-sample step of consecutive batches of `batch_size`
     anchors run as one concurrent group (every batch reads the group-start table, updates
     in batch order, all with learning rate lr).  Returns (per-batch merged, table, mom, var)."""
@@ -81,7 +83,7 @@ def oracle_group_step(prob: Problem, anchors, cnt, nbr, neg, gamma, lr, pass_, g
     _, merged = O.ep_group_step(t, m, v, prob.f_indptr, prob.f_ids, [int(anchors[i]) for i in keep],
                                 [list(map(int, nbr[i, :cnt[i]])) for i in keep], [1.0] * len(keep),
                                 [list(map(int, neg[i])) for i in keep], gamma, [lr] * nb,
-                                [i // batch_size for i in keep], pass_, grad_mode)
+                                [i // batch_size for i in keep], pass_, grad_mode, None, attention)
     return merged, t, m, v
 
 

@@ -1,0 +1,220 @@
+"""GPU parity of the attention-model EP step (AttentionFeatureModel, model.rs:158-249;
+attention_mean, attention.rs:100-131) through the C ABI, against the FP64 oracle
+(oracle.ep_oracle.attention_example_grads).
+
+Samples are drawn by the GPU sampler and the same lists are fed to the oracle.  Bars
+(normwise relative, FP32 path): loss <= 1e-5, constructions <= 1e-5, merged gradient
+<= 1e-5; Adam m <= 1e-5, v <= 1e-4, theta within 1e-5 where the gradient is above FP32
+noise.  The problems repeat features inside nodes (h(v) merges them with count
+2 x multiplicity, a neighbour's attention keeps each occurrence) and include
+single-feature nodes (V of head 0).
+"""
+import numpy as np
+import pytest
+
+from cloverleaf_amd import _native as N
+from tests.problems import Problem, dense_grad, make_problem, oracle_group_step, oracle_step
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def att_problem(seed, heads, dk, dm, n=140, V=120, fmax=10, zipf=1.2, avg_deg=8):
+    W = heads * (2 * dk + dm)
+    p = make_problem(seed=seed, n=n, V=V, D=W, fmax=fmax, zipf=zipf, avg_deg=avg_deg, table_scale=0.6)
+    return p
+
+
+def att_ctx(prob, heads, dk, dm, S=5, K=4, gamma=0.5, B=32, grad_mode=N.EP_GRAD_FIRST_LEAF, precision=N.EP_PREC_FP32,
+            concurrent=1, seg_tile=(0, 0), use_graph=0):
+    cfg = N.default_config()
+    cfg.dims, cfg.batch_size, cfg.negatives, cfg.gamma = dm, B, K, gamma
+    cfg.attention_dk, cfg.attention_heads = dk, heads
+    cfg.max_nodes_kind, cfg.max_nodes = N.EP_SAMPLE_FIXED, S
+    cfg.grad_mode, cfg.precision = grad_mode, precision
+    cfg.concurrent_batches = concurrent
+    cfg.seg_vpl, cfg.seg_unroll = seg_tile
+    cfg.use_graph = use_graph
+    cfg.valid_pct = 0.0
+    ctx = N.Context(cfg)
+    ctx.set_graph(prob.indptr, prob.indices)
+    ctx.set_features(prob.f_indptr, prob.f_ids, prob.V)
+    ctx.set_table(prob.table)
+    return ctx
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+def has_repeats(prob):
+    for i in range(prob.n):
+        fs = prob.f_ids[prob.f_indptr[i]:prob.f_indptr[i + 1]]
+        if len(set(fs.tolist())) < len(fs):
+            return True
+    return False
+
+
+SHAPES = [(1, 8, 16), (2, 4, 8), (2, 8, 16), (4, 8, 16), (2, 16, 32), (2, 32, 64), (4, 32, 128)]
+
+
+@pytest.mark.parametrize("heads,dk,dm", SHAPES)
+@pytest.mark.parametrize("grad_mode", [N.EP_GRAD_FIRST_LEAF, N.EP_GRAD_EXACT])
+def test_attention_step_matches_oracle(heads, dk, dm, grad_mode):
+    prob = att_problem(seed=heads * 100 + dk + dm, heads=heads, dk=dk, dm=dm)
+    assert has_repeats(prob)
+    S, K, gamma = 5, 4, 0.5
+    print(f"[attention] H={heads} dk={dk} dm={dm} W={prob.D} mode={grad_mode}", flush=True)
+    ctx = att_ctx(prob, heads, dk, dm, S, K, gamma, B=48, grad_mode=grad_mode)
+    rng = np.random.default_rng(heads + dk)
+    anchors = rng.choice(prob.n, size=48, replace=False).astype(np.uint32)
+    cnt, nbr, neg = ctx.sample_batch(anchors, 1)
+    out = ctx.step_explicit(anchors, cnt, nbr, neg, pass_=1, lr=0.01, apply_update=True, want_cons=True)
+    mode = "first_leaf" if grad_mode == N.EP_GRAD_FIRST_LEAF else "exact"
+    res, merged, t64, m64, v64 = oracle_step(prob, anchors, cnt, nbr, neg, gamma, 0.01, 1, mode,
+                                             attention=(heads, dk))
+    ref_loss = np.array([r.loss if (r is not None and r.contributes) else 0.0 for r in res])
+    assert np.count_nonzero(ref_loss) > 10
+    assert rel(out["loss"], ref_loss) <= TOL
+    for i, r in enumerate(res):
+        if r is not None:
+            assert rel(out["cons"][i], np.stack(r.vectors)) <= TOL
+    G64 = dense_grad(merged, prob.V, prob.D)
+    Gg = np.zeros_like(G64)
+    assert len(set(out["feat"].tolist())) == len(out["feat"])
+    assert set(out["feat"].tolist()) == set(merged.keys())
+    Gg[out["feat"]] = out["grad"]
+    err = rel(Gg, G64)
+    print(f"[attention] loss ok, grad rel err {err:.2e} over {len(merged)} rows", flush=True)
+    assert err <= TOL
+    new = ctx.get_table()
+    m, v = ctx.get_adam()
+    rows = sorted(merged)
+    assert rel(m[rows], m64[rows]) <= 1e-5
+    assert rel(v[rows], v64[rows]) <= 1e-4
+    mask = np.abs(G64[rows]) > 1e-3 * np.abs(G64).max()
+    assert np.max(np.abs(new[rows] - t64[rows])[mask]) < 1e-5
+    untouched = np.setdiff1d(np.arange(prob.V), rows)
+    assert np.array_equal(new[untouched], prob.table[untouched])
+    ctx.close()
+
+
+def test_attention_single_feature_nodes_and_hubs():
+    # many single-feature nodes (V of head 0), hub rows with long sorted runs (pieces path)
+    heads, dk, dm = 2, 8, 16
+    prob = make_problem(seed=5, n=300, V=12, D=heads * (2 * dk + dm), fmax=3, zipf=1.5, avg_deg=30,
+                        table_scale=0.6)
+    ctx = att_ctx(prob, heads, dk, dm, S=8, K=6, gamma=1.0, B=300)
+    anchors = np.arange(300, dtype=np.uint32)
+    cnt, nbr, neg = ctx.sample_batch(anchors, 1)
+    out = ctx.step_explicit(anchors, cnt, nbr, neg, 1, 0.01, apply_update=False, want_cons=True)
+    res, merged, *_ = oracle_step(prob, anchors, cnt, nbr, neg, 1.0, 0.01, 1, "first_leaf", attention=(heads, dk))
+    ref_loss = np.array([r.loss if (r is not None and r.contributes) else 0.0 for r in res])
+    assert rel(out["loss"], ref_loss) <= TOL
+    G64 = dense_grad(merged, prob.V, prob.D)
+    Gg = np.zeros_like(G64)
+    Gg[out["feat"]] = out["grad"]
+    assert rel(Gg, G64) <= TOL
+    ctx.close()
+
+
+@pytest.mark.parametrize("precision", [N.EP_PREC_BF16, N.EP_PREC_BF16_MASTER])
+def test_attention_bf16_gradient_on_bf16_rows(precision):
+    heads, dk, dm = 4, 32, 128
+    prob = att_problem(seed=31, heads=heads, dk=dk, dm=dm)
+    ctx = att_ctx(prob, heads, dk, dm, precision=precision, B=40)
+    anchors = np.arange(0, 120, 3, dtype=np.uint32)
+    cnt, nbr, neg = ctx.sample_batch(anchors, 1)
+    out = ctx.step_explicit(anchors, cnt, nbr, neg, 1, 0.01, apply_update=True)
+    u = prob.table.view(np.uint32)
+    tb = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).view(np.float32)
+    pb = Problem(prob.indptr, prob.indices, prob.f_indptr, prob.f_ids, prob.V, prob.D, tb)
+    res, merged, t64, *_ = oracle_step(pb, anchors, cnt, nbr, neg, 0.5, 0.01, 1, "first_leaf", attention=(heads, dk))
+    G64 = dense_grad(merged, prob.V, prob.D)
+    Gg = np.zeros_like(G64)
+    Gg[out["feat"]] = out["grad"]
+    # FP32 arithmetic over the bf16 rows the kernels read: within 1e-5 of FP64 on the same rows
+    assert rel(Gg, G64) <= TOL
+    tbf = (ctx.get_table_bf16().astype(np.uint32) << 16).view(np.float32)
+    rows = sorted(merged)
+    untouched = np.setdiff1d(np.arange(prob.V), rows)
+    assert np.array_equal(tbf[untouched], tb[untouched])
+    mask = np.abs(G64[rows]) > 1e-3 * np.abs(G64).max()
+    # updated rows: within one bf16 ulp of the exact update (the master, when kept, starts
+    # from the FP32 rows; the update delta depends only on the gradient)
+    x = t64 if precision == N.EP_PREC_BF16 else prob.table.astype(np.float64) + (t64 - tb.astype(np.float64))
+    got, x = tbf[rows][mask].astype(np.float64), x[rows][mask]
+    assert np.all(np.abs(got - x) <= np.abs(x) * 2.0 ** -7 + 1e-7)
+    ctx.close()
+
+
+@pytest.mark.parametrize("P", [2, 4])
+def test_attention_concurrent_batches_match_oracle(P):
+    heads, dk, dm = 2, 8, 16
+    B = 20
+    prob = att_problem(seed=70 + P, heads=heads, dk=dk, dm=dm, n=200, V=60, zipf=1.3, avg_deg=10)
+    ctx = att_ctx(prob, heads, dk, dm, S=5, K=4, B=B, concurrent=P)
+    rng = np.random.default_rng(P)
+    anchors = rng.choice(prob.n, size=P * B - 3, replace=False).astype(np.uint32)
+    cnt, nbr, neg = ctx.sample_batch(anchors, 1)
+    out = ctx.step_explicit(anchors, cnt, nbr, neg, 1, 0.01, apply_update=True)
+    merged, t64, m64, v64 = oracle_group_step(prob, anchors, cnt, nbr, neg, 0.5, 0.01, 1, "first_leaf", B,
+                                              attention=(heads, dk))
+    rows_g = out["feat"] & ((1 << 27) - 1)
+    bat_g = out["feat"] >> 27
+    for k in range(P):
+        Gk = dense_grad(merged[k], prob.V, prob.D)
+        sel = bat_g == k
+        assert set(rows_g[sel].tolist()) == set(merged[k])
+        Gg = np.zeros_like(Gk)
+        Gg[rows_g[sel]] = out["grad"][sel]
+        assert rel(Gg, Gk) <= TOL
+    m, v = ctx.get_adam()
+    rows = sorted(set().union(*[set(mk) for mk in merged]))
+    assert rel(m[rows], m64[rows]) <= 1e-5
+    assert rel(v[rows], v64[rows]) <= 1e-4
+    ctx.close()
+
+
+@pytest.mark.parametrize("precision,use_graph", [(N.EP_PREC_FP32, 0), (N.EP_PREC_BF16, 1)])
+def test_attention_training_runs_and_learns(precision, use_graph):
+    # whole passes through ep_learn (sampling, validation forward, tuner, optional graph)
+    heads, dk, dm = 4, 8, 16
+    prob = att_problem(seed=91, heads=heads, dk=dk, dm=dm, n=400, V=150, avg_deg=10)
+    cfg = N.default_config()
+    cfg.dims, cfg.batch_size, cfg.negatives, cfg.gamma, cfg.alpha = dm, 32, 4, 0.5, 0.01
+    cfg.attention_dk, cfg.attention_heads = dk, heads
+    cfg.max_nodes_kind, cfg.max_nodes = N.EP_SAMPLE_FIXED, 6
+    cfg.precision, cfg.use_graph, cfg.passes, cfg.valid_pct = precision, use_graph, 6, 0.1
+    with N.Context(cfg) as ctx:
+        ctx.set_graph(prob.indptr, prob.indices)
+        ctx.set_features(prob.f_indptr, prob.f_ids, prob.V)
+        ctx.set_table(prob.table)
+        hist = []
+        ctx.learn(lambda p, s, tr, va, lr: hist.append((tr, va)))
+        t = ctx.get_table()
+    assert len(hist) == 6 and np.all(np.isfinite(t))
+    print(f"[attention] train err {hist[0][0]:.4f} -> {hist[-1][0]:.4f}, valid {hist[0][1]:.4f} -> {hist[-1][1]:.4f}")
+    assert hist[-1][0] < hist[0][0]
+
+
+def test_attention_table_init_is_unit_rows():
+    # randomize_embedding_store (mod.rs:381-392) at the attention row width 768: x = 2u - 1,
+    # each row divided by its norm; every value its own draw
+    heads, dk, dm = 4, 32, 128
+    prob = att_problem(seed=3, heads=heads, dk=dk, dm=dm, n=60, V=2000)
+    cfg = N.default_config()
+    cfg.dims, cfg.attention_dk, cfg.attention_heads = dm, dk, heads
+    with N.Context(cfg) as ctx:
+        ctx.set_graph(prob.indptr, prob.indices)
+        ctx.set_features(prob.f_indptr, prob.f_ids, prob.V)
+        ctx.set_table(None)
+        t = ctx.get_table().astype(np.float64)
+    assert t.shape == (2000, 768)
+    assert np.allclose(np.linalg.norm(t, axis=1), 1.0, atol=1e-5)
+    x = t * np.linalg.norm(t, axis=1, keepdims=True) / np.sqrt(768 / 3.0)  # ~U(-1,1) scaled by E|x|
+    assert abs(t.mean()) < 1e-3
+    for lag in (1, 4, 8, 12):  # no repeated groups inside a lane's 12-value slice
+        assert abs(np.corrcoef(x[:, :-lag].ravel(), x[:, lag:].ravel())[0, 1]) < 0.02
+    assert len(np.unique(t[0])) == 768

@@ -48,6 +48,32 @@ def test_invalid_config_rejected(field, value):
     assert N.load().ep_last_error()
 
 
+@pytest.mark.parametrize("fields", [
+    dict(attention_dk=5, attention_heads=4, dims=128),            # d_k not built
+    dict(attention_dk=32, attention_heads=4, dims=128, context_window=3),  # Sliding attention
+    dict(attention_dk=8, attention_heads=2, dims=6),              # d_model not a multiple of 4
+    dict(attention_dk=8, attention_heads=0, dims=16),             # no heads
+    dict(attention_dk=8, attention_heads=3, dims=16),             # row width 96 not built
+    dict(attention_dk=8, attention_heads=2, dims=16, scatter=1),  # attention needs the sort scatter
+])
+def test_invalid_attention_config_rejected(fields):
+    c = N.default_config()
+    for k, v in fields.items():
+        setattr(c, k, v)
+    h = ctypes.c_void_p()
+    rc = N.load().ep_create(ctypes.byref(c), ctypes.byref(h))
+    assert rc == -1 and not h.value
+    assert N.load().ep_last_error()
+
+
+def test_attention_row_width():
+    c = N.default_config()
+    c.dims, c.attention_dk, c.attention_heads = 128, 32, 4
+    assert N.row_width(c) == 4 * (2 * 32 + 128) == 768  # model.rs:241-243
+    c.attention_dk = 0
+    assert N.row_width(c) == 128
+
+
 # Random123 kat_vectors for philox4x32 with 10 rounds
 PHILOX_KAT = [
     ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
