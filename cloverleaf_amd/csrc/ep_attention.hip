@@ -7,8 +7,8 @@
 //
 // An example's attention blocks are numbered from blkoff[b]: block 0 is h(v), blocks
 // 1..cnt are the sampled neighbours of ~h(v) (one attention per neighbour, averaged),
-// blocks cnt+1..cnt+K are the negatives h(u_k).  One wavefront per block, one item per
-// lane (at most 64 items):
+// blocks cnt+1..cnt+K are the negatives h(u_k).  One wavefront per block of n <= 64
+// items:
 //   h(v), h(u_k)  one item per distinct feature of the node, count 2 x multiplicity
 //                 (model.rs:267-268 adds 1 + 1 per occurrence);
 //   neighbour     the node's listed features, count = multiplicity in the node
@@ -16,6 +16,16 @@
 // Per head: S_ij = (Q_i . K_j) m_ij / sqrt(d_k), m_ij = c_i c_j when >= 1 (else 1),
 // A = row softmax(S) (exact exp), w_j = sum_i A_ij, out_h = (1/n) sum_j w_j V_j; the
 // block output is the mean over heads; one item returns V of head 0 (attention.rs:113-115).
+//
+// The n x n products run on the matrix cores, FP32 in and out
+// (v_mfma_f32_32x32x2f32: 32x32 tiles, two k values per instruction, so a d_k <= 32
+// contraction is 16 instructions per tile).  Items are padded to 32 or 64 with zero
+// rows; padded scores are masked.  Lane l of a tile holds column l % 32 and the 16 rows
+// 8 (r / 4) + 4 (l / 32) + r % 4 of its registers r.  Two layouts of the scores:
+//   K Q^T  (lane = query i, registers = keys j): the row softmax needs only the partner
+//          lane l ^ 32; the probabilities there are the A operand of dQ = dT K;
+//   Q K^T  (lane = key j, registers = queries i): column sums w_j need only the
+//          partner lane; the probabilities there are the A operand of dK = dT^T Q.
 //
 // k_att_forward   block outputs
 // k_att_loss      one wave per example: a, p = mean of the neighbour blocks, n_k; margin
@@ -32,7 +42,21 @@ namespace {
 
 constexpr int NW = kBlockWaves;  // waves per 256-thread block (k_att_loss)
 constexpr int kAttMaxItems = 64;
-constexpr int kAttLdsStride = 65;  // row stride of the n x n score tile (conflict-free columns)
+constexpr int kTileStride = 36;  // LDS row stride (floats) of the Q / K operand tiles
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ inline f32x16 mfma32(float a, float b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+}
+
+// Row of register r (0..15) of a 32x32 MFMA result tile held by lane l.
+__device__ inline int mrow(int r, int l) { return 8 * (r >> 2) + 4 * (l >> 5) + (r & 3); }
+
+__device__ inline float count_factor(float ci, float cj) {
+  const float mm = ci * cj;
+  return mm >= 1.0f ? mm : 1.0f;  // attention.rs:183-186
+}
 
 template <typename T>
 __device__ inline float ld1(const T* p);
@@ -55,10 +79,12 @@ __device__ inline float4 ld4<uint16_t>(const uint16_t* p) {
 
 // LDS of one block's wave (floats / words).
 struct AttLds {
-  float* K;      // [64][DK]: K_h (forward, backward), then Q_h (backward dK pass)
-  float* A;      // [64][65]: scores, probabilities, then dT
-  float* invZ;   // [64]
-  float* u;      // [64]
+  float* Q;      // [64][kTileStride]: Q_h of every item (backward B operand)
+  float* K;      // [64][kTileStride]: K_h of every item
+  float* M;      // [64] row max of the scaled scores
+  float* iZ;     // [64] 1 / softmax denominator
+  float* u;      // [64] dL/dw_j
+  float* r;      // [64] sum_j A_ij u_j
   float* g;      // [att_dm] block adjoint
   uint32_t* f;   // [64] feature of each item
   float* cnt;    // [64] count of each item
@@ -66,20 +92,19 @@ struct AttLds {
   uint32_t* raw; // [64] the node's listed features
 };
 
-template <int DK>
-__device__ inline size_t att_lds_words(int dm) {
-  return (size_t)kAttMaxItems * DK + (size_t)kAttMaxItems * kAttLdsStride + 2 * kAttMaxItems + (size_t)dm +
-         4 * kAttMaxItems;
+__host__ __device__ inline size_t att_lds_floats(int dm) {
+  return 2 * (size_t)kAttMaxItems * kTileStride + 8 * (size_t)kAttMaxItems + (size_t)dm;
 }
 
-template <int DK>
 __device__ inline AttLds att_lds(float* sm, int dm) {
   AttLds L;
-  L.K = sm;
-  L.A = L.K + kAttMaxItems * DK;
-  L.invZ = L.A + kAttMaxItems * kAttLdsStride;
-  L.u = L.invZ + kAttMaxItems;
-  L.g = L.u + kAttMaxItems;
+  L.Q = sm;
+  L.K = L.Q + kAttMaxItems * kTileStride;
+  L.M = L.K + kAttMaxItems * kTileStride;
+  L.iZ = L.M + kAttMaxItems;
+  L.u = L.iZ + kAttMaxItems;
+  L.r = L.u + kAttMaxItems;
+  L.g = L.r + kAttMaxItems;
   L.f = (uint32_t*)(L.g + dm);
   L.cnt = (float*)(L.f + kAttMaxItems);
   L.q = (uint32_t*)(L.cnt + kAttMaxItems);
@@ -91,48 +116,25 @@ __device__ inline AttLds att_lds(float* sm, int dm) {
 struct AttBlock {
   uint32_t b;      // example in the step
   uint32_t ci;     // construction: 0 a, 1 p, 2+k negative k
-  uint32_t cnt;    // sampled neighbours of the example
   uint32_t node;
   uint32_t n;      // items
   uint32_t slot0;  // key position of the node's first feature occurrence
   bool neighbour;  // block inside ~h(v)
 };
 
-// Finds block g's example (last b with blkoff[b] <= g), node, items (into LDS) and slot
-// base.  Returns false for waves past the step's blocks.
+// Block g's example, node, slot base (from the block map of k_att_map) and items (into
+// LDS).  Returns false for waves past the step's blocks.
 __device__ inline bool att_setup(const StepConfig& c, const StepState* st, uint32_t g, const AttLds& L,
                                  AttBlock& B) {
   const uint32_t lane = lane_id();
   const uint32_t nb = c.blkoff[c.batch];
   if (g >= nb) return false;
-  uint32_t lo = 0, hi = c.batch;  // blkoff[lo] <= g < blkoff[hi]
-  while (hi - lo > 1u) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (c.blkoff[mid] <= g) lo = mid; else hi = mid;
-  }
-  B.b = lo;
-  const uint32_t j = g - c.blkoff[lo];
-  B.cnt = c.nbr_cnt[lo];
-  const uint32_t v = c.perm[st->anchor_base + lo];
-  const uint32_t* pn = c.all_nbrs ? c.indices + c.indptr[v] : c.nbr + (size_t)lo * c.S;
-  auto node_of = [&](uint32_t t) -> uint32_t {
-    return t == 0u ? v : (t <= B.cnt ? pn[t - 1u] : c.neg[(size_t)lo * c.K + (t - 1u - B.cnt)]);
-  };
-  B.node = node_of(j);
-  B.neighbour = j >= 1u && j <= B.cnt;
-  B.ci = j == 0u ? 0u : (B.neighbour ? 1u : 2u + (j - 1u - B.cnt));
-  // key position: the example's occurrences are listed node by node in block order
-  uint32_t pre = 0;
-  for (uint32_t t0 = 0; t0 < j; t0 += 64u) {
-    const uint32_t t = t0 + lane;
-    uint32_t x = 0;
-    if (t < j) {
-      const uint32_t u = node_of(t);
-      x = (uint32_t)(c.findptr[u + 1] - c.findptr[u]);
-    }
-    pre += wave_sum_u32(x);
-  }
-  B.slot0 = c.keyoff[lo] + pre;
+  const uint32_t meta = c.bmeta[g];
+  B.b = meta >> 5;
+  B.ci = meta & 31u;
+  B.neighbour = B.ci == 1u;
+  B.node = c.bnode[g];
+  B.slot0 = c.bslot[g];
   const uint64_t fs = c.findptr[B.node];
   const uint32_t nraw = (uint32_t)(c.findptr[B.node + 1] - fs);  // <= 64 (checked by the host)
   const uint32_t fq = lane < nraw ? c.fids[fs + lane] : 0xFFFFFFFFu;
@@ -147,6 +149,7 @@ __device__ inline bool att_setup(const StepConfig& c, const StepState* st, uint3
   }
   if (B.neighbour) {
     B.n = nraw;
+    if (lane >= nraw) L.cnt[lane] = 0.f;  // padded items
     if (lane < nraw) {
       L.f[lane] = fq;
       L.cnt[lane] = (float)mult;
@@ -155,6 +158,7 @@ __device__ inline bool att_setup(const StepConfig& c, const StepState* st, uint3
   } else {
     const uint64_t m = __ballot(lane < nraw && first);
     B.n = (uint32_t)__popcll(m);
+    if (lane >= B.n) L.cnt[lane] = 0.f;  // padded items
     if (lane < nraw && first) {
       const uint32_t i = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
       L.f[i] = fq;
@@ -166,52 +170,104 @@ __device__ inline bool att_setup(const StepConfig& c, const StepState* st, uint3
   return true;
 }
 
-// Row i (lane i < n) of the head's attention into L.A[i][*]: scores, then the softmax
-// probabilities.  K_h must be in L.K.
-template <int DK>
-__device__ inline void att_row_softmax(const AttLds& L, uint32_t n, uint32_t i, const float (&q)[DK], float ci,
-                                       float inv_sqrt_dk) {
-  float* Ai = L.A + (size_t)i * kAttLdsStride;
-  float mx = -INFINITY;
-  for (uint32_t j = 0; j < n; ++j) {
-    const float* Kj = L.K + (size_t)j * DK;
-    float s = 0.f;
+// Q_h and K_h values of item i for lane half hf: elements 16 hf .. 16 hf + 15 (zero past
+// d_k and for padded items).
+template <int DK, typename T>
+__device__ inline void load_qk16(const T* __restrict__ row, bool valid, int h, int H, int hf, float (&q)[16],
+                                 float (&k)[16]) {
 #pragma unroll
-    for (int t = 0; t < DK; t += 4) {
-      const float4 k4 = *reinterpret_cast<const float4*>(Kj + t);
-      s += q[t] * k4.x;
-      s += q[t + 1] * k4.y;
-      s += q[t + 2] * k4.z;
-      s += q[t + 3] * k4.w;
+  for (int t = 0; t < 16; ++t) q[t] = k[t] = 0.f;
+  if (!valid) return;
+  const int k0 = 16 * hf;
+#pragma unroll
+  for (int t = 0; t < 16; t += 4) {
+    if (k0 + t < DK) {
+      const float4 a = ld4<T>(row + h * DK + k0 + t);
+      const float4 b = ld4<T>(row + H * DK + h * DK + k0 + t);
+      q[t] = a.x; q[t + 1] = a.y; q[t + 2] = a.z; q[t + 3] = a.w;
+      k[t] = b.x; k[t + 1] = b.y; k[t + 2] = b.z; k[t + 3] = b.w;
     }
-    const float mm = ci * L.cnt[j];
-    if (mm >= 1.0f) s *= mm;
-    s *= inv_sqrt_dk;
-    Ai[j] = s;
-    mx = fmaxf(mx, s);
   }
-  float z = 0.f;
-  for (uint32_t j = 0; j < n; ++j) {
-    const float e = expf(Ai[j] - mx);
-    Ai[j] = e;
-    z += e;
-  }
-  const float iz = 1.0f / z;
-  for (uint32_t j = 0; j < n; ++j) Ai[j] *= iz;
 }
 
-template <int DK, typename T>
-__device__ inline void load_qk(const T* __restrict__ row, int h, int H, float (&q)[DK], float* kdst) {
+// Row statistics of query tile I from the K Q^T tiles: scaled, masked scores into T,
+// returns the row max and 1 / denominator of query i = 32 I + l % 32 (both halves).
+__device__ inline void row_softmax_stats(f32x16 (&T)[2], int NT, uint32_t n, int I, float ci, const AttLds& L,
+                                         float isq, float* mx_out, float* iz_out) {
+  const int l = (int)lane_id();
+  const uint32_t i = 32u * I + (l & 31);
+  float mx = -INFINITY;
+  #pragma unroll
+  for (int J = 0; J < 2; ++J) {
+    if (J >= NT) break;
 #pragma unroll
-  for (int t = 0; t < DK; t += 4) {
-    const float4 a = ld4<T>(row + h * DK + t);
-    q[t] = a.x; q[t + 1] = a.y; q[t + 2] = a.z; q[t + 3] = a.w;
-    const float4 k = ld4<T>(row + H * DK + h * DK + t);
-    *reinterpret_cast<float4*>(kdst + t) = k;
+    for (int r = 0; r < 16; ++r) {
+      const uint32_t j = 32u * J + mrow(r, l);
+      float s = T[J][r] * count_factor(ci, L.cnt[j]) * isq;
+      s = (j < n && i < n) ? s : -INFINITY;
+      T[J][r] = s;
+      mx = fmaxf(mx, s);
+    }
   }
+  mx = fmaxf(mx, __shfl_xor(mx, 32, EP_WAVE));
+  const float m0 = (i < n) ? mx : 0.f;
+  float z = 0.f;
+  #pragma unroll
+  for (int J = 0; J < 2; ++J) {
+    if (J >= NT) break;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const float e = expf(T[J][r] - m0);  // masked scores give exp(-inf) = 0
+      T[J][r] = e;
+      z += e;
+    }
+  }
+  z += __shfl_xor(z, 32, EP_WAVE);
+  *mx_out = m0;
+  *iz_out = (i < n) ? 1.0f / z : 0.f;
 }
 
 }  // namespace
+
+// ------------------------------------------------------------------------------------
+// k_att_map: one wave per example; for each of its attention blocks the node, the key
+// position of the node's first feature occurrence (the example's occurrences are listed
+// node by node in block order from keyoff[b]) and example << 5 | construction.
+// ------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_att_map(StepConfig c, StepState* st) {
+  const uint32_t lane = lane_id();
+  const uint32_t b = blockIdx.x * NW + (threadIdx.x >> 6);
+  if (b >= c.batch || b >= st->batch_n) return;
+  const uint32_t cnt = c.nbr_cnt[b];
+  if (cnt == 0u) return;
+  const uint32_t K = c.K, E = 1u + cnt + K;
+  const uint32_t g0 = c.blkoff[b];
+  if (g0 + E > c.blkoff[c.batch]) return;  // over capacity: the step reports an error
+  const uint32_t v = c.perm[st->anchor_base + b];
+  const uint32_t* pn = c.all_nbrs ? c.indices + c.indptr[v] : c.nbr + (size_t)b * c.S;
+  uint32_t carry = c.keyoff[b];
+  for (uint32_t j0 = 0; j0 < E; j0 += 64u) {
+    const uint32_t j = j0 + lane;
+    uint32_t node = 0, len = 0, ci = 0;
+    if (j < E) {
+      node = j == 0u ? v : (j <= cnt ? pn[j - 1u] : c.neg[(size_t)b * K + (j - 1u - cnt)]);
+      len = (uint32_t)(c.findptr[node + 1] - c.findptr[node]);
+      ci = j == 0u ? 0u : (j <= cnt ? 1u : 2u + (j - 1u - cnt));
+    }
+    uint32_t inc = len;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(inc, off, EP_WAVE);
+      if ((int)lane >= off) inc += y;
+    }
+    if (j < E) {
+      c.bnode[g0 + j] = node;
+      c.bslot[g0 + j] = carry + inc - len;
+      c.bmeta[g0 + j] = (b << 5) | ci;
+    }
+    carry += __shfl(inc, 63, EP_WAVE);
+  }
+}
 
 // ------------------------------------------------------------------------------------
 // k_att_forward: one wave (64-thread block) per attention block.
@@ -221,47 +277,115 @@ __global__ __launch_bounds__(64) void k_att_forward(StepConfig c, StepState* st)
   extern __shared__ float sm[];
   const int H = c.att_heads, dm = c.att_dm;
   const uint32_t W = (uint32_t)c.dims;
-  const AttLds L = att_lds<DK>(sm, dm);
+  const AttLds L = att_lds(sm, dm);
   AttBlock B;
   if (!att_setup(c, st, blockIdx.x, L, B)) return;
-  const uint32_t lane = lane_id();
+  const int l = (int)lane_id(), hf = l >> 5, c32 = l & 31;
   const T* table = (const T*)c.table;
   float* out = c.bout + (size_t)blockIdx.x * dm;
   const uint32_t n = B.n;
-  if (lane == 0) c.bitems[blockIdx.x] = n;
+  if (l == 0) c.bitems[blockIdx.x] = n;
   if (n == 1u) {  // attention.rs:113-115: the value vector of head 0
     const T* row = table + (size_t)L.f[0] * W + 2u * H * DK;
-    for (int t = lane; t < dm; t += 64) out[t] = ld1<T>(row + t);
+    for (int t = l; t < dm; t += 64) out[t] = ld1<T>(row + t);
     return;
   }
-  const float inv_sqrt_dk = 1.0f / sqrtf((float)DK);
-  float acc[4] = {0.f, 0.f, 0.f, 0.f};
-  const float ci = lane < n ? L.cnt[lane] : 0.f;
+  const float isq = 1.0f / sqrtf((float)DK);
+  const int NT = (int)((n + 31u) / 32u);
+  // out_h accumulation: lane half hf takes items jj = hf (mod 2), lane l % 32 the
+  // values 4 (l % 32) + 128 v of d_model
+  float4 acc[2] = {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
   for (int h = 0; h < H; ++h) {
-    float q[DK];
-    if (lane < n) load_qk<DK, T>(table + (size_t)L.f[lane] * W, h, H, q, L.K + (size_t)lane * DK);
-    __syncthreads();
-    if (lane < n) att_row_softmax<DK>(L, n, lane, q, ci, inv_sqrt_dk);
-    __syncthreads();
-    float w = 0.f;
-    if (lane < n)
-      for (uint32_t i = 0; i < n; ++i) w += L.A[(size_t)i * kAttLdsStride + lane];
-    for (uint32_t j = 0; j < n; ++j) {
-      const float wj = __shfl(w, (int)j, EP_WAVE);
-      const T* vrow = table + (size_t)L.f[j] * W + 2u * H * DK + (uint32_t)h * dm;
+    float qa[2][16], ka[2][16];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int t = (int)lane + 64 * r;
-        if (t < dm) acc[r] += wj * ld1<T>(vrow + t);
+    for (int I = 0; I < 2; ++I) {
+      const uint32_t i = 32u * I + c32;
+      const bool ok = I < NT && i < n;
+      load_qk16<DK, T>(table + (size_t)(ok ? L.f[i] : 0u) * W, ok, h, H, hf, qa[I], ka[I]);
+    }
+    // softmax statistics per query row from the K Q^T tiles
+    #pragma unroll
+    for (int I = 0; I < 2; ++I) {
+      if (I >= NT) break;
+      f32x16 Tt[2];
+      #pragma unroll
+      for (int J = 0; J < 2; ++J) {
+        if (J >= NT) break;
+        f32x16 a = {};
+#pragma unroll
+        for (int t = 0; t < 16; ++t) a = mfma32(ka[J][t], qa[I][t], a);
+        Tt[J] = a;
+      }
+      const uint32_t i = 32u * I + c32;
+      const float ci = i < n ? L.cnt[i] : 0.f;
+      float mx, iz;
+      row_softmax_stats(Tt, NT, n, I, ci, L, isq, &mx, &iz);
+      if (hf == 0) { L.M[i] = mx; L.iZ[i] = iz; }
+    }
+    __syncthreads();
+    // column sums w_j from the Q K^T tiles, then out_h += sum_j w_j V_j
+    #pragma unroll
+    for (int J = 0; J < 2; ++J) {
+      if (J >= NT) break;
+      const uint32_t j = 32u * J + c32;
+      const float cj = j < n ? L.cnt[j] : 0.f;
+      float wj = 0.f;
+      #pragma unroll
+      for (int I = 0; I < 2; ++I) {
+        if (I >= NT) break;
+        f32x16 a = {};
+#pragma unroll
+        for (int t = 0; t < 16; ++t) a = mfma32(qa[I][t], ka[J][t], a);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const uint32_t i = 32u * I + mrow(r, l);
+          const float s = a[r] * count_factor(L.cnt[i], cj) * isq;
+          wj += (i < n && j < n) ? expf(s - L.M[i]) * L.iZ[i] : 0.f;
+        }
+      }
+      wj += __shfl_xor(wj, 32, EP_WAVE);
+      const uint32_t jn = min(32u, n - 32u * J);
+      for (uint32_t j0 = 0; j0 < jn; j0 += 8u) {
+        // 4 rows per lane half in flight
+        float4 vv[4][2];
+        float ww[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t jj = j0 + 2u * u + (uint32_t)hf;
+          const bool ok = jj < jn;
+          const float w = __shfl(wj, (int)(ok ? jj : 0u), EP_WAVE);
+          ww[u] = ok ? w : 0.f;
+          const T* vrow = table + (size_t)L.f[32u * J + (ok ? jj : 0u)] * W + 2u * H * DK + (uint32_t)h * dm;
+#pragma unroll
+          for (int v = 0; v < 2; ++v) {
+            const int d = 4 * c32 + 128 * v;
+            vv[u][v] = (ok && d < dm) ? ld4<T>(vrow + d) : make_float4(0.f, 0.f, 0.f, 0.f);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int v = 0; v < 2; ++v) {
+            acc[v].x += ww[u] * vv[u][v].x;
+            acc[v].y += ww[u] * vv[u][v].y;
+            acc[v].z += ww[u] * vv[u][v].z;
+            acc[v].w += ww[u] * vv[u][v].w;
+          }
       }
     }
     __syncthreads();
   }
   const float scale = 1.0f / ((float)n * (float)H);
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int t = (int)lane + 64 * r;
-    if (t < dm) out[t] = acc[r] * scale;
+  for (int v = 0; v < 2; ++v) {
+    float4 a = acc[v];
+    a.x += __shfl_xor(a.x, 32, EP_WAVE);
+    a.y += __shfl_xor(a.y, 32, EP_WAVE);
+    a.z += __shfl_xor(a.z, 32, EP_WAVE);
+    a.w += __shfl_xor(a.w, 32, EP_WAVE);
+    const int d = 4 * c32 + 128 * v;
+    if (hf == 0 && d < dm) *reinterpret_cast<float4*>(out + d) = make_float4(a.x * scale, a.y * scale, a.z * scale,
+                                                                              a.w * scale);
   }
 }
 
@@ -464,122 +588,172 @@ __global__ __launch_bounds__(256) void k_att_loss(StepConfig c, StepState* st) {
 // gradient row [dQ | dK | dV] of every item at adj[slot * W], slot = the key position
 // of the item's feature occurrence.  Per head, with g the block adjoint:
 //   u_j = (V_j . g) / (n H)          dL/dw_j
-//   dS_ij = A_ij (u_j - sum_k A_ik u_k),   dT = dS m / sqrt(d_k)
-//   dQ_i = sum_j dT_ij K_j,  dK_j = sum_i dT_ij Q_i,  dV_j = w_j g / (n H)
+//   dS_ij = A_ij (u_j - r_i), r_i = sum_k A_ik u_k,   dT = dS m / sqrt(d_k)
+//   dQ = dT K,  dK = dT^T Q  (matrix cores),  dV_j = w_j g / (n H)
 // ------------------------------------------------------------------------------------
 template <int DK, typename T>
 __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st) {
   extern __shared__ float sm[];
   const int H = c.att_heads, dm = c.att_dm;
   const uint32_t W = (uint32_t)c.dims;
-  const AttLds L = att_lds<DK>(sm, dm);
+  const AttLds L = att_lds(sm, dm);
   if (st->n_keys > c.key_cap) return;
   AttBlock B;
   if (!att_setup(c, st, blockIdx.x, L, B)) return;
   const uint32_t NC = c.K + 2u;
   if (!c.elig[(size_t)B.b * NC + B.ci]) return;
-  const uint32_t lane = lane_id();
+  const int l = (int)lane_id(), hf = l >> 5, c32 = l & 31;
   const T* table = (const T*)c.table;
   const uint32_t n = B.n;
-  for (int t = lane; t < dm; t += 64) L.g[t] = c.badj[(size_t)blockIdx.x * dm + t];
+  for (int t = l; t < dm; t += 64) L.g[t] = c.badj[(size_t)blockIdx.x * dm + t];
   __syncthreads();
   const uint32_t qoff = 2u * H * DK;  // first V column
   if (n == 1u) {
     float* grow = c.adj + (size_t)(B.slot0 + L.q[0]) * W;
-    for (uint32_t t = lane; t < W; t += 64u) {
+    for (uint32_t t = l; t < W; t += 64u) {
       const uint32_t vt = t - qoff;
       grow[t] = (t >= qoff && vt < (uint32_t)dm) ? L.g[vt] : 0.f;
     }
     return;
   }
-  const float inv_sqrt_dk = 1.0f / sqrtf((float)DK);
+  const float isq = 1.0f / sqrtf((float)DK);
   const float scale = 1.0f / ((float)n * (float)H);
-  const float ci = lane < n ? L.cnt[lane] : 0.f;
-  const uint32_t my_slot = lane < n ? B.slot0 + L.q[lane] : 0u;
-  float* my_row = c.adj + (size_t)my_slot * W;
+  const int NT = (int)((n + 31u) / 32u);
+  auto slot_row = [&](uint32_t i) -> float* { return c.adj + (size_t)(B.slot0 + L.q[i]) * W; };
   for (int h = 0; h < H; ++h) {
-    float q[DK];
-    if (lane < n) {
-      const T* row = table + (size_t)L.f[lane] * W;
-      load_qk<DK, T>(row, h, H, q, L.K + (size_t)lane * DK);
-      // u_i = (V_i,h . g) / (n H)
-      const T* vrow = row + qoff + (size_t)h * dm;
-      float s = 0.f;
-      for (int t = 0; t < dm; t += 4) {
-        const float4 v4 = ld4<T>(vrow + t);
-        s += v4.x * L.g[t] + v4.y * L.g[t + 1] + v4.z * L.g[t + 2] + v4.w * L.g[t + 3];
-      }
-      L.u[lane] = s * scale;
-    }
-    __syncthreads();
-    float r_i = 0.f;
-    if (lane < n) {
-      att_row_softmax<DK>(L, n, lane, q, ci, inv_sqrt_dk);
-      const float* Ai = L.A + (size_t)lane * kAttLdsStride;
-      for (uint32_t j = 0; j < n; ++j) r_i += Ai[j] * L.u[j];
-    }
-    __syncthreads();
-    float w = 0.f;  // column sums: w_j
-    if (lane < n)
-      for (uint32_t i = 0; i < n; ++i) w += L.A[(size_t)i * kAttLdsStride + lane];
-    __syncthreads();
-    // row pass: dT into the tile, dQ_i
-    float dq[DK];
+    float qa[2][16], ka[2][16];
 #pragma unroll
-    for (int t = 0; t < DK; ++t) dq[t] = 0.f;
-    if (lane < n) {
-      float* Ai = L.A + (size_t)lane * kAttLdsStride;
-      for (uint32_t j = 0; j < n; ++j) {
-        float mm = ci * L.cnt[j];
-        if (!(mm >= 1.0f)) mm = 1.0f;
-        const float dT = Ai[j] * (L.u[j] - r_i) * mm * inv_sqrt_dk;
-        Ai[j] = dT;
-        const float* Kj = L.K + (size_t)j * DK;
+    for (int I = 0; I < 2; ++I) {
+      const uint32_t i = 32u * I + c32;
+      const bool ok = I < NT && i < n;
+      load_qk16<DK, T>(table + (size_t)(ok ? L.f[i] : 0u) * W, ok, h, H, hf, qa[I], ka[I]);
+      if (I < NT) {
 #pragma unroll
-        for (int t = 0; t < DK; t += 4) {
-          const float4 k4 = *reinterpret_cast<const float4*>(Kj + t);
-          dq[t] += dT * k4.x;
-          dq[t + 1] += dT * k4.y;
-          dq[t + 2] += dT * k4.z;
-          dq[t + 3] += dT * k4.w;
+        for (int t = 0; t < 16; t += 4) {
+          *reinterpret_cast<float4*>(L.Q + (size_t)i * kTileStride + 16 * hf + t) =
+              make_float4(qa[I][t], qa[I][t + 1], qa[I][t + 2], qa[I][t + 3]);
+          *reinterpret_cast<float4*>(L.K + (size_t)i * kTileStride + 16 * hf + t) =
+              make_float4(ka[I][t], ka[I][t + 1], ka[I][t + 2], ka[I][t + 3]);
         }
       }
-#pragma unroll
-      for (int t = 0; t < DK; t += 4)
-        *reinterpret_cast<float4*>(my_row + h * DK + t) = make_float4(dq[t], dq[t + 1], dq[t + 2], dq[t + 3]);
     }
-    __syncthreads();
-    if (lane < n)
+    // u_j: lane pair (l, l ^ 32) splits item j's d_model values
+    #pragma unroll
+    for (int J = 0; J < 2; ++J) {
+      if (J >= NT) break;
+      const uint32_t j = 32u * J + c32;
+      float p = 0.f;
+      if (j < n) {
+        const T* vrow = table + (size_t)L.f[j] * W + qoff + (size_t)h * dm;
+        const int half = dm >> 1, d0 = hf * half;  // d_model is a multiple of 8
+        for (int t0 = 0; t0 < half; t0 += 16) {
+          float4 v4[4];
 #pragma unroll
-      for (int t = 0; t < DK; t += 4)
-        *reinterpret_cast<float4*>(L.K + (size_t)lane * DK + t) = make_float4(q[t], q[t + 1], q[t + 2], q[t + 3]);
-    __syncthreads();
-    // column pass: dK_j = sum_i dT_ij Q_i
-    if (lane < n) {
-      float dk[DK];
+          for (int u = 0; u < 4; ++u)
+            v4[u] = (t0 + 4 * u < half) ? ld4<T>(vrow + d0 + t0 + 4 * u) : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-      for (int t = 0; t < DK; ++t) dk[t] = 0.f;
-      for (uint32_t i = 0; i < n; ++i) {
-        const float dT = L.A[(size_t)i * kAttLdsStride + lane];
-        const float* Qi = L.K + (size_t)i * DK;
-#pragma unroll
-        for (int t = 0; t < DK; t += 4) {
-          const float4 q4 = *reinterpret_cast<const float4*>(Qi + t);
-          dk[t] += dT * q4.x;
-          dk[t + 1] += dT * q4.y;
-          dk[t + 2] += dT * q4.z;
-          dk[t + 3] += dT * q4.w;
+          for (int u = 0; u < 4; ++u) {
+            if (t0 + 4 * u < half) {
+              const float* gg = L.g + d0 + t0 + 4 * u;
+              p += v4[u].x * gg[0] + v4[u].y * gg[1] + v4[u].z * gg[2] + v4[u].w * gg[3];
+            }
+          }
         }
       }
-#pragma unroll
-      for (int t = 0; t < DK; t += 4)
-        *reinterpret_cast<float4*>(my_row + H * DK + h * DK + t) = make_float4(dk[t], dk[t + 1], dk[t + 2], dk[t + 3]);
+      p += __shfl_xor(p, 32, EP_WAVE);
+      if (hf == 0) L.u[j] = j < n ? p * scale : 0.f;
     }
-    // dV_j = w_j g / (n H), row by row across the wave
-    for (uint32_t j = 0; j < n; ++j) {
-      const float cw = __shfl(w, (int)j, EP_WAVE) * scale;
-      float* vr = c.adj + (size_t)(B.slot0 + L.q[j]) * W + qoff + (size_t)h * dm;
-      for (int t = lane; t < dm; t += 64) vr[t] = cw * L.g[t];
+    __syncthreads();
+    // query tiles (K Q^T layout): probabilities, r_i, dT, dQ = dT K
+    #pragma unroll
+    for (int I = 0; I < 2; ++I) {
+      if (I >= NT) break;
+      f32x16 Tt[2];
+      #pragma unroll
+      for (int J = 0; J < 2; ++J) {
+        if (J >= NT) break;
+        f32x16 a = {};
+#pragma unroll
+        for (int t = 0; t < 16; ++t) a = mfma32(ka[J][t], qa[I][t], a);
+        Tt[J] = a;
+      }
+      const uint32_t i = 32u * I + c32;
+      const float ci = i < n ? L.cnt[i] : 0.f;
+      float mx, iz;
+      row_softmax_stats(Tt, NT, n, I, ci, L, isq, &mx, &iz);
+      float ri = 0.f;
+      #pragma unroll
+      for (int J = 0; J < 2; ++J) {
+        if (J >= NT) break;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const uint32_t j = 32u * J + mrow(r, l);
+          Tt[J][r] *= iz;  // A_ij
+          ri += Tt[J][r] * L.u[j];
+        }
+      }
+      ri += __shfl_xor(ri, 32, EP_WAVE);
+      if (hf == 0) { L.M[i] = mx; L.iZ[i] = iz; L.r[i] = ri; }
+      f32x16 dq = {};
+      #pragma unroll
+      for (int J = 0; J < 2; ++J) {
+        if (J >= NT) break;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+          const uint32_t j = 32u * J + mrow(t, l);
+          const float dT = Tt[J][t] * (L.u[j] - ri) * count_factor(ci, L.cnt[j]) * isq;
+          dq = mfma32(dT, L.K[(size_t)j * kTileStride + c32], dq);
+        }
+      }
+      if (c32 < DK) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const uint32_t ir = 32u * I + mrow(r, l);
+          if (ir < n) slot_row(ir)[h * DK + c32] = dq[r];
+        }
+      }
+    }
+    __syncthreads();
+    // key tiles (Q K^T layout): w_j, dT, dK = dT^T Q, dV_j = w_j g / (n H)
+    #pragma unroll
+    for (int J = 0; J < 2; ++J) {
+      if (J >= NT) break;
+      const uint32_t j = 32u * J + c32;
+      const float cj = j < n ? L.cnt[j] : 0.f;
+      const float uj = j < n ? L.u[j] : 0.f;
+      float wj = 0.f;
+      f32x16 dk = {};
+      #pragma unroll
+      for (int I = 0; I < 2; ++I) {
+        if (I >= NT) break;
+        f32x16 a = {};
+#pragma unroll
+        for (int t = 0; t < 16; ++t) a = mfma32(qa[I][t], ka[J][t], a);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const uint32_t i = 32u * I + mrow(r, l);
+          const float f = count_factor(L.cnt[i], cj) * isq;
+          const float A = (i < n && j < n) ? expf(a[r] * f - L.M[i]) * L.iZ[i] : 0.f;
+          wj += A;
+          a[r] = A * (uj - L.r[i]) * f;  // dT_ij
+        }
+#pragma unroll
+        for (int t = 0; t < 16; ++t) dk = mfma32(a[t], L.Q[(size_t)(32u * I + mrow(t, l)) * kTileStride + c32], dk);
+      }
+      wj += __shfl_xor(wj, 32, EP_WAVE);
+      if (c32 < DK) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const uint32_t jr = 32u * J + mrow(r, l);
+          if (jr < n) slot_row(jr)[H * DK + h * DK + c32] = dk[r];
+        }
+      }
+      const uint32_t jn = min(32u, n - 32u * J);
+      for (uint32_t jj = 0; jj < jn; ++jj) {
+        const float cw = __shfl(wj, (int)jj, EP_WAVE) * scale;
+        float* vr = slot_row(32u * J + jj) + qoff + (size_t)h * dm;
+        for (int t = l; t < dm; t += 64) vr[t] = cw * L.g[t];
+      }
     }
     __syncthreads();
   }
@@ -589,13 +763,8 @@ __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st
 // Launchers
 // ------------------------------------------------------------------------------------
 bool att_shape_supported(int heads, int dk, int dm) {
-  return heads >= 1 && heads <= 16 && (dk == 4 || dk == 8 || dk == 16 || dk == 32) && dm >= 4 && dm <= 256 &&
-         dm % 4 == 0;
-}
-
-static size_t att_lds_bytes(int dk, int dm) {
-  return ((size_t)kAttMaxItems * dk + (size_t)kAttMaxItems * kAttLdsStride + 2 * kAttMaxItems + (size_t)dm +
-          4 * kAttMaxItems) * 4;
+  return heads >= 1 && heads <= 16 && (dk == 4 || dk == 8 || dk == 16 || dk == 32) && dm >= 8 && dm <= 256 &&
+         dm % 8 == 0;
 }
 
 #define EP_ATT_DISPATCH(dk, BODY)                      \
@@ -608,7 +777,8 @@ static size_t att_lds_bytes(int dk, int dm) {
   }
 
 hipError_t launch_att_forward(const StepConfig& c, StepState* st, hipStream_t s) {
-  const size_t lds = att_lds_bytes(c.att_dk, c.att_dm);
+  const size_t lds = att_lds_floats(c.att_dm) * 4;
+  hipLaunchKernelGGL(k_att_map, dim3((c.batch + NW - 1) / NW), dim3(256), 0, s, c, st);
   EP_ATT_DISPATCH(c.att_dk, {
     if (c.bf16) hipLaunchKernelGGL((k_att_forward<DK, uint16_t>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
     else hipLaunchKernelGGL((k_att_forward<DK, float>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
@@ -618,7 +788,7 @@ hipError_t launch_att_forward(const StepConfig& c, StepState* st, hipStream_t s)
 }
 
 hipError_t launch_att_backward(const StepConfig& c, StepState* st, hipStream_t s) {
-  const size_t lds = att_lds_bytes(c.att_dk, c.att_dm);
+  const size_t lds = att_lds_floats(c.att_dm) * 4;
   EP_ATT_DISPATCH(c.att_dk, {
     if (c.bf16) hipLaunchKernelGGL((k_att_backward<DK, uint16_t>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
     else hipLaunchKernelGGL((k_att_backward<DK, float>), dim3(c.blk_cap), dim3(64), lds, s, c, st);

@@ -204,6 +204,7 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   c->release(b.long_pbase); c->release(b.long_runs); c->release(b.piece_st); c->release(b.piece_info);
   c->release(b.part); c->release(b.srec); c->release(b.mrec); c->release(b.scount); c->release(b.mcount);
   c->release(b.blkoff); c->release(b.bitems); c->release(b.bout); c->release(b.badj);
+  c->release(b.bnode); c->release(b.bslot); c->release(b.bmeta);
   c->release(c->d_explicit);
   c->release(c->d_xout);
   if (c->step_exec) { (void)hipGraphExecDestroy(c->step_exec); c->step_exec = nullptr; c->step_exec_ready = false; }
@@ -230,6 +231,9 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
     const uint32_t dm = cfg.dims;
     EP_TRY(c->alloc(&b.blkoff, batch + 1));
     EP_TRY(c->alloc(&b.bitems, blk_cap));
+    EP_TRY(c->alloc(&b.bnode, blk_cap));
+    EP_TRY(c->alloc(&b.bslot, blk_cap));
+    EP_TRY(c->alloc(&b.bmeta, blk_cap));
     EP_TRY(c->alloc(&b.bout, (size_t)blk_cap * dm));
     EP_TRY(c->alloc(&b.badj, (size_t)blk_cap * dm));
   }
@@ -562,7 +566,7 @@ int validate_config(const ep_config* cfg) {
   } else {
     if (cfg->context_window != 0) return fail(EP_EINVAL, "context_window (Sliding attention) is not supported");
     if (!ep::att_shape_supported((int)cfg->attention_heads, (int)cfg->attention_dk, (int)cfg->dims))
-      return fail(EP_EINVAL, "attention: d_k must be 4, 8, 16 or 32, heads 1..16, dims a multiple of 4 in [4, 256]");
+      return fail(EP_EINVAL, "attention: d_k must be 4, 8, 16 or 32, heads 1..16, dims a multiple of 8 in [8, 256]");
     if (!ep::row_dims_supported((int)row_width(*cfg)))
       return fail(EP_EINVAL, "attention: row width heads*(2*d_k+dims) must be one of 8,16,32,64,100,128,256,384,"
                              "512,768");

@@ -170,6 +170,9 @@ struct StepConfig {
   int att_heads, att_dk, att_dm;
   uint32_t* blkoff;        // [batch + 1]
   uint32_t* bitems;        // [blk_cap] items of each block (written by the forward)
+  uint32_t* bnode;         // [blk_cap] node of each block        (written by k_att_map)
+  uint32_t* bslot;         // [blk_cap] key position of its first feature occurrence
+  uint32_t* bmeta;         // [blk_cap] example << 5 | construction
   float* bout;             // [blk_cap * att_dm]
   float* badj;             // [blk_cap * att_dm]
   uint32_t blk_cap;

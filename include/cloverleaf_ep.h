@@ -53,7 +53,7 @@ enum { EP_SCATTER_SORT = 0, EP_SCATTER_ATOMIC = 1 };
  * of EmbeddingPropagator::new (src/lib.rs:1205-1307).                           */
 typedef struct ep_config {
   uint32_t dims;               /* d_model; mean model: 8,16,32,64,100,128,256;      */
-                               /* attention: 4..256, a multiple of 4, with a table  */
+                               /* attention: 8..256, a multiple of 8, with a table  */
                                /* row width heads*(2 d_k + dims) of 8..256, 384,    */
                                /* 512 or 768                                        */
   uint32_t batch_size;         /* default 50                                        */
