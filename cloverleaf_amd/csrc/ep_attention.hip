@@ -77,38 +77,50 @@ __device__ inline float4 ld4<uint16_t>(const uint16_t* p) {
                      __uint_as_float(t.y & 0xffff0000u));
 }
 
-// LDS of one block's wave (floats / words).
+// LDS of one block's wave (floats / words).  The per-item arrays come first; the last
+// region holds the Q / K operand tiles (backward) or the value accumulators (forward).
 struct AttLds {
-  float* Q;      // [64][kTileStride]: Q_h of every item (backward B operand)
-  float* K;      // [64][kTileStride]: K_h of every item
   float* M;      // [64] row max of the scaled scores
   float* iZ;     // [64] 1 / softmax denominator
-  float* u;      // [64] dL/dw_j
   float* r;      // [64] sum_j A_ij u_j
-  float* g;      // [att_dm] block adjoint
   uint32_t* f;   // [64] feature of each item
   float* cnt;    // [64] count of each item
   uint32_t* q;   // [64] position of each item in the node's feature list
   uint32_t* raw; // [64] the node's listed features
+  float* u;      // [64][H] dL/dw_jh
+  float* w;      // [64][H] column sums w_jh
+  float* g;      // [att_dm] block adjoint
+  float* Q;      // [64][kTileStride]: Q_h of every item (backward B operand)
+  float* K;      // [64][kTileStride]: K_h of every item
+  float* acc;    // [H * att_dm] value sweep (forward; aliases Q / K)
 };
 
-__host__ __device__ inline size_t att_lds_floats(int dm) {
-  return 2 * (size_t)kAttMaxItems * kTileStride + 8 * (size_t)kAttMaxItems + (size_t)dm;
+constexpr int kMaxHeads = 16;
+
+__host__ __device__ inline size_t att_lds_small(int dm, int H) {
+  return 7 * (size_t)kAttMaxItems + 2 * (size_t)kAttMaxItems * H + (size_t)((dm + 3) & ~3);
 }
 
-__device__ inline AttLds att_lds(float* sm, int dm) {
+// forward: the value accumulators; backward: the operand tiles
+__host__ __device__ inline size_t att_lds_floats(int dm, int H, bool backward) {
+  return att_lds_small(dm, H) + (backward ? 2 * (size_t)kAttMaxItems * kTileStride : (size_t)H * dm);
+}
+
+__device__ inline AttLds att_lds(float* sm, int dm, int H) {
   AttLds L;
-  L.Q = sm;
-  L.K = L.Q + kAttMaxItems * kTileStride;
-  L.M = L.K + kAttMaxItems * kTileStride;
+  L.M = sm;
   L.iZ = L.M + kAttMaxItems;
-  L.u = L.iZ + kAttMaxItems;
-  L.r = L.u + kAttMaxItems;
-  L.g = L.r + kAttMaxItems;
-  L.f = (uint32_t*)(L.g + dm);
+  L.r = L.iZ + kAttMaxItems;
+  L.f = (uint32_t*)(L.r + kAttMaxItems);
   L.cnt = (float*)(L.f + kAttMaxItems);
   L.q = (uint32_t*)(L.cnt + kAttMaxItems);
   L.raw = L.q + kAttMaxItems;
+  L.u = (float*)(L.raw + kAttMaxItems);
+  L.w = L.u + kAttMaxItems * H;
+  L.g = L.w + kAttMaxItems * H;
+  L.Q = sm + att_lds_small(dm, H);  // 16-byte aligned: the small part is a multiple of 4 floats
+  L.K = L.Q + kAttMaxItems * kTileStride;
+  L.acc = L.Q;
   return L;
 }
 
@@ -270,14 +282,17 @@ __global__ __launch_bounds__(256) void k_att_map(StepConfig c, StepState* st) {
 }
 
 // ------------------------------------------------------------------------------------
-// k_att_forward: one wave (64-thread block) per attention block.
+// k_att_forward: one wave (64-thread block) per attention block.  Per head: scores on
+// the matrix cores, softmax statistics, column sums w_jh (LDS); the next head's Q / K
+// are loaded meanwhile.  Then one sweep over the items' value parts (all heads of an
+// item are contiguous): out = sum_j sum_h w_jh V_jh / (n H).
 // ------------------------------------------------------------------------------------
 template <int DK, typename T>
 __global__ __launch_bounds__(64) void k_att_forward(StepConfig c, StepState* st) {
   extern __shared__ float sm[];
   const int H = c.att_heads, dm = c.att_dm;
   const uint32_t W = (uint32_t)c.dims;
-  const AttLds L = att_lds(sm, dm);
+  const AttLds L = att_lds(sm, dm, H);
   AttBlock B;
   if (!att_setup(c, st, blockIdx.x, L, B)) return;
   const int l = (int)lane_id(), hf = l >> 5, c32 = l & 31;
@@ -285,30 +300,34 @@ __global__ __launch_bounds__(64) void k_att_forward(StepConfig c, StepState* st)
   float* out = c.bout + (size_t)blockIdx.x * dm;
   const uint32_t n = B.n;
   if (l == 0) c.bitems[blockIdx.x] = n;
+  const uint32_t vbase = 2u * H * DK;  // first V column
   if (n == 1u) {  // attention.rs:113-115: the value vector of head 0
-    const T* row = table + (size_t)L.f[0] * W + 2u * H * DK;
+    const T* row = table + (size_t)L.f[0] * W + vbase;
     for (int t = l; t < dm; t += 64) out[t] = ld1<T>(row + t);
     return;
   }
   const float isq = 1.0f / sqrtf((float)DK);
   const int NT = (int)((n + 31u) / 32u);
-  // out_h accumulation: lane half hf takes items jj = hf (mod 2), lane l % 32 the
-  // values 4 (l % 32) + 128 v of d_model
-  float4 acc[2] = {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
-  for (int h = 0; h < H; ++h) {
-    float qa[2][16], ka[2][16];
+  float qa[2][16], ka[2][16], qn[2][16], kn[2][16];
 #pragma unroll
-    for (int I = 0; I < 2; ++I) {
+  for (int I = 0; I < 2; ++I) {
+    const uint32_t i = 32u * I + c32;
+    const bool ok = I < NT && i < n;
+    load_qk16<DK, T>(table + (size_t)(ok ? L.f[i] : 0u) * W, ok, 0, H, hf, qa[I], ka[I]);
+  }
+  for (int h = 0; h < H; ++h) {
+#pragma unroll
+    for (int I = 0; I < 2; ++I) {  // next head's operands in flight
       const uint32_t i = 32u * I + c32;
-      const bool ok = I < NT && i < n;
-      load_qk16<DK, T>(table + (size_t)(ok ? L.f[i] : 0u) * W, ok, h, H, hf, qa[I], ka[I]);
+      const bool ok = h + 1 < H && I < NT && i < n;
+      load_qk16<DK, T>(table + (size_t)(ok ? L.f[i] : 0u) * W, ok, h + 1, H, hf, qn[I], kn[I]);
     }
     // softmax statistics per query row from the K Q^T tiles
-    #pragma unroll
+#pragma unroll
     for (int I = 0; I < 2; ++I) {
       if (I >= NT) break;
       f32x16 Tt[2];
-      #pragma unroll
+#pragma unroll
       for (int J = 0; J < 2; ++J) {
         if (J >= NT) break;
         f32x16 a = {};
@@ -323,14 +342,14 @@ __global__ __launch_bounds__(64) void k_att_forward(StepConfig c, StepState* st)
       if (hf == 0) { L.M[i] = mx; L.iZ[i] = iz; }
     }
     __syncthreads();
-    // column sums w_j from the Q K^T tiles, then out_h += sum_j w_j V_j
-    #pragma unroll
+    // column sums w_jh from the Q K^T tiles
+#pragma unroll
     for (int J = 0; J < 2; ++J) {
       if (J >= NT) break;
       const uint32_t j = 32u * J + c32;
       const float cj = j < n ? L.cnt[j] : 0.f;
       float wj = 0.f;
-      #pragma unroll
+#pragma unroll
       for (int I = 0; I < 2; ++I) {
         if (I >= NT) break;
         f32x16 a = {};
@@ -339,53 +358,56 @@ __global__ __launch_bounds__(64) void k_att_forward(StepConfig c, StepState* st)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           const uint32_t i = 32u * I + mrow(r, l);
-          const float s = a[r] * count_factor(L.cnt[i], cj) * isq;
-          wj += (i < n && j < n) ? expf(s - L.M[i]) * L.iZ[i] : 0.f;
+          const float sc = a[r] * count_factor(L.cnt[i], cj) * isq;
+          wj += (i < n && j < n) ? expf(sc - L.M[i]) * L.iZ[i] : 0.f;
         }
       }
       wj += __shfl_xor(wj, 32, EP_WAVE);
-      const uint32_t jn = min(32u, n - 32u * J);
-      for (uint32_t j0 = 0; j0 < jn; j0 += 8u) {
-        // 4 rows per lane half in flight
-        float4 vv[4][2];
-        float ww[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const uint32_t jj = j0 + 2u * u + (uint32_t)hf;
-          const bool ok = jj < jn;
-          const float w = __shfl(wj, (int)(ok ? jj : 0u), EP_WAVE);
-          ww[u] = ok ? w : 0.f;
-          const T* vrow = table + (size_t)L.f[32u * J + (ok ? jj : 0u)] * W + 2u * H * DK + (uint32_t)h * dm;
-#pragma unroll
-          for (int v = 0; v < 2; ++v) {
-            const int d = 4 * c32 + 128 * v;
-            vv[u][v] = (ok && d < dm) ? ld4<T>(vrow + d) : make_float4(0.f, 0.f, 0.f, 0.f);
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int v = 0; v < 2; ++v) {
-            acc[v].x += ww[u] * vv[u][v].x;
-            acc[v].y += ww[u] * vv[u][v].y;
-            acc[v].z += ww[u] * vv[u][v].z;
-            acc[v].w += ww[u] * vv[u][v].w;
-          }
-      }
+      if (hf == 0) L.w[j * H + h] = j < n ? wj : 0.f;
     }
     __syncthreads();
-  }
-  const float scale = 1.0f / ((float)n * (float)H);
 #pragma unroll
-  for (int v = 0; v < 2; ++v) {
-    float4 a = acc[v];
-    a.x += __shfl_xor(a.x, 32, EP_WAVE);
-    a.y += __shfl_xor(a.y, 32, EP_WAVE);
-    a.z += __shfl_xor(a.z, 32, EP_WAVE);
-    a.w += __shfl_xor(a.w, 32, EP_WAVE);
-    const int d = 4 * c32 + 128 * v;
-    if (hf == 0 && d < dm) *reinterpret_cast<float4*>(out + d) = make_float4(a.x * scale, a.y * scale, a.z * scale,
-                                                                              a.w * scale);
+    for (int I = 0; I < 2; ++I)
+#pragma unroll
+      for (int t = 0; t < 16; ++t) { qa[I][t] = qn[I][t]; ka[I][t] = kn[I][t]; }
+  }
+  // value sweep: lane l covers 8 consecutive values p of an item's value part
+  const int VP = H * dm;
+  float* sacc = L.acc;
+  for (int c0 = 0; c0 < VP; c0 += 512) {
+    const int p = c0 + 8 * l;
+    const bool pv = p < VP;
+    const int hh = pv ? p / dm : 0;
+    float a8[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (uint32_t j0 = 0; j0 < n; j0 += 4u) {
+      float4 v[4][2];
+      float w[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t j = j0 + (uint32_t)u;
+        const bool ok = pv && j < n;
+        w[u] = ok ? L.w[j * H + hh] : 0.f;
+        const T* row = table + (size_t)L.f[ok ? j : 0u] * W + vbase + (ok ? p : 0);
+        v[u][0] = ok ? ld4<T>(row) : make_float4(0.f, 0.f, 0.f, 0.f);
+        v[u][1] = ok ? ld4<T>(row + 4) : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a8[0] += w[u] * v[u][0].x; a8[1] += w[u] * v[u][0].y; a8[2] += w[u] * v[u][0].z; a8[3] += w[u] * v[u][0].w;
+        a8[4] += w[u] * v[u][1].x; a8[5] += w[u] * v[u][1].y; a8[6] += w[u] * v[u][1].z; a8[7] += w[u] * v[u][1].w;
+      }
+    }
+    if (pv) {
+      *reinterpret_cast<float4*>(sacc + p) = make_float4(a8[0], a8[1], a8[2], a8[3]);
+      *reinterpret_cast<float4*>(sacc + p + 4) = make_float4(a8[4], a8[5], a8[6], a8[7]);
+    }
+  }
+  __syncthreads();
+  const float scale = 1.0f / ((float)n * (float)H);
+  for (int d = l; d < dm; d += 64) {
+    float a = 0.f;
+    for (int h = 0; h < H; ++h) a += sacc[h * dm + d];
+    out[d] = a * scale;
   }
 }
 
@@ -596,7 +618,7 @@ __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st
   extern __shared__ float sm[];
   const int H = c.att_heads, dm = c.att_dm;
   const uint32_t W = (uint32_t)c.dims;
-  const AttLds L = att_lds(sm, dm);
+  const AttLds L = att_lds(sm, dm, H);
   if (st->n_keys > c.key_cap) return;
   AttBlock B;
   if (!att_setup(c, st, blockIdx.x, L, B)) return;
@@ -763,7 +785,7 @@ __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st
 // Launchers
 // ------------------------------------------------------------------------------------
 bool att_shape_supported(int heads, int dk, int dm) {
-  return heads >= 1 && heads <= 16 && (dk == 4 || dk == 8 || dk == 16 || dk == 32) && dm >= 8 && dm <= 256 &&
+  return heads >= 1 && heads <= kMaxHeads && (dk == 4 || dk == 8 || dk == 16 || dk == 32) && dm >= 8 && dm <= 256 &&
          dm % 8 == 0;
 }
 
@@ -777,7 +799,7 @@ bool att_shape_supported(int heads, int dk, int dm) {
   }
 
 hipError_t launch_att_forward(const StepConfig& c, StepState* st, hipStream_t s) {
-  const size_t lds = att_lds_floats(c.att_dm) * 4;
+  const size_t lds = att_lds_floats(c.att_dm, c.att_heads, false) * 4;
   hipLaunchKernelGGL(k_att_map, dim3((c.batch + NW - 1) / NW), dim3(256), 0, s, c, st);
   EP_ATT_DISPATCH(c.att_dk, {
     if (c.bf16) hipLaunchKernelGGL((k_att_forward<DK, uint16_t>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
@@ -788,7 +810,7 @@ hipError_t launch_att_forward(const StepConfig& c, StepState* st, hipStream_t s)
 }
 
 hipError_t launch_att_backward(const StepConfig& c, StepState* st, hipStream_t s) {
-  const size_t lds = att_lds_floats(c.att_dm) * 4;
+  const size_t lds = att_lds_floats(c.att_dm, c.att_heads, true) * 4;
   EP_ATT_DISPATCH(c.att_dk, {
     if (c.bf16) hipLaunchKernelGGL((k_att_backward<DK, uint16_t>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
     else hipLaunchKernelGGL((k_att_backward<DK, float>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
