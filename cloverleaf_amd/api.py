@@ -546,23 +546,37 @@ class EmbeddingPropagator:
 # Inference aggregate (src/lib.rs:1766-2207, aggregator.rs:34-47)
 # --------------------------------------------------------------------------------------
 class FeatureAggregator:
-    def __init__(self, kind: str):
+    def __init__(self, kind: str, num_heads: int = 0, d_k: int = 0, window: Optional[int] = None):
         self.kind = kind
+        self.num_heads, self.d_k, self.window = num_heads, d_k, window
 
     @staticmethod
     def Averaged():
         return FeatureAggregator("averaged")
 
+    @staticmethod
+    def Attention(num_heads: int, d_k: int, window: Optional[int] = None):
+        """lib.rs:1835-1838; Sliding attention (window) is not supported by this build."""
+        if window is not None:
+            raise ValueError("window (Sliding attention) is not supported by this build")
+        return FeatureAggregator("attention", int(num_heads), int(d_k), None)
+
+    def __repr__(self):
+        if self.kind == "attention":
+            return f"FeatureAggregator<Attention<num_heads={self.num_heads},d_k={self.d_k},window=None>"
+        return "FeatureAggregator<Averaged>"
+
 
 class NodeEmbedder:
     def __init__(self, feat_agg: FeatureAggregator):
-        if feat_agg.kind != "averaged":
-            raise ValueError("only FeatureAggregator.Averaged is supported by this build")
+        if feat_agg.kind not in ("averaged", "attention"):
+            raise ValueError("only FeatureAggregator.Averaged and .Attention are supported by this build")
         self.feat_agg = feat_agg
 
     def embed_feature_set(self, feat_set: FeatureSet, feature_embeddings: NodeEmbeddings,
                           device: int = 0) -> NodeEmbeddings:
-        """lib.rs:2092-2124: node embedding = mean of its (known) feature rows, on the GPU."""
+        """lib.rs:2092-2124: node embedding from its (known) feature rows, on the GPU: their
+        mean (Averaged) or Full attention over them with count 1 each (Attention)."""
         ip, ids = feat_set.to_csr()
         # translate FeatureSet feature ids into the embedding vocabulary, dropping unknown ones
         fv, ev = feat_set.feature_vocab, feature_embeddings.vocab_
@@ -580,11 +594,19 @@ class NodeEmbedder:
             t_ids = mapped[keep].astype(np.uint32)
         table = feature_embeddings.embeddings
         cfg = N.default_config()
-        cfg.dims = table.shape[1]
         cfg.device = int(device)
+        fa = self.feat_agg
+        if fa.kind == "attention":
+            # lib.rs:2041-2046: d_model = (width - 2 H d_k) / H
+            cfg.attention_heads, cfg.attention_dk = fa.num_heads, fa.d_k
+            cfg.dims = (table.shape[1] - 2 * fa.num_heads * fa.d_k) // fa.num_heads
+            if N.row_width(cfg) != table.shape[1]:
+                raise ValueError(f"embedding width {table.shape[1]} does not fit {fa.num_heads} heads of d_k={fa.d_k}")
+        else:
+            cfg.dims = table.shape[1]
         with N.Context(cfg) as ctx:
             # a one-node placeholder feature list: only the table is needed
             ctx.set_features(np.array([0, 1], dtype=np.uint64), np.array([0], dtype=np.uint32), table.shape[0])
             ctx.set_table(table)
-            out = ctx.embed_averaged(t_ip, t_ids)
+            out = ctx.embed_attention(t_ip, t_ids) if fa.kind == "attention" else ctx.embed_averaged(t_ip, t_ids)
         return NodeEmbeddings(feat_set.vocab_, out, Distance.Cosine)

@@ -33,6 +33,8 @@
 // k_att_backward  recomputes each eligible block and writes one gradient row of width W
 //                 per item at the slot of the item's feature occurrence (key position);
 //                 the sorted segment reduce sums a feature's occurrence rows.
+#include <algorithm>
+
 #include "ep_device.h"
 #include "ep_kernels.h"
 
@@ -75,6 +77,26 @@ __device__ inline float4 ld4<uint16_t>(const uint16_t* p) {
   const uint2 t = *reinterpret_cast<const uint2*>(p);
   return make_float4(__uint_as_float(t.x << 16), __uint_as_float(t.x & 0xffff0000u), __uint_as_float(t.y << 16),
                      __uint_as_float(t.y & 0xffff0000u));
+}
+
+// Eight consecutive values (offset a multiple of 8): two 16-byte loads of FP32, one of BF16.
+template <typename T>
+__device__ inline void ld8(const T* p, float (&v)[8]);
+template <>
+__device__ inline void ld8<float>(const float* p, float (&v)[8]) {
+  const float4 a = *reinterpret_cast<const float4*>(p);
+  const float4 b = *reinterpret_cast<const float4*>(p + 4);
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+template <>
+__device__ inline void ld8<uint16_t>(const uint16_t* p, float (&v)[8]) {
+  const uint4 t = *reinterpret_cast<const uint4*>(p);
+  const uint32_t w[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[2 * k] = __uint_as_float(w[k] << 16);
+    v[2 * k + 1] = __uint_as_float(w[k] & 0xffff0000u);
+  }
 }
 
 // LDS of one block's wave (floats / words).  The per-item arrays come first; the last
@@ -191,13 +213,26 @@ __device__ inline void load_qk16(const T* __restrict__ row, bool valid, int h, i
   for (int t = 0; t < 16; ++t) q[t] = k[t] = 0.f;
   if (!valid) return;
   const int k0 = 16 * hf;
+  if constexpr (DK % 8 == 0) {
 #pragma unroll
-  for (int t = 0; t < 16; t += 4) {
-    if (k0 + t < DK) {
-      const float4 a = ld4<T>(row + h * DK + k0 + t);
-      const float4 b = ld4<T>(row + H * DK + h * DK + k0 + t);
-      q[t] = a.x; q[t + 1] = a.y; q[t + 2] = a.z; q[t + 3] = a.w;
-      k[t] = b.x; k[t + 1] = b.y; k[t + 2] = b.z; k[t + 3] = b.w;
+    for (int t = 0; t < 16; t += 8) {
+      if (k0 + t < DK) {
+        float a[8], b[8];
+        ld8<T>(row + h * DK + k0 + t, a);
+        ld8<T>(row + H * DK + h * DK + k0 + t, b);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) { q[t + e] = a[e]; k[t + e] = b[e]; }
+      }
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < 16; t += 4) {
+      if (k0 + t < DK) {
+        const float4 a = ld4<T>(row + h * DK + k0 + t);
+        const float4 b = ld4<T>(row + H * DK + h * DK + k0 + t);
+        q[t] = a.x; q[t + 1] = a.y; q[t + 2] = a.z; q[t + 3] = a.w;
+        k[t] = b.x; k[t + 1] = b.y; k[t + 2] = b.z; k[t + 3] = b.w;
+      }
     }
   }
 }
@@ -282,24 +317,14 @@ __global__ __launch_bounds__(256) void k_att_map(StepConfig c, StepState* st) {
 }
 
 // ------------------------------------------------------------------------------------
-// k_att_forward: one wave (64-thread block) per attention block.  Per head: scores on
-// the matrix cores, softmax statistics, column sums w_jh (LDS); the next head's Q / K
-// are loaded meanwhile.  Then one sweep over the items' value parts (all heads of an
-// item are contiguous): out = sum_j sum_h w_jh V_jh / (n H).
-// ------------------------------------------------------------------------------------
+// Output of one attention block whose n items (features f, counts cnt) are in LDS:
+// per head scores on the matrix cores, softmax statistics and column sums w_jh (LDS),
+// the next head's Q / K loaded meanwhile; then one sweep over the items' value parts
+// (all heads of an item are contiguous): out = sum_j sum_h w_jh V_jh / (n H).
 template <int DK, typename T>
-__global__ __launch_bounds__(64) void k_att_forward(StepConfig c, StepState* st) {
-  extern __shared__ float sm[];
-  const int H = c.att_heads, dm = c.att_dm;
-  const uint32_t W = (uint32_t)c.dims;
-  const AttLds L = att_lds(sm, dm, H);
-  AttBlock B;
-  if (!att_setup(c, st, blockIdx.x, L, B)) return;
+__device__ inline void att_block_output(const T* __restrict__ table, uint32_t W, int H, int dm, const AttLds& L,
+                                        uint32_t n, float* __restrict__ out) {
   const int l = (int)lane_id(), hf = l >> 5, c32 = l & 31;
-  const T* table = (const T*)c.table;
-  float* out = c.bout + (size_t)blockIdx.x * dm;
-  const uint32_t n = B.n;
-  if (l == 0) c.bitems[blockIdx.x] = n;
   const uint32_t vbase = 2u * H * DK;  // first V column
   if (n == 1u) {  // attention.rs:113-115: the value vector of head 0
     const T* row = table + (size_t)L.f[0] * W + vbase;
@@ -380,22 +405,21 @@ __global__ __launch_bounds__(64) void k_att_forward(StepConfig c, StepState* st)
     const int hh = pv ? p / dm : 0;
     float a8[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (uint32_t j0 = 0; j0 < n; j0 += 4u) {
-      float4 v[4][2];
+      float v[4][8];
       float w[4];
+      bool okk[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const uint32_t j = j0 + (uint32_t)u;
-        const bool ok = pv && j < n;
-        w[u] = ok ? L.w[j * H + hh] : 0.f;
-        const T* row = table + (size_t)L.f[ok ? j : 0u] * W + vbase + (ok ? p : 0);
-        v[u][0] = ok ? ld4<T>(row) : make_float4(0.f, 0.f, 0.f, 0.f);
-        v[u][1] = ok ? ld4<T>(row + 4) : make_float4(0.f, 0.f, 0.f, 0.f);
+        okk[u] = pv && j < n;
+        w[u] = okk[u] ? L.w[j * H + hh] : 0.f;
+        const T* row = table + (size_t)L.f[okk[u] ? j : 0u] * W + vbase + (okk[u] ? p : 0);
+        ld8<T>(row, v[u]);  // an in-range row when !okk (not added)
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        a8[0] += w[u] * v[u][0].x; a8[1] += w[u] * v[u][0].y; a8[2] += w[u] * v[u][0].z; a8[3] += w[u] * v[u][0].w;
-        a8[4] += w[u] * v[u][1].x; a8[5] += w[u] * v[u][1].y; a8[6] += w[u] * v[u][1].z; a8[7] += w[u] * v[u][1].w;
-      }
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) a8[e] += okk[u] ? w[u] * v[u][e] : 0.f;
     }
     if (pv) {
       *reinterpret_cast<float4*>(sacc + p) = make_float4(a8[0], a8[1], a8[2], a8[3]);
@@ -408,6 +432,50 @@ __global__ __launch_bounds__(64) void k_att_forward(StepConfig c, StepState* st)
     float a = 0.f;
     for (int h = 0; h < H; ++h) a += sacc[h * dm + d];
     out[d] = a * scale;
+  }
+}
+
+
+// ------------------------------------------------------------------------------------
+// k_att_forward: one wave (64-thread block) per attention block.
+// ------------------------------------------------------------------------------------
+template <int DK, typename T>
+__global__ __launch_bounds__(64) void k_att_forward(StepConfig c, StepState* st) {
+  extern __shared__ float sm[];
+  const int H = c.att_heads, dm = c.att_dm;
+  const AttLds L = att_lds(sm, dm, H);
+  AttBlock B;
+  if (!att_setup(c, st, blockIdx.x, L, B)) return;
+  if (lane_id() == 0) c.bitems[blockIdx.x] = B.n;
+  att_block_output<DK, T>((const T*)c.table, (uint32_t)c.dims, H, dm, L, B.n, c.bout + (size_t)blockIdx.x * dm);
+}
+
+// ------------------------------------------------------------------------------------
+// k_att_embed: AttentionAggregator::construct (aggregator.rs:137-153) for inference, one
+// wave per node: Full attention over the node's listed features with count 1 each.
+// Nodes without features keep a zero row.
+// ------------------------------------------------------------------------------------
+template <int DK, typename T>
+__global__ __launch_bounds__(64) void k_att_embed(const T* __restrict__ table, uint32_t W, int H, int dm,
+                                                  const uint64_t* __restrict__ findptr,
+                                                  const uint32_t* __restrict__ fids, uint64_t n_nodes,
+                                                  float* __restrict__ out) {
+  extern __shared__ float sm[];
+  const AttLds L = att_lds(sm, dm, H);
+  const uint32_t lane = lane_id();
+  for (uint64_t node = blockIdx.x; node < n_nodes; node += gridDim.x) {
+    const uint64_t fs = findptr[node];
+    const uint32_t n = (uint32_t)(findptr[node + 1] - fs);  // <= 64 (checked by the host)
+    float* o = out + node * (uint64_t)dm;
+    if (n == 0u) {
+      for (int t = lane; t < dm; t += 64) o[t] = 0.f;
+      continue;
+    }
+    L.f[lane] = lane < n ? fids[fs + lane] : 0u;
+    L.cnt[lane] = lane < n ? 1.0f : 0.f;
+    __syncthreads();
+    att_block_output<DK, T>(table, W, H, dm, L, n, o);
+    __syncthreads();
   }
 }
 
@@ -806,6 +874,21 @@ hipError_t launch_att_forward(const StepConfig& c, StepState* st, hipStream_t s)
     else hipLaunchKernelGGL((k_att_forward<DK, float>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
   });
   hipLaunchKernelGGL(k_att_loss, dim3((c.batch + NW - 1) / NW), dim3(256), 0, s, c, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_att_embed(const StepConfig& c, const uint64_t* findptr, const uint32_t* fids, uint64_t n_nodes,
+                            float* out, hipStream_t s) {
+  const size_t lds = att_lds_floats(c.att_dm, c.att_heads, false) * 4;
+  const unsigned grid = (unsigned)std::min<uint64_t>(n_nodes, 1u << 20);
+  EP_ATT_DISPATCH(c.att_dk, {
+    if (c.bf16)
+      hipLaunchKernelGGL((k_att_embed<DK, uint16_t>), dim3(grid), dim3(64), lds, s, (const uint16_t*)c.table,
+                         (uint32_t)c.dims, c.att_heads, c.att_dm, findptr, fids, n_nodes, out);
+    else
+      hipLaunchKernelGGL((k_att_embed<DK, float>), dim3(grid), dim3(64), lds, s, (const float*)c.table,
+                         (uint32_t)c.dims, c.att_heads, c.att_dm, findptr, fids, n_nodes, out);
+  });
   return hipGetLastError();
 }
 

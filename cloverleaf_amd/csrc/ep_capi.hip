@@ -1157,6 +1157,37 @@ int ep_embed_averaged(ep_ctx* c, uint64_t n_nodes, const uint64_t* f_indptr, con
   return EP_OK;
 }
 
+int ep_embed_attention(ep_ctx* c, uint64_t n_nodes, const uint64_t* f_indptr, const uint32_t* f_ids, float* out) {
+  if (!c || !f_indptr || !out) return fail(EP_EINVAL, "NULL argument");
+  if (!c->table_ready) return fail(EP_ESTATE, "no table");
+  if (c->cfg.attention_dk == 0) return fail(EP_ESTATE, "the context is not configured for attention");
+  const uint64_t tot = f_indptr[n_nodes];
+  for (uint64_t i = 0; i < n_nodes; ++i) {
+    if (f_indptr[i + 1] < f_indptr[i]) return fail(EP_EINVAL, "f_indptr must be non-decreasing");
+    if (f_indptr[i + 1] - f_indptr[i] > 64) return fail(EP_EINVAL, "attention supports at most 64 features per node");
+  }
+  for (uint64_t i = 0; i < tot; ++i)
+    if (f_ids[i] >= c->n_vocab) return fail(EP_EINVAL, "feature id out of range");
+  EP_HIP(hipSetDevice(c->device));
+  const size_t dm = c->cfg.dims;
+  uint64_t* dp = nullptr;
+  uint32_t* di = nullptr;
+  float* dout = nullptr;
+  EP_TRY(c->alloc(&dp, n_nodes + 1));
+  EP_TRY(c->alloc(&di, tot));
+  EP_TRY(c->alloc(&dout, (size_t)n_nodes * dm));
+  EP_HIP(hipMemcpy(dp, f_indptr, (n_nodes + 1) * sizeof(uint64_t), hipMemcpyHostToDevice));
+  if (tot) EP_HIP(hipMemcpy(di, f_ids, tot * sizeof(uint32_t), hipMemcpyHostToDevice));
+  refresh_base(c);
+  EP_HIP(ep::launch_att_embed(c->base, dp, di, n_nodes, dout, c->stream));
+  EP_HIP(hipStreamSynchronize(c->stream));
+  EP_HIP(hipMemcpy(out, dout, (size_t)n_nodes * dm * sizeof(float), hipMemcpyDeviceToHost));
+  c->release(dp);
+  c->release(di);
+  c->release(dout);
+  return EP_OK;
+}
+
 int ep_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
   if (!ctr || !key || !out) return fail(EP_EINVAL, "NULL argument");
   const ep::u32x4 r = ep::philox4x32(ep::u32x4{ctr[0], ctr[1], ctr[2], ctr[3]}, key[0], key[1]);

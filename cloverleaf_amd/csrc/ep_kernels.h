@@ -209,6 +209,9 @@ bool row_dims_supported(int d);
 bool att_shape_supported(int heads, int dk, int dm);
 hipError_t launch_att_forward(const StepConfig& c, StepState* st, hipStream_t s);
 hipError_t launch_att_backward(const StepConfig& c, StepState* st, hipStream_t s);
+// AttentionAggregator inference (count 1 per listed feature): out[n_nodes * att_dm].
+hipError_t launch_att_embed(const StepConfig& c, const uint64_t* findptr, const uint32_t* fids, uint64_t n_nodes,
+                            float* out, hipStream_t s);
 // Segments of the sorted keys longer than kSegShort entries are summed as pieces of
 // kPieceEntries entries by k_pieces and combined in piece order by k_long_combine.
 constexpr uint32_t kSegShort = 32;

@@ -198,6 +198,13 @@ int ep_bench_steps(ep_ctx* ctx, uint32_t warmup, uint32_t steps, int instrument,
  * aggregator.rs:34-47): out[n_nodes*width] = mean of each node's feature rows.  */
 int ep_embed_averaged(ep_ctx* ctx, uint64_t n_nodes, const uint64_t* f_indptr, const uint32_t* f_ids, float* out);
 
+/* NodeEmbedder(FeatureAggregator.Attention(heads, d_k, None)).embed_feature_set
+ * (src/lib.rs:2092-2124, 2028-2036; AttentionAggregator, aggregator.rs:124-154):
+ * out[n_nodes*dims] = Full attention over each node's listed features, count 1
+ * each, with the context's attention_heads / attention_dk; at most 64 features
+ * per node; nodes without features get a zero row.                             */
+int ep_embed_attention(ep_ctx* ctx, uint64_t n_nodes, const uint64_t* f_indptr, const uint32_t* f_ids, float* out);
+
 /* Philox4x32-10 on the host (the device RNG), for known-answer tests.           */
 int ep_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 

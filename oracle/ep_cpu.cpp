@@ -13,7 +13,11 @@
 //   * per-batch HashMap merge (mod.rs:223-233);
 //   * Hogwild sparse Adam, t = pass + 1, unsynchronised row writes (optimizer.rs:43-81);
 //   * cosine LR with 20% warm-up (scheduler.rs:39-47, mod.rs:132-137).
-// The mean aggregator with max_features = All and MarginLoss (loss.rs:59-73).
+// The mean aggregator with max_features = All and MarginLoss (loss.rs:59-73); with
+// --heads H --dk d_k the attention aggregator instead (AttentionFeatureModel,
+// model.rs:158-249, attention.rs:100-296, Full attention): rows of H (2 d_k + dims)
+// values, gradients by the hand-written adjoint of the attention (the reference takes
+// them from its autodiff tape).
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -56,6 +60,7 @@ struct Args {
   std::string dir;
   uint64_t nodes = 0;
   uint32_t vocab = 0, dims = 64, batch = 128, negatives = 5, max_nodes = 10, threads = 1, passes = 1;
+  uint32_t heads = 0, dk = 0;  // attention model when dk > 0
   float gamma = 0.5f, alpha = 0.01f, budget_s = 0.f;
   uint64_t seed = 20222022ull;
   bool json = false;
@@ -102,6 +107,90 @@ struct Model {
     for (uint64_t q = findptr[node]; q < findptr[node + 1]; ++q) c.add(fids[q], w);
   }
 
+  uint32_t width() const { return a.dk ? a.heads * (2 * a.dk + a.dims) : a.dims; }
+
+  // Full attention over items (features f, counts c), attention_mean (attention.rs:100-131):
+  // out[dims]; with g != nullptr instead adds each item's gradient row (width W) to gout.
+  void attention(const std::vector<uint32_t>& f, const std::vector<float>& c, float* out, const float* g,
+                 float* gout) const {
+    const uint32_t H = a.heads, dk = a.dk, dm = a.dims, W = width(), n = (uint32_t)f.size();
+    const uint32_t vb = 2 * H * dk;
+    if (n == 1) {  // attention.rs:113-115
+      const float* e = &table[(size_t)f[0] * W + vb];
+      if (!g) for (uint32_t d = 0; d < dm; ++d) out[d] = e[d];
+      else for (uint32_t d = 0; d < dm; ++d) gout[vb + d] += g[d];
+      return;
+    }
+    const float isq = 1.f / std::sqrt((float)dk);
+    const float sc = 1.f / ((float)n * (float)H);
+    std::vector<float> A((size_t)n * n), w(n), u(n), r(n);
+    if (!g) std::fill(out, out + dm, 0.f);
+    for (uint32_t h = 0; h < H; ++h) {
+      for (uint32_t i = 0; i < n; ++i) {
+        const float* qi = &table[(size_t)f[i] * W + h * dk];
+        float mx = -INFINITY;
+        for (uint32_t j = 0; j < n; ++j) {
+          const float* kj = &table[(size_t)f[j] * W + H * dk + h * dk];
+          float s = 0.f;
+          for (uint32_t t = 0; t < dk; ++t) s += qi[t] * kj[t];
+          const float mm = c[i] * c[j];
+          if (mm >= 1.f) s *= mm;
+          s *= isq;
+          A[(size_t)i * n + j] = s;
+          mx = std::max(mx, s);
+        }
+        float z = 0.f;
+        for (uint32_t j = 0; j < n; ++j) { const float e = std::exp(A[(size_t)i * n + j] - mx); A[(size_t)i * n + j] = e; z += e; }
+        for (uint32_t j = 0; j < n; ++j) A[(size_t)i * n + j] /= z;
+      }
+      for (uint32_t j = 0; j < n; ++j) {
+        float s = 0.f;
+        for (uint32_t i = 0; i < n; ++i) s += A[(size_t)i * n + j];
+        w[j] = s;
+      }
+      if (!g) {
+        for (uint32_t j = 0; j < n; ++j) {
+          const float* vj = &table[(size_t)f[j] * W + vb + h * dm];
+          for (uint32_t d = 0; d < dm; ++d) out[d] += w[j] * vj[d];
+        }
+        continue;
+      }
+      for (uint32_t j = 0; j < n; ++j) {
+        const float* vj = &table[(size_t)f[j] * W + vb + h * dm];
+        float s = 0.f;
+        for (uint32_t d = 0; d < dm; ++d) s += vj[d] * g[d];
+        u[j] = s * sc;
+        float* gv = &gout[(size_t)j * W + vb + h * dm];
+        for (uint32_t d = 0; d < dm; ++d) gv[d] += w[j] * sc * g[d];
+      }
+      for (uint32_t i = 0; i < n; ++i) {
+        float s = 0.f;
+        for (uint32_t j = 0; j < n; ++j) s += A[(size_t)i * n + j] * u[j];
+        r[i] = s;
+      }
+      for (uint32_t i = 0; i < n; ++i) {
+        const float* qi = &table[(size_t)f[i] * W + h * dk];
+        float* gq = &gout[(size_t)i * W + h * dk];
+        for (uint32_t j = 0; j < n; ++j) {
+          float mm = c[i] * c[j];
+          if (!(mm >= 1.f)) mm = 1.f;
+          const float dT = A[(size_t)i * n + j] * (u[j] - r[i]) * mm * isq;
+          const float* kj = &table[(size_t)f[j] * W + H * dk + h * dk];
+          float* gk = &gout[(size_t)j * W + H * dk + h * dk];
+          for (uint32_t t = 0; t < dk; ++t) { gq[t] += dT * kj[t]; gk[t] += dT * qi[t]; }
+        }
+      }
+    }
+    if (!g) for (uint32_t d = 0; d < dm; ++d) out[d] *= sc;
+  }
+
+  // Items of a neighbour's attention inside ~h(v): the listed features, count = multiplicity
+  void neighbour_items(uint32_t node, std::vector<uint32_t>& f, std::vector<float>& c) const {
+    f.assign(fids.begin() + findptr[node], fids.begin() + findptr[node + 1]);
+    c.resize(f.size());
+    for (size_t i = 0; i < f.size(); ++i) c[i] = (float)std::count(f.begin(), f.end(), f[i]);
+  }
+
   void mean(const Cons& c, float* out) const {  // mean_embeddings (model.rs:448-463)
     const uint32_t D = a.dims;
     std::fill(out, out + D, 0.f);
@@ -122,6 +211,7 @@ struct Model {
     const uint32_t deg = (uint32_t)(s1 - s0);
     if (deg == 0) return 0.f;
     std::vector<Cons> cons(2 + K);
+    std::vector<uint32_t> sampled;
     collect(cons[0], v, 1.0f);
     // ~h(v): Fixed(S).sample(deg) = (min(S,deg), deg/r); reservoir when deg > S (utils.rs:128-145)
     const uint32_t S = a.max_nodes;
@@ -136,10 +226,34 @@ struct Model {
         if (idx < S) res[idx] = indices[s0 + i];
       }
       for (uint32_t u : res) collect(cons[1], u, scale);
+      sampled.swap(res);
     }
-    for (uint32_t k = 0; k < K; ++k) collect(cons[2 + k], train[rng.below((uint32_t)train.size())], 1.0f);
+    std::vector<uint32_t> negs(K), nbrs;
+    for (uint32_t k = 0; k < K; ++k) {
+      negs[k] = train[rng.below((uint32_t)train.size())];
+      collect(cons[2 + k], negs[k], 1.0f);
+    }
     std::vector<float> x((2 + K) * (size_t)D);
-    for (uint32_t c = 0; c < 2 + K; ++c) mean(cons[c], &x[(size_t)c * D]);
+    const bool att = a.dk != 0;
+    if (!att) {
+      for (uint32_t c = 0; c < 2 + K; ++c) mean(cons[c], &x[(size_t)c * D]);
+    } else {
+      // the sampled neighbours again, in order (one attention each, model.rs:419-446)
+      if (deg <= S) nbrs.assign(indices.begin() + s0, indices.begin() + s1);
+      else nbrs = sampled;
+      attention(cons[0].feats, cons[0].counts, &x[0], nullptr, nullptr);
+      std::fill(&x[D], &x[2 * D], 0.f);
+      std::vector<uint32_t> nf;
+      std::vector<float> nc, tmp(D);
+      for (uint32_t u : nbrs) {
+        neighbour_items(u, nf, nc);
+        attention(nf, nc, tmp.data(), nullptr, nullptr);
+        for (uint32_t d = 0; d < D; ++d) x[D + d] += tmp[d];
+      }
+      for (uint32_t d = 0; d < D; ++d) x[D + d] /= (float)nbrs.size();
+      for (uint32_t k = 0; k < K; ++k)
+        attention(cons[2 + k].feats, cons[2 + k].counts, &x[(size_t)(2 + k) * D], nullptr, nullptr);
+    }
     const float* av = &x[0];
     const float* pv = &x[D];
     auto dist = [&](const float* p, const float* q) {
@@ -186,6 +300,33 @@ struct Model {
       bool finite = true;
       for (uint32_t d = 0; d < D; ++d) finite &= std::isfinite(gc[d]);
       if (!finite) continue;
+      if (att) {
+        // leaf gradients of the construction: one row per distinct feature
+        const uint32_t W = width();
+        std::vector<float> rows((size_t)cons[c].feats.size() * W, 0.f);
+        if (c == 1) {
+          std::vector<uint32_t> nf;
+          std::vector<float> nc, gi(D), tmp;
+          for (uint32_t d = 0; d < D; ++d) gi[d] = gc[d] / (float)nbrs.size();
+          for (uint32_t u : nbrs) {
+            neighbour_items(u, nf, nc);
+            tmp.assign((size_t)nf.size() * W, 0.f);
+            attention(nf, nc, nullptr, gi.data(), tmp.data());
+            for (size_t i = 0; i < nf.size(); ++i) {
+              float* dst = &rows[(size_t)cons[1].pos.at(nf[i]) * W];
+              for (uint32_t t = 0; t < W; ++t) dst[t] += tmp[i * W + t];
+            }
+          }
+        } else {
+          attention(cons[c].feats, cons[c].counts, nullptr, gc, rows.data());
+        }
+        for (size_t i = 0; i < cons[c].feats.size(); ++i) {
+          const uint32_t f = cons[c].feats[i];
+          if (grads.count(f)) continue;
+          grads.emplace(f, std::vector<float>(rows.begin() + i * W, rows.begin() + (i + 1) * W));
+        }
+        continue;
+      }
       float C = 0.f;
       for (float q : cons[c].counts) C += q;
       for (size_t i = 0; i < cons[c].feats.size(); ++i) {
@@ -201,7 +342,7 @@ struct Model {
   }
 
   void adam(const std::unordered_map<uint32_t, std::vector<float>>& all, float lr, float pass) {
-    const uint32_t D = a.dims;
+    const uint32_t D = width();
     const float t = pass + 1.f, b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     const float bc1 = 1.f - std::pow(b1, t), bc2 = 1.f - std::pow(b2, t);
     for (const auto& kv : all) {
@@ -239,6 +380,8 @@ int main(int argc, char** argv) {
     else if (k == "--nodes") a.nodes = std::stoull(nx());
     else if (k == "--vocab") a.vocab = (uint32_t)std::stoul(nx());
     else if (k == "--dims") a.dims = (uint32_t)std::stoul(nx());
+    else if (k == "--heads") a.heads = (uint32_t)std::stoul(nx());
+    else if (k == "--dk") a.dk = (uint32_t)std::stoul(nx());
     else if (k == "--batch") a.batch = (uint32_t)std::stoul(nx());
     else if (k == "--negatives") a.negatives = (uint32_t)std::stoul(nx());
     else if (k == "--max-nodes") a.max_nodes = (uint32_t)std::stoul(nx());
@@ -259,7 +402,7 @@ int main(int argc, char** argv) {
   const auto fids = read_bin<uint32_t>(a.dir + "/fids.bin");
   if (indptr.size() != a.nodes + 1 || findptr.size() != a.nodes + 1) { std::fprintf(stderr, "size mismatch\n"); return 2; }
   Model m{a, indptr, indices, findptr, fids, {}, {}, {}, {}};
-  const uint32_t D = a.dims;
+  const uint32_t D = m.width();  // table row width
   // randomize_embedding_store (mod.rs:381-392)
   m.table.resize((size_t)a.vocab * D);
   m.mom.assign(m.table.size(), 0.f);

@@ -218,3 +218,43 @@ def test_attention_table_init_is_unit_rows():
     for lag in (1, 4, 8, 12):  # no repeated groups inside a lane's 12-value slice
         assert abs(np.corrcoef(x[:, :-lag].ravel(), x[:, lag:].ravel())[0, 1]) < 0.02
     assert len(np.unique(t[0])) == 768
+
+
+@pytest.mark.parametrize("heads,dk,dm", [(2, 8, 16), (4, 32, 128)])
+@pytest.mark.parametrize("precision", [N.EP_PREC_FP32, N.EP_PREC_BF16])
+def test_attention_embed_matches_oracle(heads, dk, dm, precision):
+    # AttentionAggregator::construct (aggregator.rs:137-153): every listed feature, count 1
+    from oracle import ep_oracle as O
+    prob = att_problem(seed=40 + dk, heads=heads, dk=dk, dm=dm, n=300, V=100, fmax=64)
+    ctx = att_ctx(prob, heads, dk, dm, precision=precision)
+    # nodes to embed: the problem's nodes plus one without features
+    ip = np.concatenate([prob.f_indptr, prob.f_indptr[-1:]]).astype(np.uint64)
+    out = ctx.embed_attention(ip, prob.f_ids)
+    table = prob.table
+    if precision == N.EP_PREC_BF16:
+        u = table.view(np.uint32)
+        table = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).view(np.float32)
+    ref = np.stack([O.attention_embed(table, prob.f_ids[prob.f_indptr[i]:prob.f_indptr[i + 1]], heads, dk)
+                    for i in range(prob.n)])
+    assert rel(out[:prob.n], ref) <= TOL
+    assert np.all(out[prob.n] == 0)
+    ctx.close()
+
+
+def test_node_embedder_attention_facade():
+    import cloverleaf_amd as C
+    heads, dk, dm = 2, 8, 16
+    prob = att_problem(seed=44, heads=heads, dk=dk, dm=dm, n=200, V=80)
+    g = C.Graph.from_csr(prob.indptr, prob.indices)
+    fs = C.FeatureSet.from_csr(g, prob.f_indptr, prob.f_ids, prob.V)
+    ep = C.EmbeddingPropagator(alpha=0.05, loss=C.EPLoss.margin(0.5, 4), batch_size=32, dims=dm, passes=2,
+                               max_nodes=5, attention=dk, attention_heads=heads, indicator=False)
+    emb = ep.learn_features(g, fs)
+    assert emb.embeddings.shape == (prob.V, heads * (2 * dk + dm))
+    assert ep.history[-1]["train"] < ep.history[0]["train"]
+    node = C.NodeEmbedder(C.FeatureAggregator.Attention(heads, dk)).embed_feature_set(fs, emb)
+    from oracle import ep_oracle as O
+    ref = np.stack([O.attention_embed(emb.embeddings, prob.f_ids[prob.f_indptr[i]:prob.f_indptr[i + 1]], heads, dk)
+                    for i in range(prob.n)])
+    assert node.embeddings.shape == (prob.n, dm)
+    assert rel(node.embeddings, ref) <= TOL
