@@ -211,7 +211,8 @@ def main():
         del src, dst
 
     t0 = time.time()
-    w = synth.make_workload(args.workload, scale=args.scale)
+    w = synth.make_workload(args.workload, scale=args.scale,
+                            log=lambda m: print(f"[bench] {m} ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True))
     print(f"[bench] workload {w.name}: {w.n_nodes} nodes, {len(w.indices)} CSR entries, vocab {w.vocab} "
           f"(generated in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
 

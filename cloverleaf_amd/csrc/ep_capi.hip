@@ -91,6 +91,7 @@ struct ep_ctx {
   uint32_t cap_batch = 0;
   uint32_t cap_S = 0;
   uint32_t key_cap = 0;
+  uint64_t unique_cap = 0;  // merged-gradient records the test hook may need
   ep::StepConfig base{};
   ep::StepState* d_state = nullptr;
   ep::StepState* d_vstate = nullptr;
@@ -246,8 +247,9 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
     EP_TRY(c->alloc(&b.slots[i], keycap));
   }
   EP_TRY(c->alloc(&b.ghist, (size_t)2048 * sort_grid));
-  EP_TRY(c->alloc(&b.G, (size_t)unique_cap * D));
-  EP_TRY(c->alloc(&b.ufeat, unique_cap));
+  // merged-gradient records (G, ufeat) only serve the test hook's gradient output; they
+  // are allocated by ep_step_explicit on first use
+  c->unique_cap = unique_cap;
   EP_TRY(c->alloc(&b.long_st, long_cap));
   EP_TRY(c->alloc(&b.long_len, long_cap));
   EP_TRY(c->alloc(&b.long_pbase, long_cap));
@@ -970,6 +972,10 @@ int ep_step_explicit(ep_ctx* c, const uint32_t* anchors, uint32_t n, const uint3
   EP_TRY(ensure_step_buffers(c, std::max(n, group_anchors(c))));
   refresh_base(c);
   if (cons && !c->d_xout) EP_TRY(c->alloc(&c->d_xout, (size_t)c->cap_batch * NC * D));
+  if (!c->base.G) {
+    EP_TRY(c->alloc(&c->base.G, (size_t)c->unique_cap * D));
+    EP_TRY(c->alloc(&c->base.ufeat, c->unique_cap));
+  }
   ep::StepConfig sc = c->base;
   sc.perm = c->d_explicit;
   sc.negpool = c->d_explicit;

@@ -121,7 +121,18 @@ template <int D>
 __device__ inline void load_slice(const float* __restrict__ row, int lane, float (&v)[RowShape<D>::PL]) {
   constexpr int PL = RowShape<D>::PL;
   const int base = lane * PL;
-  if constexpr (PL == 4 && D % 4 == 0) {
+  if constexpr (PL > 4 && PL % 4 == 0 && D % PL == 0) {  // wide rows: 16-byte chunks
+    if (base < D) {
+#pragma unroll
+      for (int k = 0; k < PL; k += 4) {
+        const float4 t = *reinterpret_cast<const float4*>(row + base + k);
+        v[k] = t.x; v[k + 1] = t.y; v[k + 2] = t.z; v[k + 3] = t.w;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < PL; ++k) v[k] = 0.f;
+    }
+  } else if constexpr (PL == 4 && D % 4 == 0) {
     if (base < D) {
       const float4 t = *reinterpret_cast<const float4*>(row + base);
       v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
@@ -141,7 +152,19 @@ template <int D>
 __device__ inline void load_slice(const uint16_t* __restrict__ row, int lane, float (&v)[RowShape<D>::PL]) {
   constexpr int PL = RowShape<D>::PL;
   const int base = lane * PL;
-  if constexpr (PL == 4 && D % 4 == 0) {
+  if constexpr (PL > 4 && PL % 4 == 0 && D % PL == 0) {  // wide rows: 8-byte chunks
+    if (base < D) {
+#pragma unroll
+      for (int k = 0; k < PL; k += 4) {
+        const uint2 t = *reinterpret_cast<const uint2*>(row + base + k);
+        v[k] = __uint_as_float(t.x << 16); v[k + 1] = __uint_as_float(t.x & 0xffff0000u);
+        v[k + 2] = __uint_as_float(t.y << 16); v[k + 3] = __uint_as_float(t.y & 0xffff0000u);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < PL; ++k) v[k] = 0.f;
+    }
+  } else if constexpr (PL == 4 && D % 4 == 0) {
     if (base < D) {
       const uint2 t = *reinterpret_cast<const uint2*>(row + base);
       v[0] = __uint_as_float(t.x << 16); v[1] = __uint_as_float(t.x & 0xffff0000u);
@@ -162,7 +185,13 @@ template <int D>
 __device__ inline void store_slice(float* __restrict__ row, int lane, const float (&v)[RowShape<D>::PL]) {
   constexpr int PL = RowShape<D>::PL;
   const int base = lane * PL;
-  if constexpr (PL == 4 && D % 4 == 0) {
+  if constexpr (PL > 4 && PL % 4 == 0 && D % PL == 0) {  // wide rows: 16-byte chunks
+    if (base < D) {
+#pragma unroll
+      for (int k = 0; k < PL; k += 4)
+        *reinterpret_cast<float4*>(row + base + k) = make_float4(v[k], v[k + 1], v[k + 2], v[k + 3]);
+    }
+  } else if constexpr (PL == 4 && D % 4 == 0) {
     if (base < D) *reinterpret_cast<float4*>(row + base) = make_float4(v[0], v[1], v[2], v[3]);
   } else if constexpr (PL == 2 && D % 2 == 0) {
     if (base < D) *reinterpret_cast<float2*>(row + base) = make_float2(v[0], v[1]);

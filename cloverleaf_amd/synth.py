@@ -152,6 +152,12 @@ WORKLOADS = {
         graph=("chung_lu", dict(n=10_000_000, m=200_000_000, beta=2.1, dmax=100_000, directed=True)),
         feats=dict(count_kind=COUNT_LOGUNIFORM, p1=1.0, p2=128.0, vocab=5_000_000, zipf_s=1.05),
         dims=256, batch=8192, negatives=20, max_nodes=32, gamma=0.5, alpha=0.01, passes=1),
+    "ep-cfg4-50m": dict(
+        description="Chung-Lu undirected N=50M, 1B edges, beta=2.2; features U{1..64}, Zipf(1.1) over 20M; "
+                    "d=256, B=16384, K=20, S=20 (BF16 table + FP32 master + SR), margin 0.5, lr 0.01",
+        graph=("chung_lu", dict(n=50_000_000, m=1_000_000_000, beta=2.2, dmax=100_000, directed=False)),
+        feats=dict(count_kind=COUNT_UNIFORM, p1=1.0, p2=64.0, vocab=20_000_000, zipf_s=1.1),
+        dims=256, batch=16384, negatives=20, max_nodes=20, gamma=0.5, alpha=0.01, passes=1),
     "ep-cfg5-5m": dict(
         description="Chung-Lu undirected N=5M, 100M edges, beta=2.3; features U{1..64}, Zipf(1.1) over 2M; "
                     "attention H=4, d_k=32, d_model=128 (rows of 768), B=4096, K=10, S=20, margin 0.5, lr 0.01",
@@ -161,8 +167,9 @@ WORKLOADS = {
 }
 
 
-def make_workload(name: str, scale: float = 1.0, seed_graph: int = 1, seed_feat: int = 2) -> Workload:
-    """Generates a named workload; `scale` < 1 shrinks nodes, edges and vocab proportionally."""
+def make_workload(name: str, scale: float = 1.0, seed_graph: int = 1, seed_feat: int = 2, log=None) -> Workload:
+    """Generates a named workload; `scale` < 1 shrinks nodes, edges and vocab proportionally.
+    `log` (optional) receives a line when the graph is done."""
     spec = WORKLOADS[name]
     kind, g = spec["graph"]
     g = dict(g)
@@ -183,6 +190,8 @@ def make_workload(name: str, scale: float = 1.0, seed_graph: int = 1, seed_feat:
         m = max(16, int(g["m"] * scale))
         indptr, indices = bipartite(nq, ni, m, g["beta"], g["dmax"] * (scale if scale < 1 else 1), seed_graph)
         n = nq + ni
+    if log is not None:
+        log(f"graph of {name}: {len(indptr) - 1} nodes, {len(indices)} CSR entries")
     f = dict(spec["feats"])
     vocab = max(16, int(f["vocab"] * scale))
     f_indptr, f_ids = features(n, f["count_kind"], f["p1"], f["p2"], vocab, f["zipf_s"], seed_feat, comm)
