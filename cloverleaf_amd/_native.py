@@ -114,6 +114,7 @@ SIGNATURES = {
     "ep_bench_steps": (c_int, [c_void_p, c_uint32, c_uint32, c_int, POINTER(EPBenchResult)]),
     "ep_embed_averaged": (c_int, [c_void_p, c_uint64, _u64p, _u32p, _f32p]),
     "ep_embed_attention": (c_int, [c_void_p, c_uint64, _u64p, _u32p, _f32p]),
+    "ep_embed_weighted": (c_int, [c_void_p, c_uint64, _u64p, _u32p, _f32p, c_uint32, c_float, _f32p]),
     "ep_philox4x32": (c_int, [_u32p, _u32p, _u32p]),
     "ep_last_error": (c_char_p, []),
 }
@@ -312,6 +313,16 @@ class Context:
         out = np.empty((n, self.cfg.dims), dtype=np.float32)
         check(load().ep_embed_attention(self.handle, n, ptr(f_indptr, c_uint64), ptr(f_ids, c_uint32),
                                         ptr(out, c_float)), "ep_embed_attention")
+        return out
+
+    def embed_weighted(self, f_indptr: np.ndarray, f_ids: np.ndarray, p_w: np.ndarray, alpha: float) -> np.ndarray:
+        f_indptr = np.ascontiguousarray(f_indptr, dtype=np.uint64)
+        f_ids = np.ascontiguousarray(f_ids, dtype=np.uint32)
+        p_w = np.ascontiguousarray(p_w, dtype=np.float32)
+        n = len(f_indptr) - 1
+        out = np.empty((n, row_width(self.cfg)), dtype=np.float32)
+        check(load().ep_embed_weighted(self.handle, n, ptr(f_indptr, c_uint64), ptr(f_ids, c_uint32), ptr(p_w, c_float),
+                                       len(p_w), float(alpha), ptr(out, c_float)), "ep_embed_weighted")
         return out
 
 

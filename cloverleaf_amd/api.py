@@ -137,6 +137,51 @@ class Graph:
     def __repr__(self):
         return f"Graph<Nodes={self.nodes()}, Edges={self.edges()}>"
 
+    def save(self, path: str, comp_level: Optional[int] = None):
+        """Graph.save (src/lib.rs:414-430): from_type<TAB>from<TAB>to_type<TAB>to<TAB>weight per
+        edge, the weight the edge's probability (CDFtoP of the row); `.gz` paths are gzipped."""
+        with _open_text_write(path, comp_level) as fh:
+            for i in range(self.nodes()):
+                s, e = int(self.indptr[i]), int(self.indptr[i + 1])
+                if s == e:
+                    continue
+                cdf = self.weights[s:e] if self.weights is not None else _uniform_cdf(e - s)
+                p = np.empty(e - s, dtype=np.float32)
+                p[0] = cdf[0]
+                p[1:] = cdf[1:] - cdf[:-1]
+                ft, fn = self.vocab_.name(i)
+                for j, w in zip(self.indices[s:e], p):
+                    tt, tn = self.vocab_.name(int(j))
+                    fh.write(f"{ft}\t{fn}\t{tt}\t{tn}\t{_f32_display(w)}\n")
+
+    @staticmethod
+    def load(path: str, edge_type: str, chunk_size: Optional[int] = None, skip_rows: Optional[int] = None,
+             weighted: Optional[bool] = None, deduplicate: Optional[bool] = None) -> "Graph":
+        """Graph.load (src/lib.rs:472-500, GraphReader::load io.rs:270-323): five tab-separated
+        fields per line, from_type, from, to_type, to, weight (ignored unless weighted,
+        default true); Undirected adds the reverse edge; deduplicate defaults to false."""
+        weighted = True if weighted is None else bool(weighted)
+        b = GraphBuilder()
+        with _open_text(path) as fh:
+            for i, line in enumerate(fh):
+                if i < (skip_rows or 0):
+                    continue
+                pieces = line.rstrip("\n").split("\t")
+                if len(pieces) != 5:
+                    raise ValueError(f"{i}: Malformed graph file: Expected 5 fields!")
+                w = 1.0
+                if weighted:
+                    try:
+                        w = float(np.float32(pieces[4]))
+                    except ValueError as e:
+                        raise ValueError(f"{i}: Malformed graph file! {e} - {pieces[4]!r}") from None
+                b.add_edge((pieces[0], pieces[1]), (pieces[2], pieces[3]), w, edge_type)
+        vocab = b.vocab
+        g = b.build_graph(deduplicate=False if deduplicate is None else bool(deduplicate))
+        if g is None:
+            return Graph(np.zeros(len(vocab) + 1, dtype=np.uint64), np.zeros(0, dtype=np.uint32), None, vocab)
+        return g
+
 
 def _uniform_cdf(k: int) -> np.ndarray:
     if k == 0:
@@ -340,6 +385,20 @@ def _open_text(path: str):
     return open(path, "r")
 
 
+def _open_text_write(path: str, comp_level: Optional[int] = None):
+    """open_file_for_writing (io.rs:219-231): gzip for `.gz` paths (level 1 by default)."""
+    if path.endswith(".gz"):
+        import gzip
+        return gzip.open(path, "wt", compresslevel=1 if comp_level is None else int(comp_level))
+    return open(path, "w")
+
+
+def _f32_display(x) -> str:
+    """An f32 the way Rust's `{}` prints it: the shortest digits that read back to the same
+    value, positional."""
+    return np.format_float_positional(np.float32(x), unique=True, trim="-")
+
+
 # --------------------------------------------------------------------------------------
 # NodeEmbeddings (src/lib.rs:2535-2894)
 # --------------------------------------------------------------------------------------
@@ -375,6 +434,52 @@ class NodeEmbeddings:
 
     def __repr__(self):
         return f"NodeEmbeddings<Nodes={self.len()}, Dims={self.dims()}>"
+
+    def save(self, path: str, comp_level: Optional[int] = None):
+        """NodeEmbeddings.save (src/lib.rs:2828-2840, EmbeddingWriter io.rs:24-78): one line per
+        node, node_type<TAB>name<TAB>[v1,v2,...] with the shortest round-trip f32 digits;
+        `.gz` paths are gzipped."""
+        with _open_text_write(path, comp_level) as fh:
+            for i in range(self.len()):
+                t, n = self.vocab_.name(i)
+                vals = ",".join(str(x) for x in self.embeddings[i])  # np.float32: shortest digits
+                fh.write(f"{t}\t{n}\t[{vals}]\n")
+
+    @staticmethod
+    def load(path: str, distance: Optional[str] = None, filter_type=None, chunk_size: Optional[int] = None,
+             skip_rows: Optional[int] = None) -> "NodeEmbeddings":
+        """NodeEmbeddings.load (src/lib.rs:2868-2894, EmbeddingReader::load io.rs:135-200):
+        node_type<TAB>name<TAB>[v1,...] lines, optionally only the node types in filter_type
+        (a string or a list); a repeated node or a different width is an error."""
+        types = None
+        if filter_type is not None:
+            types = {filter_type} if isinstance(filter_type, str) else set(filter_type)
+        vocab = Vocab()
+        rows: List[np.ndarray] = []
+        with _open_text(path) as fh:
+            for i, line in enumerate(fh):
+                if i < (skip_rows or 0):
+                    continue
+                line = line.rstrip("\n")
+                if types is not None:
+                    k = line.find("\t")
+                    if k < 0 or line[:k] not in types:
+                        continue
+                pieces = line.split("\t")
+                if len(pieces) != 3:
+                    raise ValueError(f"Error parsing line: {line}")
+                try:
+                    emb = np.array([np.float32(x.strip()) for x in pieces[2][1:-1].split(",")], dtype=np.float32)
+                except ValueError:
+                    raise ValueError(f"Error parsing line: {line}") from None
+                nid = vocab.get_or_insert(pieces[0], pieces[1])
+                if nid < len(rows):
+                    raise KeyError(f"found duplicate node at {len(rows)}!")
+                if rows and len(emb) != len(rows[0]):
+                    raise ValueError("Embeddings have different sizes!")
+                rows.append(emb)
+        table = np.stack(rows) if rows else np.zeros((0, 0), dtype=np.float32)
+        return NodeEmbeddings(vocab, table, distance or Distance.Cosine)
 
 
 # --------------------------------------------------------------------------------------
@@ -545,14 +650,69 @@ class EmbeddingPropagator:
 # --------------------------------------------------------------------------------------
 # Inference aggregate (src/lib.rs:1766-2207, aggregator.rs:34-47)
 # --------------------------------------------------------------------------------------
+def unigram_probability(fs: "FeatureSet") -> np.ndarray:
+    """UnigramProbability::new (aggregator.rs:55-72): count of each feature id over every
+    node's feature list / total, in f32."""
+    ip, ids = fs.to_csr()
+    counts = np.bincount(ids.astype(np.int64), minlength=fs.num_features())
+    total = int(counts.sum())
+    return counts.astype(np.float32) / np.float32(max(total, 1))
+
+
 class FeatureAggregator:
-    def __init__(self, kind: str, num_heads: int = 0, d_k: int = 0, window: Optional[int] = None):
+    def __init__(self, kind: str, num_heads: int = 0, d_k: int = 0, window: Optional[int] = None,
+                 alpha: float = 0.0, vocab: Optional[Vocab] = None, unigrams: Optional[np.ndarray] = None):
         self.kind = kind
         self.num_heads, self.d_k, self.window = num_heads, d_k, window
+        self.alpha, self.vocab, self.unigrams = alpha, vocab, unigrams
 
     @staticmethod
     def Averaged():
         return FeatureAggregator("averaged")
+
+    @staticmethod
+    def Weighted(alpha: float, fs: "FeatureSet"):
+        """lib.rs:1855-1861: SIF weights alpha / (alpha + p(f)) from fs's unigram counts."""
+        return FeatureAggregator("weighted", alpha=float(alpha), vocab=fs.feature_vocab, unigrams=unigram_probability(fs))
+
+    def save(self, path: str):
+        """lib.rs:1876-1911: the aggregator as text (Weighted keeps its unigram table)."""
+        with open(path, "w") as fh:
+            if self.kind == "averaged":
+                fh.write("Averaged\n")
+            elif self.kind == "attention":
+                fh.write(f"Attention\n{self.num_heads}\n{self.d_k}\n{self.window or 0}\n")
+            else:
+                fh.write(f"Weighted\n{self.alpha}\n")
+                for i, p in enumerate(self.unigrams):
+                    if i >= len(self.vocab):
+                        break
+                    t, n = self.vocab.name(i)
+                    fh.write(f"{t}\t{n}\t{float(p)}\n")
+
+    @staticmethod
+    def load(path: str) -> "FeatureAggregator":
+        """lib.rs:1925-2000."""
+        with open(path) as fh:
+            kind = fh.readline().rstrip("\n")
+            if kind == "Averaged":
+                return FeatureAggregator.Averaged()
+            if kind == "Attention":
+                h, dk, win = (int(fh.readline().strip()) for _ in range(3))
+                return FeatureAggregator.Attention(h, dk, None if win == 0 else win)
+            if kind == "Weighted":
+                alpha = float(fh.readline().strip())
+                vocab, p_w = Vocab(), []
+                for line in fh:
+                    pieces = line.rstrip("\n").split("\t")
+                    if len(pieces) != 3:
+                        raise ValueError("Malformed feature line! Need node_type<TAB>name<TAB>weight ...")
+                    i = vocab.get_or_insert(pieces[0], pieces[1])
+                    if i < len(p_w):
+                        raise ValueError(f"Duplicate feature found:{pieces[0]} {pieces[1]}")
+                    p_w.append(float(pieces[2]))
+                return FeatureAggregator("weighted", alpha=alpha, vocab=vocab, unigrams=np.asarray(p_w, np.float32))
+        raise ValueError(f"unknown aggregator type {kind!r}")
 
     @staticmethod
     def Attention(num_heads: int, d_k: int, window: Optional[int] = None):
@@ -564,19 +724,22 @@ class FeatureAggregator:
     def __repr__(self):
         if self.kind == "attention":
             return f"FeatureAggregator<Attention<num_heads={self.num_heads},d_k={self.d_k},window=None>"
+        if self.kind == "weighted":
+            return f"FeatureAggregator<Weighted<alpha={self.alpha}>>"
         return "FeatureAggregator<Averaged>"
 
 
 class NodeEmbedder:
     def __init__(self, feat_agg: FeatureAggregator):
-        if feat_agg.kind not in ("averaged", "attention"):
-            raise ValueError("only FeatureAggregator.Averaged and .Attention are supported by this build")
+        if feat_agg.kind not in ("averaged", "attention", "weighted"):
+            raise ValueError(f"unknown aggregator {feat_agg.kind}")
         self.feat_agg = feat_agg
 
     def embed_feature_set(self, feat_set: FeatureSet, feature_embeddings: NodeEmbeddings,
                           device: int = 0) -> NodeEmbeddings:
         """lib.rs:2092-2124: node embedding from its (known) feature rows, on the GPU: their
-        mean (Averaged) or Full attention over them with count 1 each (Attention)."""
+        mean (Averaged), SIF-weighted mean (Weighted) or Full attention over them with
+        count 1 each (Attention)."""
         ip, ids = feat_set.to_csr()
         # translate FeatureSet feature ids into the embedding vocabulary, dropping unknown ones
         fv, ev = feat_set.feature_vocab, feature_embeddings.vocab_
@@ -608,5 +771,12 @@ class NodeEmbedder:
             # a one-node placeholder feature list: only the table is needed
             ctx.set_features(np.array([0, 1], dtype=np.uint64), np.array([0], dtype=np.uint32), table.shape[0])
             ctx.set_table(table)
-            out = ctx.embed_attention(t_ip, t_ids) if fa.kind == "attention" else ctx.embed_averaged(t_ip, t_ids)
+            if fa.kind == "attention":
+                out = ctx.embed_attention(t_ip, t_ids)
+            elif fa.kind == "weighted":
+                # the unigram table is indexed by the translated (embedding) feature ids, as
+                # WeightedAggregator::construct does (aggregator.rs:103-108)
+                out = ctx.embed_weighted(t_ip, t_ids, fa.unigrams, fa.alpha)
+            else:
+                out = ctx.embed_averaged(t_ip, t_ids)
         return NodeEmbeddings(feat_set.vocab_, out, Distance.Cosine)

@@ -1163,6 +1163,42 @@ int ep_embed_averaged(ep_ctx* c, uint64_t n_nodes, const uint64_t* f_indptr, con
   return EP_OK;
 }
 
+int ep_embed_weighted(ep_ctx* c, uint64_t n_nodes, const uint64_t* f_indptr, const uint32_t* f_ids,
+                      const float* p_w, uint32_t n_p, float alpha, float* out) {
+  if (!c || !f_indptr || !out || !p_w) return fail(EP_EINVAL, "NULL argument");
+  if (!c->table_ready) return fail(EP_ESTATE, "no table");
+  if (!(alpha > 0.f) || !std::isfinite(alpha)) return fail(EP_EINVAL, "alpha must be finite and > 0");
+  const uint64_t tot = f_indptr[n_nodes];
+  for (uint64_t i = 0; i < n_nodes; ++i)
+    if (f_indptr[i + 1] < f_indptr[i]) return fail(EP_EINVAL, "f_indptr must be non-decreasing");
+  for (uint64_t i = 0; i < tot; ++i)
+    if (f_ids[i] >= c->n_vocab || f_ids[i] >= n_p) return fail(EP_EINVAL, "feature id out of range");
+  for (uint32_t i = 0; i < n_p; ++i)
+    if (!(p_w[i] >= 0.f)) return fail(EP_EINVAL, "unigram probabilities must be >= 0");
+  EP_HIP(hipSetDevice(c->device));
+  const size_t W = row_width(c->cfg);
+  uint64_t* dp = nullptr;
+  uint32_t* di = nullptr;
+  float* dw = nullptr;
+  float* dout = nullptr;
+  EP_TRY(c->alloc(&dp, n_nodes + 1));
+  EP_TRY(c->alloc(&di, tot));
+  EP_TRY(c->alloc(&dw, n_p));
+  EP_TRY(c->alloc(&dout, (size_t)n_nodes * W));
+  EP_HIP(hipMemcpy(dp, f_indptr, (n_nodes + 1) * sizeof(uint64_t), hipMemcpyHostToDevice));
+  if (tot) EP_HIP(hipMemcpy(di, f_ids, tot * sizeof(uint32_t), hipMemcpyHostToDevice));
+  EP_HIP(hipMemcpy(dw, p_w, (size_t)n_p * sizeof(float), hipMemcpyHostToDevice));
+  refresh_base(c);
+  EP_HIP(ep::launch_embed_weighted(c->base, dp, di, dw, alpha, n_nodes, dout, c->stream));
+  EP_HIP(hipStreamSynchronize(c->stream));
+  EP_HIP(hipMemcpy(out, dout, (size_t)n_nodes * W * sizeof(float), hipMemcpyDeviceToHost));
+  c->release(dp);
+  c->release(di);
+  c->release(dw);
+  c->release(dout);
+  return EP_OK;
+}
+
 int ep_embed_attention(ep_ctx* c, uint64_t n_nodes, const uint64_t* f_indptr, const uint32_t* f_ids, float* out) {
   if (!c || !f_indptr || !out) return fail(EP_EINVAL, "NULL argument");
   if (!c->table_ready) return fail(EP_ESTATE, "no table");

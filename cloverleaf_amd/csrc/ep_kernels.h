@@ -199,6 +199,8 @@ hipError_t launch_init_table(const StepConfig& c, uint64_t seed, hipStream_t s);
 hipError_t launch_fill_bf16(const float* src, uint16_t* dst, uint64_t n, uint32_t D, uint32_t stride, hipStream_t s);
 hipError_t launch_embed_averaged(const StepConfig& c, const uint64_t* findptr, const uint32_t* fids,
                                  uint64_t n_nodes, float* out, hipStream_t s);
+hipError_t launch_embed_weighted(const StepConfig& c, const uint64_t* findptr, const uint32_t* fids,
+                                 const float* p_w, float alpha, uint64_t n_nodes, float* out, hipStream_t s);
 
 bool dims_supported(int d);
 // Row widths of the table-row kernels (init, update, averaged embedding): the mean

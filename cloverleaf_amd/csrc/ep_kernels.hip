@@ -1850,6 +1850,53 @@ __global__ __launch_bounds__(256) void k_embed_avg(const T* __restrict__ table, 
   }
 }
 
+// WeightedAggregator::construct (aggregator.rs:98-116): node = sum_f w_f E_f / sum_f w_f
+// over the listed features, w_f = alpha / (alpha + p_w[f]) (SIF weights from unigram
+// probabilities); nodes without features keep a zero row.
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_embed_weighted(const T* __restrict__ table, const uint64_t* __restrict__ findptr,
+                                                        const uint32_t* __restrict__ fids, const float* __restrict__ p_w,
+                                                        float alpha, uint64_t n_nodes, float* __restrict__ out) {
+  constexpr int PL = RowShape<D>::PL;
+  constexpr int U = 8;
+  const int lane = threadIdx.x & 63;
+  const uint64_t gw = (uint64_t)blockIdx.x * NW + (threadIdx.x >> 6);
+  const uint64_t nwaves = (uint64_t)gridDim.x * NW;
+  for (uint64_t node = gw; node < n_nodes; node += nwaves) {
+    float acc[PL];
+#pragma unroll
+    for (int q = 0; q < PL; ++q) acc[q] = 0.f;
+    float wsum = 0.f;
+    const uint64_t fs = findptr[node], fe = findptr[node + 1];
+    for (uint64_t base = fs; base < fe; base += EP_WAVE) {
+      const int n = (int)((fe - base) < (uint64_t)EP_WAVE ? (fe - base) : (uint64_t)EP_WAVE);
+      const uint32_t myf = lane < n ? fids[base + lane] : 0u;
+      const float myw = lane < n ? alpha / (alpha + p_w[myf]) : 0.f;
+      for (int j = 0; j < n; j += U) {
+        float r[U][PL], w[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int t = (j + u < n) ? (j + u) : (n - 1);
+          const uint32_t f = __shfl(myf, t, EP_WAVE);
+          w[u] = __shfl(myw, t, EP_WAVE);
+          load_slice<D>(table + (size_t)f * D, lane, r[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (j + u < n) {
+            wsum += w[u];
+#pragma unroll
+            for (int q = 0; q < PL; ++q) acc[q] += w[u] * r[u][q];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < PL; ++q) acc[q] = fe > fs ? acc[q] / wsum : 0.f;
+    store_slice<D>(out + node * D, lane, acc);
+  }
+}
+
 // ------------------------------------------------------------------------------------
 // Dispatch
 // ------------------------------------------------------------------------------------
@@ -2052,6 +2099,19 @@ hipError_t launch_init_table(const StepConfig& c, uint64_t seed, hipStream_t s) 
 
 hipError_t launch_fill_bf16(const float* src, uint16_t* dst, uint64_t n, uint32_t D, uint32_t stride, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_bf16, dim3(2048), dim3(256), 0, s, src, dst, n, D, stride);
+  return hipGetLastError();
+}
+
+hipError_t launch_embed_weighted(const StepConfig& c, const uint64_t* findptr, const uint32_t* fids,
+                                 const float* p_w, float alpha, uint64_t n_nodes, float* out, hipStream_t s) {
+  EP_ROW_DISPATCH(c.dims, {
+    if (c.bf16)
+      hipLaunchKernelGGL((k_embed_weighted<D, uint16_t>), dim3(2048), dim3(256), 0, s, (const uint16_t*)c.table,
+                         findptr, fids, p_w, alpha, n_nodes, out);
+    else
+      hipLaunchKernelGGL((k_embed_weighted<D, float>), dim3(2048), dim3(256), 0, s, (const float*)c.table, findptr,
+                         fids, p_w, alpha, n_nodes, out);
+  });
   return hipGetLastError();
 }
 

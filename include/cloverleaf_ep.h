@@ -198,6 +198,14 @@ int ep_bench_steps(ep_ctx* ctx, uint32_t warmup, uint32_t steps, int instrument,
  * aggregator.rs:34-47): out[n_nodes*width] = mean of each node's feature rows.  */
 int ep_embed_averaged(ep_ctx* ctx, uint64_t n_nodes, const uint64_t* f_indptr, const uint32_t* f_ids, float* out);
 
+/* NodeEmbedder(FeatureAggregator.Weighted(alpha, fs)).embed_feature_set
+ * (src/lib.rs:2092-2124, 2019-2021; WeightedAggregator, aggregator.rs:82-116):
+ * out[n_nodes*width] = sum_f w_f E_f / sum_f w_f over each node's listed features,
+ * w_f = alpha / (alpha + p_w[f]), p_w[n_p] the unigram probabilities
+ * (UnigramProbability, aggregator.rs:50-79); nodes without features get a zero row. */
+int ep_embed_weighted(ep_ctx* ctx, uint64_t n_nodes, const uint64_t* f_indptr, const uint32_t* f_ids,
+                      const float* p_w, uint32_t n_p, float alpha, float* out);
+
 /* NodeEmbedder(FeatureAggregator.Attention(heads, d_k, None)).embed_feature_set
  * (src/lib.rs:2092-2124, 2028-2036; AttentionAggregator, aggregator.rs:124-154):
  * out[n_nodes*dims] = Full attention over each node's listed features, count 1

@@ -682,6 +682,20 @@ def averaged_node_embeddings(table, f_indptr, f_ids) -> np.ndarray:
     return out
 
 
+def weighted_node_embeddings(table, f_indptr, f_ids, p_w, alpha) -> np.ndarray:
+    """WeightedAggregator::construct (aggregator.rs:98-116), float64: sum_f w_f E_f / sum_f w_f
+    with w_f = alpha / (alpha + p_w[f]) over the listed features; featureless nodes zero."""
+    n = len(f_indptr) - 1
+    out = np.zeros((n, table.shape[1]), dtype=np.float64)
+    p_w = np.asarray(p_w, dtype=np.float64)
+    for i in range(n):
+        fs = np.asarray(f_ids[f_indptr[i]:f_indptr[i + 1]], dtype=np.int64)
+        if len(fs):
+            w = alpha / (alpha + p_w[fs])
+            out[i] = (w[:, None] * table[fs].astype(np.float64)).sum(axis=0) / w.sum()
+    return out
+
+
 def roc_auc(pos_scores: np.ndarray, neg_scores: np.ndarray) -> float:
     """Mann-Whitney ROC-AUC with average ranks for ties."""
     pos = np.asarray(pos_scores, dtype=np.float64)

@@ -442,3 +442,39 @@ def test_learn_reduces_loss_and_embed_averaged():
     ref = O.averaged_node_embeddings(emb.embeddings.astype(np.float64), w.f_indptr.astype(np.int64),
                                      w.f_ids.astype(np.int64))
     assert rel(node.embeddings, ref) < 1e-5
+
+
+@pytest.mark.parametrize("precision", [N.EP_PREC_FP32, N.EP_PREC_BF16])
+def test_embed_weighted_matches_oracle(precision):
+    # WeightedAggregator::construct (aggregator.rs:98-116): SIF weights alpha / (alpha + p_f)
+    prob = make_problem(seed=8, n=500, V=300, D=64, fmax=12)
+    ctx = make_ctx(prob, 5, 3, 0.5, precision=precision)
+    p_w = np.bincount(prob.f_ids.astype(np.int64), minlength=prob.V).astype(np.float32)
+    p_w /= np.float32(p_w.sum())
+    ip = np.concatenate([prob.f_indptr, prob.f_indptr[-1:]]).astype(np.uint64)  # plus a featureless node
+    out = ctx.embed_weighted(ip, prob.f_ids, p_w, 1e-3)
+    table = prob.table
+    if precision == N.EP_PREC_BF16:
+        u = table.view(np.uint32)
+        table = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).view(np.float32)
+    ref = O.weighted_node_embeddings(table.astype(np.float64), ip.astype(np.int64), prob.f_ids.astype(np.int64),
+                                     p_w, 1e-3)
+    assert rel(out, ref) < 1e-5
+    assert np.all(out[-1] == 0)
+    ctx.close()
+
+
+def test_node_embedder_weighted_facade():
+    import cloverleaf_amd as C
+    from cloverleaf_amd import synth
+
+    w = synth.make_workload("ep-cfg1-10k", scale=0.1)
+    g = C.Graph.from_csr(w.indptr, w.indices)
+    fs = C.FeatureSet.from_csr(g, w.f_indptr, w.f_ids, w.vocab)
+    rng = np.random.default_rng(0)
+    emb = C.NodeEmbeddings(fs.feature_vocab, rng.normal(size=(w.vocab, 16)).astype(np.float32), C.Distance.Cosine)
+    agg = C.FeatureAggregator.Weighted(1e-3, fs)
+    node = C.NodeEmbedder(agg).embed_feature_set(fs, emb)
+    ref = O.weighted_node_embeddings(emb.embeddings.astype(np.float64), w.f_indptr.astype(np.int64),
+                                     w.f_ids.astype(np.int64), agg.unigrams, 1e-3)
+    assert rel(node.embeddings, ref) < 1e-5

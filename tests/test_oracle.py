@@ -665,3 +665,22 @@ def test_averaged_node_embeddings_closed_form():
     table = np.array([[1.0, 1.0], [3.0, -1.0]])
     out = O.averaged_node_embeddings(table, np.array([0, 2, 2, 3]), np.array([0, 1, 1]))
     assert np.allclose(out, [[2.0, 0.0], [0.0, 0.0], [3.0, -1.0]])
+
+
+# --------------------------------------------------------------------------------------
+# SIF-weighted inference aggregate: aggregator.rs:82-116
+# --------------------------------------------------------------------------------------
+def test_weighted_aggregate_closed_form():
+    table = np.array([[1.0, 0.0], [0.0, 3.0], [5.0, 5.0]])
+    p_w = np.array([0.5, 0.25, 0.25])
+    f_indptr = np.array([0, 3, 3, 4])  # node 0: [0, 1, 1]; node 1: none; node 2: [2]
+    f_ids = np.array([0, 1, 1, 2])
+    alpha = 0.5
+    w0, w1 = 0.5 / 1.0, 0.5 / 0.75  # alpha / (alpha + p)
+    out = O.weighted_node_embeddings(table, f_indptr, f_ids, p_w, alpha)
+    assert np.allclose(out[0], (w0 * table[0] + 2 * w1 * table[1]) / (w0 + 2 * w1))
+    assert np.all(out[1] == 0)            # no features: the row stays zero
+    assert np.allclose(out[2], table[2])  # one feature: its row
+    # a large alpha makes the weights equal: the plain mean of the listed rows
+    big = O.weighted_node_embeddings(table, f_indptr, f_ids, p_w, 1e12)
+    assert np.allclose(big, O.averaged_node_embeddings(table, f_indptr, f_ids))
