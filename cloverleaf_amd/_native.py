@@ -21,6 +21,9 @@ EP_LW_NONE, EP_LW_LOG, EP_LW_EXPONENTIAL = 0, 1, 2
 EP_PREC_FP32, EP_PREC_BF16, EP_PREC_BF16_MASTER = 0, 1, 2
 EP_GRAD_FIRST_LEAF, EP_GRAD_EXACT = 0, 1
 EP_SCATTER_SORT, EP_SCATTER_ATOMIC = 0, 1
+EP_LOSS_MARGIN, EP_LOSS_CONTRASTIVE, EP_LOSS_STARSPACE, EP_LOSS_RANK, EP_LOSS_RANKSPACE, EP_LOSS_PPR = range(6)
+LOSS_KINDS = {"margin": EP_LOSS_MARGIN, "contrastive": EP_LOSS_CONTRASTIVE, "starspace": EP_LOSS_STARSPACE,
+              "rank": EP_LOSS_RANK, "rankspace": EP_LOSS_RANKSPACE, "ppr": EP_LOSS_PPR}
 
 ERROR_NAMES = {-1: "EP_EINVAL", -2: "EP_ESTATE", -3: "EP_EHIP", -4: "EP_ENOMEM", -5: "EP_ECAPACITY"}
 
@@ -56,7 +59,14 @@ class EPConfig(Structure):
         ("attention_dk", c_uint32),
         ("attention_heads", c_uint32),
         ("context_window", c_uint32),
+        ("loss_kind", c_int32),
+        ("loss_param2", c_float),
     ]
+
+
+def list_cap(cfg: "EPConfig") -> int:
+    """Positive-list capacity per anchor: max_nodes, or K random-walk endpoints (PPR)."""
+    return int(cfg.negatives) if cfg.loss_kind == EP_LOSS_PPR else int(cfg.max_nodes)
 
 
 def row_width(cfg: "EPConfig") -> int:
@@ -260,7 +270,7 @@ class Context:
     def sample_batch(self, anchors: np.ndarray, pass_: int):
         anchors = np.ascontiguousarray(anchors, dtype=np.uint32)
         n = len(anchors)
-        S, K = self.cfg.max_nodes, self.cfg.negatives
+        S, K = list_cap(self.cfg), self.cfg.negatives
         cnt = np.zeros(n, dtype=np.uint32)
         nbr = np.zeros((n, S), dtype=np.uint32)
         neg = np.zeros((n, K), dtype=np.uint32)
@@ -273,7 +283,7 @@ class Context:
         """One step on explicit samples.  Returns dict(loss, cons, feat, grad)."""
         anchors = np.ascontiguousarray(anchors, dtype=np.uint32)
         n = len(anchors)
-        S, K, D, W = self.cfg.max_nodes, self.cfg.negatives, self.cfg.dims, row_width(self.cfg)
+        S, K, D, W = list_cap(self.cfg), self.cfg.negatives, self.cfg.dims, row_width(self.cfg)
         nbr_cnt = np.ascontiguousarray(nbr_cnt, dtype=np.uint32)
         nbr = np.ascontiguousarray(nbr, dtype=np.uint32).reshape(n, S)
         neg = np.ascontiguousarray(neg, dtype=np.uint32).reshape(n, K)

@@ -7,7 +7,7 @@ Class and argument names, defaults and side effects follow the reference:
 * ``Graph``, ``GraphBuilder``, ``EdgeType``      src/lib.rs:292-502, 836-937
 * ``FeatureSet``                                 src/lib.rs:1432-1691, src/feature_store.rs
 * ``NodeEmbeddings``, ``Distance``               src/lib.rs:2535-2894
-* ``FeatureAggregator``, ``NodeEmbedder``        src/lib.rs:1766-2207 (Averaged only)
+* ``FeatureAggregator``, ``NodeEmbedder``        src/lib.rs:1766-2207 (Averaged, Weighted, Attention)
 
 The training step itself runs in libcloverleaf_ep.so (HIP kernels on the GPU);
 this module only converts names to integer CSR arrays and back.
@@ -24,6 +24,12 @@ from . import _native as N
 
 FQNode = Tuple[str, str]
 SEED = 20222022  # src/lib.rs:90
+
+
+def _f32_debug(x) -> str:
+    """An f32 the way Rust's `{:?}` prints it: shortest round-trip digits, at least one
+    fractional digit."""
+    return np.format_float_positional(np.float32(x), unique=True, trim="0")
 
 
 # --------------------------------------------------------------------------------------
@@ -570,8 +576,6 @@ class EmbeddingPropagator:
         loss = loss or EPLoss.margin(1.0, 1)
         mf_kind, mf = _parse_sample(max_features, "max_features")
         mn_kind, mn = _parse_sample(max_nodes, "max_nodes")
-        if loss.kind != "margin":
-            raise ValueError(f"loss {loss.kind} is not supported by this build (margin only)")
         if attention is not None and context_window is not None:
             raise ValueError("context_window (Sliding attention) is not supported by this build")
         if mf_kind != N.EP_SAMPLE_ALL:
@@ -586,7 +590,9 @@ class EmbeddingPropagator:
         cfg.dims = 100 if dims is None else int(dims)
         cfg.passes = 100 if passes is None else int(passes)
         cfg.negatives = loss.negatives
+        cfg.loss_kind = N.LOSS_KINDS[loss.kind]
         cfg.gamma = loss.params[0]
+        cfg.loss_param2 = float(loss.params[1]) if len(loss.params) > 1 else 0.0
         cfg.valid_pct = 0.1 if valid_pct is None else float(valid_pct)
         cfg.seed = SEED if seed is None else int(seed)
         cfg.max_nodes_kind, cfg.max_nodes = mn_kind, mn
@@ -612,9 +618,14 @@ class EmbeddingPropagator:
 
     def __repr__(self):
         c = self.cfg
-        return (f"EmbeddingPropagation {{ alpha: {c.alpha}, loss: MarginLoss({c.gamma}, {c.negatives}), "
+        k, g, p2, K = c.loss_kind, _f32_debug(c.gamma), _f32_debug(c.loss_param2), c.negatives
+        # the Debug form of Loss (src/algos/ep/loss.rs:15-36)
+        loss = {N.EP_LOSS_MARGIN: f"MarginLoss({g}, {K})", N.EP_LOSS_CONTRASTIVE: f"Contrastive({g}, {p2}, {K})",
+                N.EP_LOSS_STARSPACE: f"StarSpace({g}, {K})", N.EP_LOSS_RANK: f"RankLoss({g}, {K})",
+                N.EP_LOSS_RANKSPACE: f"RankSpace({g}, {K})", N.EP_LOSS_PPR: f"PPR({g}, {K}, {p2})"}[k]
+        return (f"EmbeddingPropagation {{ alpha: {_f32_debug(c.alpha)}, loss: {loss}, "
                 f"batch_size: {c.batch_size}, d_model: {c.dims}, passes: {c.passes}, seed: {c.seed}, "
-                f"valid_pct: {c.valid_pct} }}")
+                f"valid_pct: {_f32_debug(c.valid_pct)} }}")
 
     def learn_features(self, graph: Graph, features: FeatureSet,
                        feature_embeddings: Optional[NodeEmbeddings] = None) -> NodeEmbeddings:

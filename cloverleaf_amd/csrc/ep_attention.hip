@@ -37,6 +37,7 @@
 
 #include "ep_device.h"
 #include "ep_kernels.h"
+#include "ep_loss.h"
 
 namespace ep {
 
@@ -547,6 +548,94 @@ __global__ __launch_bounds__(256) void k_att_loss(StepConfig c, StepState* st) {
     const int t = (int)lane + 64 * r;
     return t < dm ? c.bout[(size_t)(g0 + 1u + cnt + k) * dm + t] : 0.f;
   };
+  if (c.loss_kind != 0 && c.loss_kind != 5) {
+    // losses other than margin: norms and dots, coefficients (lane 0), adjoints
+    __shared__ float s_nrm[NW][32], s_dan[NW][32], s_ca[NW][32], s_cns[NW][32], s_cna[NW][32], s_cp[NW][2];
+    float* nrm = s_nrm[w];
+    float* dan = s_dan[w];
+    float* ca = s_ca[w];
+    float* cns = s_cns[w];
+    float* cna = s_cna[w];
+    float* cp = s_cp[w];
+    float qa = 0.f, qp = 0.f, dap = 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) { qa += a[r] * a[r]; qp += p[r] * p[r]; dap += a[r] * p[r]; }
+    qa = wave_sum(qa); qp = wave_sum(qp); dap = wave_sum(dap);
+    if (lane == 0) { nrm[0] = sqrtf(qa); nrm[1] = sqrtf(qp); }
+    for (uint32_t k = 0; k < K; ++k) {
+      float qn = 0.f, dn = 0.f;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) { const float x = neg_val(k, r); qn += x * x; dn += a[r] * x; }
+      qn = wave_sum(qn); dn = wave_sum(dn);
+      if (lane == 0) { nrm[2 + k] = sqrtf(qn); dan[k] = dn; }
+    }
+    __builtin_amdgcn_wave_barrier();
+    float Lo = 0.f;
+    uint32_t reach = 0;
+    if (lane == 0) Lo = other_loss(c.loss_kind, c.gamma, c.loss_p2, K, nrm, dap, dan, ca, cp, cns, cna, &reach);
+    __builtin_amdgcn_wave_barrier();
+    Lo = __shfl(Lo, 0, EP_WAVE);
+    reach = __shfl(reach, 0, EP_WAVE);
+    float div = 1.f;
+    if (c.lw_kind == 1) div = logf(1.f + (float)deg);
+    else if (c.lw_kind == 2) div = powf((float)deg, c.lw_exp);
+    const float Lw = Lo / div;
+    const bool contrib = (Lw > 0.f) && !isnan(Lw);
+    if (!contrib) reach = 0u;
+    if (lane == 0) {
+      c.loss[b] = contrib ? Lw : 0.f;
+      if (c.pass_loss) c.pass_loss[st->anchor_base + b] = contrib ? Lw : 0.f;
+    }
+    if (c.xout) {
+      float* xo = c.xout + (size_t)b * NC * dm;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int t = (int)lane + 64 * r;
+        if (t < dm) {
+          xo[t] = a[r];
+          xo[dm + t] = p[r];
+          for (uint32_t k = 0; k < K; ++k) xo[(size_t)(2u + k) * dm + t] = neg_val(k, r);
+        }
+      }
+    }
+    if (!c.train) return;
+    const float scale = 1.f / div;
+    for (uint32_t ci = 0; ci < NC; ++ci) {
+      bool e = (reach >> ci) & 1u;
+      float g[4];
+      bool ok = true;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        float x;
+        if (ci == 0u) {
+          x = ca[0] * a[r] + ca[1] * p[r];
+          for (uint32_t k = 0; k < K; ++k) x += ca[2 + k] * neg_val(k, r);
+        } else if (ci == 1u) {
+          x = cp[0] * a[r] + cp[1] * p[r];
+        } else {
+          x = cns[ci - 2u] * neg_val(ci - 2u, r) + cna[ci - 2u] * a[r];
+        }
+        g[r] = x * scale;
+        if ((int)lane + 64 * r < dm) ok = ok && isfinite(g[r]);
+      }
+      e = e && !wave_any(!ok);
+      if (e) {
+        // block adjoints: a and n_k directly, each neighbour block dL/dp / cnt
+        const uint32_t first = ci == 0u ? g0 : (ci == 1u ? g0 + 1u : g0 + 1u + cnt + (ci - 2u));
+        const uint32_t nb = ci == 1u ? cnt : 1u;
+        const float f = ci == 1u ? 1.0f / (float)cnt : 1.0f;
+        for (uint32_t j = 0; j < nb; ++j) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int t = (int)lane + 64 * r;
+            if (t < dm) c.badj[(size_t)(first + j) * dm + t] = g[r] * f;
+          }
+        }
+      }
+      if (lane == 0) c.elig[(size_t)b * NC + ci] = e ? 1 : 0;
+    }
+    return;
+  }
   float sq = 0.f;
 #pragma unroll
   for (int r = 0; r < 4; ++r) sq += (p[r] - a[r]) * (p[r] - a[r]);

@@ -153,7 +153,16 @@ uint32_t row_width(const ep_config& cfg) {
   return cfg.attention_dk ? cfg.attention_heads * (2u * cfg.attention_dk + cfg.dims) : cfg.dims;
 }
 
+bool ppr_loss(const ep_config& cfg) { return cfg.loss_kind == EP_LOSS_PPR; }
+
+// Positive-list capacity of the step buffers: max_nodes (Fixed), the K walk endpoints
+// (PPR), 0 with every neighbour (the CSR rows are read in place).
+uint32_t list_cap(const ep_config& cfg) {
+  return ppr_loss(cfg) ? cfg.negatives : (cfg.max_nodes_kind == EP_SAMPLE_FIXED ? cfg.max_nodes : 0u);
+}
+
 uint32_t neighbour_cap(const ep_ctx* c) {
+  if (ppr_loss(c->cfg)) return c->cfg.negatives;
   return c->cfg.max_nodes_kind == EP_SAMPLE_FIXED ? c->cfg.max_nodes : c->deg_max;
 }
 
@@ -172,10 +181,10 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   const ep_config& cfg = c->cfg;
   const uint32_t K = cfg.negatives, NC = K + 2, D = row_width(cfg);
   const bool att = cfg.attention_dk != 0;
-  const uint32_t S = cfg.max_nodes_kind == EP_SAMPLE_FIXED ? cfg.max_nodes : 0;
+  const uint32_t S = list_cap(cfg);
   // key capacity: sum of the largest `batch` example sizes
   uint64_t keycap;
-  if (cfg.max_nodes_kind == EP_SAMPLE_FIXED) {
+  if (cfg.max_nodes_kind == EP_SAMPLE_FIXED || ppr_loss(cfg)) {
     keycap = (uint64_t)batch * example_rows_max(c);
   } else {
     std::vector<uint32_t> d = c->degrees;
@@ -277,7 +286,7 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   }
   b.batch = batch;
   b.S = S;
-  b.all_nbrs = cfg.max_nodes_kind == EP_SAMPLE_ALL ? 1 : 0;
+  b.all_nbrs = (cfg.max_nodes_kind == EP_SAMPLE_ALL && !ppr_loss(cfg)) ? 1 : 0;
   b.key_cap = (uint32_t)keycap;
   b.sort_grid = sort_grid;
   b.long_cap = long_cap;
@@ -350,6 +359,8 @@ void refresh_base(ep_ctx* c) {
   b.att_dm = (int)cfg.dims;
   b.K = cfg.negatives;
   b.gamma = cfg.gamma;
+  b.loss_kind = cfg.loss_kind;
+  b.loss_p2 = cfg.loss_param2;
   b.lw_kind = cfg.loss_weighting;
   b.lw_exp = cfg.loss_weight_exp;
   b.grad_mode = cfg.grad_mode;
@@ -583,6 +594,11 @@ int validate_config(const ep_config* cfg) {
     return fail(EP_EINVAL, "max_nodes must be All or Fixed");
   if (cfg->max_features_kind != EP_SAMPLE_ALL) return fail(EP_EINVAL, "max_features sampling is not supported");
   if (cfg->hard_negatives != 0) return fail(EP_EINVAL, "hard_negatives is not supported");
+  if (cfg->loss_kind < EP_LOSS_MARGIN || cfg->loss_kind > EP_LOSS_PPR) return fail(EP_EINVAL, "loss_kind");
+  if (!std::isfinite(cfg->gamma) || !std::isfinite(cfg->loss_param2))
+    return fail(EP_EINVAL, "loss parameters must be finite");
+  if (cfg->loss_kind == EP_LOSS_PPR && !(cfg->loss_param2 >= 0.f && cfg->loss_param2 <= 1.f))
+    return fail(EP_EINVAL, "ppr restart_p must be in [0, 1]");
   if (!(cfg->noise >= 0.f) || !std::isfinite(cfg->noise)) return fail(EP_EINVAL, "noise must be finite and >= 0");
   if (cfg->weighted_neighbor_sampling || cfg->weighted_neighbor_averaging)
     return fail(EP_EINVAL, "weighted neighbour sampling/averaging is not supported");
@@ -921,7 +937,8 @@ int ep_learn(ep_ctx* c, ep_progress_fn progress, void* user) {
 int ep_sample_batch(ep_ctx* c, const uint32_t* anchors, uint32_t n, uint32_t pass, uint32_t* nbr_cnt, uint32_t* nbr,
                     uint32_t* neg) {
   if (!c || !anchors || !nbr_cnt || !nbr || !neg || n == 0) return fail(EP_EINVAL, "bad argument");
-  if (c->cfg.max_nodes_kind != EP_SAMPLE_FIXED) return fail(EP_EINVAL, "ep_sample_batch needs max_nodes Fixed");
+  if (c->cfg.max_nodes_kind != EP_SAMPLE_FIXED && !ppr_loss(c->cfg))
+    return fail(EP_EINVAL, "ep_sample_batch needs max_nodes Fixed");
   for (uint32_t i = 0; i < n; ++i) if (anchors[i] >= c->n_nodes) return fail(EP_EINVAL, "anchor out of range");
   EP_HIP(hipSetDevice(c->device));
   if (!c->split_set) { EP_TRY(ensure_split(c)); EP_TRY(upload_split(c)); }
@@ -950,10 +967,11 @@ int ep_step_explicit(ep_ctx* c, const uint32_t* anchors, uint32_t n, const uint3
                      const uint32_t* neg, uint32_t pass, float lr, int apply_update, float* loss, float* cons,
                      uint32_t* n_unique, uint32_t* feat, float* grad) {
   if (!c || !anchors || !nbr_cnt || !nbr || !neg || n == 0) return fail(EP_EINVAL, "bad argument");
-  if (c->cfg.max_nodes_kind != EP_SAMPLE_FIXED) return fail(EP_EINVAL, "ep_step_explicit needs max_nodes Fixed");
+  if (c->cfg.max_nodes_kind != EP_SAMPLE_FIXED && !ppr_loss(c->cfg))
+    return fail(EP_EINVAL, "ep_step_explicit needs max_nodes Fixed");
   if (!(lr >= 0.f)) return fail(EP_EINVAL, "lr must be >= 0");
   if (!c->d_indptr || !c->d_findptr) return fail(EP_ESTATE, "graph and features must be set first");
-  const uint32_t S = c->cfg.max_nodes, K = c->cfg.negatives, D = row_width(c->cfg), NC = K + 2;
+  const uint32_t S = list_cap(c->cfg), K = c->cfg.negatives, D = row_width(c->cfg), NC = K + 2;
   for (uint32_t i = 0; i < n; ++i) {
     if (anchors[i] >= c->n_nodes) return fail(EP_EINVAL, "anchor out of range");
     if (nbr_cnt[i] > S) return fail(EP_EINVAL, "nbr_cnt exceeds max_nodes");

@@ -49,6 +49,7 @@ enum RngPurpose : uint32_t {
   RNG_ROUNDING = 3,
   RNG_NOISE = 4,
   RNG_INIT = 5,
+  RNG_WALK = 6,
 };
 
 // Uniform integer in [0, n) from a 32-bit draw (multiply-shift; bias <= n / 2^32).

@@ -90,6 +90,9 @@ struct StepConfig {
   int all_nbrs;            // 1: use every neighbour (Sample::All)
   uint32_t K;              // negatives
   float gamma;
+  int loss_kind;           // EP_LOSS_*: 0 margin, 1 contrastive, 2 starspace, 3 rank,
+                           // 4 rankspace, 5 ppr (margin over random-walk positives)
+  float loss_p2;           // contrastive negative margin / ppr restart probability
   int lw_kind;             // 0 none, 1 log, 2 exponential
   float lw_exp;
   int grad_mode;           // 0 first_leaf, 1 exact
@@ -203,6 +206,8 @@ hipError_t launch_embed_weighted(const StepConfig& c, const uint64_t* findptr, c
                                  const float* p_w, float alpha, uint64_t n_nodes, float* out, hipStream_t s);
 
 bool dims_supported(int d);
+
+
 // Row widths of the table-row kernels (init, update, averaged embedding): the mean
 // model's widths and the attention row widths.
 bool row_dims_supported(int d);

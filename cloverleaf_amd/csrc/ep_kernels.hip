@@ -11,6 +11,7 @@
 // device-side counts.
 #include "ep_device.h"
 #include "ep_kernels.h"
+#include "ep_loss.h"
 
 namespace ep {
 
@@ -234,7 +235,8 @@ __global__ void k_step_begin(StepConfig c, StepState* st, float lr_override) {
 // (Floyd's algorithm; picks in LDS, the duplicate test is one ballot), all
 // neighbours when deg <= S; K negatives uniform with replacement from the pool; R_e =
 // feature occurrences of the example.  model.rs:354-377 + utils.rs:193-216 (Fixed(S))
-// and node_sampler.rs:185-188.  Draw i of a purpose is word i%4 of Philox block i/4.
+// and node_sampler.rs:185-188; the PPR loss takes random-walk endpoints instead of
+// neighbours.  Draw i of a purpose is word i%4 of Philox block i/4.
 // ------------------------------------------------------------------------------------
 __device__ inline uint32_t draw_word(uint32_t pos, uint32_t pass, uint32_t purpose, uint32_t i, uint32_t k0,
                                      uint32_t k1) {
@@ -264,7 +266,42 @@ __global__ __launch_bounds__(256) void k_sample(StepConfig c, StepState* st) {
   uint32_t* negs = c.neg + (size_t)b * c.K;
   const uint32_t k0 = (uint32_t)c.seed, k1 = (uint32_t)(c.seed >> 32);
   uint32_t cnt;
-  if (!c.explicit_samples) {
+  if (!c.explicit_samples && c.loss_kind == 5) {
+    // Loss::PPR positives (loss.rs:191-226): K random walks from v of at most 10 steps;
+    // after the first step a walk stops with probability restart_p at a node != v; a walk
+    // that ends on v falls back to a uniform neighbour of v; no neighbours: v itself.
+    uint32_t endp = 0;
+    bool some = false;
+    if ((uint32_t)lane < c.K) {
+      const uint32_t r0 = (uint32_t)lane * 32u;
+      uint32_t node = v;
+      for (uint32_t i = 1;; ++i) {
+        const uint64_t e0 = c.indptr[node];
+        const uint32_t dn = (uint32_t)(c.indptr[node + 1] - e0);
+        if (dn == 0u || i > 10u) break;
+        node = c.indices[e0 + uniform_below(draw_word(pos, pass, RNG_WALK, r0 + 2u * i, k0, k1), dn)];
+        if (i > 1u && uniform01(draw_word(pos, pass, RNG_WALK, r0 + 2u * i + 1u, k0, k1)) < c.loss_p2 && node != v)
+          break;
+      }
+      if (node != v) {
+        endp = node;
+        some = true;
+      } else if (deg > 0u) {
+        endp = c.indices[s0 + uniform_below(draw_word(pos, pass, RNG_WALK, r0 + 30u, k0, k1), deg)];
+        some = true;
+      }
+    }
+    const uint64_t m = __ballot(some);
+    if (some) out[__popcll(m & ((1ull << lane) - 1ull))] = endp;
+    cnt = (uint32_t)__popcll(m);
+    if (cnt == 0u) {
+      if (lane == 0) out[0] = v;
+      cnt = 1u;
+    }
+    for (uint32_t k = lane; k < c.K; k += 64u)
+      negs[k] = c.negpool[uniform_below(draw_word(pos, pass, RNG_NEGATIVES, k, k0, k1), c.n_negpool)];
+    if (lane == 0) c.nbr_cnt[b] = cnt;
+  } else if (!c.explicit_samples) {
     if (c.all_nbrs) {
       cnt = deg;
     } else if (deg <= c.S) {
@@ -462,6 +499,97 @@ __device__ inline void gather_lds_wide(const T* __restrict__ table, const uint32
   }
 }
 
+// Loss, adjoints and eligibility of one example for the losses other than margin (block
+// of k_forward; xc holds a, p, n_1..n_K, sn the occurrence totals).
+template <int D>
+__device__ inline void forward_other_loss(const StepConfig& c, StepState* st, uint32_t b, uint32_t deg, const float* xc,
+                                          const float* sn, uint8_t* s_el) {
+  constexpr int PL = RowShape<D>::PL;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint32_t K = c.K, NC = K + 2u;
+  const int base = lane * PL;
+  __shared__ float s_nrm[32], s_dot[32], s_ca[32], s_cns[32], s_cna[32], s_cp[2], s_scale;
+  __shared__ uint32_t s_reach;
+  __shared__ int s_contrib2;
+  for (uint32_t i = w; i < NC; i += NW) {  // |x_i| and a.x_i
+    float sq = 0.f, dt = 0.f;
+#pragma unroll
+    for (int q = 0; q < PL; ++q) {
+      if (base + q < D) {
+        const float x = xc[i * D + base + q];
+        sq += x * x;
+        dt += xc[base + q] * x;
+      }
+    }
+    sq = wave_sum(sq);
+    dt = wave_sum(dt);
+    if (lane == 0) { s_nrm[i] = sqrtf(sq); s_dot[i] = dt; }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t reach = 0;
+    const float L = other_loss(c.loss_kind, c.gamma, c.loss_p2, K, s_nrm, s_dot[1], s_dot + 2, s_ca, s_cp, s_cns,
+                               s_cna, &reach);
+    float div = 1.f;
+    if (c.lw_kind == 1) div = logf(1.f + (float)deg);
+    else if (c.lw_kind == 2) div = powf((float)deg, c.lw_exp);
+    const float Lw = L / div;
+    const int contrib = (Lw > 0.f) && !isnan(Lw);
+    s_contrib2 = contrib;
+    s_reach = contrib ? reach : 0u;
+    s_scale = 1.f / div;
+    c.loss[b] = contrib ? Lw : 0.f;
+    if (c.pass_loss) c.pass_loss[st->anchor_base + b] = contrib ? Lw : 0.f;
+  }
+  __syncthreads();
+  if (c.xout) {
+    float* xo = c.xout + (size_t)b * NC * D;
+    for (uint32_t t = tid; t < NC * D; t += blockDim.x) xo[t] = xc[t];
+  }
+  if (!c.train) return;
+  const uint32_t reach = s_reach;
+  const float scale = s_scale;
+  for (uint32_t ci = w; ci < NC; ci += NW) {
+    bool e = (reach >> ci) & 1u;
+    float g[PL];
+#pragma unroll
+    for (int q = 0; q < PL; ++q) g[q] = 0.f;
+    if (e) {
+#pragma unroll
+      for (int q = 0; q < PL; ++q) {
+        if (base + q < D) {
+          const float av = xc[base + q];
+          float x;
+          if (ci == 0u) {
+            x = 0.f;
+            for (uint32_t y = 0; y < NC; ++y) x += s_ca[y] * xc[y * D + base + q];
+          } else if (ci == 1u) {
+            x = s_cp[0] * av + s_cp[1] * xc[D + base + q];
+          } else {
+            x = s_cns[ci - 2u] * xc[ci * D + base + q] + s_cna[ci - 2u] * av;
+          }
+          g[q] = x * scale;
+        }
+      }
+      bool bad = false;
+#pragma unroll
+      for (int q = 0; q < PL; ++q) bad |= (base + q < D) && !isfinite(g[q]);
+      if (wave_any(bad)) e = false;
+    }
+    const float co = 1.0f / sn[ci];
+    if (e) {
+#pragma unroll
+      for (int q = 0; q < PL; ++q) g[q] *= co;
+      store_slice<D>(c.adj + ((size_t)b * NC + ci) * D, lane, g);
+    }
+    if (lane == 0) {
+      c.elig[(size_t)b * NC + ci] = e ? 1 : 0;
+      c.coef[(size_t)b * NC + ci] = co;
+      s_el[ci] = e ? 1 : 0;
+    }
+  }
+}
+
 // 32-bit words of the float part of the forward kernel's LDS (xc, part, dist, sn),
 // rounded up to an even count so the 64-bit entry array after it is aligned.
 __host__ __device__ inline uint32_t forward_float_words(uint32_t NC, uint32_t K, uint32_t D) {
@@ -621,6 +749,11 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
       xc[D + t] = (((part[t] + part[D + t]) + part[2 * D + t]) + part[3 * D + t]) / ntot;
   }
   __syncthreads();
+  if (c.loss_kind != 0 && c.loss_kind != 5) {
+    forward_other_loss<D>(c, st, b, deg, xc, sn, s_el);
+    if (!(staged && c.fuse_emit)) return;
+    if (!c.train) return;
+  } else {
   // distances: D0 = |p - a|, D_i = |p - n_i|
   for (uint32_t i = w; i < K + 1u; i += NW) {
     const float* other = (i == 0u) ? xc : xc + (i + 1u) * D;
@@ -722,6 +855,7 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
       c.coef[(size_t)b * NC + ci] = co;
       s_el[ci] = e ? 1 : 0;
     }
+  }
   }
   // Key emission for staged examples (k_emit handles the rest): one (feature, slot) key
   // per occurrence, sentinel unless the construction is eligible and, under first_leaf,

@@ -51,6 +51,17 @@ enum { EP_SCATTER_SORT = 0, EP_SCATTER_ATOMIC = 1 };
 
 /* Mirrors EmbeddingPropagation (src/algos/ep/mod.rs:40-75) plus the model options
  * of EmbeddingPropagator::new (src/lib.rs:1205-1307).                           */
+/* Losses (src/algos/ep/loss.rs:15-38).  PPR: margin loss against random-walk
+ * positives (K walks with restart, loss.rs:191-226) instead of sampled neighbours. */
+enum {
+  EP_LOSS_MARGIN = 0,
+  EP_LOSS_CONTRASTIVE = 1,
+  EP_LOSS_STARSPACE = 2,
+  EP_LOSS_RANK = 3,
+  EP_LOSS_RANKSPACE = 4,
+  EP_LOSS_PPR = 5
+};
+
 typedef struct ep_config {
   uint32_t dims;               /* d_model; mean model: 8,16,32,64,100,128,256;      */
                                /* attention: 8..256, a multiple of 8, with a table  */
@@ -94,6 +105,11 @@ typedef struct ep_config {
                                /* heads * (2 d_k + dims) wide (model.rs:241-243)     */
   uint32_t context_window;     /* `context_window`: must be 0 (Sliding attention is  */
                                /* not built)                                         */
+  int32_t loss_kind;           /* EP_LOSS_* (EPLoss, lib.rs:939-1110; loss.rs:15-38); */
+                               /* gamma is the first parameter: margin / starspace   */
+                               /* gamma, contrastive positive margin, rank tau,      */
+                               /* rankspace gamma, ppr gamma                         */
+  float loss_param2;           /* contrastive negative margin / ppr restart_p        */
 } ep_config;
 
 typedef struct ep_ctx ep_ctx;

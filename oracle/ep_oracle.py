@@ -28,6 +28,8 @@ Pinning (what checks each function; see tests/test_oracle.py):
   euclidean                      reference KAT loss.rs:250-255 (sqrt 8)
   construct_mean                 closed forms on hand-made inputs, invariants
   margin_loss_adjoints           central finite differences, closed form
+  loss_adjoints                  il2norm KAT (loss.rs:258-264); central finite differences and
+                                 torch fp64 autograd of every loss kind; closed forms
   example_grads(first_leaf)      brute force with per-construction autograd leaves (torch fp64)
   example_grads(exact)           central finite differences of the loss w.r.t. the table
   adam_update                    closed form of the first update (m=v=0)
@@ -253,6 +255,106 @@ def margin_loss_adjoints(a: np.ndarray, p: np.ndarray, negs: Sequence[np.ndarray
     return L, active, g_a, g_p, g_n
 
 
+def _unit(x: np.ndarray):
+    n = float(np.sqrt(np.sum(x * x)))
+    return x / n, n
+
+
+def _unit_adjoint(x: np.ndarray, g_unit: np.ndarray) -> np.ndarray:
+    """Adjoint through x -> x/|x| (il2norm, loss.rs:228-234): (g - u (u.g)) / |x|."""
+    u, n = _unit(x)
+    return (g_unit - u * float(u @ g_unit)) / n
+
+
+def loss_adjoints(kind: str, a: np.ndarray, p: np.ndarray, negs: Sequence[np.ndarray], params: Sequence[float]):
+    """Loss::compute (loss.rs:56-151) and its adjoints, float64.
+
+    a = h(v), p = ~h(v) (thv), n_i = h(u_i).  Returns (L, reached, g_a, g_p, [g_n_i]):
+    ``reached`` marks the constructions inside the loss graph (the others have no
+    gradient, like an unreached autodiff leaf); a gradient is None when not reached.
+      margin / ppr (gamma)     mean over active i of gamma + |p-a| - |p-n_i|
+      starspace (gamma)        mean over active i of gamma - (p^.a^ - a^.n_i^)   (x^ = x/|x|)
+      contrastive (pm, nm)     mean over the positive terms of max(0, pm - p^.a^) and
+                               max(0, a^.n_i^ - nm)
+      rank (tau)               -ln s_pos when s_pos < tau, s = softmax(n_1.a, .., n_K.a, a.p)
+                               (exact exp; the reference's exp_approx is a series kept in
+                               simple_grad)
+      rankspace (gamma)        starspace(gamma) + rank(gamma)
+    """
+    a = np.asarray(a, dtype=np.float64)
+    p = np.asarray(p, dtype=np.float64)
+    negs = [np.asarray(n, dtype=np.float64) for n in negs]
+    K = len(negs)
+    if kind in ("margin", "ppr"):
+        L, active, g_a, g_p, g_n = margin_loss_adjoints(a, p, negs, params[0])
+        reached = [L > 0, L > 0] + [bool(x) and L > 0 for x in active]
+        return L, reached, g_a, g_p, g_n
+    ga = np.zeros_like(a)
+    gp = np.zeros_like(p)
+    gn = [np.zeros_like(n) for n in negs]
+    reached = [False] * (K + 2)
+    L = 0.0
+    if kind in ("starspace", "rankspace"):
+        gamma = params[0]
+        au, _ = _unit(a)
+        pu, _ = _unit(p)
+        nus = [_unit(n)[0] for n in negs]
+        rd = float(pu @ au)
+        ls = [gamma - (rd - float(au @ nu)) for nu in nus]
+        act = [l > 0 for l in ls]
+        nA = sum(act)
+        if nA:
+            L += sum(l for l, x in zip(ls, act) if x) / nA
+            g_au = sum((nus[i] - pu) for i in range(K) if act[i]) / nA
+            g_pu = -au * (sum(act) / nA)
+            ga += _unit_adjoint(a, g_au)
+            gp += _unit_adjoint(p, g_pu)
+            reached[0] = reached[1] = True
+            for i in range(K):
+                if act[i]:
+                    gn[i] += _unit_adjoint(negs[i], au / nA)
+                    reached[2 + i] = True
+    if kind == "contrastive":
+        pm, nm = params[0], params[1]
+        au, _ = _unit(a)
+        pu, _ = _unit(p)
+        nus = [_unit(n)[0] for n in negs]
+        pos = pm - float(pu @ au)
+        negt = [float(au @ nu) - nm for nu in nus]
+        terms = [t for t in negt if t > 0] + ([pos] if pos > 0 else [])
+        if terms:
+            m = len(terms)
+            L = sum(terms) / m
+            g_au = np.zeros_like(a)
+            if pos > 0:
+                g_au -= pu / m
+                gp += _unit_adjoint(p, -au / m)
+                reached[1] = True
+            for i in range(K):
+                if negt[i] > 0:
+                    g_au += nus[i] / m
+                    gn[i] += _unit_adjoint(negs[i], au / m)
+                    reached[2 + i] = True
+            ga += _unit_adjoint(a, g_au)
+            reached[0] = True
+    if kind in ("rank", "rankspace"):
+        tau = params[0]
+        ds = np.array([float(n @ a) for n in negs] + [float(a @ p)])
+        z = np.exp(ds - ds.max())
+        sm = z / z.sum()
+        if sm[-1] < tau:
+            L += -math.log(sm[-1])
+            ga += sum(sm[i] * negs[i] for i in range(K)) + (sm[-1] - 1.0) * p
+            gp += (sm[-1] - 1.0) * a
+            for i in range(K):
+                gn[i] += sm[i] * a
+            reached = [True] * (K + 2)
+    if not (L > 0):
+        reached = [False] * (K + 2)
+    return (L, reached, ga if reached[0] else None, gp if reached[1] else None,
+            [g if r else None for g, r in zip(gn, reached[2:])])
+
+
 def loss_weight_divisor(kind: str, degree: int, weight: float = 0.0) -> float:
     """LossWeighting (mod.rs:190-200): loss is divided by this value."""
     if kind == "none":
@@ -279,7 +381,7 @@ class ExampleResult:
 
 
 def example_grads(table, f_indptr, f_ids, anchor, nbrs, nbr_scale, negs, gamma,
-                  grad_mode="first_leaf", loss_div=1.0) -> ExampleResult:
+                  grad_mode="first_leaf", loss_div=1.0, loss=None) -> ExampleResult:
     """Forward + adjoint + gradient extraction for one anchor with explicit samples.
 
     Constructions are ordered a=h(v), p=~h(v), n_1..n_K (mod.rs:300-320).
@@ -298,7 +400,8 @@ def example_grads(table, f_indptr, f_ids, anchor, nbrs, nbr_scale, negs, gamma,
     for u in negs:
         cons.append(build_anchor_construction(f_indptr, f_ids, u))
     vecs = [construct_mean(table, c) for c in cons]
-    L, active, g_a, g_p, g_n = margin_loss_adjoints(vecs[0], vecs[1], vecs[2:], gamma)
+    kind, params = loss if loss is not None else ("margin", (gamma,))
+    L, _, g_a, g_p, g_n = loss_adjoints(kind, vecs[0], vecs[1], vecs[2:], params)
     L = L / loss_div
     adjs = [g_a, g_p] + list(g_n)
     adjs = [None if g is None else g / loss_div for g in adjs]
@@ -386,26 +489,27 @@ def adam_update(table, mom, var, grads: Dict[int, np.ndarray], lr: float, pass_:
         table[f] = table[f] - lr * (mh / (np.sqrt(vh) + ADAM_EPS))
 
 
-def _example(table, f_indptr, f_ids, v, nbrs, nbr_scale, negs, gamma, grad_mode, div, attention):
+def _example(table, f_indptr, f_ids, v, nbrs, nbr_scale, negs, gamma, grad_mode, div, attention, loss=None):
     if attention is None:
-        return example_grads(table, f_indptr, f_ids, v, nbrs, nbr_scale, negs, gamma, grad_mode, div)
+        return example_grads(table, f_indptr, f_ids, v, nbrs, nbr_scale, negs, gamma, grad_mode, div, loss)
     heads, d_k = attention
-    return attention_example_grads(table, f_indptr, f_ids, v, nbrs, negs, gamma, heads, d_k, grad_mode, div)
+    return attention_example_grads(table, f_indptr, f_ids, v, nbrs, negs, gamma, heads, d_k, grad_mode, div, loss)
 
 
 def ep_batch_step(table, mom, var, f_indptr, f_ids, anchors, nbr_lists, nbr_scales, neg_lists,
-                  gamma, lr, pass_, grad_mode="first_leaf", loss_divs=None, attention=None):
+                  gamma, lr, pass_, grad_mode="first_leaf", loss_divs=None, attention=None, loss=None):
     """One batch: per-example forward/backward, merge, Adam (mod.rs:174-252).
 
     Updates table/mom/var in place (float64 arrays).  Returns (results, merged grads).
     The Adam update runs only when at least one example contributes (mod.rs:237).
-    attention = (heads, d_k) selects the attention model (attention_example_grads).
+    attention = (heads, d_k) selects the attention model (attention_example_grads);
+    loss = (kind, params) another loss than margin(gamma) (loss_adjoints).
     """
     results = []
     for i, v in enumerate(anchors):
         div = 1.0 if loss_divs is None else loss_divs[i]
         results.append(_example(table, f_indptr, f_ids, v, nbr_lists[i], nbr_scales[i], neg_lists[i], gamma,
-                                grad_mode, div, attention))
+                                grad_mode, div, attention, loss))
     merged = merge_batch(results, table.shape[1])
     if any(r.contributes for r in results):
         adam_update(table, mom, var, merged, lr, pass_)
@@ -582,7 +686,7 @@ def _scatter_rows(out: Dict[int, np.ndarray], feats, rows: np.ndarray):
 
 
 def attention_example_grads(table, f_indptr, f_ids, anchor, nbrs, negs, gamma, heads, d_k,
-                            grad_mode="first_leaf", loss_div=1.0) -> ExampleResult:
+                            grad_mode="first_leaf", loss_div=1.0, loss=None) -> ExampleResult:
     """Forward + adjoint + gradient extraction for one anchor under the attention model.
 
     AttentionFeatureModel (model.rs:158-249):
@@ -612,7 +716,8 @@ def attention_example_grads(table, f_indptr, f_ids, anchor, nbrs, negs, gamma, h
     p = sum(fwd(it) for it in nb_sets) / float(len(nb_sets))
     nv = [fwd(it) for it in neg_sets]
     vecs = [a, p] + nv
-    L, active, g_a, g_p, g_n = margin_loss_adjoints(a, p, nv, gamma)
+    kind, params = loss if loss is not None else ("margin", (gamma,))
+    L, _, g_a, g_p, g_n = loss_adjoints(kind, a, p, nv, params)
     L = L / loss_div
     adjs = [g_a, g_p] + list(g_n)
     adjs = [None if g is None else g / loss_div for g in adjs]

@@ -49,9 +49,9 @@ def make_problem(seed=0, n=120, V=300, D=32, avg_deg=6, fmax=8, zipf=1.1, zero_d
 
 
 def oracle_step(prob: Problem, anchors, cnt, nbr, neg, gamma, lr, pass_, grad_mode, table=None, mom=None, var=None,
-                loss_divs=None, attention=None):
+                loss_divs=None, attention=None, loss=None):
     """FP64 oracle of one explicit-sample step; returns (results, merged, table, mom, var).
-    attention = (heads, d_k) selects the attention model."""
+    attention = (heads, d_k) selects the attention model; loss = (kind, params) the loss."""
     t = (prob.table if table is None else table).astype(np.float64).copy()
     m = np.zeros_like(t) if mom is None else mom.astype(np.float64).copy()
     v = np.zeros_like(t) if var is None else var.astype(np.float64).copy()
@@ -63,7 +63,7 @@ def oracle_step(prob: Problem, anchors, cnt, nbr, neg, gamma, lr, pass_, grad_mo
     res_k, merged = O.ep_batch_step(t, m, v, prob.f_indptr, prob.f_ids, [int(anchors[i]) for i in keep],
                                     [nbr_lists[i] for i in keep], [1.0] * len(keep),
                                     [list(map(int, neg[i])) for i in keep], gamma, lr, pass_, grad_mode,
-                                    None if loss_divs is None else [loss_divs[i] for i in keep], attention)
+                                    None if loss_divs is None else [loss_divs[i] for i in keep], attention, loss)
     it = iter(res_k)
     results = [next(it) if i in keep else None for i in range(B)]
     return results, merged, t, m, v

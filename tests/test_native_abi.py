@@ -38,7 +38,8 @@ def test_config_defaults_match_reference():
 
 @pytest.mark.parametrize("field,value", [("dims", 7), ("negatives", 0), ("batch_size", 0), ("hard_negatives", 2),
                                          ("noise", -0.5), ("noise", float("nan")), ("max_features_kind", 1),
-                                         ("valid_pct", 1.5), ("concurrent_batches", 3), ("seg_vpl", 5)])
+                                         ("valid_pct", 1.5), ("concurrent_batches", 3), ("seg_vpl", 5),
+                                         ("loss_kind", 6), ("loss_kind", -1), ("gamma", float("inf"))])
 def test_invalid_config_rejected(field, value):
     c = N.default_config()
     setattr(c, field, value)
@@ -64,6 +65,15 @@ def test_invalid_attention_config_rejected(fields):
     rc = N.load().ep_create(ctypes.byref(c), ctypes.byref(h))
     assert rc == -1 and not h.value
     assert N.load().ep_last_error()
+
+
+def test_ppr_restart_probability_checked():
+    for r in (-0.1, 1.5, float("nan")):
+        c = N.default_config()
+        c.loss_kind, c.loss_param2 = N.EP_LOSS_PPR, r
+        h = ctypes.c_void_p()
+        assert N.load().ep_create(ctypes.byref(c), ctypes.byref(h)) == -1 and not h.value
+        assert b"restart" in N.load().ep_last_error() or b"finite" in N.load().ep_last_error()
 
 
 def test_attention_row_width():

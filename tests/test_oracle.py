@@ -684,3 +684,98 @@ def test_weighted_aggregate_closed_form():
     # a large alpha makes the weights equal: the plain mean of the listed rows
     big = O.weighted_node_embeddings(table, f_indptr, f_ids, p_w, 1e12)
     assert np.allclose(big, O.averaged_node_embeddings(table, f_indptr, f_ids))
+
+
+# --------------------------------------------------------------------------------------
+# Other losses: loss.rs:56-151
+# --------------------------------------------------------------------------------------
+def test_il2norm_kat():
+    # loss.rs:258-264: il2norm([1, 3]) = [1, 3] / sqrt(10)
+    u, n = O._unit(np.array([1.0, 3.0]))
+    assert np.allclose(u, np.array([1.0, 3.0]) / math.sqrt(10.0)) and n == math.sqrt(10.0)
+
+
+def _torch_loss(kind, a, p, negs, params):
+    """The reference's Loss::compute written again in torch (fp64), gradients by autograd."""
+    A = torch.tensor(a, dtype=torch.float64, requires_grad=True)
+    Pp = torch.tensor(p, dtype=torch.float64, requires_grad=True)
+    Ns = [torch.tensor(n, dtype=torch.float64, requires_grad=True) for n in negs]
+    unit = lambda x: x / torch.sqrt((x * x).sum())
+
+    def starspace(gamma):
+        rd = unit(Pp) @ unit(A)
+        ls = [gamma - (rd - unit(A) @ unit(n)) for n in Ns]
+        ls = [l for l in ls if l.item() > 0]
+        return sum(ls) / len(ls) if ls else None
+
+    def rank(tau):
+        ds = torch.stack([n @ A for n in Ns] + [A @ Pp])
+        sm = torch.softmax(ds, dim=0)
+        return -torch.log(sm[-1]) if sm[-1].item() < tau else None
+
+    if kind == "starspace":
+        L = starspace(params[0])
+    elif kind == "rank":
+        L = rank(params[0])
+    elif kind == "rankspace":
+        parts = [x for x in (starspace(params[0]), rank(params[0])) if x is not None]
+        L = sum(parts) if parts else None
+    else:  # contrastive
+        pos = params[0] - unit(Pp) @ unit(A)
+        ts = [t for t in [unit(A) @ unit(n) - params[1] for n in Ns] if t.item() > 0]
+        if pos.item() > 0:
+            ts.append(pos)
+        L = sum(ts) / len(ts) if ts else None
+    if L is None:
+        return 0.0, [None] * (len(negs) + 2)
+    L.backward()
+    return L.item(), [None if x.grad is None else x.grad.numpy() for x in [A, Pp] + Ns]
+
+
+LOSS_CASES = [("starspace", (0.3,)), ("starspace", (1.2,)), ("contrastive", (0.6, 0.1)), ("contrastive", (-0.9, -0.2)),
+              ("rank", (0.9,)), ("rank", (0.05,)), ("rankspace", (0.4,)), ("rankspace", (0.9,))]
+
+
+@pytest.mark.parametrize("kind,params", LOSS_CASES)
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_loss_adjoints_vs_torch_and_fd(kind, params, seed):
+    rng = np.random.default_rng(seed)
+    d, K = 5, 4
+    a, p = rng.normal(size=(2, d))
+    negs = list(rng.normal(size=(K, d)))
+    L, reached, ga, gp, gn = O.loss_adjoints(kind, a, p, negs, params)
+    Lt, gt = _torch_loss(kind, a, p, negs, params)
+    assert abs(L - Lt) < 1e-12
+    for g, r, t in zip([ga, gp] + gn, reached, gt):
+        assert (g is None) == (t is None) == (not r)
+        if g is not None:
+            assert np.allclose(g, t, atol=1e-12)
+    if L > 0:
+        eps = 1e-6
+        vecs = [a, p] + negs
+        for vi in range(len(vecs)):
+            for j in range(d):
+                vp = [v.copy() for v in vecs]
+                vm = [v.copy() for v in vecs]
+                vp[vi][j] += eps
+                vm[vi][j] -= eps
+                fd = (O.loss_adjoints(kind, vp[0], vp[1], vp[2:], params)[0]
+                      - O.loss_adjoints(kind, vm[0], vm[1], vm[2:], params)[0]) / (2 * eps)
+                g = ([ga, gp] + gn)[vi]
+                assert abs(fd - (0.0 if g is None else g[j])) < 1e-6
+
+
+def test_loss_closed_forms():
+    a, p = np.array([1.0, 0.0]), np.array([0.0, 2.0])     # a^.p^ = 0
+    n1, n2 = np.array([3.0, 0.0]), np.array([-1.0, 0.0])  # a^.n^ = 1, -1
+    # starspace: l_i = gamma - (0 - a^.n_i^) -> 0.5 + 1, 0.5 - 1 (inactive)
+    L, r, *_ = O.loss_adjoints("starspace", a, p, [n1, n2], (0.5,))
+    assert L == 1.5 and r == [True, True, True, False]
+    # contrastive: pos = 0.7 - 0; negatives a^.n^ - 0.2 -> 0.8 (kept), -1.2 (dropped)
+    L, r, *_ = O.loss_adjoints("contrastive", a, p, [n1, n2], (0.7, 0.2))
+    assert abs(L - (0.8 + 0.7) / 2) < 1e-15 and r == [True, True, True, False]
+    # rank: ds = (3, -1, 0); s_pos = 1 / (e^3 + e^-1 + 1)
+    L, r, *_ = O.loss_adjoints("rank", a, p, [n1, n2], (0.5,))
+    assert abs(L - math.log(math.exp(3) + math.exp(-1) + 1)) < 1e-12 and all(r)
+    L, r, *_ = O.loss_adjoints("rank", a, p, [n1, n2], (0.01,))  # s_pos ~ 0.045 >= tau: no loss
+    assert L == 0 and not any(r)
