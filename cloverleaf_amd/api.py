@@ -582,14 +582,13 @@ class EmbeddingPropagator:
             raise ValueError("max_features sampling is not supported by this build")
         if weighted_neighbor_sampling or weighted_neighbor_averaging:
             raise ValueError("weighted neighbour sampling/averaging is not supported by this build")
-        if hard_negatives:
-            raise ValueError("hard negatives are not supported by this build")
         cfg = N.default_config()
         cfg.alpha = 0.9 if alpha is None else float(alpha)
         cfg.batch_size = 50 if batch_size is None else int(batch_size)
         cfg.dims = 100 if dims is None else int(dims)
         cfg.passes = 100 if passes is None else int(passes)
         cfg.negatives = loss.negatives
+        cfg.hard_negatives = 0 if hard_negatives is None else int(hard_negatives)
         cfg.loss_kind = N.LOSS_KINDS[loss.kind]
         cfg.gamma = loss.params[0]
         cfg.loss_param2 = float(loss.params[1]) if len(loss.params) > 1 else 0.0

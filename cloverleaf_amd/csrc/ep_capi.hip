@@ -361,6 +361,7 @@ void refresh_base(ep_ctx* c) {
   b.gamma = cfg.gamma;
   b.loss_kind = cfg.loss_kind;
   b.loss_p2 = cfg.loss_param2;
+  b.hard_negs = std::min(cfg.hard_negatives, cfg.negatives);
   b.lw_kind = cfg.loss_weighting;
   b.lw_exp = cfg.loss_weight_exp;
   b.grad_mode = cfg.grad_mode;
@@ -593,7 +594,6 @@ int validate_config(const ep_config* cfg) {
   if (cfg->max_nodes_kind != EP_SAMPLE_FIXED && cfg->max_nodes_kind != EP_SAMPLE_ALL)
     return fail(EP_EINVAL, "max_nodes must be All or Fixed");
   if (cfg->max_features_kind != EP_SAMPLE_ALL) return fail(EP_EINVAL, "max_features sampling is not supported");
-  if (cfg->hard_negatives != 0) return fail(EP_EINVAL, "hard_negatives is not supported");
   if (cfg->loss_kind < EP_LOSS_MARGIN || cfg->loss_kind > EP_LOSS_PPR) return fail(EP_EINVAL, "loss_kind");
   if (!std::isfinite(cfg->gamma) || !std::isfinite(cfg->loss_param2))
     return fail(EP_EINVAL, "loss parameters must be finite");

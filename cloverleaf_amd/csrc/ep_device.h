@@ -50,6 +50,7 @@ enum RngPurpose : uint32_t {
   RNG_NOISE = 4,
   RNG_INIT = 5,
   RNG_WALK = 6,
+  RNG_HARD_WALK = 7,
 };
 
 // Uniform integer in [0, n) from a 32-bit draw (multiply-shift; bias <= n / 2^32).

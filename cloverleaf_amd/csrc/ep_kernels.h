@@ -83,6 +83,7 @@ struct StepConfig {
   const uint32_t* perm;    // anchors of the pass, in order
   const uint32_t* negpool; // negatives drawn uniformly from this pool
   uint32_t n_negpool;
+  uint32_t hard_negs;      // random-walk hard negatives per example (min(hard_negatives, K))
   uint32_t n_batch_total;  // anchors in the pass
   uint32_t batch;          // anchors per step: group size x rbatch (buffer capacity)
   uint32_t rbatch;         // B, the reference batch size

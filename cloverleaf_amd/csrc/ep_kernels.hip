@@ -245,6 +245,65 @@ __device__ inline uint32_t draw_word(uint32_t pos, uint32_t pass, uint32_t purpo
   return q == 0 ? r.x : q == 1 ? r.y : q == 2 ? r.z : r.w;
 }
 
+// The walk of random_walk (loss.rs:193-226, node_sampler.rs:109-140): from v, at most 10
+// steps to a uniform neighbour; from the second step on it stops with probability
+// restart_p at a node != v; it also stops at a node without neighbours.  Step i draws
+// words r0 + 2i (neighbour) and r0 + 2i + 1 (stop).  Returns the node it ends on.
+__device__ inline uint32_t walk_end(const StepConfig& c, uint32_t v, float restart_p, uint32_t pos, uint32_t pass,
+                                    uint32_t purpose, uint32_t r0, uint32_t k0, uint32_t k1) {
+  uint32_t node = v;
+  for (uint32_t i = 1;; ++i) {
+    const uint64_t e0 = c.indptr[node];
+    const uint32_t dn = (uint32_t)(c.indptr[node + 1] - e0);
+    if (dn == 0u || i > 10u) break;
+    node = c.indices[e0 + uniform_below(draw_word(pos, pass, purpose, r0 + 2u * i, k0, k1), dn)];
+    if (i > 1u && uniform01(draw_word(pos, pass, purpose, r0 + 2u * i + 1u, k0, k1)) < restart_p && node != v)
+      break;
+  }
+  return node;
+}
+
+// K negatives of one example (one wave).  RandomWalkHardSampler (node_sampler.rs:83-107):
+// with H = min(hard_negatives, K) > 0, up to 2H walks (restart 0.25) run in order and
+// each endpoint that is neither v nor one of v's neighbours and not already taken becomes
+// a negative, until H are taken; lane j runs walk j and the order is kept by ballots.
+// The rest are uniform with replacement from the pool (node_sampler.rs:102-105).
+__device__ inline void sample_negatives(const StepConfig& c, uint32_t v, uint64_t s0, uint32_t deg, uint32_t pos,
+                                        uint32_t pass, uint32_t k0, uint32_t k1, uint32_t* negs, int lane) {
+  uint32_t taken = 0;
+  if (c.hard_negs) {
+    const uint32_t H = c.hard_negs;
+    bool ok = false;
+    uint32_t cand = v;
+    if ((uint32_t)lane < 2u * H) {
+      cand = walk_end(c, v, 0.25f, pos, pass, RNG_HARD_WALK, (uint32_t)lane * 32u, k0, k1);
+      ok = cand != v;
+    }
+    // drop endpoints adjacent to v: the wave reads v's edges 64 at a time
+    if (wave_any(ok)) {
+      for (uint32_t j0 = 0; j0 < deg; j0 += 64u) {
+        const uint32_t e = j0 + (uint32_t)lane < deg ? c.indices[s0 + j0 + lane] : v;
+        const uint32_t nj = min(64u, deg - j0);
+        for (uint32_t t = 0; t < nj; ++t) ok &= (uint32_t)__shfl((int)e, (int)t) != cand;
+      }
+    }
+    // keep the first walk of each endpoint
+    bool dup = false;
+    for (uint32_t t = 0; t < 2u * H; ++t) {
+      const uint32_t ct = (uint32_t)__shfl((int)cand, (int)t);
+      const bool okt = __shfl((int)ok, (int)t) != 0;
+      dup |= okt && (t < (uint32_t)lane) && ct == cand;
+    }
+    const bool keep = ok && !dup;
+    const uint64_t m = __ballot(keep);
+    const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (keep && rank < H) negs[rank] = cand;
+    taken = min((uint32_t)__popcll(m), H);
+  }
+  for (uint32_t k = taken + lane; k < c.K; k += 64u)
+    negs[k] = c.negpool[uniform_below(draw_word(pos, pass, RNG_NEGATIVES, k, k0, k1), c.n_negpool)];
+}
+
 __global__ __launch_bounds__(256) void k_sample(StepConfig c, StepState* st) {
   const int lane = threadIdx.x & 63;
   const uint32_t b = blockIdx.x * NW + (threadIdx.x >> 6);
@@ -274,15 +333,7 @@ __global__ __launch_bounds__(256) void k_sample(StepConfig c, StepState* st) {
     bool some = false;
     if ((uint32_t)lane < c.K) {
       const uint32_t r0 = (uint32_t)lane * 32u;
-      uint32_t node = v;
-      for (uint32_t i = 1;; ++i) {
-        const uint64_t e0 = c.indptr[node];
-        const uint32_t dn = (uint32_t)(c.indptr[node + 1] - e0);
-        if (dn == 0u || i > 10u) break;
-        node = c.indices[e0 + uniform_below(draw_word(pos, pass, RNG_WALK, r0 + 2u * i, k0, k1), dn)];
-        if (i > 1u && uniform01(draw_word(pos, pass, RNG_WALK, r0 + 2u * i + 1u, k0, k1)) < c.loss_p2 && node != v)
-          break;
-      }
+      const uint32_t node = walk_end(c, v, c.loss_p2, pos, pass, RNG_WALK, r0, k0, k1);
       if (node != v) {
         endp = node;
         some = true;
@@ -298,8 +349,7 @@ __global__ __launch_bounds__(256) void k_sample(StepConfig c, StepState* st) {
       if (lane == 0) out[0] = v;
       cnt = 1u;
     }
-    for (uint32_t k = lane; k < c.K; k += 64u)
-      negs[k] = c.negpool[uniform_below(draw_word(pos, pass, RNG_NEGATIVES, k, k0, k1), c.n_negpool)];
+    sample_negatives(c, v, s0, deg, pos, pass, k0, k1, negs, lane);
     if (lane == 0) c.nbr_cnt[b] = cnt;
   } else if (!c.explicit_samples) {
     if (c.all_nbrs) {
@@ -324,8 +374,7 @@ __global__ __launch_bounds__(256) void k_sample(StepConfig c, StepState* st) {
       }
       for (uint32_t j = lane; j < c.S; j += 64u) out[j] = c.indices[s0 + pk[j]];
     }
-    for (uint32_t k = lane; k < c.K; k += 64u)
-      negs[k] = c.negpool[uniform_below(draw_word(pos, pass, RNG_NEGATIVES, k, k0, k1), c.n_negpool)];
+    sample_negatives(c, v, s0, deg, pos, pass, k0, k1, negs, lane);
     if (lane == 0) c.nbr_cnt[b] = cnt;
   } else {
     cnt = c.nbr_cnt[b];

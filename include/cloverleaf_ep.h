@@ -79,7 +79,8 @@ typedef struct ep_config {
   uint32_t max_features;
   int32_t loss_weighting;      /* EP_LW_*                                           */
   float loss_weight_exp;       /* exponent of EP_LW_EXPONENTIAL                     */
-  uint32_t hard_negatives;     /* must be 0                                         */
+  uint32_t hard_negatives;     /* random-walk hard negatives per example, at most K */
+                               /* used (RandomWalkHardSampler, node_sampler.rs:83-107)*/
   float noise;                 /* gradient noise sd at step 0, decaying to 1e-9 over */
                                /* the run (mod.rs:139-147, 238-247); 0 = off       */
   int32_t weighted_neighbor_sampling;   /* must be 0                                */

@@ -36,7 +36,7 @@ def test_config_defaults_match_reference():
     assert c.max_nodes_kind == N.EP_SAMPLE_ALL and c.max_features_kind == N.EP_SAMPLE_ALL
 
 
-@pytest.mark.parametrize("field,value", [("dims", 7), ("negatives", 0), ("batch_size", 0), ("hard_negatives", 2),
+@pytest.mark.parametrize("field,value", [("dims", 7), ("negatives", 0), ("batch_size", 0),
                                          ("noise", -0.5), ("noise", float("nan")), ("max_features_kind", 1),
                                          ("valid_pct", 1.5), ("concurrent_batches", 3), ("seg_vpl", 5),
                                          ("loss_kind", 6), ("loss_kind", -1), ("gamma", float("inf"))])
