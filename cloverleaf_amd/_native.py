@@ -111,6 +111,9 @@ SIGNATURES = {
     "ep_create": (c_int, [POINTER(EPConfig), POINTER(c_void_p)]),
     "ep_destroy": (None, [c_void_p]),
     "ep_set_graph": (c_int, [c_void_p, c_uint64, _u64p, _u32p]),
+    "ep_set_edge_weights": (c_int, [c_void_p, _f32p]),
+    "ep_sample_weights": (c_int, [c_void_p, c_uint32, _f32p]),
+    "ep_set_explicit_weights": (c_int, [c_void_p, c_uint32, _f32p]),
     "ep_set_features": (c_int, [c_void_p, c_uint64, c_uint32, _u64p, _u32p]),
     "ep_set_table": (c_int, [c_void_p, _f32p]),
     "ep_get_table": (c_int, [c_void_p, _f32p]),
@@ -218,6 +221,22 @@ class Context:
         n = len(indptr) - 1
         check(load().ep_set_graph(self.handle, n, ptr(indptr, c_uint64), ptr(indices, c_uint32)), "ep_set_graph")
         self.n_nodes = n
+
+    def set_edge_weights(self, cdf: np.ndarray):
+        """Per-edge CDF of each row (CumCSR weights), one value per CSR entry."""
+        cdf = np.ascontiguousarray(cdf, dtype=np.float32)
+        check(load().ep_set_edge_weights(self.handle, ptr(cdf, c_float)), "ep_set_edge_weights")
+
+    def sample_weights(self, n: int) -> np.ndarray:
+        """Neighbour weights w_j of the last sample_batch (n x max_nodes)."""
+        w = np.zeros((n, list_cap(self.cfg)), dtype=np.float32)
+        check(load().ep_sample_weights(self.handle, n, ptr(w, c_float)), "ep_sample_weights")
+        return w
+
+    def set_explicit_weights(self, w: np.ndarray):
+        """Neighbour weights (n x max_nodes) used by the next step_explicit calls."""
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        check(load().ep_set_explicit_weights(self.handle, w.shape[0], ptr(w, c_float)), "ep_set_explicit_weights")
 
     def set_features(self, f_indptr: np.ndarray, f_ids: np.ndarray, n_vocab: int):
         f_indptr = np.ascontiguousarray(f_indptr, dtype=np.uint64)

@@ -125,6 +125,15 @@ class Graph:
     def degree_array(self) -> np.ndarray:
         return np.diff(self.indptr.astype(np.int64))
 
+    def edge_cdf(self) -> np.ndarray:
+        """Per-row CDF of the edge weights (CumCSR), uniform rows for an unweighted graph."""
+        if self.weights is not None:
+            return self.weights
+        deg = self.degree_array()
+        start = np.repeat(self.indptr[:-1].astype(np.int64), deg)
+        pos = (np.arange(len(self.indices), dtype=np.int64) - start + 1).astype(np.float32)
+        return np.minimum(pos / np.repeat(deg, deg).astype(np.float32), np.float32(1.0))
+
     def get_edges(self, node: FQNode, normalized: Optional[bool] = None):
         i = _node_id(self.vocab_, node)
         s, e = int(self.indptr[i]), int(self.indptr[i + 1])
@@ -565,7 +574,8 @@ class EmbeddingPropagator:
     the reference's concurrently running batches; 1 = strictly sequential), ``device``.
     ``attention=d_k`` with ``attention_heads`` trains the attention aggregator (Full
     attention, rows of attention_heads * (2 d_k + dims) values, at most 64 features per
-    node); ``context_window`` (Sliding attention) is not supported.
+    node); ``context_window`` (Sliding attention) is not supported.  Weighted neighbour
+    sampling and averaging use the graph's edge weights (mean model only).
     """
 
     def __init__(self, alpha=None, loss=None, batch_size=None, dims=None, passes=None, seed=None, max_nodes=None,
@@ -580,8 +590,8 @@ class EmbeddingPropagator:
             raise ValueError("context_window (Sliding attention) is not supported by this build")
         if mf_kind != N.EP_SAMPLE_ALL:
             raise ValueError("max_features sampling is not supported by this build")
-        if weighted_neighbor_sampling or weighted_neighbor_averaging:
-            raise ValueError("weighted neighbour sampling/averaging is not supported by this build")
+        if (weighted_neighbor_sampling or weighted_neighbor_averaging) and attention is not None:
+            raise ValueError("weighted neighbour sampling/averaging with attention is not supported by this build")
         cfg = N.default_config()
         cfg.alpha = 0.9 if alpha is None else float(alpha)
         cfg.batch_size = 50 if batch_size is None else int(batch_size)
@@ -589,6 +599,8 @@ class EmbeddingPropagator:
         cfg.passes = 100 if passes is None else int(passes)
         cfg.negatives = loss.negatives
         cfg.hard_negatives = 0 if hard_negatives is None else int(hard_negatives)
+        cfg.weighted_neighbor_sampling = 1 if weighted_neighbor_sampling else 0
+        cfg.weighted_neighbor_averaging = 1 if weighted_neighbor_averaging else 0
         cfg.loss_kind = N.LOSS_KINDS[loss.kind]
         cfg.gamma = loss.params[0]
         cfg.loss_param2 = float(loss.params[1]) if len(loss.params) > 1 else 0.0
@@ -642,6 +654,8 @@ class EmbeddingPropagator:
                 raise ValueError(f"warm-start table has shape {warm.shape}, expected {(n_vocab, width)}")
         with N.Context(self.cfg) as ctx:
             ctx.set_graph(graph.indptr, graph.indices)
+            if self.cfg.weighted_neighbor_averaging:
+                ctx.set_edge_weights(graph.edge_cdf())
             ctx.set_features(f_indptr, f_ids, n_vocab)
             ctx.set_table(warm)
             self.history = []

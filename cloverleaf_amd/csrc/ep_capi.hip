@@ -67,6 +67,8 @@ struct ep_ctx {
   std::vector<uint32_t> degrees;
   uint64_t* d_indptr = nullptr;
   uint32_t* d_indices = nullptr;
+  float* d_cdf = nullptr;        // per-edge CDF (weighted neighbours), ep_set_edge_weights
+  std::vector<float> explicit_w;  // neighbour weights for ep_step_explicit (test hook)
   uint64_t f_nodes = 0, f_total = 0;
   uint32_t n_vocab = 0, f_max = 0;
   uint64_t* d_findptr = nullptr;
@@ -166,6 +168,9 @@ uint32_t neighbour_cap(const ep_ctx* c) {
   return c->cfg.max_nodes_kind == EP_SAMPLE_FIXED ? c->cfg.max_nodes : c->deg_max;
 }
 
+// weighted_neighbor_averaging applies to ~h(v) from sampled neighbours (not PPR walks).
+bool weighted_avg(const ep_config& cfg) { return cfg.weighted_neighbor_averaging && !ppr_loss(cfg); }
+
 // Anchors per training step: concurrent_batches reference batches.
 uint32_t group_anchors(const ep_ctx* c) { return c->cfg.batch_size * std::max<uint32_t>(1u, c->cfg.concurrent_batches); }
 
@@ -176,6 +181,8 @@ uint64_t example_rows_max(const ep_ctx* c) {
 
 // Sizes every per-step buffer for batches of up to `batch` anchors.
 int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
+  if (weighted_avg(c->cfg) && !c->d_cdf)
+    return fail(EP_ESTATE, "weighted_neighbor_averaging needs the edge weights (ep_set_edge_weights)");
   if (batch <= c->cap_batch && c->base.nbr) return EP_OK;
   if (!c->d_indptr || !c->d_findptr) return fail(EP_ESTATE, "graph and features must be set first");
   const ep_config& cfg = c->cfg;
@@ -183,9 +190,10 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   const bool att = cfg.attention_dk != 0;
   const uint32_t S = list_cap(cfg);
   // key capacity: sum of the largest `batch` example sizes
-  uint64_t keycap;
+  uint64_t keycap, nbr_max;  // nbr_max: neighbours of a batch at most
   if (cfg.max_nodes_kind == EP_SAMPLE_FIXED || ppr_loss(cfg)) {
     keycap = (uint64_t)batch * example_rows_max(c);
+    nbr_max = (uint64_t)batch * S;
   } else {
     std::vector<uint32_t> d = c->degrees;
     const size_t take = std::min<size_t>(batch, d.size());
@@ -193,7 +201,13 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
     uint64_t s = 0;
     for (size_t i = 0; i < take; ++i) s += d[i];
     keycap = (uint64_t)c->f_max * (s + (uint64_t)batch * (1ull + K));
+    nbr_max = s;
   }
+  // weighted neighbours: one adjoint row per neighbour after the construction rows
+  const bool wavg = weighted_avg(cfg);
+  const uint64_t nrow_cap = wavg ? nbr_max : 0;
+  if (wavg && (uint64_t)batch * NC + nrow_cap > ep::kSlotMask)
+    return fail(EP_ECAPACITY, "batch too large: neighbour adjoint rows exceed 2^29");
   if (keycap >= (1ull << 31)) return fail(EP_ECAPACITY, "batch too large: key capacity exceeds 2^31");
   if ((uint64_t)batch * NC > ep::kSlotMask) return fail(EP_ECAPACITY, "batch too large: slots exceed 2^29");
   // attention: one gradient row per feature occurrence, slot = key position
@@ -208,6 +222,7 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   // drop old buffers
   ep::StepConfig& b = c->base;
   c->release(b.nbr_cnt); c->release(b.nbr); c->release(b.neg); c->release(b.rcount); c->release(b.keyoff);
+  c->release(b.nbr_w); c->release(b.nbroff);
   c->release(b.adj); c->release(b.coef); c->release(b.elig); c->release(b.loss);
   c->release(b.keys[0]); c->release(b.keys[1]); c->release(b.slots[0]); c->release(b.slots[1]);
   c->release(b.ghist); c->release(b.G); c->release(b.ufeat); c->release(b.long_st); c->release(b.long_len);
@@ -236,7 +251,10 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   EP_TRY(c->alloc(&b.keyoff, batch + 1));
   // adjoint rows: mean model one per construction (D wide); attention one gradient row
   // per feature occurrence
-  EP_TRY(c->alloc(&b.adj, att ? (size_t)keycap * D : (size_t)batch * NC * D));
+  EP_TRY(c->alloc(&b.adj, att ? (size_t)keycap * D : ((size_t)batch * NC + nrow_cap) * D));
+  if (wavg && S) EP_TRY(c->alloc(&b.nbr_w, (size_t)batch * S));
+  if (wavg && !S) EP_TRY(c->alloc(&b.nbroff, batch));
+  b.nrow_base = batch * NC;
   if (att) {
     const uint32_t dm = cfg.dims;
     EP_TRY(c->alloc(&b.blkoff, batch + 1));
@@ -361,6 +379,9 @@ void refresh_base(ep_ctx* c) {
   b.gamma = cfg.gamma;
   b.loss_kind = cfg.loss_kind;
   b.loss_p2 = cfg.loss_param2;
+  b.wavg = weighted_avg(cfg) ? 1 : 0;
+  b.wsample = cfg.weighted_neighbor_sampling ? 1 : 0;
+  b.cdf = c->d_cdf;
   b.hard_negs = std::min(cfg.hard_negatives, cfg.negatives);
   b.lw_kind = cfg.loss_weighting;
   b.lw_exp = cfg.loss_weight_exp;
@@ -600,8 +621,8 @@ int validate_config(const ep_config* cfg) {
   if (cfg->loss_kind == EP_LOSS_PPR && !(cfg->loss_param2 >= 0.f && cfg->loss_param2 <= 1.f))
     return fail(EP_EINVAL, "ppr restart_p must be in [0, 1]");
   if (!(cfg->noise >= 0.f) || !std::isfinite(cfg->noise)) return fail(EP_EINVAL, "noise must be finite and >= 0");
-  if (cfg->weighted_neighbor_sampling || cfg->weighted_neighbor_averaging)
-    return fail(EP_EINVAL, "weighted neighbour sampling/averaging is not supported");
+  if ((cfg->weighted_neighbor_sampling || cfg->weighted_neighbor_averaging) && cfg->attention_dk)
+    return fail(EP_EINVAL, "weighted neighbour sampling/averaging is not supported with attention");
   if (cfg->precision != EP_PREC_FP32 && cfg->precision != EP_PREC_BF16 && cfg->precision != EP_PREC_BF16_MASTER)
     return fail(EP_EINVAL, "precision");
   if (cfg->grad_mode != EP_GRAD_FIRST_LEAF && cfg->grad_mode != EP_GRAD_EXACT) return fail(EP_EINVAL, "grad_mode");
@@ -704,6 +725,7 @@ int ep_set_graph(ep_ctx* c, uint64_t n_nodes, const uint64_t* indptr, const uint
   EP_HIP(hipSetDevice(c->device));
   c->release(c->d_indptr);
   c->release(c->d_indices);
+  c->release(c->d_cdf);
   EP_TRY(c->alloc(&c->d_indptr, n_nodes + 1));
   EP_TRY(c->alloc(&c->d_indices, ne));
   EP_HIP(hipMemcpy(c->d_indptr, indptr, (n_nodes + 1) * sizeof(uint64_t), hipMemcpyHostToDevice));
@@ -714,6 +736,18 @@ int ep_set_graph(ep_ctx* c, uint64_t n_nodes, const uint64_t* indptr, const uint
   c->degrees.swap(deg);
   c->cap_batch = 0;  // capacities depend on degrees
   c->split_set = false;
+  return EP_OK;
+}
+
+int ep_set_edge_weights(ep_ctx* c, const float* cdf) {
+  if (!c || (!cdf && c->n_entries)) return fail(EP_EINVAL, "NULL argument");
+  if (!c->d_indptr) return fail(EP_ESTATE, "the graph must be set first");
+  for (uint64_t i = 0; i < c->n_entries; ++i)
+    if (!(cdf[i] >= 0.f) || !std::isfinite(cdf[i])) return fail(EP_EINVAL, "edge CDF values must be finite and >= 0");
+  EP_HIP(hipSetDevice(c->device));
+  c->release(c->d_cdf);
+  EP_TRY(c->alloc(&c->d_cdf, std::max<uint64_t>(c->n_entries, 1)));
+  if (c->n_entries) EP_HIP(hipMemcpy(c->d_cdf, cdf, c->n_entries * sizeof(float), hipMemcpyHostToDevice));
   return EP_OK;
 }
 
@@ -963,6 +997,21 @@ int ep_sample_batch(ep_ctx* c, const uint32_t* anchors, uint32_t n, uint32_t pas
   return EP_OK;
 }
 
+int ep_sample_weights(ep_ctx* c, uint32_t n, float* w) {
+  if (!c || !w || n == 0) return fail(EP_EINVAL, "bad argument");
+  if (!c->base.nbr_w) return fail(EP_ESTATE, "no neighbour weights: weighted_neighbor_averaging with max_nodes Fixed");
+  if (n > c->cap_batch) return fail(EP_EINVAL, "n exceeds the sampled batch");
+  EP_HIP(hipSetDevice(c->device));
+  EP_HIP(hipMemcpy(w, c->base.nbr_w, (size_t)n * c->base.S * sizeof(float), hipMemcpyDeviceToHost));
+  return EP_OK;
+}
+
+int ep_set_explicit_weights(ep_ctx* c, uint32_t n, const float* w) {
+  if (!c || (!w && n)) return fail(EP_EINVAL, "bad argument");
+  c->explicit_w.assign(w, w + (size_t)n * list_cap(c->cfg));
+  return EP_OK;
+}
+
 int ep_step_explicit(ep_ctx* c, const uint32_t* anchors, uint32_t n, const uint32_t* nbr_cnt, const uint32_t* nbr,
                      const uint32_t* neg, uint32_t pass, float lr, int apply_update, float* loss, float* cons,
                      uint32_t* n_unique, uint32_t* feat, float* grad) {
@@ -1007,6 +1056,11 @@ int ep_step_explicit(ep_ctx* c, const uint32_t* anchors, uint32_t n, const uint3
   EP_HIP(hipMemcpy(sc.nbr_cnt, nbr_cnt, n * sizeof(uint32_t), hipMemcpyHostToDevice));
   EP_HIP(hipMemcpy(sc.nbr, nbr, (size_t)n * S * sizeof(uint32_t), hipMemcpyHostToDevice));
   EP_HIP(hipMemcpy(sc.neg, neg, (size_t)n * K * sizeof(uint32_t), hipMemcpyHostToDevice));
+  if (sc.wavg) {
+    if (c->explicit_w.size() != (size_t)n * S)
+      return fail(EP_ESTATE, "weighted_neighbor_averaging: set n x max_nodes neighbour weights first");
+    EP_HIP(hipMemcpy(sc.nbr_w, c->explicit_w.data(), (size_t)n * S * sizeof(float), hipMemcpyHostToDevice));
+  }
   EP_TRY(set_pass(c, c->d_state, pass, nullptr));
   const bool upd = apply_update != 0;
   // run everything up to (but excluding) the update, read the merged gradient, then update

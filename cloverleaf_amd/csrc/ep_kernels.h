@@ -89,6 +89,15 @@ struct StepConfig {
   uint32_t rbatch;         // B, the reference batch size
   uint32_t S;              // neighbour cap (Fixed) ; 0 with all_nbrs
   int all_nbrs;            // 1: use every neighbour (Sample::All)
+  // Weighted neighbours (model.rs:342-386).  With wavg every occurrence of neighbour j
+  // in ~h(v) counts w_j + 1, w_j = scale * p_j * deg (p_j the edge's probability from the
+  // CDF); its gradient goes through its own adjoint row nrow_base + nbr_base(b) + j.
+  const float* cdf;        // per-edge CDF of each row (CumCSR) or nullptr
+  float* nbr_w;            // [B*S] w_j of the sampled neighbours (Fixed)
+  uint32_t* nbroff;        // [B] first neighbour row of each example (all_nbrs)
+  int wavg;                // weighted_neighbor_averaging
+  int wsample;             // weighted_neighbor_sampling
+  uint32_t nrow_base;      // first per-neighbour adjoint row
   uint32_t K;              // negatives
   float gamma;
   int loss_kind;           // EP_LOSS_*: 0 margin, 1 contrastive, 2 starspace, 3 rank,

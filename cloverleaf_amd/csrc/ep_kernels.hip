@@ -304,6 +304,93 @@ __device__ inline void sample_negatives(const StepConfig& c, uint32_t v, uint64_
     negs[k] = c.negpool[uniform_below(draw_word(pos, pass, RNG_NEGATIVES, k, k0, k1), c.n_negpool)];
 }
 
+// CDFtoP (graph.rs:630-661): probability of edge j of the row starting at s0.
+__device__ inline float edge_prob(const StepConfig& c, uint64_t s0, uint32_t j) {
+  return j == 0u ? c.cdf[s0] : c.cdf[s0 + j] - c.cdf[s0 + j - 1u];
+}
+
+// Order-preserving map of a float to an unsigned key (larger float, larger key).
+__device__ inline uint32_t float_order_key(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// weighted_reservoir_sample (utils.rs:147-161) keeps the n items with the largest
+// u^(1/w); with w_j = scale * p_j * deg the common factor leaves the order of
+// log(u_j) / p_j unchanged, so that is the key (u_j: draw j of the neighbour stream).
+__device__ inline uint32_t ares_key(const StepConfig& c, uint64_t s0, uint32_t j, uint32_t pos, uint32_t pass,
+                                    uint32_t k0, uint32_t k1) {
+  const float u = uniform01(draw_word(pos, pass, RNG_NEIGHBOURS, j, k0, k1));
+  return float_order_key(logf(u) / edge_prob(c, s0, j));
+}
+
+// One wave: the S of deg edges with the largest keys, by an MSB-first radix select
+// (8-bit digits, per-wave histogram in LDS; the keys are recomputed each pass), ties
+// at the threshold taken in edge order.  Writes the neighbours to out[0..S) in edge
+// order and their weights scale * p_j * deg to w[0..S).
+__device__ inline void ares_select(const StepConfig& c, uint64_t s0, uint32_t deg, uint32_t pos, uint32_t pass,
+                                   uint32_t k0, uint32_t k1, uint32_t* hist, uint32_t* out, float* w, int lane) {
+  const uint32_t S = c.S;
+  uint32_t prefix = 0u, pmask = 0u, need = S;
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    for (int i = lane; i < 256; i += 64) hist[i] = 0u;
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t j = lane; j < deg; j += 64u) {
+      const uint32_t k = ares_key(c, s0, j, pos, pass, k0, k1);
+      if ((k & pmask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+    }
+    __builtin_amdgcn_wave_barrier();
+    // lane l owns digits 4l..4l+3; above = elements with a larger digit
+    uint32_t h[4], own = 0u;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { h[q] = hist[4 * lane + q]; own += h[q]; }
+    uint32_t suf = own;  // inclusive suffix sum over lanes >= l
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t t = (uint32_t)__shfl_down((int)suf, off, EP_WAVE);
+      if (lane + off < 64) suf += t;
+    }
+    uint32_t above = suf - own, dsel = 0u, asel = 0u;
+    bool found = false;
+#pragma unroll
+    for (int q = 3; q >= 0; --q) {
+      if (!found && above < need && need <= above + h[q]) {
+        found = true;
+        dsel = 4u * (uint32_t)lane + (uint32_t)q;
+        asel = above;
+      }
+      above += h[q];
+    }
+    const uint64_t m = __ballot(found);
+    const int src = __ffsll((unsigned long long)m) - 1;
+    dsel = (uint32_t)__shfl((int)dsel, src);
+    asel = (uint32_t)__shfl((int)asel, src);
+    need -= asel;
+    prefix |= dsel << shift;
+    pmask |= 0xFFu << shift;
+    __builtin_amdgcn_wave_barrier();
+  }
+  const float scale = (float)deg / (float)S;
+  uint32_t taken = 0u, eq_seen = 0u;
+  for (uint32_t j0 = 0; j0 < deg; j0 += 64u) {
+    const uint32_t j = j0 + (uint32_t)lane;
+    uint32_t k = 0u;
+    if (j < deg) k = ares_key(c, s0, j, pos, pass, k0, k1);
+    const bool eq = j < deg && k == prefix;
+    const uint64_t me = __ballot(eq);
+    const uint32_t eq_rank = eq_seen + (uint32_t)__popcll(me & ((1ull << lane) - 1ull));
+    const bool sel = j < deg && (k > prefix || (eq && eq_rank < need));
+    const uint64_t ms = __ballot(sel);
+    if (sel) {
+      const uint32_t at = taken + (uint32_t)__popcll(ms & ((1ull << lane) - 1ull));
+      out[at] = c.indices[s0 + j];
+      w[at] = scale * edge_prob(c, s0, j) * (float)deg;
+    }
+    taken += (uint32_t)__popcll(ms);
+    eq_seen += (uint32_t)__popcll(me);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_sample(StepConfig c, StepState* st) {
   const int lane = threadIdx.x & 63;
   const uint32_t b = blockIdx.x * NW + (threadIdx.x >> 6);
@@ -357,6 +444,12 @@ __global__ __launch_bounds__(256) void k_sample(StepConfig c, StepState* st) {
     } else if (deg <= c.S) {
       cnt = deg;
       for (uint32_t j = lane; j < deg; j += 64u) out[j] = c.indices[s0 + j];
+      if (c.wavg)  // Fixed(S) with deg <= S: scale deg/deg = 1
+        for (uint32_t j = lane; j < deg; j += 64u) c.nbr_w[(size_t)b * c.S + j] = edge_prob(c, s0, j) * (float)deg;
+    } else if (c.wavg && c.wsample) {
+      cnt = c.S;
+      __shared__ uint32_t s_hist[NW][256];
+      ares_select(c, s0, deg, pos, pass, k0, k1, s_hist[threadIdx.x >> 6], out, c.nbr_w + (size_t)b * c.S, lane);
     } else {
       cnt = c.S;
       // Floyd: for i = 0..S-1 draw t in [0, deg-S+i]; take deg-S+i instead when t was
@@ -373,6 +466,11 @@ __global__ __launch_bounds__(256) void k_sample(StepConfig c, StepState* st) {
         __builtin_amdgcn_wave_barrier();
       }
       for (uint32_t j = lane; j < c.S; j += 64u) out[j] = c.indices[s0 + pk[j]];
+      if (c.wavg) {
+        const float scale = (float)deg / (float)c.S;
+        for (uint32_t j = lane; j < c.S; j += 64u)
+          c.nbr_w[(size_t)b * c.S + j] = scale * edge_prob(c, s0, pk[j]) * (float)deg;
+      }
     }
     sample_negatives(c, v, s0, deg, pos, pass, k0, k1, negs, lane);
     if (lane == 0) c.nbr_cnt[b] = cnt;
@@ -398,46 +496,51 @@ __global__ __launch_bounds__(256) void k_sample(StepConfig c, StepState* st) {
 // an example with neighbours) -> blkoff.
 // ------------------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void k_scan(StepConfig c, StepState* st) {
-  __shared__ uint32_t s_w[16], s_w2[16];
-  __shared__ uint32_t s_carry, s_carry2;
+  __shared__ uint32_t s_w[16], s_w2[16], s_w3[16];
+  __shared__ uint32_t s_carry, s_carry2, s_carry3;
   __shared__ unsigned long long s_nodes;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const bool att = c.model == 1;
-  if (tid == 0) { s_carry = 0u; s_carry2 = 0u; s_nodes = 0ull; }
+  const bool nrows = c.wavg && c.all_nbrs;  // per-neighbour adjoint rows at scanned offsets
+  if (tid == 0) { s_carry = 0u; s_carry2 = 0u; s_carry3 = 0u; s_nodes = 0ull; }
   __syncthreads();
   uint64_t nodes = 0;
   for (uint32_t base = 0; base < c.batch; base += 1024) {
     const uint32_t i = base + tid;
     const uint32_t x = i < c.batch ? c.rcount[i] : 0u;
-    uint32_t y = 0;
+    uint32_t y = 0, z = 0;
     if (i < c.batch && i < st->batch_n) {
       const uint32_t cn = c.nbr_cnt[i];
       if (cn > 0u) {
         nodes += cn + c.K;
         y = 1u + cn + c.K;
+        z = cn;
       }
     }
-    uint32_t inc = x, inc2 = y;
+    uint32_t inc = x, inc2 = y, inc3 = z;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
       const uint32_t t1 = __shfl_up(inc, off, EP_WAVE);
       const uint32_t t2 = __shfl_up(inc2, off, EP_WAVE);
-      if (lane >= off) { inc += t1; inc2 += t2; }
+      const uint32_t t3 = __shfl_up(inc3, off, EP_WAVE);
+      if (lane >= off) { inc += t1; inc2 += t2; inc3 += t3; }
     }
-    if (lane == 63) { s_w[w] = inc; s_w2[w] = inc2; }
+    if (lane == 63) { s_w[w] = inc; s_w2[w] = inc2; s_w3[w] = inc3; }
     __syncthreads();
-    uint32_t pre = s_carry, tot = 0, pre2 = s_carry2, tot2 = 0;
+    uint32_t pre = s_carry, tot = 0, pre2 = s_carry2, tot2 = 0, pre3 = s_carry3, tot3 = 0;
     for (int q = 0; q < 16; ++q) {
-      if (q < w) { pre += s_w[q]; pre2 += s_w2[q]; }
+      if (q < w) { pre += s_w[q]; pre2 += s_w2[q]; pre3 += s_w3[q]; }
       tot += s_w[q];
       tot2 += s_w2[q];
+      tot3 += s_w3[q];
     }
     if (i < c.batch) {
       c.keyoff[i] = pre + inc - x;
       if (att) c.blkoff[i] = pre2 + inc2 - y;
+      if (nrows) c.nbroff[i] = pre3 + inc3 - z;
     }
     __syncthreads();
-    if (tid == 0) { s_carry += tot; s_carry2 += tot2; }
+    if (tid == 0) { s_carry += tot; s_carry2 += tot2; s_carry3 += tot3; }
     __syncthreads();
   }
   atomicAdd(&s_nodes, (unsigned long long)nodes);
@@ -548,11 +651,20 @@ __device__ inline void gather_lds_wide(const T* __restrict__ table, const uint32
   }
 }
 
+// Weight w_j of neighbour j of example b (model.rs:357-361): the sampler's value for
+// Fixed(S), p_j * deg (scale 1) with every neighbour; and its own adjoint row.
+__device__ inline float nbr_weight(const StepConfig& c, uint32_t b, uint32_t j, uint64_t s0, uint32_t deg) {
+  return c.all_nbrs ? edge_prob(c, s0, j) * (float)deg : c.nbr_w[(size_t)b * c.S + j];
+}
+__device__ inline uint32_t nbr_row(const StepConfig& c, uint32_t b, uint32_t j) {
+  return c.nrow_base + (c.all_nbrs ? c.nbroff[b] : b * c.S) + j;
+}
+
 // Loss, adjoints and eligibility of one example for the losses other than margin (block
 // of k_forward; xc holds a, p, n_1..n_K, sn the occurrence totals).
 template <int D>
 __device__ inline void forward_other_loss(const StepConfig& c, StepState* st, uint32_t b, uint32_t deg, const float* xc,
-                                          const float* sn, uint8_t* s_el) {
+                                          const float* sn, uint8_t* s_el, float* stash) {
   constexpr int PL = RowShape<D>::PL;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const uint32_t K = c.K, NC = K + 2u;
@@ -630,6 +742,11 @@ __device__ inline void forward_other_loss(const StepConfig& c, StepState* st, ui
 #pragma unroll
       for (int q = 0; q < PL; ++q) g[q] *= co;
       store_slice<D>(c.adj + ((size_t)b * NC + ci) * D, lane, g);
+      if (c.wavg && ci == 1u) {
+#pragma unroll
+        for (int q = 0; q < PL; ++q)
+          if (base + q < D) stash[base + q] = g[q];
+      }
     }
     if (lane == 0) {
       c.elig[(size_t)b * NC + ci] = e ? 1 : 0;
@@ -691,6 +808,7 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
 
   __shared__ uint32_t s_wave[NW];
   __shared__ uint32_t s_np[NW];
+  __shared__ float s_npw[NW];
   // Staged path: the example's occurrence list is copied to LDS first (node ranges, then
   // feature ids, one block-wide round trip each), so every wave streams rows from a
   // contiguous id range with GU loads in flight.  Entries: e = 0 (v), 1..cnt
@@ -727,7 +845,27 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
     const uint32_t p0 = ent_off[1], p1 = ent_off[1 + cnt];
     const uint32_t per = (p1 - p0 + NW - 1) / NW;
     const uint32_t a0 = min(p1, p0 + w * per), a1 = min(p1, a0 + per);
-    if constexpr (WideShape<D, T>::OK) {
+    if (c.wavg) {
+      // weighted neighbours: neighbour j's occurrences count w_j + 1 (dealt round-robin)
+      float acc[PL];
+#pragma unroll
+      for (int q = 0; q < PL; ++q) acc[q] = 0.f;
+      float nw = 0.f;
+      for (uint32_t j = w; j < cnt; j += NW) {
+        const float cj = nbr_weight(c, b, j, vs, deg) + 1.f;
+        float t[PL];
+#pragma unroll
+        for (int q = 0; q < PL; ++q) t[q] = 0.f;
+        gather_lds<D, GU>(table, ids, ent_off[1 + j], ent_off[2 + j], lane, t);
+#pragma unroll
+        for (int q = 0; q < PL; ++q) acc[q] += cj * t[q];
+        nw += cj * (float)(ent_off[2 + j] - ent_off[1 + j]);
+      }
+#pragma unroll
+      for (int q = 0; q < PL; ++q)
+        if (base + q < D) part[w * D + base + q] = acc[q];
+      if (lane == 0) s_npw[w] = nw;
+    } else if constexpr (WideShape<D, T>::OK) {
       gather_lds_wide<D, WU, T>(table, ids, a0, a1, lane, part + w * D, 1.0f);
     } else {
       float acc[PL];
@@ -764,12 +902,25 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
 #pragma unroll
     for (int q = 0; q < PL; ++q) acc[q] = 0.f;
     uint32_t np = 0;
+    float nw = 0.f;
     for (uint32_t j = w; j < cnt; j += NW) {
       const uint32_t u = pn[j];
       const uint64_t fs = c.findptr[u], fe = c.findptr[u + 1];
-      gather_rows<D, U>(table, c.fids, fs, fe, lane, acc);
+      if (c.wavg) {
+        const float cj = nbr_weight(c, b, j, vs, deg) + 1.f;
+        float t[PL];
+#pragma unroll
+        for (int q = 0; q < PL; ++q) t[q] = 0.f;
+        gather_rows<D, U>(table, c.fids, fs, fe, lane, t);
+#pragma unroll
+        for (int q = 0; q < PL; ++q) acc[q] += cj * t[q];
+        nw += cj * (float)(fe - fs);
+      } else {
+        gather_rows<D, U>(table, c.fids, fs, fe, lane, acc);
+      }
       np += (uint32_t)(fe - fs);
     }
+    if (lane == 0) s_npw[w] = nw;
 #pragma unroll
     for (int q = 0; q < PL; ++q)
       if (base + q < D) part[w * D + base + q] = acc[q];
@@ -792,6 +943,10 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
     if (tid == 0) sn[1] = (float)(s_np[0] + s_np[1] + s_np[2] + s_np[3]);
     __syncthreads();
   }
+  if (c.wavg) {
+    if (tid == 0) sn[1] = ((s_npw[0] + s_npw[1]) + s_npw[2]) + s_npw[3];
+    __syncthreads();
+  }
   {
     const float ntot = sn[1];
     for (int t = tid; t < D; t += blockDim.x)
@@ -799,8 +954,7 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
   }
   __syncthreads();
   if (c.loss_kind != 0 && c.loss_kind != 5) {
-    forward_other_loss<D>(c, st, b, deg, xc, sn, s_el);
-    if (!(staged && c.fuse_emit)) return;
+    forward_other_loss<D>(c, st, b, deg, xc, sn, s_el, part);
     if (!c.train) return;
   } else {
   // distances: D0 = |p - a|, D_i = |p - n_i|
@@ -898,6 +1052,11 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
 #pragma unroll
       for (int q = 0; q < PL; ++q) g[q] *= co;
       store_slice<D>(c.adj + ((size_t)b * NC + ci) * D, lane, g);
+      if (c.wavg && ci == 1u) {
+#pragma unroll
+        for (int q = 0; q < PL; ++q)
+          if (base + q < D) part[base + q] = g[q];
+      }
     }
     if (lane == 0) {
       c.elig[(size_t)b * NC + ci] = e ? 1 : 0;
@@ -905,6 +1064,19 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
       s_el[ci] = e ? 1 : 0;
     }
   }
+  }
+  if (c.wavg) {
+    // weighted neighbours: neighbour j's adjoint row is (dL/dp / sum of counts) * (w_j + 1)
+    __syncthreads();
+    if (s_el[1]) {
+      for (uint32_t j = w; j < cnt; j += NW) {
+        const float cj = nbr_weight(c, b, j, vs, deg) + 1.f;
+        float g[PL];
+#pragma unroll
+        for (int q = 0; q < PL; ++q) g[q] = (base + q < D) ? part[base + q] * cj : 0.f;
+        store_slice<D>(c.adj + (size_t)nbr_row(c, b, j) * D, lane, g);
+      }
+    }
   }
   // Key emission for staged examples (k_emit handles the rest): one (feature, slot) key
   // per occurrence, sentinel unless the construction is eligible and, under first_leaf,
@@ -917,30 +1089,33 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
   const bool first_leaf = c.grad_mode == 0;
   uint32_t* hsh = ids + c.ids_cap;
   const uint32_t hmask = c.hash_entries - 1u;
-  auto cons_of = [&](uint32_t o) -> uint32_t {
+  auto entry_of = [&](uint32_t o) -> uint32_t {
     uint32_t lo = 0, hi = E;  // last entry e with ent_off[e] <= o
     while (hi - lo > 1u) {
       const uint32_t mid = (lo + hi) >> 1;
       if (ent_off[mid] <= o) lo = mid; else hi = mid;
     }
-    return lo == 0u ? 0u : (lo <= cnt ? 1u : 2u + (lo - 1u - cnt));
+    return lo;
   };
+  auto cons_of_entry = [&](uint32_t e) -> uint32_t { return e == 0u ? 0u : (e <= cnt ? 1u : 2u + (e - 1u - cnt)); };
   if (first_leaf) {
     for (uint32_t i = tid; i < c.hash_entries; i += blockDim.x) hsh[i] = HASH_EMPTY;
     __syncthreads();
     for (uint32_t o = tid; o < tot; o += blockDim.x) {
-      const uint32_t ci = cons_of(o);
+      const uint32_t ci = cons_of_entry(entry_of(o));
       if (s_el[ci]) hash_insert_min(hsh, hmask, ids[o], ci);
     }
     __syncthreads();
   }
   for (uint32_t o = tid; o < tot; o += blockDim.x) {
-    const uint32_t ci = cons_of(o);
+    const uint32_t e = entry_of(o);
+    const uint32_t ci = cons_of_entry(e);
     const uint32_t f = ids[o];
     uint32_t key = c.sentinel;
     if (s_el[ci] && (!first_leaf || hash_lookup(hsh, hmask, f) == ci)) key = key_of_row(c, f);
     c.keys[0][kb + o] = key;
-    c.slots[0][kb + o] = ((b / c.rbatch) << kSlotBatchShift) | (b * NC + ci);
+    const uint32_t row = (c.wavg && ci == 1u) ? nbr_row(c, b, e - 1u) : b * NC + ci;
+    c.slots[0][kb + o] = ((b / c.rbatch) << kSlotBatchShift) | row;
   }
 }
 
@@ -1026,7 +1201,8 @@ __global__ __launch_bounds__(256) void k_emit(StepConfig c, StepState* st) {
           c.keys[0][pos] = emit ? key_of_row(c, f) : c.sentinel;
           // mean model: the slot of the construction's adjoint; attention: the
           // occurrence's own gradient row
-          c.slots[0][pos] = ((b / c.rbatch) << kSlotBatchShift) | (att ? pos : slot0 + cj);
+          const uint32_t row = att ? pos : ((c.wavg && cj == 1u) ? nbr_row(c, b, cs + j - 1u) : slot0 + cj);
+          c.slots[0][pos] = ((b / c.rbatch) << kSlotBatchShift) | row;
         }
       }
       running += total;

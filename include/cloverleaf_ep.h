@@ -83,8 +83,13 @@ typedef struct ep_config {
                                /* used (RandomWalkHardSampler, node_sampler.rs:83-107)*/
   float noise;                 /* gradient noise sd at step 0, decaying to 1e-9 over */
                                /* the run (mod.rs:139-147, 238-247); 0 = off       */
-  int32_t weighted_neighbor_sampling;   /* must be 0                                */
-  int32_t weighted_neighbor_averaging;  /* must be 0                                */
+  int32_t weighted_neighbor_sampling;   /* reservoir by edge weight (model.rs:373-376);*/
+                                        /* changes the sample only together with     */
+                                        /* weighted_neighbor_averaging               */
+  int32_t weighted_neighbor_averaging;  /* ~h(v) counts scale*p_j*deg + 1 per        */
+                                        /* occurrence of neighbour j (model.rs:357-  */
+                                        /* 361); needs ep_set_edge_weights; mean     */
+                                        /* model only                                */
   int32_t precision;           /* EP_PREC_*                                         */
   int32_t grad_mode;           /* EP_GRAD_*                                         */
   int32_t scatter;             /* EP_SCATTER_*                                      */
@@ -156,6 +161,12 @@ void ep_destroy(ep_ctx* ctx);
  * are used by the unweighted path).  indptr[n_nodes+1], indices[indptr[n]].    */
 int ep_set_graph(ep_ctx* ctx, uint64_t n_nodes, const uint64_t* indptr, const uint32_t* indices);
 
+/* Uploads the per-edge CDF of each row (CumCSR weights, graph.rs:414-422),
+ * cdf[indptr[n]], after ep_set_graph.  Edge j of a row has probability
+ * cdf[j] - cdf[j-1] (CDFtoP, graph.rs:630-661).  Needed for
+ * weighted_neighbor_averaging.                                                  */
+int ep_set_edge_weights(ep_ctx* ctx, const float* cdf);
+
 /* Uploads the FeatureStore (src/feature_store.rs:19-25) as a CSR: node i has
  * features f_ids[f_indptr[i]..f_indptr[i+1]).  Every node must have >= 1 feature
  * (FeatureStore::fill_missing_nodes, feature_store.rs:83-89).                  */
@@ -191,6 +202,14 @@ int ep_learn(ep_ctx* ctx, ep_progress_fn progress, void* user);
  * neg[n*K].  Requires max_nodes Fixed.                                          */
 int ep_sample_batch(ep_ctx* ctx, const uint32_t* anchors, uint32_t n, uint32_t pass, uint32_t* nbr_cnt,
                     uint32_t* nbr, uint32_t* neg);
+
+/* Test hook: the neighbour weights w_j (model.rs:357-361) of the last
+ * ep_sample_batch, w[n*S] (weighted_neighbor_averaging with max_nodes Fixed).   */
+int ep_sample_weights(ep_ctx* ctx, uint32_t n, float* w);
+
+/* Test hook: neighbour weights w[n*S] for the next ep_step_explicit calls
+ * (needed with weighted_neighbor_averaging).                                    */
+int ep_set_explicit_weights(ep_ctx* ctx, uint32_t n, const float* w);
 
 /* Test hook: one training step on explicit samples (the body of the per-batch
  * closure, mod.rs:176-252).  anchors[n], nbr_cnt[n], nbr[n*S] (S = max_nodes),

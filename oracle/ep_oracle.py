@@ -188,17 +188,35 @@ def build_anchor_construction(f_indptr, f_ids, v) -> Construction:
     return c
 
 
-def build_neighbour_construction(f_indptr, f_ids, nbrs: Sequence[int], scale_weight: float) -> Construction:
+def build_neighbour_construction(f_indptr, f_ids, nbrs: Sequence[int], scale_weight) -> Construction:
     """~h(v): reconstruct_node_embedding (model.rs:342-386) -> construct_from_multiple_nodes (388-416).
 
-    ``nbrs`` is the explicit sampled neighbour list; ``scale_weight`` is the
-    unbiasing scale of ``Sample::sample`` (deg/S when subsampled, 1 otherwise),
-    used as each neighbour's weight (model.rs:360).
+    ``nbrs`` is the explicit sampled neighbour list.  ``scale_weight`` is each
+    neighbour's weight (model.rs:357-361): one number for every neighbour (the
+    unbiasing scale of ``Sample::sample``, deg/S when subsampled, 1 otherwise), or
+    one weight per neighbour (weighted averaging, see ``neighbour_weights``).
     """
     c = Construction()
-    for u in nbrs:
-        collect_node(c, node_features(f_indptr, f_ids, u), scale_weight)
+    ws = [float(scale_weight)] * len(nbrs) if np.isscalar(scale_weight) else [float(x) for x in scale_weight]
+    for u, w in zip(nbrs, ws):
+        collect_node(c, node_features(f_indptr, f_ids, u), w)
     return c
+
+
+def neighbour_weights(cdf_row: np.ndarray, picked: Sequence[int], n_sample: int, weighted_averaging: bool):
+    """Weights of the picked edges of one row (model.rs:355-361), in f32 as the reference.
+
+    ``Sample::sample`` gives (r, deg/r) (deg/deg = 1 when every neighbour is taken); with
+    weighted averaging neighbour j weighs scale * p_j * deg, p_j = CDFtoP(cdf)[j]
+    (graph.rs:630-661), else scale.  ``picked`` are positions in the row.
+    """
+    deg = len(cdf_row)
+    r = min(n_sample, deg)
+    scale = np.float32(deg) / np.float32(r)
+    if not weighted_averaging:
+        return [float(scale)] * len(picked)
+    p = cdf_to_p(np.asarray(cdf_row, dtype=np.float32)).astype(np.float32)
+    return [float(scale * p[j] * np.float32(deg)) for j in picked]
 
 
 def construct_mean(table: np.ndarray, con: Construction) -> np.ndarray:

@@ -49,9 +49,10 @@ def make_problem(seed=0, n=120, V=300, D=32, avg_deg=6, fmax=8, zipf=1.1, zero_d
 
 
 def oracle_step(prob: Problem, anchors, cnt, nbr, neg, gamma, lr, pass_, grad_mode, table=None, mom=None, var=None,
-                loss_divs=None, attention=None, loss=None):
+                loss_divs=None, attention=None, loss=None, nbr_weights=None):
     """FP64 oracle of one explicit-sample step; returns (results, merged, table, mom, var).
-    attention = (heads, d_k) selects the attention model; loss = (kind, params) the loss."""
+    attention = (heads, d_k) selects the attention model; loss = (kind, params) the loss;
+    nbr_weights[i][j] is neighbour j's weight (weighted averaging; default uniform)."""
     t = (prob.table if table is None else table).astype(np.float64).copy()
     m = np.zeros_like(t) if mom is None else mom.astype(np.float64).copy()
     v = np.zeros_like(t) if var is None else var.astype(np.float64).copy()
@@ -61,7 +62,9 @@ def oracle_step(prob: Problem, anchors, cnt, nbr, neg, gamma, lr, pass_, grad_mo
     # anchors with no neighbourhood are skipped (no example)
     keep = [i for i in range(B) if cnt[i] > 0]
     res_k, merged = O.ep_batch_step(t, m, v, prob.f_indptr, prob.f_ids, [int(anchors[i]) for i in keep],
-                                    [nbr_lists[i] for i in keep], [1.0] * len(keep),
+                                    [nbr_lists[i] for i in keep],
+                                    [1.0] * len(keep) if nbr_weights is None
+                                    else [list(map(float, nbr_weights[i][:cnt[i]])) for i in keep],
                                     [list(map(int, neg[i])) for i in keep], gamma, lr, pass_, grad_mode,
                                     None if loss_divs is None else [loss_divs[i] for i in keep], attention, loss)
     it = iter(res_k)

@@ -108,3 +108,186 @@ def test_hard_negatives_zero_is_uniform_and_reproducible():
     got = np.bincount(n0.ravel(), minlength=prob.n)
     exp = n0.size / prob.n
     assert float(((got - exp) ** 2 / exp).sum()) < 40  # 9 dof
+
+
+# --------------------------------------------------------------------------------------
+# Weighted neighbour sampling and averaging (model.rs:342-386, utils.rs:147-161)
+# --------------------------------------------------------------------------------------
+def weighted_problem(seed, n=150, V=300, D=32, hub_deg=40):
+    """A random graph with random positive edge weights (per-row CDFs) and a few hubs."""
+    from tests.problems import make_problem
+    prob = make_problem(seed=seed, n=n, V=V, D=D)
+    # add hub rows: nodes 0..2 connect to hub_deg distinct others
+    rng = np.random.default_rng(seed)
+    rows = [set(prob.indices[int(prob.indptr[i]):int(prob.indptr[i + 1])].tolist()) for i in range(n)]
+    for h in range(3):
+        rows[h] |= set(rng.choice(np.arange(3, n), size=hub_deg, replace=False).tolist())
+    indptr = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.uint64)
+    indices = np.array([j for r in rows for j in sorted(r)], dtype=np.uint32)
+    w = rng.uniform(0.05, 1.0, size=len(indices)).astype(np.float32)
+    cdf = np.empty_like(w)
+    for i in range(n):
+        s, e = int(indptr[i]), int(indptr[i + 1])
+        if e > s:
+            c = np.cumsum(w[s:e], dtype=np.float32)
+            cdf[s:e] = c / c[-1]
+    prob.indptr, prob.indices = indptr, indices
+    return prob, cdf
+
+
+def weighted_ctx(prob, cdf, S, K, wsample, B=48, all_nbrs=False, seed=20222022, passes=1, lr=None,
+                 grad_mode=N.EP_GRAD_FIRST_LEAF):
+    cfg = N.default_config()
+    cfg.grad_mode = grad_mode
+    cfg.dims, cfg.batch_size, cfg.negatives, cfg.gamma = prob.D, B, K, 0.5
+    if all_nbrs:
+        cfg.max_nodes_kind = N.EP_SAMPLE_ALL
+    else:
+        cfg.max_nodes_kind, cfg.max_nodes = N.EP_SAMPLE_FIXED, S
+    cfg.weighted_neighbor_averaging, cfg.weighted_neighbor_sampling = 1, 1 if wsample else 0
+    cfg.valid_pct = 0.0
+    cfg.seed = seed
+    cfg.passes = passes
+    if lr is not None:
+        cfg.alpha = lr
+    ctx = N.Context(cfg)
+    ctx.set_graph(prob.indptr, prob.indices)
+    ctx.set_edge_weights(cdf)
+    ctx.set_features(prob.f_indptr, prob.f_ids, prob.V)
+    ctx.set_table(prob.table)
+    return ctx
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+@pytest.mark.parametrize("wsample", [0, 1])
+@pytest.mark.parametrize("grad_mode", ["first_leaf", "exact"])
+def test_weighted_averaging_step_matches_oracle(wsample, grad_mode):
+    from oracle import ep_oracle as O
+    from tests.problems import dense_grad, oracle_step
+    prob, cdf = weighted_problem(7 + wsample)
+    S, K = 6, 5
+    ctx = weighted_ctx(prob, cdf, S, K, wsample,
+                       grad_mode=N.EP_GRAD_EXACT if grad_mode == "exact" else N.EP_GRAD_FIRST_LEAF)
+    anchors = np.concatenate([[0, 1, 2], np.random.default_rng(3).choice(np.arange(3, prob.n), 45, replace=False)])
+    anchors = anchors.astype(np.uint32)
+    cnt, nbr, neg = ctx.sample_batch(anchors, 1)
+    w = ctx.sample_weights(len(anchors))
+    # the sampler's weights are the reference's scale * p_j * deg, bit for bit
+    for i, v in enumerate(anchors):
+        s, e = int(prob.indptr[v]), int(prob.indptr[v + 1])
+        row = prob.indices[s:e].tolist()
+        picked = [row.index(int(u)) for u in nbr[i, :cnt[i]]]
+        assert len(set(picked)) == len(picked)
+        expect = O.neighbour_weights(cdf[s:e], picked, S, True)
+        assert np.array_equal(w[i, :cnt[i]], np.array(expect, dtype=np.float32)), (i, w[i, :cnt[i]], expect)
+    assert np.all(cnt[:3] == S)
+    ctx.set_explicit_weights(w)
+    out = ctx.step_explicit(anchors, cnt, nbr, neg, pass_=1, lr=0.01, apply_update=True, want_cons=True)
+    res, merged, t64, m64, v64 = oracle_step(prob, anchors, cnt, nbr, neg, 0.5, 0.01, 1, grad_mode,
+                                             nbr_weights=w.astype(np.float64))
+    ref_loss = np.array([r.loss if (r is not None and r.contributes) else 0.0 for r in res])
+    print(f"[sampling] weighted averaging wsample={wsample} {grad_mode}: {np.count_nonzero(ref_loss)} examples",
+          flush=True)
+    assert rel(out["loss"], ref_loss) <= 1e-5
+    for i, r in enumerate(res):
+        if r is not None:
+            assert rel(out["cons"][i], np.stack(r.vectors)) <= 1e-5
+    G64 = dense_grad(merged, prob.V, prob.D)
+    Gg = np.zeros_like(G64)
+    assert set(out["feat"].tolist()) == set(merged.keys())
+    Gg[out["feat"]] = out["grad"]
+    assert rel(Gg, G64) <= 1e-5
+    m, v = ctx.get_adam()
+    rows = sorted(merged)
+    assert rel(m[rows], m64[rows]) <= 1e-5
+    ctx.close()
+
+
+def test_weighted_reservoir_inclusion_frequencies():
+    # one hub row of 16 edges with weights 1..16, S = 4; A-Res keeps the 4 largest u^(1/p)
+    n_leaf = 16
+    n = n_leaf + 1
+    indptr = np.array([0, n_leaf] + [n_leaf] * n_leaf, dtype=np.uint64)
+    indices = np.arange(1, n, dtype=np.uint32)
+    wts = np.arange(1, n_leaf + 1, dtype=np.float32)
+    cdf = (np.cumsum(wts) / wts.sum()).astype(np.float32)
+    f_indptr = np.arange(n + 1, dtype=np.uint64)
+    f_ids = np.arange(n, dtype=np.uint32)
+    prob = Problem(indptr, indices, f_indptr, f_ids, n, 8, np.ones((n, 8), np.float32))
+    S, B = 4, 8192
+    ctx = weighted_ctx(prob, cdf, S, 2, 1, B=B)
+    anchors = np.zeros(B, dtype=np.uint32)
+    cnt, nbr, _ = ctx.sample_batch(anchors, 1)
+    w = ctx.sample_weights(B)
+    ctx.close()
+    assert np.all(cnt == S)
+    assert all(len(set(r.tolist())) == S for r in nbr)
+    got = np.bincount(nbr.ravel(), minlength=n)[1:].astype(np.float64)
+    # numpy A-Res with the same keys
+    rng = np.random.default_rng(0)
+    M = 200000
+    p = np.diff(np.concatenate([[0.0], cdf.astype(np.float64)]))
+    keys = rng.random((M, n_leaf)) ** (1.0 / p)
+    top = np.argsort(-keys, axis=1)[:, :S]
+    sim = np.bincount(top.ravel(), minlength=n_leaf) / M * B
+    chi2 = float(((got - sim) ** 2 / sim).sum())
+    print(f"[sampling] weighted reservoir chi2 {chi2:.1f} (15 dof)", flush=True)
+    assert chi2 < 15 + 6 * np.sqrt(30)
+    # every weight is scale * p_j * deg with scale = 16/4
+    pj = np.diff(np.concatenate([[np.float32(0)], cdf])).astype(np.float32)
+    assert np.allclose(w, np.float32(4.0) * pj[nbr.astype(np.int64) - 1] * np.float32(16), rtol=1e-6)
+
+
+def test_weighted_all_neighbours_equals_fixed_cap():
+    # max_nodes=All and Fixed(S >= every degree) take every neighbour with scale 1: one
+    # training pass gives the same table (CDF read in the forward vs sampler weights)
+    prob, cdf = weighted_problem(11, n=120, V=200, D=32, hub_deg=30)
+    deg_max = int(np.diff(prob.indptr.astype(np.int64)).max())
+    tables = []
+    for all_nbrs in (True, False):
+        ctx = weighted_ctx(prob, cdf, deg_max, 4, 1, B=32, all_nbrs=all_nbrs, passes=2, lr=0.05)
+        ctx.learn()
+        tables.append(ctx.get_table())
+        ctx.close()
+    assert not np.array_equal(tables[0], prob.table)
+    assert np.allclose(tables[0], tables[1], rtol=0, atol=1e-6), np.abs(tables[0] - tables[1]).max()
+
+
+def test_weighted_averaging_requires_edge_weights():
+    prob, _ = weighted_problem(5)
+    cfg = N.default_config()
+    cfg.dims, cfg.batch_size, cfg.negatives = prob.D, 16, 2
+    cfg.max_nodes_kind, cfg.max_nodes = N.EP_SAMPLE_FIXED, 4
+    cfg.weighted_neighbor_averaging = 1
+    ctx = N.Context(cfg)
+    ctx.set_graph(prob.indptr, prob.indices)
+    ctx.set_features(prob.f_indptr, prob.f_ids, prob.V)
+    with pytest.raises(N.NativeError, match="edge weights"):
+        ctx.sample_batch(np.arange(4, dtype=np.uint32), 1)
+    ctx.close()
+
+
+def test_weighted_facade_learns():
+    import cloverleaf_amd as cl
+    prob, cdf = weighted_problem(13, n=100, V=150, D=16, hub_deg=20)
+    b = cl.GraphBuilder()
+    for i in range(prob.n):
+        s, e = int(prob.indptr[i]), int(prob.indptr[i + 1])
+        p = np.diff(np.concatenate([[0.0], cdf[s:e].astype(np.float64)]))
+        for j, w in zip(prob.indices[s:e], p):
+            b.add_edge(("n", str(i)), ("n", str(int(j))), float(w), cl.EdgeType.Directed)
+    g = b.build_graph()
+    fs = cl.FeatureSet.new_from_graph(g)
+    for i in range(prob.n):
+        if g.contains_node(("n", str(i))):
+            fs.set_features(("n", str(i)), [("f", str(int(x))) for x in prob.f_ids[prob.f_indptr[i]:prob.f_indptr[i + 1]]])
+    ep = cl.EmbeddingPropagator(alpha=0.05, loss=cl.EPLoss.margin(0.5, 3), batch_size=16, dims=16, passes=3,
+                                max_nodes=5, weighted_neighbor_sampling=True, weighted_neighbor_averaging=True,
+                                indicator=False)
+    emb = ep.learn_features(g, fs)
+    assert emb.len() == fs.num_features() and emb.dims() == 16
+    assert np.all(np.isfinite(emb.embeddings))
+    assert ep.history[-1]["train"] < ep.history[0]["train"]

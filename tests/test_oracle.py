@@ -201,18 +201,19 @@ def test_exact_grads_central_fd(seed):
             assert abs(fd - g) < 1e-6, (f, j, fd, g)
 
 
-def torch_first_leaf(table, f_indptr, f_ids, v, nbrs, negs, gamma):
+def torch_first_leaf(table, f_indptr, f_ids, v, nbrs, negs, gamma, nbr_weights=None):
     """The reference's mechanism (model.rs:257-278, mod.rs:329-378), in torch fp64:
     one autograd leaf per (construction, feature); keep the first leaf's gradient per
     feature in the order a, p, n_1..K; leaves the loss does not reach have no grad."""
-    def spec(nodes, weight):
+    def spec(nodes, weights):
         out = []
-        for u in nodes:
+        for u, weight in zip(nodes, weights):
             for f in f_ids[f_indptr[u]:f_indptr[u + 1]]:
                 out.append((int(f), weight + 1.0))
         return out
 
-    specs = [spec([v], 1.0), spec(nbrs, 1.0)] + [spec([u], 1.0) for u in negs]
+    nw = [1.0] * len(nbrs) if nbr_weights is None else list(nbr_weights)
+    specs = [spec([v], [1.0]), spec(nbrs, nw)] + [spec([u], [1.0]) for u in negs]
     leaves, vecs = [], []
     for sp in specs:
         d = {}
@@ -254,6 +255,59 @@ def test_first_leaf_matches_per_leaf_autograd(seed):
     assert set(r.grads) == set(g)
     for f in g:
         assert np.allclose(r.grads[f], g[f], atol=1e-12)
+
+
+def test_weighted_neighbour_construction_closed_form():
+    # model.rs:357-361 + 264-276: neighbour j's occurrences count w_j + 1
+    f_indptr = np.array([0, 2, 3], dtype=np.uint64)
+    f_ids = np.array([0, 1, 1], dtype=np.uint32)
+    con = O.build_neighbour_construction(f_indptr, f_ids, [0, 1], [3.0, 0.5])
+    assert con.counts == {0: 4.0, 1: 5.5}
+    table = np.array([[1.0, 0.0], [0.0, 2.0]])
+    assert np.allclose(O.construct_mean(table, con), np.array([4.0, 11.0]) / 9.5)
+    # one weight for every neighbour: the unweighted path
+    assert (O.build_neighbour_construction(f_indptr, f_ids, [0, 1], [2.0, 2.0]).counts
+            == O.build_neighbour_construction(f_indptr, f_ids, [0, 1], 2.0).counts)
+
+
+def test_neighbour_weights_kat():
+    f = np.float32
+    cdf = np.array([0.2, 0.5, 1.0], dtype=np.float32)
+    # Fixed(2) of 3: scale 3/2; weighted: scale * p_j * deg
+    w = O.neighbour_weights(cdf, [0, 2], 2, True)
+    assert w == [float(f(1.5) * f(0.2) * f(3)), float(f(1.5) * (cdf[2] - cdf[1]) * f(3))]
+    assert np.allclose(w, [0.9, 2.25])
+    # every neighbour taken: scale 1
+    assert np.allclose(O.neighbour_weights(cdf, [0, 1, 2], 5, True), [0.6, 0.9, 1.5])
+    # unweighted averaging: the scale only
+    assert O.neighbour_weights(cdf, [1, 2], 2, False) == [1.5, 1.5]
+    # a uniform CDF gives every neighbour the scale
+    cdf_u = (np.arange(1, 5) / 4).astype(np.float32)
+    assert np.allclose(O.neighbour_weights(cdf_u, [0, 3], 2, True), [2.0, 2.0])
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_weighted_neighbours_exact_fd_and_first_leaf(seed):
+    table, f_indptr, f_ids, v, nbrs, negs = _toy_example(seed)
+    ws = list(np.random.default_rng(seed + 10).uniform(0.2, 3.0, size=len(nbrs)))
+    gamma = 2.0
+    r = O.example_grads(table, f_indptr, f_ids, v, nbrs, ws, negs, gamma, "exact")
+    assert r.contributes
+    eps = 1e-6
+    loss = lambda t: O.example_grads(t, f_indptr, f_ids, v, nbrs, ws, negs, gamma, "exact").loss
+    for fr in range(table.shape[0]):
+        for j in range(table.shape[1]):
+            tp, tm = table.copy(), table.copy()
+            tp[fr, j] += eps
+            tm[fr, j] -= eps
+            fd = (loss(tp) - loss(tm)) / (2 * eps)
+            g = r.grads[fr][j] if fr in r.grads else 0.0
+            assert abs(fd - g) < 1e-6, (fr, j, fd, g)
+    fl = O.example_grads(table, f_indptr, f_ids, v, nbrs, ws, negs, gamma, "first_leaf")
+    L, g = torch_first_leaf(table, f_indptr, f_ids, v, nbrs, negs, gamma, nbr_weights=ws)
+    assert abs(fl.loss - L) < 1e-12 and set(fl.grads) == set(g)
+    for fr in g:
+        assert np.allclose(fl.grads[fr], g[fr], atol=1e-12)
 
 
 def test_first_leaf_differs_from_exact_only_on_shared_features():
