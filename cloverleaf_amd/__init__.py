@@ -15,6 +15,7 @@ from .api import (  # noqa: F401
     Graph,
     GraphBuilder,
     LossWeighting,
+    NeighborhoodAligner,
     NodeEmbedder,
     NodeEmbeddings,
     Vocab,
@@ -22,5 +23,6 @@ from .api import (  # noqa: F401
 
 __all__ = [
     "Distance", "EdgeType", "EmbeddingPropagator", "EPLoss", "FeatureAggregator", "FeatureNamespace",
-    "FeatureSet", "Graph", "GraphBuilder", "LossWeighting", "NodeEmbedder", "NodeEmbeddings", "Vocab",
+    "FeatureSet", "Graph", "GraphBuilder", "LossWeighting", "NeighborhoodAligner", "NodeEmbedder",
+    "NodeEmbeddings", "Vocab",
 ]

@@ -10,6 +10,7 @@ import ctypes
 import os
 from ctypes import POINTER, Structure, c_char_p, c_double, c_float, c_int, c_int32, c_uint16, c_uint32, c_uint64, c_void_p
 from pathlib import Path
+from typing import Optional
 
 import numpy as np
 
@@ -112,6 +113,9 @@ SIGNATURES = {
     "ep_destroy": (None, [c_void_p]),
     "ep_set_graph": (c_int, [c_void_p, c_uint64, _u64p, _u32p]),
     "ep_set_edge_weights": (c_int, [c_void_p, _f32p]),
+    "ep_aligner_create": (c_int, [c_int, c_uint64, _u64p, _u32p, _f32p, c_uint32, _f32p, POINTER(c_void_p)]),
+    "ep_aligner_run": (c_int, [c_void_p, c_float, c_uint32, c_uint64, c_uint64, _f32p]),
+    "ep_aligner_destroy": (None, [c_void_p]),
     "ep_sample_weights": (c_int, [c_void_p, c_uint32, _f32p]),
     "ep_set_explicit_weights": (c_int, [c_void_p, c_uint32, _f32p]),
     "ep_set_features": (c_int, [c_void_p, c_uint64, c_uint32, _u64p, _u32p]),
@@ -177,6 +181,50 @@ def default_config() -> EPConfig:
     cfg = EPConfig()
     check(load().ep_config_default(ctypes.byref(cfg)), "ep_config_default")
     return cfg
+
+
+class Aligner:
+    """The graph and node embeddings resident on the device for NeighborhoodAligner runs."""
+
+    def __init__(self, indptr, indices, cdf, emb, device: int = 0):
+        indptr = np.ascontiguousarray(indptr, dtype=np.uint64)
+        indices = np.ascontiguousarray(indices, dtype=np.uint32)
+        cdf = np.ascontiguousarray(cdf, dtype=np.float32)
+        emb = np.ascontiguousarray(emb, dtype=np.float32)
+        self.n, self.dims = emb.shape
+        if len(indptr) != self.n + 1:
+            raise ValueError("embeddings must have one row per graph node")
+        h = c_void_p()
+        check(load().ep_aligner_create(int(device), self.n, ptr(indptr, c_uint64), ptr(indices, c_uint32),
+                                       ptr(cdf, c_float), self.dims, ptr(emb, c_float), ctypes.byref(h)),
+              "ep_aligner_create")
+        self.handle = h
+
+    def run(self, alpha: Optional[float], max_neighbors: Optional[int], first: int = 0,
+            count: Optional[int] = None) -> np.ndarray:
+        count = self.n - first if count is None else count
+        out = np.empty((count, self.dims), dtype=np.float32)
+        a = float("nan") if alpha is None else float(alpha)
+        check(load().ep_aligner_run(self.handle, a, int(max_neighbors or 0), int(first), int(count),
+                                    ptr(out, c_float)), "ep_aligner_run")
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().ep_aligner_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Context:

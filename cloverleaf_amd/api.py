@@ -8,6 +8,7 @@ Class and argument names, defaults and side effects follow the reference:
 * ``FeatureSet``                                 src/lib.rs:1432-1691, src/feature_store.rs
 * ``NodeEmbeddings``, ``Distance``               src/lib.rs:2535-2894
 * ``FeatureAggregator``, ``NodeEmbedder``        src/lib.rs:1766-2207 (Averaged, Weighted, Attention)
+* ``NeighborhoodAligner``                        src/lib.rs:3040-3171, src/algos/alignment.rs
 
 The training step itself runs in libcloverleaf_ep.so (HIP kernels on the GPU);
 this module only converts names to integer CSR arrays and back.
@@ -804,3 +805,53 @@ class NodeEmbedder:
             else:
                 out = ctx.embed_averaged(t_ip, t_ids)
         return NodeEmbeddings(feat_set.vocab_, out, Distance.Cosine)
+
+
+# --------------------------------------------------------------------------------------
+# NeighborhoodAligner (src/lib.rs:3040-3171, src/algos/alignment.rs)
+# --------------------------------------------------------------------------------------
+class NeighborhoodAligner:
+    """Shifts node embeddings toward the weighted centroid of their neighbourhood
+    (alignment.rs:25-75), on the GPU:
+
+        new_v = (1 - a) * sum_j p_j E[u_j] + a * E[v]
+
+    over the first ``max_neighbors`` edges of v (all by default; the edge probabilities
+    p_j are not renormalised), with a = ``alpha``, or min(1 / ln(deg + 1), 1) when alpha
+    is None.  Row i of the embeddings belongs to graph node i.  A node without edges
+    keeps a * E[v] (the reference indexes a missing first weight there).
+    """
+
+    def __init__(self, alpha: Optional[float] = None, max_neighbors: Optional[int] = None, *, device: int = 0):
+        self.alpha = None if alpha is None else float(np.float32(alpha))
+        self.max_neighbors = None if max_neighbors is None else int(max_neighbors)
+        self.device = device
+
+    def __repr__(self):
+        a = "None" if self.alpha is None else f"Some({_f32_debug(self.alpha)})"
+        m = "None" if self.max_neighbors is None else f"Some({self.max_neighbors})"
+        return f"NeighborhoodAligner<Alpha={a}, MaxNeighbors={m}>"
+
+    def _aligner(self, embeddings: NodeEmbeddings, graph: Graph) -> N.Aligner:
+        if embeddings.len() != graph.nodes():
+            raise ValueError("embeddings must have one row per graph node (the graph's vocabulary)")
+        return N.Aligner(graph.indptr, graph.indices, graph.edge_cdf(), embeddings.embeddings, self.device)
+
+    def align(self, embeddings: NodeEmbeddings, graph: Graph) -> NodeEmbeddings:
+        """NeighborhoodAligner.align (lib.rs:3095-3113): a new NodeEmbeddings."""
+        with self._aligner(embeddings, graph) as al:
+            out = al.run(self.alpha, self.max_neighbors)
+        return NodeEmbeddings(embeddings.vocab_, out, embeddings.distance)
+
+    def align_to_disk(self, path: str, embeddings: NodeEmbeddings, graph: Graph, chunk_size: Optional[int] = None,
+                      comp_level: Optional[int] = None):
+        """NeighborhoodAligner.align_to_disk (lib.rs:3140-3168): aligns chunk_size nodes at a
+        time (default 10000) and streams them out in the NodeEmbeddings text format."""
+        cs = 10_000 if chunk_size is None else max(1, int(chunk_size))
+        with self._aligner(embeddings, graph) as al, _open_text_write(path, comp_level) as fh:
+            for first in range(0, graph.nodes(), cs):
+                count = min(cs, graph.nodes() - first)
+                block = al.run(self.alpha, self.max_neighbors, first, count)
+                for k in range(count):
+                    t, n = embeddings.vocab_.name(first + k)
+                    fh.write(f"{t}\t{n}\t[{','.join(str(x) for x in block[k])}]\n")

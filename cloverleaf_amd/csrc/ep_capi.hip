@@ -1302,6 +1302,89 @@ int ep_embed_attention(ep_ctx* c, uint64_t n_nodes, const uint64_t* f_indptr, co
   return EP_OK;
 }
 
+// NeighborhoodAligner (alignment.rs, lib.rs:3040-3171): the graph (with its edge CDF) and
+// the node embeddings stay on the device; runs align node ranges.
+struct ep_aligner {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t n = 0;
+  uint32_t dims = 0;
+  uint64_t* indptr = nullptr;
+  uint32_t* indices = nullptr;
+  float* cdf = nullptr;
+  float* emb = nullptr;
+  float* out = nullptr;
+  uint64_t out_rows = 0;
+};
+
+static void aligner_free(ep_aligner* a) {
+  if (!a) return;
+  if (a->stream) (void)hipSetDevice(a->device);
+  void* ps[] = {a->indptr, a->indices, a->cdf, a->emb, a->out};
+  for (void* q : ps)
+    if (q) (void)hipFree(q);
+  if (a->stream) (void)hipStreamDestroy(a->stream);
+  delete a;
+}
+
+int ep_aligner_create(int device, uint64_t n_nodes, const uint64_t* indptr, const uint32_t* indices, const float* cdf,
+                      uint32_t dims, const float* emb, ep_aligner** out) {
+  if (!out || !indptr || !emb) return fail(EP_EINVAL, "NULL argument");
+  *out = nullptr;
+  if (n_nodes == 0) return fail(EP_EINVAL, "n_nodes must be > 0");
+  if (!ep::row_dims_supported((int)dims)) return fail(EP_EINVAL, "embedding width not built for alignment");
+  if (indptr[0] != 0) return fail(EP_EINVAL, "indptr[0] must be 0");
+  for (uint64_t i = 0; i < n_nodes; ++i)
+    if (indptr[i + 1] < indptr[i]) return fail(EP_EINVAL, "indptr must be non-decreasing");
+  const uint64_t ne = indptr[n_nodes];
+  if (ne && (!indices || !cdf)) return fail(EP_EINVAL, "NULL argument");
+  for (uint64_t i = 0; i < ne; ++i) {
+    if (indices[i] >= n_nodes) return fail(EP_EINVAL, "edge target out of range");
+    if (!std::isfinite(cdf[i])) return fail(EP_EINVAL, "edge CDF values must be finite");
+  }
+  ep_aligner* a = new ep_aligner();
+  a->device = device;
+  a->n = n_nodes;
+  a->dims = dims;
+  auto bail = [&](int code, const std::string& msg) { aligner_free(a); return fail(code, msg); };
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc((void**)&a->indptr, (n_nodes + 1) * sizeof(uint64_t));
+  if (e == hipSuccess) e = hipMalloc((void**)&a->indices, std::max<uint64_t>(ne, 1) * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc((void**)&a->cdf, std::max<uint64_t>(ne, 1) * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc((void**)&a->emb, (size_t)n_nodes * dims * sizeof(float));
+  if (e == hipSuccess) e = hipMemcpy(a->indptr, indptr, (n_nodes + 1) * sizeof(uint64_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess && ne) e = hipMemcpy(a->indices, indices, ne * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess && ne) e = hipMemcpy(a->cdf, cdf, ne * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(a->emb, emb, (size_t)n_nodes * dims * sizeof(float), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return bail(EP_EHIP, std::string("aligner setup: ") + hipGetErrorString(e));
+  *out = a;
+  return EP_OK;
+}
+
+int ep_aligner_run(ep_aligner* a, float alpha, uint32_t max_neighbors, uint64_t first, uint64_t count, float* out) {
+  if (!a || !out) return fail(EP_EINVAL, "NULL argument");
+  if (first > a->n || count > a->n - first) return fail(EP_EINVAL, "node range out of bounds");
+  const int adaptive = std::isnan(alpha) ? 1 : 0;
+  if (!adaptive && !std::isfinite(alpha)) return fail(EP_EINVAL, "alpha must be finite (NaN: adaptive)");
+  if (count == 0) return EP_OK;
+  EP_HIP(hipSetDevice(a->device));
+  if (a->out_rows < count) {
+    if (a->out) EP_HIP(hipFree(a->out));
+    a->out = nullptr;
+    a->out_rows = 0;
+    EP_HIP(hipMalloc((void**)&a->out, (size_t)count * a->dims * sizeof(float)));
+    a->out_rows = count;
+  }
+  EP_HIP(ep::launch_align((int)a->dims, a->emb, a->indptr, a->indices, a->cdf, adaptive ? 0.f : alpha, adaptive,
+                          max_neighbors, first, count, a->out, a->stream));
+  EP_HIP(hipMemcpyAsync(out, a->out, (size_t)count * a->dims * sizeof(float), hipMemcpyDeviceToHost, a->stream));
+  EP_HIP(hipStreamSynchronize(a->stream));
+  return EP_OK;
+}
+
+void ep_aligner_destroy(ep_aligner* a) { aligner_free(a); }
+
 int ep_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
   if (!ctr || !key || !out) return fail(EP_EINVAL, "NULL argument");
   const ep::u32x4 r = ep::philox4x32(ep::u32x4{ctr[0], ctr[1], ctr[2], ctr[3]}, key[0], key[1]);

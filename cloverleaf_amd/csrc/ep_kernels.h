@@ -221,6 +221,9 @@ bool dims_supported(int d);
 // Row widths of the table-row kernels (init, update, averaged embedding): the mean
 // model's widths and the attention row widths.
 bool row_dims_supported(int d);
+hipError_t launch_align(int dims, const float* emb, const uint64_t* indptr, const uint32_t* indices, const float* cdf,
+                        float alpha, int adaptive, uint32_t max_nb, uint64_t first, uint64_t count, float* out,
+                        hipStream_t s);
 // Attention model kernels (ep_attention.hip): forward runs the block outputs and the
 // loss/adjoints; backward writes the per-occurrence gradient rows (train steps).
 bool att_shape_supported(int heads, int dk, int dm);

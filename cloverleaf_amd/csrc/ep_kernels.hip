@@ -2257,6 +2257,64 @@ __global__ __launch_bounds__(256) void k_embed_weighted(const T* __restrict__ ta
 }
 
 // ------------------------------------------------------------------------------------
+// k_align: NeighborhoodAligner::align (alignment.rs:25-75), one wave per node v of
+// [first, first + count): out = (1 - a) * sum_j p_j E[u_j] + a * E[v] over the first
+// min(deg, max_nb) edges (max_nb 0: all; the probabilities are not renormalised), p_j the
+// edge probability (CDFtoP of the row's CDF), a = alpha, or min(1 / ln(deg + 1), 1) with
+// the full degree when adaptive.  Each lane adds its columns in edge order; U rows in
+// flight.  A node without edges gets a * E[v] (the reference indexes weights[0] there).
+// ------------------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(256) void k_align(const float* __restrict__ emb, const uint64_t* __restrict__ indptr,
+                                               const uint32_t* __restrict__ indices, const float* __restrict__ cdf,
+                                               float alpha, int adaptive, uint32_t max_nb, uint64_t first,
+                                               uint64_t count, float* __restrict__ out) {
+  constexpr int PL = RowShape<D>::PL;
+  constexpr int U = 8;
+  const int lane = threadIdx.x & 63;
+  const uint64_t gw = (uint64_t)blockIdx.x * NW + (threadIdx.x >> 6);
+  const uint64_t nwaves = (uint64_t)gridDim.x * NW;
+  for (uint64_t i = gw; i < count; i += nwaves) {
+    const uint64_t v = first + i;
+    const uint64_t s0 = indptr[v];
+    const uint32_t deg = (uint32_t)(indptr[v + 1] - s0);
+    const uint32_t m = max_nb ? min(deg, max_nb) : deg;
+    float acc[PL];
+#pragma unroll
+    for (int q = 0; q < PL; ++q) acc[q] = 0.f;
+    for (uint32_t base = 0; base < m; base += 64u) {
+      const int n = (int)min(64u, m - base);
+      const uint32_t e = base + (uint32_t)lane;
+      const uint32_t myu = lane < n ? indices[s0 + e] : 0u;
+      const float myp = lane < n ? (e == 0u ? cdf[s0] : cdf[s0 + e] - cdf[s0 + e - 1u]) : 0.f;
+      for (int j = 0; j < n; j += U) {
+        float r[U][PL], w[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int t = (j + u < n) ? (j + u) : (n - 1);
+          const uint32_t uu = (uint32_t)__shfl((int)myu, t, EP_WAVE);
+          w[u] = __shfl(myp, t, EP_WAVE);
+          load_slice<D>(emb + (size_t)uu * D, lane, r[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (j + u < n) {
+#pragma unroll
+            for (int q = 0; q < PL; ++q) acc[q] += w[u] * r[u][q];
+          }
+        }
+      }
+    }
+    const float a = adaptive ? fminf(1.f / logf((float)deg + 1.f), 1.f) : alpha;
+    float o[PL];
+    load_slice<D>(emb + (size_t)v * D, lane, o);
+#pragma unroll
+    for (int q = 0; q < PL; ++q) acc[q] = acc[q] * (1.f - a) + o[q] * a;
+    store_slice<D>(out + (size_t)i * D, lane, acc);
+  }
+}
+
+// ------------------------------------------------------------------------------------
 // Dispatch
 // ------------------------------------------------------------------------------------
 #define EP_DIMS(X) X(8) X(16) X(32) X(64) X(100) X(128) X(256)
@@ -2470,6 +2528,18 @@ hipError_t launch_embed_weighted(const StepConfig& c, const uint64_t* findptr, c
     else
       hipLaunchKernelGGL((k_embed_weighted<D, float>), dim3(2048), dim3(256), 0, s, (const float*)c.table, findptr,
                          fids, p_w, alpha, n_nodes, out);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_align(int dims, const float* emb, const uint64_t* indptr, const uint32_t* indices, const float* cdf,
+                        float alpha, int adaptive, uint32_t max_nb, uint64_t first, uint64_t count, float* out,
+                        hipStream_t s) {
+  const uint64_t want = (count + NW - 1) / NW + 1;
+  const unsigned grid = (unsigned)(want < 4096u ? want : 4096u);
+  EP_ROW_DISPATCH(dims, {
+    hipLaunchKernelGGL((k_align<D>), dim3(grid), dim3(256), 0, s, emb, indptr, indices, cdf, alpha, adaptive, max_nb,
+                       first, count, out);
   });
   return hipGetLastError();
 }

@@ -249,6 +249,19 @@ int ep_embed_weighted(ep_ctx* ctx, uint64_t n_nodes, const uint64_t* f_indptr, c
  * per node; nodes without features get a zero row.                             */
 int ep_embed_attention(ep_ctx* ctx, uint64_t n_nodes, const uint64_t* f_indptr, const uint32_t* f_ids, float* out);
 
+/* NeighborhoodAligner (src/algos/alignment.rs:25-75; lib.rs:3040-3171 align and
+ * align_to_disk).  Create uploads the graph (CumCSR offsets, columns and per-row
+ * edge CDF) and the node embeddings emb[n_nodes*dims] (row i is graph node i);
+ * run writes out[count*dims] for nodes [first, first+count):
+ *   out_v = (1 - a) * sum_j p_j emb[u_j] + a * emb[v]
+ * over the first min(deg, max_neighbors) edges (0 = all), p_j = CDFtoP, a = alpha,
+ * or min(1 / ln(deg + 1), 1) when alpha is NaN (the reference's None).            */
+typedef struct ep_aligner ep_aligner;
+int ep_aligner_create(int device, uint64_t n_nodes, const uint64_t* indptr, const uint32_t* indices, const float* cdf,
+                      uint32_t dims, const float* emb, ep_aligner** out);
+int ep_aligner_run(ep_aligner* a, float alpha, uint32_t max_neighbors, uint64_t first, uint64_t count, float* out);
+void ep_aligner_destroy(ep_aligner* a);
+
 /* Philox4x32-10 on the host (the device RNG), for known-answer tests.           */
 int ep_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 

@@ -836,3 +836,37 @@ def roc_auc(pos_scores: np.ndarray, neg_scores: np.ndarray) -> float:
         i = j + 1
     rp = ranks[:len(pos)].sum()
     return float((rp - len(pos) * (len(pos) + 1) / 2.0) / (len(pos) * len(neg)))
+
+
+# --------------------------------------------------------------------------------------
+# NeighborhoodAligner (src/algos/alignment.rs:25-75)
+# --------------------------------------------------------------------------------------
+def neighborhood_align(indptr, indices, cdf, emb, alpha: Optional[float], max_neighbors: Optional[int]):
+    """NeighborhoodAligner::align for every node, FP64 (alignment.rs:25-75).
+
+    new_v = (1 - a) * sum_{j < m} p_j E[u_j] + a * E[v], m = min(deg, max_neighbors)
+    (the first m edges; p_j = CDFtoP of the row, not renormalised after the cut),
+    a = alpha, or min(1 / ln(deg + 1), 1) with the full degree when alpha is None
+    (alignment.rs:62-71).  The reference indexes weights[0] of an edgeless node; here
+    such a node gets a * E[v].
+    """
+    from scipy import sparse
+    indptr = np.asarray(indptr, dtype=np.int64)
+    n = len(indptr) - 1
+    emb = np.asarray(emb, dtype=np.float64)
+    deg = np.diff(indptr)
+    rows, cols, vals = [], [], []
+    for v in range(n):
+        s, e = int(indptr[v]), int(indptr[v + 1])
+        m = e - s if max_neighbors is None else min(e - s, int(max_neighbors))
+        p = cdf_to_p(np.asarray(cdf[s:e], dtype=np.float32).astype(np.float64))[:m]
+        rows.extend([v] * m)
+        cols.extend(int(u) for u in indices[s:s + m])
+        vals.extend(p.tolist())
+    A = sparse.csr_matrix((vals, (rows, cols)), shape=(n, n))
+    if alpha is None:
+        with np.errstate(divide="ignore"):
+            a = np.minimum(1.0 / np.log(deg.astype(np.float64) + 1.0), 1.0)
+    else:
+        a = np.full(n, float(alpha))
+    return (A @ emb) * (1.0 - a)[:, None] + emb * a[:, None]

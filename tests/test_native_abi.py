@@ -106,3 +106,15 @@ def test_null_arguments_rejected(native):
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(RuntimeError, match="no CPU fallback"):
         N.load(tmp_path / "nope.so")
+
+
+def test_aligner_rejects_bad_input():
+    ip = np.array([0, 1, 2], dtype=np.uint64)
+    ix = np.array([1, 0], dtype=np.uint32)
+    cdf = np.ones(2, dtype=np.float32)
+    with pytest.raises(N.NativeError, match="width"):
+        N.Aligner(ip, ix, cdf, np.zeros((2, 7), np.float32))
+    with pytest.raises(N.NativeError, match="out of range"):
+        N.Aligner(ip, np.array([1, 5], dtype=np.uint32), cdf, np.zeros((2, 8), np.float32))
+    with pytest.raises(ValueError):
+        N.Aligner(ip, ix, cdf, np.zeros((3, 8), np.float32))

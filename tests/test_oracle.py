@@ -833,3 +833,62 @@ def test_loss_closed_forms():
     assert abs(L - math.log(math.exp(3) + math.exp(-1) + 1)) < 1e-12 and all(r)
     L, r, *_ = O.loss_adjoints("rank", a, p, [n1, n2], (0.01,))  # s_pos ~ 0.045 >= tau: no loss
     assert L == 0 and not any(r)
+
+
+# --------------------------------------------------------------------------------------
+# NeighborhoodAligner (alignment.rs:25-75)
+# --------------------------------------------------------------------------------------
+def _align_toy():
+    # 0 -> 1 (0.25), 0 -> 2 (0.75); 1 -> 0 (1.0); 2 has no edges
+    indptr = np.array([0, 2, 3, 3])
+    indices = np.array([1, 2, 0])
+    cdf = np.array([0.25, 1.0, 1.0], dtype=np.float32)
+    emb = np.array([[1.0, 0.0], [0.0, 2.0], [4.0, 4.0]])
+    return indptr, indices, cdf, emb
+
+
+def test_align_closed_form():
+    indptr, indices, cdf, emb = _align_toy()
+    out = O.neighborhood_align(indptr, indices, cdf, emb, 0.5, None)
+    assert np.allclose(out[0], 0.5 * (0.25 * emb[1] + 0.75 * emb[2]) + 0.5 * emb[0])   # [2.0, 1.75]
+    assert np.allclose(out[0], [2.0, 1.75])
+    assert np.allclose(out[1], 0.5 * emb[0] + 0.5 * emb[1])
+    assert np.allclose(out[2], 0.5 * emb[2])  # no edges
+    # alpha 1: unchanged; alpha 0: the weighted neighbourhood sum
+    assert np.allclose(O.neighborhood_align(indptr, indices, cdf, emb, 1.0, None), emb)
+    assert np.allclose(O.neighborhood_align(indptr, indices, cdf, emb, 0.0, None)[0], [3.0, 3.5])
+
+
+def test_align_max_neighbors_does_not_renormalise():
+    indptr, indices, cdf, emb = _align_toy()
+    out = O.neighborhood_align(indptr, indices, cdf, emb, 0.0, 1)
+    assert np.allclose(out[0], 0.25 * emb[1])  # only the first edge, weight 0.25
+
+
+def test_align_adaptive_alpha():
+    indptr, indices, cdf, emb = _align_toy()
+    out = O.neighborhood_align(indptr, indices, cdf, emb, None, None)
+    a0 = 1.0 / math.log(3.0)  # degree 2: 0.910
+    assert np.allclose(out[0], (1 - a0) * np.array([3.0, 3.5]) + a0 * emb[0])
+    # degree 1: 1 / ln 2 > 1 -> 1; degree 0: 1 / ln 1 = inf -> 1
+    assert np.allclose(out[1], emb[1]) and np.allclose(out[2], emb[2])
+    # the cut keeps the full degree for alpha
+    cut = O.neighborhood_align(indptr, indices, cdf, emb, None, 1)
+    assert np.allclose(cut[0], (1 - a0) * 0.25 * emb[1] + a0 * emb[0])
+
+
+def test_align_uniform_weights_give_neighbour_mean():
+    rng = np.random.default_rng(4)
+    n = 30
+    rows, cols, _ = O.csr_from_edges([(int(a), int(b), 1.0) for a, b in rng.integers(0, n, (120, 2)) if a != b],
+                                     deduplicate=True)
+    indptr = np.zeros(n + 1, dtype=np.int64)
+    indptr[:len(rows)] = rows
+    indptr[len(rows):] = rows[-1]
+    cdf = np.concatenate([np.arange(1, d + 1, dtype=np.float32) / np.float32(d) for d in np.diff(indptr)])
+    emb = rng.normal(size=(n, 5))
+    out = O.neighborhood_align(indptr, cols, cdf, emb, 0.0, None)
+    for v in range(n):
+        s, e = indptr[v], indptr[v + 1]
+        want = emb[cols[s:e]].mean(axis=0) if e > s else np.zeros(5)
+        assert np.allclose(out[v], want, atol=1e-6)
