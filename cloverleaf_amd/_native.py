@@ -17,7 +17,7 @@ import numpy as np
 PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "libcloverleaf_ep.so"
 
-EP_SAMPLE_ALL, EP_SAMPLE_FIXED = 0, 1
+EP_SAMPLE_ALL, EP_SAMPLE_FIXED, EP_SAMPLE_PROBABILITY = 0, 1, 2
 EP_LW_NONE, EP_LW_LOG, EP_LW_EXPONENTIAL = 0, 1, 2
 EP_PREC_FP32, EP_PREC_BF16, EP_PREC_BF16_MASTER = 0, 1, 2
 EP_GRAD_FIRST_LEAF, EP_GRAD_EXACT = 0, 1
@@ -62,6 +62,7 @@ class EPConfig(Structure):
         ("context_window", c_uint32),
         ("loss_kind", c_int32),
         ("loss_param2", c_float),
+        ("max_features_p", c_float),
     ]
 
 

@@ -546,19 +546,20 @@ class EPLoss:
         return f"EPLoss<{self.kind}{self.params}, negatives={self.negatives}>"
 
 
-def _parse_sample(x: Optional[float], what: str) -> Tuple[int, int]:
-    """Sample::new (src/algos/utils.rs:173-184): (0,1) -> Probability, else Fixed(int)."""
+def _parse_sample(x: Optional[float], what: str) -> Tuple[int, int, float]:
+    """Sample::new (src/algos/utils.rs:173-184): (0,1) -> Probability(p), else Fixed(int).
+    Returns (kind, k, p)."""
     if x is None:
-        return N.EP_SAMPLE_ALL, 0
+        return N.EP_SAMPLE_ALL, 0, 0.0
     x = float(x)
     if x < 0:
         raise ValueError(f"{what} must be either None, or (0, N)")
     if 0 < x < 1:
-        raise ValueError(f"{what}: probability sampling is not supported by this build")
+        return N.EP_SAMPLE_PROBABILITY, 0, float(np.float32(x))
     k = int(x)
     if k == 0:
         raise ValueError(f"{what}: Fixed(0) samples nothing")
-    return N.EP_SAMPLE_FIXED, k
+    return N.EP_SAMPLE_FIXED, k, 0.0
 
 
 # --------------------------------------------------------------------------------------
@@ -585,12 +586,14 @@ class EmbeddingPropagator:
                  attention_heads=None, context_window=None, noise=None, *, precision="fp32",
                  grad_mode="first_leaf", scatter="sort", use_graph=False, concurrent_batches=1, device=0):
         loss = loss or EPLoss.margin(1.0, 1)
-        mf_kind, mf = _parse_sample(max_features, "max_features")
-        mn_kind, mn = _parse_sample(max_nodes, "max_nodes")
+        mf_kind, mf, mf_p = _parse_sample(max_features, "max_features")
+        mn_kind, mn, _ = _parse_sample(max_nodes, "max_nodes")
+        if mn_kind == N.EP_SAMPLE_PROBABILITY:
+            raise ValueError("max_nodes: probability sampling is not supported by this build")
         if attention is not None and context_window is not None:
             raise ValueError("context_window (Sliding attention) is not supported by this build")
-        if mf_kind != N.EP_SAMPLE_ALL:
-            raise ValueError("max_features sampling is not supported by this build")
+        if mf_kind != N.EP_SAMPLE_ALL and attention is not None:
+            raise ValueError("max_features sampling with attention is not supported by this build")
         if (weighted_neighbor_sampling or weighted_neighbor_averaging) and attention is not None:
             raise ValueError("weighted neighbour sampling/averaging with attention is not supported by this build")
         cfg = N.default_config()
@@ -608,7 +611,7 @@ class EmbeddingPropagator:
         cfg.valid_pct = 0.1 if valid_pct is None else float(valid_pct)
         cfg.seed = SEED if seed is None else int(seed)
         cfg.max_nodes_kind, cfg.max_nodes = mn_kind, mn
-        cfg.max_features_kind, cfg.max_features = mf_kind, mf
+        cfg.max_features_kind, cfg.max_features, cfg.max_features_p = mf_kind, mf, mf_p
         if loss_weighting is not None:
             cfg.loss_weighting = loss_weighting.kind
             cfg.loss_weight_exp = loss_weighting.weight

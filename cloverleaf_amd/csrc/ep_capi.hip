@@ -171,6 +171,12 @@ uint32_t neighbour_cap(const ep_ctx* c) {
 // weighted_neighbor_averaging applies to ~h(v) from sampled neighbours (not PPR walks).
 bool weighted_avg(const ep_config& cfg) { return cfg.weighted_neighbor_averaging && !ppr_loss(cfg); }
 
+bool feature_sampling(const ep_config& cfg) { return cfg.max_features_kind != EP_SAMPLE_ALL; }
+
+// ~h(v) with a count per neighbour (one adjoint row each): weighted averaging or
+// max_features sampling (the feature scale differs between neighbours).
+bool neighbour_rows(const ep_config& cfg) { return weighted_avg(cfg) || feature_sampling(cfg); }
+
 // Anchors per training step: concurrent_batches reference batches.
 uint32_t group_anchors(const ep_ctx* c) { return c->cfg.batch_size * std::max<uint32_t>(1u, c->cfg.concurrent_batches); }
 
@@ -183,6 +189,8 @@ uint64_t example_rows_max(const ep_ctx* c) {
 int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   if (weighted_avg(c->cfg) && !c->d_cdf)
     return fail(EP_ESTATE, "weighted_neighbor_averaging needs the edge weights (ep_set_edge_weights)");
+  if (feature_sampling(c->cfg) && c->f_max > 64)
+    return fail(EP_EINVAL, "max_features sampling supports at most 64 features per node");
   if (batch <= c->cap_batch && c->base.nbr) return EP_OK;
   if (!c->d_indptr || !c->d_findptr) return fail(EP_ESTATE, "graph and features must be set first");
   const ep_config& cfg = c->cfg;
@@ -205,8 +213,9 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   }
   // weighted neighbours: one adjoint row per neighbour after the construction rows
   const bool wavg = weighted_avg(cfg);
-  const uint64_t nrow_cap = wavg ? nbr_max : 0;
-  if (wavg && (uint64_t)batch * NC + nrow_cap > ep::kSlotMask)
+  const bool nrows = neighbour_rows(cfg) && !att;
+  const uint64_t nrow_cap = nrows ? nbr_max : 0;
+  if (nrows && (uint64_t)batch * NC + nrow_cap > ep::kSlotMask)
     return fail(EP_ECAPACITY, "batch too large: neighbour adjoint rows exceed 2^29");
   if (keycap >= (1ull << 31)) return fail(EP_ECAPACITY, "batch too large: key capacity exceeds 2^31");
   if ((uint64_t)batch * NC > ep::kSlotMask) return fail(EP_ECAPACITY, "batch too large: slots exceed 2^29");
@@ -253,7 +262,7 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   // per feature occurrence
   EP_TRY(c->alloc(&b.adj, att ? (size_t)keycap * D : ((size_t)batch * NC + nrow_cap) * D));
   if (wavg && S) EP_TRY(c->alloc(&b.nbr_w, (size_t)batch * S));
-  if (wavg && !S) EP_TRY(c->alloc(&b.nbroff, batch));
+  if (nrows && !S) EP_TRY(c->alloc(&b.nbroff, batch));
   b.nrow_base = batch * NC;
   if (att) {
     const uint32_t dm = cfg.dims;
@@ -311,6 +320,7 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   b.piece_cap = piece_cap;
   b.hash_entries = (uint32_t)hash;
   b.ent_cap = (uint32_t)std::min<uint64_t>(256, 1ull + neighbour_cap(c) + K);
+  b.sel_cap = feature_sampling(cfg) ? b.ent_cap : 0u;
   b.ids_cap = (uint32_t)std::min<uint64_t>(8192, example_rows_max(c));
   // fuse emission into the forward while the first-leaf hash is small (<= 16 KiB of LDS)
   b.fuse_emit = (!att && (cfg.grad_mode == EP_GRAD_EXACT || hash <= 4096)) ? 1 : 0;
@@ -382,6 +392,10 @@ void refresh_base(ep_ctx* c) {
   b.wavg = weighted_avg(cfg) ? 1 : 0;
   b.wsample = cfg.weighted_neighbor_sampling ? 1 : 0;
   b.cdf = c->d_cdf;
+  b.mf_kind = cfg.max_features_kind;
+  b.mf_k = cfg.max_features;
+  b.mf_p = cfg.max_features_p;
+  b.nbr_rows = (neighbour_rows(cfg) && !cfg.attention_dk) ? 1 : 0;
   b.hard_negs = std::min(cfg.hard_negatives, cfg.negatives);
   b.lw_kind = cfg.loss_weighting;
   b.lw_exp = cfg.loss_weight_exp;
@@ -614,7 +628,15 @@ int validate_config(const ep_config* cfg) {
     return fail(EP_EINVAL, "max_nodes (Fixed) must be in [1, 1024]");
   if (cfg->max_nodes_kind != EP_SAMPLE_FIXED && cfg->max_nodes_kind != EP_SAMPLE_ALL)
     return fail(EP_EINVAL, "max_nodes must be All or Fixed");
-  if (cfg->max_features_kind != EP_SAMPLE_ALL) return fail(EP_EINVAL, "max_features sampling is not supported");
+  if (cfg->max_features_kind != EP_SAMPLE_ALL && cfg->max_features_kind != EP_SAMPLE_FIXED &&
+      cfg->max_features_kind != EP_SAMPLE_PROBABILITY)
+    return fail(EP_EINVAL, "max_features must be All, Fixed or Probability");
+  if (cfg->max_features_kind == EP_SAMPLE_FIXED && cfg->max_features == 0)
+    return fail(EP_EINVAL, "max_features (Fixed) must be >= 1");
+  if (cfg->max_features_kind == EP_SAMPLE_PROBABILITY && !(cfg->max_features_p > 0.f && cfg->max_features_p < 1.f))
+    return fail(EP_EINVAL, "max_features (Probability) must be in (0, 1)");
+  if (cfg->max_features_kind != EP_SAMPLE_ALL && cfg->attention_dk)
+    return fail(EP_EINVAL, "max_features sampling is not supported with attention");
   if (cfg->loss_kind < EP_LOSS_MARGIN || cfg->loss_kind > EP_LOSS_PPR) return fail(EP_EINVAL, "loss_kind");
   if (!std::isfinite(cfg->gamma) || !std::isfinite(cfg->loss_param2))
     return fail(EP_EINVAL, "loss parameters must be finite");

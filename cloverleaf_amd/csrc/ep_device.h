@@ -51,6 +51,7 @@ enum RngPurpose : uint32_t {
   RNG_INIT = 5,
   RNG_WALK = 6,
   RNG_HARD_WALK = 7,
+  RNG_FEATURES = 8,  // max_features: purpose RNG_FEATURES | (entry << 4), one stream per node of an example
 };
 
 // Uniform integer in [0, n) from a 32-bit draw (multiply-shift; bias <= n / 2^32).

@@ -98,6 +98,14 @@ struct StepConfig {
   int wavg;                // weighted_neighbor_averaging
   int wsample;             // weighted_neighbor_sampling
   uint32_t nrow_base;      // first per-neighbour adjoint row
+  // max_features (utils.rs:193-216, model.rs:264-269): 0 All, 1 Fixed(mf_k), 2
+  // Probability(mf_p); every node of an example keeps a subset of its (at most 64)
+  // features, each kept occurrence counting weight + scale
+  int mf_kind;
+  uint32_t mf_k;
+  float mf_p;
+  uint32_t sel_cap;        // forward LDS stage: selection masks (ent_cap with max_features, else 0)
+  int nbr_rows;            // ~h(v) with per-neighbour counts and adjoint rows (wavg or max_features)
   uint32_t K;              // negatives
   float gamma;
   int loss_kind;           // EP_LOSS_*: 0 margin, 1 contrastive, 2 starspace, 3 rank,

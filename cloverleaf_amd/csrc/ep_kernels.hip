@@ -391,6 +391,56 @@ __device__ inline void ares_select(const StepConfig& c, uint64_t s0, uint32_t de
   }
 }
 
+// max_features selection of one node of an example (utils.rs:193-216 with at_least_one,
+// model.rs:264-269): bit i of mask = position i of the node's feature list is kept, and
+// scale is added to the weight of every kept occurrence.  Fixed(k), n > k: Floyd's
+// k-subset; Probability(p): every position independently with probability p, one uniform
+// position when none is kept (scale (1 - p^n) / p, as the reference writes it).  Stream:
+// purpose RNG_FEATURES | (e << 4), e = 0 (v), 1..cnt (neighbours), then the negatives.
+struct FeatSel {
+  uint64_t mask;
+  float scale;
+};
+__device__ inline FeatSel feature_select(const StepConfig& c, uint32_t n, uint32_t pos, uint32_t pass, uint32_t e,
+                                         uint32_t k0, uint32_t k1) {
+  const uint64_t all = n >= 64u ? ~0ull : ((1ull << n) - 1ull);
+  const uint32_t purpose = (uint32_t)RNG_FEATURES | (e << 4);
+  if (c.mf_kind == 1) {
+    if (n <= c.mf_k) return FeatSel{all, 1.f};
+    uint64_t m = 0;
+    for (uint32_t i = n - c.mf_k; i < n; ++i) {
+      const uint32_t t = uniform_below(draw_word(pos, pass, purpose, i, k0, k1), i + 1u);
+      m |= ((m >> t) & 1ull) ? (1ull << i) : (1ull << t);
+    }
+    return FeatSel{m, (float)n / (float)c.mf_k};
+  }
+  uint64_t m = 0;
+  for (uint32_t i0 = 0; i0 < n; i0 += 4u) {
+    const u32x4 r = philox4x32(u32x4{pos, pass, purpose, i0 >> 2}, k0, k1);
+    const uint32_t w4[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+    for (uint32_t q = 0; q < 4u; ++q)
+      if (i0 + q < n && uniform01(w4[q]) < c.mf_p) m |= 1ull << (i0 + q);
+  }
+  if (m) return FeatSel{m, 1.f / c.mf_p};
+  const uint32_t t = uniform_below(draw_word(pos, pass, purpose, 64u, k0, k1), n);
+  return FeatSel{1ull << t, (1.f - powf(c.mf_p, (float)n)) / c.mf_p};
+}
+
+// Number of kept features of a node (the popcount of its selection).
+__device__ inline uint32_t feature_count(const StepConfig& c, uint32_t n, uint32_t pos, uint32_t pass, uint32_t e,
+                                         uint32_t k0, uint32_t k1) {
+  if (c.mf_kind == 0) return n;
+  if (c.mf_kind == 1) return min(n, c.mf_k);
+  return (uint32_t)__popcll(feature_select(c, n, pos, pass, e, k0, k1).mask);
+}
+
+// Position of the q-th set bit (q < popcount).
+__device__ inline uint32_t nth_set_bit(uint64_t m, uint32_t q) {
+  for (uint32_t i = 0; i < q; ++i) m &= m - 1ull;
+  return (uint32_t)__ffsll((unsigned long long)m) - 1u;
+}
+
 __global__ __launch_bounds__(256) void k_sample(StepConfig c, StepState* st) {
   const int lane = threadIdx.x & 63;
   const uint32_t b = blockIdx.x * NW + (threadIdx.x >> 6);
@@ -483,7 +533,7 @@ __global__ __launch_bounds__(256) void k_sample(StepConfig c, StepState* st) {
     const uint32_t E = 1u + cnt + c.K;  // v, neighbours, negatives
     for (uint32_t e = lane; e < E; e += 64u) {
       const uint32_t u = e == 0u ? v : (e <= cnt ? pn[e - 1u] : negs[e - 1u - cnt]);
-      R += (uint32_t)(c.findptr[u + 1] - c.findptr[u]);
+      R += feature_count(c, (uint32_t)(c.findptr[u + 1] - c.findptr[u]), pos, pass, e, k0, k1);
     }
     R = wave_sum_u32(R);
   }
@@ -501,7 +551,7 @@ __global__ __launch_bounds__(1024) void k_scan(StepConfig c, StepState* st) {
   __shared__ unsigned long long s_nodes;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const bool att = c.model == 1;
-  const bool nrows = c.wavg && c.all_nbrs;  // per-neighbour adjoint rows at scanned offsets
+  const bool nrows = c.nbr_rows && c.all_nbrs;  // per-neighbour adjoint rows at scanned offsets
   if (tid == 0) { s_carry = 0u; s_carry2 = 0u; s_carry3 = 0u; s_nodes = 0ull; }
   __syncthreads();
   uint64_t nodes = 0;
@@ -581,6 +631,35 @@ __device__ inline void gather_lds(const T* __restrict__ table, const uint32_t* i
   }
 }
 
+// Accumulate the rows of the features kept by a max_features mask (bit i: fids[fs + i]),
+// U row loads in flight.
+template <int D, int U, typename T>
+__device__ inline void gather_masked(const T* __restrict__ table, const uint32_t* __restrict__ fids, uint64_t fs,
+                                     uint64_t mask, int lane, float (&acc)[RowShape<D>::PL]) {
+  constexpr int PL = RowShape<D>::PL;
+  while (mask) {
+    uint32_t f[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      ok[u] = mask != 0ull;
+      const uint32_t i = ok[u] ? (uint32_t)__ffsll((unsigned long long)mask) - 1u : 0u;
+      mask &= mask - 1ull;
+      f[u] = fids[fs + i];
+    }
+    float r[U][PL];
+#pragma unroll
+    for (int u = 0; u < U; ++u) load_slice<D>(table + (size_t)(ok[u] ? f[u] : f[0]) * D, lane, r[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (ok[u]) {
+#pragma unroll
+        for (int q = 0; q < PL; ++q) acc[q] += r[u][q];
+      }
+    }
+  }
+}
+
 // Wide gather: lane groups of LW = D / VW lanes each load one row with 16-byte loads
 // (VW = 4 FP32 or 8 BF16 values per lane), RPW = 64 / LW rows per wave instruction and U
 // rows in flight per group; the group partial sums are added across groups and lanes of
@@ -651,10 +730,15 @@ __device__ inline void gather_lds_wide(const T* __restrict__ table, const uint32
   }
 }
 
-// Weight w_j of neighbour j of example b (model.rs:357-361): the sampler's value for
-// Fixed(S), p_j * deg (scale 1) with every neighbour; and its own adjoint row.
+// Weight w_j of neighbour j of example b before the feature scale, and its own adjoint
+// row: 1 for PPR walk endpoints (loss.rs:171-177); with weighted averaging the sampler's
+// value (Fixed) or p_j * deg (every neighbour); else the max_nodes scale, deg / S when
+// subsampled and 1 otherwise (model.rs:355-361).
 __device__ inline float nbr_weight(const StepConfig& c, uint32_t b, uint32_t j, uint64_t s0, uint32_t deg) {
-  return c.all_nbrs ? edge_prob(c, s0, j) * (float)deg : c.nbr_w[(size_t)b * c.S + j];
+  if (c.loss_kind == 5) return 1.f;
+  if (c.wavg) return c.all_nbrs ? edge_prob(c, s0, j) * (float)deg : c.nbr_w[(size_t)b * c.S + j];
+  if (c.all_nbrs || deg <= c.S) return 1.f;
+  return (float)deg / (float)c.S;
 }
 __device__ inline uint32_t nbr_row(const StepConfig& c, uint32_t b, uint32_t j) {
   return c.nrow_base + (c.all_nbrs ? c.nbroff[b] : b * c.S) + j;
@@ -742,7 +826,7 @@ __device__ inline void forward_other_loss(const StepConfig& c, StepState* st, ui
 #pragma unroll
       for (int q = 0; q < PL; ++q) g[q] *= co;
       store_slice<D>(c.adj + ((size_t)b * NC + ci) * D, lane, g);
-      if (c.wavg && ci == 1u) {
+      if (c.nbr_rows && ci == 1u) {
 #pragma unroll
         for (int q = 0; q < PL; ++q)
           if (base + q < D) stash[base + q] = g[q];
@@ -805,6 +889,8 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
   const uint32_t deg = (uint32_t)(c.indptr[v + 1] - vs);
   const uint32_t* pn = c.all_nbrs ? c.indices + vs : c.nbr + (size_t)b * c.S;
   const int base = lane * PL;
+  const uint32_t apos = st->anchor_base + b, apass = st->pass;
+  const uint32_t k0 = (uint32_t)c.seed, k1 = (uint32_t)(c.seed >> 32);
 
   __shared__ uint32_t s_wave[NW];
   __shared__ uint32_t s_np[NW];
@@ -817,8 +903,10 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
   const uint32_t Re = c.rcount[b];
   const bool staged = (E <= c.ent_cap) && (Re <= c.ids_cap);
   uint64_t* ent_fs = (uint64_t*)(sm + forward_float_words(NC, K, D));
-  uint32_t* ent_off = (uint32_t*)(ent_fs + c.ent_cap);
-  uint32_t* ids = ent_off + c.ent_cap + 1;
+  uint64_t* ent_mask = ent_fs + c.ent_cap;  // max_features selections
+  uint32_t* ent_off = (uint32_t*)(ent_mask + c.sel_cap);
+  float* ent_sc = (float*)(ent_off + c.ent_cap + 1);
+  uint32_t* ids = (uint32_t*)(ent_sc + c.sel_cap);
   if (staged) {
     uint32_t ne = 0;
     if ((uint32_t)tid < E) {
@@ -826,6 +914,12 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
       const uint64_t fs = c.findptr[node];
       ne = (uint32_t)(c.findptr[node + 1] - fs);
       ent_fs[tid] = fs;
+      if (c.mf_kind) {
+        const FeatSel fsel = feature_select(c, ne, apos, apass, (uint32_t)tid, k0, k1);
+        ent_mask[tid] = fsel.mask;
+        ent_sc[tid] = fsel.scale;
+        ne = (uint32_t)__popcll(fsel.mask);
+      }
     }
     uint32_t tot;
     const uint32_t excl = block_exclusive_scan(ne, s_wave, &tot);
@@ -838,21 +932,23 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
         const uint32_t mid = (lo + hi) >> 1;
         if (ent_off[mid] <= o) lo = mid; else hi = mid;
       }
-      ids[o] = c.fids[ent_fs[lo] + (o - ent_off[lo])];
+      const uint32_t q = o - ent_off[lo];
+      ids[o] = c.fids[ent_fs[lo] + (c.mf_kind ? nth_set_bit(ent_mask[lo], q) : q)];
     }
     __syncthreads();
     // ~h(v): the pooled neighbour occurrences split evenly over the waves
     const uint32_t p0 = ent_off[1], p1 = ent_off[1 + cnt];
     const uint32_t per = (p1 - p0 + NW - 1) / NW;
     const uint32_t a0 = min(p1, p0 + w * per), a1 = min(p1, a0 + per);
-    if (c.wavg) {
-      // weighted neighbours: neighbour j's occurrences count w_j + 1 (dealt round-robin)
+    if (c.nbr_rows) {
+      // neighbour j's occurrences count w_j + s_j (s_j its max_features scale, else 1),
+      // neighbours dealt round-robin
       float acc[PL];
 #pragma unroll
       for (int q = 0; q < PL; ++q) acc[q] = 0.f;
       float nw = 0.f;
       for (uint32_t j = w; j < cnt; j += NW) {
-        const float cj = nbr_weight(c, b, j, vs, deg) + 1.f;
+        const float cj = nbr_weight(c, b, j, vs, deg) + (c.mf_kind ? ent_sc[1 + j] : 1.f);
         float t[PL];
 #pragma unroll
         for (int q = 0; q < PL; ++q) t[q] = 0.f;
@@ -906,19 +1002,29 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
     for (uint32_t j = w; j < cnt; j += NW) {
       const uint32_t u = pn[j];
       const uint64_t fs = c.findptr[u], fe = c.findptr[u + 1];
-      if (c.wavg) {
-        const float cj = nbr_weight(c, b, j, vs, deg) + 1.f;
+      if (c.nbr_rows) {
+        float cj = nbr_weight(c, b, j, vs, deg);
         float t[PL];
 #pragma unroll
         for (int q = 0; q < PL; ++q) t[q] = 0.f;
-        gather_rows<D, U>(table, c.fids, fs, fe, lane, t);
+        uint32_t r = (uint32_t)(fe - fs);
+        if (c.mf_kind) {
+          const FeatSel fsel = feature_select(c, r, apos, apass, 1u + j, k0, k1);
+          gather_masked<D, U>(table, c.fids, fs, fsel.mask, lane, t);
+          r = (uint32_t)__popcll(fsel.mask);
+          cj += fsel.scale;
+        } else {
+          gather_rows<D, U>(table, c.fids, fs, fe, lane, t);
+          cj += 1.f;
+        }
 #pragma unroll
         for (int q = 0; q < PL; ++q) acc[q] += cj * t[q];
-        nw += cj * (float)(fe - fs);
+        nw += cj * (float)r;
+        np += r;
       } else {
         gather_rows<D, U>(table, c.fids, fs, fe, lane, acc);
+        np += (uint32_t)(fe - fs);
       }
-      np += (uint32_t)(fe - fs);
     }
     if (lane == 0) s_npw[w] = nw;
 #pragma unroll
@@ -932,8 +1038,14 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
 #pragma unroll
       for (int q = 0; q < PL; ++q) a2[q] = 0.f;
       const uint64_t fs = c.findptr[node], fe = c.findptr[node + 1];
-      gather_rows<D, U>(table, c.fids, fs, fe, lane, a2);
-      const float n = (float)(fe - fs);
+      float n = (float)(fe - fs);
+      if (c.mf_kind) {
+        const FeatSel fsel = feature_select(c, (uint32_t)(fe - fs), apos, apass, i == 0u ? 0u : cnt + i, k0, k1);
+        gather_masked<D, U>(table, c.fids, fs, fsel.mask, lane, a2);
+        n = (float)__popcll(fsel.mask);
+      } else {
+        gather_rows<D, U>(table, c.fids, fs, fe, lane, a2);
+      }
 #pragma unroll
       for (int q = 0; q < PL; ++q)
         if (base + q < D) xc[ci * D + base + q] = a2[q] / n;
@@ -943,7 +1055,7 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
     if (tid == 0) sn[1] = (float)(s_np[0] + s_np[1] + s_np[2] + s_np[3]);
     __syncthreads();
   }
-  if (c.wavg) {
+  if (c.nbr_rows) {
     if (tid == 0) sn[1] = ((s_npw[0] + s_npw[1]) + s_npw[2]) + s_npw[3];
     __syncthreads();
   }
@@ -1052,7 +1164,7 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
 #pragma unroll
       for (int q = 0; q < PL; ++q) g[q] *= co;
       store_slice<D>(c.adj + ((size_t)b * NC + ci) * D, lane, g);
-      if (c.wavg && ci == 1u) {
+      if (c.nbr_rows && ci == 1u) {
 #pragma unroll
         for (int q = 0; q < PL; ++q)
           if (base + q < D) part[base + q] = g[q];
@@ -1065,12 +1177,21 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
     }
   }
   }
-  if (c.wavg) {
-    // weighted neighbours: neighbour j's adjoint row is (dL/dp / sum of counts) * (w_j + 1)
+  if (c.nbr_rows) {
+    // neighbour j's adjoint row is (dL/dp / sum of counts) * (w_j + s_j)
     __syncthreads();
     if (s_el[1]) {
       for (uint32_t j = w; j < cnt; j += NW) {
-        const float cj = nbr_weight(c, b, j, vs, deg) + 1.f;
+        float sj = 1.f;
+        if (c.mf_kind) {
+          if (staged) {
+            sj = ent_sc[1 + j];
+          } else {
+            const uint32_t u = pn[j];
+            sj = feature_select(c, (uint32_t)(c.findptr[u + 1] - c.findptr[u]), apos, apass, 1u + j, k0, k1).scale;
+          }
+        }
+        const float cj = nbr_weight(c, b, j, vs, deg) + sj;
         float g[PL];
 #pragma unroll
         for (int q = 0; q < PL; ++q) g[q] = (base + q < D) ? part[base + q] * cj : 0.f;
@@ -1114,7 +1235,7 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
     uint32_t key = c.sentinel;
     if (s_el[ci] && (!first_leaf || hash_lookup(hsh, hmask, f) == ci)) key = key_of_row(c, f);
     c.keys[0][kb + o] = key;
-    const uint32_t row = (c.wavg && ci == 1u) ? nbr_row(c, b, e - 1u) : b * NC + ci;
+    const uint32_t row = (c.nbr_rows && ci == 1u) ? nbr_row(c, b, e - 1u) : b * NC + ci;
     c.slots[0][kb + o] = ((b / c.rbatch) << kSlotBatchShift) | row;
   }
 }
@@ -1142,7 +1263,10 @@ __global__ __launch_bounds__(256) void k_emit(StepConfig c, StepState* st) {
   __shared__ uint32_t s_off[257];
   __shared__ uint32_t s_node[256];
   __shared__ uint8_t s_ci[256];
+  __shared__ uint64_t s_mask[256];  // max_features selections
   const uint32_t H = c.hash_entries, hmask = H - 1u;
+  const uint32_t apos = st->anchor_base + b, apass = st->pass;
+  const uint32_t k0 = (uint32_t)c.seed, k1 = (uint32_t)(c.seed >> 32);
   const bool first_leaf = c.grad_mode == 0;
   const bool att = c.model == 1;
   const uint32_t v = c.perm[st->anchor_base + b];
@@ -1165,6 +1289,11 @@ __global__ __launch_bounds__(256) void k_emit(StepConfig c, StepState* st) {
         else if (e <= cnt) { node = pn[e - 1u]; ci = 1; }
         else { node = c.neg[(size_t)b * K + (e - cnt - 1u)]; ci = (uint8_t)(2u + (e - cnt - 1u)); }
         ne = (uint32_t)(c.findptr[node + 1] - c.findptr[node]);
+        if (c.mf_kind) {
+          const uint64_t m = feature_select(c, ne, apos, apass, e, k0, k1).mask;
+          s_mask[tid] = m;
+          ne = (uint32_t)__popcll(m);
+        }
       }
       uint32_t total;
       const uint32_t excl = block_exclusive_scan(ne, s_wave, &total);
@@ -1186,7 +1315,7 @@ __global__ __launch_bounds__(256) void k_emit(StepConfig c, StepState* st) {
         const uint32_t cj = s_ci[j];
         const uint64_t nfs = c.findptr[nd];
         const uint32_t qpos = o - s_off[j];
-        const uint32_t f = c.fids[nfs + qpos];
+        const uint32_t f = c.fids[nfs + (c.mf_kind ? nth_set_bit(s_mask[j], qpos) : qpos)];
         const bool ok = s_el[cj] != 0;
         if (phase == 0) {
           if (ok) hash_insert_min(hsh, hmask, f, cj);
@@ -1201,7 +1330,7 @@ __global__ __launch_bounds__(256) void k_emit(StepConfig c, StepState* st) {
           c.keys[0][pos] = emit ? key_of_row(c, f) : c.sentinel;
           // mean model: the slot of the construction's adjoint; attention: the
           // occurrence's own gradient row
-          const uint32_t row = att ? pos : ((c.wavg && cj == 1u) ? nbr_row(c, b, cs + j - 1u) : slot0 + cj);
+          const uint32_t row = att ? pos : ((c.nbr_rows && cj == 1u) ? nbr_row(c, b, cs + j - 1u) : slot0 + cj);
           c.slots[0][pos] = ((b / c.rbatch) << kSlotBatchShift) | row;
         }
       }
@@ -2386,8 +2515,8 @@ hipError_t launch_scan(const StepConfig& c, StepState* st, hipStream_t s) {
 size_t forward_lds_bytes(const StepConfig& c) {
   const uint32_t NC = c.K + 2;
   const size_t hash = (c.train && c.fuse_emit && c.grad_mode == 0) ? (size_t)c.hash_entries * 4 : 0;
-  return (size_t)forward_float_words(NC, c.K, c.dims) * 4 + (size_t)c.ent_cap * 8 + (size_t)(c.ent_cap + 1) * 4 +
-         (size_t)c.ids_cap * 4 + hash;
+  return (size_t)forward_float_words(NC, c.K, c.dims) * 4 + (size_t)c.ent_cap * 8 + (size_t)c.sel_cap * 12 +
+         (size_t)(c.ent_cap + 1) * 4 + (size_t)c.ids_cap * 4 + hash;
 }
 
 hipError_t launch_forward(const StepConfig& c, StepState* st, hipStream_t s) {

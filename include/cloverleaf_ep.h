@@ -34,7 +34,7 @@ enum {
 };
 
 /* Sample kinds of max_nodes / max_features (src/algos/utils.rs:166-191).       */
-enum { EP_SAMPLE_ALL = 0, EP_SAMPLE_FIXED = 1 };
+enum { EP_SAMPLE_ALL = 0, EP_SAMPLE_FIXED = 1, EP_SAMPLE_PROBABILITY = 2 };
 /* LossWeighting (src/algos/ep/mod.rs:31-36).                                   */
 enum { EP_LW_NONE = 0, EP_LW_LOG = 1, EP_LW_EXPONENTIAL = 2 };
 /* Table precision.  FP32: FP32 table.  BF16: BF16 table; each touched row is taken
@@ -75,8 +75,11 @@ typedef struct ep_config {
   float valid_pct;             /* default 0.1                                       */
   int32_t max_nodes_kind;      /* EP_SAMPLE_ALL (default) / EP_SAMPLE_FIXED         */
   uint32_t max_nodes;          /* S when Fixed                                      */
-  int32_t max_features_kind;   /* EP_SAMPLE_ALL only                                */
-  uint32_t max_features;
+  int32_t max_features_kind;   /* EP_SAMPLE_*: every node of an example keeps a    */
+                               /* subset of its features (model.rs:264-269); Fixed  */
+                               /* and Probability need <= 64 features per node and  */
+                               /* the mean model                                    */
+  uint32_t max_features;       /* k when Fixed                                      */
   int32_t loss_weighting;      /* EP_LW_*                                           */
   float loss_weight_exp;       /* exponent of EP_LW_EXPONENTIAL                     */
   uint32_t hard_negatives;     /* random-walk hard negatives per example, at most K */
@@ -116,6 +119,7 @@ typedef struct ep_config {
                                /* gamma, contrastive positive margin, rank tau,      */
                                /* rankspace gamma, ppr gamma                         */
   float loss_param2;           /* contrastive negative margin / ppr restart_p        */
+  float max_features_p;        /* p in (0, 1) when max_features is Probability      */
 } ep_config;
 
 typedef struct ep_ctx ep_ctx;

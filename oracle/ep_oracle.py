@@ -139,6 +139,15 @@ def sample_spec(kind: str, k: float, n: int) -> Tuple[int, float]:
     raise ValueError(kind)
 
 
+def probability_sample(p: float, n: int, k: int) -> Tuple[int, float]:
+    """``Sample::Probability(p).sample(n, at_least_one=true)`` given the Binomial draw k
+    (utils.rs:203-210), in f32 as the reference: (k, 1/p), or (1, (1 - p^n)/p) when k = 0."""
+    p32 = np.float32(p)
+    if k == 0:
+        return 1, float((np.float32(1.0) - np.float32(p32 ** np.float32(n))) / p32)
+    return k, float(np.float32(1.0) / p32)
+
+
 # --------------------------------------------------------------------------------------
 # Constructions (src/algos/ep/model.rs)
 # --------------------------------------------------------------------------------------
@@ -181,25 +190,38 @@ def node_features(f_indptr, f_ids, node) -> np.ndarray:
     return np.asarray(f_ids[int(f_indptr[node]):int(f_indptr[node + 1])], dtype=np.int64)
 
 
-def build_anchor_construction(f_indptr, f_ids, v) -> Construction:
-    """h(v): construct_node_embedding (model.rs:283-301) called with weight 1 (mod.rs:300-301)."""
+def _selected(f_indptr, f_ids, u, sel):
+    """The node's features taken by max_features (sel = (positions, scale)) or all of them."""
+    feats = node_features(f_indptr, f_ids, u)
+    if sel is None:
+        return feats, 1.0
+    pos, scale = sel
+    return feats[np.asarray(sorted(pos), dtype=np.int64)], float(scale)
+
+
+def build_anchor_construction(f_indptr, f_ids, v, sel=None) -> Construction:
+    """h(v): construct_node_embedding (model.rs:283-301) called with weight 1 (mod.rs:300-301).
+    ``sel`` = (positions, scale): the features max_features took and its scale."""
     c = Construction()
-    collect_node(c, node_features(f_indptr, f_ids, v), 1.0)
+    feats, sc = _selected(f_indptr, f_ids, v, sel)
+    collect_node(c, feats, 1.0, sc)
     return c
 
 
-def build_neighbour_construction(f_indptr, f_ids, nbrs: Sequence[int], scale_weight) -> Construction:
+def build_neighbour_construction(f_indptr, f_ids, nbrs: Sequence[int], scale_weight, sels=None) -> Construction:
     """~h(v): reconstruct_node_embedding (model.rs:342-386) -> construct_from_multiple_nodes (388-416).
 
     ``nbrs`` is the explicit sampled neighbour list.  ``scale_weight`` is each
     neighbour's weight (model.rs:357-361): one number for every neighbour (the
     unbiasing scale of ``Sample::sample``, deg/S when subsampled, 1 otherwise), or
-    one weight per neighbour (weighted averaging, see ``neighbour_weights``).
+    one weight per neighbour (weighted averaging, see ``neighbour_weights``).  ``sels[j]``
+    = (positions, scale) are the features max_features took from neighbour j.
     """
     c = Construction()
     ws = [float(scale_weight)] * len(nbrs) if np.isscalar(scale_weight) else [float(x) for x in scale_weight]
-    for u, w in zip(nbrs, ws):
-        collect_node(c, node_features(f_indptr, f_ids, u), w)
+    for j, (u, w) in enumerate(zip(nbrs, ws)):
+        feats, sc = _selected(f_indptr, f_ids, u, None if sels is None else sels[j])
+        collect_node(c, feats, w, sc)
     return c
 
 
@@ -399,7 +421,7 @@ class ExampleResult:
 
 
 def example_grads(table, f_indptr, f_ids, anchor, nbrs, nbr_scale, negs, gamma,
-                  grad_mode="first_leaf", loss_div=1.0, loss=None) -> ExampleResult:
+                  grad_mode="first_leaf", loss_div=1.0, loss=None, feat_sel=None) -> ExampleResult:
     """Forward + adjoint + gradient extraction for one anchor with explicit samples.
 
     Constructions are ordered a=h(v), p=~h(v), n_1..n_K (mod.rs:300-320).
@@ -412,11 +434,15 @@ def example_grads(table, f_indptr, f_ids, anchor, nbrs, nbr_scale, negs, gamma,
       "first_leaf"  the reference rule (mod.rs:347-352, 366-367): the first
                     eligible construction containing f, in the order above.
       "exact"       the true gradient: sum over every eligible construction.
+
+    feat_sel (max_features sampling): one (positions, scale) per node in the order
+    v, nbrs, negs; None takes every feature.
     """
-    cons = [build_anchor_construction(f_indptr, f_ids, anchor),
-            build_neighbour_construction(f_indptr, f_ids, nbrs, nbr_scale)]
-    for u in negs:
-        cons.append(build_anchor_construction(f_indptr, f_ids, u))
+    fs = [None] * (1 + len(nbrs) + len(negs)) if feat_sel is None else list(feat_sel)
+    cons = [build_anchor_construction(f_indptr, f_ids, anchor, fs[0]),
+            build_neighbour_construction(f_indptr, f_ids, nbrs, nbr_scale, fs[1:1 + len(nbrs)])]
+    for k, u in enumerate(negs):
+        cons.append(build_anchor_construction(f_indptr, f_ids, u, fs[1 + len(nbrs) + k]))
     vecs = [construct_mean(table, c) for c in cons]
     kind, params = loss if loss is not None else ("margin", (gamma,))
     L, _, g_a, g_p, g_n = loss_adjoints(kind, vecs[0], vecs[1], vecs[2:], params)
@@ -507,15 +533,17 @@ def adam_update(table, mom, var, grads: Dict[int, np.ndarray], lr: float, pass_:
         table[f] = table[f] - lr * (mh / (np.sqrt(vh) + ADAM_EPS))
 
 
-def _example(table, f_indptr, f_ids, v, nbrs, nbr_scale, negs, gamma, grad_mode, div, attention, loss=None):
+def _example(table, f_indptr, f_ids, v, nbrs, nbr_scale, negs, gamma, grad_mode, div, attention, loss=None,
+             feat_sel=None):
     if attention is None:
-        return example_grads(table, f_indptr, f_ids, v, nbrs, nbr_scale, negs, gamma, grad_mode, div, loss)
+        return example_grads(table, f_indptr, f_ids, v, nbrs, nbr_scale, negs, gamma, grad_mode, div, loss, feat_sel)
     heads, d_k = attention
     return attention_example_grads(table, f_indptr, f_ids, v, nbrs, negs, gamma, heads, d_k, grad_mode, div, loss)
 
 
 def ep_batch_step(table, mom, var, f_indptr, f_ids, anchors, nbr_lists, nbr_scales, neg_lists,
-                  gamma, lr, pass_, grad_mode="first_leaf", loss_divs=None, attention=None, loss=None):
+                  gamma, lr, pass_, grad_mode="first_leaf", loss_divs=None, attention=None, loss=None,
+                  feat_sels=None):
     """One batch: per-example forward/backward, merge, Adam (mod.rs:174-252).
 
     Updates table/mom/var in place (float64 arrays).  Returns (results, merged grads).
@@ -527,7 +555,7 @@ def ep_batch_step(table, mom, var, f_indptr, f_ids, anchors, nbr_lists, nbr_scal
     for i, v in enumerate(anchors):
         div = 1.0 if loss_divs is None else loss_divs[i]
         results.append(_example(table, f_indptr, f_ids, v, nbr_lists[i], nbr_scales[i], neg_lists[i], gamma,
-                                grad_mode, div, attention, loss))
+                                grad_mode, div, attention, loss, None if feat_sels is None else feat_sels[i]))
     merged = merge_batch(results, table.shape[1])
     if any(r.contributes for r in results):
         adam_update(table, mom, var, merged, lr, pass_)

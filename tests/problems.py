@@ -49,10 +49,11 @@ def make_problem(seed=0, n=120, V=300, D=32, avg_deg=6, fmax=8, zipf=1.1, zero_d
 
 
 def oracle_step(prob: Problem, anchors, cnt, nbr, neg, gamma, lr, pass_, grad_mode, table=None, mom=None, var=None,
-                loss_divs=None, attention=None, loss=None, nbr_weights=None):
+                loss_divs=None, attention=None, loss=None, nbr_weights=None, feat_sels=None):
     """FP64 oracle of one explicit-sample step; returns (results, merged, table, mom, var).
     attention = (heads, d_k) selects the attention model; loss = (kind, params) the loss;
-    nbr_weights[i][j] is neighbour j's weight (weighted averaging; default uniform)."""
+    nbr_weights[i][j] is neighbour j's weight (weighted averaging; default uniform);
+    feat_sels[i] the max_features selections of example i (see example_grads)."""
     t = (prob.table if table is None else table).astype(np.float64).copy()
     m = np.zeros_like(t) if mom is None else mom.astype(np.float64).copy()
     v = np.zeros_like(t) if var is None else var.astype(np.float64).copy()
@@ -66,7 +67,8 @@ def oracle_step(prob: Problem, anchors, cnt, nbr, neg, gamma, lr, pass_, grad_mo
                                     [1.0] * len(keep) if nbr_weights is None
                                     else [list(map(float, nbr_weights[i][:cnt[i]])) for i in keep],
                                     [list(map(int, neg[i])) for i in keep], gamma, lr, pass_, grad_mode,
-                                    None if loss_divs is None else [loss_divs[i] for i in keep], attention, loss)
+                                    None if loss_divs is None else [loss_divs[i] for i in keep], attention, loss,
+                                    None if feat_sels is None else [feat_sels[i] for i in keep])
     it = iter(res_k)
     results = [next(it) if i in keep else None for i in range(B)]
     return results, merged, t, m, v

@@ -892,3 +892,56 @@ def test_align_uniform_weights_give_neighbour_mean():
         s, e = indptr[v], indptr[v + 1]
         want = emb[cols[s:e]].mean(axis=0) if e > s else np.zeros(5)
         assert np.allclose(out[v], want, atol=1e-6)
+
+
+# --------------------------------------------------------------------------------------
+# max_features sampling (utils.rs:193-216, model.rs:257-278)
+# --------------------------------------------------------------------------------------
+def test_probability_sample_kat():
+    # Binomial draw k = 0 with at_least_one: one item, scale (1 - p^n) / p (as written)
+    k, s = O.probability_sample(0.25, 3, 0)
+    assert k == 1 and s == float((np.float32(1) - np.float32(0.25) ** np.float32(3)) / np.float32(0.25))
+    assert abs(s - 3.9375) < 1e-7
+    assert O.probability_sample(0.25, 3, 2) == (2, 4.0)
+    assert O.sample_spec("fixed", 2, 5) == (2, 2.5) and O.sample_spec("fixed", 7, 5) == (5, 1.0)
+
+
+def test_feature_selection_constructions_closed_form():
+    # node 0: features [0, 1, 2, 3]; node 1: [1, 4]; node 2: [2]
+    f_indptr = np.array([0, 4, 6, 7], dtype=np.uint64)
+    f_ids = np.array([0, 1, 2, 3, 1, 4, 2], dtype=np.uint32)
+    # h(v) keeps positions {1, 3} with scale 2: counts 1 + 2 each -> plain mean of rows 1, 3
+    a = O.build_anchor_construction(f_indptr, f_ids, 0, ([3, 1], 2.0))
+    assert a.counts == {1: 3.0, 3: 3.0}
+    # ~h(v) over nodes 1 (weight 1.5, keeps position 1 = feature 4, scale 2) and 2 (weight
+    # 1.5, all, scale 1): counts 1.5 + 2 for feature 4 and 1.5 + 1 for feature 2
+    p = O.build_neighbour_construction(f_indptr, f_ids, [1, 2], 1.5, [([1], 2.0), ([0], 1.0)])
+    assert p.counts == {4: 3.5, 2: 2.5}
+    table = np.eye(5)
+    assert np.allclose(O.construct_mean(table, p), (3.5 * table[4] + 2.5 * table[2]) / 6.0)
+    # taking every position with scale 1 is the unsampled path
+    full = O.build_neighbour_construction(f_indptr, f_ids, [0, 1], 1.0, [([0, 1, 2, 3], 1.0), ([0, 1], 1.0)])
+    assert full.counts == O.build_neighbour_construction(f_indptr, f_ids, [0, 1], 1.0).counts
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_feature_selection_exact_grads_fd(seed):
+    table, f_indptr, f_ids, v, nbrs, negs = _toy_example(seed)
+    rng = np.random.default_rng(seed + 20)
+    sel = []
+    for u in [v] + list(nbrs) + list(negs):
+        n = int(f_indptr[u + 1] - f_indptr[u])
+        r = int(rng.integers(1, n + 1))
+        sel.append((sorted(rng.choice(n, r, replace=False).tolist()), n / r))
+    gamma = 2.0
+    r = O.example_grads(table, f_indptr, f_ids, v, nbrs, 1.0, negs, gamma, "exact", feat_sel=sel)
+    assert r.contributes
+    eps = 1e-6
+    loss = lambda t: O.example_grads(t, f_indptr, f_ids, v, nbrs, 1.0, negs, gamma, "exact", feat_sel=sel).loss
+    for fr in range(table.shape[0]):
+        for j in range(table.shape[1]):
+            tp, tm = table.copy(), table.copy()
+            tp[fr, j] += eps
+            tm[fr, j] -= eps
+            g = r.grads[fr][j] if fr in r.grads else 0.0
+            assert abs((loss(tp) - loss(tm)) / (2 * eps) - g) < 1e-6
