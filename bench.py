@@ -171,6 +171,43 @@ def pmc_traffic(kernel, workload, form):
     return ent["fetch_bytes"] + ent["write_bytes"]
 
 
+def bench_align(w, args):
+    """NeighborhoodAligner.align over every node of the workload's graph (edge weights:
+    uniform rows), random FP32 node embeddings of --align-dims; the graph and embeddings
+    are resident in HBM, each run aligns all nodes (kernel timed with HIP events)."""
+    from cloverleaf_amd import _native as N
+    from cloverleaf_amd.api import Graph
+
+    g = Graph.from_csr(w.indptr, w.indices)
+    cdf = g.edge_cdf()
+    d = args.align_dims
+    emb = np.random.default_rng(1).standard_normal((w.n_nodes, d), dtype=np.float32)
+    deg = np.diff(w.indptr.astype(np.int64))
+    m = np.minimum(deg, args.align_max_neighbors) if args.align_max_neighbors else deg
+    algo = float(((m + 2) * d * 4 + 8 * m).sum())  # neighbour rows + own row + output row, edge id + probability
+    ms = []
+    with N.Aligner(w.indptr, w.indices, cdf, emb) as al:
+        for i in range(args.warmup + args.steps):
+            al.run(0.5, args.align_max_neighbors or None)
+            if i >= args.warmup:
+                ms.append(al.last_kernel_ms())
+    avg = float(np.mean(ms))
+    gbs = algo / (avg * 1e-3) / 1e9
+    print(f"[bench] align: {w.n_nodes} nodes, {int(m.sum())} edges read, d={d}: {avg:.3f} ms per run, "
+          f"{gbs:.0f} GB/s algorithmic", file=sys.stderr, flush=True)
+    print(json.dumps({
+        "metric": "NeighborhoodAligner.align nodes/sec", "value": round(w.n_nodes / (avg * 1e-3), 1),
+        "unit": "nodes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(avg, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded graph, random node embeddings)",
+        "config": {"workload": w.name, "nodes": w.n_nodes, "csr_entries": int(len(w.indices)), "dims": d,
+                   "max_neighbors": args.align_max_neighbors or None, "alpha": 0.5},
+        "roofline": {"bound": "hbm", "kernel": "k_align", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": algo},
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,6 +226,10 @@ def main():
                     help="launch the batched forms eagerly (for hardware-counter runs)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (HBM-counter calibration)")
+    ap.add_argument("--align", action="store_true",
+                    help="measure NeighborhoodAligner.align (k_align) on the workload's graph instead of training")
+    ap.add_argument("--align-dims", type=int, default=128)
+    ap.add_argument("--align-max-neighbors", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -216,6 +257,9 @@ def main():
     print(f"[bench] workload {w.name}: {w.n_nodes} nodes, {len(w.indices)} CSR entries, vocab {w.vocab} "
           f"(generated in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
 
+    if args.align:
+        bench_align(w, args)
+        return
     if not args.concurrent:
         args.concurrent = 2 if w.att_dk else 8
     results = {}

@@ -117,6 +117,7 @@ SIGNATURES = {
     "ep_aligner_create": (c_int, [c_int, c_uint64, _u64p, _u32p, _f32p, c_uint32, _f32p, POINTER(c_void_p)]),
     "ep_aligner_run": (c_int, [c_void_p, c_float, c_uint32, c_uint64, c_uint64, _f32p]),
     "ep_aligner_destroy": (None, [c_void_p]),
+    "ep_aligner_last_kernel_ms": (c_float, [c_void_p]),
     "ep_sample_weights": (c_int, [c_void_p, c_uint32, _f32p]),
     "ep_set_explicit_weights": (c_int, [c_void_p, c_uint32, _f32p]),
     "ep_set_features": (c_int, [c_void_p, c_uint64, c_uint32, _u64p, _u32p]),
@@ -209,6 +210,9 @@ class Aligner:
         check(load().ep_aligner_run(self.handle, a, int(max_neighbors or 0), int(first), int(count),
                                     ptr(out, c_float)), "ep_aligner_run")
         return out
+
+    def last_kernel_ms(self) -> float:
+        return float(load().ep_aligner_last_kernel_ms(self.handle))
 
     def close(self):
         if getattr(self, "handle", None):

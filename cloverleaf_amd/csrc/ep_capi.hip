@@ -1337,6 +1337,8 @@ struct ep_aligner {
   float* emb = nullptr;
   float* out = nullptr;
   uint64_t out_rows = 0;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  float last_ms = 0.f;  // duration of the last run's kernel (HIP events)
 };
 
 static void aligner_free(ep_aligner* a) {
@@ -1345,6 +1347,8 @@ static void aligner_free(ep_aligner* a) {
   void* ps[] = {a->indptr, a->indices, a->cdf, a->emb, a->out};
   for (void* q : ps)
     if (q) (void)hipFree(q);
+  for (hipEvent_t e : a->ev)
+    if (e) (void)hipEventDestroy(e);
   if (a->stream) (void)hipStreamDestroy(a->stream);
   delete a;
 }
@@ -1371,6 +1375,8 @@ int ep_aligner_create(int device, uint64_t n_nodes, const uint64_t* indptr, cons
   auto bail = [&](int code, const std::string& msg) { aligner_free(a); return fail(code, msg); };
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&a->ev[0]);
+  if (e == hipSuccess) e = hipEventCreate(&a->ev[1]);
   if (e == hipSuccess) e = hipMalloc((void**)&a->indptr, (n_nodes + 1) * sizeof(uint64_t));
   if (e == hipSuccess) e = hipMalloc((void**)&a->indices, std::max<uint64_t>(ne, 1) * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMalloc((void**)&a->cdf, std::max<uint64_t>(ne, 1) * sizeof(float));
@@ -1398,12 +1404,17 @@ int ep_aligner_run(ep_aligner* a, float alpha, uint32_t max_neighbors, uint64_t 
     EP_HIP(hipMalloc((void**)&a->out, (size_t)count * a->dims * sizeof(float)));
     a->out_rows = count;
   }
+  EP_HIP(hipEventRecord(a->ev[0], a->stream));
   EP_HIP(ep::launch_align((int)a->dims, a->emb, a->indptr, a->indices, a->cdf, adaptive ? 0.f : alpha, adaptive,
                           max_neighbors, first, count, a->out, a->stream));
+  EP_HIP(hipEventRecord(a->ev[1], a->stream));
   EP_HIP(hipMemcpyAsync(out, a->out, (size_t)count * a->dims * sizeof(float), hipMemcpyDeviceToHost, a->stream));
   EP_HIP(hipStreamSynchronize(a->stream));
+  EP_HIP(hipEventElapsedTime(&a->last_ms, a->ev[0], a->ev[1]));
   return EP_OK;
 }
+
+float ep_aligner_last_kernel_ms(const ep_aligner* a) { return a ? a->last_ms : 0.f; }
 
 void ep_aligner_destroy(ep_aligner* a) { aligner_free(a); }
 

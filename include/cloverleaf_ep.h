@@ -265,6 +265,8 @@ int ep_aligner_create(int device, uint64_t n_nodes, const uint64_t* indptr, cons
                       uint32_t dims, const float* emb, ep_aligner** out);
 int ep_aligner_run(ep_aligner* a, float alpha, uint32_t max_neighbors, uint64_t first, uint64_t count, float* out);
 void ep_aligner_destroy(ep_aligner* a);
+/* Measurement hook: duration (ms, HIP events) of the last run's alignment kernel.  */
+float ep_aligner_last_kernel_ms(const ep_aligner* a);
 
 /* Philox4x32-10 on the host (the device RNG), for known-answer tests.           */
 int ep_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
