@@ -63,6 +63,7 @@ class EPConfig(Structure):
         ("loss_kind", c_int32),
         ("loss_param2", c_float),
         ("max_features_p", c_float),
+        ("max_nodes_p", c_float),
     ]
 
 
@@ -274,6 +275,14 @@ class Context:
         n = len(indptr) - 1
         check(load().ep_set_graph(self.handle, n, ptr(indptr, c_uint64), ptr(indices, c_uint32)), "ep_set_graph")
         self.n_nodes = n
+        self.deg_max = int(np.diff(indptr.astype(np.int64)).max(initial=0))
+
+    def list_width(self) -> int:
+        """Width of the neighbour lists of sample_batch / step_explicit: max_nodes (Fixed),
+        K (PPR), the largest degree (max_nodes Probability)."""
+        if self.cfg.max_nodes_kind == EP_SAMPLE_PROBABILITY and self.cfg.loss_kind != EP_LOSS_PPR:
+            return max(self.deg_max, 1)
+        return list_cap(self.cfg)
 
     def set_edge_weights(self, cdf: np.ndarray):
         """Per-edge CDF of each row (CumCSR weights), one value per CSR entry."""
@@ -282,7 +291,7 @@ class Context:
 
     def sample_weights(self, n: int) -> np.ndarray:
         """Neighbour weights w_j of the last sample_batch (n x max_nodes)."""
-        w = np.zeros((n, list_cap(self.cfg)), dtype=np.float32)
+        w = np.zeros((n, self.list_width()), dtype=np.float32)
         check(load().ep_sample_weights(self.handle, n, ptr(w, c_float)), "ep_sample_weights")
         return w
 
@@ -342,7 +351,7 @@ class Context:
     def sample_batch(self, anchors: np.ndarray, pass_: int):
         anchors = np.ascontiguousarray(anchors, dtype=np.uint32)
         n = len(anchors)
-        S, K = list_cap(self.cfg), self.cfg.negatives
+        S, K = self.list_width(), self.cfg.negatives
         cnt = np.zeros(n, dtype=np.uint32)
         nbr = np.zeros((n, S), dtype=np.uint32)
         neg = np.zeros((n, K), dtype=np.uint32)
@@ -355,7 +364,7 @@ class Context:
         """One step on explicit samples.  Returns dict(loss, cons, feat, grad)."""
         anchors = np.ascontiguousarray(anchors, dtype=np.uint32)
         n = len(anchors)
-        S, K, D, W = list_cap(self.cfg), self.cfg.negatives, self.cfg.dims, row_width(self.cfg)
+        S, K, D, W = self.list_width(), self.cfg.negatives, self.cfg.dims, row_width(self.cfg)
         nbr_cnt = np.ascontiguousarray(nbr_cnt, dtype=np.uint32)
         nbr = np.ascontiguousarray(nbr, dtype=np.uint32).reshape(n, S)
         neg = np.ascontiguousarray(neg, dtype=np.uint32).reshape(n, K)

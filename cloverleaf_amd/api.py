@@ -587,9 +587,9 @@ class EmbeddingPropagator:
                  grad_mode="first_leaf", scatter="sort", use_graph=False, concurrent_batches=1, device=0):
         loss = loss or EPLoss.margin(1.0, 1)
         mf_kind, mf, mf_p = _parse_sample(max_features, "max_features")
-        mn_kind, mn, _ = _parse_sample(max_nodes, "max_nodes")
-        if mn_kind == N.EP_SAMPLE_PROBABILITY:
-            raise ValueError("max_nodes: probability sampling is not supported by this build")
+        mn_kind, mn, mn_p = _parse_sample(max_nodes, "max_nodes")
+        if mn_kind == N.EP_SAMPLE_PROBABILITY and attention is not None:
+            raise ValueError("max_nodes probability sampling with attention is not supported by this build")
         if attention is not None and context_window is not None:
             raise ValueError("context_window (Sliding attention) is not supported by this build")
         if mf_kind != N.EP_SAMPLE_ALL and attention is not None:
@@ -610,7 +610,7 @@ class EmbeddingPropagator:
         cfg.loss_param2 = float(loss.params[1]) if len(loss.params) > 1 else 0.0
         cfg.valid_pct = 0.1 if valid_pct is None else float(valid_pct)
         cfg.seed = SEED if seed is None else int(seed)
-        cfg.max_nodes_kind, cfg.max_nodes = mn_kind, mn
+        cfg.max_nodes_kind, cfg.max_nodes, cfg.max_nodes_p = mn_kind, mn, mn_p
         cfg.max_features_kind, cfg.max_features, cfg.max_features_p = mf_kind, mf, mf_p
         if loss_weighting is not None:
             cfg.loss_weighting = loss_weighting.kind

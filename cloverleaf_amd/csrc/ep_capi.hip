@@ -65,6 +65,8 @@ struct ep_ctx {
   uint64_t n_nodes = 0, n_entries = 0;
   uint32_t deg_max = 0;
   std::vector<uint32_t> degrees;
+  std::vector<uint64_t> h_indptr;  // host copy of the CSR offsets (hooks of max_nodes Probability)
+  std::vector<uint32_t> last_anchors, last_cnt;  // the last ep_sample_batch (its weights hook)
   uint64_t* d_indptr = nullptr;
   uint32_t* d_indices = nullptr;
   float* d_cdf = nullptr;        // per-edge CDF (weighted neighbours), ep_set_edge_weights
@@ -173,6 +175,12 @@ bool weighted_avg(const ep_config& cfg) { return cfg.weighted_neighbor_averaging
 
 bool feature_sampling(const ep_config& cfg) { return cfg.max_features_kind != EP_SAMPLE_ALL; }
 
+// max_nodes Probability: variable-length lists stored at the anchors' CSR offsets.
+bool prob_nodes(const ep_config& cfg) { return cfg.max_nodes_kind == EP_SAMPLE_PROBABILITY && !ppr_loss(cfg); }
+
+// Width of the neighbour lists of the sample / explicit-step hooks.
+uint32_t hook_list_width(const ep_ctx* c) { return prob_nodes(c->cfg) ? std::max<uint32_t>(c->deg_max, 1u) : list_cap(c->cfg); }
+
 // ~h(v) with a count per neighbour (one adjoint row each): weighted averaging or
 // max_features sampling (the feature scale differs between neighbours).
 bool neighbour_rows(const ep_config& cfg) { return weighted_avg(cfg) || feature_sampling(cfg); }
@@ -231,7 +239,7 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   // drop old buffers
   ep::StepConfig& b = c->base;
   c->release(b.nbr_cnt); c->release(b.nbr); c->release(b.neg); c->release(b.rcount); c->release(b.keyoff);
-  c->release(b.nbr_w); c->release(b.nbroff);
+  c->release(b.nbr_w); c->release(b.nbroff); c->release(b.nbr_sel); c->release(b.nbr_selw);
   c->release(b.adj); c->release(b.coef); c->release(b.elig); c->release(b.loss);
   c->release(b.keys[0]); c->release(b.keys[1]); c->release(b.slots[0]); c->release(b.slots[1]);
   c->release(b.ghist); c->release(b.G); c->release(b.ufeat); c->release(b.long_st); c->release(b.long_len);
@@ -263,6 +271,10 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   EP_TRY(c->alloc(&b.adj, att ? (size_t)keycap * D : ((size_t)batch * NC + nrow_cap) * D));
   if (wavg && S) EP_TRY(c->alloc(&b.nbr_w, (size_t)batch * S));
   if (nrows && !S) EP_TRY(c->alloc(&b.nbroff, batch));
+  if (prob_nodes(cfg)) {
+    EP_TRY(c->alloc(&b.nbr_sel, std::max<uint64_t>(c->n_entries, 1)));
+    if (nrows) EP_TRY(c->alloc(&b.nbr_selw, std::max<uint64_t>(c->n_entries, 1)));
+  }
   b.nrow_base = batch * NC;
   if (att) {
     const uint32_t dm = cfg.dims;
@@ -396,6 +408,8 @@ void refresh_base(ep_ctx* c) {
   b.mf_k = cfg.max_features;
   b.mf_p = cfg.max_features_p;
   b.nbr_rows = (neighbour_rows(cfg) && !cfg.attention_dk) ? 1 : 0;
+  b.pnbrs = prob_nodes(cfg) ? 1 : 0;
+  b.mn_p = cfg.max_nodes_p;
   b.hard_negs = std::min(cfg.hard_negatives, cfg.negatives);
   b.lw_kind = cfg.loss_weighting;
   b.lw_exp = cfg.loss_weight_exp;
@@ -626,8 +640,13 @@ int validate_config(const ep_config* cfg) {
   if (cfg->passes == 0) return fail(EP_EINVAL, "passes must be > 0");
   if (cfg->max_nodes_kind == EP_SAMPLE_FIXED && (cfg->max_nodes == 0 || cfg->max_nodes > 1024))
     return fail(EP_EINVAL, "max_nodes (Fixed) must be in [1, 1024]");
-  if (cfg->max_nodes_kind != EP_SAMPLE_FIXED && cfg->max_nodes_kind != EP_SAMPLE_ALL)
-    return fail(EP_EINVAL, "max_nodes must be All or Fixed");
+  if (cfg->max_nodes_kind != EP_SAMPLE_FIXED && cfg->max_nodes_kind != EP_SAMPLE_ALL &&
+      cfg->max_nodes_kind != EP_SAMPLE_PROBABILITY)
+    return fail(EP_EINVAL, "max_nodes must be All, Fixed or Probability");
+  if (cfg->max_nodes_kind == EP_SAMPLE_PROBABILITY && !(cfg->max_nodes_p > 0.f && cfg->max_nodes_p < 1.f))
+    return fail(EP_EINVAL, "max_nodes (Probability) must be in (0, 1)");
+  if (cfg->max_nodes_kind == EP_SAMPLE_PROBABILITY && cfg->attention_dk)
+    return fail(EP_EINVAL, "max_nodes Probability is not supported with attention");
   if (cfg->max_features_kind != EP_SAMPLE_ALL && cfg->max_features_kind != EP_SAMPLE_FIXED &&
       cfg->max_features_kind != EP_SAMPLE_PROBABILITY)
     return fail(EP_EINVAL, "max_features must be All, Fixed or Probability");
@@ -756,6 +775,7 @@ int ep_set_graph(ep_ctx* c, uint64_t n_nodes, const uint64_t* indptr, const uint
   c->n_entries = ne;
   c->deg_max = dmax;
   c->degrees.swap(deg);
+  c->h_indptr.assign(indptr, indptr + n_nodes + 1);
   c->cap_batch = 0;  // capacities depend on degrees
   c->split_set = false;
   return EP_OK;
@@ -993,9 +1013,15 @@ int ep_learn(ep_ctx* c, ep_progress_fn progress, void* user) {
 int ep_sample_batch(ep_ctx* c, const uint32_t* anchors, uint32_t n, uint32_t pass, uint32_t* nbr_cnt, uint32_t* nbr,
                     uint32_t* neg) {
   if (!c || !anchors || !nbr_cnt || !nbr || !neg || n == 0) return fail(EP_EINVAL, "bad argument");
-  if (c->cfg.max_nodes_kind != EP_SAMPLE_FIXED && !ppr_loss(c->cfg))
-    return fail(EP_EINVAL, "ep_sample_batch needs max_nodes Fixed");
+  if (c->cfg.max_nodes_kind == EP_SAMPLE_ALL && !ppr_loss(c->cfg))
+    return fail(EP_EINVAL, "ep_sample_batch needs max_nodes Fixed or Probability");
   for (uint32_t i = 0; i < n; ++i) if (anchors[i] >= c->n_nodes) return fail(EP_EINVAL, "anchor out of range");
+  if (prob_nodes(c->cfg)) {
+    std::vector<uint32_t> a(anchors, anchors + n);
+    std::sort(a.begin(), a.end());
+    if (std::adjacent_find(a.begin(), a.end()) != a.end())
+      return fail(EP_EINVAL, "max_nodes Probability: the anchors of a batch must be distinct");
+  }
   EP_HIP(hipSetDevice(c->device));
   if (!c->split_set) { EP_TRY(ensure_split(c)); EP_TRY(upload_split(c)); }
   else if (!c->d_train) EP_TRY(upload_split(c));
@@ -1014,13 +1040,37 @@ int ep_sample_batch(ep_ctx* c, const uint32_t* anchors, uint32_t n, uint32_t pas
   EP_HIP(ep::launch_sample(sc, c->d_vstate, c->stream));
   EP_HIP(hipStreamSynchronize(c->stream));
   EP_HIP(hipMemcpy(nbr_cnt, sc.nbr_cnt, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
-  EP_HIP(hipMemcpy(nbr, sc.nbr, (size_t)n * sc.S * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (sc.pnbrs) {
+    // lists at the anchors' CSR offsets, returned n x deg_max
+    const uint32_t L = hook_list_width(c);
+    std::fill(nbr, nbr + (size_t)n * L, 0u);
+    for (uint32_t i = 0; i < n; ++i)
+      if (nbr_cnt[i])
+        EP_HIP(hipMemcpy(nbr + (size_t)i * L, sc.nbr_sel + c->h_indptr[anchors[i]], nbr_cnt[i] * sizeof(uint32_t),
+                         hipMemcpyDeviceToHost));
+  } else {
+    EP_HIP(hipMemcpy(nbr, sc.nbr, (size_t)n * sc.S * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  }
   EP_HIP(hipMemcpy(neg, sc.neg, (size_t)n * sc.K * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  c->last_anchors.assign(anchors, anchors + n);
+  c->last_cnt.assign(nbr_cnt, nbr_cnt + n);
   return EP_OK;
 }
 
 int ep_sample_weights(ep_ctx* c, uint32_t n, float* w) {
   if (!c || !w || n == 0) return fail(EP_EINVAL, "bad argument");
+  if (c->base.pnbrs) {
+    if (!c->base.nbr_selw) return fail(EP_ESTATE, "no neighbour weights: needs weighted averaging or max_features");
+    if (n != c->last_anchors.size()) return fail(EP_EINVAL, "n must be the last sample_batch's size");
+    const uint32_t L = hook_list_width(c);
+    EP_HIP(hipSetDevice(c->device));
+    std::fill(w, w + (size_t)n * L, 0.f);
+    for (uint32_t i = 0; i < n; ++i)
+      if (c->last_cnt[i])
+        EP_HIP(hipMemcpy(w + (size_t)i * L, c->base.nbr_selw + c->h_indptr[c->last_anchors[i]],
+                         c->last_cnt[i] * sizeof(float), hipMemcpyDeviceToHost));
+    return EP_OK;
+  }
   if (!c->base.nbr_w) return fail(EP_ESTATE, "no neighbour weights: weighted_neighbor_averaging with max_nodes Fixed");
   if (n > c->cap_batch) return fail(EP_EINVAL, "n exceeds the sampled batch");
   EP_HIP(hipSetDevice(c->device));
@@ -1030,7 +1080,7 @@ int ep_sample_weights(ep_ctx* c, uint32_t n, float* w) {
 
 int ep_set_explicit_weights(ep_ctx* c, uint32_t n, const float* w) {
   if (!c || (!w && n)) return fail(EP_EINVAL, "bad argument");
-  c->explicit_w.assign(w, w + (size_t)n * list_cap(c->cfg));
+  c->explicit_w.assign(w, w + (size_t)n * hook_list_width(c));
   return EP_OK;
 }
 
@@ -1038,14 +1088,22 @@ int ep_step_explicit(ep_ctx* c, const uint32_t* anchors, uint32_t n, const uint3
                      const uint32_t* neg, uint32_t pass, float lr, int apply_update, float* loss, float* cons,
                      uint32_t* n_unique, uint32_t* feat, float* grad) {
   if (!c || !anchors || !nbr_cnt || !nbr || !neg || n == 0) return fail(EP_EINVAL, "bad argument");
-  if (c->cfg.max_nodes_kind != EP_SAMPLE_FIXED && !ppr_loss(c->cfg))
-    return fail(EP_EINVAL, "ep_step_explicit needs max_nodes Fixed");
+  if (c->cfg.max_nodes_kind == EP_SAMPLE_ALL && !ppr_loss(c->cfg))
+    return fail(EP_EINVAL, "ep_step_explicit needs max_nodes Fixed or Probability");
   if (!(lr >= 0.f)) return fail(EP_EINVAL, "lr must be >= 0");
   if (!c->d_indptr || !c->d_findptr) return fail(EP_ESTATE, "graph and features must be set first");
-  const uint32_t S = list_cap(c->cfg), K = c->cfg.negatives, D = row_width(c->cfg), NC = K + 2;
+  const bool pn = prob_nodes(c->cfg);
+  const uint32_t S = hook_list_width(c), K = c->cfg.negatives, D = row_width(c->cfg), NC = K + 2;
+  if (pn) {
+    std::vector<uint32_t> a(anchors, anchors + n);
+    std::sort(a.begin(), a.end());
+    if (std::adjacent_find(a.begin(), a.end()) != a.end())
+      return fail(EP_EINVAL, "max_nodes Probability: the anchors of a batch must be distinct");
+  }
   for (uint32_t i = 0; i < n; ++i) {
     if (anchors[i] >= c->n_nodes) return fail(EP_EINVAL, "anchor out of range");
     if (nbr_cnt[i] > S) return fail(EP_EINVAL, "nbr_cnt exceeds max_nodes");
+    if (pn && nbr_cnt[i] > c->degrees[anchors[i]]) return fail(EP_EINVAL, "nbr_cnt exceeds the anchor's degree");
     for (uint32_t j = 0; j < nbr_cnt[i]; ++j)
       if (nbr[(size_t)i * S + j] >= c->n_nodes) return fail(EP_EINVAL, "neighbour out of range");
     for (uint32_t k = 0; k < K; ++k)
@@ -1076,9 +1134,23 @@ int ep_step_explicit(ep_ctx* c, const uint32_t* anchors, uint32_t n, const uint3
   sc.xout = cons ? c->d_xout : nullptr;
   EP_HIP(hipMemcpy(c->d_explicit, anchors, n * sizeof(uint32_t), hipMemcpyHostToDevice));
   EP_HIP(hipMemcpy(sc.nbr_cnt, nbr_cnt, n * sizeof(uint32_t), hipMemcpyHostToDevice));
-  EP_HIP(hipMemcpy(sc.nbr, nbr, (size_t)n * S * sizeof(uint32_t), hipMemcpyHostToDevice));
+  if (pn) {
+    // the lists (and weights) go where the sampler puts them: at the anchors' CSR offsets
+    if (sc.nbr_rows && c->explicit_w.size() != (size_t)n * S)
+      return fail(EP_ESTATE, "max_nodes Probability: set n x deg_max neighbour weights first");
+    for (uint32_t i = 0; i < n; ++i) {
+      if (!nbr_cnt[i]) continue;
+      const uint64_t off = c->h_indptr[anchors[i]];
+      EP_HIP(hipMemcpy(sc.nbr_sel + off, nbr + (size_t)i * S, nbr_cnt[i] * sizeof(uint32_t), hipMemcpyHostToDevice));
+      if (sc.nbr_rows)
+        EP_HIP(hipMemcpy(sc.nbr_selw + off, c->explicit_w.data() + (size_t)i * S, nbr_cnt[i] * sizeof(float),
+                         hipMemcpyHostToDevice));
+    }
+  } else {
+    EP_HIP(hipMemcpy(sc.nbr, nbr, (size_t)n * S * sizeof(uint32_t), hipMemcpyHostToDevice));
+  }
   EP_HIP(hipMemcpy(sc.neg, neg, (size_t)n * K * sizeof(uint32_t), hipMemcpyHostToDevice));
-  if (sc.wavg) {
+  if (sc.wavg && !pn) {
     if (c->explicit_w.size() != (size_t)n * S)
       return fail(EP_ESTATE, "weighted_neighbor_averaging: set n x max_nodes neighbour weights first");
     EP_HIP(hipMemcpy(sc.nbr_w, c->explicit_w.data(), (size_t)n * S * sizeof(float), hipMemcpyHostToDevice));

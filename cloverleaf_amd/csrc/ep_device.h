@@ -52,6 +52,7 @@ enum RngPurpose : uint32_t {
   RNG_WALK = 6,
   RNG_HARD_WALK = 7,
   RNG_FEATURES = 8,  // max_features: purpose RNG_FEATURES | (entry << 4), one stream per node of an example
+  RNG_NBR_KEEP = 9,  // max_nodes Probability: one keep draw per edge of the anchor
 };
 
 // Uniform integer in [0, n) from a 32-bit draw (multiply-shift; bias <= n / 2^32).

@@ -106,6 +106,12 @@ struct StepConfig {
   float mf_p;
   uint32_t sel_cap;        // forward LDS stage: selection masks (ent_cap with max_features, else 0)
   int nbr_rows;            // ~h(v) with per-neighbour counts and adjoint rows (wavg or max_features)
+  // max_nodes Probability(mn_p) (utils.rs:203-210): the sampled list of an anchor v and
+  // the neighbours' weights live at v's CSR offset (anchors of a step are distinct)
+  int pnbrs;
+  float mn_p;
+  uint32_t* nbr_sel;       // [E]
+  float* nbr_selw;         // [E] (with nbr_rows)
   uint32_t K;              // negatives
   float gamma;
   int loss_kind;           // EP_LOSS_*: 0 margin, 1 contrastive, 2 starspace, 3 rank,

@@ -304,6 +304,12 @@ __device__ inline void sample_negatives(const StepConfig& c, uint32_t v, uint64_
     negs[k] = c.negpool[uniform_below(draw_word(pos, pass, RNG_NEGATIVES, k, k0, k1), c.n_negpool)];
 }
 
+// The neighbour list of example b (anchor row at s0): the CSR row itself (every
+// neighbour), the Probability sample at the anchor's CSR offset, or the Fixed sample.
+__device__ inline const uint32_t* nbr_list(const StepConfig& c, uint32_t b, uint64_t s0) {
+  return c.all_nbrs ? c.indices + s0 : (c.pnbrs ? c.nbr_sel + s0 : c.nbr + (size_t)b * c.S);
+}
+
 // CDFtoP (graph.rs:630-661): probability of edge j of the row starting at s0.
 __device__ inline float edge_prob(const StepConfig& c, uint64_t s0, uint32_t j) {
   return j == 0u ? c.cdf[s0] : c.cdf[s0 + j] - c.cdf[s0 + j - 1u];
@@ -328,9 +334,9 @@ __device__ inline uint32_t ares_key(const StepConfig& c, uint64_t s0, uint32_t j
 // (8-bit digits, per-wave histogram in LDS; the keys are recomputed each pass), ties
 // at the threshold taken in edge order.  Writes the neighbours to out[0..S) in edge
 // order and their weights scale * p_j * deg to w[0..S).
-__device__ inline void ares_select(const StepConfig& c, uint64_t s0, uint32_t deg, uint32_t pos, uint32_t pass,
-                                   uint32_t k0, uint32_t k1, uint32_t* hist, uint32_t* out, float* w, int lane) {
-  const uint32_t S = c.S;
+__device__ inline void ares_select(const StepConfig& c, uint64_t s0, uint32_t deg, uint32_t S, float scale,
+                                   uint32_t pos, uint32_t pass, uint32_t k0, uint32_t k1, uint32_t* hist,
+                                   uint32_t* out, float* w, int lane) {
   uint32_t prefix = 0u, pmask = 0u, need = S;
   for (int shift = 24; shift >= 0; shift -= 8) {
     for (int i = lane; i < 256; i += 64) hist[i] = 0u;
@@ -370,7 +376,6 @@ __device__ inline void ares_select(const StepConfig& c, uint64_t s0, uint32_t de
     pmask |= 0xFFu << shift;
     __builtin_amdgcn_wave_barrier();
   }
-  const float scale = (float)deg / (float)S;
   uint32_t taken = 0u, eq_seen = 0u;
   for (uint32_t j0 = 0; j0 < deg; j0 += 64u) {
     const uint32_t j = j0 + (uint32_t)lane;
@@ -384,7 +389,7 @@ __device__ inline void ares_select(const StepConfig& c, uint64_t s0, uint32_t de
     if (sel) {
       const uint32_t at = taken + (uint32_t)__popcll(ms & ((1ull << lane) - 1ull));
       out[at] = c.indices[s0 + j];
-      w[at] = scale * edge_prob(c, s0, j) * (float)deg;
+      if (w) w[at] = scale * edge_prob(c, s0, j) * (float)deg;
     }
     taken += (uint32_t)__popcll(ms);
     eq_seen += (uint32_t)__popcll(me);
@@ -488,6 +493,43 @@ __global__ __launch_bounds__(256) void k_sample(StepConfig c, StepState* st) {
     }
     sample_negatives(c, v, s0, deg, pos, pass, k0, k1, negs, lane);
     if (lane == 0) c.nbr_cnt[b] = cnt;
+  } else if (!c.explicit_samples && c.pnbrs) {
+    // Sample::Probability(p).sample(deg, true) (utils.rs:203-210) and the reservoirs of
+    // model.rs:363-377: every edge is kept with probability p (a Binomial count with a
+    // uniform subset of that size), scale 1/p; none kept: one uniform edge, scale
+    // (1 - p^deg) / p.  With weighted sampling and averaging the kept count is then
+    // re-drawn by the edge-weight reservoir.  List and weights go to v's CSR offset.
+    uint32_t* o2 = c.nbr_sel + s0;
+    float* w2 = c.nbr_rows ? c.nbr_selw + s0 : nullptr;
+    uint32_t k = 0;
+    float scale = 1.f / c.mn_p;
+    for (uint32_t j0 = 0; j0 < deg; j0 += 64u) {
+      const uint32_t j = j0 + (uint32_t)lane;
+      const bool keep = j < deg && uniform01(draw_word(pos, pass, RNG_NBR_KEEP, j, k0, k1)) < c.mn_p;
+      const uint64_t m = __ballot(keep);
+      if (keep) {
+        const uint32_t at = k + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        o2[at] = c.indices[s0 + j];
+        if (w2) w2[at] = c.wavg ? scale * edge_prob(c, s0, j) * (float)deg : scale;
+      }
+      k += (uint32_t)__popcll(m);
+    }
+    if (deg > 0u && k == 0u) {
+      const uint32_t t = uniform_below(draw_word(pos, pass, RNG_NBR_KEEP, deg, k0, k1), deg);
+      scale = (1.f - powf(c.mn_p, (float)deg)) / c.mn_p;
+      if (lane == 0) {
+        o2[0] = c.indices[s0 + t];
+        if (w2) w2[0] = c.wavg ? scale * edge_prob(c, s0, t) * (float)deg : scale;
+      }
+      k = 1u;
+    }
+    if (c.wavg && c.wsample && k < deg) {
+      __shared__ uint32_t s_hist2[NW][256];
+      ares_select(c, s0, deg, k, scale, pos, pass, k0, k1, s_hist2[threadIdx.x >> 6], o2, w2, lane);
+    }
+    cnt = k;
+    sample_negatives(c, v, s0, deg, pos, pass, k0, k1, negs, lane);
+    if (lane == 0) c.nbr_cnt[b] = cnt;
   } else if (!c.explicit_samples) {
     if (c.all_nbrs) {
       cnt = deg;
@@ -499,7 +541,8 @@ __global__ __launch_bounds__(256) void k_sample(StepConfig c, StepState* st) {
     } else if (c.wavg && c.wsample) {
       cnt = c.S;
       __shared__ uint32_t s_hist[NW][256];
-      ares_select(c, s0, deg, pos, pass, k0, k1, s_hist[threadIdx.x >> 6], out, c.nbr_w + (size_t)b * c.S, lane);
+      ares_select(c, s0, deg, c.S, (float)deg / (float)c.S, pos, pass, k0, k1, s_hist[threadIdx.x >> 6], out,
+                  c.nbr_w + (size_t)b * c.S, lane);
     } else {
       cnt = c.S;
       // Floyd: for i = 0..S-1 draw t in [0, deg-S+i]; take deg-S+i instead when t was
@@ -529,7 +572,7 @@ __global__ __launch_bounds__(256) void k_sample(StepConfig c, StepState* st) {
   }
   uint32_t R = 0;
   if (cnt > 0u) {
-    const uint32_t* pn = c.all_nbrs ? c.indices + s0 : out;
+    const uint32_t* pn = nbr_list(c, b, s0);
     const uint32_t E = 1u + cnt + c.K;  // v, neighbours, negatives
     for (uint32_t e = lane; e < E; e += 64u) {
       const uint32_t u = e == 0u ? v : (e <= cnt ? pn[e - 1u] : negs[e - 1u - cnt]);
@@ -551,7 +594,7 @@ __global__ __launch_bounds__(1024) void k_scan(StepConfig c, StepState* st) {
   __shared__ unsigned long long s_nodes;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const bool att = c.model == 1;
-  const bool nrows = c.nbr_rows && c.all_nbrs;  // per-neighbour adjoint rows at scanned offsets
+  const bool nrows = c.nbr_rows && (c.all_nbrs || c.pnbrs);  // per-neighbour rows at scanned offsets
   if (tid == 0) { s_carry = 0u; s_carry2 = 0u; s_carry3 = 0u; s_nodes = 0ull; }
   __syncthreads();
   uint64_t nodes = 0;
@@ -736,12 +779,13 @@ __device__ inline void gather_lds_wide(const T* __restrict__ table, const uint32
 // subsampled and 1 otherwise (model.rs:355-361).
 __device__ inline float nbr_weight(const StepConfig& c, uint32_t b, uint32_t j, uint64_t s0, uint32_t deg) {
   if (c.loss_kind == 5) return 1.f;
+  if (c.pnbrs) return c.nbr_selw[s0 + j];
   if (c.wavg) return c.all_nbrs ? edge_prob(c, s0, j) * (float)deg : c.nbr_w[(size_t)b * c.S + j];
   if (c.all_nbrs || deg <= c.S) return 1.f;
   return (float)deg / (float)c.S;
 }
 __device__ inline uint32_t nbr_row(const StepConfig& c, uint32_t b, uint32_t j) {
-  return c.nrow_base + (c.all_nbrs ? c.nbroff[b] : b * c.S) + j;
+  return c.nrow_base + ((c.all_nbrs || c.pnbrs) ? c.nbroff[b] : b * c.S) + j;
 }
 
 // Loss, adjoints and eligibility of one example for the losses other than margin (block
@@ -887,7 +931,7 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
   }
   const uint64_t vs = c.indptr[v];
   const uint32_t deg = (uint32_t)(c.indptr[v + 1] - vs);
-  const uint32_t* pn = c.all_nbrs ? c.indices + vs : c.nbr + (size_t)b * c.S;
+  const uint32_t* pn = nbr_list(c, b, vs);
   const int base = lane * PL;
   const uint32_t apos = st->anchor_base + b, apass = st->pass;
   const uint32_t k0 = (uint32_t)c.seed, k1 = (uint32_t)(c.seed >> 32);
@@ -1270,7 +1314,7 @@ __global__ __launch_bounds__(256) void k_emit(StepConfig c, StepState* st) {
   const bool first_leaf = c.grad_mode == 0;
   const bool att = c.model == 1;
   const uint32_t v = c.perm[st->anchor_base + b];
-  const uint32_t* pn = c.all_nbrs ? c.indices + c.indptr[v] : c.nbr + (size_t)b * c.S;
+  const uint32_t* pn = nbr_list(c, b, c.indptr[v]);
   if (tid < (int)NC) s_el[tid] = c.elig[(size_t)b * NC + tid];
   if (first_leaf)
     for (uint32_t i = tid; i < H; i += blockDim.x) hsh[i] = HASH_EMPTY;

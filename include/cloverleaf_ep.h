@@ -73,7 +73,7 @@ typedef struct ep_config {
   float alpha;                 /* learning rate, default 0.9                        */
   float gamma;                 /* MarginLoss margin, default 1.0                    */
   float valid_pct;             /* default 0.1                                       */
-  int32_t max_nodes_kind;      /* EP_SAMPLE_ALL (default) / EP_SAMPLE_FIXED         */
+  int32_t max_nodes_kind;      /* EP_SAMPLE_ALL (default) / FIXED / PROBABILITY     */
   uint32_t max_nodes;          /* S when Fixed                                      */
   int32_t max_features_kind;   /* EP_SAMPLE_*: every node of an example keeps a    */
                                /* subset of its features (model.rs:264-269); Fixed  */
@@ -120,6 +120,8 @@ typedef struct ep_config {
                                /* rankspace gamma, ppr gamma                         */
   float loss_param2;           /* contrastive negative margin / ppr restart_p        */
   float max_features_p;        /* p in (0, 1) when max_features is Probability      */
+  float max_nodes_p;           /* p in (0, 1) when max_nodes is Probability (mean   */
+                               /* model)                                            */
 } ep_config;
 
 typedef struct ep_ctx ep_ctx;

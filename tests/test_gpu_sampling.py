@@ -291,3 +291,122 @@ def test_weighted_facade_learns():
     assert emb.len() == fs.num_features() and emb.dims() == 16
     assert np.all(np.isfinite(emb.embeddings))
     assert ep.history[-1]["train"] < ep.history[0]["train"]
+
+
+# --------------------------------------------------------------------------------------
+# max_nodes = Probability(p) (utils.rs:203-210, model.rs:355-377)
+# --------------------------------------------------------------------------------------
+def prob_ctx(prob, p, K, B, cdf=None, wsample=0, mf=None, grad_mode=N.EP_GRAD_FIRST_LEAF):
+    cfg = N.default_config()
+    cfg.dims, cfg.batch_size, cfg.negatives, cfg.gamma = prob.D, B, K, 0.5
+    cfg.max_nodes_kind, cfg.max_nodes_p = N.EP_SAMPLE_PROBABILITY, p
+    cfg.grad_mode = grad_mode
+    if cdf is not None:
+        cfg.weighted_neighbor_averaging, cfg.weighted_neighbor_sampling = 1, wsample
+    if mf is not None:
+        cfg.max_features_kind, cfg.max_features, cfg.max_features_p = mf
+    cfg.valid_pct = 0.0
+    ctx = N.Context(cfg)
+    ctx.set_graph(prob.indptr, prob.indices)
+    if cdf is not None:
+        ctx.set_edge_weights(cdf)
+    ctx.set_features(prob.f_indptr, prob.f_ids, prob.V)
+    ctx.set_table(prob.table)
+    return ctx, cfg
+
+
+def test_probability_neighbours_are_binomial_uniform_subsets():
+    # 2000 anchors with 12 neighbours each (uniform edge rows); p = 0.15
+    n_anchor, deg, n_leaf = 2000, 12, 500
+    rng = np.random.default_rng(2)
+    rows = [rng.choice(np.arange(n_anchor, n_anchor + n_leaf), deg, replace=False) for _ in range(n_anchor)]
+    n = n_anchor + n_leaf
+    indptr = np.concatenate([[0], np.cumsum([deg] * n_anchor + [0] * n_leaf)]).astype(np.uint64)
+    indices = np.concatenate(rows).astype(np.uint32)
+    cdf = np.tile((np.arange(1, deg + 1) / deg).astype(np.float32), n_anchor)
+    f_indptr = np.arange(n + 1, dtype=np.uint64)
+    prob = Problem(indptr, indices, f_indptr, np.arange(n, dtype=np.uint32), n, 8, np.ones((n, 8), np.float32))
+    p = 0.15
+    ctx, _ = prob_ctx(prob, p, 2, n_anchor, cdf=cdf)
+    anchors = np.arange(n_anchor, dtype=np.uint32)
+    cnt, nbr, _ = ctx.sample_batch(anchors, 1)
+    w = ctx.sample_weights(n_anchor)
+    ctx.close()
+    from scipy.stats import binom
+    pk = binom.pmf(np.arange(deg + 1), deg, p)
+    pk[1] += pk[0]  # at least one
+    pk[0] = 0.0
+    got = np.bincount(cnt, minlength=deg + 1).astype(np.float64)
+    keep = pk * n_anchor > 5
+    chi2 = float(((got[keep] - pk[keep] * n_anchor) ** 2 / (pk[keep] * n_anchor)).sum())
+    print(f"[sampling] probability counts chi2 {chi2:.1f} ({keep.sum() - 1} dof)", flush=True)
+    assert chi2 < keep.sum() - 1 + 6 * np.sqrt(2 * (keep.sum() - 1))
+    # the kept edges are distinct neighbours of the anchor, every position equally likely
+    pos_hits = np.zeros(deg)
+    for i in range(n_anchor):
+        row = list(rows[i])
+        kept = nbr[i, :cnt[i]].tolist()
+        assert len(set(kept)) == len(kept) and set(kept) <= set(row)
+        for u in kept:
+            pos_hits[row.index(u)] += 1
+    exp = pos_hits.sum() / deg
+    assert float(((pos_hits - exp) ** 2 / exp).sum()) < 11 + 6 * np.sqrt(22)
+    # weights: scale 1/p (or (1 - p^deg)/p when none was kept) times p_j * deg = 1
+    for i in range(0, n_anchor, 97):
+        assert np.allclose(w[i, :cnt[i]], 1.0 / np.float32(p), rtol=2e-6)
+
+
+@pytest.mark.parametrize("mode", ["first_leaf", "exact"])
+def test_probability_neighbours_step_matches_oracle(mode):
+    from tests import feature_rules as R
+    from tests.problems import dense_grad, oracle_step
+    prob, cdf = weighted_problem(31)
+    gm = N.EP_GRAD_EXACT if mode == "exact" else N.EP_GRAD_FIRST_LEAF
+    for mf in (None, (R.FIXED, 3, 0.0)):
+        ctx, cfg = prob_ctx(prob, 0.3, 5, 48, cdf=cdf if mf else None, mf=mf, grad_mode=gm)
+        anchors = np.concatenate([[0, 1, 2], np.random.default_rng(4).choice(np.arange(3, prob.n), 45,
+                                                                             replace=False)]).astype(np.uint32)
+        cnt, nbr, neg = ctx.sample_batch(anchors, 1)
+        assert cnt[:3].max() > 5 and cnt[:3].min() >= 1
+        # a negative equal to an example's only neighbour makes n_k = ~h(v) up to FP32
+        # rounding: the distance gradient is then a direction of rounding noise (the FP64
+        # oracle sees an exact zero and drops it).  Keep such degenerate pairs out.
+        for i in range(len(anchors)):
+            for k in range(neg.shape[1]):
+                while cnt[i] == 1 and neg[i, k] == nbr[i, 0]:
+                    neg[i, k] = (neg[i, k] + 1) % prob.n
+        sels, weights = None, None
+        if mf:
+            weights = ctx.sample_weights(len(anchors))
+            ctx.set_explicit_weights(weights)
+            sels = [R.example_selections(R.FIXED, 3, 0.0, prob.f_indptr, cfg.seed, i, 1, int(anchors[i]),
+                                         nbr[i, :cnt[i]], neg[i]) for i in range(len(anchors))]
+        out = ctx.step_explicit(anchors, cnt, nbr, neg, pass_=1, lr=0.01, apply_update=True, want_cons=True)
+        res, merged, t64, m64, v64 = oracle_step(prob, anchors, cnt, nbr, neg, 0.5, 0.01, 1, mode,
+                                                 nbr_weights=None if weights is None else weights.astype(np.float64),
+                                                 feat_sels=sels)
+        ref_loss = np.array([r.loss if (r is not None and r.contributes) else 0.0 for r in res])
+        print(f"[sampling] probability neighbours mf={mf} {mode}: {np.count_nonzero(ref_loss)} examples", flush=True)
+        assert rel(out["loss"], ref_loss) <= 1e-5
+        for i, r in enumerate(res):
+            if r is not None:
+                assert rel(out["cons"][i], np.stack(r.vectors)) <= 1e-5
+        G64 = dense_grad(merged, prob.V, prob.D)
+        Gg = np.zeros_like(G64)
+        assert set(out["feat"].tolist()) == set(merged.keys())
+        Gg[out["feat"]] = out["grad"]
+        assert rel(Gg, G64) <= 1e-5
+        ctx.close()
+
+
+def test_probability_neighbours_learn_through_facade():
+    import cloverleaf_amd as C
+    from cloverleaf_amd.synth import make_workload
+    w = make_workload("ep-cfg1-10k-comm")
+    g = C.Graph.from_csr(w.indptr, w.indices)
+    fs = C.FeatureSet.from_csr(g, w.f_indptr, w.f_ids, w.vocab)
+    ep = C.EmbeddingPropagator(alpha=0.05, loss=C.EPLoss.margin(0.5, 5), batch_size=256, dims=32, passes=3,
+                               max_nodes=0.4, weighted_neighbor_averaging=True, indicator=False)
+    emb = ep.learn_features(g, fs)
+    assert np.all(np.isfinite(emb.embeddings))
+    assert ep.history[-1]["train"] < ep.history[0]["train"]
