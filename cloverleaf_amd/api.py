@@ -576,8 +576,9 @@ class EmbeddingPropagator:
     the reference's concurrently running batches; 1 = strictly sequential), ``device``.
     ``attention=d_k`` with ``attention_heads`` trains the attention aggregator (Full
     attention, rows of attention_heads * (2 d_k + dims) values, at most 64 features per
-    node); ``context_window`` (Sliding attention) is not supported.  Weighted neighbour
-    sampling and averaging use the graph's edge weights (mean model only).
+    node); ``context_window`` selects Sliding attention (window_size >= 1).  Weighted
+    neighbour sampling and averaging use the graph's edge weights; as in the reference,
+    the attention model ignores weighted averaging and its weighted sampling is uniform.
     """
 
     def __init__(self, alpha=None, loss=None, batch_size=None, dims=None, passes=None, seed=None, max_nodes=None,
@@ -588,14 +589,14 @@ class EmbeddingPropagator:
         loss = loss or EPLoss.margin(1.0, 1)
         mf_kind, mf, mf_p = _parse_sample(max_features, "max_features")
         mn_kind, mn, mn_p = _parse_sample(max_nodes, "max_nodes")
-        if mn_kind == N.EP_SAMPLE_PROBABILITY and attention is not None:
-            raise ValueError("max_nodes probability sampling with attention is not supported by this build")
-        if attention is not None and context_window is not None:
-            raise ValueError("context_window (Sliding attention) is not supported by this build")
-        if mf_kind != N.EP_SAMPLE_ALL and attention is not None:
-            raise ValueError("max_features sampling with attention is not supported by this build")
-        if (weighted_neighbor_sampling or weighted_neighbor_averaging) and attention is not None:
-            raise ValueError("weighted neighbour sampling/averaging with attention is not supported by this build")
+        if attention is not None and context_window is not None and int(context_window) < 1:
+            raise ValueError("context_window must be >= 1 (window_size 0 is not supported by this build)")
+        if mf_kind != N.EP_SAMPLE_ALL and attention is not None and context_window is None:
+            raise ValueError("max_features with attention (Random attention) is not supported by this build")
+        if attention is not None:
+            # lib.rs:1288-1300: the attention model takes max_features only as Random
+            # attention's size (Sliding ignores it) and never averages by weight
+            mf_kind, mf, mf_p = N.EP_SAMPLE_ALL, 0, 0.0
         cfg = N.default_config()
         cfg.alpha = 0.9 if alpha is None else float(alpha)
         cfg.batch_size = 50 if batch_size is None else int(batch_size)
@@ -627,6 +628,7 @@ class EmbeddingPropagator:
             # lib.rs:1288-1299: AttentionFeatureModel with Full attention (max_features All)
             cfg.attention_dk = int(attention)
             cfg.attention_heads = 1 if attention_heads is None else int(attention_heads)
+            cfg.context_window = 0 if context_window is None else int(context_window)
         self.cfg = cfg
         self.indicator = True if indicator is None else bool(indicator)
         self.history: List[dict] = []
@@ -744,14 +746,15 @@ class FeatureAggregator:
 
     @staticmethod
     def Attention(num_heads: int, d_k: int, window: Optional[int] = None):
-        """lib.rs:1835-1838; Sliding attention (window) is not supported by this build."""
-        if window is not None:
-            raise ValueError("window (Sliding attention) is not supported by this build")
-        return FeatureAggregator("attention", int(num_heads), int(d_k), None)
+        """lib.rs:1835-1838; window: Sliding attention with that window_size."""
+        if window is not None and int(window) < 1:
+            raise ValueError("window must be >= 1 (window_size 0 is not supported by this build)")
+        return FeatureAggregator("attention", int(num_heads), int(d_k), None if window is None else int(window))
 
     def __repr__(self):
         if self.kind == "attention":
-            return f"FeatureAggregator<Attention<num_heads={self.num_heads},d_k={self.d_k},window=None>"
+            w = "None" if self.window is None else f"Some({self.window})"
+            return f"FeatureAggregator<Attention<num_heads={self.num_heads},d_k={self.d_k},window={w}>"
         if self.kind == "weighted":
             return f"FeatureAggregator<Weighted<alpha={self.alpha}>>"
         return "FeatureAggregator<Averaged>"
@@ -790,6 +793,7 @@ class NodeEmbedder:
         if fa.kind == "attention":
             # lib.rs:2041-2046: d_model = (width - 2 H d_k) / H
             cfg.attention_heads, cfg.attention_dk = fa.num_heads, fa.d_k
+            cfg.context_window = fa.window or 0
             cfg.dims = (table.shape[1] - 2 * fa.num_heads * fa.d_k) // fa.num_heads
             if N.row_width(cfg) != table.shape[1]:
                 raise ValueError(f"embedding width {table.shape[1]} does not fit {fa.num_heads} heads of d_k={fa.d_k}")

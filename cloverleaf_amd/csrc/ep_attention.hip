@@ -116,7 +116,14 @@ struct AttLds {
   float* Q;      // [64][kTileStride]: Q_h of every item (backward B operand)
   float* K;      // [64][kTileStride]: K_h of every item
   float* acc;    // [H * att_dm] value sweep (forward; aliases Q / K)
+  uint32_t win;  // Sliding attention: entries with |i - j| <= win exist; ~0u: Full
 };
+
+// Entry (i, j) of a block of n items exists: both are items and, under Sliding
+// attention, |i - j| <= window_size (compute_sliding_attention_matrix, attention.rs:194-218).
+__device__ inline bool att_pair(uint32_t i, uint32_t j, uint32_t n, uint32_t win) {
+  return i < n && j < n && (i > j ? i - j : j - i) <= win;
+}
 
 constexpr int kMaxHeads = 16;
 
@@ -129,8 +136,9 @@ __host__ __device__ inline size_t att_lds_floats(int dm, int H, bool backward) {
   return att_lds_small(dm, H) + (backward ? 2 * (size_t)kAttMaxItems * kTileStride : (size_t)H * dm);
 }
 
-__device__ inline AttLds att_lds(float* sm, int dm, int H) {
+__device__ inline AttLds att_lds(float* sm, int dm, int H, uint32_t win) {
   AttLds L;
+  L.win = win;
   L.M = sm;
   L.iZ = L.M + kAttMaxItems;
   L.r = L.iZ + kAttMaxItems;
@@ -182,12 +190,15 @@ __device__ inline bool att_setup(const StepConfig& c, const StepState* st, uint3
     mult += eq ? 1u : 0u;
     first = first && !(eq && t < lane);
   }
-  if (B.neighbour) {
+  const bool sliding = c.att_win != 0xFFFFFFFFu;
+  if (B.neighbour || sliding) {
+    // listed features in order (a repeated feature is repeated); Sliding attention does
+    // not scale scores by counts, and count 0 gives the factor 1
     B.n = nraw;
     if (lane >= nraw) L.cnt[lane] = 0.f;  // padded items
     if (lane < nraw) {
       L.f[lane] = fq;
-      L.cnt[lane] = (float)mult;
+      L.cnt[lane] = sliding ? 0.f : (float)mult;
       L.q[lane] = lane;
     }
   } else {
@@ -252,7 +263,7 @@ __device__ inline void row_softmax_stats(f32x16 (&T)[2], int NT, uint32_t n, int
     for (int r = 0; r < 16; ++r) {
       const uint32_t j = 32u * J + mrow(r, l);
       float s = T[J][r] * count_factor(ci, L.cnt[j]) * isq;
-      s = (j < n && i < n) ? s : -INFINITY;
+      s = att_pair(i, j, n, L.win) ? s : -INFINITY;
       T[J][r] = s;
       mx = fmaxf(mx, s);
     }
@@ -292,7 +303,7 @@ __global__ __launch_bounds__(256) void k_att_map(StepConfig c, StepState* st) {
   const uint32_t g0 = c.blkoff[b];
   if (g0 + E > c.blkoff[c.batch]) return;  // over capacity: the step reports an error
   const uint32_t v = c.perm[st->anchor_base + b];
-  const uint32_t* pn = c.all_nbrs ? c.indices + c.indptr[v] : c.nbr + (size_t)b * c.S;
+  const uint32_t* pn = nbr_list(c, b, c.indptr[v]);
   uint32_t carry = c.keyoff[b];
   for (uint32_t j0 = 0; j0 < E; j0 += 64u) {
     const uint32_t j = j0 + lane;
@@ -385,7 +396,7 @@ __device__ inline void att_block_output(const T* __restrict__ table, uint32_t W,
         for (int r = 0; r < 16; ++r) {
           const uint32_t i = 32u * I + mrow(r, l);
           const float sc = a[r] * count_factor(L.cnt[i], cj) * isq;
-          wj += (i < n && j < n) ? expf(sc - L.M[i]) * L.iZ[i] : 0.f;
+          wj += att_pair(i, j, n, L.win) ? expf(sc - L.M[i]) * L.iZ[i] : 0.f;
         }
       }
       wj += __shfl_xor(wj, 32, EP_WAVE);
@@ -444,7 +455,7 @@ template <int DK, typename T>
 __global__ __launch_bounds__(64) void k_att_forward(StepConfig c, StepState* st) {
   extern __shared__ float sm[];
   const int H = c.att_heads, dm = c.att_dm;
-  const AttLds L = att_lds(sm, dm, H);
+  const AttLds L = att_lds(sm, dm, H, c.att_win);
   AttBlock B;
   if (!att_setup(c, st, blockIdx.x, L, B)) return;
   if (lane_id() == 0) c.bitems[blockIdx.x] = B.n;
@@ -460,9 +471,9 @@ template <int DK, typename T>
 __global__ __launch_bounds__(64) void k_att_embed(const T* __restrict__ table, uint32_t W, int H, int dm,
                                                   const uint64_t* __restrict__ findptr,
                                                   const uint32_t* __restrict__ fids, uint64_t n_nodes,
-                                                  float* __restrict__ out) {
+                                                  uint32_t win, float* __restrict__ out) {
   extern __shared__ float sm[];
-  const AttLds L = att_lds(sm, dm, H);
+  const AttLds L = att_lds(sm, dm, H, win);
   const uint32_t lane = lane_id();
   for (uint64_t node = blockIdx.x; node < n_nodes; node += gridDim.x) {
     const uint64_t fs = findptr[node];
@@ -775,7 +786,7 @@ __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st
   extern __shared__ float sm[];
   const int H = c.att_heads, dm = c.att_dm;
   const uint32_t W = (uint32_t)c.dims;
-  const AttLds L = att_lds(sm, dm, H);
+  const AttLds L = att_lds(sm, dm, H, c.att_win);
   if (st->n_keys > c.key_cap) return;
   AttBlock B;
   if (!att_setup(c, st, blockIdx.x, L, B)) return;
@@ -912,7 +923,7 @@ __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st
         for (int r = 0; r < 16; ++r) {
           const uint32_t i = 32u * I + mrow(r, l);
           const float f = count_factor(L.cnt[i], cj) * isq;
-          const float A = (i < n && j < n) ? expf(a[r] * f - L.M[i]) * L.iZ[i] : 0.f;
+          const float A = att_pair(i, j, n, L.win) ? expf(a[r] * f - L.M[i]) * L.iZ[i] : 0.f;
           wj += A;
           a[r] = A * (uj - L.r[i]) * f;  // dT_ij
         }
@@ -973,10 +984,10 @@ hipError_t launch_att_embed(const StepConfig& c, const uint64_t* findptr, const 
   EP_ATT_DISPATCH(c.att_dk, {
     if (c.bf16)
       hipLaunchKernelGGL((k_att_embed<DK, uint16_t>), dim3(grid), dim3(64), lds, s, (const uint16_t*)c.table,
-                         (uint32_t)c.dims, c.att_heads, c.att_dm, findptr, fids, n_nodes, out);
+                         (uint32_t)c.dims, c.att_heads, c.att_dm, findptr, fids, n_nodes, c.att_win, out);
     else
       hipLaunchKernelGGL((k_att_embed<DK, float>), dim3(grid), dim3(64), lds, s, (const float*)c.table,
-                         (uint32_t)c.dims, c.att_heads, c.att_dm, findptr, fids, n_nodes, out);
+                         (uint32_t)c.dims, c.att_heads, c.att_dm, findptr, fids, n_nodes, c.att_win, out);
   });
   return hipGetLastError();
 }

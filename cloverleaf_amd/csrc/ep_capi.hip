@@ -171,7 +171,11 @@ uint32_t neighbour_cap(const ep_ctx* c) {
 }
 
 // weighted_neighbor_averaging applies to ~h(v) from sampled neighbours (not PPR walks).
-bool weighted_avg(const ep_config& cfg) { return cfg.weighted_neighbor_averaging && !ppr_loss(cfg); }
+// The attention model does not average by weight (model.rs:214-218 passes false), and its
+// weighted sampling draws over equal weights, which is a uniform sample.
+bool weighted_avg(const ep_config& cfg) {
+  return cfg.weighted_neighbor_averaging && !ppr_loss(cfg) && !cfg.attention_dk;
+}
 
 bool feature_sampling(const ep_config& cfg) { return cfg.max_features_kind != EP_SAMPLE_ALL; }
 
@@ -396,13 +400,14 @@ void refresh_base(ep_ctx* c) {
   b.model = cfg.attention_dk ? 1 : 0;
   b.att_heads = (int)cfg.attention_heads;
   b.att_dk = (int)cfg.attention_dk;
+  b.att_win = cfg.context_window ? cfg.context_window : 0xFFFFFFFFu;  // Sliding window_size, else Full
   b.att_dm = (int)cfg.dims;
   b.K = cfg.negatives;
   b.gamma = cfg.gamma;
   b.loss_kind = cfg.loss_kind;
   b.loss_p2 = cfg.loss_param2;
   b.wavg = weighted_avg(cfg) ? 1 : 0;
-  b.wsample = cfg.weighted_neighbor_sampling ? 1 : 0;
+  b.wsample = (cfg.weighted_neighbor_sampling && !cfg.attention_dk) ? 1 : 0;
   b.cdf = c->d_cdf;
   b.mf_kind = cfg.max_features_kind;
   b.mf_k = cfg.max_features;
@@ -627,7 +632,6 @@ int validate_config(const ep_config* cfg) {
   if (cfg->attention_dk == 0) {
     if (!ep::dims_supported((int)cfg->dims)) return fail(EP_EINVAL, "dims must be one of 8,16,32,64,100,128,256");
   } else {
-    if (cfg->context_window != 0) return fail(EP_EINVAL, "context_window (Sliding attention) is not supported");
     if (!ep::att_shape_supported((int)cfg->attention_heads, (int)cfg->attention_dk, (int)cfg->dims))
       return fail(EP_EINVAL, "attention: d_k must be 4, 8, 16 or 32, heads 1..16, dims a multiple of 8 in [8, 256]");
     if (!ep::row_dims_supported((int)row_width(*cfg)))
@@ -645,8 +649,6 @@ int validate_config(const ep_config* cfg) {
     return fail(EP_EINVAL, "max_nodes must be All, Fixed or Probability");
   if (cfg->max_nodes_kind == EP_SAMPLE_PROBABILITY && !(cfg->max_nodes_p > 0.f && cfg->max_nodes_p < 1.f))
     return fail(EP_EINVAL, "max_nodes (Probability) must be in (0, 1)");
-  if (cfg->max_nodes_kind == EP_SAMPLE_PROBABILITY && cfg->attention_dk)
-    return fail(EP_EINVAL, "max_nodes Probability is not supported with attention");
   if (cfg->max_features_kind != EP_SAMPLE_ALL && cfg->max_features_kind != EP_SAMPLE_FIXED &&
       cfg->max_features_kind != EP_SAMPLE_PROBABILITY)
     return fail(EP_EINVAL, "max_features must be All, Fixed or Probability");
@@ -662,8 +664,6 @@ int validate_config(const ep_config* cfg) {
   if (cfg->loss_kind == EP_LOSS_PPR && !(cfg->loss_param2 >= 0.f && cfg->loss_param2 <= 1.f))
     return fail(EP_EINVAL, "ppr restart_p must be in [0, 1]");
   if (!(cfg->noise >= 0.f) || !std::isfinite(cfg->noise)) return fail(EP_EINVAL, "noise must be finite and >= 0");
-  if ((cfg->weighted_neighbor_sampling || cfg->weighted_neighbor_averaging) && cfg->attention_dk)
-    return fail(EP_EINVAL, "weighted neighbour sampling/averaging is not supported with attention");
   if (cfg->precision != EP_PREC_FP32 && cfg->precision != EP_PREC_BF16 && cfg->precision != EP_PREC_BF16_MASTER)
     return fail(EP_EINVAL, "precision");
   if (cfg->grad_mode != EP_GRAD_FIRST_LEAF && cfg->grad_mode != EP_GRAD_EXACT) return fail(EP_EINVAL, "grad_mode");

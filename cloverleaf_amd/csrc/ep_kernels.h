@@ -195,6 +195,7 @@ struct StepConfig {
   // feature occurrence: adj[slot * dims] with slot = the occurrence's key position.
   int model;               // 0 mean, 1 attention
   int att_heads, att_dk, att_dm;
+  uint32_t att_win;        // Sliding attention window_size (attention.rs:194-218); ~0u: Full
   uint32_t* blkoff;        // [batch + 1]
   uint32_t* bitems;        // [blk_cap] items of each block (written by the forward)
   uint32_t* bnode;         // [blk_cap] node of each block        (written by k_att_map)
@@ -204,6 +205,13 @@ struct StepConfig {
   float* badj;             // [blk_cap * att_dm]
   uint32_t blk_cap;
 };
+
+// The neighbour list of example b (anchor row at s0): the CSR row itself (every
+// neighbour), the Probability sample at the anchor's CSR offset, or the Fixed sample.
+__device__ inline const uint32_t* nbr_list(const StepConfig& c, uint32_t b, uint64_t s0) {
+  return c.all_nbrs ? c.indices + s0 : (c.pnbrs ? c.nbr_sel + s0 : c.nbr + (size_t)b * c.S);
+}
+
 
 // Launchers (all asynchronous on `stream`).
 hipError_t launch_step_begin(const StepConfig& c, StepState* st, hipStream_t s);

@@ -304,12 +304,6 @@ __device__ inline void sample_negatives(const StepConfig& c, uint32_t v, uint64_
     negs[k] = c.negpool[uniform_below(draw_word(pos, pass, RNG_NEGATIVES, k, k0, k1), c.n_negpool)];
 }
 
-// The neighbour list of example b (anchor row at s0): the CSR row itself (every
-// neighbour), the Probability sample at the anchor's CSR offset, or the Fixed sample.
-__device__ inline const uint32_t* nbr_list(const StepConfig& c, uint32_t b, uint64_t s0) {
-  return c.all_nbrs ? c.indices + s0 : (c.pnbrs ? c.nbr_sel + s0 : c.nbr + (size_t)b * c.S);
-}
-
 // CDFtoP (graph.rs:630-661): probability of edge j of the row starting at s0.
 __device__ inline float edge_prob(const StepConfig& c, uint64_t s0, uint32_t j) {
   return j == 0u ? c.cdf[s0] : c.cdf[s0 + j] - c.cdf[s0 + j - 1u];
@@ -1366,9 +1360,10 @@ __global__ __launch_bounds__(256) void k_emit(StepConfig c, StepState* st) {
         } else {
           bool emit = ok && (!first_leaf || hash_lookup(hsh, hmask, f) == cj);
           const uint32_t pos = kbase + running + o;
-          if (att && emit && cj != 1u) {
-            // h(v), h(u_k): one attention item (and gradient row) per distinct feature,
-            // at the feature's first position in the node's list
+          if (att && emit && cj != 1u && c.att_win == 0xFFFFFFFFu) {
+            // h(v), h(u_k) under Full attention: one attention item (and gradient row)
+            // per distinct feature, at the feature's first position in the node's list
+            // (Sliding attention keeps every listed occurrence)
             for (uint32_t t = 0; t < qpos && emit; ++t) emit = c.fids[nfs + t] != f;
           }
           c.keys[0][pos] = emit ? key_of_row(c, f) : c.sentinel;

@@ -112,8 +112,8 @@ typedef struct ep_config {
                                /* at most 64 features per node                       */
   uint32_t attention_heads;    /* `attention_heads` (default 1); table rows are      */
                                /* heads * (2 d_k + dims) wide (model.rs:241-243)     */
-  uint32_t context_window;     /* `context_window`: must be 0 (Sliding attention is  */
-                               /* not built)                                         */
+  uint32_t context_window;     /* `context_window` k >= 1: Sliding attention with    */
+                               /* window_size k (attention.rs:194-218); 0: Full      */
   int32_t loss_kind;           /* EP_LOSS_* (EPLoss, lib.rs:939-1110; loss.rs:15-38); */
                                /* gamma is the first parameter: margin / starspace   */
                                /* gamma, contrastive positive margin, rank tau,      */

@@ -537,8 +537,10 @@ def _example(table, f_indptr, f_ids, v, nbrs, nbr_scale, negs, gamma, grad_mode,
              feat_sel=None):
     if attention is None:
         return example_grads(table, f_indptr, f_ids, v, nbrs, nbr_scale, negs, gamma, grad_mode, div, loss, feat_sel)
-    heads, d_k = attention
-    return attention_example_grads(table, f_indptr, f_ids, v, nbrs, negs, gamma, heads, d_k, grad_mode, div, loss)
+    heads, d_k = attention[:2]
+    window = attention[2] if len(attention) > 2 else None  # (heads, d_k, window): Sliding attention
+    return attention_example_grads(table, f_indptr, f_ids, v, nbrs, negs, gamma, heads, d_k, grad_mode, div, loss,
+                                   window)
 
 
 def ep_batch_step(table, mom, var, f_indptr, f_ids, anchors, nbr_lists, nbr_scales, neg_lists,
@@ -637,8 +639,40 @@ def attention_matrix_full(Q: np.ndarray, K: np.ndarray, counts: np.ndarray) -> n
     return np.where(num >= 1.0, S * num, S)
 
 
-def attention_mean(rows: np.ndarray, counts: np.ndarray, heads: int, d_k: int) -> np.ndarray:
-    """attention_mean with Full attention (attention.rs:100-131), float64.
+def attention_window_mask(n: int, window: Optional[int]) -> np.ndarray:
+    """Entries (i, j) that exist in the attention matrix: all of them (Full), or
+    |i - j| <= window_size (compute_sliding_attention_matrix, attention.rs:194-218)."""
+    if window is None:
+        return np.ones((n, n), dtype=bool)
+    idx = np.arange(n)
+    return np.abs(idx[:, None] - idx[None, :]) <= int(window)
+
+
+def attention_scores(Q: np.ndarray, K: np.ndarray, counts: np.ndarray, window: Optional[int]):
+    """Scores before the 1/sqrt(d_k): Full scales by the counts (attention.rs:171-191),
+    Sliding does not (194-218).  Returns (S, mask, m) with m the score scale."""
+    n = Q.shape[0]
+    if window is None:
+        num = np.outer(counts, counts)
+        m = np.where(num >= 1.0, num, 1.0)
+    else:
+        m = np.ones((n, n))
+    return (Q @ K.T) * m, attention_window_mask(n, window), m
+
+
+def masked_softmax_rows(S: np.ndarray, mask: np.ndarray) -> np.ndarray:
+    """compute_attention_softmax (attention.rs:253-280): each row's softmax over its
+    existing entries; missing entries are 0."""
+    A = np.zeros_like(S)
+    for i in range(S.shape[0]):
+        A[i, mask[i]] = softmax_exact(S[i, mask[i]])
+    return A
+
+
+def attention_mean(rows: np.ndarray, counts: np.ndarray, heads: int, d_k: int,
+                   window: Optional[int] = None) -> np.ndarray:
+    """attention_mean (attention.rs:100-131), float64; Full attention, or Sliding with
+    window_size ``window``.
 
     rows: n x W feature rows, counts: n multiplicities.  For n == 1 the value
     vector of head 0 is returned (attention.rs:113-115).  Otherwise, per head,
@@ -652,14 +686,14 @@ def attention_mean(rows: np.ndarray, counts: np.ndarray, heads: int, d_k: int) -
         return V[0][0].copy()
     out = np.zeros(d_model)
     for h in range(heads):
-        S = attention_matrix_full(Q[h], K[h], np.asarray(counts, dtype=np.float64)) / math.sqrt(d_k)
-        A = np.stack([softmax_exact(S[i]) for i in range(n)])
+        S, mask, _ = attention_scores(Q[h], K[h], np.asarray(counts, dtype=np.float64), window)
+        A = masked_softmax_rows(S / math.sqrt(d_k), mask)
         out += (A.sum(axis=0) @ V[h]) / n
     return out / heads
 
 
 def attention_mean_backward(rows: np.ndarray, counts: np.ndarray, heads: int, d_k: int,
-                            g_out: np.ndarray) -> np.ndarray:
+                            g_out: np.ndarray, window: Optional[int] = None) -> np.ndarray:
     """Adjoint of ``attention_mean`` w.r.t. the feature rows (float64).
 
     With w_j = sum_i A_ij:  dV_j = w_j g / (n H);
@@ -676,11 +710,9 @@ def attention_mean_backward(rows: np.ndarray, counts: np.ndarray, heads: int, d_
     if n == 1:
         grad[0, base:base + d_model] = g_out
         return grad
-    num = np.outer(counts, counts)
-    m = np.where(num >= 1.0, num, 1.0)
     for h in range(heads):
-        S = (Q[h] @ K[h].T) * m / math.sqrt(d_k)
-        A = np.stack([softmax_exact(S[i]) for i in range(n)])
+        S, mask, m = attention_scores(Q[h], K[h], counts, window)
+        A = masked_softmax_rows(S / math.sqrt(d_k), mask)
         w = A.sum(axis=0)
         scale = 1.0 / (n * heads)
         grad[:, base + h * d_model: base + (h + 1) * d_model] += np.outer(w, g_out) * scale
@@ -692,18 +724,21 @@ def attention_mean_backward(rows: np.ndarray, counts: np.ndarray, heads: int, d_
     return grad
 
 
-def attention_node_items(f_indptr, f_ids, node):
+def attention_node_items(f_indptr, f_ids, node, sliding: bool = False):
     """Items of h(node) under the attention model.
 
     attention_construct_node_embedding (model.rs:306-335) collects the node's features
     with weight 1 (each occurrence adds 1 + 1, model.rs:267-268) into a map and runs
     attention over the map's entries: one item per distinct feature, count = 2 x its
     multiplicity.  Items are in first-seen order; Full attention is permutation
-    invariant, so the map's iteration order does not change the result.
+    invariant, so the map's iteration order does not change the result.  Sliding
+    attention keeps the feature order (model.rs:320-329): one item per listed
+    occurrence (a repeated feature is repeated), each reading its feature's entry.
     """
     con = Construction()
-    collect_node(con, node_features(f_indptr, f_ids, node), 1.0)
-    feats = list(con.counts.keys())
+    feats_listed = node_features(f_indptr, f_ids, node)
+    collect_node(con, feats_listed, 1.0)
+    feats = [int(f) for f in feats_listed] if sliding else list(con.counts.keys())
     return feats, [con.counts[f] for f in feats]
 
 
@@ -732,7 +767,7 @@ def _scatter_rows(out: Dict[int, np.ndarray], feats, rows: np.ndarray):
 
 
 def attention_example_grads(table, f_indptr, f_ids, anchor, nbrs, negs, gamma, heads, d_k,
-                            grad_mode="first_leaf", loss_div=1.0, loss=None) -> ExampleResult:
+                            grad_mode="first_leaf", loss_div=1.0, loss=None, window=None) -> ExampleResult:
     """Forward + adjoint + gradient extraction for one anchor under the attention model.
 
     AttentionFeatureModel (model.rs:158-249):
@@ -746,17 +781,18 @@ def attention_example_grads(table, f_indptr, f_ids, anchor, nbrs, negs, gamma, h
     feature in every neighbour reads the same leaf, so the leaf's gradient sums them.
     Leaf gradients are full table rows [Q | K | V] of width heads * (2 d_k + d_model).
     The first_leaf / exact rules and the eligibility of constructions are those of
-    example_grads.
+    example_grads.  window: Sliding attention with that window_size (None: Full).
     """
     table = np.asarray(table)
     W = table.shape[1]
-    node_sets = [attention_node_items(f_indptr, f_ids, anchor)]
+    sl = window is not None
+    node_sets = [attention_node_items(f_indptr, f_ids, anchor, sl)]
     nb_sets = [attention_neighbour_items(f_indptr, f_ids, u) for u in nbrs]
-    neg_sets = [attention_node_items(f_indptr, f_ids, u) for u in negs]
+    neg_sets = [attention_node_items(f_indptr, f_ids, u, sl) for u in negs]
 
     def fwd(items):
         feats, counts = items
-        return attention_mean(table[np.asarray(feats, dtype=np.int64)], np.asarray(counts), heads, d_k)
+        return attention_mean(table[np.asarray(feats, dtype=np.int64)], np.asarray(counts), heads, d_k, window)
 
     a = fwd(node_sets[0])
     p = sum(fwd(it) for it in nb_sets) / float(len(nb_sets))
@@ -773,7 +809,7 @@ def attention_example_grads(table, f_indptr, f_ids, anchor, nbrs, negs, gamma, h
     def bwd(items, g):
         feats, counts = items
         rows = table[np.asarray(feats, dtype=np.int64)]
-        return attention_mean_backward(rows, np.asarray(counts), heads, d_k, g)
+        return attention_mean_backward(rows, np.asarray(counts), heads, d_k, g, window)
 
     # leaf gradients per construction, in construction order a, p, n_1..n_K
     leaves: List[Optional[Dict[int, np.ndarray]]] = []
@@ -806,11 +842,12 @@ def attention_example_grads(table, f_indptr, f_ids, anchor, nbrs, negs, gamma, h
     return ExampleResult(L, True, grads, vecs, adjs)
 
 
-def attention_embed(table, feats, heads, d_k) -> np.ndarray:
-    """AttentionAggregator::construct (aggregator.rs:137-153): Full attention over the
-    listed features with count 1 each (float64)."""
+def attention_embed(table, feats, heads, d_k, window=None) -> np.ndarray:
+    """AttentionAggregator::construct (aggregator.rs:137-153): attention over the listed
+    features with count 1 each (float64); Full, or Sliding with ``window``."""
     items = attention_inference_items(feats)
-    return attention_mean(np.asarray(table)[np.asarray(items[0], dtype=np.int64)], np.asarray(items[1]), heads, d_k)
+    return attention_mean(np.asarray(table)[np.asarray(items[0], dtype=np.int64)], np.asarray(items[1]), heads, d_k,
+                          window)
 
 
 # --------------------------------------------------------------------------------------

@@ -27,11 +27,14 @@ def att_problem(seed, heads, dk, dm, n=140, V=120, fmax=10, zipf=1.2, avg_deg=8)
 
 
 def att_ctx(prob, heads, dk, dm, S=5, K=4, gamma=0.5, B=32, grad_mode=N.EP_GRAD_FIRST_LEAF, precision=N.EP_PREC_FP32,
-            concurrent=1, seg_tile=(0, 0), use_graph=0):
+            concurrent=1, seg_tile=(0, 0), use_graph=0, window=0, max_nodes_p=None):
     cfg = N.default_config()
     cfg.dims, cfg.batch_size, cfg.negatives, cfg.gamma = dm, B, K, gamma
     cfg.attention_dk, cfg.attention_heads = dk, heads
+    cfg.context_window = window
     cfg.max_nodes_kind, cfg.max_nodes = N.EP_SAMPLE_FIXED, S
+    if max_nodes_p is not None:
+        cfg.max_nodes_kind, cfg.max_nodes_p = N.EP_SAMPLE_PROBABILITY, max_nodes_p
     cfg.grad_mode, cfg.precision = grad_mode, precision
     cfg.concurrent_batches = concurrent
     cfg.seg_vpl, cfg.seg_unroll = seg_tile
@@ -258,3 +261,94 @@ def test_node_embedder_attention_facade():
                     for i in range(prob.n)])
     assert node.embeddings.shape == (prob.n, dm)
     assert rel(node.embeddings, ref) <= TOL
+
+
+
+# --------------------------------------------------------------------------------------
+# Sliding attention (attention.rs:194-218, model.rs:320-329) and Probability neighbours
+# --------------------------------------------------------------------------------------
+@pytest.mark.parametrize("heads,dk,dm,window", [(2, 8, 16, 1), (1, 8, 16, 3), (4, 32, 128, 2), (2, 16, 32, 50)])
+@pytest.mark.parametrize("grad_mode", [N.EP_GRAD_FIRST_LEAF, N.EP_GRAD_EXACT])
+def test_sliding_attention_step_matches_oracle(heads, dk, dm, window, grad_mode):
+    prob = att_problem(seed=heads * 100 + dk + dm + window, heads=heads, dk=dk, dm=dm)
+    assert has_repeats(prob)
+    print(f"[attention] sliding window={window} H={heads} dk={dk} dm={dm} mode={grad_mode}", flush=True)
+    ctx = att_ctx(prob, heads, dk, dm, 5, 4, 0.5, B=48, grad_mode=grad_mode, window=window)
+    anchors = np.random.default_rng(window).choice(prob.n, size=48, replace=False).astype(np.uint32)
+    cnt, nbr, neg = ctx.sample_batch(anchors, 1)
+    out = ctx.step_explicit(anchors, cnt, nbr, neg, pass_=1, lr=0.01, apply_update=True, want_cons=True)
+    mode = "first_leaf" if grad_mode == N.EP_GRAD_FIRST_LEAF else "exact"
+    res, merged, t64, m64, v64 = oracle_step(prob, anchors, cnt, nbr, neg, 0.5, 0.01, 1, mode,
+                                             attention=(heads, dk, window))
+    ref_loss = np.array([r.loss if (r is not None and r.contributes) else 0.0 for r in res])
+    assert np.count_nonzero(ref_loss) > 10
+    assert rel(out["loss"], ref_loss) <= TOL
+    for i, r in enumerate(res):
+        if r is not None:
+            assert rel(out["cons"][i], np.stack(r.vectors)) <= TOL
+    G64 = dense_grad(merged, prob.V, prob.D)
+    Gg = np.zeros_like(G64)
+    assert set(out["feat"].tolist()) == set(merged.keys())
+    Gg[out["feat"]] = out["grad"]
+    assert rel(Gg, G64) <= TOL
+    m, v = ctx.get_adam()
+    rows = sorted(merged)
+    assert rel(m[rows], m64[rows]) <= TOL
+    ctx.close()
+
+
+@pytest.mark.parametrize("window", [1, 4])
+def test_sliding_attention_embed_matches_oracle(window):
+    from oracle import ep_oracle as O
+    heads, dk, dm = 2, 8, 16
+    prob = att_problem(seed=70 + window, heads=heads, dk=dk, dm=dm, n=300, V=100, fmax=64)
+    ctx = att_ctx(prob, heads, dk, dm, window=window)
+    out = ctx.embed_attention(prob.f_indptr, prob.f_ids)
+    ref = np.stack([O.attention_embed(prob.table, prob.f_ids[prob.f_indptr[i]:prob.f_indptr[i + 1]], heads, dk,
+                                      window) for i in range(prob.n)])
+    assert rel(out, ref) <= TOL
+    ctx.close()
+
+
+def test_attention_probability_neighbours_match_oracle():
+    heads, dk, dm = 2, 8, 16
+    prob = att_problem(seed=81, heads=heads, dk=dk, dm=dm)
+    ctx = att_ctx(prob, heads, dk, dm, K=4, B=48, max_nodes_p=0.5)
+    anchors = np.random.default_rng(81).choice(prob.n, size=48, replace=False).astype(np.uint32)
+    cnt, nbr, neg = ctx.sample_batch(anchors, 1)
+    deg = np.diff(prob.indptr.astype(np.int64))[anchors]
+    assert np.all(cnt <= np.maximum(deg, 1)) and np.any(cnt < deg)
+    for i in range(len(anchors)):  # keep an example's only neighbour out of its negatives
+        for k in range(neg.shape[1]):
+            while cnt[i] == 1 and neg[i, k] == nbr[i, 0]:
+                neg[i, k] = (neg[i, k] + 1) % prob.n
+    out = ctx.step_explicit(anchors, cnt, nbr, neg, pass_=1, lr=0.01, apply_update=True, want_cons=True)
+    res, merged, *_ = oracle_step(prob, anchors, cnt, nbr, neg, 0.5, 0.01, 1, "first_leaf", attention=(heads, dk))
+    ref_loss = np.array([r.loss if (r is not None and r.contributes) else 0.0 for r in res])
+    assert rel(out["loss"], ref_loss) <= TOL
+    G64 = dense_grad(merged, prob.V, prob.D)
+    Gg = np.zeros_like(G64)
+    Gg[out["feat"]] = out["grad"]
+    assert rel(Gg, G64) <= TOL
+    ctx.close()
+
+
+def test_sliding_attention_facade_learns_and_embeds():
+    import cloverleaf_amd as C
+    heads, dk, dm = 2, 8, 16
+    prob = att_problem(seed=91, heads=heads, dk=dk, dm=dm, n=200, V=80)
+    g = C.Graph.from_csr(prob.indptr, prob.indices)
+    fs = C.FeatureSet.from_csr(g, prob.f_indptr, prob.f_ids, prob.V)
+    ep = C.EmbeddingPropagator(alpha=0.05, loss=C.EPLoss.margin(0.5, 4), batch_size=32, dims=dm, passes=3,
+                               max_nodes=5, attention=dk, attention_heads=heads, context_window=2,
+                               weighted_neighbor_sampling=True, indicator=False)
+    emb = ep.learn_features(g, fs)
+    assert emb.dims() == heads * (2 * dk + dm) and np.all(np.isfinite(emb.embeddings))
+    assert ep.history[-1]["train"] < ep.history[0]["train"]
+    agg = C.FeatureAggregator.Attention(heads, dk, 2)
+    assert repr(agg).endswith("window=Some(2)>")
+    ne = C.NodeEmbedder(agg).embed_feature_set(fs, emb)
+    from oracle import ep_oracle as O
+    ref = np.stack([O.attention_embed(emb.embeddings, prob.f_ids[prob.f_indptr[i]:prob.f_indptr[i + 1]], heads, dk, 2)
+                    for i in range(prob.n)])
+    assert rel(ne.embeddings, ref) <= TOL

@@ -51,7 +51,6 @@ def test_invalid_config_rejected(field, value):
 
 @pytest.mark.parametrize("fields", [
     dict(attention_dk=5, attention_heads=4, dims=128),            # d_k not built
-    dict(attention_dk=32, attention_heads=4, dims=128, context_window=3),  # Sliding attention
     dict(attention_dk=8, attention_heads=2, dims=12),             # d_model not a multiple of 8
     dict(attention_dk=8, attention_heads=0, dims=16),             # no heads
     dict(attention_dk=8, attention_heads=3, dims=16),             # row width 96 not built

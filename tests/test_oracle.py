@@ -945,3 +945,70 @@ def test_feature_selection_exact_grads_fd(seed):
             tm[fr, j] -= eps
             g = r.grads[fr][j] if fr in r.grads else 0.0
             assert abs((loss(tp) - loss(tm)) / (2 * eps) - g) < 1e-6
+
+
+# --------------------------------------------------------------------------------------
+# Sliding attention (attention.rs:194-218; model.rs:320-329)
+# --------------------------------------------------------------------------------------
+def test_sliding_attention_matrix_kat():
+    # attention.rs test_attention_matrix_cw: window 1 drops (0, 2) and (2, 0); window 10 is full
+    Q, K, V, dm = O.attention_split(ATT_ROWS, heads=1, d_k=1)
+    S, mask, _ = O.attention_scores(Q[0], K[0], np.ones(3), 1)
+    assert mask.tolist() == [[True, True, False], [True, True, True], [False, True, True]]
+    assert np.where(mask, S, 0.0).tolist() == [[1.0, -0.0, 0.0], [-0.0, 0.0, 0.0], [0.0, 0.0, 1.0]]
+    S10, mask10, _ = O.attention_scores(Q[0], K[0], np.ones(3), 10)
+    assert mask10.all() and np.array_equal(S10, O.attention_matrix_full(Q[0], K[0], np.ones(3)))
+
+
+def test_sliding_attention_ignores_counts_and_softmaxes_the_window():
+    Q, K, V, dm = O.attention_split(ATT_ROWS, heads=1, d_k=1)
+    counts = np.array([2.0, 3.0, 2.0])
+    S, mask, m = O.attention_scores(Q[0], K[0], counts, 1)
+    assert np.all(m == 1.0)  # no count scaling (attention.rs:194-218)
+    A = O.masked_softmax_rows(S, mask)
+    assert A[0, 2] == 0 and A[2, 0] == 0 and np.allclose(A.sum(axis=1), 1.0)
+    assert np.allclose(A[0, :2], O.softmax_exact(np.array([1.0, 0.0])))
+    # a window covering every pair equals Full attention with counts below 1
+    assert np.allclose(O.attention_mean(ATT_ROWS, np.full(3, 0.5), 1, 1, window=5),
+                       O.attention_mean(ATT_ROWS, np.full(3, 0.5), 1, 1))
+
+
+@pytest.mark.parametrize("window", [0, 1, 2])
+def test_sliding_attention_backward_vs_fd(window):
+    rng = np.random.default_rng(window + 3)
+    heads, dk, dm, n = 2, 2, 3, 6
+    rows = rng.normal(size=(n, heads * (2 * dk + dm)))
+    counts = rng.integers(1, 3, size=n).astype(np.float64)
+    g = rng.normal(size=dm)
+    an = O.attention_mean_backward(rows, counts, heads, dk, g, window)
+    eps = 1e-6
+    for i in range(n):
+        for j in range(rows.shape[1]):
+            rp, rm = rows.copy(), rows.copy()
+            rp[i, j] += eps
+            rm[i, j] -= eps
+            fd = (O.attention_mean(rp, counts, heads, dk, window) @ g
+                  - O.attention_mean(rm, counts, heads, dk, window) @ g) / (2 * eps)
+            assert abs(fd - an[i, j]) < 1e-7
+
+
+def test_sliding_items_keep_the_listed_order():
+    f_indptr = np.array([0, 4], dtype=np.uint64)
+    f_ids = np.array([5, 2, 5, 7], dtype=np.uint32)
+    assert O.attention_node_items(f_indptr, f_ids, 0, sliding=True) == ([5, 2, 5, 7], [4.0, 2.0, 4.0, 2.0])
+    assert O.attention_node_items(f_indptr, f_ids, 0) == ([5, 2, 7], [4.0, 2.0, 2.0])
+
+
+def test_sliding_attention_example_grads_fd():
+    table, f_indptr, f_ids, v, nbrs, negs = _toy_example(5, V=12, D=2 * (2 * 2 + 3))
+    r = O.attention_example_grads(table, f_indptr, f_ids, v, nbrs, negs, 2.0, 2, 2, "exact", window=1)
+    assert r.contributes
+    eps = 1e-6
+    loss = lambda t: O.attention_example_grads(t, f_indptr, f_ids, v, nbrs, negs, 2.0, 2, 2, "exact", window=1).loss
+    for fr in range(table.shape[0]):
+        for j in range(table.shape[1]):
+            tp, tm = table.copy(), table.copy()
+            tp[fr, j] += eps
+            tm[fr, j] -= eps
+            g = r.grads[fr][j] if fr in r.grads else 0.0
+            assert abs((loss(tp) - loss(tm)) / (2 * eps) - g) < 1e-6
