@@ -591,11 +591,9 @@ class EmbeddingPropagator:
         mn_kind, mn, mn_p = _parse_sample(max_nodes, "max_nodes")
         if attention is not None and context_window is not None and int(context_window) < 1:
             raise ValueError("context_window must be >= 1 (window_size 0 is not supported by this build)")
-        if mf_kind != N.EP_SAMPLE_ALL and attention is not None and context_window is None:
-            raise ValueError("max_features with attention (Random attention) is not supported by this build")
-        if attention is not None:
-            # lib.rs:1288-1300: the attention model takes max_features only as Random
-            # attention's size (Sliding ignores it) and never averages by weight
+        if attention is not None and context_window is not None:
+            # lib.rs:1288-1300: Sliding attention ignores max_features; without a window,
+            # max_features is Random attention's column sample (kept in the config)
             mf_kind, mf, mf_p = N.EP_SAMPLE_ALL, 0, 0.0
         cfg = N.default_config()
         cfg.alpha = 0.9 if alpha is None else float(alpha)

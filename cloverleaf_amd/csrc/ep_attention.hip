@@ -117,18 +117,74 @@ struct AttLds {
   float* K;      // [64][kTileStride]: K_h of every item
   float* acc;    // [H * att_dm] value sweep (forward; aliases Q / K)
   uint32_t win;  // Sliding attention: entries with |i - j| <= win exist; ~0u: Full
+  // Random attention: row masks of the current head (nullptr otherwise) and the stream
+  // of the block: example position rpos in the pass, pass, block index re in the example
+  uint64_t* rmask;
+  int rkind;
+  uint32_t rk;
+  float rp;
+  uint32_t rpos, rpass, re, k0, k1;
 };
 
 // Entry (i, j) of a block of n items exists: both are items and, under Sliding
-// attention, |i - j| <= window_size (compute_sliding_attention_matrix, attention.rs:194-218).
-__device__ inline bool att_pair(uint32_t i, uint32_t j, uint32_t n, uint32_t win) {
-  return i < n && j < n && (i > j ? i - j : j - i) <= win;
+// attention, |i - j| <= window_size (compute_sliding_attention_matrix, attention.rs:194-218),
+// under Random attention j is one of row i's sampled columns.
+template <bool RND>
+__device__ inline bool att_pair(const AttLds& L, uint32_t i, uint32_t j, uint32_t n) {
+  if (!(i < n && j < n)) return false;
+  if constexpr (RND) return ((L.rmask[i] >> j) & 1ull) != 0ull;
+  return (i > j ? i - j : j - i) <= L.win;
+}
+
+__device__ inline uint32_t att_word(const AttLds& L, uint32_t purpose, uint32_t i) {
+  const u32x4 r = philox4x32(u32x4{L.rpos, L.rpass, purpose, i >> 2}, L.k0, L.k1);
+  const uint32_t q = i & 3u;
+  return q == 0 ? r.x : q == 1 ? r.y : q == 2 ? r.z : r.w;
+}
+
+// Random attention for head h of a block of n items (compute_random_attention_matrix,
+// attention.rs:221-250): (k, scale) = num_features.sample(n, true) once per head (Fixed:
+// min(K, n), n / k; Probability: a Binomial count of n keeps, at least one, 1 / p or
+// (1 - p^n) / p), then row i attends to k distinct columns chosen uniformly (Floyd's
+// algorithm, lane i); every item's q, k and v are scaled by `scale`, which is returned.
+// Fills L.rmask (the wave is the block).
+__device__ inline float att_random_head(const AttLds& L, int h, uint32_t n) {
+  const uint32_t lane = lane_id();
+  const uint32_t tag = (L.re << 8) | ((uint32_t)h << 4);
+  uint32_t k;
+  float scale;
+  if (L.rkind == 1) {
+    k = min(L.rk, n);
+    scale = (float)n / (float)k;
+  } else {
+    const bool keep = lane < n && uniform01(att_word(L, RNG_ATT_COUNT | tag, lane)) < L.rp;
+    k = (uint32_t)__popcll(__ballot(keep));
+    scale = 1.f / L.rp;
+    if (k == 0u) {
+      k = 1u;
+      scale = (1.f - powf(L.rp, (float)n)) / L.rp;
+    }
+  }
+  uint64_t m = 0ull;
+  if (lane < n) {
+    if (k >= n) {
+      m = n >= 64u ? ~0ull : ((1ull << n) - 1ull);
+    } else {
+      for (uint32_t t = n - k; t < n; ++t) {
+        const uint32_t r = uniform_below(att_word(L, RNG_ATT_COLS | tag, lane * 64u + t), t + 1u);
+        m |= ((m >> r) & 1ull) ? (1ull << t) : (1ull << r);
+      }
+    }
+  }
+  L.rmask[lane] = m;
+  __syncthreads();
+  return scale;
 }
 
 constexpr int kMaxHeads = 16;
 
 __host__ __device__ inline size_t att_lds_small(int dm, int H) {
-  return 7 * (size_t)kAttMaxItems + 2 * (size_t)kAttMaxItems * H + (size_t)((dm + 3) & ~3);
+  return 2 * (size_t)kAttMaxItems + 7 * (size_t)kAttMaxItems + 2 * (size_t)kAttMaxItems * H + (size_t)((dm + 3) & ~3);
 }
 
 // forward: the value accumulators; backward: the operand tiles
@@ -139,7 +195,12 @@ __host__ __device__ inline size_t att_lds_floats(int dm, int H, bool backward) {
 __device__ inline AttLds att_lds(float* sm, int dm, int H, uint32_t win) {
   AttLds L;
   L.win = win;
-  L.M = sm;
+  L.rmask = nullptr;  // set by the training kernels under Random attention
+  L.rkind = 0;
+  L.rk = 0u;
+  L.rp = 0.f;
+  L.rpos = L.rpass = L.re = L.k0 = L.k1 = 0u;
+  L.M = sm + 2 * kAttMaxItems;  // after the 64 row masks
   L.iZ = L.M + kAttMaxItems;
   L.r = L.iZ + kAttMaxItems;
   L.f = (uint32_t*)(L.r + kAttMaxItems);
@@ -191,6 +252,7 @@ __device__ inline bool att_setup(const StepConfig& c, const StepState* st, uint3
     first = first && !(eq && t < lane);
   }
   const bool sliding = c.att_win != 0xFFFFFFFFu;
+  const bool nocount = sliding || c.att_rkind != 0;  // Sliding and Random attention: no count scaling
   if (B.neighbour || sliding) {
     // listed features in order (a repeated feature is repeated); Sliding attention does
     // not scale scores by counts, and count 0 gives the factor 1
@@ -198,7 +260,7 @@ __device__ inline bool att_setup(const StepConfig& c, const StepState* st, uint3
     if (lane >= nraw) L.cnt[lane] = 0.f;  // padded items
     if (lane < nraw) {
       L.f[lane] = fq;
-      L.cnt[lane] = sliding ? 0.f : (float)mult;
+      L.cnt[lane] = nocount ? 0.f : (float)mult;
       L.q[lane] = lane;
     }
   } else {
@@ -208,7 +270,7 @@ __device__ inline bool att_setup(const StepConfig& c, const StepState* st, uint3
     if (lane < nraw && first) {
       const uint32_t i = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
       L.f[i] = fq;
-      L.cnt[i] = 2.0f * (float)mult;
+      L.cnt[i] = nocount ? 0.f : 2.0f * (float)mult;
       L.q[i] = lane;
     }
   }
@@ -251,6 +313,7 @@ __device__ inline void load_qk16(const T* __restrict__ row, bool valid, int h, i
 
 // Row statistics of query tile I from the K Q^T tiles: scaled, masked scores into T,
 // returns the row max and 1 / denominator of query i = 32 I + l % 32 (both halves).
+template <bool RND>
 __device__ inline void row_softmax_stats(f32x16 (&T)[2], int NT, uint32_t n, int I, float ci, const AttLds& L,
                                          float isq, float* mx_out, float* iz_out) {
   const int l = (int)lane_id();
@@ -263,7 +326,7 @@ __device__ inline void row_softmax_stats(f32x16 (&T)[2], int NT, uint32_t n, int
     for (int r = 0; r < 16; ++r) {
       const uint32_t j = 32u * J + mrow(r, l);
       float s = T[J][r] * count_factor(ci, L.cnt[j]) * isq;
-      s = att_pair(i, j, n, L.win) ? s : -INFINITY;
+      s = att_pair<RND>(L, i, j, n) ? s : -INFINITY;
       T[J][r] = s;
       mx = fmaxf(mx, s);
     }
@@ -333,7 +396,7 @@ __global__ __launch_bounds__(256) void k_att_map(StepConfig c, StepState* st) {
 // per head scores on the matrix cores, softmax statistics and column sums w_jh (LDS),
 // the next head's Q / K loaded meanwhile; then one sweep over the items' value parts
 // (all heads of an item are contiguous): out = sum_j sum_h w_jh V_jh / (n H).
-template <int DK, typename T>
+template <int DK, typename T, bool RND>
 __device__ inline void att_block_output(const T* __restrict__ table, uint32_t W, int H, int dm, const AttLds& L,
                                         uint32_t n, float* __restrict__ out) {
   const int l = (int)lane_id(), hf = l >> 5, c32 = l & 31;
@@ -353,6 +416,10 @@ __device__ inline void att_block_output(const T* __restrict__ table, uint32_t W,
     load_qk16<DK, T>(table + (size_t)(ok ? L.f[i] : 0u) * W, ok, 0, H, hf, qa[I], ka[I]);
   }
   for (int h = 0; h < H; ++h) {
+    // Random attention: this head's row masks and item scale (scores scale by its square)
+    float hs = 1.f;
+    if constexpr (RND) hs = att_random_head(L, h, n);
+    const float isq_h = isq * hs * hs;
 #pragma unroll
     for (int I = 0; I < 2; ++I) {  // next head's operands in flight
       const uint32_t i = 32u * I + c32;
@@ -375,7 +442,7 @@ __device__ inline void att_block_output(const T* __restrict__ table, uint32_t W,
       const uint32_t i = 32u * I + c32;
       const float ci = i < n ? L.cnt[i] : 0.f;
       float mx, iz;
-      row_softmax_stats(Tt, NT, n, I, ci, L, isq, &mx, &iz);
+      row_softmax_stats<RND>(Tt, NT, n, I, ci, L, isq_h, &mx, &iz);
       if (hf == 0) { L.M[i] = mx; L.iZ[i] = iz; }
     }
     __syncthreads();
@@ -395,12 +462,12 @@ __device__ inline void att_block_output(const T* __restrict__ table, uint32_t W,
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           const uint32_t i = 32u * I + mrow(r, l);
-          const float sc = a[r] * count_factor(L.cnt[i], cj) * isq;
-          wj += att_pair(i, j, n, L.win) ? expf(sc - L.M[i]) * L.iZ[i] : 0.f;
+          const float sc = a[r] * count_factor(L.cnt[i], cj) * isq_h;
+          wj += att_pair<RND>(L, i, j, n) ? expf(sc - L.M[i]) * L.iZ[i] : 0.f;
         }
       }
       wj += __shfl_xor(wj, 32, EP_WAVE);
-      if (hf == 0) L.w[j * H + h] = j < n ? wj : 0.f;
+      if (hf == 0) L.w[j * H + h] = j < n ? wj * hs : 0.f;
     }
     __syncthreads();
 #pragma unroll
@@ -448,18 +515,35 @@ __device__ inline void att_block_output(const T* __restrict__ table, uint32_t W,
 }
 
 
+// Random attention stream of block g (example B.b) for the training kernels.
+__device__ inline void att_random_setup(const StepConfig& c, const StepState* st, uint32_t g, const AttBlock& B,
+                                        float* sm, AttLds& L) {
+  if (!c.att_rkind) return;
+  L.rmask = (uint64_t*)sm;
+  L.rkind = c.att_rkind;
+  L.rk = c.att_rk;
+  L.rp = c.att_rp;
+  L.rpos = st->anchor_base + B.b;
+  L.rpass = st->pass;
+  L.re = g - c.blkoff[B.b];
+  L.k0 = (uint32_t)c.seed;
+  L.k1 = (uint32_t)(c.seed >> 32);
+}
+
 // ------------------------------------------------------------------------------------
 // k_att_forward: one wave (64-thread block) per attention block.
 // ------------------------------------------------------------------------------------
-template <int DK, typename T>
+template <int DK, typename T, bool RND>
 __global__ __launch_bounds__(64) void k_att_forward(StepConfig c, StepState* st) {
   extern __shared__ float sm[];
   const int H = c.att_heads, dm = c.att_dm;
-  const AttLds L = att_lds(sm, dm, H, c.att_win);
+  AttLds L = att_lds(sm, dm, H, c.att_win);
   AttBlock B;
   if (!att_setup(c, st, blockIdx.x, L, B)) return;
+  att_random_setup(c, st, blockIdx.x, B, sm, L);
   if (lane_id() == 0) c.bitems[blockIdx.x] = B.n;
-  att_block_output<DK, T>((const T*)c.table, (uint32_t)c.dims, H, dm, L, B.n, c.bout + (size_t)blockIdx.x * dm);
+  att_block_output<DK, T, RND>((const T*)c.table, (uint32_t)c.dims, H, dm, L, B.n,
+                               c.bout + (size_t)blockIdx.x * dm);
 }
 
 // ------------------------------------------------------------------------------------
@@ -486,7 +570,7 @@ __global__ __launch_bounds__(64) void k_att_embed(const T* __restrict__ table, u
     L.f[lane] = lane < n ? fids[fs + lane] : 0u;
     L.cnt[lane] = lane < n ? 1.0f : 0.f;
     __syncthreads();
-    att_block_output<DK, T>(table, W, H, dm, L, n, o);
+    att_block_output<DK, T, false>(table, W, H, dm, L, n, o);
     __syncthreads();
   }
 }
@@ -781,15 +865,16 @@ __global__ __launch_bounds__(256) void k_att_loss(StepConfig c, StepState* st) {
 //   dS_ij = A_ij (u_j - r_i), r_i = sum_k A_ik u_k,   dT = dS m / sqrt(d_k)
 //   dQ = dT K,  dK = dT^T Q  (matrix cores),  dV_j = w_j g / (n H)
 // ------------------------------------------------------------------------------------
-template <int DK, typename T>
+template <int DK, typename T, bool RND>
 __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st) {
   extern __shared__ float sm[];
   const int H = c.att_heads, dm = c.att_dm;
   const uint32_t W = (uint32_t)c.dims;
-  const AttLds L = att_lds(sm, dm, H, c.att_win);
+  AttLds L = att_lds(sm, dm, H, c.att_win);
   if (st->n_keys > c.key_cap) return;
   AttBlock B;
   if (!att_setup(c, st, blockIdx.x, L, B)) return;
+  att_random_setup(c, st, blockIdx.x, B, sm, L);
   const uint32_t NC = c.K + 2u;
   if (!c.elig[(size_t)B.b * NC + B.ci]) return;
   const int l = (int)lane_id(), hf = l >> 5, c32 = l & 31;
@@ -811,6 +896,9 @@ __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st
   const int NT = (int)((n + 31u) / 32u);
   auto slot_row = [&](uint32_t i) -> float* { return c.adj + (size_t)(B.slot0 + L.q[i]) * W; };
   for (int h = 0; h < H; ++h) {
+    float hs = 1.f;  // Random attention: this head's row masks and item scale
+    if constexpr (RND) hs = att_random_head(L, h, n);
+    const float isq_h = isq * hs * hs;
     float qa[2][16], ka[2][16];
 #pragma unroll
     for (int I = 0; I < 2; ++I) {
@@ -851,7 +939,7 @@ __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st
         }
       }
       p += __shfl_xor(p, 32, EP_WAVE);
-      if (hf == 0) L.u[j] = j < n ? p * scale : 0.f;
+      if (hf == 0) L.u[j] = j < n ? p * scale * hs : 0.f;
     }
     __syncthreads();
     // query tiles (K Q^T layout): probabilities, r_i, dT, dQ = dT K
@@ -870,7 +958,7 @@ __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st
       const uint32_t i = 32u * I + c32;
       const float ci = i < n ? L.cnt[i] : 0.f;
       float mx, iz;
-      row_softmax_stats(Tt, NT, n, I, ci, L, isq, &mx, &iz);
+      row_softmax_stats<RND>(Tt, NT, n, I, ci, L, isq_h, &mx, &iz);
       float ri = 0.f;
       #pragma unroll
       for (int J = 0; J < 2; ++J) {
@@ -891,7 +979,7 @@ __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
           const uint32_t j = 32u * J + mrow(t, l);
-          const float dT = Tt[J][t] * (L.u[j] - ri) * count_factor(ci, L.cnt[j]) * isq;
+          const float dT = Tt[J][t] * (L.u[j] - ri) * count_factor(ci, L.cnt[j]) * isq_h;
           dq = mfma32(dT, L.K[(size_t)j * kTileStride + c32], dq);
         }
       }
@@ -922,8 +1010,8 @@ __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           const uint32_t i = 32u * I + mrow(r, l);
-          const float f = count_factor(L.cnt[i], cj) * isq;
-          const float A = att_pair(i, j, n, L.win) ? expf(a[r] * f - L.M[i]) * L.iZ[i] : 0.f;
+          const float f = count_factor(L.cnt[i], cj) * isq_h;
+          const float A = att_pair<RND>(L, i, j, n) ? expf(a[r] * f - L.M[i]) * L.iZ[i] : 0.f;
           wj += A;
           a[r] = A * (uj - L.r[i]) * f;  // dT_ij
         }
@@ -940,7 +1028,7 @@ __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st
       }
       const uint32_t jn = min(32u, n - 32u * J);
       for (uint32_t jj = 0; jj < jn; ++jj) {
-        const float cw = __shfl(wj, (int)jj, EP_WAVE) * scale;
+        const float cw = __shfl(wj, (int)jj, EP_WAVE) * scale * hs;
         float* vr = slot_row(32u * J + jj) + qoff + (size_t)h * dm;
         for (int t = l; t < dm; t += 64) vr[t] = cw * L.g[t];
       }
@@ -970,8 +1058,13 @@ hipError_t launch_att_forward(const StepConfig& c, StepState* st, hipStream_t s)
   const size_t lds = att_lds_floats(c.att_dm, c.att_heads, false) * 4;
   hipLaunchKernelGGL(k_att_map, dim3((c.batch + NW - 1) / NW), dim3(256), 0, s, c, st);
   EP_ATT_DISPATCH(c.att_dk, {
-    if (c.bf16) hipLaunchKernelGGL((k_att_forward<DK, uint16_t>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
-    else hipLaunchKernelGGL((k_att_forward<DK, float>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
+    if (c.att_rkind) {
+      if (c.bf16) hipLaunchKernelGGL((k_att_forward<DK, uint16_t, true>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
+      else hipLaunchKernelGGL((k_att_forward<DK, float, true>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
+    } else {
+      if (c.bf16) hipLaunchKernelGGL((k_att_forward<DK, uint16_t, false>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
+      else hipLaunchKernelGGL((k_att_forward<DK, float, false>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
+    }
   });
   hipLaunchKernelGGL(k_att_loss, dim3((c.batch + NW - 1) / NW), dim3(256), 0, s, c, st);
   return hipGetLastError();
@@ -995,8 +1088,13 @@ hipError_t launch_att_embed(const StepConfig& c, const uint64_t* findptr, const 
 hipError_t launch_att_backward(const StepConfig& c, StepState* st, hipStream_t s) {
   const size_t lds = att_lds_floats(c.att_dm, c.att_heads, true) * 4;
   EP_ATT_DISPATCH(c.att_dk, {
-    if (c.bf16) hipLaunchKernelGGL((k_att_backward<DK, uint16_t>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
-    else hipLaunchKernelGGL((k_att_backward<DK, float>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
+    if (c.att_rkind) {
+      if (c.bf16) hipLaunchKernelGGL((k_att_backward<DK, uint16_t, true>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
+      else hipLaunchKernelGGL((k_att_backward<DK, float, true>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
+    } else {
+      if (c.bf16) hipLaunchKernelGGL((k_att_backward<DK, uint16_t, false>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
+      else hipLaunchKernelGGL((k_att_backward<DK, float, false>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
+    }
   });
   return hipGetLastError();
 }

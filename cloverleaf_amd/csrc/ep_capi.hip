@@ -177,7 +177,9 @@ bool weighted_avg(const ep_config& cfg) {
   return cfg.weighted_neighbor_averaging && !ppr_loss(cfg) && !cfg.attention_dk;
 }
 
-bool feature_sampling(const ep_config& cfg) { return cfg.max_features_kind != EP_SAMPLE_ALL; }
+// max_features as feature dropout (mean model); the attention model takes it as Random
+// attention's column sample instead (lib.rs:1288-1300).
+bool feature_sampling(const ep_config& cfg) { return cfg.max_features_kind != EP_SAMPLE_ALL && !cfg.attention_dk; }
 
 // max_nodes Probability: variable-length lists stored at the anchors' CSR offsets.
 bool prob_nodes(const ep_config& cfg) { return cfg.max_nodes_kind == EP_SAMPLE_PROBABILITY && !ppr_loss(cfg); }
@@ -409,7 +411,12 @@ void refresh_base(ep_ctx* c) {
   b.wavg = weighted_avg(cfg) ? 1 : 0;
   b.wsample = (cfg.weighted_neighbor_sampling && !cfg.attention_dk) ? 1 : 0;
   b.cdf = c->d_cdf;
-  b.mf_kind = cfg.max_features_kind;
+  b.mf_kind = feature_sampling(cfg) ? cfg.max_features_kind : 0;
+  // Random attention: attention with max_features and no context_window (lib.rs:1290-1296)
+  const bool rnd = cfg.attention_dk && cfg.context_window == 0 && cfg.max_features_kind != EP_SAMPLE_ALL;
+  b.att_rkind = rnd ? cfg.max_features_kind : 0;
+  b.att_rk = cfg.max_features;
+  b.att_rp = cfg.max_features_p;
   b.mf_k = cfg.max_features;
   b.mf_p = cfg.max_features_p;
   b.nbr_rows = (neighbour_rows(cfg) && !cfg.attention_dk) ? 1 : 0;
@@ -656,8 +663,6 @@ int validate_config(const ep_config* cfg) {
     return fail(EP_EINVAL, "max_features (Fixed) must be >= 1");
   if (cfg->max_features_kind == EP_SAMPLE_PROBABILITY && !(cfg->max_features_p > 0.f && cfg->max_features_p < 1.f))
     return fail(EP_EINVAL, "max_features (Probability) must be in (0, 1)");
-  if (cfg->max_features_kind != EP_SAMPLE_ALL && cfg->attention_dk)
-    return fail(EP_EINVAL, "max_features sampling is not supported with attention");
   if (cfg->loss_kind < EP_LOSS_MARGIN || cfg->loss_kind > EP_LOSS_PPR) return fail(EP_EINVAL, "loss_kind");
   if (!std::isfinite(cfg->gamma) || !std::isfinite(cfg->loss_param2))
     return fail(EP_EINVAL, "loss parameters must be finite");

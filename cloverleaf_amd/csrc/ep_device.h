@@ -53,6 +53,8 @@ enum RngPurpose : uint32_t {
   RNG_HARD_WALK = 7,
   RNG_FEATURES = 8,  // max_features: purpose RNG_FEATURES | (entry << 4), one stream per node of an example
   RNG_NBR_KEEP = 9,  // max_nodes Probability: one keep draw per edge of the anchor
+  RNG_ATT_COUNT = 10,  // Random attention, Probability size: | (block << 8) | (head << 4)
+  RNG_ATT_COLS = 11,   // Random attention, columns of each row: | (block << 8) | (head << 4)
 };
 
 // Uniform integer in [0, n) from a 32-bit draw (multiply-shift; bias <= n / 2^32).

@@ -196,6 +196,11 @@ struct StepConfig {
   int model;               // 0 mean, 1 attention
   int att_heads, att_dk, att_dm;
   uint32_t att_win;        // Sliding attention window_size (attention.rs:194-218); ~0u: Full
+  // Random attention (attention.rs:221-250): max_features of the attention model as the
+  // per-row column sample, 0 none, 1 Fixed(att_rk), 2 Probability(att_rp)
+  int att_rkind;
+  uint32_t att_rk;
+  float att_rp;
   uint32_t* blkoff;        // [batch + 1]
   uint32_t* bitems;        // [blk_cap] items of each block (written by the forward)
   uint32_t* bnode;         // [blk_cap] node of each block        (written by k_att_map)

@@ -539,8 +539,9 @@ def _example(table, f_indptr, f_ids, v, nbrs, nbr_scale, negs, gamma, grad_mode,
         return example_grads(table, f_indptr, f_ids, v, nbrs, nbr_scale, negs, gamma, grad_mode, div, loss, feat_sel)
     heads, d_k = attention[:2]
     window = attention[2] if len(attention) > 2 else None  # (heads, d_k, window): Sliding attention
+    rand_blocks = attention[3] if len(attention) > 3 else None  # Random attention: per-block masks
     return attention_example_grads(table, f_indptr, f_ids, v, nbrs, negs, gamma, heads, d_k, grad_mode, div, loss,
-                                   window)
+                                   window, rand_blocks)
 
 
 def ep_batch_step(table, mom, var, f_indptr, f_ids, anchors, nbr_lists, nbr_scales, neg_lists,
@@ -556,8 +557,9 @@ def ep_batch_step(table, mom, var, f_indptr, f_ids, anchors, nbr_lists, nbr_scal
     results = []
     for i, v in enumerate(anchors):
         div = 1.0 if loss_divs is None else loss_divs[i]
+        att = attention[i] if isinstance(attention, list) else attention  # per example (Random attention)
         results.append(_example(table, f_indptr, f_ids, v, nbr_lists[i], nbr_scales[i], neg_lists[i], gamma,
-                                grad_mode, div, attention, loss, None if feat_sels is None else feat_sels[i]))
+                                grad_mode, div, att, loss, None if feat_sels is None else feat_sels[i]))
     merged = merge_batch(results, table.shape[1])
     if any(r.contributes for r in results):
         adam_update(table, mom, var, merged, lr, pass_)
@@ -670,7 +672,7 @@ def masked_softmax_rows(S: np.ndarray, mask: np.ndarray) -> np.ndarray:
 
 
 def attention_mean(rows: np.ndarray, counts: np.ndarray, heads: int, d_k: int,
-                   window: Optional[int] = None) -> np.ndarray:
+                   window: Optional[int] = None, rand=None) -> np.ndarray:
     """attention_mean (attention.rs:100-131), float64; Full attention, or Sliding with
     window_size ``window``.
 
@@ -678,6 +680,10 @@ def attention_mean(rows: np.ndarray, counts: np.ndarray, heads: int, d_k: int,
     vector of head 0 is returned (attention.rs:113-115).  Otherwise, per head,
     A = softmax_row(S / sqrt(d_k)), out_h = (1/n) sum_i sum_j A_ij V_j, and
     the result is the mean over heads.
+
+    rand: Random attention (compute_random_attention_matrix, attention.rs:221-250), one
+    (mask, scale) per head: row i attends to the columns mask[i], every item's q, k and v
+    are scaled by `scale` (so the scores by scale^2), no count scaling.
     """
     rows = np.asarray(rows, dtype=np.float64)
     Q, K, V, d_model = attention_split(rows, heads, d_k)
@@ -687,13 +693,17 @@ def attention_mean(rows: np.ndarray, counts: np.ndarray, heads: int, d_k: int,
     out = np.zeros(d_model)
     for h in range(heads):
         S, mask, _ = attention_scores(Q[h], K[h], np.asarray(counts, dtype=np.float64), window)
+        sc = 1.0
+        if rand is not None:
+            mask, sc = np.asarray(rand[h][0], dtype=bool), float(rand[h][1])
+            S = (Q[h] @ K[h].T) * sc * sc
         A = masked_softmax_rows(S / math.sqrt(d_k), mask)
-        out += (A.sum(axis=0) @ V[h]) / n
+        out += (A.sum(axis=0) @ V[h]) * sc / n
     return out / heads
 
 
 def attention_mean_backward(rows: np.ndarray, counts: np.ndarray, heads: int, d_k: int,
-                            g_out: np.ndarray, window: Optional[int] = None) -> np.ndarray:
+                            g_out: np.ndarray, window: Optional[int] = None, rand=None) -> np.ndarray:
     """Adjoint of ``attention_mean`` w.r.t. the feature rows (float64).
 
     With w_j = sum_i A_ij:  dV_j = w_j g / (n H);
@@ -712,11 +722,16 @@ def attention_mean_backward(rows: np.ndarray, counts: np.ndarray, heads: int, d_
         return grad
     for h in range(heads):
         S, mask, m = attention_scores(Q[h], K[h], counts, window)
+        sc = 1.0
+        if rand is not None:  # Random attention: q, k, v scaled by sc, no count scaling
+            mask, sc = np.asarray(rand[h][0], dtype=bool), float(rand[h][1])
+            m = np.full((n, n), sc * sc)
+            S = (Q[h] @ K[h].T) * m
         A = masked_softmax_rows(S / math.sqrt(d_k), mask)
         w = A.sum(axis=0)
         scale = 1.0 / (n * heads)
-        grad[:, base + h * d_model: base + (h + 1) * d_model] += np.outer(w, g_out) * scale
-        dA = np.tile((V[h] @ g_out) * scale, (n, 1))
+        grad[:, base + h * d_model: base + (h + 1) * d_model] += np.outer(w, g_out) * scale * sc
+        dA = np.tile((V[h] @ g_out) * scale * sc, (n, 1))
         dS = A * (dA - np.sum(A * dA, axis=1, keepdims=True))
         dT = dS * m / math.sqrt(d_k)
         grad[:, h * d_k:(h + 1) * d_k] += dT @ K[h]
@@ -767,7 +782,8 @@ def _scatter_rows(out: Dict[int, np.ndarray], feats, rows: np.ndarray):
 
 
 def attention_example_grads(table, f_indptr, f_ids, anchor, nbrs, negs, gamma, heads, d_k,
-                            grad_mode="first_leaf", loss_div=1.0, loss=None, window=None) -> ExampleResult:
+                            grad_mode="first_leaf", loss_div=1.0, loss=None, window=None,
+                            rand_blocks=None) -> ExampleResult:
     """Forward + adjoint + gradient extraction for one anchor under the attention model.
 
     AttentionFeatureModel (model.rs:158-249):
@@ -782,6 +798,8 @@ def attention_example_grads(table, f_indptr, f_ids, anchor, nbrs, negs, gamma, h
     Leaf gradients are full table rows [Q | K | V] of width heads * (2 d_k + d_model).
     The first_leaf / exact rules and the eligibility of constructions are those of
     example_grads.  window: Sliding attention with that window_size (None: Full).
+    rand_blocks: Random attention, per block (0 = a, 1.. the neighbours, then the
+    negatives) the per-head (mask, scale) of attention_mean.
     """
     table = np.asarray(table)
     W = table.shape[1]
@@ -790,13 +808,15 @@ def attention_example_grads(table, f_indptr, f_ids, anchor, nbrs, negs, gamma, h
     nb_sets = [attention_neighbour_items(f_indptr, f_ids, u) for u in nbrs]
     neg_sets = [attention_node_items(f_indptr, f_ids, u, sl) for u in negs]
 
-    def fwd(items):
-        feats, counts = items
-        return attention_mean(table[np.asarray(feats, dtype=np.int64)], np.asarray(counts), heads, d_k, window)
+    rb = (lambda e: None) if rand_blocks is None else (lambda e: rand_blocks[e])
 
-    a = fwd(node_sets[0])
-    p = sum(fwd(it) for it in nb_sets) / float(len(nb_sets))
-    nv = [fwd(it) for it in neg_sets]
+    def fwd(items, e):
+        feats, counts = items
+        return attention_mean(table[np.asarray(feats, dtype=np.int64)], np.asarray(counts), heads, d_k, window, rb(e))
+
+    a = fwd(node_sets[0], 0)
+    p = sum(fwd(it, 1 + j) for j, it in enumerate(nb_sets)) / float(len(nb_sets))
+    nv = [fwd(it, 1 + len(nb_sets) + k) for k, it in enumerate(neg_sets)]
     vecs = [a, p] + nv
     kind, params = loss if loss is not None else ("margin", (gamma,))
     L, _, g_a, g_p, g_n = loss_adjoints(kind, a, p, nv, params)
@@ -806,10 +826,10 @@ def attention_example_grads(table, f_indptr, f_ids, anchor, nbrs, negs, gamma, h
     if not (L > 0.0) or math.isnan(L):
         return ExampleResult(L if not math.isnan(L) else float("nan"), False, {}, vecs, adjs)
 
-    def bwd(items, g):
+    def bwd(items, g, e):
         feats, counts = items
         rows = table[np.asarray(feats, dtype=np.int64)]
-        return attention_mean_backward(rows, np.asarray(counts), heads, d_k, g, window)
+        return attention_mean_backward(rows, np.asarray(counts), heads, d_k, g, window, rb(e))
 
     # leaf gradients per construction, in construction order a, p, n_1..n_K
     leaves: List[Optional[Dict[int, np.ndarray]]] = []
@@ -819,11 +839,11 @@ def attention_example_grads(table, f_indptr, f_ids, anchor, nbrs, negs, gamma, h
             continue
         out: Dict[int, np.ndarray] = {}
         if ci == 1:
-            for it in nb_sets:
-                _scatter_rows(out, it[0], bwd(it, adj / float(len(nb_sets))))
+            for j, it in enumerate(nb_sets):
+                _scatter_rows(out, it[0], bwd(it, adj / float(len(nb_sets)), 1 + j))
         else:
             it = node_sets[0] if ci == 0 else neg_sets[ci - 2]
-            _scatter_rows(out, it[0], bwd(it, adj))
+            _scatter_rows(out, it[0], bwd(it, adj, 0 if ci == 0 else 1 + len(nb_sets) + ci - 2))
         leaves.append(out)
     grads: Dict[int, np.ndarray] = {}
     for out in leaves:

@@ -51,7 +51,8 @@ def make_problem(seed=0, n=120, V=300, D=32, avg_deg=6, fmax=8, zipf=1.1, zero_d
 def oracle_step(prob: Problem, anchors, cnt, nbr, neg, gamma, lr, pass_, grad_mode, table=None, mom=None, var=None,
                 loss_divs=None, attention=None, loss=None, nbr_weights=None, feat_sels=None):
     """FP64 oracle of one explicit-sample step; returns (results, merged, table, mom, var).
-    attention = (heads, d_k) selects the attention model; loss = (kind, params) the loss;
+    attention = (heads, d_k[, window[, random blocks]]) selects the attention model (a list:
+    one tuple per anchor); loss = (kind, params) the loss;
     nbr_weights[i][j] is neighbour j's weight (weighted averaging; default uniform);
     feat_sels[i] the max_features selections of example i (see example_grads)."""
     t = (prob.table if table is None else table).astype(np.float64).copy()
@@ -62,6 +63,8 @@ def oracle_step(prob: Problem, anchors, cnt, nbr, neg, gamma, lr, pass_, grad_mo
     results, merged = [], {}
     # anchors with no neighbourhood are skipped (no example)
     keep = [i for i in range(B) if cnt[i] > 0]
+    if isinstance(attention, list):
+        attention = [attention[i] for i in keep]
     res_k, merged = O.ep_batch_step(t, m, v, prob.f_indptr, prob.f_ids, [int(anchors[i]) for i in keep],
                                     [nbr_lists[i] for i in keep],
                                     [1.0] * len(keep) if nbr_weights is None

@@ -1012,3 +1012,55 @@ def test_sliding_attention_example_grads_fd():
             tm[fr, j] -= eps
             g = r.grads[fr][j] if fr in r.grads else 0.0
             assert abs((loss(tp) - loss(tm)) / (2 * eps) - g) < 1e-6
+
+
+# --------------------------------------------------------------------------------------
+# Random attention (attention.rs:221-250)
+# --------------------------------------------------------------------------------------
+def test_random_attention_full_masks_equal_scaled_rows():
+    rng = np.random.default_rng(12)
+    heads, dk, dm, n = 2, 3, 4, 5
+    rows = rng.normal(size=(n, heads * (2 * dk + dm)))
+    low = np.full(n, 0.5)  # counts below 1: no count scaling under Full attention
+    full = [(np.ones((n, n), bool), 1.0)] * heads
+    assert np.allclose(O.attention_mean(rows, low, heads, dk, rand=full), O.attention_mean(rows, low, heads, dk))
+    # the item scale multiplies q, k and v: the same as scaling every row
+    sc = [(np.ones((n, n), bool), 1.7)] * heads
+    assert np.allclose(O.attention_mean(rows, low, heads, dk, rand=sc), O.attention_mean(1.7 * rows, low, heads, dk))
+    # counts do not scale the scores
+    assert np.allclose(O.attention_mean(rows, np.full(n, 3.0), heads, dk, rand=full),
+                       O.attention_mean(rows, low, heads, dk))
+
+
+def test_random_attention_mask_kat():
+    # one head, d_k 1: row 0 attends to {0}, row 1 to {0, 2}, row 2 to {1}
+    mask = np.array([[1, 0, 0], [1, 0, 1], [0, 1, 0]], dtype=bool)
+    Q, K, V, dm = O.attention_split(ATT_ROWS, heads=1, d_k=1)
+    out = O.attention_mean(ATT_ROWS, np.ones(3), 1, 1, rand=[(mask, 1.0)])
+    a1 = O.softmax_exact(np.array([0.0 * -1.0, 0.0 * 1.0]))  # row 1: q_1 . k_0, q_1 . k_2
+    A = np.array([[1.0, 0.0, 0.0], [a1[0], 0.0, a1[1]], [0.0, 1.0, 0.0]])
+    assert np.allclose(out, (A.sum(axis=0) @ V[0]) / 3)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_random_attention_backward_vs_fd(seed):
+    rng = np.random.default_rng(seed)
+    heads, dk, dm, n = 2, 2, 3, 6
+    rows = rng.normal(size=(n, heads * (2 * dk + dm)))
+    counts = rng.integers(1, 3, size=n).astype(np.float64)
+    rand = []
+    for h in range(heads):
+        m = rng.random((n, n)) < 0.5
+        m[np.arange(n), rng.integers(0, n, n)] = True  # every row has a column
+        rand.append((m, float(rng.uniform(0.5, 2.0))))
+    g = rng.normal(size=dm)
+    an = O.attention_mean_backward(rows, counts, heads, dk, g, rand=rand)
+    eps = 1e-6
+    for i in range(n):
+        for j in range(rows.shape[1]):
+            rp, rm = rows.copy(), rows.copy()
+            rp[i, j] += eps
+            rm[i, j] -= eps
+            fd = (O.attention_mean(rp, counts, heads, dk, rand=rand) @ g
+                  - O.attention_mean(rm, counts, heads, dk, rand=rand) @ g) / (2 * eps)
+            assert abs(fd - an[i, j]) < 1e-7
