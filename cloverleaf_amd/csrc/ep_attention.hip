@@ -45,7 +45,7 @@ namespace {
 
 constexpr int NW = kBlockWaves;  // waves per 256-thread block (k_att_loss)
 constexpr int kAttMaxItems = 64;
-constexpr int kTileStride = 36;  // LDS row stride (floats) of the Q / K operand tiles
+constexpr int kTileStride = 34;  // LDS row stride (floats) of the Q / K operand tiles (8-byte rows)
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
@@ -129,11 +129,15 @@ struct AttLds {
 // Entry (i, j) of a block of n items exists: both are items and, under Sliding
 // attention, |i - j| <= window_size (compute_sliding_attention_matrix, attention.rs:194-218),
 // under Random attention j is one of row i's sampled columns.
-template <bool RND>
+// Attention modes of the block kernels (compile-time): Full, Sliding, Random.
+enum : int { AM_FULL = 0, AM_SLIDING = 1, AM_RANDOM = 2 };
+
+template <int AM>
 __device__ inline bool att_pair(const AttLds& L, uint32_t i, uint32_t j, uint32_t n) {
   if (!(i < n && j < n)) return false;
-  if constexpr (RND) return ((L.rmask[i] >> j) & 1ull) != 0ull;
-  return (i > j ? i - j : j - i) <= L.win;
+  if constexpr (AM == AM_RANDOM) return ((L.rmask[i] >> j) & 1ull) != 0ull;
+  if constexpr (AM == AM_SLIDING) return (i > j ? i - j : j - i) <= L.win;
+  return true;
 }
 
 __device__ inline uint32_t att_word(const AttLds& L, uint32_t purpose, uint32_t i) {
@@ -183,16 +187,21 @@ __device__ inline float att_random_head(const AttLds& L, int h, uint32_t n) {
 
 constexpr int kMaxHeads = 16;
 
-__host__ __device__ inline size_t att_lds_small(int dm, int H) {
-  return 2 * (size_t)kAttMaxItems + 7 * (size_t)kAttMaxItems + 2 * (size_t)kAttMaxItems * H + (size_t)((dm + 3) & ~3);
+// Per-item arrays.  Random attention adds the 64 row masks in front; the backward works
+// one head at a time and keeps u for that head only (no w).  LDS per block bounds the
+// blocks per CU of the backward: at H = 4, d = 128 it takes 19968 bytes, 8 blocks per CU
+// (the VGPR bound), where 20992 would allow 7.
+__host__ __device__ inline size_t att_lds_small(int dm, int H, bool rnd, bool backward) {
+  return (rnd ? 2 * (size_t)kAttMaxItems : 0) + 7 * (size_t)kAttMaxItems +
+         (backward ? (size_t)kAttMaxItems : 2 * (size_t)kAttMaxItems * H) + (size_t)((dm + 3) & ~3);
 }
 
 // forward: the value accumulators; backward: the operand tiles
-__host__ __device__ inline size_t att_lds_floats(int dm, int H, bool backward) {
-  return att_lds_small(dm, H) + (backward ? 2 * (size_t)kAttMaxItems * kTileStride : (size_t)H * dm);
+__host__ __device__ inline size_t att_lds_floats(int dm, int H, bool backward, bool rnd) {
+  return att_lds_small(dm, H, rnd, backward) + (backward ? 2 * (size_t)kAttMaxItems * kTileStride : (size_t)H * dm);
 }
 
-__device__ inline AttLds att_lds(float* sm, int dm, int H, uint32_t win) {
+__device__ inline AttLds att_lds(float* sm, int dm, int H, uint32_t win, bool rnd, bool backward = false) {
   AttLds L;
   L.win = win;
   L.rmask = nullptr;  // set by the training kernels under Random attention
@@ -200,7 +209,7 @@ __device__ inline AttLds att_lds(float* sm, int dm, int H, uint32_t win) {
   L.rk = 0u;
   L.rp = 0.f;
   L.rpos = L.rpass = L.re = L.k0 = L.k1 = 0u;
-  L.M = sm + 2 * kAttMaxItems;  // after the 64 row masks
+  L.M = sm + (rnd ? 2 * kAttMaxItems : 0);  // after the 64 row masks (Random attention)
   L.iZ = L.M + kAttMaxItems;
   L.r = L.iZ + kAttMaxItems;
   L.f = (uint32_t*)(L.r + kAttMaxItems);
@@ -208,9 +217,9 @@ __device__ inline AttLds att_lds(float* sm, int dm, int H, uint32_t win) {
   L.q = (uint32_t*)(L.cnt + kAttMaxItems);
   L.raw = L.q + kAttMaxItems;
   L.u = (float*)(L.raw + kAttMaxItems);
-  L.w = L.u + kAttMaxItems * H;
-  L.g = L.w + kAttMaxItems * H;
-  L.Q = sm + att_lds_small(dm, H);  // 16-byte aligned: the small part is a multiple of 4 floats
+  L.w = backward ? nullptr : L.u + kAttMaxItems * H;
+  L.g = backward ? L.u + kAttMaxItems : L.w + kAttMaxItems * H;
+  L.Q = sm + att_lds_small(dm, H, rnd, backward);  // 16-byte aligned: the small part is a multiple of 4 floats
   L.K = L.Q + kAttMaxItems * kTileStride;
   L.acc = L.Q;
   return L;
@@ -313,7 +322,7 @@ __device__ inline void load_qk16(const T* __restrict__ row, bool valid, int h, i
 
 // Row statistics of query tile I from the K Q^T tiles: scaled, masked scores into T,
 // returns the row max and 1 / denominator of query i = 32 I + l % 32 (both halves).
-template <bool RND>
+template <int AM>
 __device__ inline void row_softmax_stats(f32x16 (&T)[2], int NT, uint32_t n, int I, float ci, const AttLds& L,
                                          float isq, float* mx_out, float* iz_out) {
   const int l = (int)lane_id();
@@ -326,7 +335,7 @@ __device__ inline void row_softmax_stats(f32x16 (&T)[2], int NT, uint32_t n, int
     for (int r = 0; r < 16; ++r) {
       const uint32_t j = 32u * J + mrow(r, l);
       float s = T[J][r] * count_factor(ci, L.cnt[j]) * isq;
-      s = att_pair<RND>(L, i, j, n) ? s : -INFINITY;
+      s = att_pair<AM>(L, i, j, n) ? s : -INFINITY;
       T[J][r] = s;
       mx = fmaxf(mx, s);
     }
@@ -396,7 +405,7 @@ __global__ __launch_bounds__(256) void k_att_map(StepConfig c, StepState* st) {
 // per head scores on the matrix cores, softmax statistics and column sums w_jh (LDS),
 // the next head's Q / K loaded meanwhile; then one sweep over the items' value parts
 // (all heads of an item are contiguous): out = sum_j sum_h w_jh V_jh / (n H).
-template <int DK, typename T, bool RND>
+template <int DK, typename T, int AM>
 __device__ inline void att_block_output(const T* __restrict__ table, uint32_t W, int H, int dm, const AttLds& L,
                                         uint32_t n, float* __restrict__ out) {
   const int l = (int)lane_id(), hf = l >> 5, c32 = l & 31;
@@ -418,7 +427,7 @@ __device__ inline void att_block_output(const T* __restrict__ table, uint32_t W,
   for (int h = 0; h < H; ++h) {
     // Random attention: this head's row masks and item scale (scores scale by its square)
     float hs = 1.f;
-    if constexpr (RND) hs = att_random_head(L, h, n);
+    if constexpr (AM == AM_RANDOM) hs = att_random_head(L, h, n);
     const float isq_h = isq * hs * hs;
 #pragma unroll
     for (int I = 0; I < 2; ++I) {  // next head's operands in flight
@@ -442,7 +451,7 @@ __device__ inline void att_block_output(const T* __restrict__ table, uint32_t W,
       const uint32_t i = 32u * I + c32;
       const float ci = i < n ? L.cnt[i] : 0.f;
       float mx, iz;
-      row_softmax_stats<RND>(Tt, NT, n, I, ci, L, isq_h, &mx, &iz);
+      row_softmax_stats<AM>(Tt, NT, n, I, ci, L, isq_h, &mx, &iz);
       if (hf == 0) { L.M[i] = mx; L.iZ[i] = iz; }
     }
     __syncthreads();
@@ -463,7 +472,7 @@ __device__ inline void att_block_output(const T* __restrict__ table, uint32_t W,
         for (int r = 0; r < 16; ++r) {
           const uint32_t i = 32u * I + mrow(r, l);
           const float sc = a[r] * count_factor(L.cnt[i], cj) * isq_h;
-          wj += att_pair<RND>(L, i, j, n) ? expf(sc - L.M[i]) * L.iZ[i] : 0.f;
+          wj += att_pair<AM>(L, i, j, n) ? expf(sc - L.M[i]) * L.iZ[i] : 0.f;
         }
       }
       wj += __shfl_xor(wj, 32, EP_WAVE);
@@ -533,16 +542,16 @@ __device__ inline void att_random_setup(const StepConfig& c, const StepState* st
 // ------------------------------------------------------------------------------------
 // k_att_forward: one wave (64-thread block) per attention block.
 // ------------------------------------------------------------------------------------
-template <int DK, typename T, bool RND>
+template <int DK, typename T, int AM>
 __global__ __launch_bounds__(64) void k_att_forward(StepConfig c, StepState* st) {
   extern __shared__ float sm[];
   const int H = c.att_heads, dm = c.att_dm;
-  AttLds L = att_lds(sm, dm, H, c.att_win);
+  AttLds L = att_lds(sm, dm, H, c.att_win, AM == AM_RANDOM);
   AttBlock B;
   if (!att_setup(c, st, blockIdx.x, L, B)) return;
   att_random_setup(c, st, blockIdx.x, B, sm, L);
   if (lane_id() == 0) c.bitems[blockIdx.x] = B.n;
-  att_block_output<DK, T, RND>((const T*)c.table, (uint32_t)c.dims, H, dm, L, B.n,
+  att_block_output<DK, T, AM>((const T*)c.table, (uint32_t)c.dims, H, dm, L, B.n,
                                c.bout + (size_t)blockIdx.x * dm);
 }
 
@@ -551,13 +560,13 @@ __global__ __launch_bounds__(64) void k_att_forward(StepConfig c, StepState* st)
 // wave per node: Full attention over the node's listed features with count 1 each.
 // Nodes without features keep a zero row.
 // ------------------------------------------------------------------------------------
-template <int DK, typename T>
+template <int DK, typename T, int AM>
 __global__ __launch_bounds__(64) void k_att_embed(const T* __restrict__ table, uint32_t W, int H, int dm,
                                                   const uint64_t* __restrict__ findptr,
                                                   const uint32_t* __restrict__ fids, uint64_t n_nodes,
                                                   uint32_t win, float* __restrict__ out) {
   extern __shared__ float sm[];
-  const AttLds L = att_lds(sm, dm, H, win);
+  const AttLds L = att_lds(sm, dm, H, win, false);
   const uint32_t lane = lane_id();
   for (uint64_t node = blockIdx.x; node < n_nodes; node += gridDim.x) {
     const uint64_t fs = findptr[node];
@@ -570,7 +579,7 @@ __global__ __launch_bounds__(64) void k_att_embed(const T* __restrict__ table, u
     L.f[lane] = lane < n ? fids[fs + lane] : 0u;
     L.cnt[lane] = lane < n ? 1.0f : 0.f;
     __syncthreads();
-    att_block_output<DK, T, false>(table, W, H, dm, L, n, o);
+    att_block_output<DK, T, AM>(table, W, H, dm, L, n, o);
     __syncthreads();
   }
 }
@@ -865,12 +874,12 @@ __global__ __launch_bounds__(256) void k_att_loss(StepConfig c, StepState* st) {
 //   dS_ij = A_ij (u_j - r_i), r_i = sum_k A_ik u_k,   dT = dS m / sqrt(d_k)
 //   dQ = dT K,  dK = dT^T Q  (matrix cores),  dV_j = w_j g / (n H)
 // ------------------------------------------------------------------------------------
-template <int DK, typename T, bool RND>
+template <int DK, typename T, int AM>
 __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st) {
   extern __shared__ float sm[];
   const int H = c.att_heads, dm = c.att_dm;
   const uint32_t W = (uint32_t)c.dims;
-  AttLds L = att_lds(sm, dm, H, c.att_win);
+  AttLds L = att_lds(sm, dm, H, c.att_win, AM == AM_RANDOM, true);
   if (st->n_keys > c.key_cap) return;
   AttBlock B;
   if (!att_setup(c, st, blockIdx.x, L, B)) return;
@@ -897,7 +906,7 @@ __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st
   auto slot_row = [&](uint32_t i) -> float* { return c.adj + (size_t)(B.slot0 + L.q[i]) * W; };
   for (int h = 0; h < H; ++h) {
     float hs = 1.f;  // Random attention: this head's row masks and item scale
-    if constexpr (RND) hs = att_random_head(L, h, n);
+    if constexpr (AM == AM_RANDOM) hs = att_random_head(L, h, n);
     const float isq_h = isq * hs * hs;
     float qa[2][16], ka[2][16];
 #pragma unroll
@@ -907,11 +916,9 @@ __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st
       load_qk16<DK, T>(table + (size_t)(ok ? L.f[i] : 0u) * W, ok, h, H, hf, qa[I], ka[I]);
       if (I < NT) {
 #pragma unroll
-        for (int t = 0; t < 16; t += 4) {
-          *reinterpret_cast<float4*>(L.Q + (size_t)i * kTileStride + 16 * hf + t) =
-              make_float4(qa[I][t], qa[I][t + 1], qa[I][t + 2], qa[I][t + 3]);
-          *reinterpret_cast<float4*>(L.K + (size_t)i * kTileStride + 16 * hf + t) =
-              make_float4(ka[I][t], ka[I][t + 1], ka[I][t + 2], ka[I][t + 3]);
+        for (int t = 0; t < 16; t += 2) {
+          *reinterpret_cast<float2*>(L.Q + (size_t)i * kTileStride + 16 * hf + t) = make_float2(qa[I][t], qa[I][t + 1]);
+          *reinterpret_cast<float2*>(L.K + (size_t)i * kTileStride + 16 * hf + t) = make_float2(ka[I][t], ka[I][t + 1]);
         }
       }
     }
@@ -958,7 +965,7 @@ __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st
       const uint32_t i = 32u * I + c32;
       const float ci = i < n ? L.cnt[i] : 0.f;
       float mx, iz;
-      row_softmax_stats<RND>(Tt, NT, n, I, ci, L, isq_h, &mx, &iz);
+      row_softmax_stats<AM>(Tt, NT, n, I, ci, L, isq_h, &mx, &iz);
       float ri = 0.f;
       #pragma unroll
       for (int J = 0; J < 2; ++J) {
@@ -1011,7 +1018,7 @@ __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st
         for (int r = 0; r < 16; ++r) {
           const uint32_t i = 32u * I + mrow(r, l);
           const float f = count_factor(L.cnt[i], cj) * isq_h;
-          const float A = att_pair<RND>(L, i, j, n) ? expf(a[r] * f - L.M[i]) * L.iZ[i] : 0.f;
+          const float A = att_pair<AM>(L, i, j, n) ? expf(a[r] * f - L.M[i]) * L.iZ[i] : 0.f;
           wj += A;
           a[r] = A * (uj - L.r[i]) * f;  // dT_ij
         }
@@ -1045,6 +1052,19 @@ bool att_shape_supported(int heads, int dk, int dm) {
          dm % 8 == 0;
 }
 
+// Attention mode of the block kernels: Random (max_features), Sliding (window), Full.
+#define EP_ATT_MODE(rkind, win, BODY)                                   \
+  if (rkind) {                                                          \
+    constexpr int AM = AM_RANDOM;                                       \
+    BODY;                                                               \
+  } else if ((win) != 0xFFFFFFFFu) {                                    \
+    constexpr int AM = AM_SLIDING;                                      \
+    BODY;                                                               \
+  } else {                                                              \
+    constexpr int AM = AM_FULL;                                         \
+    BODY;                                                               \
+  }
+
 #define EP_ATT_DISPATCH(dk, BODY)                      \
   switch (dk) {                                        \
     case 4: { constexpr int DK = 4; BODY; } break;     \
@@ -1055,16 +1075,13 @@ bool att_shape_supported(int heads, int dk, int dm) {
   }
 
 hipError_t launch_att_forward(const StepConfig& c, StepState* st, hipStream_t s) {
-  const size_t lds = att_lds_floats(c.att_dm, c.att_heads, false) * 4;
+  const size_t lds = att_lds_floats(c.att_dm, c.att_heads, false, c.att_rkind != 0) * 4;
   hipLaunchKernelGGL(k_att_map, dim3((c.batch + NW - 1) / NW), dim3(256), 0, s, c, st);
   EP_ATT_DISPATCH(c.att_dk, {
-    if (c.att_rkind) {
-      if (c.bf16) hipLaunchKernelGGL((k_att_forward<DK, uint16_t, true>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
-      else hipLaunchKernelGGL((k_att_forward<DK, float, true>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
-    } else {
-      if (c.bf16) hipLaunchKernelGGL((k_att_forward<DK, uint16_t, false>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
-      else hipLaunchKernelGGL((k_att_forward<DK, float, false>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
-    }
+    EP_ATT_MODE(c.att_rkind, c.att_win, {
+      if (c.bf16) hipLaunchKernelGGL((k_att_forward<DK, uint16_t, AM>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
+      else hipLaunchKernelGGL((k_att_forward<DK, float, AM>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
+    });
   });
   hipLaunchKernelGGL(k_att_loss, dim3((c.batch + NW - 1) / NW), dim3(256), 0, s, c, st);
   return hipGetLastError();
@@ -1072,29 +1089,38 @@ hipError_t launch_att_forward(const StepConfig& c, StepState* st, hipStream_t s)
 
 hipError_t launch_att_embed(const StepConfig& c, const uint64_t* findptr, const uint32_t* fids, uint64_t n_nodes,
                             float* out, hipStream_t s) {
-  const size_t lds = att_lds_floats(c.att_dm, c.att_heads, false) * 4;
+  const size_t lds = att_lds_floats(c.att_dm, c.att_heads, false, false) * 4;
   const unsigned grid = (unsigned)std::min<uint64_t>(n_nodes, 1u << 20);
+  // inference: Full or Sliding (AttentionAggregator has no Random attention, lib.rs:2028-2033)
   EP_ATT_DISPATCH(c.att_dk, {
-    if (c.bf16)
-      hipLaunchKernelGGL((k_att_embed<DK, uint16_t>), dim3(grid), dim3(64), lds, s, (const uint16_t*)c.table,
-                         (uint32_t)c.dims, c.att_heads, c.att_dm, findptr, fids, n_nodes, c.att_win, out);
-    else
-      hipLaunchKernelGGL((k_att_embed<DK, float>), dim3(grid), dim3(64), lds, s, (const float*)c.table,
-                         (uint32_t)c.dims, c.att_heads, c.att_dm, findptr, fids, n_nodes, c.att_win, out);
+    if (c.att_win != 0xFFFFFFFFu) {
+      constexpr int AM = AM_SLIDING;
+      if (c.bf16)
+        hipLaunchKernelGGL((k_att_embed<DK, uint16_t, AM>), dim3(grid), dim3(64), lds, s, (const uint16_t*)c.table,
+                           (uint32_t)c.dims, c.att_heads, c.att_dm, findptr, fids, n_nodes, c.att_win, out);
+      else
+        hipLaunchKernelGGL((k_att_embed<DK, float, AM>), dim3(grid), dim3(64), lds, s, (const float*)c.table,
+                           (uint32_t)c.dims, c.att_heads, c.att_dm, findptr, fids, n_nodes, c.att_win, out);
+    } else {
+      constexpr int AM = AM_FULL;
+      if (c.bf16)
+        hipLaunchKernelGGL((k_att_embed<DK, uint16_t, AM>), dim3(grid), dim3(64), lds, s, (const uint16_t*)c.table,
+                           (uint32_t)c.dims, c.att_heads, c.att_dm, findptr, fids, n_nodes, c.att_win, out);
+      else
+        hipLaunchKernelGGL((k_att_embed<DK, float, AM>), dim3(grid), dim3(64), lds, s, (const float*)c.table,
+                           (uint32_t)c.dims, c.att_heads, c.att_dm, findptr, fids, n_nodes, c.att_win, out);
+    }
   });
   return hipGetLastError();
 }
 
 hipError_t launch_att_backward(const StepConfig& c, StepState* st, hipStream_t s) {
-  const size_t lds = att_lds_floats(c.att_dm, c.att_heads, true) * 4;
+  const size_t lds = att_lds_floats(c.att_dm, c.att_heads, true, c.att_rkind != 0) * 4;
   EP_ATT_DISPATCH(c.att_dk, {
-    if (c.att_rkind) {
-      if (c.bf16) hipLaunchKernelGGL((k_att_backward<DK, uint16_t, true>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
-      else hipLaunchKernelGGL((k_att_backward<DK, float, true>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
-    } else {
-      if (c.bf16) hipLaunchKernelGGL((k_att_backward<DK, uint16_t, false>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
-      else hipLaunchKernelGGL((k_att_backward<DK, float, false>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
-    }
+    EP_ATT_MODE(c.att_rkind, c.att_win, {
+      if (c.bf16) hipLaunchKernelGGL((k_att_backward<DK, uint16_t, AM>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
+      else hipLaunchKernelGGL((k_att_backward<DK, float, AM>), dim3(c.blk_cap), dim3(64), lds, s, c, st);
+    });
   });
   return hipGetLastError();
 }
