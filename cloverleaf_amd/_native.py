@@ -119,6 +119,11 @@ SIGNATURES = {
     "ep_aligner_run": (c_int, [c_void_p, c_float, c_uint32, c_uint64, c_uint64, _f32p]),
     "ep_aligner_destroy": (None, [c_void_p]),
     "ep_aligner_last_kernel_ms": (c_float, [c_void_p]),
+    "ep_graph_load_tsv": (c_int, [c_char_p, c_int, c_int, c_int, c_uint64, POINTER(c_void_p)]),
+    "ep_graph_file_sizes": (c_int, [c_void_p, _u64p, _u64p, _u64p]),
+    "ep_graph_file_copy": (c_int, [c_void_p, _u64p, _u32p, _f32p, c_char_p]),
+    "ep_graph_file_free": (None, [c_void_p]),
+    "ep_graph_file_error": (c_char_p, []),
     "ep_sample_weights": (c_int, [c_void_p, c_uint32, _f32p]),
     "ep_set_explicit_weights": (c_int, [c_void_p, c_uint32, _f32p]),
     "ep_set_features": (c_int, [c_void_p, c_uint64, c_uint32, _u64p, _u32p]),
@@ -415,6 +420,32 @@ class Context:
         check(load().ep_embed_weighted(self.handle, n, ptr(f_indptr, c_uint64), ptr(f_ids, c_uint32), ptr(p_w, c_float),
                                        len(p_w), float(alpha), ptr(out, c_float)), "ep_embed_weighted")
         return out
+
+
+def load_graph_tsv(path: str, undirected: bool, weighted: bool = True, deduplicate: bool = False,
+                   skip_rows: int = 0):
+    """The native graph file loader (GraphReader::load).  Returns (indptr u64, indices u32,
+    per-row CDF f32, [(node_type, name)] in id order)."""
+    lib = load()
+    h = c_void_p()
+    rc = lib.ep_graph_load_tsv(os.fsencode(path), int(bool(undirected)), int(bool(weighted)),
+                               int(bool(deduplicate)), int(skip_rows), ctypes.byref(h))
+    if rc != 0:
+        raise ValueError(lib.ep_graph_file_error().decode(errors="replace"))
+    try:
+        n, e, nb = c_uint64(), c_uint64(), c_uint64()
+        check(lib.ep_graph_file_sizes(h, ctypes.byref(n), ctypes.byref(e), ctypes.byref(nb)), "ep_graph_file_sizes")
+        indptr = np.zeros(n.value + 1, dtype=np.uint64)
+        indices = np.zeros(max(e.value, 1), dtype=np.uint32)
+        cdf = np.zeros(max(e.value, 1), dtype=np.float32)
+        names = ctypes.create_string_buffer(max(nb.value, 1))
+        check(lib.ep_graph_file_copy(h, ptr(indptr, c_uint64), ptr(indices, c_uint32), ptr(cdf, c_float), names),
+              "ep_graph_file_copy")
+        text = names.raw[:nb.value].decode("utf-8", errors="surrogateescape")
+        pairs = [tuple(line.split("\t", 1)) for line in text.split("\n")[:-1]]
+        return indptr, indices[:e.value], cdf[:e.value], pairs
+    finally:
+        lib.ep_graph_file_free(h)
 
 
 def philox4x32(ctr, key):

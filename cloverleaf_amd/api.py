@@ -173,30 +173,14 @@ class Graph:
     @staticmethod
     def load(path: str, edge_type: str, chunk_size: Optional[int] = None, skip_rows: Optional[int] = None,
              weighted: Optional[bool] = None, deduplicate: Optional[bool] = None) -> "Graph":
-        """Graph.load (src/lib.rs:472-500, GraphReader::load io.rs:270-323): five tab-separated
-        fields per line, from_type, from, to_type, to, weight (ignored unless weighted,
-        default true); Undirected adds the reverse edge; deduplicate defaults to false."""
-        weighted = True if weighted is None else bool(weighted)
-        b = GraphBuilder()
-        with _open_text(path) as fh:
-            for i, line in enumerate(fh):
-                if i < (skip_rows or 0):
-                    continue
-                pieces = line.rstrip("\n").split("\t")
-                if len(pieces) != 5:
-                    raise ValueError(f"{i}: Malformed graph file: Expected 5 fields!")
-                w = 1.0
-                if weighted:
-                    try:
-                        w = float(np.float32(pieces[4]))
-                    except ValueError as e:
-                        raise ValueError(f"{i}: Malformed graph file! {e} - {pieces[4]!r}") from None
-                b.add_edge((pieces[0], pieces[1]), (pieces[2], pieces[3]), w, edge_type)
-        vocab = b.vocab
-        g = b.build_graph(deduplicate=False if deduplicate is None else bool(deduplicate))
-        if g is None:
-            return Graph(np.zeros(len(vocab) + 1, dtype=np.uint64), np.zeros(0, dtype=np.uint32), None, vocab)
-        return g
+        """Graph.load (src/lib.rs:472-500, GraphReader::load io.rs:270-323) in the native
+        loader: five tab-separated fields per line, from_type, from, to_type, to, weight
+        (ignored unless weighted, default true); Undirected adds the reverse edge;
+        deduplicate defaults to false; gzip for `.gz` paths."""
+        indptr, indices, cdf, names = N.load_graph_tsv(
+            path, edge_type == EdgeType.Undirected, True if weighted is None else bool(weighted),
+            False if deduplicate is None else bool(deduplicate), int(skip_rows or 0))
+        return Graph(indptr, indices, cdf, Vocab(names))
 
 
 def _uniform_cdf(k: int) -> np.ndarray:
@@ -215,7 +199,7 @@ def _rows_to_cdf(indptr: np.ndarray, weights: np.ndarray) -> np.ndarray:
         if s == e:
             continue
         w = out[s:e]
-        denom = np.float32(w.sum(dtype=np.float32))
+        denom = np.cumsum(w, dtype=np.float32)[-1]  # sequential f32 sum, as iter().sum()
         if denom == 0:
             w[:] = 1.0
             denom = np.float32(e - s)

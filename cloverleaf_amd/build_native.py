@@ -21,6 +21,7 @@ BUILD = PKG / "_build"
 ROOT = PKG.parent
 
 HIP_SOURCES = ["ep_kernels.hip", "ep_attention.hip", "ep_capi.hip"]
+HOST_SOURCES = ["ep_io.cpp"]  # host-only parts of libcloverleaf_ep.so (g++, zlib)
 SYNTH_SOURCES = ["synth.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -58,9 +59,14 @@ def build_ep(verbose: bool = False) -> Path:
 
     with ThreadPoolExecutor(max_workers=len(HIP_SOURCES)) as ex:
         list(ex.map(compile_one, zip(HIP_SOURCES, objs)))
+    cxx = shutil.which("g++") or "g++"
+    for src in HOST_SOURCES:
+        obj = BUILD / (Path(src).stem + ".o")
+        _run([cxx, "-O3", "-std=c++17", "-fPIC", f"-I{ROOT / 'include'}", "-c", str(CSRC / src), "-o", str(obj)])
+        objs.append(obj)
     lib = PKG / "libcloverleaf_ep.so"
     tmp = PKG / "libcloverleaf_ep.so.tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-lz"])
     os.replace(tmp, lib)
     return lib
 

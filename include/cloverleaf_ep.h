@@ -270,6 +270,22 @@ void ep_aligner_destroy(ep_aligner* a);
 /* Measurement hook: duration (ms, HIP events) of the last run's alignment kernel.  */
 float ep_aligner_last_kernel_ms(const ep_aligner* a);
 
+/* Graph file loader (host): GraphReader::load (src/io.rs:270-323) with
+ * CSR::construct_from_edges (graph.rs:70-142) and CumCSR::convert (graph.rs:414-422).
+ * Lines "from_type\tfrom\tto_type\tto\tweight" (gzip for a ".gz" path); the first
+ * skip_rows lines are skipped; undirected adds every edge reversed; weighted parses the
+ * weight (else 1); deduplicate sorts by (from, to) and adds repeated edges' weights.
+ * Node ids follow first appearance.  sizes, then copy: indptr[n+1], indices[E], the
+ * per-row CDF cdf[E] and names = "type\tname\n" per node in id order.  Errors:
+ * a negative EP_E* code, message in ep_graph_file_error().                       */
+typedef struct ep_graph_file ep_graph_file;
+int ep_graph_load_tsv(const char* path, int undirected, int weighted, int deduplicate, uint64_t skip_rows,
+                      ep_graph_file** out);
+int ep_graph_file_sizes(const ep_graph_file* g, uint64_t* n_nodes, uint64_t* n_entries, uint64_t* name_bytes);
+int ep_graph_file_copy(const ep_graph_file* g, uint64_t* indptr, uint32_t* indices, float* cdf, char* names);
+void ep_graph_file_free(ep_graph_file* g);
+const char* ep_graph_file_error(void);
+
 /* Philox4x32-10 on the host (the device RNG), for known-answer tests.           */
 int ep_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
