@@ -56,6 +56,10 @@ __device__ inline f32x16 mfma32(float a, float b, f32x16 c) {
 // Row of register r (0..15) of a 32x32 MFMA result tile held by lane l.
 __device__ inline int mrow(int r, int l) { return 8 * (r >> 2) + 4 * (l >> 5) + (r & 3); }
 
+// e^x for the softmax, where x = score - row max <= 0 or -inf: 2^(x log2 e) on the hardware
+// exponential (about 1 ulp; results below 2^-126 flush to 0, next to the row max's 1).
+__device__ inline float softmax_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+
 __device__ inline float count_factor(float ci, float cj) {
   const float mm = ci * cj;
   return mm >= 1.0f ? mm : 1.0f;  // attention.rs:183-186
@@ -348,7 +352,7 @@ __device__ inline void row_softmax_stats(f32x16 (&T)[2], int NT, uint32_t n, int
     if (J >= NT) break;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const float e = expf(T[J][r] - m0);  // masked scores give exp(-inf) = 0
+      const float e = softmax_exp(T[J][r] - m0);  // masked scores give exp(-inf) = 0
       T[J][r] = e;
       z += e;
     }
@@ -417,23 +421,20 @@ __device__ inline void att_block_output(const T* __restrict__ table, uint32_t W,
   }
   const float isq = 1.0f / sqrtf((float)DK);
   const int NT = (int)((n + 31u) / 32u);
-  float qa[2][16], ka[2][16], qn[2][16], kn[2][16];
-#pragma unroll
-  for (int I = 0; I < 2; ++I) {
-    const uint32_t i = 32u * I + c32;
-    const bool ok = I < NT && i < n;
-    load_qk16<DK, T>(table + (size_t)(ok ? L.f[i] : 0u) * W, ok, 0, H, hf, qa[I], ka[I]);
-  }
+  // Each head's Q / K operands load at the head's start: prefetching the next head's
+  // holds 64 more VGPRs and drops the kernel from 3 to 2 waves per SIMD, which measured
+  // slower.
+  float qa[2][16], ka[2][16];
   for (int h = 0; h < H; ++h) {
     // Random attention: this head's row masks and item scale (scores scale by its square)
     float hs = 1.f;
     if constexpr (AM == AM_RANDOM) hs = att_random_head(L, h, n);
     const float isq_h = isq * hs * hs;
 #pragma unroll
-    for (int I = 0; I < 2; ++I) {  // next head's operands in flight
+    for (int I = 0; I < 2; ++I) {
       const uint32_t i = 32u * I + c32;
-      const bool ok = h + 1 < H && I < NT && i < n;
-      load_qk16<DK, T>(table + (size_t)(ok ? L.f[i] : 0u) * W, ok, h + 1, H, hf, qn[I], kn[I]);
+      const bool ok = I < NT && i < n;
+      load_qk16<DK, T>(table + (size_t)(ok ? L.f[i] : 0u) * W, ok, h, H, hf, qa[I], ka[I]);
     }
     // softmax statistics per query row from the K Q^T tiles
 #pragma unroll
@@ -472,17 +473,13 @@ __device__ inline void att_block_output(const T* __restrict__ table, uint32_t W,
         for (int r = 0; r < 16; ++r) {
           const uint32_t i = 32u * I + mrow(r, l);
           const float sc = a[r] * count_factor(L.cnt[i], cj) * isq_h;
-          wj += att_pair<AM>(L, i, j, n) ? expf(sc - L.M[i]) * L.iZ[i] : 0.f;
+          wj += att_pair<AM>(L, i, j, n) ? softmax_exp(sc - L.M[i]) * L.iZ[i] : 0.f;
         }
       }
       wj += __shfl_xor(wj, 32, EP_WAVE);
       if (hf == 0) L.w[j * H + h] = j < n ? wj * hs : 0.f;
     }
-    __syncthreads();
-#pragma unroll
-    for (int I = 0; I < 2; ++I)
-#pragma unroll
-      for (int t = 0; t < 16; ++t) { qa[I][t] = qn[I][t]; ka[I][t] = kn[I][t]; }
+    __syncthreads();  // the next head rewrites the row statistics read above
   }
   // value sweep: lane l covers 8 consecutive values p of an item's value part
   const int VP = H * dm;
@@ -1018,7 +1015,7 @@ __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st
         for (int r = 0; r < 16; ++r) {
           const uint32_t i = 32u * I + mrow(r, l);
           const float f = count_factor(L.cnt[i], cj) * isq_h;
-          const float A = att_pair<AM>(L, i, j, n) ? expf(a[r] * f - L.M[i]) * L.iZ[i] : 0.f;
+          const float A = att_pair<AM>(L, i, j, n) ? softmax_exp(a[r] * f - L.M[i]) * L.iZ[i] : 0.f;
           wj += A;
           a[r] = A * (uj - L.r[i]) * f;  // dT_ij
         }
