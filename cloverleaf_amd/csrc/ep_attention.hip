@@ -291,37 +291,116 @@ __device__ inline bool att_setup(const StepConfig& c, const StepState* st, uint3
   return true;
 }
 
-// Q_h and K_h values of item i for lane half hf: elements 16 hf .. 16 hf + 15 (zero past
-// d_k and for padded items).
+// Elements 16 hf .. 16 hf + 15 of one head's d_k-wide part (Q_h or K_h) of an item for
+// lane half hf as FP32 (zero past d_k and for padded items).
 template <int DK, typename T>
-__device__ inline void load_qk16(const T* __restrict__ row, bool valid, int h, int H, int hf, float (&q)[16],
-                                 float (&k)[16]) {
+__device__ inline void load16(const T* __restrict__ part, bool valid, int hf, float (&x)[16]) {
 #pragma unroll
-  for (int t = 0; t < 16; ++t) q[t] = k[t] = 0.f;
+  for (int t = 0; t < 16; ++t) x[t] = 0.f;
   if (!valid) return;
   const int k0 = 16 * hf;
   if constexpr (DK % 8 == 0) {
 #pragma unroll
     for (int t = 0; t < 16; t += 8) {
       if (k0 + t < DK) {
-        float a[8], b[8];
-        ld8<T>(row + h * DK + k0 + t, a);
-        ld8<T>(row + H * DK + h * DK + k0 + t, b);
+        float a[8];
+        ld8<T>(part + k0 + t, a);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) { q[t + e] = a[e]; k[t + e] = b[e]; }
+        for (int e = 0; e < 8; ++e) x[t + e] = a[e];
       }
     }
   } else {
 #pragma unroll
     for (int t = 0; t < 16; t += 4) {
       if (k0 + t < DK) {
-        const float4 a = ld4<T>(row + h * DK + k0 + t);
-        const float4 b = ld4<T>(row + H * DK + h * DK + k0 + t);
-        q[t] = a.x; q[t + 1] = a.y; q[t + 2] = a.z; q[t + 3] = a.w;
-        k[t] = b.x; k[t + 1] = b.y; k[t + 2] = b.z; k[t + 3] = b.w;
+        const float4 a = ld4<T>(part + k0 + t);
+        x[t] = a.x; x[t + 1] = a.y; x[t + 2] = a.z; x[t + 3] = a.w;
       }
     }
   }
+}
+
+// Score-tile operand of one lane: elements 16 hf .. 16 hf + 15 of an item's Q_h or K_h
+// (zero past d_k and for padded items).  FP32 table: 16 floats, the A / B values of 16
+// v_mfma_f32_32x32x2f32.  BF16 table: the same 16 values as 8 packed pairs, the operands
+// of 2 v_mfma_f32_32x32x16_bf16; the values are bf16 already, so every product is the
+// same exact FP32 number and only the FP32 accumulation order differs.
+template <typename T>
+struct QKop;
+template <>
+struct QKop<float> {
+  float v[16];
+  __device__ float at(int t) const { return v[t]; }
+};
+template <>
+struct QKop<uint16_t> {
+  uint32_t v[8];
+  __device__ float at(int t) const {
+    return __uint_as_float((t & 1) ? (v[t >> 1] & 0xffff0000u) : (v[t >> 1] << 16));
+  }
+};
+
+template <int DK>
+__device__ inline void load_op(const float* __restrict__ part, bool valid, int hf, QKop<float>& x) {
+  load16<DK, float>(part, valid, hf, x.v);
+}
+template <int DK>
+__device__ inline void load_op(const uint16_t* __restrict__ part, bool valid, int hf, QKop<uint16_t>& x) {
+#pragma unroll
+  for (int e = 0; e < 8; ++e) x.v[e] = 0u;
+  if (!valid) return;
+  const int k0 = 16 * hf;
+  if constexpr (DK % 8 == 0) {
+#pragma unroll
+    for (int t = 0; t < 16; t += 8) {
+      if (k0 + t < DK) {
+        const uint4 r = *reinterpret_cast<const uint4*>(part + k0 + t);
+        x.v[t / 2] = r.x; x.v[t / 2 + 1] = r.y; x.v[t / 2 + 2] = r.z; x.v[t / 2 + 3] = r.w;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < 16; t += 4) {
+      if (k0 + t < DK) {
+        const uint2 r = *reinterpret_cast<const uint2*>(part + k0 + t);
+        x.v[t / 2] = r.x; x.v[t / 2 + 1] = r.y;
+      }
+    }
+  }
+}
+
+// Q_h and K_h operands of item i for lane half hf.
+template <int DK, typename T>
+__device__ inline void load_qk(const T* __restrict__ row, bool valid, int h, int H, int hf, QKop<T>& q,
+                               QKop<T>& k) {
+  load_op<DK>(row + h * DK, valid, hf, q);
+  load_op<DK>(row + H * DK + h * DK, valid, hf, k);
+}
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// acc + A B^T for 32 x d_k tiles A and B (row: lane % 32 of the operand).  Lane half hf
+// holds k-slots 16 hf .. 16 hf + 15 of both operands, the same slots in both, so any
+// assignment of slots to the instructions' k positions gives the same sum; slots at or
+// past d_k are zero and are skipped.
+template <int DK>
+__device__ inline f32x16 score_tile(const QKop<float>& a, const QKop<float>& b, f32x16 acc) {
+  constexpr int TN = DK < 16 ? DK : 16;
+#pragma unroll
+  for (int t = 0; t < TN; ++t) acc = mfma32(a.v[t], b.v[t], acc);
+  return acc;
+}
+template <int DK>
+__device__ inline f32x16 score_tile(const QKop<uint16_t>& a, const QKop<uint16_t>& b, f32x16 acc) {
+  constexpr int SN = DK <= 8 ? 1 : 2;
+#pragma unroll
+  for (int s = 0; s < SN; ++s) {
+    const uint4 ra = make_uint4(a.v[4 * s], a.v[4 * s + 1], a.v[4 * s + 2], a.v[4 * s + 3]);
+    const uint4 rb = make_uint4(b.v[4 * s], b.v[4 * s + 1], b.v[4 * s + 2], b.v[4 * s + 3]);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, ra), __builtin_bit_cast(bf16x8, rb),
+                                                  acc, 0, 0, 0);
+  }
+  return acc;
 }
 
 // Row statistics of query tile I from the K Q^T tiles: scaled, masked scores into T,
@@ -407,7 +486,7 @@ __global__ __launch_bounds__(256) void k_att_map(StepConfig c, StepState* st) {
 // ------------------------------------------------------------------------------------
 // Output of one attention block whose n items (features f, counts cnt) are in LDS:
 // per head scores on the matrix cores, softmax statistics and column sums w_jh (LDS),
-// the next head's Q / K loaded meanwhile; then one sweep over the items' value parts
+// each head's Q / K loaded at its start; then one sweep over the items' value parts
 // (all heads of an item are contiguous): out = sum_j sum_h w_jh V_jh / (n H).
 template <int DK, typename T, int AM>
 __device__ inline void att_block_output(const T* __restrict__ table, uint32_t W, int H, int dm, const AttLds& L,
@@ -424,7 +503,7 @@ __device__ inline void att_block_output(const T* __restrict__ table, uint32_t W,
   // Each head's Q / K operands load at the head's start: prefetching the next head's
   // holds 64 more VGPRs and drops the kernel from 3 to 2 waves per SIMD, which measured
   // slower.
-  float qa[2][16], ka[2][16];
+  QKop<T> qa[2], ka[2];
   for (int h = 0; h < H; ++h) {
     // Random attention: this head's row masks and item scale (scores scale by its square)
     float hs = 1.f;
@@ -434,7 +513,7 @@ __device__ inline void att_block_output(const T* __restrict__ table, uint32_t W,
     for (int I = 0; I < 2; ++I) {
       const uint32_t i = 32u * I + c32;
       const bool ok = I < NT && i < n;
-      load_qk16<DK, T>(table + (size_t)(ok ? L.f[i] : 0u) * W, ok, h, H, hf, qa[I], ka[I]);
+      load_qk<DK, T>(table + (size_t)(ok ? L.f[i] : 0u) * W, ok, h, H, hf, qa[I], ka[I]);
     }
     // softmax statistics per query row from the K Q^T tiles
 #pragma unroll
@@ -444,10 +523,7 @@ __device__ inline void att_block_output(const T* __restrict__ table, uint32_t W,
 #pragma unroll
       for (int J = 0; J < 2; ++J) {
         if (J >= NT) break;
-        f32x16 a = {};
-#pragma unroll
-        for (int t = 0; t < 16; ++t) a = mfma32(ka[J][t], qa[I][t], a);
-        Tt[J] = a;
+        Tt[J] = score_tile<DK>(ka[J], qa[I], f32x16{});
       }
       const uint32_t i = 32u * I + c32;
       const float ci = i < n ? L.cnt[i] : 0.f;
@@ -466,9 +542,7 @@ __device__ inline void att_block_output(const T* __restrict__ table, uint32_t W,
 #pragma unroll
       for (int I = 0; I < 2; ++I) {
         if (I >= NT) break;
-        f32x16 a = {};
-#pragma unroll
-        for (int t = 0; t < 16; ++t) a = mfma32(qa[I][t], ka[J][t], a);
+        const f32x16 a = score_tile<DK>(qa[I], ka[J], f32x16{});
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           const uint32_t i = 32u * I + mrow(r, l);
@@ -905,17 +979,19 @@ __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st
     float hs = 1.f;  // Random attention: this head's row masks and item scale
     if constexpr (AM == AM_RANDOM) hs = att_random_head(L, h, n);
     const float isq_h = isq * hs * hs;
-    float qa[2][16], ka[2][16];
+    QKop<T> qa[2], ka[2];
 #pragma unroll
     for (int I = 0; I < 2; ++I) {
       const uint32_t i = 32u * I + c32;
       const bool ok = I < NT && i < n;
-      load_qk16<DK, T>(table + (size_t)(ok ? L.f[i] : 0u) * W, ok, h, H, hf, qa[I], ka[I]);
+      load_qk<DK, T>(table + (size_t)(ok ? L.f[i] : 0u) * W, ok, h, H, hf, qa[I], ka[I]);
       if (I < NT) {
 #pragma unroll
         for (int t = 0; t < 16; t += 2) {
-          *reinterpret_cast<float2*>(L.Q + (size_t)i * kTileStride + 16 * hf + t) = make_float2(qa[I][t], qa[I][t + 1]);
-          *reinterpret_cast<float2*>(L.K + (size_t)i * kTileStride + 16 * hf + t) = make_float2(ka[I][t], ka[I][t + 1]);
+          *reinterpret_cast<float2*>(L.Q + (size_t)i * kTileStride + 16 * hf + t) =
+              make_float2(qa[I].at(t), qa[I].at(t + 1));
+          *reinterpret_cast<float2*>(L.K + (size_t)i * kTileStride + 16 * hf + t) =
+              make_float2(ka[I].at(t), ka[I].at(t + 1));
         }
       }
     }
@@ -954,10 +1030,7 @@ __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st
       #pragma unroll
       for (int J = 0; J < 2; ++J) {
         if (J >= NT) break;
-        f32x16 a = {};
-#pragma unroll
-        for (int t = 0; t < 16; ++t) a = mfma32(ka[J][t], qa[I][t], a);
-        Tt[J] = a;
+        Tt[J] = score_tile<DK>(ka[J], qa[I], f32x16{});
       }
       const uint32_t i = 32u * I + c32;
       const float ci = i < n ? L.cnt[i] : 0.f;
@@ -1008,9 +1081,7 @@ __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st
       #pragma unroll
       for (int I = 0; I < 2; ++I) {
         if (I >= NT) break;
-        f32x16 a = {};
-#pragma unroll
-        for (int t = 0; t < 16; ++t) a = mfma32(qa[I][t], ka[J][t], a);
+        f32x16 a = score_tile<DK>(qa[I], ka[J], f32x16{});
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           const uint32_t i = 32u * I + mrow(r, l);
