@@ -292,10 +292,12 @@ def main():
     roof = None
     if w.att_dk and ph["forward"] >= ph["short_update"]:
         # the attention forward + backward (scores, softmax, weighted values, their
-        # adjoints) on the FP32 vector ALUs, against the FP32 dense peak
+        # adjoints) on matrix cores and vector ALUs, against the FP32 dense peak
         tf = head["attention_tflops"]
+        mm = ("score tiles on BF16 MFMA, dQ / dK on FP32 MFMA, softmax and values on FP32 vector ALUs; "
+              "priced at the FP32 peak" if head_form.startswith("bf16") else "FP32 MFMA and vector ALUs")
         roof = {"bound": "mfma", "kernel": "k_att_forward + k_att_loss + k_att_backward (attention blocks: "
-                                           "scores, softmax, values and their adjoints, FP32)",
+                                           f"scores, softmax, values and their adjoints; {mm})",
                 "achieved": round(tf, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(tf / FP32_PEAK_TFLOPS, 4), "traffic": None,
                 "algorithmic_flops_per_launch": round(head["attention_flops_per_step"])}
