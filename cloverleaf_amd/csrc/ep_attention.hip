@@ -60,6 +60,42 @@ __device__ inline int mrow(int r, int l) { return 8 * (r >> 2) + 4 * (l >> 5) + 
 // exponential (about 1 ulp; results below 2^-126 flush to 0, next to the row max's 1).
 __device__ inline float softmax_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
 
+// 4 x 4 transpose inside every group of 4 lanes (DPP quad permutes, all lanes active):
+// lane 4k + m ends with x[j] = what lane 4k + j held in x[m].
+template <int CTRL>
+__device__ inline float dpp_quad(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+__device__ inline void quad_transpose(float& x0, float& x1, float& x2, float& x3) {
+  const uint32_t m = lane_id() & 3u;
+  const bool b0 = (m & 1u) != 0u, b1 = (m & 2u) != 0u;
+  // swap across lane bit 0 (quad_perm [1,0,3,2]) the registers whose bit 0 differs
+  float t0 = dpp_quad<0xB1>(b0 ? x0 : x1);
+  float t1 = dpp_quad<0xB1>(b0 ? x2 : x3);
+  if (b0) { x0 = t0; x2 = t1; } else { x1 = t0; x3 = t1; }
+  // then across lane bit 1 (quad_perm [2,3,0,1]) the registers whose bit 1 differs
+  t0 = dpp_quad<0x4E>(b1 ? x0 : x2);
+  t1 = dpp_quad<0x4E>(b1 ? x1 : x3);
+  if (b1) { x0 = t0; x1 = t1; } else { x2 = t0; x3 = t1; }
+}
+
+// Store columns [col0, col0 + d_k) of a 32 x 32 result tile (rows: item 32 T + row) into the
+// gradient rows of its items: registers 4q .. 4q + 3 hold rows 8q + 4 hf + 0..3 of column
+// l % 32; after a quad transpose lane 4k + m holds row 8q + 4 hf + m, columns 4k .. 4k + 3,
+// one 16-byte store (row offsets are multiples of 4 floats).
+template <int DK, typename RowFn>
+__device__ inline void store_tile_cols(f32x16 v, int T, uint32_t n, uint32_t col0, RowFn slot_row) {
+  const int l = (int)lane_id(), hf = l >> 5, c32 = l & 31;
+  const int col = c32 & ~3;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    float x0 = v[4 * q], x1 = v[4 * q + 1], x2 = v[4 * q + 2], x3 = v[4 * q + 3];
+    quad_transpose(x0, x1, x2, x3);
+    const uint32_t r = 32u * (uint32_t)T + 8u * (uint32_t)q + 4u * (uint32_t)hf + (uint32_t)(c32 & 3);
+    if (col < DK && r < n) *reinterpret_cast<float4*>(slot_row(r) + col0 + col) = make_float4(x0, x1, x2, x3);
+  }
+}
+
 __device__ inline float count_factor(float ci, float cj) {
   const float mm = ci * cj;
   return mm >= 1.0f ? mm : 1.0f;  // attention.rs:183-186
@@ -1060,13 +1096,7 @@ __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st
           dq = mfma32(dT, L.K[(size_t)j * kTileStride + c32], dq);
         }
       }
-      if (c32 < DK) {
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const uint32_t ir = 32u * I + mrow(r, l);
-          if (ir < n) slot_row(ir)[h * DK + c32] = dq[r];
-        }
-      }
+      store_tile_cols<DK>(dq, I, n, (uint32_t)(h * DK), slot_row);
     }
     __syncthreads();
     // key tiles (Q K^T layout): w_j, dT, dK = dT^T Q, dV_j = w_j g / (n H)
@@ -1094,18 +1124,20 @@ __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st
         for (int t = 0; t < 16; ++t) dk = mfma32(a[t], L.Q[(size_t)(32u * I + mrow(t, l)) * kTileStride + c32], dk);
       }
       wj += __shfl_xor(wj, 32, EP_WAVE);
-      if (c32 < DK) {
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const uint32_t jr = 32u * J + mrow(r, l);
-          if (jr < n) slot_row(jr)[H * DK + h * DK + c32] = dk[r];
-        }
-      }
+      store_tile_cols<DK>(dk, J, n, (uint32_t)(H * DK + h * DK), slot_row);
+      // dV rows, two items per pass: lane half hf writes item 32 J + jj + hf, 4 values per
+      // lane (d_model and the row offsets are multiples of 8 floats, g is 16-byte aligned)
       const uint32_t jn = min(32u, n - 32u * J);
-      for (uint32_t jj = 0; jj < jn; ++jj) {
-        const float cw = __shfl(wj, (int)jj, EP_WAVE) * scale * hs;
-        float* vr = slot_row(32u * J + jj) + qoff + (size_t)h * dm;
-        for (int t = l; t < dm; t += 64) vr[t] = cw * L.g[t];
+      for (uint32_t jj = 0; jj < jn; jj += 2u) {
+        const uint32_t jx = jj + (uint32_t)hf;
+        const float cw = __shfl(wj, (int)min(jx, 31u), EP_WAVE) * scale * hs;
+        if (jx < jn) {
+          float* vr = slot_row(32u * J + jx) + qoff + (size_t)h * dm;
+          for (int t = 4 * c32; t < dm; t += 128) {
+            const float4 g4 = *reinterpret_cast<const float4*>(L.g + t);
+            *reinterpret_cast<float4*>(vr + t) = make_float4(cw * g4.x, cw * g4.y, cw * g4.z, cw * g4.w);
+          }
+        }
       }
     }
     __syncthreads();
