@@ -1031,31 +1031,64 @@ __global__ __launch_bounds__(64) void k_att_backward(StepConfig c, StepState* st
         }
       }
     }
-    // u_j: lane pair (l, l ^ 32) splits item j's d_model values
-    #pragma unroll
-    for (int J = 0; J < 2; ++J) {
-      if (J >= NT) break;
-      const uint32_t j = 32u * J + c32;
-      float p = 0.f;
-      if (j < n) {
-        const T* vrow = table + (size_t)L.f[j] * W + qoff + (size_t)h * dm;
-        const int half = dm >> 1, d0 = hf * half;  // d_model is a multiple of 8
-        for (int t0 = 0; t0 < half; t0 += 16) {
-          float4 v4[4];
+    // u_j: 8-lane groups, item j0 + 8 e + l / 8 per group; each lane loads 16 bytes (4
+    // FP32 or 8 BF16 values), so one load instruction reads 8 contiguous 128-byte segments;
+    // two items per group and a 128-value chunk of each are in flight (padded items read
+    // row 0, unused)
+    {
+      constexpr int VE = std::is_same<T, uint16_t>::value ? 8 : 4;  // values per lane and load
+      constexpr int NU = 128 / (8 * VE);                            // loads per lane and chunk
+      const int gi = l >> 3, li = l & 7;
+      for (uint32_t j0 = 0; j0 < 32u * (uint32_t)NT; j0 += 16u) {
+        float p[2] = {0.f, 0.f};
+        for (int t0 = 0; t0 < dm; t0 += 128) {
+          float vv[2][NU][VE];
 #pragma unroll
-          for (int u = 0; u < 4; ++u)
-            v4[u] = (t0 + 4 * u < half) ? ld4<T>(vrow + d0 + t0 + 4 * u) : make_float4(0.f, 0.f, 0.f, 0.f);
+          for (int e = 0; e < 2; ++e) {
+            const uint32_t j = j0 + 8u * (uint32_t)e + (uint32_t)gi;
+            const T* vrow = table + (size_t)(j < n ? L.f[j] : 0u) * W + qoff + (size_t)h * dm;
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            if (t0 + 4 * u < half) {
-              const float* gg = L.g + d0 + t0 + 4 * u;
-              p += v4[u].x * gg[0] + v4[u].y * gg[1] + v4[u].z * gg[2] + v4[u].w * gg[3];
+            for (int u = 0; u < NU; ++u) {
+              const int t = t0 + VE * li + 8 * VE * u;  // d_model is a multiple of 8
+              const bool ok = j < n && t < dm;
+              if constexpr (VE == 8) {
+                if (ok) {
+                  ld8<T>(vrow + t, vv[e][u]);
+                } else {
+#pragma unroll
+                  for (int x = 0; x < VE; ++x) vv[e][u][x] = 0.f;
+                }
+              } else {
+                const float4 v4 = ok ? ld4<T>(vrow + t) : make_float4(0.f, 0.f, 0.f, 0.f);
+                vv[e][u][0] = v4.x; vv[e][u][1] = v4.y; vv[e][u][2] = v4.z; vv[e][u][3] = v4.w;
+              }
+            }
+          }
+#pragma unroll
+          for (int e = 0; e < 2; ++e) {
+#pragma unroll
+            for (int u = 0; u < NU; ++u) {
+              const int t = t0 + VE * li + 8 * VE * u;
+              if (t < dm) {
+#pragma unroll
+                for (int x = 0; x < VE; x += 4) {
+                  const float4 g4 = *reinterpret_cast<const float4*>(L.g + t + x);
+                  p[e] += vv[e][u][x] * g4.x + vv[e][u][x + 1] * g4.y + vv[e][u][x + 2] * g4.z + vv[e][u][x + 3] * g4.w;
+                }
+              }
             }
           }
         }
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          float q = p[e];
+          q += __shfl_xor(q, 1, EP_WAVE);
+          q += __shfl_xor(q, 2, EP_WAVE);
+          q += __shfl_xor(q, 4, EP_WAVE);
+          const uint32_t j = j0 + 8u * (uint32_t)e + (uint32_t)gi;
+          if (li == 0) L.u[j] = j < n ? q * scale * hs : 0.f;
+        }
       }
-      p += __shfl_xor(p, 32, EP_WAVE);
-      if (hf == 0) L.u[j] = j < n ? p * scale * hs : 0.f;
     }
     __syncthreads();
     // query tiles (K Q^T layout): probabilities, r_i, dT, dQ = dT K
