@@ -124,6 +124,11 @@ SIGNATURES = {
     "ep_graph_file_copy": (c_int, [c_void_p, _u64p, _u32p, _f32p, c_char_p]),
     "ep_graph_file_free": (None, [c_void_p]),
     "ep_graph_file_error": (c_char_p, []),
+    "ep_features_load_tsv": (c_int, [c_char_p, c_char_p, _u64p, c_uint64, c_char_p, _u64p, c_uint64, c_int, c_char_p,
+                                     POINTER(c_void_p)]),
+    "ep_feature_file_sizes": (c_int, [c_void_p, _u64p, _u64p, _u64p, _u64p]),
+    "ep_feature_file_copy": (c_int, [c_void_p, _u32p, _u64p, _u32p, c_char_p, _u64p]),
+    "ep_feature_file_free": (None, [c_void_p]),
     "ep_sample_weights": (c_int, [c_void_p, c_uint32, _f32p]),
     "ep_set_explicit_weights": (c_int, [c_void_p, c_uint32, _f32p]),
     "ep_set_features": (c_int, [c_void_p, c_uint64, c_uint32, _u64p, _u32p]),
@@ -446,6 +451,59 @@ def load_graph_tsv(path: str, undirected: bool, weighted: bool = True, deduplica
         return indptr, indices[:e.value], cdf[:e.value], pairs
     finally:
         lib.ep_graph_file_free(h)
+
+
+EP_FNS_STATIC, EP_FNS_NODE_TYPE, EP_FNS_PREFIX = 0, 1, 2
+
+
+def _name_table(names):
+    """[(type, name)] -> (blob bytes, offsets u64[2n + 1]) of the loaders' name tables."""
+    parts = []
+    for t, nm in names:
+        parts.append(str(t).encode("utf-8", errors="surrogateescape"))
+        parts.append(str(nm).encode("utf-8", errors="surrogateescape"))
+    off = np.zeros(len(parts) + 1, dtype=np.uint64)
+    if parts:
+        np.cumsum(np.fromiter((len(x) for x in parts), dtype=np.uint64, count=len(parts)), out=off[1:])
+    return b"".join(parts), off
+
+
+def _split_names(blob: bytes, off: np.ndarray, n: int):
+    o = off.tolist()
+    if blob.isascii():  # byte offsets are character offsets: decode once
+        t = blob.decode("ascii")
+        return [(t[o[2 * k]:o[2 * k + 1]], t[o[2 * k + 1]:o[2 * k + 2]]) for k in range(n)]
+    return [(blob[o[2 * k]:o[2 * k + 1]].decode("utf-8", errors="surrogateescape"),
+             blob[o[2 * k + 1]:o[2 * k + 2]].decode("utf-8", errors="surrogateescape")) for k in range(n)]
+
+
+def load_features_tsv(path: str, node_names, feature_names, ns_kind: int, ns_value: str = ""):
+    """The native feature file loader (FeatureSet.load_into).  Returns (set nodes u32,
+    indptr u64, feature ids u32, [(type, name)] of the features first seen in the file,
+    whose ids follow the len(feature_names) known ones)."""
+    lib = load()
+    nb, noff = _name_table(node_names)
+    fb, foff = _name_table(feature_names)
+    h = c_void_p()
+    rc = lib.ep_features_load_tsv(os.fsencode(path), nb, ptr(noff, c_uint64), len(node_names), fb,
+                                  ptr(foff, c_uint64), len(feature_names), int(ns_kind),
+                                  str(ns_value).encode("utf-8", errors="surrogateescape"), ctypes.byref(h))
+    if rc != 0:
+        raise ValueError(lib.ep_graph_file_error().decode(errors="replace"))
+    try:
+        ns_, ni, nn, nbytes = c_uint64(), c_uint64(), c_uint64(), c_uint64()
+        check(lib.ep_feature_file_sizes(h, ctypes.byref(ns_), ctypes.byref(ni), ctypes.byref(nn),
+                                        ctypes.byref(nbytes)), "ep_feature_file_sizes")
+        nodes = np.zeros(max(ns_.value, 1), dtype=np.uint32)
+        indptr = np.zeros(ns_.value + 1, dtype=np.uint64)
+        ids = np.zeros(max(ni.value, 1), dtype=np.uint32)
+        blob = ctypes.create_string_buffer(max(nbytes.value, 1))
+        off = np.zeros(2 * nn.value + 1, dtype=np.uint64)
+        check(lib.ep_feature_file_copy(h, ptr(nodes, c_uint32), ptr(indptr, c_uint64), ptr(ids, c_uint32), blob,
+                                       ptr(off, c_uint64)), "ep_feature_file_copy")
+        return nodes[:ns_.value], indptr, ids[:ni.value], _split_names(blob.raw[:nbytes.value], off, nn.value)
+    finally:
+        lib.ep_feature_file_free(h)
 
 
 def philox4x32(ctr, key):

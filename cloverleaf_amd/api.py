@@ -52,6 +52,12 @@ class Vocab:
             self.index[key] = i
         return i
 
+    def extend_new(self, names: Sequence[FQNode]):
+        """Appends names known to be absent, ids len(self), len(self) + 1, ..."""
+        base = len(self.names)
+        self.names.extend(names)
+        self.index.update(zip(names, range(base, base + len(names))))
+
     def get(self, node: FQNode) -> Optional[int]:
         return self.index.get((str(node[0]), str(node[1])))
 
@@ -323,18 +329,24 @@ class FeatureSet:
         return [self.feature_vocab.name(f) for f in self.features[i]]
 
     def load_into(self, path: str, f_ns: Optional[FeatureNamespace] = None):
-        """FeatureSet.load_into (src/lib.rs:1579-1606): node_type<TAB>name<TAB>f1 f2 ..."""
-        self._materialize()
+        """FeatureSet.load_into (src/lib.rs:1579-1606): node_type<TAB>name<TAB>f1 f2 ... lines,
+        parsed by the library's native loader (ep_features_load_tsv).  Nodes missing from
+        the vocabulary are skipped; a later line of a node replaces its features."""
         ns = f_ns or FeatureNamespace.single("feat")
-        with _open_text(path) as fh:
-            for i, line in enumerate(fh):
-                pieces = line.rstrip("\n").split("\t")
-                if len(pieces) != 3:
-                    raise ValueError(f"Line {i + 1}:Malformed feature line! Need node_type<TAB>name<TAB>f1 f2 ...")
-                toks = [ns.parse(pieces[0], s) or ("feat", s) for s in pieces[2].split()]
-                nid = self.vocab_.get((pieces[0], pieces[1]))
-                if nid is not None:
-                    self.features[nid] = [self.feature_vocab.get_or_insert(t, n) for t, n in toks]
+        kind = {"static": N.EP_FNS_STATIC, "node_type": N.EP_FNS_NODE_TYPE, "prefix": N.EP_FNS_PREFIX}[ns.kind]
+        nodes, ip, ids, new = N.load_features_tsv(path, self.vocab_.names, self.feature_vocab.names, kind,
+                                                  ns.value or "")
+        self.feature_vocab.extend_new(new)  # ids len(feature_vocab), len + 1, ... in the loader's order
+        n = len(self.vocab_)
+        if len(nodes) == n and self._csr is None and not any(self.features):
+            # every node set by the file: keep its CSR
+            self._csr = (np.ascontiguousarray(ip, dtype=np.uint64), np.ascontiguousarray(ids, dtype=np.uint32))
+            self._csr_nodes = n
+            self.features = []
+            return
+        self._materialize()
+        for k, v in enumerate(nodes.tolist()):
+            self.features[v] = ids[int(ip[k]):int(ip[k + 1])].astype(np.int64).tolist()
 
     def fill_missing_nodes(self):
         """feature_store.rs:83-89: featureless nodes get the anonymous feature ("node", str(i))."""

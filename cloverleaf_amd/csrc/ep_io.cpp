@@ -12,6 +12,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <string_view>
 #include <unordered_map>
@@ -101,7 +102,126 @@ struct NodeKeyHash {
   }
 };
 
+// (type, name) -> id table: open addressing over a flat arena of the names' bytes, so a
+// lookup hashes two string views and allocates nothing.
+class NameTable {
+ public:
+  void reserve(size_t n) {
+    size_t cap = 16;
+    while (cap < 2 * n) cap <<= 1;
+    if (cap > slots_.size()) rehash(cap);
+  }
+  size_t size() const { return ents_.size(); }
+  // id of (t, n), or ~0u
+  uint32_t find(std::string_view t, std::string_view n) const {
+    if (slots_.empty()) return ~0u;
+    const uint64_t h = hash(t, n);
+    for (size_t i = h & mask_;; i = (i + 1) & mask_) {
+      const uint32_t e = slots_[i];
+      if (e == 0u) return ~0u;
+      if (ents_[e - 1].h == h && equal(ents_[e - 1], t, n)) return e - 1;
+    }
+  }
+  // id of (t, n), inserted with the next id when new (*fresh = true)
+  uint32_t find_or_insert(std::string_view t, std::string_view n, bool* fresh) {
+    if (2 * (ents_.size() + 1) > slots_.size()) rehash(slots_.empty() ? 16 : 2 * slots_.size());
+    const uint64_t h = hash(t, n);
+    size_t i = h & mask_;
+    for (;; i = (i + 1) & mask_) {
+      const uint32_t e = slots_[i];
+      if (e == 0u) break;
+      if (ents_[e - 1].h == h && equal(ents_[e - 1], t, n)) {
+        *fresh = false;
+        return e - 1;
+      }
+    }
+    const uint32_t id = (uint32_t)ents_.size();
+    ents_.push_back(Ent{arena_.size(), (uint32_t)t.size(), (uint32_t)n.size(), h});
+    arena_.insert(arena_.end(), t.begin(), t.end());
+    arena_.insert(arena_.end(), n.begin(), n.end());
+    slots_[i] = id + 1u;
+    *fresh = true;
+    return id;
+  }
+  std::string_view type(uint32_t id) const { return {arena_.data() + ents_[id].off, ents_[id].tlen}; }
+  std::string_view name(uint32_t id) const {
+    return {arena_.data() + ents_[id].off + ents_[id].tlen, ents_[id].nlen};
+  }
+
+ private:
+  struct Ent {
+    uint64_t off;
+    uint32_t tlen, nlen;
+    uint64_t h;
+  };
+  static uint64_t hash(std::string_view t, std::string_view n) {
+    const uint64_t a = std::hash<std::string_view>()(t), b = std::hash<std::string_view>()(n);
+    uint64_t x = a * 0x9E3779B97F4A7C15ull ^ (b + 0x632BE59BD9B4E019ull + (a << 6) + (a >> 2));
+    x ^= x >> 29;  // mix the high bits into the slot index
+    return x;
+  }
+  bool equal(const Ent& e, std::string_view t, std::string_view n) const {
+    return e.tlen == t.size() && e.nlen == n.size() &&
+           std::memcmp(arena_.data() + e.off, t.data(), t.size()) == 0 &&
+           std::memcmp(arena_.data() + e.off + e.tlen, n.data(), n.size()) == 0;
+  }
+  void rehash(size_t cap) {
+    slots_.assign(cap, 0u);
+    mask_ = cap - 1;
+    for (uint32_t id = 0; id < (uint32_t)ents_.size(); ++id) {
+      size_t i = ents_[id].h & mask_;
+      while (slots_[i] != 0u) i = (i + 1) & mask_;
+      slots_[i] = id + 1u;
+    }
+  }
+  std::vector<char> arena_;
+  std::vector<Ent> ents_;
+  std::vector<uint32_t> slots_;
+  size_t mask_ = 0;
+};
+
+// Byte length of the Unicode White_Space character (char::is_whitespace, what Rust's
+// str::split_whitespace splits on) that starts at p, 0 if none.  UTF-8: U+0009-000D, U+0020,
+// U+0085, U+00A0, U+1680, U+2000-200A, U+2028, U+2029, U+202F, U+205F, U+3000.
+size_t ws_len(const unsigned char* p, const unsigned char* end) {
+  const unsigned char c = p[0];
+  if (c == ' ' || (c >= 0x09 && c <= 0x0D)) return 1;
+  if (c == 0xC2 && end - p >= 2 && (p[1] == 0x85 || p[1] == 0xA0)) return 2;
+  if (end - p >= 3) {
+    if (c == 0xE1 && p[1] == 0x9A && p[2] == 0x80) return 3;
+    if (c == 0xE2 && p[1] == 0x80 && ((p[2] >= 0x80 && p[2] <= 0x8A) || p[2] == 0xA8 || p[2] == 0xA9 || p[2] == 0xAF))
+      return 3;
+    if (c == 0xE2 && p[1] == 0x81 && p[2] == 0x9F) return 3;
+    if (c == 0xE3 && p[1] == 0x80 && p[2] == 0x80) return 3;
+  }
+  return 0;
+}
+
+// Inserts the (type, name) pairs k = 0..n-1 of a name table, blob[off[2k], off[2k+1]) and
+// blob[off[2k+1], off[2k+2]), with ids 0..n-1 (a repeated pair is malformed).
+bool read_names(const char* blob, const uint64_t* off, uint64_t n, NameTable* out) {
+  if (n == 0) return true;
+  if (!blob || !off) return false;
+  out->reserve(n);
+  for (uint64_t k = 0; k < n; ++k) {
+    if (off[2 * k] > off[2 * k + 1] || off[2 * k + 1] > off[2 * k + 2]) return false;
+    bool fresh;
+    out->find_or_insert(std::string_view(blob + off[2 * k], off[2 * k + 1] - off[2 * k]),
+                        std::string_view(blob + off[2 * k + 1], off[2 * k + 2] - off[2 * k + 1]), &fresh);
+    if (!fresh) return false;
+  }
+  return true;
+}
+
 }  // namespace
+
+struct ep_feature_file {
+  std::vector<uint32_t> nodes;     // nodes the file set, ascending id
+  std::vector<uint64_t> indptr;    // their feature lists in ids
+  std::vector<uint32_t> ids;
+  NameTable feats;                 // known features, then those first seen in the file
+  uint64_t n_known = 0;            // new features: ids n_known, n_known + 1, ...
+};
 
 struct ep_graph_file {
   std::vector<uint64_t> indptr;
@@ -239,5 +359,123 @@ int ep_graph_file_copy(const ep_graph_file* g, uint64_t* indptr, uint32_t* indic
 void ep_graph_file_free(ep_graph_file* g) { delete g; }
 
 const char* ep_graph_file_error(void) { return g_io_err.c_str(); }
+
+int ep_features_load_tsv(const char* path, const char* node_blob, const uint64_t* node_off, uint64_t n_nodes,
+                         const char* feat_blob, const uint64_t* feat_off, uint64_t n_feats, int ns_kind,
+                         const char* ns_value, ep_feature_file** out) {
+  if (!path || !out || (ns_kind != EP_FNS_NODE_TYPE && !ns_value)) return io_fail(EP_EINVAL, "NULL argument");
+  if (ns_kind < EP_FNS_STATIC || ns_kind > EP_FNS_PREFIX) return io_fail(EP_EINVAL, "unknown feature namespace");
+  *out = nullptr;
+  if (n_nodes >= 0xFFFFFFFFull || n_feats >= 0xFFFFFFFFull) return io_fail(EP_EINVAL, "too many names");
+  auto res = std::make_unique<ep_feature_file>();
+  NameTable nodes;
+  if (!read_names(node_blob, node_off, n_nodes, &nodes) || !read_names(feat_blob, feat_off, n_feats, &res->feats))
+    return io_fail(EP_EINVAL, "malformed name table");
+  res->n_known = n_feats;
+  const std::string ns = ns_value ? ns_value : "";
+  const std::string_view nsv(ns);
+  LineReader rd;
+  if (!rd.open(path)) return io_fail(EP_EINVAL, std::string("cannot open ") + path);
+  // every accepted line appends its list to a log; the last line of a node wins
+  std::vector<uint64_t> log_start{0};
+  std::vector<uint32_t> log_ids;
+  std::vector<uint32_t> last(nodes.size(), 0xFFFFFFFFu);
+  uint64_t line_no = 0;
+  while (rd.next()) {
+    ++line_no;
+    const std::string& ln = rd.line;
+    // line.split('\t') must give exactly three pieces (lib.rs:1591-1595)
+    const size_t t1 = ln.find('\t');
+    const size_t t2 = t1 == std::string::npos ? std::string::npos : ln.find('\t', t1 + 1);
+    if (t1 == std::string::npos || t2 == std::string::npos || ln.find('\t', t2 + 1) != std::string::npos)
+      return io_fail(EP_EINVAL, "Line " + std::to_string(line_no) +
+                                    ":Malformed feature line! Need node_type<TAB>name<TAB>f1 f2 ...");
+    const std::string_view ntype(ln.data(), t1), nname(ln.data() + t1 + 1, t2 - t1 - 1);
+    const uint32_t node = nodes.find(ntype, nname);
+    if (node == ~0u) continue;  // unknown node: the line is ignored, its tokens too
+    const unsigned char* p = (const unsigned char*)ln.data() + t2 + 1;
+    const unsigned char* end = (const unsigned char*)ln.data() + ln.size();
+    while (p < end) {
+      size_t w;
+      while (p < end && (w = ws_len(p, end)) != 0) p += w;
+      if (p >= end) break;
+      const unsigned char* q = p;
+      while (q < end && ws_len(q, end) == 0) ++q;
+      const std::string_view tok((const char*)p, (size_t)(q - p));
+      p = q;
+      // FeatureNamespace::parse (lib.rs:1391-1403), ("feat", token) when it gives none
+      std::string_view ft, fn;
+      if (ns_kind == EP_FNS_STATIC) {
+        ft = nsv;
+        fn = tok;
+      } else if (ns_kind == EP_FNS_NODE_TYPE) {
+        ft = ntype;
+        fn = tok;
+      } else {
+        const size_t idx = nsv.empty() ? 0 : tok.find(nsv);
+        if (idx == std::string_view::npos) {
+          ft = "feat";
+          fn = tok;
+        } else {  // the name starts one byte past the delimiter's start, as in the reference
+          ft = tok.substr(0, idx);
+          fn = idx + 1 <= tok.size() ? tok.substr(idx + 1) : std::string_view();
+        }
+      }
+      if (res->feats.size() >= 0xFFFFFFFEull) return io_fail(EP_EINVAL, "too many features");
+      bool fresh;
+      log_ids.push_back(res->feats.find_or_insert(ft, fn, &fresh));
+    }
+    last[node] = (uint32_t)(log_start.size() - 1);
+    log_start.push_back(log_ids.size());
+  }
+  res->indptr.push_back(0);
+  for (uint32_t v = 0; v < (uint32_t)last.size(); ++v) {
+    if (last[v] == 0xFFFFFFFFu) continue;
+    const uint64_t a = log_start[last[v]], z = log_start[last[v] + 1];
+    res->nodes.push_back(v);
+    res->ids.insert(res->ids.end(), log_ids.begin() + (ptrdiff_t)a, log_ids.begin() + (ptrdiff_t)z);
+    res->indptr.push_back(res->ids.size());
+  }
+  *out = res.release();
+  return EP_OK;
+}
+
+int ep_feature_file_sizes(const ep_feature_file* f, uint64_t* n_set, uint64_t* n_ids, uint64_t* n_new,
+                          uint64_t* new_name_bytes) {
+  if (!f || !n_set || !n_ids || !n_new || !new_name_bytes) return io_fail(EP_EINVAL, "NULL argument");
+  *n_set = f->nodes.size();
+  *n_ids = f->ids.size();
+  *n_new = f->feats.size() - f->n_known;
+  uint64_t b = 0;
+  for (uint32_t id = (uint32_t)f->n_known; id < (uint32_t)f->feats.size(); ++id)
+    b += f->feats.type(id).size() + f->feats.name(id).size();
+  *new_name_bytes = b;
+  return EP_OK;
+}
+
+int ep_feature_file_copy(const ep_feature_file* f, uint32_t* nodes, uint64_t* indptr, uint32_t* ids,
+                         char* new_blob, uint64_t* new_off) {
+  if (!f || !indptr || !new_off || (!f->nodes.empty() && !nodes) || (!f->ids.empty() && !ids) ||
+      (f->feats.size() > f->n_known && !new_blob))
+    return io_fail(EP_EINVAL, "NULL argument");
+  if (!f->nodes.empty()) std::memcpy(nodes, f->nodes.data(), f->nodes.size() * sizeof(uint32_t));
+  std::memcpy(indptr, f->indptr.data(), f->indptr.size() * sizeof(uint64_t));
+  if (!f->ids.empty()) std::memcpy(ids, f->ids.data(), f->ids.size() * sizeof(uint32_t));
+  uint64_t o = 0;
+  new_off[0] = 0;
+  for (uint64_t k = 0; k + f->n_known < f->feats.size(); ++k) {
+    const uint32_t id = (uint32_t)(f->n_known + k);
+    const std::string_view t = f->feats.type(id), n = f->feats.name(id);
+    std::memcpy(new_blob + o, t.data(), t.size());
+    o += t.size();
+    new_off[2 * k + 1] = o;
+    std::memcpy(new_blob + o, n.data(), n.size());
+    o += n.size();
+    new_off[2 * k + 2] = o;
+  }
+  return EP_OK;
+}
+
+void ep_feature_file_free(ep_feature_file* f) { delete f; }
 
 }  // extern "C"

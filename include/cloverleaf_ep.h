@@ -277,7 +277,8 @@ float ep_aligner_last_kernel_ms(const ep_aligner* a);
  * weight (else 1); deduplicate sorts by (from, to) and adds repeated edges' weights.
  * Node ids follow first appearance.  sizes, then copy: indptr[n+1], indices[E], the
  * per-row CDF cdf[E] and names = "type\tname\n" per node in id order.  Errors:
- * a negative EP_E* code, message in ep_graph_file_error().                       */
+ * a negative EP_E* code, message in ep_graph_file_error() (the message of the last
+ * failed call of either file loader on this thread).                       */
 typedef struct ep_graph_file ep_graph_file;
 int ep_graph_load_tsv(const char* path, int undirected, int weighted, int deduplicate, uint64_t skip_rows,
                       ep_graph_file** out);
@@ -285,6 +286,34 @@ int ep_graph_file_sizes(const ep_graph_file* g, uint64_t* n_nodes, uint64_t* n_e
 int ep_graph_file_copy(const ep_graph_file* g, uint64_t* indptr, uint32_t* indices, float* cdf, char* names);
 void ep_graph_file_free(ep_graph_file* g);
 const char* ep_graph_file_error(void);
+
+/* Feature file loader (host): FeatureSet.load_into (src/lib.rs:1579-1606) with
+ * FeatureStore::set_features (src/feature_store.rs:36-48).  Lines
+ * "node_type\tname\tf1 f2 ..." (gzip for a ".gz" path), exactly three tab-separated
+ * fields, else an error naming the line; tokens split on Unicode whitespace
+ * (str::split_whitespace).  Each token's (type, name) follows FeatureNamespace::parse
+ * (lib.rs:1391-1403): EP_FNS_STATIC (ns_value, token), EP_FNS_NODE_TYPE (node_type,
+ * token), EP_FNS_PREFIX split at the first ns_value (name from one byte past its start),
+ * ("feat", token) when the prefix does not occur.  Lines of nodes missing from the node
+ * table are ignored, tokens included; a later line of a node replaces an earlier one.
+ * Name tables are a byte blob with offsets: entry k is (blob[off[2k], off[2k+1]),
+ * blob[off[2k+1], off[2k+2])).  Known features keep their ids; new ones get ids
+ * n_feats, n_feats + 1, ... in first-seen order.  sizes, then copy: the set nodes
+ * (ascending), their lists as indptr[n_set + 1] / ids, the new feature names with
+ * new_off[2 n_new + 1].  Errors: a negative EP_E* code, message in
+ * ep_graph_file_error().                                                        */
+#define EP_FNS_STATIC 0
+#define EP_FNS_NODE_TYPE 1
+#define EP_FNS_PREFIX 2
+typedef struct ep_feature_file ep_feature_file;
+int ep_features_load_tsv(const char* path, const char* node_blob, const uint64_t* node_off, uint64_t n_nodes,
+                         const char* feat_blob, const uint64_t* feat_off, uint64_t n_feats, int ns_kind,
+                         const char* ns_value, ep_feature_file** out);
+int ep_feature_file_sizes(const ep_feature_file* f, uint64_t* n_set, uint64_t* n_ids, uint64_t* n_new,
+                          uint64_t* new_name_bytes);
+int ep_feature_file_copy(const ep_feature_file* f, uint32_t* nodes, uint64_t* indptr, uint32_t* ids,
+                         char* new_blob, uint64_t* new_off);
+void ep_feature_file_free(ep_feature_file* f);
 
 /* Philox4x32-10 on the host (the device RNG), for known-answer tests.           */
 int ep_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
