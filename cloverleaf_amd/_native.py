@@ -26,7 +26,9 @@ EP_LOSS_MARGIN, EP_LOSS_CONTRASTIVE, EP_LOSS_STARSPACE, EP_LOSS_RANK, EP_LOSS_RA
 LOSS_KINDS = {"margin": EP_LOSS_MARGIN, "contrastive": EP_LOSS_CONTRASTIVE, "starspace": EP_LOSS_STARSPACE,
               "rank": EP_LOSS_RANK, "rankspace": EP_LOSS_RANKSPACE, "ppr": EP_LOSS_PPR}
 
-ERROR_NAMES = {-1: "EP_EINVAL", -2: "EP_ESTATE", -3: "EP_EHIP", -4: "EP_ENOMEM", -5: "EP_ECAPACITY"}
+ERROR_NAMES = {-1: "EP_EINVAL", -2: "EP_ESTATE", -3: "EP_EHIP", -4: "EP_ENOMEM", -5: "EP_ECAPACITY",
+               -6: "EP_EDUPLICATE"}
+EP_EDUPLICATE = -6
 
 
 class EPConfig(Structure):
@@ -129,6 +131,10 @@ SIGNATURES = {
     "ep_feature_file_sizes": (c_int, [c_void_p, _u64p, _u64p, _u64p, _u64p]),
     "ep_feature_file_copy": (c_int, [c_void_p, _u32p, _u64p, _u32p, c_char_p, _u64p]),
     "ep_feature_file_free": (None, [c_void_p]),
+    "ep_embeddings_load_tsv": (c_int, [c_char_p, c_char_p, _u64p, c_uint64, c_int, c_uint64, POINTER(c_void_p)]),
+    "ep_embedding_file_sizes": (c_int, [c_void_p, _u64p, _u64p, _u64p]),
+    "ep_embedding_file_copy": (c_int, [c_void_p, _f32p, c_char_p, _u64p]),
+    "ep_embedding_file_free": (None, [c_void_p]),
     "ep_sample_weights": (c_int, [c_void_p, c_uint32, _f32p]),
     "ep_set_explicit_weights": (c_int, [c_void_p, c_uint32, _f32p]),
     "ep_set_features": (c_int, [c_void_p, c_uint64, c_uint32, _u64p, _u32p]),
@@ -504,6 +510,35 @@ def load_features_tsv(path: str, node_names, feature_names, ns_kind: int, ns_val
         return nodes[:ns_.value], indptr, ids[:ni.value], _split_names(blob.raw[:nbytes.value], off, nn.value)
     finally:
         lib.ep_feature_file_free(h)
+
+
+def load_embeddings_tsv(path: str, filter_types=None, skip_rows: int = 0):
+    """The native embedding file loader (NodeEmbeddings.load).  Returns ([(type, name)] in
+    file order, FP32 table rows x dims).  A repeated node raises KeyError, other
+    malformed input ValueError."""
+    lib = load()
+    types = [] if filter_types is None else [str(t).encode("utf-8", errors="surrogateescape") for t in filter_types]
+    toff = np.zeros(len(types) + 1, dtype=np.uint64)
+    if types:
+        np.cumsum(np.fromiter((len(t) for t in types), dtype=np.uint64, count=len(types)), out=toff[1:])
+    h = c_void_p()
+    rc = lib.ep_embeddings_load_tsv(os.fsencode(path), b"".join(types), ptr(toff, c_uint64), len(types),
+                                    int(filter_types is not None), int(skip_rows), ctypes.byref(h))
+    if rc != 0:
+        msg = lib.ep_graph_file_error().decode(errors="replace")
+        raise (KeyError if rc == EP_EDUPLICATE else ValueError)(msg)
+    try:
+        n, d, nb = c_uint64(), c_uint64(), c_uint64()
+        check(lib.ep_embedding_file_sizes(h, ctypes.byref(n), ctypes.byref(d), ctypes.byref(nb)),
+              "ep_embedding_file_sizes")
+        table = np.zeros((n.value, d.value), dtype=np.float32)
+        blob = ctypes.create_string_buffer(max(nb.value, 1))
+        off = np.zeros(2 * n.value + 1, dtype=np.uint64)
+        check(lib.ep_embedding_file_copy(h, ptr(table, c_float) if table.size else None, blob, ptr(off, c_uint64)),
+              "ep_embedding_file_copy")
+        return _split_names(blob.raw[:nb.value], off, n.value), table
+    finally:
+        lib.ep_embedding_file_free(h)
 
 
 def philox4x32(ctr, key):

@@ -462,35 +462,16 @@ class NodeEmbeddings:
              skip_rows: Optional[int] = None) -> "NodeEmbeddings":
         """NodeEmbeddings.load (src/lib.rs:2868-2894, EmbeddingReader::load io.rs:135-200):
         node_type<TAB>name<TAB>[v1,...] lines, optionally only the node types in filter_type
-        (a string or a list); a repeated node or a different width is an error."""
+        (a string or a list); a repeated node or a different width is an error.  Parsed by
+        the library's native loader (ep_embeddings_load_tsv)."""
         types = None
         if filter_type is not None:
-            types = {filter_type} if isinstance(filter_type, str) else set(filter_type)
+            types = sorted({filter_type} if isinstance(filter_type, str) else set(filter_type))
+        names, table = N.load_embeddings_tsv(path, types, skip_rows or 0)
         vocab = Vocab()
-        rows: List[np.ndarray] = []
-        with _open_text(path) as fh:
-            for i, line in enumerate(fh):
-                if i < (skip_rows or 0):
-                    continue
-                line = line.rstrip("\n")
-                if types is not None:
-                    k = line.find("\t")
-                    if k < 0 or line[:k] not in types:
-                        continue
-                pieces = line.split("\t")
-                if len(pieces) != 3:
-                    raise ValueError(f"Error parsing line: {line}")
-                try:
-                    emb = np.array([np.float32(x.strip()) for x in pieces[2][1:-1].split(",")], dtype=np.float32)
-                except ValueError:
-                    raise ValueError(f"Error parsing line: {line}") from None
-                nid = vocab.get_or_insert(pieces[0], pieces[1])
-                if nid < len(rows):
-                    raise KeyError(f"found duplicate node at {len(rows)}!")
-                if rows and len(emb) != len(rows[0]):
-                    raise ValueError("Embeddings have different sizes!")
-                rows.append(emb)
-        table = np.stack(rows) if rows else np.zeros((0, 0), dtype=np.float32)
+        vocab.extend_new(names)
+        if not names:
+            table = np.zeros((0, 0), dtype=np.float32)
         return NodeEmbeddings(vocab, table, distance or Distance.Cosine)
 
 

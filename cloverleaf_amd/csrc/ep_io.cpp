@@ -223,6 +223,12 @@ struct ep_feature_file {
   uint64_t n_known = 0;            // new features: ids n_known, n_known + 1, ...
 };
 
+struct ep_embedding_file {
+  NameTable names;          // nodes in file order
+  uint64_t dims = 0;
+  std::vector<float> table;  // names.size() x dims
+};
+
 struct ep_graph_file {
   std::vector<uint64_t> indptr;
   std::vector<uint32_t> indices;
@@ -477,5 +483,100 @@ int ep_feature_file_copy(const ep_feature_file* f, uint32_t* nodes, uint64_t* in
 }
 
 void ep_feature_file_free(ep_feature_file* f) { delete f; }
+
+int ep_embeddings_load_tsv(const char* path, const char* type_blob, const uint64_t* type_off, uint64_t n_types,
+                           int filter, uint64_t skip_rows, ep_embedding_file** out) {
+  if (!path || !out || (filter && n_types && (!type_blob || !type_off))) return io_fail(EP_EINVAL, "NULL argument");
+  *out = nullptr;
+  NameTable types;  // filter types as (type, "")
+  if (filter) {
+    for (uint64_t k = 0; k < n_types; ++k) {
+      if (type_off[k] > type_off[k + 1]) return io_fail(EP_EINVAL, "malformed type table");
+      bool fresh;
+      types.find_or_insert(std::string_view(type_blob + type_off[k], type_off[k + 1] - type_off[k]), {}, &fresh);
+    }
+  }
+  LineReader rd;
+  if (!rd.open(path)) return io_fail(EP_EINVAL, std::string("cannot open ") + path);
+  auto res = std::make_unique<ep_embedding_file>();
+  std::vector<float> emb;
+  uint64_t line_no = 0;
+  while (rd.next()) {
+    if (line_no++ < skip_rows) continue;  // RecordReader skips before filtering (io.rs:99)
+    const std::string& ln = rd.line;
+    const size_t t1 = ln.find('\t');
+    // is_node_type (io.rs:226-231): the text before the first tab is a wanted type
+    if (filter && (t1 == std::string::npos || types.find(std::string_view(ln.data(), t1), {}) == ~0u)) continue;
+    // line_to_embedding (io.rs:255-268): three tab fields, "[v1,v2,...]" with the first and
+    // last byte dropped, values split at ',' and trimmed, each a whole f32 (fast_float::parse)
+    const size_t t2 = t1 == std::string::npos ? std::string::npos : ln.find('\t', t1 + 1);
+    bool ok = t2 != std::string::npos && ln.find('\t', t2 + 1) == std::string::npos && ln.size() - t2 - 1 >= 2;
+    emb.clear();
+    if (ok) {
+      const char* a = ln.data() + t2 + 2;
+      const char* z = ln.data() + ln.size() - 1;
+      while (ok) {
+        const char* c = static_cast<const char*>(std::memchr(a, ',', (size_t)(z - a)));
+        const char* e = c ? c : z;
+        const unsigned char* b = (const unsigned char*)a;
+        const unsigned char* q = (const unsigned char*)e;
+        size_t w;
+        while (b < q && (w = ws_len(b, q)) != 0) b += w;
+        while (b < q) {  // trailing White_Space: find the last character boundary
+          const unsigned char* r = q - 1;
+          while (r > b && (*r & 0xC0) == 0x80) --r;
+          if (ws_len(r, q) == (size_t)(q - r)) q = r;
+          else break;
+        }
+        float v;
+        ok = parse_f32(std::string_view((const char*)b, (size_t)(q - b)), &v);
+        if (ok) emb.push_back(v);
+        if (!c) break;
+        a = c + 1;
+      }
+    }
+    if (!ok) return io_fail(EP_EINVAL, "Error parsing line: " + ln);
+    const uint64_t i = res->names.size();
+    if (i == 0) res->dims = emb.size();
+    bool fresh;
+    res->names.find_or_insert(std::string_view(ln.data(), t1), std::string_view(ln.data() + t1 + 1, t2 - t1 - 1),
+                              &fresh);
+    if (!fresh) return io_fail(EP_EDUPLICATE, "found duplicate node at " + std::to_string(i) + "!");
+    if (emb.size() != res->dims) return io_fail(EP_EINVAL, "Embeddings have different sizes!");
+    res->table.insert(res->table.end(), emb.begin(), emb.end());
+  }
+  *out = res.release();
+  return EP_OK;
+}
+
+int ep_embedding_file_sizes(const ep_embedding_file* f, uint64_t* n_rows, uint64_t* dims, uint64_t* name_bytes) {
+  if (!f || !n_rows || !dims || !name_bytes) return io_fail(EP_EINVAL, "NULL argument");
+  *n_rows = f->names.size();
+  *dims = f->dims;
+  uint64_t b = 0;
+  for (uint32_t id = 0; id < (uint32_t)f->names.size(); ++id) b += f->names.type(id).size() + f->names.name(id).size();
+  *name_bytes = b;
+  return EP_OK;
+}
+
+int ep_embedding_file_copy(const ep_embedding_file* f, float* table, char* name_blob, uint64_t* name_off) {
+  if (!f || !name_off || (!f->table.empty() && !table) || (f->names.size() && !name_blob))
+    return io_fail(EP_EINVAL, "NULL argument");
+  if (!f->table.empty()) std::memcpy(table, f->table.data(), f->table.size() * sizeof(float));
+  uint64_t o = 0;
+  name_off[0] = 0;
+  for (uint32_t id = 0; id < (uint32_t)f->names.size(); ++id) {
+    const std::string_view t = f->names.type(id), n = f->names.name(id);
+    std::memcpy(name_blob + o, t.data(), t.size());
+    o += t.size();
+    name_off[2 * (uint64_t)id + 1] = o;
+    std::memcpy(name_blob + o, n.data(), n.size());
+    o += n.size();
+    name_off[2 * (uint64_t)id + 2] = o;
+  }
+  return EP_OK;
+}
+
+void ep_embedding_file_free(ep_embedding_file* f) { delete f; }
 
 }  // extern "C"

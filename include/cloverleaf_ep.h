@@ -31,6 +31,7 @@ enum {
   EP_EHIP = -3,        /* HIP runtime error                                     */
   EP_ENOMEM = -4,      /* device allocation failed                              */
   EP_ECAPACITY = -5,   /* a step exceeded a preallocated device capacity        */
+  EP_EDUPLICATE = -6,  /* a file names a node twice (PyKeyError in the reference) */
 };
 
 /* Sample kinds of max_nodes / max_features (src/algos/utils.rs:166-191).       */
@@ -314,6 +315,24 @@ int ep_feature_file_sizes(const ep_feature_file* f, uint64_t* n_set, uint64_t* n
 int ep_feature_file_copy(const ep_feature_file* f, uint32_t* nodes, uint64_t* indptr, uint32_t* ids,
                          char* new_blob, uint64_t* new_off);
 void ep_feature_file_free(ep_feature_file* f);
+
+/* Embedding file loader (host): NodeEmbeddings.load (src/lib.rs:2868-2894) with
+ * EmbeddingReader::load (src/io.rs:135-200).  Lines "node_type\tname\t[v1,v2,...]"
+ * (gzip for a ".gz" path); the first skip_rows lines are skipped, then, when filter is
+ * set, lines whose text before the first tab is not one of the n_types types (blob
+ * type_blob, entry k = [type_off[k], type_off[k+1])) are dropped.  A line is three tab
+ * fields; its third loses its first and last byte and splits at ',' into values trimmed
+ * of Unicode whitespace, each a whole f32 (fast_float::parse); otherwise "Error parsing
+ * line: ..." (EP_EINVAL).  A repeated node is EP_EDUPLICATE ("found duplicate node at
+ * i!"); a row of another width EP_EINVAL ("Embeddings have different sizes!").  sizes,
+ * then copy: the table (rows x dims, file order) and the node names as a blob with
+ * name_off[2 rows + 1] (entry k = (type, name)).  Messages in ep_graph_file_error().    */
+typedef struct ep_embedding_file ep_embedding_file;
+int ep_embeddings_load_tsv(const char* path, const char* type_blob, const uint64_t* type_off, uint64_t n_types,
+                           int filter, uint64_t skip_rows, ep_embedding_file** out);
+int ep_embedding_file_sizes(const ep_embedding_file* f, uint64_t* n_rows, uint64_t* dims, uint64_t* name_bytes);
+int ep_embedding_file_copy(const ep_embedding_file* f, float* table, char* name_blob, uint64_t* name_off);
+void ep_embedding_file_free(ep_embedding_file* f);
 
 /* Philox4x32-10 on the host (the device RNG), for known-answer tests.           */
 int ep_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -165,3 +165,69 @@ def test_facade_load_into(tmp_path):
     assert fs.num_features() == 3
     fs.fill_missing_nodes()
     assert fs.get_features(("n", "0")) == [("node", "0")]
+
+
+# --------------------------------------------------------------------------------------
+# The native embedding file loader (ep_embeddings_load_tsv) behind NodeEmbeddings.load
+# (src/lib.rs:2868-2894, EmbeddingReader::load io.rs:135-200, line_to_embedding 255-268,
+# is_node_type 226-231, RecordReader's skip 99)
+# --------------------------------------------------------------------------------------
+def test_embeddings_known_answers(tmp_path):
+    p = tmp_path / "e.tsv"
+    # values trimmed of Unicode whitespace; Rust float forms (sign, exponent, inf, nan)
+    write(p, "u\ta\t[1.5, -2e-3,　3]\ni\tb\t[+0.25,inf,-1E2]\nu\tc\t[NaN, 7 ,8.]\n")
+    names, t = N.load_embeddings_tsv(str(p))
+    assert names == [("u", "a"), ("i", "b"), ("u", "c")]
+    assert t.dtype == np.float32 and t.shape == (3, 3)
+    assert t[0].tolist() == [1.5, np.float32(-2e-3), 3.0]
+    assert t[1].tolist() == [0.25, np.inf, -100.0]
+    assert np.isnan(t[2, 0]) and t[2, 1:].tolist() == [7.0, 8.0]
+    # skip_rows counts lines before the type filter; lines without a tab never match a filter
+    write(p, "u\tskip\t[bad\nno-tab-here\nu\ta\t[1,2]\ni\tb\t[3,4]\nu\tc\t[5,6]\n")
+    names, t = N.load_embeddings_tsv(str(p), ["u"], skip_rows=1)
+    assert names == [("u", "a"), ("u", "c")] and t.tolist() == [[1, 2], [5, 6]]
+
+
+def test_embeddings_rounding_is_correct(tmp_path):
+    # a whole-string f32 parse rounds once (fast_float).  Just above 1 + 2^-24, the midpoint of
+    # 1 and its f32 successor: through float64 it becomes the midpoint and ties to even (1.0);
+    # rounded once it goes up.  Also the largest finite value, a subnormal, a tie to even.
+    vals = ["1.000000059604644775390625000001", "3.4028235e38", "1e-45", "16777217"]
+    p = tmp_path / "e.tsv"
+    write(p, "u\ta\t[" + ",".join(vals) + "]\n")
+    _, t = N.load_embeddings_tsv(str(p))
+    assert np.float32(float(vals[0])) == np.float32(1.0)  # the double-rounded value
+    assert t[0, 0] == np.nextafter(np.float32(1.0), np.float32(2.0))
+    assert t[0, 1] == np.finfo(np.float32).max
+    assert t[0, 2] == np.float32(2.0 ** -149)
+    assert t[0, 3] == np.float32(16777216.0)
+
+
+def test_embeddings_errors(tmp_path):
+    p = tmp_path / "e.tsv"
+    write(p, "u\ta\t[1,2]\nu\ta\t[3,4]\n")
+    with pytest.raises(KeyError, match="found duplicate node at 1"):
+        N.load_embeddings_tsv(str(p))
+    write(p, "u\ta\t[1,2]\nu\tb\t[3]\n")
+    with pytest.raises(ValueError, match="different sizes"):
+        N.load_embeddings_tsv(str(p))
+    for bad in ["u\ta\t[1,x]", "u\ta\t[]", "u\ta\t[1,,2]", "u\ta\t[1]\textra", "u\ta", "u\ta\t["]:
+        write(p, bad + "\n")
+        with pytest.raises(ValueError, match="Error parsing line"):
+            N.load_embeddings_tsv(str(p))
+
+
+@pytest.mark.parametrize("gz", [False, True])
+def test_embeddings_save_load_round_trip(tmp_path, gz):
+    rng = np.random.default_rng(int(gz))
+    vocab = C.api.Vocab([("u" if i % 3 else "i", f"n{i}") for i in range(50)])
+    table = rng.standard_normal((50, 7)).astype(np.float32)
+    table[3, 2] = np.float32(1e-7)
+    emb = C.NodeEmbeddings(vocab, table, C.Distance.Cosine)
+    p = tmp_path / ("e.tsv.gz" if gz else "e.tsv")
+    emb.save(str(p))
+    back = C.NodeEmbeddings.load(str(p))
+    assert list(back.vocab_.names) == vocab.names
+    assert np.array_equal(back.embeddings, table)
+    only_i = C.NodeEmbeddings.load(str(p), filter_type="i")
+    assert [t for t, _ in only_i.vocab_.names] == ["i"] * 17
