@@ -15,7 +15,6 @@
 #include <memory>
 #include <string>
 #include <string_view>
-#include <unordered_map>
 #include <vector>
 
 #include "cloverleaf_ep.h"
@@ -92,15 +91,6 @@ bool parse_f32(std::string_view s, float* out) {
   return true;
 }
 
-struct NodeKey {
-  std::string type, name;
-  bool operator==(const NodeKey& o) const { return type == o.type && name == o.name; }
-};
-struct NodeKeyHash {
-  size_t operator()(const NodeKey& k) const {
-    return std::hash<std::string>()(k.type) * 1000003u ^ std::hash<std::string>()(k.name);
-  }
-};
 
 // (type, name) -> id table: open addressing over a flat arena of the names' bytes, so a
 // lookup hashes two string views and allocates nothing.
@@ -233,7 +223,7 @@ struct ep_graph_file {
   std::vector<uint64_t> indptr;
   std::vector<uint32_t> indices;
   std::vector<float> cdf;
-  std::vector<NodeKey> names;
+  NameTable names;
 };
 
 extern "C" {
@@ -244,21 +234,15 @@ int ep_graph_load_tsv(const char* path, int undirected, int weighted, int dedupl
   *out = nullptr;
   LineReader rd;
   if (!rd.open(path)) return io_fail(EP_EINVAL, std::string("cannot open ") + path);
-  std::unordered_map<NodeKey, uint32_t, NodeKeyHash> vocab;
-  std::vector<NodeKey> names;
+  NameTable names;  // node ids in first-appearance order
   struct Edge {
     uint32_t from, to;
     float w;
   };
   std::vector<Edge> edges;
   auto id_of = [&](std::string_view t, std::string_view n) -> uint32_t {
-    NodeKey k{std::string(t), std::string(n)};
-    auto it = vocab.find(k);
-    if (it != vocab.end()) return it->second;
-    const uint32_t id = (uint32_t)names.size();
-    vocab.emplace(k, id);
-    names.push_back(std::move(k));
-    return id;
+    bool fresh;
+    return names.find_or_insert(t, n, &fresh);
   };
   uint64_t i = 0;
   for (; rd.next(); ++i) {
@@ -338,7 +322,7 @@ int ep_graph_file_sizes(const ep_graph_file* g, uint64_t* n_nodes, uint64_t* n_e
   *n_nodes = g->names.size();
   *n_entries = g->indices.size();
   uint64_t b = 0;
-  for (const NodeKey& k : g->names) b += k.type.size() + k.name.size() + 2;
+  for (uint32_t id = 0; id < (uint32_t)g->names.size(); ++id) b += g->names.type(id).size() + g->names.name(id).size() + 2;
   *name_bytes = b;
   return EP_OK;
 }
@@ -351,12 +335,13 @@ int ep_graph_file_copy(const ep_graph_file* g, uint64_t* indptr, uint32_t* indic
     std::memcpy(cdf, g->cdf.data(), g->cdf.size() * sizeof(float));
   }
   char* o = names;
-  for (const NodeKey& k : g->names) {
-    std::memcpy(o, k.type.data(), k.type.size());
-    o += k.type.size();
+  for (uint32_t id = 0; id < (uint32_t)g->names.size(); ++id) {
+    const std::string_view kt = g->names.type(id), kn = g->names.name(id);
+    std::memcpy(o, kt.data(), kt.size());
+    o += kt.size();
     *o++ = '\t';
-    std::memcpy(o, k.name.data(), k.name.size());
-    o += k.name.size();
+    std::memcpy(o, kn.data(), kn.size());
+    o += kn.size();
     *o++ = '\n';
   }
   return EP_OK;
