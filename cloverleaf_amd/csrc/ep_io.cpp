@@ -152,8 +152,8 @@ class NameTable {
   }
   bool equal(const Ent& e, std::string_view t, std::string_view n) const {
     return e.tlen == t.size() && e.nlen == n.size() &&
-           std::memcmp(arena_.data() + e.off, t.data(), t.size()) == 0 &&
-           std::memcmp(arena_.data() + e.off + e.tlen, n.data(), n.size()) == 0;
+           (t.empty() || std::memcmp(arena_.data() + e.off, t.data(), t.size()) == 0) &&
+           (n.empty() || std::memcmp(arena_.data() + e.off + e.tlen, n.data(), n.size()) == 0);
   }
   void rehash(size_t cap) {
     slots_.assign(cap, 0u);
@@ -474,11 +474,12 @@ int ep_embeddings_load_tsv(const char* path, const char* type_blob, const uint64
   if (!path || !out || (filter && n_types && (!type_blob || !type_off))) return io_fail(EP_EINVAL, "NULL argument");
   *out = nullptr;
   NameTable types;  // filter types as (type, "")
+  const std::string_view none("", 0);
   if (filter) {
     for (uint64_t k = 0; k < n_types; ++k) {
       if (type_off[k] > type_off[k + 1]) return io_fail(EP_EINVAL, "malformed type table");
       bool fresh;
-      types.find_or_insert(std::string_view(type_blob + type_off[k], type_off[k + 1] - type_off[k]), {}, &fresh);
+      types.find_or_insert(std::string_view(type_blob + type_off[k], type_off[k + 1] - type_off[k]), none, &fresh);
     }
   }
   LineReader rd;
@@ -491,7 +492,7 @@ int ep_embeddings_load_tsv(const char* path, const char* type_blob, const uint64
     const std::string& ln = rd.line;
     const size_t t1 = ln.find('\t');
     // is_node_type (io.rs:226-231): the text before the first tab is a wanted type
-    if (filter && (t1 == std::string::npos || types.find(std::string_view(ln.data(), t1), {}) == ~0u)) continue;
+    if (filter && (t1 == std::string::npos || types.find(std::string_view(ln.data(), t1), none) == ~0u)) continue;
     // line_to_embedding (io.rs:255-268): three tab fields, "[v1,v2,...]" with the first and
     // last byte dropped, values split at ',' and trimmed, each a whole f32 (fast_float::parse)
     const size_t t2 = t1 == std::string::npos ? std::string::npos : ln.find('\t', t1 + 1);
