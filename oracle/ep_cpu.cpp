@@ -65,6 +65,9 @@ struct Args {
   uint64_t seed = 20222022ull;
   bool json = false;
   std::string table_out, train_file;
+  std::string explicit_dir;  // one deterministic batch on explicit lists (see run_explicit)
+  float lr = 0.01f;
+  uint32_t pass = 1;
 };
 
 template <typename T>
@@ -205,42 +208,48 @@ struct Model {
   }
 
   // One example; returns loss (<= 0 means no gradient) and fills grads (first-leaf rule).
+  // Samples as the reference: ~h(v) from Fixed(S).sample(deg) = (min(S,deg), deg/r), all
+  // neighbours when deg <= S, else a reservoir over the row (utils.rs:128-145); K
+  // negatives uniform with replacement from the train ids (node_sampler.rs:102-105).
   float example(uint32_t v, XorShift& rng, std::unordered_map<uint32_t, std::vector<float>>& grads) const {
-    const uint32_t D = a.dims, K = a.negatives;
+    const uint32_t K = a.negatives;
     const uint64_t s0 = indptr[v], s1 = indptr[v + 1];
     const uint32_t deg = (uint32_t)(s1 - s0);
     if (deg == 0) return 0.f;
-    std::vector<Cons> cons(2 + K);
-    std::vector<uint32_t> sampled;
-    collect(cons[0], v, 1.0f);
-    // ~h(v): Fixed(S).sample(deg) = (min(S,deg), deg/r); reservoir when deg > S (utils.rs:128-145)
     const uint32_t S = a.max_nodes;
     const uint32_t r = std::min(S, deg);
     const float scale = (float)deg / (float)r;
+    std::vector<uint32_t> nbrs;
     if (deg <= S) {
-      for (uint64_t j = s0; j < s1; ++j) collect(cons[1], indices[j], scale);
+      nbrs.assign(indices.begin() + s0, indices.begin() + s1);
     } else {
-      std::vector<uint32_t> res(indices.begin() + s0, indices.begin() + s0 + S);
+      nbrs.assign(indices.begin() + s0, indices.begin() + s0 + S);
       for (uint32_t i = S; i < deg; ++i) {
         const uint32_t idx = rng.below(i);
-        if (idx < S) res[idx] = indices[s0 + i];
+        if (idx < S) nbrs[idx] = indices[s0 + i];
       }
-      for (uint32_t u : res) collect(cons[1], u, scale);
-      sampled.swap(res);
     }
-    std::vector<uint32_t> negs(K), nbrs;
-    for (uint32_t k = 0; k < K; ++k) {
-      negs[k] = train[rng.below((uint32_t)train.size())];
-      collect(cons[2 + k], negs[k], 1.0f);
-    }
+    std::vector<uint32_t> negs(K);
+    for (uint32_t k = 0; k < K; ++k) negs[k] = train[rng.below((uint32_t)train.size())];
+    return example_lists(v, nbrs, scale, negs, grads);
+  }
+
+  // One example on given neighbour and negative lists (the neighbours' features count
+  // scale + 1 each, model.rs:267-268).
+  float example_lists(uint32_t v, const std::vector<uint32_t>& nbrs, float scale, const std::vector<uint32_t>& negs,
+                      std::unordered_map<uint32_t, std::vector<float>>& grads) const {
+    const uint32_t D = a.dims, K = a.negatives;
+    if (nbrs.empty()) return 0.f;
+    std::vector<Cons> cons(2 + K);
+    collect(cons[0], v, 1.0f);
+    for (uint32_t u : nbrs) collect(cons[1], u, scale);
+    for (uint32_t k = 0; k < K; ++k) collect(cons[2 + k], negs[k], 1.0f);
     std::vector<float> x((2 + K) * (size_t)D);
     const bool att = a.dk != 0;
     if (!att) {
       for (uint32_t c = 0; c < 2 + K; ++c) mean(cons[c], &x[(size_t)c * D]);
     } else {
       // the sampled neighbours again, in order (one attention each, model.rs:419-446)
-      if (deg <= S) nbrs.assign(indices.begin() + s0, indices.begin() + s1);
-      else nbrs = sampled;
       attention(cons[0].feats, cons[0].counts, &x[0], nullptr, nullptr);
       std::fill(&x[D], &x[2 * D], 0.f);
       std::vector<uint32_t> nf;
@@ -369,6 +378,61 @@ float cos_decay(uint64_t step, float alpha, uint64_t total) {
   return mn + ((float)step / (float)warm) * (alpha - mn);
 }
 
+template <typename T>
+void write_bin(const std::string& path, const std::vector<T>& v) {
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) { std::fprintf(stderr, "cannot write %s\n", path.c_str()); std::exit(2); }
+  std::fwrite(v.data(), sizeof(T), v.size(), f);
+  std::fclose(f);
+}
+
+// One batch on explicit lists, sequentially: dir holds table.bin (vocab x width f32),
+// anchors.bin (n u32), cnt.bin (n u32), nbr.bin (n x max_nodes u32, the first cnt[i]
+// used), neg.bin (n x negatives u32).  Every example's first-leaf gradients are merged
+// in anchor order (mod.rs:223-233) and one Adam update with --lr and t = --pass + 1 is
+// applied (optimizer.rs:43-81).  Writes out_loss.bin (n: the loss of each contributing
+// example, else 0), out_grad.bin (merged gradient, dense), out_table.bin, out_m.bin,
+// out_v.bin.
+int run_explicit(Model& m) {
+  const Args& a = m.a;
+  const std::string d = a.explicit_dir;
+  const uint32_t W = m.width();
+  m.table = read_bin<float>(d + "/table.bin");
+  if (m.table.size() != (size_t)a.vocab * W) { std::fprintf(stderr, "table size mismatch\n"); return 2; }
+  m.mom.assign(m.table.size(), 0.f);
+  m.var.assign(m.table.size(), 0.f);
+  const auto anchors = read_bin<uint32_t>(d + "/anchors.bin");
+  const auto cnt = read_bin<uint32_t>(d + "/cnt.bin");
+  const auto nbr = read_bin<uint32_t>(d + "/nbr.bin");
+  const auto neg = read_bin<uint32_t>(d + "/neg.bin");
+  const size_t n = anchors.size(), S = a.max_nodes, K = a.negatives;
+  if (cnt.size() != n || nbr.size() != n * S || neg.size() != n * K) { std::fprintf(stderr, "list size mismatch\n"); return 2; }
+  std::vector<float> loss(n, 0.f), grad(m.table.size(), 0.f);
+  std::unordered_map<uint32_t, std::vector<float>> all;
+  std::vector<uint32_t> order;  // merged rows in first-seen order
+  for (size_t i = 0; i < n; ++i) {
+    std::vector<uint32_t> nb(nbr.begin() + i * S, nbr.begin() + i * S + cnt[i]);
+    std::vector<uint32_t> ng(neg.begin() + i * K, neg.begin() + (i + 1) * K);
+    std::unordered_map<uint32_t, std::vector<float>> g;
+    const float L = m.example_lists(anchors[i], nb, 1.0f, ng, g);
+    if (!(L > 0.f)) continue;
+    loss[i] = L;
+    for (auto& kv : g) {
+      auto it = all.find(kv.first);
+      if (it == all.end()) { all.emplace(kv.first, std::move(kv.second)); order.push_back(kv.first); }
+      else for (uint32_t t = 0; t < W; ++t) it->second[t] += kv.second[t];
+    }
+  }
+  for (const auto& kv : all) std::copy(kv.second.begin(), kv.second.end(), grad.begin() + (size_t)kv.first * W);
+  if (!all.empty()) m.adam(all, a.lr, (float)a.pass);
+  write_bin(d + "/out_loss.bin", loss);
+  write_bin(d + "/out_grad.bin", grad);
+  write_bin(d + "/out_table.bin", m.table);
+  write_bin(d + "/out_m.bin", m.mom);
+  write_bin(d + "/out_v.bin", m.var);
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -394,6 +458,9 @@ int main(int argc, char** argv) {
     else if (k == "--table-out") a.table_out = nx();
     else if (k == "--train") a.train_file = nx();
     else if (k == "--json") a.json = true;
+    else if (k == "--explicit") a.explicit_dir = nx();
+    else if (k == "--lr") a.lr = std::stof(nx());
+    else if (k == "--pass") a.pass = (uint32_t)std::stoul(nx());
     else { std::fprintf(stderr, "unknown argument %s\n", k.c_str()); return 2; }
   }
   const auto indptr = read_bin<uint64_t>(a.dir + "/indptr.bin");
@@ -402,6 +469,7 @@ int main(int argc, char** argv) {
   const auto fids = read_bin<uint32_t>(a.dir + "/fids.bin");
   if (indptr.size() != a.nodes + 1 || findptr.size() != a.nodes + 1) { std::fprintf(stderr, "size mismatch\n"); return 2; }
   Model m{a, indptr, indices, findptr, fids, {}, {}, {}, {}};
+  if (!a.explicit_dir.empty()) return run_explicit(m);
   const uint32_t D = m.width();  // table row width
   // randomize_embedding_store (mod.rs:381-392)
   m.table.resize((size_t)a.vocab * D);
