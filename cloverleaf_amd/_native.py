@@ -66,6 +66,7 @@ class EPConfig(Structure):
         ("loss_param2", c_float),
         ("max_features_p", c_float),
         ("max_nodes_p", c_float),
+        ("fp16_adjoints", c_int32),
     ]
 
 

@@ -550,7 +550,9 @@ class EmbeddingPropagator:
     ("first_leaf", the reference rule, | "exact"), ``scatter`` ("sort" | "atomic"),
     ``use_graph`` (replay steps from a captured hipGraph), ``concurrent_batches`` (1, 2, 4
     or 8 batches per step, reading the same parameters and updating in batch order, like
-    the reference's concurrently running batches; 1 = strictly sequential), ``device``.
+    the reference's concurrently running batches; 1 = strictly sequential), ``device``,
+    ``fp16_adjoints`` (BF16 tables, mean model: the adjoint rows the update gathers are
+    held in FP16 with a power-of-two scale per row).
     ``attention=d_k`` with ``attention_heads`` trains the attention aggregator (Full
     attention, rows of attention_heads * (2 d_k + dims) values, at most 64 features per
     node); ``context_window`` selects Sliding attention (window_size >= 1).  Weighted
@@ -562,7 +564,8 @@ class EmbeddingPropagator:
                  weighted_neighbor_sampling=None, weighted_neighbor_averaging=None, max_features=None,
                  loss_weighting=None, valid_pct=None, hard_negatives=None, indicator=None, attention=None,
                  attention_heads=None, context_window=None, noise=None, *, precision="fp32",
-                 grad_mode="first_leaf", scatter="sort", use_graph=False, concurrent_batches=1, device=0):
+                 grad_mode="first_leaf", scatter="sort", use_graph=False, concurrent_batches=1, device=0,
+                 fp16_adjoints=False):
         loss = loss or EPLoss.margin(1.0, 1)
         mf_kind, mf, mf_p = _parse_sample(max_features, "max_features")
         mn_kind, mn, mn_p = _parse_sample(max_nodes, "max_nodes")
@@ -597,6 +600,7 @@ class EmbeddingPropagator:
         cfg.scatter = {"sort": N.EP_SCATTER_SORT, "atomic": N.EP_SCATTER_ATOMIC}[scatter]
         cfg.use_graph = 1 if use_graph else 0
         cfg.concurrent_batches = int(concurrent_batches)
+        cfg.fp16_adjoints = 1 if fp16_adjoints else 0
         cfg.noise = 0.0 if noise is None else float(noise)
         cfg.device = int(device)
         if attention is not None:

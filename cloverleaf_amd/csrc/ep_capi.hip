@@ -191,6 +191,11 @@ uint32_t hook_list_width(const ep_ctx* c) { return prob_nodes(c->cfg) ? std::max
 // max_features sampling (the feature scale differs between neighbours).
 bool neighbour_rows(const ep_config& cfg) { return weighted_avg(cfg) || feature_sampling(cfg); }
 
+// FP16 adjoint rows: requested, with a BF16 table, the mean model and the sort scatter.
+bool fp16_adj(const ep_config& cfg) {
+  return cfg.fp16_adjoints && cfg.precision != EP_PREC_FP32 && !cfg.attention_dk && cfg.scatter == EP_SCATTER_SORT;
+}
+
 // Anchors per training step: concurrent_batches reference batches.
 uint32_t group_anchors(const ep_ctx* c) { return c->cfg.batch_size * std::max<uint32_t>(1u, c->cfg.concurrent_batches); }
 
@@ -246,10 +251,12 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   ep::StepConfig& b = c->base;
   c->release(b.nbr_cnt); c->release(b.nbr); c->release(b.neg); c->release(b.rcount); c->release(b.keyoff);
   c->release(b.nbr_w); c->release(b.nbroff); c->release(b.nbr_sel); c->release(b.nbr_selw);
-  c->release(b.adj); c->release(b.coef); c->release(b.elig); c->release(b.loss);
+  c->release(b.adj); c->release(b.adjh); c->release(b.ascale); c->release(b.coef); c->release(b.elig);
+  c->release(b.loss);
   c->release(b.keys[0]); c->release(b.keys[1]); c->release(b.slots[0]); c->release(b.slots[1]);
   c->release(b.ghist); c->release(b.G); c->release(b.ufeat); c->release(b.long_st); c->release(b.long_len);
-  c->release(b.long_pbase); c->release(b.long_runs); c->release(b.piece_st); c->release(b.piece_info);
+  c->release(b.long_pbase); c->release(b.long_runs); c->release(b.long_rlen); c->release(b.piece_st);
+  c->release(b.piece_info);
   c->release(b.part); c->release(b.srec); c->release(b.mrec); c->release(b.scount); c->release(b.mcount);
   c->release(b.blkoff); c->release(b.bitems); c->release(b.bout); c->release(b.badj);
   c->release(b.bnode); c->release(b.bslot); c->release(b.bmeta);
@@ -260,9 +267,11 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   const uint32_t P = std::max<uint32_t>(1u, cfg.concurrent_batches);
   // merged-gradient records (test hook): one per (row, batch) run
   const uint64_t unique_cap = std::min<uint64_t>((uint64_t)c->n_vocab * P, keycap) + 1;
-  // long rows have > kSegShort entries; each run of one is cut into pieces of kPieceEntries
-  const uint32_t long_cap = (uint32_t)(keycap / (ep::kSegShort + 1) + 1);
-  const uint32_t piece_cap = (uint32_t)(keycap / ep::kPieceEntries + (uint64_t)long_cap * P + 1);
+  // long rows run past the plan's 128-entry window (>= 65 entries); their runs of more
+  // than kPieceEntries entries are cut into pieces of kPieceEntries, fewer than
+  // (run entries) / 16 per run
+  const uint32_t long_cap = (uint32_t)(keycap / 65 + 1);
+  const uint32_t piece_cap = (uint32_t)(keycap / 16 + 1);
   // persistent row kernels: 2048 blocks of kBlockWaves waves
   const int unique_grid = 2048;
   // radix-sort grid: ~8K keys per block at capacity, a multiple of 256 in [256, 2048]
@@ -274,7 +283,12 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   EP_TRY(c->alloc(&b.keyoff, batch + 1));
   // adjoint rows: mean model one per construction (D wide); attention one gradient row
   // per feature occurrence
-  EP_TRY(c->alloc(&b.adj, att ? (size_t)keycap * D : ((size_t)batch * NC + nrow_cap) * D));
+  if (fp16_adj(cfg)) {
+    EP_TRY(c->alloc(&b.adjh, ((size_t)batch * NC + nrow_cap) * D));
+    EP_TRY(c->alloc(&b.ascale, (size_t)batch * NC + nrow_cap));
+  } else {
+    EP_TRY(c->alloc(&b.adj, att ? (size_t)keycap * D : ((size_t)batch * NC + nrow_cap) * D));
+  }
   if (wavg && S) EP_TRY(c->alloc(&b.nbr_w, (size_t)batch * S));
   if (nrows && !S) EP_TRY(c->alloc(&b.nbroff, batch));
   if (prob_nodes(cfg)) {
@@ -308,6 +322,7 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   EP_TRY(c->alloc(&b.long_len, long_cap));
   EP_TRY(c->alloc(&b.long_pbase, long_cap));
   EP_TRY(c->alloc(&b.long_runs, (size_t)long_cap * ep::kMaxGroup));
+  EP_TRY(c->alloc(&b.long_rlen, (size_t)long_cap * ep::kMaxGroup));
   // plan regions: wave w of the row kernels covers cpw chunks of 64 keys; up to 64 short
   // and 2 medium rows start in a chunk
   const uint64_t nwaves = (uint64_t)unique_grid * ep::kBlockWaves;
@@ -340,8 +355,16 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   b.ent_cap = (uint32_t)std::min<uint64_t>(256, 1ull + neighbour_cap(c) + K);
   b.sel_cap = feature_sampling(cfg) ? b.ent_cap : 0u;
   b.ids_cap = (uint32_t)std::min<uint64_t>(8192, example_rows_max(c));
-  // fuse emission into the forward while the first-leaf hash is small (<= 16 KiB of LDS)
-  b.fuse_emit = (!att && (cfg.grad_mode == EP_GRAD_EXACT || hash <= 4096)) ? 1 : 0;
+  // emission fused into the forward (mean model): its first-leaf hash takes the LDS of the
+  // constructions and partial sums, (NC + waves) * D floats, after the adjoints are out
+  b.fuse_emit = att ? 0 : 1;
+  {
+    const uint64_t words = (uint64_t)(NC + ep::kBlockWaves) * D;
+    uint32_t fh = 1;
+    while ((uint64_t)fh * 2 <= words) fh *= 2;
+    b.fuse_hash = fh;
+    b.fuse_max = fh / 8 * 5;
+  }
   b.unique_grid = unique_grid;
   c->cap_batch = batch;
   c->cap_S = S;
@@ -681,6 +704,10 @@ int validate_config(const ep_config* cfg) {
     return fail(EP_EINVAL, "concurrent_batches must be 1, 2, 4 or 8");
   if (cfg->concurrent_batches > 1 && cfg->scatter != EP_SCATTER_SORT)
     return fail(EP_EINVAL, "concurrent_batches > 1 needs the sort scatter");
+  if (cfg->fp16_adjoints != 0 && cfg->fp16_adjoints != 1) return fail(EP_EINVAL, "fp16_adjoints must be 0 or 1");
+  if (cfg->fp16_adjoints && (cfg->precision == EP_PREC_FP32 || cfg->attention_dk || cfg->scatter != EP_SCATTER_SORT))
+    return fail(EP_EINVAL, "fp16_adjoints needs a BF16 table (precision bf16 or bf16-master), the mean model and the "
+                           "sort scatter");
   if ((cfg->seg_vpl == 0) != (cfg->seg_unroll == 0) ||
       (cfg->seg_vpl != 0 && !ep::seg_variant_supported((int)row_width(*cfg), cfg->seg_vpl, cfg->seg_unroll)))
     return fail(EP_EINVAL, "seg_vpl/seg_unroll must both be 0 (tuned) or a built variant: vpl in {4,8,12,16} "
@@ -965,6 +992,7 @@ int ep_learn(ep_ctx* c, ep_progress_fn progress, void* user) {
   vc.n_negpool = (uint32_t)c->valid.size();
   vc.n_batch_total = (uint32_t)c->valid.size();
   vc.train = 0;
+  vc.lw_kind = 0;  // the validation sums each anchor's unweighted forward loss (mod.rs:272-278)
   vc.seed = c->cfg.seed - 1ull;
   vc.pass_loss = c->d_valid_loss;
   vc.steps_per_pass = (uint32_t)((c->valid.size() + vc.batch - 1) / vc.batch);

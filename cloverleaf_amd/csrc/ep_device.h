@@ -208,8 +208,78 @@ __device__ inline void store_slice(float* __restrict__ row, int lane, const floa
 }
 
 // ------------------------------------------------------------------------------------
+// FP16 row slices (IEEE binary16, round to nearest even on store), same lane layout as
+// the FP32 slices: the values are widened exactly on load.
+// ------------------------------------------------------------------------------------
+__device__ inline uint16_t f32_to_f16_rne(float x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
+__device__ inline float f16_to_f32(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
+
+template <int D>
+__device__ inline void store_slice_f16(uint16_t* __restrict__ row, int lane, const float (&v)[RowShape<D>::PL],
+                                       float mul) {
+  constexpr int PL = RowShape<D>::PL;
+  const int base = lane * PL;
+  if constexpr (PL % 4 == 0 && D % PL == 0) {  // 8-byte chunks
+    if (base < D) {
+#pragma unroll
+      for (int k = 0; k < PL; k += 4) {
+        uint2 t;
+        t.x = (uint32_t)f32_to_f16_rne(v[k] * mul) | ((uint32_t)f32_to_f16_rne(v[k + 1] * mul) << 16);
+        t.y = (uint32_t)f32_to_f16_rne(v[k + 2] * mul) | ((uint32_t)f32_to_f16_rne(v[k + 3] * mul) << 16);
+        *reinterpret_cast<uint2*>(row + base + k) = t;
+      }
+    }
+  } else if constexpr (PL == 2 && D % 2 == 0) {
+    if (base < D)
+      *reinterpret_cast<uint32_t*>(row + base) =
+          (uint32_t)f32_to_f16_rne(v[0] * mul) | ((uint32_t)f32_to_f16_rne(v[1] * mul) << 16);
+  } else {
+#pragma unroll
+    for (int j = 0; j < PL; ++j) if (base + j < D) row[base + j] = f32_to_f16_rne(v[j] * mul);
+  }
+}
+
+// v = mul * row slice (zero outside the row)
+template <int D>
+__device__ inline void load_slice_f16(const uint16_t* __restrict__ row, int lane, float (&v)[RowShape<D>::PL],
+                                      float mul) {
+  constexpr int PL = RowShape<D>::PL;
+  const int base = lane * PL;
+  if constexpr (PL % 4 == 0 && D % PL == 0) {
+    if (base < D) {
+#pragma unroll
+      for (int k = 0; k < PL; k += 4) {
+        const uint2 t = *reinterpret_cast<const uint2*>(row + base + k);
+        v[k] = mul * f16_to_f32((uint16_t)(t.x & 0xffffu));
+        v[k + 1] = mul * f16_to_f32((uint16_t)(t.x >> 16));
+        v[k + 2] = mul * f16_to_f32((uint16_t)(t.y & 0xffffu));
+        v[k + 3] = mul * f16_to_f32((uint16_t)(t.y >> 16));
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < PL; ++k) v[k] = 0.f;
+    }
+  } else if constexpr (PL == 2 && D % 2 == 0) {
+    if (base < D) {
+      const uint32_t t = *reinterpret_cast<const uint32_t*>(row + base);
+      v[0] = mul * f16_to_f32((uint16_t)(t & 0xffffu));
+      v[1] = mul * f16_to_f32((uint16_t)(t >> 16));
+    } else { v[0] = v[1] = 0.f; }
+  } else {
+#pragma unroll
+    for (int j = 0; j < PL; ++j) v[j] = (base + j < D) ? mul * f16_to_f32(row[base + j]) : 0.f;
+  }
+}
+
+// ------------------------------------------------------------------------------------
 // Wave reductions (wave64)
 // ------------------------------------------------------------------------------------
+__device__ inline float wave_max(float x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x = fmaxf(x, __shfl_xor(x, off, EP_WAVE));
+  return x;
+}
+
 __device__ inline float wave_sum(float x) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, EP_WAVE);

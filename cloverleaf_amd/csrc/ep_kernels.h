@@ -135,7 +135,12 @@ struct StepConfig {
   uint32_t* neg;           // [B*K]
   uint32_t* rcount;        // [B]
   uint32_t* keyoff;        // [B+1]
-  float* adj;              // [B*NC*D]
+  float* adj;              // [B*NC*D] FP32 adjoint rows (FP32 tables, attention gradient rows)
+  // FP16 adjoint rows (BF16 tables, mean model): row r holds g / s in FP16 (round to
+  // nearest) with s = max |g| of the row in ascale[r]; readers take s * value in FP32.
+  // nullptr: the FP32 rows in adj.
+  uint16_t* adjh;          // [B*NC*D]
+  float* ascale;           // [B*NC]
   float* coef;             // [B*NC]
   uint8_t* elig;           // [B*NC]
   float* loss;             // [B]
@@ -145,6 +150,10 @@ struct StepConfig {
   uint32_t ent_cap;        // forward LDS stage: max entries (nodes) per example, <= 256
   uint32_t ids_cap;        // forward LDS stage: max feature occurrences per example
   int fuse_emit;           // k_forward emits the keys of LDS-staged examples
+  uint32_t fuse_hash;      // entries of k_forward's first-leaf hash (power of two that fits the
+                           // construction region of its LDS)
+  uint32_t fuse_max;       // first_leaf: k_forward emits examples of at most this many
+                           // occurrences (5/8 of fuse_hash), k_emit the larger ones
   // keys
   uint32_t* keys[2];
   uint32_t* slots[2];
@@ -167,13 +176,18 @@ struct StepConfig {
   uint32_t mrec_cap;
   uint32_t* scount;        // [unique_grid * NW]
   uint32_t* mcount;        // [unique_grid * NW]
-  // Long rows (running past the window): first sorted entry, first piece, run count and
-  // per run (piece count | batch << 24); the pieces of a row follow each other run by
-  // run.  A piece is at most kPieceEntries entries of one run: start entry and length.
+  // Long rows (running past the window, so at least 65 entries): first sorted entry,
+  // first piece, run count and per run (piece count | batch << 24) and its entry count.
+  // Only runs of more than kPieceEntries entries are cut into pieces (a piece is a range
+  // of one run: start entry and length; kPieceEntries entries, or more for hot runs so a
+  // run has at most 64 pieces); the combine kernel sums the shorter runs straight from
+  // their adjoint rows.  The pieces of a row follow each other run by run, so a step has
+  // fewer than n_real / 16 pieces.
   uint32_t* long_st;       // [long_cap]
   uint32_t* long_len;      // runs of the row
   uint32_t* long_pbase;
   uint32_t* long_runs;     // [long_cap * kMaxGroup]
+  uint32_t* long_rlen;     // [long_cap * kMaxGroup] entries of each run
   uint32_t* piece_st;      // [piece_cap]
   uint32_t* piece_info;    // [piece_cap] entries of the piece
   float* part;             // [piece_cap*D]

@@ -22,6 +22,12 @@ static constexpr uint32_t HASH_EMPTY = 0xFFFFFFFFu;
 static constexpr uint32_t HASH_FBITS = 27;
 static constexpr uint32_t HASH_FMASK = (1u << HASH_FBITS) - 1u;
 
+// Independent partial sums of the gathers: row u of each group of U in-flight rows adds
+// into partial u % kGatherSums, and the partials are added at the end (blocked
+// summation: the FP32 rounding error of a mean over n rows grows with n / kGatherSums
+// instead of n, which keeps means over thousands of rows within 1e-5 of FP64).
+static constexpr int kGatherSums = 4;
+
 // ------------------------------------------------------------------------------------
 // Helpers
 // ------------------------------------------------------------------------------------
@@ -40,6 +46,34 @@ __device__ inline uint32_t key_of_row(const StepConfig& c, uint32_t f) {
 }
 __device__ inline uint32_t row_of_key(const StepConfig& c, uint32_t k) {
   return mul_mod_mersenne(k, c.key_ainv, c.key_m, c.key_bits);
+}
+
+// Adjoint row `slot` (whole-wave slice layout): FP32 rows in c.adj, or, with c.adjh,
+// FP16 rows scaled by a power of two 2^e >= max |g| of the row (exact scaling, so the
+// only rounding is the FP16 one: 2^-11 relative to the row's largest value).
+template <int D>
+__device__ inline void store_adj_row(const StepConfig& c, size_t slot, int lane, const float (&g)[RowShape<D>::PL]) {
+  if (c.adjh) {
+    constexpr int PL = RowShape<D>::PL;
+    float mx = 0.f;
+#pragma unroll
+    for (int q = 0; q < PL; ++q)
+      if (lane * PL + q < D) mx = fmaxf(mx, fabsf(g[q]));
+    mx = wave_max(mx);
+    int ex = 0;
+    (void)frexpf(mx, &ex);  // mx = m 2^ex, m in [0.5, 1) (ex = 0 for mx = 0)
+    ex = ex > 127 ? 127 : ex;
+    store_slice_f16<D>(c.adjh + slot * D, lane, g, ldexpf(1.0f, -ex));
+    if (lane == 0) c.ascale[slot] = ldexpf(1.0f, ex);
+  } else {
+    store_slice<D>(c.adj + slot * D, lane, g);
+  }
+}
+
+template <int D, bool H>
+__device__ inline void load_adj_row(const StepConfig& c, size_t slot, int lane, float (&r)[RowShape<D>::PL]) {
+  if constexpr (H) load_slice_f16<D>(c.adjh + slot * D, lane, r, c.ascale[slot]);
+  else load_slice<D>(c.adj + slot * D, lane, r);
 }
 
 // First index i in [lo, hi) with K[i] >= x (hi if none), K sorted; 64-ary search by
@@ -70,6 +104,12 @@ template <int D, int U, typename T>
 __device__ inline void gather_rows(const T* __restrict__ table, const uint32_t* __restrict__ fids,
                                    uint64_t fs, uint64_t fe, int lane, float (&acc)[RowShape<D>::PL]) {
   constexpr int PL = RowShape<D>::PL;
+  constexpr int A = U < kGatherSums ? U : kGatherSums;
+  float part[A][PL];
+#pragma unroll
+  for (int i = 0; i < A; ++i)
+#pragma unroll
+    for (int q = 0; q < PL; ++q) part[i][q] = 0.f;
   for (uint64_t base = fs; base < fe; base += EP_WAVE) {
     const int n = (int)((fe - base) < (uint64_t)EP_WAVE ? (fe - base) : (uint64_t)EP_WAVE);
     const uint32_t myf = lane < n ? fids[base + lane] : 0u;
@@ -85,11 +125,17 @@ __device__ inline void gather_rows(const T* __restrict__ table, const uint32_t* 
       for (int u = 0; u < U; ++u) {
         if (j + u < n) {
 #pragma unroll
-          for (int q = 0; q < PL; ++q) acc[q] += r[u][q];
+          for (int q = 0; q < PL; ++q) part[u % A][q] += r[u][q];
         }
       }
     }
   }
+#pragma unroll
+  for (int i = 1; i < A; ++i)
+#pragma unroll
+    for (int q = 0; q < PL; ++q) part[0][q] += part[i][q];
+#pragma unroll
+  for (int q = 0; q < PL; ++q) acc[q] += part[0][q];
 }
 
 // 256-thread block exclusive scan of one value per thread; returns the block total in *total.
@@ -651,6 +697,12 @@ template <int D, int U, typename T>
 __device__ inline void gather_lds(const T* __restrict__ table, const uint32_t* ids, uint32_t a, uint32_t b, int lane,
                                   float (&acc)[RowShape<D>::PL]) {
   constexpr int PL = RowShape<D>::PL;
+  constexpr int A = U < kGatherSums ? U : kGatherSums;
+  float part[A][PL];
+#pragma unroll
+  for (int i = 0; i < A; ++i)
+#pragma unroll
+    for (int q = 0; q < PL; ++q) part[i][q] = 0.f;
   for (uint32_t j = a; j < b; j += U) {
     float r[U][PL];
 #pragma unroll
@@ -662,10 +714,16 @@ __device__ inline void gather_lds(const T* __restrict__ table, const uint32_t* i
     for (int u = 0; u < U; ++u) {
       if (j + u < b) {
 #pragma unroll
-        for (int q = 0; q < PL; ++q) acc[q] += r[u][q];
+        for (int q = 0; q < PL; ++q) part[u % A][q] += r[u][q];
       }
     }
   }
+#pragma unroll
+  for (int i = 1; i < A; ++i)
+#pragma unroll
+    for (int q = 0; q < PL; ++q) part[0][q] += part[i][q];
+#pragma unroll
+  for (int q = 0; q < PL; ++q) acc[q] += part[0][q];
 }
 
 // Accumulate the rows of the features kept by a max_features mask (bit i: fids[fs + i]),
@@ -737,9 +795,12 @@ __device__ inline void gather_lds_wide(const T* __restrict__ table, const uint32
   using WS = WideShape<D, T>;
   constexpr int VW = WS::VW, LW = WS::LW, RPW = WS::RPW;
   const int g = lane / LW, lg = lane % LW;
-  float acc[VW];
+  constexpr int A = U < kGatherSums ? U : kGatherSums;
+  float part[A][VW];
 #pragma unroll
-  for (int k = 0; k < VW; ++k) acc[k] = 0.f;
+  for (int i = 0; i < A; ++i)
+#pragma unroll
+    for (int k = 0; k < VW; ++k) part[i][k] = 0.f;
   for (uint32_t j = a + (uint32_t)g; j < b; j += (uint32_t)(RPW * U)) {
     float r[U][VW];
 #pragma unroll
@@ -752,9 +813,16 @@ __device__ inline void gather_lds_wide(const T* __restrict__ table, const uint32
     for (int u = 0; u < U; ++u) {
       if (j + (uint32_t)(u * RPW) < b) {
 #pragma unroll
-        for (int k = 0; k < VW; ++k) acc[k] += r[u][k];
+        for (int k = 0; k < VW; ++k) part[u % A][k] += r[u][k];
       }
     }
+  }
+  float acc[VW];
+#pragma unroll
+  for (int k = 0; k < VW; ++k) {
+    acc[k] = part[0][k];
+#pragma unroll
+    for (int i = 1; i < A; ++i) acc[k] += part[i][k];
   }
 #pragma unroll
   for (int off = LW; off < 64; off <<= 1) {
@@ -863,7 +931,7 @@ __device__ inline void forward_other_loss(const StepConfig& c, StepState* st, ui
     if (e) {
 #pragma unroll
       for (int q = 0; q < PL; ++q) g[q] *= co;
-      store_slice<D>(c.adj + ((size_t)b * NC + ci) * D, lane, g);
+      store_adj_row<D>(c, (size_t)b * NC + ci, lane, g);
       if (c.nbr_rows && ci == 1u) {
 #pragma unroll
         for (int q = 0; q < PL; ++q)
@@ -1201,7 +1269,7 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
     if (e) {
 #pragma unroll
       for (int q = 0; q < PL; ++q) g[q] *= co;
-      store_slice<D>(c.adj + ((size_t)b * NC + ci) * D, lane, g);
+      store_adj_row<D>(c, (size_t)b * NC + ci, lane, g);
       if (c.nbr_rows && ci == 1u) {
 #pragma unroll
         for (int q = 0; q < PL; ++q)
@@ -1233,21 +1301,25 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
         float g[PL];
 #pragma unroll
         for (int q = 0; q < PL; ++q) g[q] = (base + q < D) ? part[base + q] * cj : 0.f;
-        store_slice<D>(c.adj + (size_t)nbr_row(c, b, j) * D, lane, g);
+        store_adj_row<D>(c, (size_t)nbr_row(c, b, j), lane, g);
       }
     }
   }
   // Key emission for staged examples (k_emit handles the rest): one (feature, slot) key
   // per occurrence, sentinel unless the construction is eligible and, under first_leaf,
-  // the feature's first eligible construction (mod.rs:347-352, 366-367).
+  // the feature's first eligible construction (mod.rs:347-352, 366-367).  The
+  // first-leaf hash lives in the construction / partial-sum region of the LDS, dead by
+  // now; examples with more occurrences than fuse_max (5/8 of its entries) are left to
+  // k_emit.
   if (!(staged && c.fuse_emit)) return;
+  if (c.grad_mode == 0 && Re > c.fuse_max) return;
   if (st->n_keys > c.key_cap) return;
   __syncthreads();
   const uint32_t tot = ent_off[E];
   const uint32_t kb = c.keyoff[b];
   const bool first_leaf = c.grad_mode == 0;
-  uint32_t* hsh = ids + c.ids_cap;
-  const uint32_t hmask = c.hash_entries - 1u;
+  uint32_t* hsh = reinterpret_cast<uint32_t*>(sm);
+  const uint32_t hmask = c.fuse_hash - 1u;
   auto entry_of = [&](uint32_t o) -> uint32_t {
     uint32_t lo = 0, hi = E;  // last entry e with ent_off[e] <= o
     while (hi - lo > 1u) {
@@ -1258,7 +1330,7 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
   };
   auto cons_of_entry = [&](uint32_t e) -> uint32_t { return e == 0u ? 0u : (e <= cnt ? 1u : 2u + (e - 1u - cnt)); };
   if (first_leaf) {
-    for (uint32_t i = tid; i < c.hash_entries; i += blockDim.x) hsh[i] = HASH_EMPTY;
+    for (uint32_t i = tid; i < c.fuse_hash; i += blockDim.x) hsh[i] = HASH_EMPTY;
     __syncthreads();
     for (uint32_t o = tid; o < tot; o += blockDim.x) {
       const uint32_t ci = cons_of_entry(entry_of(o));
@@ -1294,7 +1366,9 @@ __global__ __launch_bounds__(256) void k_emit(StepConfig c, StepState* st) {
   if (cnt == 0u) return;
   const int tid = threadIdx.x;
   const uint32_t K = c.K, NC = K + 2u;
-  if (c.fuse_emit && 1u + cnt + K <= c.ent_cap && c.rcount[b] <= c.ids_cap) return;  // emitted by k_forward
+  if (c.fuse_emit && 1u + cnt + K <= c.ent_cap && c.rcount[b] <= c.ids_cap &&
+      (c.grad_mode != 0 || c.rcount[b] <= c.fuse_max))
+    return;  // emitted by k_forward
   extern __shared__ uint32_t hsh[];
   __shared__ uint32_t s_wave[NW];
   __shared__ uint8_t s_el[32];
@@ -1546,27 +1620,32 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t* __restrict__
 // the persistent master when there is one, else from the bf16 row itself), updated, and
 // written back to the bf16 table with stochastic rounding (hash bits keyed by row,
 // column and step).
+// The bias corrections enter as the step's reciprocals ibc = 1/bc (computed once per
+// kernel), the square root and the quotient through the hardware v_sqrt_f32 /
+// v_rcp_f32 (1 ulp each): a few ulp from the IEEE divisions of the formula, at a
+// quarter of their instructions (the update applies one Adam step per batch of the
+// group to every value of every touched row).
 // ------------------------------------------------------------------------------------
-__device__ inline void adam_math(float& th, float& m, float& v, float g, float lr, float bc1, float bc2) {
+__device__ inline void adam_math(float& th, float& m, float& v, float g, float lr, float ibc1, float ibc2) {
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   m = b1 * m + (1.0f - b1) * g;
   v = b2 * v + (1.0f - b2) * (g * g);
-  const float mh = m / bc1;
-  const float vh = v / bc2;
-  th -= lr * (mh / (sqrtf(vh) + eps));
+  const float den = __builtin_amdgcn_sqrtf(v * ibc2) + eps;
+  th -= lr * (m * ibc1) * __builtin_amdgcn_rcpf(den);
 }
 
 // Per-step scalars of the update, read once per kernel (before any store, so the
 // compiler keeps them in registers instead of reloading them after each store).
 struct StepScalars {
   float lr0;         // learning rate of the group's first batch
-  float bc1, bc2;
+  float ibc1, ibc2;  // reciprocals of the Adam bias corrections 1 - beta^t
   uint32_t sr_step;  // step number keying the stochastic-rounding bits
   uint32_t step0;    // step number of the group's first batch (noise key)
   float sd0;         // noise sd of the group's first batch
 };
 __device__ inline StepScalars step_scalars(const StepState* st) {
-  return StepScalars{st->lr[0], st->bc1, st->bc2, (uint32_t)st->next_step, (uint32_t)st->step0, st->noise_sd[0]};
+  return StepScalars{st->lr[0], 1.0f / st->bc1, 1.0f / st->bc2, (uint32_t)st->next_step, (uint32_t)st->step0,
+                     st->noise_sd[0]};
 }
 
 // 16 stochastic-rounding bits for column 4*col4 + q of row f at this step.
@@ -1631,7 +1710,7 @@ __device__ inline void adam_row(const StepConfig& c, const StepScalars& sv, uint
   else if (c.bf16) load_slice<D>((const uint16_t*)c.table + off, lane, th);
   else load_slice<D>((const float*)c.table + off, lane, th);
 #pragma unroll
-  for (int q = 0; q < PL; ++q) adam_math(th[q], m[q], v[q], g[q], sv.lr0, sv.bc1, sv.bc2);
+  for (int q = 0; q < PL; ++q) adam_math(th[q], m[q], v[q], g[q], sv.lr0, sv.ibc1, sv.ibc2);
   store_slice<D>(c.mom + ooff, lane, m);
   store_slice<D>(c.var + ooff, lane, v);
   if (c.bf16) {
@@ -1663,6 +1742,22 @@ __device__ inline void adam_row(const StepConfig& c, const StepScalars& sv, uint
 // instead of updating (test hook).
 // ------------------------------------------------------------------------------------
 
+// Pieces of a run of `len` entries of a long row: none for runs of at most PIECE entries
+// (summed by the combine kernel itself), else pieces of long_piece_len(len) entries: PIECE,
+// or for hot runs a multiple of PIECE that keeps a run within kRunPieces pieces, so the
+// combine kernel's one wave per row adds at most kRunPieces partial rows per run (fewer
+// than len / 16 pieces in every case).
+static constexpr uint32_t kRunPieces = 64;
+__device__ inline uint32_t long_piece_len(uint32_t len) {
+  const uint32_t per = (len + kRunPieces - 1u) / kRunPieces;
+  return per <= PIECE ? PIECE : (per + PIECE - 1u) / PIECE * PIECE;
+}
+__device__ inline uint32_t long_run_pieces(uint32_t len) {
+  if (len <= PIECE) return 0u;
+  const uint32_t pl = long_piece_len(len);
+  return (len + pl - 1u) / pl;
+}
+
 // Pieces of the runs of one long row, starting at piece pb; run_at(j) gives run j's
 // absolute [start, end) and batch.  Lanes lane0, lane0 + stride, ... write the pieces.
 template <typename RunAt>
@@ -1677,13 +1772,17 @@ __device__ inline void write_long_row(const StepConfig& c, uint32_t idx, uint32_
   for (uint32_t j = 0; j < nruns; ++j) {
     uint32_t a, z, bq;
     run_at(j, &a, &z, &bq);
-    const uint32_t np = (z - a + PIECE - 1u) / PIECE;
-    if (lane0 == 0u) c.long_runs[(size_t)idx * kMaxGroup + j] = np | (bq << 24);
+    const uint32_t np = long_run_pieces(z - a);
+    const uint32_t pl = long_piece_len(z - a);
+    if (lane0 == 0u) {
+      c.long_runs[(size_t)idx * kMaxGroup + j] = np | (bq << 24);
+      c.long_rlen[(size_t)idx * kMaxGroup + j] = z - a;
+    }
     for (uint32_t q = lane0; q < np; q += stride) {
-      const uint32_t ps = a + q * PIECE;
+      const uint32_t ps = a + q * pl;
       if (p + q < c.piece_cap) {
         c.piece_st[p + q] = ps;
-        c.piece_info[p + q] = z - ps < PIECE ? z - ps : PIECE;
+        c.piece_info[p + q] = z - ps < pl ? z - ps : pl;
       }
     }
     p += np;
@@ -1768,7 +1867,7 @@ __global__ __launch_bounds__(256) void k_seg_plan(StepConfig c, StepState* st, c
       uint32_t pcs = 0, nr = 0;
       for (uint32_t b = 0; b <= PM; ++b) {
         if (bnd[b + 1] > bnd[b]) ++nr;
-        pcs += (bnd[b + 1] - bnd[b] + PIECE - 1u) / PIECE;
+        pcs += long_run_pieces(bnd[b + 1] - bnd[b]);
       }
       uint32_t idx = 0, pb = 0;
       if (lane == 0) {
@@ -1823,7 +1922,7 @@ __global__ __launch_bounds__(256) void k_seg_plan(StepConfig c, StepState* st, c
 // columns strided by L), G = 64/L rows per wave, U adjoint rows in flight per group.
 // Every (VPL, U) adds each run's rows in entry order: identical bits, so the tuner picks
 // the fastest.
-template <int D, int VPL, int U>
+template <int D, int VPL, int U, bool H>
 __global__ __launch_bounds__(256) void k_seg_update(StepConfig c, StepState* st, const uint32_t* __restrict__ Ss) {
   static_assert(D % VPL == 0 && VPL % 4 == 0 && D / VPL <= 64, "float4 lane slices");
   constexpr int L = D / VPL;      // lanes per row
@@ -1852,19 +1951,40 @@ __global__ __launch_bounds__(256) void k_seg_update(StepConfig c, StepState* st,
   __syncthreads();
   const bool noisy = c.noise > 1e-9f;
   uint32_t done = 0;
-  for (uint32_t j0 = 0; j0 < nrec; j0 += (uint32_t)G) {
-    const uint32_t j = j0 + (uint32_t)grp;
-    const bool has = gact && j < nrec;
-    const uint4 rec = has ? recs[j] : make_uint4(0u, 0u, 0u, 0u);
-    const uint32_t start = rec.x, f = rec.y, rmask = rec.z;
-    const uint32_t len = rec.w & 63u;
-    // the row's slots, spread over the group's lanes (entry e at lane e % L, register e / L)
-    uint32_t sl[NS];
+  // Software pipeline over the wave's rows: iteration k issues, back to back, the row's
+  // theta/m/v, the next row's slots (its record arrived one iteration earlier), the
+  // record two rows ahead and the row's adjoint rows (its slots arrived one iteration
+  // earlier), so each row costs one memory latency instead of the chain record ->
+  // slots -> adjoint rows.  The arithmetic and its order are those of one row at a time.
+  auto load_rec = [&](uint32_t jj) -> uint4 {
+    return (gact && jj < nrec) ? recs[jj] : make_uint4(0u, 0u, 0u, 0u);
+  };
+  // the row's slots, spread over the group's lanes (entry e at lane e % L, register e / L)
+  auto load_slots = [&](const uint4& r, uint32_t (&out)[NS]) {
 #pragma unroll
     for (int i = 0; i < NS; ++i) {
       const uint32_t e = (uint32_t)(lg + i * L);
-      sl[i] = (has && e < len) ? (Ss[start + e] & kSlotMask) : 0u;
+      out[i] = (e < (r.w & 63u)) ? (Ss[r.x + e] & kSlotMask) : 0u;
     }
+  };
+  uint4 rec_next = load_rec((uint32_t)grp);
+  uint32_t sl_next[NS];
+  load_slots(rec_next, sl_next);
+  uint4 rec_after = load_rec((uint32_t)(G + grp));
+  for (uint32_t j0 = 0; j0 < nrec; j0 += (uint32_t)G) {
+    const uint32_t j = j0 + (uint32_t)grp;
+    const bool has = gact && j < nrec;
+    uint4 rec = rec_next;
+    if constexpr (G == 1) {
+      // one row per wave: its record is wave-uniform (scalar registers and branches)
+      rec = make_uint4(__builtin_amdgcn_readfirstlane(rec.x), __builtin_amdgcn_readfirstlane(rec.y),
+                       __builtin_amdgcn_readfirstlane(rec.z), __builtin_amdgcn_readfirstlane(rec.w));
+    }
+    uint32_t sl[NS];
+#pragma unroll
+    for (int i = 0; i < NS; ++i) sl[i] = sl_next[i];
+    const uint32_t f = rec.y, rmask = rec.z;
+    const uint32_t len = rec.w & 63u;
     float4 m4[NQ], v4[NQ], t4[NQ];
     const size_t roff = (size_t)f * D;
     const size_t ooff = (size_t)f * c.ostride;  // optimizer record [master |] m | v
@@ -1886,10 +2006,15 @@ __global__ __launch_bounds__(256) void k_seg_update(StepConfig c, StepState* st,
         }
       }
     }
+    rec_next = rec_after;
+    load_slots(rec_next, sl_next);
+    rec_after = load_rec(j0 + (uint32_t)(2 * G) + (uint32_t)grp);
     const uint32_t nruns = has ? (uint32_t)__popc(rmask) : 0u;
     uint32_t maxruns = nruns;
+    if constexpr (G > 1) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) maxruns = max(maxruns, (uint32_t)__shfl_xor(maxruns, off, EP_WAVE));
+      for (int off = 32; off > 0; off >>= 1) maxruns = max(maxruns, (uint32_t)__shfl_xor(maxruns, off, EP_WAVE));
+    }
     bool upd = false;
     uint32_t rest = rmask;  // run starts not yet consumed
     for (uint32_t rq = 0; rq < maxruns; ++rq) {
@@ -1899,8 +2024,10 @@ __global__ __launch_bounds__(256) void k_seg_update(StepConfig c, StepState* st,
       const uint32_t z = hr ? (rest ? (uint32_t)__builtin_ctz(rest) : len) : 0u;
       const uint32_t bq = (rec.w >> (6u + 3u * rq)) & 7u;
       uint32_t maxlen = z - a;
+      if constexpr (G > 1) {
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) maxlen = max(maxlen, (uint32_t)__shfl_xor(maxlen, off, EP_WAVE));
+        for (int off = 32; off > 0; off >>= 1) maxlen = max(maxlen, (uint32_t)__shfl_xor(maxlen, off, EP_WAVE));
+      }
       float4 acc[NQ];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) acc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1910,16 +2037,32 @@ __global__ __launch_bounds__(256) void k_seg_update(StepConfig c, StepState* st,
         for (int u = 0; u < U; ++u) {
           const uint32_t e = a + u0 + (uint32_t)u;
           const bool valid = hr && e < z;
-          const uint32_t ei = e / (uint32_t)L;
-          uint32_t pick = sl[0];
+          uint32_t slot;
+          if constexpr (G == 1 && NS == 1) {
+            slot = __builtin_amdgcn_readlane(sl[0], (int)(e & 63u));  // entry e sits in lane e
+          } else {
+            const uint32_t ei = e / (uint32_t)L;
+            uint32_t pick = sl[0];
 #pragma unroll
-          for (int i = 1; i < NS; ++i) pick = (ei == (uint32_t)i) ? sl[i] : pick;
-          const uint32_t slot = __shfl(pick, grp * L + (int)(e % (uint32_t)L), EP_WAVE);
+            for (int i = 1; i < NS; ++i) pick = (ei == (uint32_t)i) ? sl[i] : pick;
+            slot = __shfl(pick, grp * L + (int)(e % (uint32_t)L), EP_WAVE);
+          }
+          if constexpr (H) {
+            const float sc = valid ? c.ascale[slot] : 0.f;
 #pragma unroll
-          for (int q = 0; q < NQ; ++q) {
-            const uint32_t col = (uint32_t)(q * L + lg) * 4u;
-            r[u][q] = valid ? *reinterpret_cast<const float4*>(c.adj + (size_t)slot * D + col)
-                            : make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int q = 0; q < NQ; ++q) {
+              const uint32_t col = (uint32_t)(q * L + lg) * 4u;
+              const uint2 t = valid ? *reinterpret_cast<const uint2*>(c.adjh + (size_t)slot * D + col) : make_uint2(0u, 0u);
+              r[u][q] = make_float4(sc * f16_to_f32((uint16_t)(t.x & 0xffffu)), sc * f16_to_f32((uint16_t)(t.x >> 16)),
+                                    sc * f16_to_f32((uint16_t)(t.y & 0xffffu)), sc * f16_to_f32((uint16_t)(t.y >> 16)));
+            }
+          } else {
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+              const uint32_t col = (uint32_t)(q * L + lg) * 4u;
+              r[u][q] = valid ? *reinterpret_cast<const float4*>(c.adj + (size_t)slot * D + col)
+                              : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
           }
         }
 #pragma unroll
@@ -1961,10 +2104,10 @@ __global__ __launch_bounds__(256) void k_seg_update(StepConfig c, StepState* st,
         const float lr = s_lr[bq];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-          adam_math(t4[q].x, m4[q].x, v4[q].x, acc[q].x, lr, sv.bc1, sv.bc2);
-          adam_math(t4[q].y, m4[q].y, v4[q].y, acc[q].y, lr, sv.bc1, sv.bc2);
-          adam_math(t4[q].z, m4[q].z, v4[q].z, acc[q].z, lr, sv.bc1, sv.bc2);
-          adam_math(t4[q].w, m4[q].w, v4[q].w, acc[q].w, lr, sv.bc1, sv.bc2);
+          adam_math(t4[q].x, m4[q].x, v4[q].x, acc[q].x, lr, sv.ibc1, sv.ibc2);
+          adam_math(t4[q].y, m4[q].y, v4[q].y, acc[q].y, lr, sv.ibc1, sv.ibc2);
+          adam_math(t4[q].z, m4[q].z, v4[q].z, acc[q].z, lr, sv.ibc1, sv.ibc2);
+          adam_math(t4[q].w, m4[q].w, v4[q].w, acc[q].w, lr, sv.ibc1, sv.ibc2);
         }
         upd = true;
       }
@@ -2053,14 +2196,34 @@ __device__ inline bool apply_run(const StepConfig& c, StepState* st, const StepS
   for (int q = 0; q < PL; ++q) bad |= (base + q < D) && isnan(acc[q]);
   if (wave_any(bad)) return false;
 #pragma unroll
-  for (int q = 0; q < PL; ++q) adam_math(th[q], m[q], v[q], acc[q], lr, sv.bc1, sv.bc2);
+  for (int q = 0; q < PL; ++q) adam_math(th[q], m[q], v[q], acc[q], lr, sv.ibc1, sv.ibc2);
   return true;
+}
+
+// acc += the first n of the U rows r[0..U), added as a pairwise tree: a long reduction
+// done U rows at a time then has a rounding error that grows with its length / U
+// instead of its length (U a power of two; rows past n count as zero).
+template <int U, int PL>
+__device__ inline void add_rows_tree(float (&acc)[PL], float (&r)[U][PL], uint32_t n) {
+  static_assert((U & (U - 1)) == 0, "U is a power of two");
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+#pragma unroll
+    for (int q = 0; q < PL; ++q) r[u][q] = (uint32_t)u < n ? r[u][q] : 0.f;
+#pragma unroll
+  for (int w = 1; w < U; w *= 2)
+#pragma unroll
+    for (int u = 0; u + w < U; u += 2 * w)
+#pragma unroll
+      for (int q = 0; q < PL; ++q) r[u][q] += r[u + w][q];
+#pragma unroll
+  for (int q = 0; q < PL; ++q) acc[q] += r[0][q];
 }
 
 // Partial sums of the pieces of long rows (each piece: <= kPieceEntries entries of one
 // run), then the medium rows of this wave's plan region: one wave per row (<= 128
 // entries), its runs reduced from the adjoint rows and applied in batch order.
-template <int D>
+template <int D, bool H>
 __global__ __launch_bounds__(256) void k_pieces(StepConfig c, StepState* st, const uint32_t* __restrict__ Ks,
                                                 const uint32_t* __restrict__ Ss) {
   constexpr int PL = RowShape<D>::PL;
@@ -2077,25 +2240,22 @@ __global__ __launch_bounds__(256) void k_pieces(StepConfig c, StepState* st, con
   const uint32_t np = st->n_pieces < c.piece_cap ? st->n_pieces : c.piece_cap;
   for (uint32_t gp = gw; gp < np; gp += nwaves) {
     const uint32_t s0 = c.piece_st[gp];
-    const uint32_t m = c.piece_info[gp];  // <= PIECE <= 64
-    const uint32_t sj = (uint32_t)lane < m ? (Ss[s0 + lane] & kSlotMask) : 0u;
+    const uint32_t m = c.piece_info[gp];  // a multiple of PIECE entries at most
     float acc[PL];
 #pragma unroll
     for (int q = 0; q < PL; ++q) acc[q] = 0.f;
-    for (uint32_t t0 = 0; t0 < m; t0 += U) {
-      float r[U][PL];
+    for (uint32_t c0 = 0; c0 < m; c0 += 64u) {  // 64 slots per lane-held chunk
+      const uint32_t mm = m - c0 < 64u ? m - c0 : 64u;
+      const uint32_t sj = (uint32_t)lane < mm ? (Ss[s0 + c0 + lane] & kSlotMask) : 0u;
+      for (uint32_t t0 = 0; t0 < mm; t0 += U) {
+        float r[U][PL];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const uint32_t t = (t0 + u < m) ? (t0 + u) : (m - 1u);
-        const uint32_t s = __shfl(sj, (int)t, EP_WAVE);
-        load_slice<D>(c.adj + (size_t)s * D, lane, r[u]);
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        if (t0 + u < m) {
-#pragma unroll
-          for (int q = 0; q < PL; ++q) acc[q] += r[u][q];
+        for (int u = 0; u < U; ++u) {
+          const uint32_t t = (t0 + u < mm) ? (t0 + u) : (mm - 1u);
+          const uint32_t s = __shfl(sj, (int)t, EP_WAVE);
+          load_adj_row<D, H>(c, s, lane, r[u]);
         }
+        add_rows_tree<U, PL>(acc, r, mm - t0);
       }
     }
     store_slice<D>(c.part + (size_t)gp * D, lane, acc);
@@ -2144,15 +2304,9 @@ __global__ __launch_bounds__(256) void k_pieces(StepConfig c, StepState* st, con
           const uint32_t t = (t0 + u < z) ? (t0 + u) : (z - 1u);
           const uint32_t sa = __shfl(ss0, (int)(t & 63u), EP_WAVE);
           const uint32_t sb = __shfl(ss1, (int)(t & 63u), EP_WAVE);
-          load_slice<D>(c.adj + (size_t)(t < 64u ? sa : sb) * D, lane, r[u]);
+          load_adj_row<D, H>(c, t < 64u ? sa : sb, lane, r[u]);
         }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          if (t0 + u < z) {
-#pragma unroll
-            for (int q = 0; q < PL; ++q) acc[q] += r[u][q];
-          }
-        }
+        add_rows_tree<U, PL>(acc, r, z - t0);
       }
       upd |= apply_run<D>(c, st, sv, s_lr[bq], s_sd[bq], f, bq, lane, acc, th, mo, va);
     }
@@ -2166,8 +2320,9 @@ __global__ __launch_bounds__(256) void k_pieces(StepConfig c, StepState* st, con
 
 // One wave per long row: loads theta/m/v once, adds the pieces of each run in order and
 // applies one Adam step per run (its batch's learning rate), stores the row once.
-template <int D>
-__global__ __launch_bounds__(256) void k_long_combine(StepConfig c, StepState* st, const uint32_t* __restrict__ Ks) {
+template <int D, bool H>
+__global__ __launch_bounds__(256) void k_long_combine(StepConfig c, StepState* st, const uint32_t* __restrict__ Ks,
+                                                      const uint32_t* __restrict__ Ss) {
   constexpr int PL = RowShape<D>::PL;
   constexpr int U = SR_UNROLL;
   const int lane = threadIdx.x & 63;
@@ -2190,8 +2345,10 @@ __global__ __launch_bounds__(256) void k_long_combine(StepConfig c, StepState* s
     if (!c.grad_out) load_row_state<D>(c, f, lane, th, m, v);
     bool upd = false;
     uint32_t pb = pb0;
+    uint32_t cur = c.long_st[idx];  // first entry of the current run
     for (uint32_t jr = 0; jr < nruns && jr < kMaxGroup; ++jr) {
       const uint32_t rinfo = c.long_runs[(size_t)idx * kMaxGroup + jr];
+      const uint32_t rlen = c.long_rlen[(size_t)idx * kMaxGroup + jr];
       const uint32_t bq = (rinfo >> 24) & 7u;
       uint32_t pcs = rinfo & 0xffffffu;
       if (pb >= c.piece_cap) pcs = 0u;
@@ -2199,6 +2356,19 @@ __global__ __launch_bounds__(256) void k_long_combine(StepConfig c, StepState* s
       float acc[PL];
 #pragma unroll
       for (int q = 0; q < PL; ++q) acc[q] = 0.f;
+      if ((rinfo & 0xffffffu) == 0u) {
+        // a run of at most PIECE entries: its adjoint rows in entry order
+        const uint32_t sj = (uint32_t)lane < rlen ? (Ss[cur + lane] & kSlotMask) : 0u;
+        for (uint32_t t0 = 0; t0 < rlen; t0 += U) {
+          float r[U][PL];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const uint32_t t = (t0 + u < rlen) ? (t0 + u) : (rlen - 1u);
+            load_adj_row<D, H>(c, __shfl(sj, (int)t, EP_WAVE), lane, r[u]);
+          }
+          add_rows_tree<U, PL>(acc, r, rlen - t0);
+        }
+      }
       for (uint32_t j0 = 0; j0 < pcs; j0 += U) {
         float r[U][PL];
 #pragma unroll
@@ -2206,15 +2376,10 @@ __global__ __launch_bounds__(256) void k_long_combine(StepConfig c, StepState* s
           const uint32_t jj = (j0 + u < pcs) ? j0 + u : pcs - 1u;
           load_slice<D>(c.part + (size_t)(pb + jj) * D, lane, r[u]);
         }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          if (j0 + u < pcs) {
-#pragma unroll
-            for (int q = 0; q < PL; ++q) acc[q] += r[u][q];
-          }
-        }
+        add_rows_tree<U, PL>(acc, r, pcs - j0);
       }
       pb += rinfo & 0xffffffu;
+      cur += rlen;
       upd |= apply_run<D>(c, st, sv, s_lr[bq], s_sd[bq], f, bq, lane, acc, th, m, v);
     }
     if (upd) {
@@ -2553,9 +2718,8 @@ hipError_t launch_scan(const StepConfig& c, StepState* st, hipStream_t s) {
 
 size_t forward_lds_bytes(const StepConfig& c) {
   const uint32_t NC = c.K + 2;
-  const size_t hash = (c.train && c.fuse_emit && c.grad_mode == 0) ? (size_t)c.hash_entries * 4 : 0;
   return (size_t)forward_float_words(NC, c.K, c.dims) * 4 + (size_t)c.ent_cap * 8 + (size_t)c.sel_cap * 12 +
-         (size_t)(c.ent_cap + 1) * 4 + (size_t)c.ids_cap * 4 + hash;
+         (size_t)(c.ent_cap + 1) * 4 + (size_t)c.ids_cap * 4;
 }
 
 hipError_t launch_forward(const StepConfig& c, StepState* st, hipStream_t s) {
@@ -2627,43 +2791,51 @@ bool seg_variant_supported(int d, int vpl, int unroll) {
   return unroll == 8 / nq || unroll == 16 / nq;
 }
 
-template <int D, int VPL, int U>
+template <int D, int VPL, int U, bool H>
 static bool try_seg_update(const StepConfig& c, StepState* st, int buf, hipStream_t s, int grid) {
   if constexpr (D % VPL == 0 && D / VPL <= 64) {
     if (c.seg_vpl == VPL && c.seg_unroll == U) {
-      hipLaunchKernelGGL((k_seg_update<D, VPL, U>), dim3(grid), dim3(256), 0, s, c, st, c.slots[buf]);
+      hipLaunchKernelGGL((k_seg_update<D, VPL, U, H>), dim3(grid), dim3(256), 0, s, c, st, c.slots[buf]);
       return true;
     }
   }
   return false;
 }
 
-template <int D>
+template <int D, bool H>
 static void launch_seg_update_variant(const StepConfig& c, StepState* st, int buf, hipStream_t s, int grid) {
-  if (try_seg_update<D, 16, 2>(c, st, buf, s, grid) || try_seg_update<D, 16, 4>(c, st, buf, s, grid) ||
-      try_seg_update<D, 12, 2>(c, st, buf, s, grid) || try_seg_update<D, 12, 4>(c, st, buf, s, grid) ||
-      try_seg_update<D, 8, 4>(c, st, buf, s, grid) || try_seg_update<D, 8, 8>(c, st, buf, s, grid) ||
-      try_seg_update<D, 4, 16>(c, st, buf, s, grid) || try_seg_update<D, 4, 8>(c, st, buf, s, grid))
+  if (try_seg_update<D, 16, 2, H>(c, st, buf, s, grid) || try_seg_update<D, 16, 4, H>(c, st, buf, s, grid) ||
+      try_seg_update<D, 12, 2, H>(c, st, buf, s, grid) || try_seg_update<D, 12, 4, H>(c, st, buf, s, grid) ||
+      try_seg_update<D, 8, 4, H>(c, st, buf, s, grid) || try_seg_update<D, 8, 8, H>(c, st, buf, s, grid) ||
+      try_seg_update<D, 4, 16, H>(c, st, buf, s, grid) || try_seg_update<D, 4, 8, H>(c, st, buf, s, grid))
     return;
   // a width no variant divides: the widest built form
   if constexpr (D % 16 == 0 && D / 16 <= 64)
-    hipLaunchKernelGGL((k_seg_update<D, 16, 2>), dim3(grid), dim3(256), 0, s, c, st, c.slots[buf]);
+    hipLaunchKernelGGL((k_seg_update<D, 16, 2, H>), dim3(grid), dim3(256), 0, s, c, st, c.slots[buf]);
   else
-    hipLaunchKernelGGL((k_seg_update<D, 4, 8>), dim3(grid), dim3(256), 0, s, c, st, c.slots[buf]);
+    hipLaunchKernelGGL((k_seg_update<D, 4, 8, H>), dim3(grid), dim3(256), 0, s, c, st, c.slots[buf]);
+}
+
+template <int D, bool H>
+static void launch_row_kernels(const StepConfig& c, StepState* st, int buf, hipStream_t s, int grid, hipEvent_t* ev) {
+  launch_seg_update_variant<D, H>(c, st, buf, s, grid);
+  if (ev) (void)hipEventRecord(ev[1], s);
+  hipLaunchKernelGGL((k_pieces<D, H>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
+  if (ev) (void)hipEventRecord(ev[2], s);
+  hipLaunchKernelGGL((k_long_combine<D, H>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
+  if (ev) (void)hipEventRecord(ev[3], s);
 }
 
 hipError_t launch_scatter_sorted(const StepConfig& c, StepState* st, int buf, hipStream_t s, hipEvent_t* ev) {
   const int grid = c.unique_grid;
   hipLaunchKernelGGL(k_seg_plan, dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
   if (ev) (void)hipEventRecord(ev[0], s);
-  EP_ROW_DISPATCH(c.dims, {
-    launch_seg_update_variant<D>(c, st, buf, s, grid);
-    if (ev) (void)hipEventRecord(ev[1], s);
-    hipLaunchKernelGGL((k_pieces<D>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
-    if (ev) (void)hipEventRecord(ev[2], s);
-    hipLaunchKernelGGL((k_long_combine<D>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf]);
-    if (ev) (void)hipEventRecord(ev[3], s);
-  });
+  if (c.adjh) {
+    // FP16 adjoint rows: the mean model's widths
+    EP_DISPATCH(c.dims, { (launch_row_kernels<D, true>)(c, st, buf, s, grid, ev); });
+  } else {
+    EP_ROW_DISPATCH(c.dims, { (launch_row_kernels<D, false>)(c, st, buf, s, grid, ev); });
+  }
   return hipGetLastError();
 }
 

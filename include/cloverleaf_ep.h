@@ -123,6 +123,11 @@ typedef struct ep_config {
   float max_features_p;        /* p in (0, 1) when max_features is Probability      */
   float max_nodes_p;           /* p in (0, 1) when max_nodes is Probability (mean   */
                                /* model)                                            */
+  int32_t fp16_adjoints;       /* 1: with a BF16 table (mean model, sort scatter)   */
+                               /* the per-construction adjoint rows the update      */
+                               /* gathers are held in FP16 with a power-of-two scale */
+                               /* per row (2^-11 relative to the row's largest      */
+                               /* value); 0: FP32 adjoint rows (default)            */
 } ep_config;
 
 typedef struct ep_ctx ep_ctx;
