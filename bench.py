@@ -1,29 +1,36 @@
 """EP training-step benchmark on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ep-cfg2-1m] [--forms ...]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ep-cfg4-50m] [--forms ...]
 
-Workload (default): the 1M-node bipartite query-item graph (300k queries, 700k items,
-20M undirected edges, Chung-Lu beta 2.2; features U{1..32} Zipf(1.1) over a 1M vocab;
-d=128, B=1024, K=10 negatives, S=20 neighbours, margin 0.5).  Synthetic, seeded inputs.
+Workload (default): the 50M-node mixed-precision configuration: undirected Chung-Lu graph,
+50M nodes, 1B edges (1.99B CSR entries), beta 2.2; features U{1..64} Zipf(1.1) over a 20M
+vocabulary; d=256, B=16384, K=20 negatives, S=20 neighbours, margin 0.5, lr 0.01.
+Synthetic, seeded inputs; random-init table.
 
 Forms timed on the same graph:
-  fp32-plain    FP32 table, one batch per step, one launch per kernel (no graph), the
-                default segment tile (no tuning), sort + segment-reduce scatter
-  fp32-batched  FP32 table, --concurrent batches per step (they read the same table and
-                update in batch order, like the reference's concurrently running
-                batches), steps replayed from a captured hipGraph, tuned segment tile
-  bf16-batched  the same with the BF16 table (FP32 master of the touched rows during the
-                update, stochastic rounding on write-back)
-The JSON `value` is the examples/s of the headline form (bf16-batched).  Every example
-is one anchor taken through h(v), ~h(v), K negatives, the margin loss, the hand-written
-backward and its batch's sparse Adam update.  A "step" in `ms_per_step` is one launch
-sequence of the headline form, i.e. --concurrent reference batches.
+  fp32-plain           FP32 table, one batch per step, one launch per kernel (no graph),
+                       the default segment tile (no tuning), sort + segment-reduce scatter
+  bf16-master-batched  BF16 table + persistent FP32 master rows, stochastic-rounding
+                       write-back; --concurrent batches per step (they read the same table
+                       and update in batch order, like the reference's concurrently running
+                       batches), steps replayed from a captured hipGraph, tuned segment tile,
+                       FP16 adjoint rows (--fp16-adjoints)
+  bf16-batched         the same without the persistent master (each touched row is taken to
+                       FP32 for its update)
+  fp32-batched         FP32 table, batched as above
+The JSON `value` is the examples/s of the headline form (bf16-master-batched, the
+configuration's BF16 table + FP32 master + SR).  Every example is one anchor taken through
+h(v), ~h(v), K negatives, the margin loss, the hand-written backward and its batch's
+sparse Adam update.  A "step" in `ms_per_step` is one launch sequence of the headline form,
+i.e. --concurrent reference batches.
 
 `roofline` is the HBM roofline of the dominant kernel of the headline form (the
 short-row update, or the forward gather when that takes longer): its algorithmic bytes
 per launch over its average launch duration, timed with HIP events on the context's
-stream.  `roofline_step` applies the same algorithmic-byte model to the whole step.
-`cpu_baseline` times the C++ restatement of the reference's rayon loop on the host
+stream.  `roofline_gather` is the forward's: the distinct feature rows one launch reads
+(counted on the host from a step's sampled lists) over its event-timed duration.
+`roofline_step` applies the algorithmic-byte model (rows with multiplicity) to the whole
+step.  `cpu_baseline` times the C++ restatement of the reference's rayon loop on the host
 cores (bounded sample of the same workload).
 """
 from __future__ import annotations
@@ -31,7 +38,6 @@ from __future__ import annotations
 import argparse
 import json
 import os
-import subprocess
 import sys
 import time
 
@@ -51,8 +57,10 @@ def gather_bytes(rows, nodes, examples, dims, bytes_per_value):
 
 
 def update_bytes(urows, dims, bytes_per_value, master=False):
-    """Per updated row: theta read+write (2b), m and v read+write (16), FP32 master (8)."""
-    return urows * dims * (2 * bytes_per_value + 16 + (8 if master else 0))
+    """Per updated value: theta read + write (2b), m and v read + write (16); with a
+    persistent FP32 master the master is read and written (8) and the table row only
+    written (b)."""
+    return urows * dims * ((bytes_per_value + 24) if master else (2 * bytes_per_value + 16))
 
 
 def attention_flops(att_items_sq, att_items, heads, dk, dm):
@@ -63,7 +71,7 @@ def attention_flops(att_items_sq, att_items, heads, dk, dm):
     return fwd, 3 * fwd
 
 
-def run_form(w, form, steps, warmup, scatter=None, concurrent=1, graph=True):
+def run_form(w, form, steps, warmup, scatter=None, concurrent=1, graph=True, fp16_adjoints=False):
     from cloverleaf_amd import _native as N
 
     cfg = N.default_config()
@@ -82,6 +90,7 @@ def run_form(w, form, steps, warmup, scatter=None, concurrent=1, graph=True):
         cfg.use_graph, cfg.concurrent_batches = (1 if graph else 0), concurrent
     prec = form.rsplit("-", 1)[0]
     cfg.precision = {"fp32": N.EP_PREC_FP32, "bf16": N.EP_PREC_BF16, "bf16-master": N.EP_PREC_BF16_MASTER}[prec]
+    cfg.fp16_adjoints = 1 if (fp16_adjoints and prec != "fp32" and not w.att_dk and cfg.scatter == N.EP_SCATTER_SORT) else 0
     with N.Context(cfg) as ctx:
         ctx.set_graph(w.indptr, w.indices)
         ctx.set_features(w.f_indptr, w.f_ids, w.vocab)
@@ -89,6 +98,7 @@ def run_form(w, form, steps, warmup, scatter=None, concurrent=1, graph=True):
         train = np.arange(w.n_nodes, dtype=np.uint32)
         ctx.set_split(train, np.zeros(0, dtype=np.uint32))
         r = ctx.bench_steps(warmup, steps, instrument=True)
+        audit = gather_audit(ctx, w, int(cfg.concurrent_batches) * w.batch)
     b = 2 if prec.startswith("bf16") else 4
     S = max(r.steps, 1)
     ph = [float(x) for x in r.ms_phase]
@@ -103,10 +113,18 @@ def run_form(w, form, steps, warmup, scatter=None, concurrent=1, graph=True):
         extra = {"attention_blocks_items_sq_per_step": r.att_items_sq / S,
                  "attention_flops_per_step": fb / S,
                  "attention_tflops": fb / S / (ph[1] / 1e3) / 1e12 if ph[1] > 0 else 0.0}
+    u_rows, occ, n_nodes = audit
+    # the forward's distinct rows per launch (P batches), ids and node offsets
+    g_unique = u_rows * W * b + 4 * occ + 16 * n_nodes
     return {
         "form": form,
+        "gather_unique_rows_per_launch": int(u_rows),
+        "gather_occurrences_per_launch": int(occ),
+        "gather_unique_bytes_per_launch": g_unique,
+        "gather_unique_gbs": g_unique / (ph[1] / 1e3) / 1e9 if ph[1] > 0 else 0.0,
         "scatter": scatter or "sort",
         "concurrent_batches": int(cfg.concurrent_batches),
+        "adjoint_rows": "fp16" if cfg.fp16_adjoints else "fp32",
         "seg_tile": [int(r.seg_vpl), int(r.seg_unroll)],
         "examples_per_s": r.examples / (r.ms_total / 1e3),
         "ms_per_step": r.ms_total / S,
@@ -127,33 +145,71 @@ def run_form(w, form, steps, warmup, scatter=None, concurrent=1, graph=True):
     }
 
 
-def cpu_baseline(w, budget_s=15.0):
-    """Times the C++ restatement of the reference loop (oracle/ep_cpu) on the host cores."""
-    exe = os.path.join(ROOT, "oracle", "_build", "ep_cpu")
-    if not os.path.exists(exe):
-        return None
-    import tempfile
+def gather_audit(ctx, w, n_anchors):
+    """Distinct feature rows and feature occurrences the forward reads for one step of
+    n_anchors anchors: the GPU sampler draws the step's neighbour and negative lists, the
+    host expands every node of every example with a neighbourhood to its feature ids."""
+    rng = np.random.default_rng(123)
+    anchors = rng.choice(w.n_nodes, size=min(n_anchors, w.n_nodes), replace=False).astype(np.uint32)
+    cnt, nbr, neg = ctx.sample_batch(anchors, 1)
+    live = cnt > 0
+    S = nbr.shape[1]
+    picked = nbr[live][np.arange(S)[None, :] < cnt[live][:, None]]
+    nodes = np.concatenate([anchors[live], picked, neg[live].ravel()]).astype(np.int64)
+    fi = np.asarray(w.f_indptr, dtype=np.int64)
+    starts, lens = fi[nodes], fi[nodes + 1] - fi[nodes]
+    tot = int(lens.sum())
+    idx = np.repeat(starts - np.concatenate([[0], np.cumsum(lens)[:-1]]), lens) + np.arange(tot)
+    return len(np.unique(np.asarray(w.f_ids)[idx])), tot, len(nodes)
 
+
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(w, budget_s=15.0):
+    """Times the C++ restatement of the reference's rayon loop (oracle/_build/libep_cpu.so,
+    ep_cpu_bench) on the host cores, in process on the workload's own arrays: examples of a
+    batch in parallel, O(deg) reservoir neighbours, first-leaf hash maps, per-batch merge,
+    sparse Adam, FP32.  The clock starts after the table initialisation; the sample is the
+    batches finished within budget_s seconds (the last one may run past it)."""
+    import ctypes
+    from ctypes import POINTER, c_double, c_float, c_uint32, c_uint64
+
+    path = os.path.join(ROOT, "oracle", "_build", "libep_cpu.so")
+    if not os.path.exists(path):
+        return None
+    lib = ctypes.CDLL(path)
+    u64p, u32p = POINTER(c_uint64), POINTER(c_uint32)
+    lib.ep_cpu_bench.argtypes = [u64p, u32p, u64p, u32p, c_uint64, c_uint32, c_uint32, c_uint32, c_uint32, c_uint32,
+                                 c_float, c_float, c_uint32, c_uint32, c_uint32, c_float, POINTER(c_double)]
+    lib.ep_cpu_bench.restype = ctypes.c_int
     cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
-    with tempfile.TemporaryDirectory() as td:
-        np.asarray(w.indptr, dtype=np.uint64).tofile(os.path.join(td, "indptr.bin"))
-        np.asarray(w.indices, dtype=np.uint32).tofile(os.path.join(td, "indices.bin"))
-        np.asarray(w.f_indptr, dtype=np.uint64).tofile(os.path.join(td, "findptr.bin"))
-        np.asarray(w.f_ids, dtype=np.uint32).tofile(os.path.join(td, "fids.bin"))
-        cmd = [exe, "--dir", td, "--nodes", str(w.n_nodes), "--vocab", str(w.vocab), "--dims", str(w.dims),
-               "--batch", str(w.batch), "--negatives", str(w.negatives), "--max-nodes", str(w.max_nodes),
-               "--gamma", str(w.gamma), "--alpha", str(w.alpha), "--threads", str(cores),
-               "--budget-s", str(budget_s), "--json"]
-        if w.att_dk:
-            cmd += ["--heads", str(w.att_heads), "--dk", str(w.att_dk)]
-        out = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=budget_s * 6 + 60)
-        if out.returncode != 0:
-            print(out.stderr, file=sys.stderr)
-            return None
-        res = json.loads(out.stdout.strip().splitlines()[-1])
-    return {"value": res["examples_per_s"], "unit": "examples/s", "cores": cores, "kind": "port",
-            "sample": f"{res['examples']} anchors ({res['batches']} concurrent batches of {w.batch}) of the same "
-                      f"workload, hogwild sparse Adam, FP32, C++ restatement of the reference's rayon loop"
+    arrs = [np.ascontiguousarray(w.indptr, dtype=np.uint64), np.ascontiguousarray(w.indices, dtype=np.uint32),
+            np.ascontiguousarray(w.f_indptr, dtype=np.uint64), np.ascontiguousarray(w.f_ids, dtype=np.uint32)]
+    ptrs = [a.ctypes.data_as(u64p if a.dtype == np.uint64 else u32p) for a in arrs]
+    out = (c_double * 3)()
+    t0 = time.time()
+    rc = lib.ep_cpu_bench(*ptrs, w.n_nodes, w.vocab, w.dims, w.batch, w.negatives, w.max_nodes, w.gamma, w.alpha,
+                          w.att_heads if w.att_dk else 0, w.att_dk, cores, budget_s, out)
+    if rc != 0:
+        return None
+    examples, seconds, batches = out[0], out[1], int(out[2])
+    print(f"[bench] cpu baseline: {examples:.0f} anchors in {seconds:.1f} s on {cores} threads "
+          f"({time.time() - t0:.1f} s with the table initialisation)", file=sys.stderr, flush=True)
+    return {"value": round(examples / seconds, 1), "unit": "examples/s", "cores": cores, "kind": "port",
+            "cpu": cpu_model(),
+            "sample": f"{examples:.0f} anchors ({batches} batches of {w.batch}, each batch's examples, merge and "
+                      f"sparse Adam over all {cores} threads) of the same workload, FP32, C++ restatement of the "
+                      f"reference's rayon loop"
                       + (" (attention aggregator)" if w.att_dk else "")}
 
 
@@ -211,17 +267,21 @@ def bench_align(w, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="ep-cfg2-1m")
+    ap.add_argument("--workload", default="ep-cfg4-50m")
     ap.add_argument("--scale", type=float, default=1.0)
-    ap.add_argument("--forms", default="fp32-plain,fp32-batched,bf16-batched")
+    ap.add_argument("--graph-scale", type=float, default=1.0,
+                    help="shrink only the graph (nodes, edges); vocabulary and model shape unchanged")
+    ap.add_argument("--forms", default="fp32-plain,bf16-master-batched,bf16-batched")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--scatters", default="sort", help="scatter paths tried for the batched forms (sort,atomic)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--concurrent", type=int, default=0,
                     help="reference batches per step in the batched forms (the plain forms use 1); default 8, "
                          "2 for attention workloads (one gradient row per feature occurrence)")
+    ap.add_argument("--fp16-adjoints", type=int, default=1,
+                    help="1: the BF16 forms hold the adjoint rows the update gathers in FP16 (per-row scale)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the batched forms eagerly (for hardware-counter runs)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -252,7 +312,7 @@ def main():
         del src, dst
 
     t0 = time.time()
-    w = synth.make_workload(args.workload, scale=args.scale,
+    w = synth.make_workload(args.workload, scale=args.scale, graph_scale=args.graph_scale,
                             log=lambda m: print(f"[bench] {m} ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True))
     print(f"[bench] workload {w.name}: {w.n_nodes} nodes, {len(w.indices)} CSR entries, vocab {w.vocab} "
           f"(generated in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
@@ -270,10 +330,10 @@ def main():
             if sc == "atomic" and args.concurrent > 1:
                 continue  # the atomic path runs one batch per step
             r = run_form(w, form, args.steps, args.warmup, scatter=sc, concurrent=args.concurrent,
-                         graph=not args.no_graph)
+                         graph=not args.no_graph, fp16_adjoints=bool(args.fp16_adjoints))
             ph = " ".join(f"{k}={v * 1e3:.0f}" for k, v in r["phase_ms"].items())
             print(f"[bench] {form}/{sc}: {r['examples_per_s']:.0f} ex/s, {r['ms_per_step']:.3f} ms/step "
-                  f"(P {r['concurrent_batches']}, tile {r['seg_tile']}, {r['unique_rows_per_step']:.0f} rows/step, "
+                  f"(P {r['concurrent_batches']}, adj {r['adjoint_rows']}, tile {r['seg_tile']}, {r['unique_rows_per_step']:.0f} rows/step, "
                   f"{r['short_rows_per_step']:.0f} short, {r['long_rows_last_step']} long, "
                   f"{r['pieces_last_step']} pieces); "
                   f"phases us: {ph}; fwd {r['forward_gbs']:.0f} GB/s, update {r['update_gbs']:.0f} GB/s, "
@@ -285,7 +345,7 @@ def main():
         if best is not None:
             results[form] = best
 
-    head_form = "bf16-batched" if "bf16-batched" in results else list(results)[0]
+    head_form = next((f for f in ("bf16-master-batched", "bf16-batched") if f in results), list(results)[0])
     head = results[head_form]
     cpu = None if args.no_cpu else cpu_baseline(w, args.cpu_budget)
     ph = head["phase_ms"]
@@ -340,6 +400,19 @@ def main():
             "frac": round(kgbs / HBM_PEAK_GBS, 4),
             "traffic": pmc_traffic(kname, w.name, head_form),
             "algorithmic_bytes_per_launch": round(kbytes),
+        },
+        "roofline_gather": {
+            "bound": "hbm",
+            "kernel": "k_forward (gather + mean + loss + adjoint + key emission): the distinct feature rows of "
+                      "the launch's examples, their ids and node offsets, over its event-timed duration",
+            "achieved": round(head["gather_unique_gbs"], 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(head["gather_unique_gbs"] / HBM_PEAK_GBS, 4),
+            "traffic": pmc_traffic("k_forward", w.name, head_form),
+            "algorithmic_bytes_per_launch": round(head["gather_unique_bytes_per_launch"]),
+            "unique_rows_per_launch": head["gather_unique_rows_per_launch"],
+            "occurrences_per_launch": head["gather_occurrences_per_launch"],
         },
         "roofline_step": {
             "bound": "hbm",

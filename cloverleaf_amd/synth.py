@@ -167,12 +167,20 @@ WORKLOADS = {
 }
 
 
-def make_workload(name: str, scale: float = 1.0, seed_graph: int = 1, seed_feat: int = 2, log=None) -> Workload:
-    """Generates a named workload; `scale` < 1 shrinks nodes, edges and vocab proportionally.
+def make_workload(name: str, scale: float = 1.0, seed_graph: int = 1, seed_feat: int = 2, log=None,
+                  graph_scale: float = 1.0) -> Workload:
+    """Generates a named workload; `scale` < 1 shrinks nodes, edges and vocab proportionally;
+    `graph_scale` < 1 shrinks only the nodes and edges (the vocabulary, the per-node feature
+    distribution and the model shape stay those of the workload).
     `log` (optional) receives a line when the graph is done."""
     spec = WORKLOADS[name]
     kind, g = spec["graph"]
     g = dict(g)
+    if graph_scale != 1.0:
+        for k in ("n", "m", "nq", "ni"):
+            if k in g:
+                g[k] = max(16, int(g[k] * graph_scale))
+        g["dmax"] = g["dmax"] * min(1.0, graph_scale)
     comm = 0
     if kind == "communities":
         n = max(16, int(g["n"] * scale))

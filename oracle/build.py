@@ -11,6 +11,8 @@ HERE = Path(__file__).resolve().parent
 
 
 def build() -> Path:
+    """oracle/_build/ep_cpu (command line: AUC runs, explicit-list batches) and
+    oracle/_build/libep_cpu.so (ep_cpu_bench, the in-process CPU baseline)."""
     out = HERE / "_build"
     out.mkdir(exist_ok=True)
     exe = out / "ep_cpu"
@@ -18,6 +20,11 @@ def build() -> Path:
     tmp = out / "ep_cpu.tmp"
     subprocess.run([cxx, "-O3", "-std=c++17", "-pthread", "-o", str(tmp), str(HERE / "ep_cpu.cpp")], check=True)
     os.replace(tmp, exe)
+    lib = out / "libep_cpu.so"
+    tmp = out / "libep_cpu.so.tmp"
+    subprocess.run([cxx, "-O3", "-std=c++17", "-pthread", "-fPIC", "-shared", "-DEP_CPU_NO_MAIN", "-o", str(tmp),
+                    str(HERE / "ep_cpu.cpp")], check=True)
+    os.replace(tmp, lib)
     return exe
 
 

@@ -4,14 +4,16 @@
 // holdout-AUC comparison.  Never linked into the shipped library.
 //
 // Follows src/algos/ep/mod.rs:92-287 with the reference's parallel structure:
-//   * batches run concurrently on a thread pool (mod.rs:174, rayon par_iter over chunks);
-//   * per node: h(v), ~h(v), K negatives, margin loss, gradients (mod.rs:183-215);
+//   * per node, in parallel over the threads (mod.rs:183): h(v), ~h(v), K negatives,
+//     margin loss, gradients (mod.rs:183-215);
 //   * per-node RNG XorShift seeded with seed + batch_idx + node_id (mod.rs:185);
 //   * ~h(v) neighbours by reservoir sampling over the whole row (utils.rs:128-145,
 //     model.rs:354-377): O(deg) like the reference;
 //   * first-leaf gradient extraction with a per-example hash map (mod.rs:329-378);
-//   * per-batch HashMap merge (mod.rs:223-233);
-//   * Hogwild sparse Adam, t = pass + 1, unsynchronised row writes (optimizer.rs:43-81);
+//   * per-batch merge of the examples' gradient maps (mod.rs:223-233), thread maps summed
+//     in shards by feature id;
+//   * sparse Adam over the merged rows, t = pass + 1, the shards in parallel
+//     (optimizer.rs:43-81);
 //   * cosine LR with 20% warm-up (scheduler.rs:39-47, mod.rs:132-137).
 // The mean aggregator with max_features = All and MarginLoss (loss.rs:59-73); with
 // --heads H --dk d_k the attention aggregator instead (AttentionFeatureModel,
@@ -83,13 +85,38 @@ std::vector<T> read_bin(const std::string& path) {
   return v;
 }
 
+// Zero-initialised float buffer from calloc: pages are committed on first touch, so the
+// Adam moments of a large vocabulary cost memory only for the rows a run updates.
+struct FBuf {
+  float* p = nullptr;
+  size_t n = 0;
+  FBuf() = default;
+  FBuf(const FBuf&) = delete;
+  FBuf& operator=(const FBuf&) = delete;
+  ~FBuf() { std::free(p); }
+  void zeros(size_t count) {
+    std::free(p);
+    p = (float*)std::calloc(std::max<size_t>(count, 1), sizeof(float));
+    if (!p) { std::fprintf(stderr, "out of memory (%zu floats)\n", count); std::exit(2); }
+    n = count;
+  }
+  void assign(const std::vector<float>& v) {
+    zeros(v.size());
+    std::copy(v.begin(), v.end(), p);
+  }
+  float& operator[](size_t i) { return p[i]; }
+  const float& operator[](size_t i) const { return p[i]; }
+  size_t size() const { return n; }
+  float* data() { return p; }
+};
+
 struct Model {
   const Args& a;
-  const std::vector<uint64_t>& indptr;
-  const std::vector<uint32_t>& indices;
-  const std::vector<uint64_t>& findptr;
-  const std::vector<uint32_t>& fids;
-  std::vector<float> table, mom, var;
+  const uint64_t* indptr;
+  const uint32_t* indices;
+  const uint64_t* findptr;
+  const uint32_t* fids;
+  FBuf table, mom, var;
   std::vector<uint32_t> train;
 
   // NodeCounts: feature -> count in first-seen order (model.rs:254-278)
@@ -189,7 +216,7 @@ struct Model {
 
   // Items of a neighbour's attention inside ~h(v): the listed features, count = multiplicity
   void neighbour_items(uint32_t node, std::vector<uint32_t>& f, std::vector<float>& c) const {
-    f.assign(fids.begin() + findptr[node], fids.begin() + findptr[node + 1]);
+    f.assign(fids + findptr[node], fids + findptr[node + 1]);
     c.resize(f.size());
     for (size_t i = 0; i < f.size(); ++i) c[i] = (float)std::count(f.begin(), f.end(), f[i]);
   }
@@ -221,9 +248,9 @@ struct Model {
     const float scale = (float)deg / (float)r;
     std::vector<uint32_t> nbrs;
     if (deg <= S) {
-      nbrs.assign(indices.begin() + s0, indices.begin() + s1);
+      nbrs.assign(indices + s0, indices + s1);
     } else {
-      nbrs.assign(indices.begin() + s0, indices.begin() + s0 + S);
+      nbrs.assign(indices + s0, indices + s0 + S);
       for (uint32_t i = S; i < deg; ++i) {
         const uint32_t idx = rng.below(i);
         if (idx < S) nbrs[idx] = indices[s0 + i];
@@ -379,12 +406,15 @@ float cos_decay(uint64_t step, float alpha, uint64_t total) {
 }
 
 template <typename T>
-void write_bin(const std::string& path, const std::vector<T>& v) {
+void write_raw(const std::string& path, const T* p, size_t n) {
   FILE* f = std::fopen(path.c_str(), "wb");
   if (!f) { std::fprintf(stderr, "cannot write %s\n", path.c_str()); std::exit(2); }
-  std::fwrite(v.data(), sizeof(T), v.size(), f);
+  std::fwrite(p, sizeof(T), n, f);
   std::fclose(f);
 }
+template <typename T>
+void write_bin(const std::string& path, const std::vector<T>& v) { write_raw(path, v.data(), v.size()); }
+void write_bin(const std::string& path, FBuf& v) { write_raw(path, v.data(), v.size()); }
 
 // One batch on explicit lists, sequentially: dir holds table.bin (vocab x width f32),
 // anchors.bin (n u32), cnt.bin (n u32), nbr.bin (n x max_nodes u32, the first cnt[i]
@@ -397,10 +427,10 @@ int run_explicit(Model& m) {
   const Args& a = m.a;
   const std::string d = a.explicit_dir;
   const uint32_t W = m.width();
-  m.table = read_bin<float>(d + "/table.bin");
+  m.table.assign(read_bin<float>(d + "/table.bin"));
   if (m.table.size() != (size_t)a.vocab * W) { std::fprintf(stderr, "table size mismatch\n"); return 2; }
-  m.mom.assign(m.table.size(), 0.f);
-  m.var.assign(m.table.size(), 0.f);
+  m.mom.zeros(m.table.size());
+  m.var.zeros(m.table.size());
   const auto anchors = read_bin<uint32_t>(d + "/anchors.bin");
   const auto cnt = read_bin<uint32_t>(d + "/cnt.bin");
   const auto nbr = read_bin<uint32_t>(d + "/nbr.bin");
@@ -433,8 +463,165 @@ int run_explicit(Model& m) {
   return 0;
 }
 
+
+struct RunStats {
+  uint64_t examples = 0, batches = 0;
+  double seconds = 0.0, train_error = 0.0;
+};
+
+// randomize_embedding_store (mod.rs:381-392): x ~ U(-1, 1), then each row / its norm;
+// rows drawn in parallel from per-row XorShift streams.
+void init_table(Model& m) {
+  const Args& a = m.a;
+  const uint32_t D = m.width();
+  m.table.zeros((size_t)a.vocab * D);
+  m.mom.zeros(m.table.size());
+  m.var.zeros(m.table.size());
+  const uint32_t T = std::max(1u, a.threads);
+  std::vector<std::thread> ts;
+  for (uint32_t t = 0; t < T; ++t)
+    ts.emplace_back([&, t]() {
+      for (uint32_t r = t; r < a.vocab; r += T) {
+        XorShift init(a.seed ^ (0x9E3779B97F4A7C15ull * (r + 1ull)));
+        float* e = &m.table[(size_t)r * D];
+        float n = 0.f;
+        for (uint32_t d = 0; d < D; ++d) { e[d] = 2.f * init.uniform() - 1.f; n += e[d] * e[d]; }
+        n = std::sqrt(n);
+        for (uint32_t d = 0; d < D; ++d) e[d] /= n;
+      }
+    });
+  for (auto& t : ts) t.join();
+}
+
+// The training loop (mod.rs:156-265): shuffled passes; each batch's examples run in
+// parallel on all threads (the per-node par_iter of mod.rs:183), each thread merging its
+// examples' first-leaf gradients into its own map; the batch merge then sums the thread
+// maps shard by shard (feature id mod threads) and every shard applies its rows' Adam
+// step (optimizer.rs:51 runs the update in parallel over rows).  Batches run one after
+// the other (the reference also overlaps batches on the rayon pool; with every core
+// busy inside each batch the throughput is the same, and no update reads a stale table).
+// Stops after the first batch past budget_s.  The clock starts after the table
+// initialisation.
+RunStats run_training(Model& m) {
+  const Args& a = m.a;
+  const uint32_t D = m.width();
+  init_table(m);
+  if (m.train.empty()) { m.train.resize(a.nodes); for (uint64_t i = 0; i < a.nodes; ++i) m.train[i] = (uint32_t)i; }
+  std::vector<uint32_t> order = m.train;
+  const uint64_t steps_per_pass = (order.size() + a.batch - 1) / a.batch;
+  const uint64_t total = steps_per_pass * a.passes;
+  const uint32_t T = std::max(1u, a.threads);
+  using GradMap = std::unordered_map<uint32_t, std::vector<float>>;
+  uint64_t step = 1, examples = 0, batches_done = 0;
+  XorShift shuf(a.seed ^ 0xABCDEF);
+  const auto t0 = std::chrono::steady_clock::now();
+  bool stop = false;
+  double err_sum = 0.0;
+  auto parallel = [&](auto&& body) {
+    std::vector<std::thread> ts;
+    for (uint32_t t = 0; t < T; ++t) ts.emplace_back(body, t);
+    for (auto& t : ts) t.join();
+  };
+  for (uint32_t pass = 1; pass <= a.passes && !stop; ++pass) {
+    for (size_t i = order.size(); i > 1; --i) std::swap(order[i - 1], order[shuf.below((uint32_t)i)]);
+    double errs = 0.0;
+    for (uint64_t bi = 0; bi < steps_per_pass && !stop; ++bi) {
+      const size_t lo = bi * a.batch, hi = std::min(order.size(), lo + a.batch);
+      std::vector<GradMap> local(T);
+      std::vector<double> err(T, 0.0);
+      std::vector<uint32_t> cnt(T, 0);
+      std::atomic<size_t> next{lo};
+      parallel([&](uint32_t t) {
+        for (;;) {
+          const size_t q = next.fetch_add(1);
+          if (q >= hi) return;
+          const uint32_t v = order[q];
+          XorShift rng(a.seed + bi + v);
+          GradMap g;
+          const float L = m.example(v, rng, g);
+          if (!(L > 0.f)) continue;
+          ++cnt[t];
+          err[t] += L;
+          for (auto& kv : g) {
+            auto it = local[t].find(kv.first);
+            if (it == local[t].end()) local[t].emplace(kv.first, std::move(kv.second));
+            else for (uint32_t d = 0; d < D; ++d) it->second[d] += kv.second[d];
+          }
+        }
+      });
+      uint32_t contributing = 0;
+      double e = 0.0;
+      for (uint32_t t = 0; t < T; ++t) { contributing += cnt[t]; e += err[t]; }
+      const float lr = cos_decay(step, a.alpha, total);  // mod.rs:234-250
+      ++step;
+      if (contributing > 0) {
+        parallel([&](uint32_t s) {  // merge shard s (mod.rs:223-233), then its Adam rows
+          GradMap shard;
+          for (uint32_t t = 0; t < T; ++t)
+            for (auto& kv : local[t]) {
+              if (kv.first % T != s) continue;
+              auto it = shard.find(kv.first);
+              if (it == shard.end()) shard.emplace(kv.first, std::move(kv.second));
+              else for (uint32_t d = 0; d < D; ++d) it->second[d] += kv.second[d];
+            }
+          m.adam(shard, lr, (float)pass);
+        });
+      }
+      errs += contributing ? e / (double)(hi - lo) : 0.0;
+      examples += hi - lo;
+      ++batches_done;
+      if (a.budget_s > 0) {
+        const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (el > a.budget_s) stop = true;
+      }
+    }
+    err_sum = errs / (double)steps_per_pass;
+    if (!a.json) std::fprintf(stderr, "pass %u train error %.5f\n", pass, err_sum);
+  }
+  RunStats r;
+  r.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  r.examples = examples;
+  r.batches = batches_done;
+  r.train_error = err_sum;
+  return r;
+}
+
 }  // namespace
 
+// The timed CPU baseline for in-process callers (oracle/_build/libep_cpu.so, built with
+// EP_CPU_NO_MAIN): trains on the given CSR arrays (mean
+// model, or attention with dk > 0) for about budget_s seconds of batches.  out[0] =
+// examples, out[1] = seconds, out[2] = batches.  Returns 0.
+extern "C" int ep_cpu_bench(const uint64_t* indptr, const uint32_t* indices, const uint64_t* findptr,
+                            const uint32_t* fids, uint64_t nodes, uint32_t vocab, uint32_t dims, uint32_t batch,
+                            uint32_t negatives, uint32_t max_nodes, float gamma, float alpha, uint32_t heads,
+                            uint32_t dk, uint32_t threads, float budget_s, double* out) {
+  Args a;
+  a.nodes = nodes;
+  a.vocab = vocab;
+  a.dims = dims;
+  a.batch = batch;
+  a.negatives = negatives;
+  a.max_nodes = max_nodes;
+  a.gamma = gamma;
+  a.alpha = alpha;
+  a.heads = heads;
+  a.dk = dk;
+  a.threads = threads;
+  a.budget_s = budget_s;
+  a.json = true;
+  Model m{a, indptr, indices, findptr, fids, {}, {}, {}, {}};
+  const RunStats r = run_training(m);
+  out[0] = (double)r.examples;
+  out[1] = r.seconds;
+  out[2] = (double)r.batches;
+  return 0;
+}
+
+namespace {
+}  // namespace
+
+#ifndef EP_CPU_NO_MAIN
 int main(int argc, char** argv) {
   Args a;
   for (int i = 1; i < argc; ++i) {
@@ -468,90 +655,15 @@ int main(int argc, char** argv) {
   const auto findptr = read_bin<uint64_t>(a.dir + "/findptr.bin");
   const auto fids = read_bin<uint32_t>(a.dir + "/fids.bin");
   if (indptr.size() != a.nodes + 1 || findptr.size() != a.nodes + 1) { std::fprintf(stderr, "size mismatch\n"); return 2; }
-  Model m{a, indptr, indices, findptr, fids, {}, {}, {}, {}};
+  Model m{a, indptr.data(), indices.data(), findptr.data(), fids.data(), {}, {}, {}, {}};
   if (!a.explicit_dir.empty()) return run_explicit(m);
-  const uint32_t D = m.width();  // table row width
-  // randomize_embedding_store (mod.rs:381-392)
-  m.table.resize((size_t)a.vocab * D);
-  m.mom.assign(m.table.size(), 0.f);
-  m.var.assign(m.table.size(), 0.f);
-  XorShift init(a.seed);
-  for (uint32_t r = 0; r < a.vocab; ++r) {
-    float* e = &m.table[(size_t)r * D];
-    float n = 0.f;
-    for (uint32_t d = 0; d < D; ++d) { e[d] = 2.f * init.uniform() - 1.f; n += e[d] * e[d]; }
-    n = std::sqrt(n);
-    for (uint32_t d = 0; d < D; ++d) e[d] /= n;
-  }
   if (!a.train_file.empty()) m.train = read_bin<uint32_t>(a.train_file);
-  else { m.train.resize(a.nodes); for (uint64_t i = 0; i < a.nodes; ++i) m.train[i] = (uint32_t)i; }
-  std::vector<uint32_t> order = m.train;
-  const uint64_t steps_per_pass = (order.size() + a.batch - 1) / a.batch;
-  const uint64_t total = steps_per_pass * a.passes;
-  std::atomic<uint64_t> step{1};
-  std::atomic<uint64_t> examples{0}, batches_done{0};
-  XorShift shuf(a.seed ^ 0xABCDEF);
-  const auto t0 = std::chrono::steady_clock::now();
-  bool stop = false;
-  double err_sum = 0.0;
-  for (uint32_t pass = 1; pass <= a.passes && !stop; ++pass) {
-    for (size_t i = order.size(); i > 1; --i) std::swap(order[i - 1], order[shuf.below((uint32_t)i)]);
-    std::atomic<uint64_t> next_batch{0};
-    std::atomic<bool> timeout{false};
-    std::vector<double> errs(steps_per_pass, 0.0);
-    auto worker = [&]() {
-      for (;;) {
-        if (timeout.load()) return;
-        const uint64_t bi = next_batch.fetch_add(1);
-        if (bi >= steps_per_pass) return;
-        const size_t lo = bi * a.batch, hi = std::min(order.size(), lo + a.batch);
-        std::unordered_map<uint32_t, std::vector<float>> all;
-        double err = 0.0;
-        uint32_t cnt = 0;
-        for (size_t q = lo; q < hi; ++q) {
-          const uint32_t v = order[q];
-          XorShift rng(a.seed + bi + v);
-          std::unordered_map<uint32_t, std::vector<float>> g;
-          const float L = m.example(v, rng, g);
-          if (L > 0.f) {
-            ++cnt;
-            err += L;
-            for (auto& kv : g) {
-              auto it = all.find(kv.first);
-              if (it == all.end()) all.emplace(kv.first, std::move(kv.second));
-              else for (uint32_t d = 0; d < D; ++d) it->second[d] += kv.second[d];
-            }
-          }
-        }
-        const uint64_t cur = step.fetch_add(1);
-        if (cnt > 0) m.adam(all, cos_decay(cur, a.alpha, total), (float)pass);
-        errs[bi] = cnt ? err / (double)(hi - lo) : 0.0;
-        examples.fetch_add(hi - lo);
-        batches_done.fetch_add(1);
-        if (a.budget_s > 0) {
-          const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-          if (el > a.budget_s) timeout.store(true);
-        }
-      }
-    };
-    std::vector<std::thread> ts;
-    for (uint32_t t = 0; t < a.threads; ++t) ts.emplace_back(worker);
-    for (auto& t : ts) t.join();
-    if (timeout.load()) stop = true;
-    double e = 0.0;
-    for (double x : errs) e += x;
-    err_sum = e / (double)steps_per_pass;
-    if (!a.json) std::fprintf(stderr, "pass %u train error %.5f\n", pass, err_sum);
-  }
-  const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  if (!a.table_out.empty()) {
-    FILE* f = std::fopen(a.table_out.c_str(), "wb");
-    std::fwrite(m.table.data(), sizeof(float), m.table.size(), f);
-    std::fclose(f);
-  }
+  const RunStats r = run_training(m);
+  if (!a.table_out.empty()) write_bin(a.table_out, m.table);
   std::printf("{\"examples\": %llu, \"batches\": %llu, \"seconds\": %.4f, \"examples_per_s\": %.2f, "
               "\"threads\": %u, \"train_error\": %.6f}\n",
-              (unsigned long long)examples.load(), (unsigned long long)batches_done.load(), el,
-              (double)examples.load() / el, a.threads, err_sum);
+              (unsigned long long)r.examples, (unsigned long long)r.batches, r.seconds,
+              (double)r.examples / r.seconds, a.threads, r.train_error);
   return 0;
 }
+#endif  // EP_CPU_NO_MAIN
