@@ -148,6 +148,7 @@ SIGNATURES = {
     "ep_sample_batch": (c_int, [c_void_p, _u32p, c_uint32, c_uint32, _u32p, _u32p, _u32p]),
     "ep_step_explicit": (c_int, [c_void_p, _u32p, c_uint32, _u32p, _u32p, _u32p, c_uint32, c_float, c_int,
                                  _f32p, _f32p, _u32p, _u32p, _f32p]),
+    "ep_step_explicit_noalloc": (c_int, [c_void_p, _u32p, c_uint32, _u32p, _u32p, _u32p, c_uint32, c_float, _f32p]),
     "ep_bench_steps": (c_int, [c_void_p, c_uint32, c_uint32, c_int, POINTER(EPBenchResult)]),
     "ep_embed_averaged": (c_int, [c_void_p, c_uint64, _u64p, _u32p, _f32p]),
     "ep_embed_attention": (c_int, [c_void_p, c_uint64, _u64p, _u32p, _f32p]),
@@ -398,6 +399,22 @@ class Context:
               "ep_step_explicit")
         k = nu.value
         return {"loss": loss, "cons": cons, "feat": feat[:k].copy(), "grad": grad[:k].copy()}
+
+    def step_explicit_noalloc(self, anchors, nbr_cnt, nbr, neg, pass_: int, lr: float) -> np.ndarray:
+        """step_explicit with the update and only the loss output, through the entry point
+        whose step arrays live in the library's per-GPU scratch (no device allocation in
+        the call).  Returns the per-example loss."""
+        anchors = np.ascontiguousarray(anchors, dtype=np.uint32)
+        n = len(anchors)
+        S, K = self.list_width(), self.cfg.negatives
+        nbr_cnt = np.ascontiguousarray(nbr_cnt, dtype=np.uint32)
+        nbr = np.ascontiguousarray(nbr, dtype=np.uint32).reshape(n, S)
+        neg = np.ascontiguousarray(neg, dtype=np.uint32).reshape(n, K)
+        loss = np.zeros(n, dtype=np.float32)
+        check(load().ep_step_explicit_noalloc(self.handle, ptr(anchors, c_uint32), n, ptr(nbr_cnt, c_uint32),
+                                              ptr(nbr, c_uint32), ptr(neg, c_uint32), int(pass_), float(lr),
+                                              ptr(loss, c_float)), "ep_step_explicit_noalloc")
+        return loss
 
     def bench_steps(self, warmup: int, steps: int, instrument: bool = True) -> EPBenchResult:
         out = EPBenchResult()

@@ -6,12 +6,21 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/cloverleaf_ep.h"
 #include "ep_device.h"
 #include "ep_kernels.h"
+
+// Step buffers of ep_step_explicit_noalloc: one array per GPU, held from the library's
+// load until the process ends.  Host code never uses this name as an address (on the
+// host it is a stand-in): the device address comes from hipGetSymbolAddress, once per
+// device (scratch_ptr).  Every call on a GPU shares that GPU's array.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 namespace {
 
@@ -53,6 +62,12 @@ uint32_t next_pow2(uint64_t x) {
   return (uint32_t)p;
 }
 
+// Device address of `scratch` on each device, resolved by ep_create with the device
+// current (hipGetSymbolAddress gives the current device's copy).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
 }  // namespace
 
 struct ep_ctx {
@@ -60,6 +75,7 @@ struct ep_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
+  unsigned char* scratch = nullptr;  // this device's `scratch` (device address)
 
   // graph / features
   uint64_t n_nodes = 0, n_entries = 0;
@@ -204,14 +220,12 @@ uint64_t example_rows_max(const ep_ctx* c) {
   return (uint64_t)c->f_max * (1ull + neighbour_cap(c) + c->cfg.negatives);
 }
 
-// Sizes every per-step buffer for batches of up to `batch` anchors.
-int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
-  if (weighted_avg(c->cfg) && !c->d_cdf)
-    return fail(EP_ESTATE, "weighted_neighbor_averaging needs the edge weights (ep_set_edge_weights)");
-  if (feature_sampling(c->cfg) && c->f_max > 64)
-    return fail(EP_EINVAL, "max_features sampling supports at most 64 features per node");
-  if (batch <= c->cap_batch && c->base.nbr) return EP_OK;
-  if (!c->d_indptr || !c->d_findptr) return fail(EP_ESTATE, "graph and features must be set first");
+// Places every per-step array of StepConfig b for batches of up to `batch` anchors and
+// sets the step's capacities and grid shapes.  place(&ptr, count) gives an array its
+// memory: hipMalloc (ensure_step_buffers) or a carve-out of the no-allocation scratch
+// (ep_step_explicit_noalloc).  *keycap_out receives the key capacity.
+template <typename Place>
+int place_step_buffers(ep_ctx* c, uint32_t batch, ep::StepConfig& b, Place&& place, uint64_t* keycap_out) {
   const ep_config& cfg = c->cfg;
   const uint32_t K = cfg.negatives, NC = K + 2, D = row_width(cfg);
   const bool att = cfg.attention_dk != 0;
@@ -247,26 +261,6 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   if (cfg.grad_mode == EP_GRAD_FIRST_LEAF && hash > 16384)
     return fail(EP_ECAPACITY, "first_leaf dedup needs <= 8192 feature occurrences per example (F_max*(1+S+K))");
 
-  // drop old buffers
-  ep::StepConfig& b = c->base;
-  c->release(b.nbr_cnt); c->release(b.nbr); c->release(b.neg); c->release(b.rcount); c->release(b.keyoff);
-  c->release(b.nbr_w); c->release(b.nbroff); c->release(b.nbr_sel); c->release(b.nbr_selw);
-  c->release(b.adj); c->release(b.adjh); c->release(b.ascale); c->release(b.coef); c->release(b.elig);
-  c->release(b.loss);
-  c->release(b.keys[0]); c->release(b.keys[1]); c->release(b.slots[0]); c->release(b.slots[1]);
-  c->release(b.ghist); c->release(b.G); c->release(b.ufeat); c->release(b.long_st); c->release(b.long_len);
-  c->release(b.long_pbase); c->release(b.long_runs); c->release(b.long_rlen); c->release(b.piece_st);
-  c->release(b.piece_info);
-  c->release(b.part); c->release(b.srec); c->release(b.mrec); c->release(b.scount); c->release(b.mcount);
-  c->release(b.blkoff); c->release(b.bitems); c->release(b.bout); c->release(b.badj);
-  c->release(b.bnode); c->release(b.bslot); c->release(b.bmeta);
-  c->release(c->d_explicit);
-  c->release(c->d_xout);
-  if (c->step_exec) { (void)hipGraphExecDestroy(c->step_exec); c->step_exec = nullptr; c->step_exec_ready = false; }
-
-  const uint32_t P = std::max<uint32_t>(1u, cfg.concurrent_batches);
-  // merged-gradient records (test hook): one per (row, batch) run
-  const uint64_t unique_cap = std::min<uint64_t>((uint64_t)c->n_vocab * P, keycap) + 1;
   // long rows run past the plan's 128-entry window (>= 65 entries); their runs of more
   // than kPieceEntries entries are cut into pieces of kPieceEntries, fewer than
   // (run entries) / 16 per run
@@ -276,74 +270,63 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   const int unique_grid = 2048;
   // radix-sort grid: ~8K keys per block at capacity, a multiple of 256 in [256, 2048]
   const int sort_grid = (int)std::min<uint64_t>(2048, std::max<uint64_t>(256, (keycap / 8192 + 255) / 256 * 256));
-  EP_TRY(c->alloc(&b.nbr_cnt, batch));
-  EP_TRY(c->alloc(&b.nbr, (size_t)batch * std::max<uint32_t>(S, 1)));
-  EP_TRY(c->alloc(&b.neg, (size_t)batch * K));
-  EP_TRY(c->alloc(&b.rcount, batch));
-  EP_TRY(c->alloc(&b.keyoff, batch + 1));
+  EP_TRY(place(&b.nbr_cnt, batch));
+  EP_TRY(place(&b.nbr, (size_t)batch * std::max<uint32_t>(S, 1)));
+  EP_TRY(place(&b.neg, (size_t)batch * K));
+  EP_TRY(place(&b.rcount, batch));
+  EP_TRY(place(&b.keyoff, batch + 1));
   // adjoint rows: mean model one per construction (D wide); attention one gradient row
   // per feature occurrence
   if (fp16_adj(cfg)) {
-    EP_TRY(c->alloc(&b.adjh, ((size_t)batch * NC + nrow_cap) * D));
-    EP_TRY(c->alloc(&b.ascale, (size_t)batch * NC + nrow_cap));
+    EP_TRY(place(&b.adjh, ((size_t)batch * NC + nrow_cap) * D));
+    EP_TRY(place(&b.ascale, (size_t)batch * NC + nrow_cap));
   } else {
-    EP_TRY(c->alloc(&b.adj, att ? (size_t)keycap * D : ((size_t)batch * NC + nrow_cap) * D));
+    EP_TRY(place(&b.adj, att ? (size_t)keycap * D : ((size_t)batch * NC + nrow_cap) * D));
   }
-  if (wavg && S) EP_TRY(c->alloc(&b.nbr_w, (size_t)batch * S));
-  if (nrows && !S) EP_TRY(c->alloc(&b.nbroff, batch));
+  if (wavg && S) EP_TRY(place(&b.nbr_w, (size_t)batch * S));
+  if (nrows && !S) EP_TRY(place(&b.nbroff, batch));
   if (prob_nodes(cfg)) {
-    EP_TRY(c->alloc(&b.nbr_sel, std::max<uint64_t>(c->n_entries, 1)));
-    if (nrows) EP_TRY(c->alloc(&b.nbr_selw, std::max<uint64_t>(c->n_entries, 1)));
+    EP_TRY(place(&b.nbr_sel, std::max<uint64_t>(c->n_entries, 1)));
+    if (nrows) EP_TRY(place(&b.nbr_selw, std::max<uint64_t>(c->n_entries, 1)));
   }
   b.nrow_base = batch * NC;
   if (att) {
     const uint32_t dm = cfg.dims;
-    EP_TRY(c->alloc(&b.blkoff, batch + 1));
-    EP_TRY(c->alloc(&b.bitems, blk_cap));
-    EP_TRY(c->alloc(&b.bnode, blk_cap));
-    EP_TRY(c->alloc(&b.bslot, blk_cap));
-    EP_TRY(c->alloc(&b.bmeta, blk_cap));
-    EP_TRY(c->alloc(&b.bout, (size_t)blk_cap * dm));
-    EP_TRY(c->alloc(&b.badj, (size_t)blk_cap * dm));
+    EP_TRY(place(&b.blkoff, batch + 1));
+    EP_TRY(place(&b.bitems, blk_cap));
+    EP_TRY(place(&b.bnode, blk_cap));
+    EP_TRY(place(&b.bslot, blk_cap));
+    EP_TRY(place(&b.bmeta, blk_cap));
+    EP_TRY(place(&b.bout, (size_t)blk_cap * dm));
+    EP_TRY(place(&b.badj, (size_t)blk_cap * dm));
   }
   b.blk_cap = (uint32_t)blk_cap;
-  EP_TRY(c->alloc(&b.coef, (size_t)batch * NC));
-  EP_TRY(c->alloc(&b.elig, (size_t)batch * NC));
-  EP_TRY(c->alloc(&b.loss, batch));
+  EP_TRY(place(&b.coef, (size_t)batch * NC));
+  EP_TRY(place(&b.elig, (size_t)batch * NC));
+  EP_TRY(place(&b.loss, batch));
   for (int i = 0; i < 2; ++i) {
-    EP_TRY(c->alloc(&b.keys[i], keycap));
-    EP_TRY(c->alloc(&b.slots[i], keycap));
+    EP_TRY(place(&b.keys[i], keycap));
+    EP_TRY(place(&b.slots[i], keycap));
   }
-  EP_TRY(c->alloc(&b.ghist, (size_t)2048 * sort_grid));
-  // merged-gradient records (G, ufeat) only serve the test hook's gradient output; they
-  // are allocated by ep_step_explicit on first use
-  c->unique_cap = unique_cap;
-  EP_TRY(c->alloc(&b.long_st, long_cap));
-  EP_TRY(c->alloc(&b.long_len, long_cap));
-  EP_TRY(c->alloc(&b.long_pbase, long_cap));
-  EP_TRY(c->alloc(&b.long_runs, (size_t)long_cap * ep::kMaxGroup));
-  EP_TRY(c->alloc(&b.long_rlen, (size_t)long_cap * ep::kMaxGroup));
+  EP_TRY(place(&b.ghist, (size_t)2048 * sort_grid));
+  EP_TRY(place(&b.long_st, long_cap));
+  EP_TRY(place(&b.long_len, long_cap));
+  EP_TRY(place(&b.long_pbase, long_cap));
+  EP_TRY(place(&b.long_runs, (size_t)long_cap * ep::kMaxGroup));
+  EP_TRY(place(&b.long_rlen, (size_t)long_cap * ep::kMaxGroup));
   // plan regions: wave w of the row kernels covers cpw chunks of 64 keys; up to 64 short
   // and 2 medium rows start in a chunk
   const uint64_t nwaves = (uint64_t)unique_grid * ep::kBlockWaves;
   const uint64_t cpw_max = ((keycap + 63) / 64 + nwaves - 1) / nwaves;
-  EP_TRY(c->alloc(&b.srec, nwaves * cpw_max * 64));
-  EP_TRY(c->alloc(&b.mrec, nwaves * cpw_max * 2));
-  EP_TRY(c->alloc(&b.scount, nwaves));
-  EP_TRY(c->alloc(&b.mcount, nwaves));
+  EP_TRY(place(&b.srec, nwaves * cpw_max * 64));
+  EP_TRY(place(&b.mrec, nwaves * cpw_max * 2));
+  EP_TRY(place(&b.scount, nwaves));
+  EP_TRY(place(&b.mcount, nwaves));
   b.srec_cap = (uint32_t)(nwaves * cpw_max * 64);
   b.mrec_cap = (uint32_t)(nwaves * cpw_max * 2);
-  EP_TRY(c->alloc(&b.piece_st, piece_cap));
-  EP_TRY(c->alloc(&b.piece_info, piece_cap));
-  EP_TRY(c->alloc(&b.part, (size_t)piece_cap * D));
-  EP_TRY(c->alloc(&c->d_explicit, batch));
-  if (!b.Gd && cfg.scatter == EP_SCATTER_ATOMIC) {
-    EP_TRY(c->alloc(&b.Gd, (size_t)c->n_vocab * D));
-    EP_TRY(c->alloc(&b.flag, c->n_vocab));
-    EP_TRY(c->alloc(&b.touched, c->n_vocab));
-    EP_HIP(hipMemsetAsync(b.Gd, 0, (size_t)c->n_vocab * D * sizeof(float), c->stream));
-    EP_HIP(hipMemsetAsync(b.flag, 0, (size_t)c->n_vocab * sizeof(uint32_t), c->stream));
-  }
+  EP_TRY(place(&b.piece_st, piece_cap));
+  EP_TRY(place(&b.piece_info, piece_cap));
+  EP_TRY(place(&b.part, (size_t)piece_cap * D));
   b.batch = batch;
   b.S = S;
   b.all_nbrs = (cfg.max_nodes_kind == EP_SAMPLE_ALL && !ppr_loss(cfg)) ? 1 : 0;
@@ -366,8 +349,70 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
     b.fuse_max = fh / 8 * 5;
   }
   b.unique_grid = unique_grid;
+  *keycap_out = keycap;
+  return EP_OK;
+}
+
+// Nulls every per-step array of a StepConfig (the ones place_step_buffers sets, the
+// gradient-output and atomic-path arrays, the constructions hook).
+void clear_step_buffers(ep::StepConfig& b) {
+  b.nbr_cnt = nullptr; b.nbr = nullptr; b.neg = nullptr; b.rcount = nullptr; b.keyoff = nullptr;
+  b.nbr_w = nullptr; b.nbroff = nullptr; b.nbr_sel = nullptr; b.nbr_selw = nullptr;
+  b.adj = nullptr; b.adjh = nullptr; b.ascale = nullptr; b.coef = nullptr; b.elig = nullptr; b.loss = nullptr;
+  b.pass_loss = nullptr; b.xout = nullptr;
+  b.keys[0] = b.keys[1] = nullptr; b.slots[0] = b.slots[1] = nullptr; b.ghist = nullptr;
+  b.G = nullptr; b.ufeat = nullptr;
+  b.srec = nullptr; b.mrec = nullptr; b.scount = nullptr; b.mcount = nullptr;
+  b.long_st = nullptr; b.long_len = nullptr; b.long_pbase = nullptr; b.long_runs = nullptr; b.long_rlen = nullptr;
+  b.piece_st = nullptr; b.piece_info = nullptr; b.part = nullptr;
+  b.Gd = nullptr; b.flag = nullptr; b.touched = nullptr;
+  b.blkoff = nullptr; b.bitems = nullptr; b.bnode = nullptr; b.bslot = nullptr; b.bmeta = nullptr;
+  b.bout = nullptr; b.badj = nullptr;
+}
+
+// Sizes every per-step buffer for batches of up to `batch` anchors.
+int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
+  if (weighted_avg(c->cfg) && !c->d_cdf)
+    return fail(EP_ESTATE, "weighted_neighbor_averaging needs the edge weights (ep_set_edge_weights)");
+  if (feature_sampling(c->cfg) && c->f_max > 64)
+    return fail(EP_EINVAL, "max_features sampling supports at most 64 features per node");
+  if (batch <= c->cap_batch && c->base.nbr) return EP_OK;
+  if (!c->d_indptr || !c->d_findptr) return fail(EP_ESTATE, "graph and features must be set first");
+  const ep_config& cfg = c->cfg;
+  const uint32_t D = row_width(cfg);
+  // drop old buffers
+  ep::StepConfig& b = c->base;
+  c->release(b.nbr_cnt); c->release(b.nbr); c->release(b.neg); c->release(b.rcount); c->release(b.keyoff);
+  c->release(b.nbr_w); c->release(b.nbroff); c->release(b.nbr_sel); c->release(b.nbr_selw);
+  c->release(b.adj); c->release(b.adjh); c->release(b.ascale); c->release(b.coef); c->release(b.elig);
+  c->release(b.loss);
+  c->release(b.keys[0]); c->release(b.keys[1]); c->release(b.slots[0]); c->release(b.slots[1]);
+  c->release(b.ghist); c->release(b.G); c->release(b.ufeat); c->release(b.long_st); c->release(b.long_len);
+  c->release(b.long_pbase); c->release(b.long_runs); c->release(b.long_rlen); c->release(b.piece_st);
+  c->release(b.piece_info);
+  c->release(b.part); c->release(b.srec); c->release(b.mrec); c->release(b.scount); c->release(b.mcount);
+  c->release(b.blkoff); c->release(b.bitems); c->release(b.bout); c->release(b.badj);
+  c->release(b.bnode); c->release(b.bslot); c->release(b.bmeta);
+  c->release(c->d_explicit);
+  c->release(c->d_xout);
+  if (c->step_exec) { (void)hipGraphExecDestroy(c->step_exec); c->step_exec = nullptr; c->step_exec_ready = false; }
+
+  uint64_t keycap = 0;
+  EP_TRY(place_step_buffers(c, batch, b, [c](auto** p, size_t n) { return c->alloc(p, n); }, &keycap));
+  const uint32_t P = std::max<uint32_t>(1u, cfg.concurrent_batches);
+  // merged-gradient records (G, ufeat) only serve the test hook's gradient output (one per
+  // (row, batch) run); they are allocated by ep_step_explicit on first use
+  c->unique_cap = std::min<uint64_t>((uint64_t)c->n_vocab * P, keycap) + 1;
+  EP_TRY(c->alloc(&c->d_explicit, batch));
+  if (!b.Gd && cfg.scatter == EP_SCATTER_ATOMIC) {
+    EP_TRY(c->alloc(&b.Gd, (size_t)c->n_vocab * D));
+    EP_TRY(c->alloc(&b.flag, c->n_vocab));
+    EP_TRY(c->alloc(&b.touched, c->n_vocab));
+    EP_HIP(hipMemsetAsync(b.Gd, 0, (size_t)c->n_vocab * D * sizeof(float), c->stream));
+    EP_HIP(hipMemsetAsync(b.flag, 0, (size_t)c->n_vocab * sizeof(uint32_t), c->stream));
+  }
   c->cap_batch = batch;
-  c->cap_S = S;
+  c->cap_S = b.S;
   c->key_cap = (uint32_t)keycap;
   return EP_OK;
 }
@@ -464,14 +509,19 @@ void refresh_base(ep_ctx* c) {
 // ev (nullable): 9 events recorded at the step start and after the scan, forward, emit,
 // sort, plan, short-row, pieces and long-row phases (atomic path: after its scatter and
 // its update, the remaining ones at the same point).
+// The k_seg_update variant of a launch that names none: the configured or tuned one,
+// (4, 8) while the tuner is choosing (every variant gives the same results).
+void pick_variant(const ep_ctx* c, ep::StepConfig& sc) {
+  if (sc.seg_vpl != 0) return;
+  sc.seg_vpl = c->seg_vpl ? c->seg_vpl : 4;
+  sc.seg_unroll = c->seg_vpl ? c->seg_unroll : 8;
+}
+
 int launch_training_step(ep_ctx* c, const ep::StepConfig& sc0, bool update, float lr_override,
                          hipEvent_t* ev = nullptr, int* sorted_buf = nullptr) {
   hipStream_t s = c->stream;
   ep::StepConfig sc = sc0;
-  if (sc.seg_vpl == 0) {
-    sc.seg_vpl = c->seg_vpl ? c->seg_vpl : 4;
-    sc.seg_unroll = c->seg_vpl ? c->seg_unroll : 8;
-  }
+  pick_variant(c, sc);
   auto mark = [&](int i) -> hipError_t { return ev ? hipEventRecord(ev[i], s) : hipSuccess; };
   EP_HIP(mark(0));
   if (lr_override >= 0.f) EP_HIP(ep::launch_step_begin_lr(sc, c->d_state, lr_override, s));
@@ -752,8 +802,21 @@ int ep_create(const ep_config* cfg, ep_ctx** out) {
   int ndev = 0;
   EP_HIP(hipGetDeviceCount(&ndev));
   if (cfg->device < 0 || cfg->device >= ndev) return fail(EP_EINVAL, "no such HIP device");
+  if (cfg->device >= kMaxDevices) return fail(EP_EINVAL, "device index beyond the supported 64");
   EP_HIP(hipSetDevice(cfg->device));
+  void* sp = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[cfg->device]) {
+      void* p = nullptr;
+      EP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      if (!p) return fail(EP_EHIP, "hipGetSymbolAddress(scratch) returned NULL");
+      g_scratch_ptr[cfg->device] = p;
+    }
+    sp = g_scratch_ptr[cfg->device];
+  }
   ep_ctx* c = new ep_ctx();
+  c->scratch = (unsigned char*)sp;
   c->cfg = *cfg;
   c->seg_vpl = cfg->seg_vpl;  // 0: tuned during the first training steps
   c->seg_unroll = cfg->seg_unroll;
@@ -1210,6 +1273,7 @@ int ep_step_explicit(ep_ctx* c, const uint32_t* anchors, uint32_t n, const uint3
                     "counter layout");
       EP_HIP(hipMemcpy((char*)c->d_state + offsetof(ep::StepState, n_unique), zero4, sizeof(zero4),
                        hipMemcpyHostToDevice));
+      pick_variant(c, sc);
       EP_HIP(ep::launch_scatter_sorted(sc, c->d_state, buf, c->stream));
     }
   } else {
@@ -1230,6 +1294,71 @@ int ep_step_explicit(ep_ctx* c, const uint32_t* anchors, uint32_t n, const uint3
       }
     }
   }
+  EP_HIP(hipStreamSynchronize(c->stream));
+  return EP_OK;
+}
+
+int ep_step_explicit_noalloc(ep_ctx* c, const uint32_t* anchors, uint32_t n, const uint32_t* nbr_cnt,
+                             const uint32_t* nbr, const uint32_t* neg, uint32_t pass, float lr, float* loss) {
+  if (!c || !anchors || !nbr_cnt || !nbr || !neg || n == 0) return fail(EP_EINVAL, "bad argument");
+  if (c->cfg.max_nodes_kind != EP_SAMPLE_FIXED && !ppr_loss(c->cfg))
+    return fail(EP_EINVAL, "ep_step_explicit_noalloc needs max_nodes Fixed (or the ppr loss)");
+  if (c->cfg.scatter != EP_SCATTER_SORT) return fail(EP_EINVAL, "ep_step_explicit_noalloc needs the sort scatter");
+  if (!(lr >= 0.f)) return fail(EP_EINVAL, "lr must be >= 0");
+  if (!c->d_indptr || !c->d_findptr) return fail(EP_ESTATE, "graph and features must be set first");
+  if (!c->table_ready) return fail(EP_ESTATE, "the table must be set first (ep_set_table allocates it)");
+  if (weighted_avg(c->cfg) && !c->d_cdf)
+    return fail(EP_ESTATE, "weighted_neighbor_averaging needs the edge weights (ep_set_edge_weights)");
+  const uint32_t S = list_cap(c->cfg), K = c->cfg.negatives;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (anchors[i] >= c->n_nodes) return fail(EP_EINVAL, "anchor out of range");
+    if (nbr_cnt[i] > S) return fail(EP_EINVAL, "nbr_cnt exceeds max_nodes");
+    for (uint32_t j = 0; j < nbr_cnt[i]; ++j)
+      if (nbr[(size_t)i * S + j] >= c->n_nodes) return fail(EP_EINVAL, "neighbour out of range");
+    for (uint32_t k = 0; k < K; ++k)
+      if (neg[(size_t)i * K + k] >= c->n_nodes) return fail(EP_EINVAL, "negative out of range");
+  }
+  const uint32_t P = std::max<uint32_t>(1u, c->cfg.concurrent_batches);
+  if (P > 1 && (n + c->cfg.batch_size - 1) / c->cfg.batch_size > P)
+    return fail(EP_EINVAL, "ep_step_explicit_noalloc: more than concurrent_batches batches of batch_size anchors");
+  const bool wavg = weighted_avg(c->cfg);
+  if (wavg && c->explicit_w.size() != (size_t)n * S)
+    return fail(EP_ESTATE, "weighted_neighbor_averaging: set n x max_nodes neighbour weights first");
+  EP_HIP(hipSetDevice(c->device));
+  refresh_base(c);
+  ep::StepConfig sc = c->base;
+  clear_step_buffers(sc);
+  // carve the step's arrays out of this device's scratch, 256-byte aligned
+  size_t need = 0;
+  auto carve = [&](auto** p, size_t count) -> int {
+    const size_t at = (need + 255) & ~(size_t)255;
+    need = at + std::max<size_t>(count * sizeof(**p), 16);
+    *p = need <= kScratchBytes ? reinterpret_cast<std::remove_reference_t<decltype(*p)>>(c->scratch + at) : nullptr;
+    return EP_OK;
+  };
+  uint32_t* d_anchors = nullptr;
+  carve(&d_anchors, n);
+  uint64_t keycap = 0;
+  EP_TRY(place_step_buffers(c, n, sc, carve, &keycap));
+  if (need > kScratchBytes)
+    return fail(EP_ECAPACITY, "ep_step_explicit_noalloc: the step needs " + std::to_string(need) +
+                                  " bytes of scratch, the library holds " + std::to_string(kScratchBytes));
+  sc.perm = d_anchors;
+  sc.negpool = d_anchors;
+  sc.n_negpool = n;
+  sc.n_batch_total = n;
+  sc.batch = n;
+  sc.rbatch = P > 1 ? c->cfg.batch_size : n;
+  sc.explicit_samples = 1;
+  EP_HIP(hipMemcpy(d_anchors, anchors, n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  EP_HIP(hipMemcpy(sc.nbr_cnt, nbr_cnt, n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  if (S) EP_HIP(hipMemcpy(sc.nbr, nbr, (size_t)n * S * sizeof(uint32_t), hipMemcpyHostToDevice));
+  EP_HIP(hipMemcpy(sc.neg, neg, (size_t)n * K * sizeof(uint32_t), hipMemcpyHostToDevice));
+  if (wavg) EP_HIP(hipMemcpy(sc.nbr_w, c->explicit_w.data(), (size_t)n * S * sizeof(float), hipMemcpyHostToDevice));
+  EP_TRY(set_pass(c, c->d_state, pass, nullptr));
+  EP_TRY(launch_training_step(c, sc, true, lr));
+  EP_TRY(check_state_err(c));
+  if (loss) EP_HIP(hipMemcpy(loss, sc.loss, n * sizeof(float), hipMemcpyDeviceToHost));
   EP_HIP(hipStreamSynchronize(c->stream));
   return EP_OK;
 }

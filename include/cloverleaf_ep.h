@@ -236,6 +236,20 @@ int ep_step_explicit(ep_ctx* ctx, const uint32_t* anchors, uint32_t n, const uin
                      const uint32_t* nbr, const uint32_t* neg, uint32_t pass, float lr, int apply_update,
                      float* loss, float* cons, uint32_t* n_unique, uint32_t* feat, float* grad);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory
+ * (serving loops, harnesses that forbid hipMalloc).  ep_step_explicit with
+ * apply_update = 1 and only the loss output, whose per-step arrays live in a 64 MiB
+ * array the library holds on each GPU from its load until the process ends, instead of
+ * buffers the context allocates: the call itself allocates nothing (no hipMalloc,
+ * hipFree or allocating library routine).  Host arrays as in ep_step_explicit; loss[n]
+ * nullable.  Needs the sort scatter, max_nodes Fixed (or the ppr loss) and a table set
+ * beforehand (ep_set_table); a step needing more than 64 MiB of arrays returns
+ * EP_ECAPACITY with the size in ep_last_error().  Synchronous.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (from several threads or
+ * contexts) race.                                                                  */
+int ep_step_explicit_noalloc(ep_ctx* ctx, const uint32_t* anchors, uint32_t n, const uint32_t* nbr_cnt,
+                             const uint32_t* nbr, const uint32_t* neg, uint32_t pass, float lr, float* loss);
+
 /* Measurement hook: runs `warmup` then `steps` training steps from the current
  * split with everything resident in HBM, timed with HIP events on the context
  * stream.  Per-kernel timings are taken in a separate instrumented run of the
