@@ -314,16 +314,21 @@ int place_step_buffers(ep_ctx* c, uint32_t batch, ep::StepConfig& b, Place&& pla
   EP_TRY(place(&b.long_pbase, long_cap));
   EP_TRY(place(&b.long_runs, (size_t)long_cap * ep::kMaxGroup));
   EP_TRY(place(&b.long_rlen, (size_t)long_cap * ep::kMaxGroup));
-  // plan regions: wave w of the row kernels covers cpw chunks of 64 keys; up to 64 short
-  // and 2 medium rows start in a chunk
+  // plan regions of cpr chunks of 64 keys; up to 64 short and 2 medium rows start in a
+  // chunk.  Steps of at least 32 chunks per plan wave get 8 regions per wave, so the
+  // update kernel's waves, taking regions from a queue, end within a few percent of each
+  // other whatever the residency of its blocks.
   const uint64_t nwaves = (uint64_t)unique_grid * ep::kBlockWaves;
-  const uint64_t cpw_max = ((keycap + 63) / 64 + nwaves - 1) / nwaves;
-  EP_TRY(place(&b.srec, nwaves * cpw_max * 64));
-  EP_TRY(place(&b.mrec, nwaves * cpw_max * 2));
-  EP_TRY(place(&b.scount, nwaves));
-  EP_TRY(place(&b.mcount, nwaves));
-  b.srec_cap = (uint32_t)(nwaves * cpw_max * 64);
-  b.mrec_cap = (uint32_t)(nwaves * cpw_max * 2);
+  const uint64_t chunks_max = (keycap + 63) / 64;
+  const uint64_t nregions = nwaves * (chunks_max >= nwaves * 32 ? 8 : 1);
+  const uint64_t cpr_max = (chunks_max + nregions - 1) / nregions;
+  EP_TRY(place(&b.srec, nregions * cpr_max * 64));
+  EP_TRY(place(&b.mrec, nregions * cpr_max * 2));
+  EP_TRY(place(&b.scount, nregions));
+  EP_TRY(place(&b.mcount, nregions));
+  b.srec_cap = (uint32_t)(nregions * cpr_max * 64);
+  b.mrec_cap = (uint32_t)(nregions * cpr_max * 2);
+  b.n_regions = (uint32_t)nregions;
   EP_TRY(place(&b.piece_st, piece_cap));
   EP_TRY(place(&b.piece_info, piece_cap));
   EP_TRY(place(&b.part, (size_t)piece_cap * D));

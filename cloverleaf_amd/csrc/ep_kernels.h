@@ -45,6 +45,7 @@ struct StepState {
   uint64_t step0;          // step number of the group's first batch
   float bc1, bc2;          // Adam bias corrections 1 - beta^t, t = pass + 1
   uint32_t n_real;         // non-sentinel keys (the sorted key count)
+  uint32_t reg_next;       // next plan region for k_seg_update (cleared by k_seg_plan)
   uint32_t dtot[4][2048];  // radix-sort digit totals, one row per sort pass (<= 11-bit digits)
 };
 
@@ -163,19 +164,22 @@ struct StepConfig {
   // scatter (sort path)
   float* G;                // [key_cap*D] compact gradient rows (bounded by unique count)
   uint32_t* ufeat;         // [key_cap]
-  // k_seg_plan wave w covers chunks [w*cpw, (w+1)*cpw) of 64 sorted entries (cpw from
-  // the step's key count and the grid) and writes, without atomics, into its own
-  // regions: short rows (<= kSegShort entries) at srec[w*cpw*64 ...] with count
-  // scount[w], one record {first entry, row id, run-start mask (bit i: entry i starts
-  // a (row, batch) run), len | batch of run j << (6 + 3j)}; medium rows (more entries,
-  // inside the 128-entry window, at most 2 per chunk) at mrec[w*cpw*2 ...] with count
-  // mcount[w], {first entry, len}.
+  // Plan regions: region r covers chunks [r*cpr, (r+1)*cpr) of 64 sorted entries (cpr =
+  // ceil(chunks / n_regions), from the step's key count); k_seg_plan wave w owns the
+  // n_regions / (unique_grid * NW) regions from w * that count and writes, without
+  // atomics, each region's short rows (<= kSegShort entries) at srec[r*cpr*64 ...] with
+  // count scount[r], one record {first entry, row id, run-start mask (bit i: entry i
+  // starts a (row, batch) run), len | batch of run j << (6 + 3j)}, and its medium rows
+  // (more entries, inside the 128-entry window, at most 2 per chunk) at mrec[r*cpr*2 ...]
+  // with count mcount[r], {first entry, len}.  k_seg_update's waves take regions from
+  // the StepState queue reg_next, so its blocks need not all be resident at once.
   uint4* srec;             // [srec_cap]
   uint32_t srec_cap;
   uint2* mrec;             // [mrec_cap]
   uint32_t mrec_cap;
-  uint32_t* scount;        // [unique_grid * NW]
-  uint32_t* mcount;        // [unique_grid * NW]
+  uint32_t* scount;        // [n_regions]
+  uint32_t* mcount;        // [n_regions]
+  uint32_t n_regions;      // a multiple of unique_grid * NW
   // Long rows (running past the window, so at least 65 entries): first sorted entry,
   // first piece, run count and per run (piece count | batch << 24) and its entry count.
   // Only runs of more than kPieceEntries entries are cut into pieces (a piece is a range
@@ -189,7 +193,7 @@ struct StepConfig {
   uint32_t* long_runs;     // [long_cap * kMaxGroup]
   uint32_t* long_rlen;     // [long_cap * kMaxGroup] entries of each run
   uint32_t* piece_st;      // [piece_cap]
-  uint32_t* piece_info;    // [piece_cap] entries of the piece
+  uint32_t* piece_info;    // [piece_cap] entries of the piece | batch << 29
   float* part;             // [piece_cap*D]
   uint32_t long_cap, piece_cap;
   // scatter (atomic path)

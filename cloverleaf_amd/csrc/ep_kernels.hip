@@ -1782,7 +1782,7 @@ __device__ inline void write_long_row(const StepConfig& c, uint32_t idx, uint32_
       const uint32_t ps = a + q * pl;
       if (p + q < c.piece_cap) {
         c.piece_st[p + q] = ps;
-        c.piece_info[p + q] = z - ps < pl ? z - ps : pl;
+        c.piece_info[p + q] = (z - ps < pl ? z - ps : pl) | (bq << 29);
       }
     }
     p += np;
@@ -1801,23 +1801,56 @@ __global__ __launch_bounds__(256) void k_seg_plan(StepConfig c, StepState* st, c
   const uint32_t n = st->n_keys <= c.key_cap ? st->n_real : 0u;  // sorted, sentinels dropped
   const uint32_t SENT = c.sentinel, PM = c.pmask;
   const uint32_t nchunks = (n + 63u) / 64u;
-  const uint32_t cpw = (nchunks + nwaves - 1u) / nwaves;
-  const uint32_t cbeg = gw * cpw, cend = min(nchunks, cbeg + cpw);
-  const size_t sbase = (size_t)gw * cpw * 64u, mbase = (size_t)gw * cpw * 2u;
-  uint32_t ns = 0, nm = 0;  // records written to this wave's regions
+  const uint32_t R = c.n_regions, rpw = R / nwaves;
+  const uint32_t cpr = (nchunks + R - 1u) / R;  // chunks per region
+  const uint32_t cbeg = min(nchunks, gw * rpw * cpr), cend = min(nchunks, (gw + 1u) * rpw * cpr);
+  if (gw == 0u && lane == 0) st->reg_next = 0u;
+  uint32_t reg = gw * rpw;  // region of the chunk being classified
+  size_t sbase = (size_t)reg * cpr * 64u, mbase = (size_t)reg * cpr * 2u;
+  uint32_t ns = 0, nm = 0;  // records written to region reg
+  // Rows (permuted ids; SENT is above every row) and batches of the window's two halves,
+  // loaded one chunk ahead: the next chunk's first half is this chunk's second half, its
+  // second half is loaded while this chunk is classified.  nprev / npb: the entry before
+  // the chunk (the previous chunk's last).
+  uint32_t nr0 = SENT, nb0 = 0u, nr1 = SENT, nb1 = 0u, nprev = 0u, npb = 0u;
+  if (cbeg < cend) {
+    const uint32_t i0 = cbeg * 64u;
+    const uint32_t j0 = i0 + lane, j1 = i0 + 64u + lane;
+    nr0 = j0 < n ? Ks[j0] : SENT;
+    nr1 = j1 < n ? Ks[j1] : SENT;
+    nb0 = j0 < n ? Ss[j0] >> kSlotBatchShift : 0u;
+    nb1 = j1 < n ? Ss[j1] >> kSlotBatchShift : 0u;
+    nprev = i0 > 0u ? Ks[i0 - 1u] : ~__shfl(nr0, 0, EP_WAVE);
+    npb = i0 > 0u ? Ss[i0 - 1u] >> kSlotBatchShift : 0u;
+  }
   for (uint32_t chunk = cbeg; chunk < cend; ++chunk) {
     const uint32_t i0 = chunk * 64u;
-    const uint32_t j0 = i0 + lane, j1 = i0 + 64u + lane;
-    const uint32_t r0 = j0 < n ? Ks[j0] : SENT;  // rows (permuted ids); SENT is above every row
-    const uint32_t r1 = j1 < n ? Ks[j1] : SENT;
-    const uint32_t b0 = j0 < n ? Ss[j0] >> kSlotBatchShift : 0u;  // batches
-    const uint32_t b1 = j1 < n ? Ss[j1] >> kSlotBatchShift : 0u;
+    if (chunk / cpr != reg) {  // the next region: close this one
+      if (lane == 0) {
+        c.scount[reg] = ns;
+        c.mcount[reg] = nm;
+      }
+      reg = chunk / cpr;
+      sbase = (size_t)reg * cpr * 64u;
+      mbase = (size_t)reg * cpr * 2u;
+      ns = nm = 0u;
+    }
+    const uint32_t r0 = nr0, r1 = nr1, b0 = nb0, b1 = nb1;
     uint32_t prev = __shfl_up(r0, 1, EP_WAVE), pb0 = __shfl_up(b0, 1, EP_WAVE);
     if (lane == 0) {
-      prev = i0 > 0u ? Ks[i0 - 1u] : ~r0;
-      pb0 = i0 > 0u ? Ss[i0 - 1u] >> kSlotBatchShift : 0u;
+      prev = nprev;
+      pb0 = npb;
     }
     const uint32_t last0 = __shfl(r0, 63, EP_WAVE), lastb0 = __shfl(b0, 63, EP_WAVE);
+    nprev = last0;
+    npb = lastb0;
+    nr0 = r1;
+    nb0 = b1;
+    {
+      const uint32_t jn = i0 + 128u + lane;  // the next chunk's second half
+      nr1 = (chunk + 1u < cend && jn < n) ? Ks[jn] : SENT;
+      nb1 = (chunk + 1u < cend && jn < n) ? Ss[jn] >> kSlotBatchShift : 0u;
+    }
     uint32_t prev1 = __shfl_up(r1, 1, EP_WAVE), pb1 = __shfl_up(b1, 1, EP_WAVE);
     if (lane == 0) {
       prev1 = last0;
@@ -1912,9 +1945,9 @@ __global__ __launch_bounds__(256) void k_seg_plan(StepConfig c, StepState* st, c
     nm += (uint32_t)__popcll(mmask);
     __builtin_amdgcn_wave_barrier();
   }
-  if (lane == 0) {
-    c.scount[gw] = ns;
-    c.mcount[gw] = nm;
+  if (lane == 0 && cbeg < cend) {
+    c.scount[reg] = ns;
+    c.mcount[reg] = nm;
   }
 }
 
@@ -1934,14 +1967,11 @@ __global__ __launch_bounds__(256) void k_seg_update(StepConfig c, StepState* st,
   const int grp = lane / L, lg = lane % L;
   const bool gact = grp < G;
   const uint64_t gmask = (L == 64) ? ~0ull : (((1ull << L) - 1ull) << ((grp * L) & 63));
-  const uint32_t gw = blockIdx.x * NW + (tid >> 6);
-  const uint32_t nwaves = gridDim.x * NW;
-  // this wave's region of short-row records (written by k_seg_plan wave gw)
+  // plan regions holding chunks, taken one at a time from the queue st->reg_next
   const uint32_t n = st->n_keys <= c.key_cap ? st->n_real : 0u;
   const uint32_t nchunks = (n + 63u) / 64u;
-  const uint32_t cpw = (nchunks + nwaves - 1u) / nwaves;
-  const uint32_t nrec = gw * cpw < nchunks ? c.scount[gw] : 0u;
-  const uint4* recs = c.srec + (size_t)gw * cpw * 64u;
+  const uint32_t cpr = (nchunks + c.n_regions - 1u) / c.n_regions;
+  const uint32_t nreg = cpr ? (nchunks + cpr - 1u) / cpr : 0u;
   const StepScalars sv = step_scalars(st);
   __shared__ float s_sd[kMaxGroup];
   if (tid < (int)kMaxGroup) {
@@ -1951,184 +1981,192 @@ __global__ __launch_bounds__(256) void k_seg_update(StepConfig c, StepState* st,
   __syncthreads();
   const bool noisy = c.noise > 1e-9f;
   uint32_t done = 0;
-  // Software pipeline over the wave's rows: iteration k issues, back to back, the row's
-  // theta/m/v, the next row's slots (its record arrived one iteration earlier), the
-  // record two rows ahead and the row's adjoint rows (its slots arrived one iteration
-  // earlier), so each row costs one memory latency instead of the chain record ->
-  // slots -> adjoint rows.  The arithmetic and its order are those of one row at a time.
-  auto load_rec = [&](uint32_t jj) -> uint4 {
-    return (gact && jj < nrec) ? recs[jj] : make_uint4(0u, 0u, 0u, 0u);
-  };
-  // the row's slots, spread over the group's lanes (entry e at lane e % L, register e / L)
-  auto load_slots = [&](const uint4& r, uint32_t (&out)[NS]) {
-#pragma unroll
-    for (int i = 0; i < NS; ++i) {
-      const uint32_t e = (uint32_t)(lg + i * L);
-      out[i] = (e < (r.w & 63u)) ? (Ss[r.x + e] & kSlotMask) : 0u;
-    }
-  };
-  uint4 rec_next = load_rec((uint32_t)grp);
-  uint32_t sl_next[NS];
-  load_slots(rec_next, sl_next);
-  uint4 rec_after = load_rec((uint32_t)(G + grp));
-  for (uint32_t j0 = 0; j0 < nrec; j0 += (uint32_t)G) {
-    const uint32_t j = j0 + (uint32_t)grp;
-    const bool has = gact && j < nrec;
-    uint4 rec = rec_next;
-    if constexpr (G == 1) {
-      // one row per wave: its record is wave-uniform (scalar registers and branches)
-      rec = make_uint4(__builtin_amdgcn_readfirstlane(rec.x), __builtin_amdgcn_readfirstlane(rec.y),
-                       __builtin_amdgcn_readfirstlane(rec.z), __builtin_amdgcn_readfirstlane(rec.w));
-    }
-    uint32_t sl[NS];
-#pragma unroll
-    for (int i = 0; i < NS; ++i) sl[i] = sl_next[i];
-    const uint32_t f = rec.y, rmask = rec.z;
-    const uint32_t len = rec.w & 63u;
-    float4 m4[NQ], v4[NQ], t4[NQ];
-    const size_t roff = (size_t)f * D;
-    const size_t ooff = (size_t)f * c.ostride;  // optimizer record [master |] m | v
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const uint32_t col = (uint32_t)(q * L + lg) * 4u;
-      m4[q] = v4[q] = t4[q] = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (has && !c.grad_out) {
-        m4[q] = *reinterpret_cast<const float4*>(c.mom + ooff + col);
-        v4[q] = *reinterpret_cast<const float4*>(c.var + ooff + col);
-        if (c.master) {
-          t4[q] = *reinterpret_cast<const float4*>(c.master + ooff + col);
-        } else if (c.bf16) {
-          const uint2 b = *reinterpret_cast<const uint2*>((const uint16_t*)c.table + roff + col);
-          t4[q] = make_float4(__uint_as_float(b.x << 16), __uint_as_float(b.x & 0xffff0000u),
-                              __uint_as_float(b.y << 16), __uint_as_float(b.y & 0xffff0000u));
-        } else {
-          t4[q] = *reinterpret_cast<const float4*>((const float*)c.table + roff + col);
-        }
+  for (;;) {
+    uint32_t reg = 0u;
+    if (lane == 0) reg = atomicAdd(&st->reg_next, 1u);
+    reg = (uint32_t)__shfl((int)reg, 0, EP_WAVE);
+    if (reg >= nreg) break;
+    const uint32_t nrec = c.scount[reg];
+    const uint4* recs = c.srec + (size_t)reg * cpr * 64u;
+    // Software pipeline over the region's rows: iteration k issues, back to back, the row's
+    // theta/m/v, the next row's slots (its record arrived one iteration earlier), the
+    // record two rows ahead and the row's adjoint rows (its slots arrived one iteration
+    // earlier), so each row costs one memory latency instead of the chain record ->
+    // slots -> adjoint rows.  The arithmetic and its order are those of one row at a time.
+    auto load_rec = [&](uint32_t jj) -> uint4 {
+      return (gact && jj < nrec) ? recs[jj] : make_uint4(0u, 0u, 0u, 0u);
+    };
+    // the row's slots, spread over the group's lanes (entry e at lane e % L, register e / L)
+    auto load_slots = [&](const uint4& r, uint32_t (&out)[NS]) {
+  #pragma unroll
+      for (int i = 0; i < NS; ++i) {
+        const uint32_t e = (uint32_t)(lg + i * L);
+        out[i] = (e < (r.w & 63u)) ? (Ss[r.x + e] & kSlotMask) : 0u;
       }
-    }
-    rec_next = rec_after;
+    };
+    uint4 rec_next = load_rec((uint32_t)grp);
+    uint32_t sl_next[NS];
     load_slots(rec_next, sl_next);
-    rec_after = load_rec(j0 + (uint32_t)(2 * G) + (uint32_t)grp);
-    const uint32_t nruns = has ? (uint32_t)__popc(rmask) : 0u;
-    uint32_t maxruns = nruns;
-    if constexpr (G > 1) {
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) maxruns = max(maxruns, (uint32_t)__shfl_xor(maxruns, off, EP_WAVE));
-    }
-    bool upd = false;
-    uint32_t rest = rmask;  // run starts not yet consumed
-    for (uint32_t rq = 0; rq < maxruns; ++rq) {
-      const bool hr = rq < nruns;
-      const uint32_t a = hr ? (uint32_t)__builtin_ctz(rest) : 0u;
-      rest = hr ? (rest & (rest - 1u)) : rest;
-      const uint32_t z = hr ? (rest ? (uint32_t)__builtin_ctz(rest) : len) : 0u;
-      const uint32_t bq = (rec.w >> (6u + 3u * rq)) & 7u;
-      uint32_t maxlen = z - a;
-      if constexpr (G > 1) {
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) maxlen = max(maxlen, (uint32_t)__shfl_xor(maxlen, off, EP_WAVE));
+    uint4 rec_after = load_rec((uint32_t)(G + grp));
+    for (uint32_t j0 = 0; j0 < nrec; j0 += (uint32_t)G) {
+      const uint32_t j = j0 + (uint32_t)grp;
+      const bool has = gact && j < nrec;
+      uint4 rec = rec_next;
+      if constexpr (G == 1) {
+        // one row per wave: its record is wave-uniform (scalar registers and branches)
+        rec = make_uint4(__builtin_amdgcn_readfirstlane(rec.x), __builtin_amdgcn_readfirstlane(rec.y),
+                         __builtin_amdgcn_readfirstlane(rec.z), __builtin_amdgcn_readfirstlane(rec.w));
       }
-      float4 acc[NQ];
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) acc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
-      for (uint32_t u0 = 0; u0 < maxlen; u0 += U) {
-        float4 r[U][NQ];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const uint32_t e = a + u0 + (uint32_t)u;
-          const bool valid = hr && e < z;
-          uint32_t slot;
-          if constexpr (G == 1 && NS == 1) {
-            slot = __builtin_amdgcn_readlane(sl[0], (int)(e & 63u));  // entry e sits in lane e
-          } else {
-            const uint32_t ei = e / (uint32_t)L;
-            uint32_t pick = sl[0];
-#pragma unroll
-            for (int i = 1; i < NS; ++i) pick = (ei == (uint32_t)i) ? sl[i] : pick;
-            slot = __shfl(pick, grp * L + (int)(e % (uint32_t)L), EP_WAVE);
-          }
-          if constexpr (H) {
-            const float sc = valid ? c.ascale[slot] : 0.f;
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-              const uint32_t col = (uint32_t)(q * L + lg) * 4u;
-              const uint2 t = valid ? *reinterpret_cast<const uint2*>(c.adjh + (size_t)slot * D + col) : make_uint2(0u, 0u);
-              r[u][q] = make_float4(sc * f16_to_f32((uint16_t)(t.x & 0xffffu)), sc * f16_to_f32((uint16_t)(t.x >> 16)),
-                                    sc * f16_to_f32((uint16_t)(t.y & 0xffffu)), sc * f16_to_f32((uint16_t)(t.y >> 16)));
-            }
-          } else {
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-              const uint32_t col = (uint32_t)(q * L + lg) * 4u;
-              r[u][q] = valid ? *reinterpret_cast<const float4*>(c.adj + (size_t)slot * D + col)
-                              : make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-#pragma unroll
-          for (int q = 0; q < NQ; ++q) {
-            acc[q].x += r[u][q].x; acc[q].y += r[u][q].y; acc[q].z += r[u][q].z; acc[q].w += r[u][q].w;
-          }
-        }
-      }
-      if (noisy && hr) {  // mod.rs:238-247: noise on the batch's merged gradient row
-        const float sd = s_sd[bq];
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-          float z[4];
-          noise4(c, f, (uint32_t)(q * L + lg), sv.step0 + bq, z);
-          acc[q].x += sd * z[0]; acc[q].y += sd * z[1]; acc[q].z += sd * z[2]; acc[q].w += sd * z[3];
-        }
-      }
-      if (c.grad_out) {
-        uint32_t ub = 0;
-        const uint64_t heads = __ballot(hr && lg == 0);
-        if (lane == 0 && heads) ub = atomicAdd(&st->n_unique, (uint32_t)__popcll(heads));
-        ub = __shfl(ub, 0, EP_WAVE);
-        const uint32_t u = ub + (uint32_t)__popcll(heads & ((grp * L == 0) ? 0ull : (~0ull >> (64 - grp * L))));
-        if (hr) {
-#pragma unroll
-          for (int q = 0; q < NQ; ++q)
-            *reinterpret_cast<float4*>(c.G + (size_t)u * D + (q * L + lg) * 4) = acc[q];
-          if (lg == 0) c.ufeat[u] = f | (bq << 27);
-        }
-        continue;
-      }
-      bool bad = false;
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) bad |= isnan(acc[q].x) || isnan(acc[q].y) || isnan(acc[q].z) || isnan(acc[q].w);
-      const bool skip = (__ballot(hr && bad) & gmask) != 0ull;  // optimizer.rs:56, per batch
-      if (hr && !skip) {
-        const float lr = s_lr[bq];
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-          adam_math(t4[q].x, m4[q].x, v4[q].x, acc[q].x, lr, sv.ibc1, sv.ibc2);
-          adam_math(t4[q].y, m4[q].y, v4[q].y, acc[q].y, lr, sv.ibc1, sv.ibc2);
-          adam_math(t4[q].z, m4[q].z, v4[q].z, acc[q].z, lr, sv.ibc1, sv.ibc2);
-          adam_math(t4[q].w, m4[q].w, v4[q].w, acc[q].w, lr, sv.ibc1, sv.ibc2);
-        }
-        upd = true;
-      }
-    }
-    if (upd) {
-      if (lg == 0) ++done;
-#pragma unroll
+      uint32_t sl[NS];
+  #pragma unroll
+      for (int i = 0; i < NS; ++i) sl[i] = sl_next[i];
+      const uint32_t f = rec.y, rmask = rec.z;
+      const uint32_t len = rec.w & 63u;
+      float4 m4[NQ], v4[NQ], t4[NQ];
+      const size_t roff = (size_t)f * D;
+      const size_t ooff = (size_t)f * c.ostride;  // optimizer record [master |] m | v
+  #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         const uint32_t col = (uint32_t)(q * L + lg) * 4u;
-        *reinterpret_cast<float4*>(c.mom + ooff + col) = m4[q];
-        *reinterpret_cast<float4*>(c.var + ooff + col) = v4[q];
-        if (c.bf16) {
-          if (c.master) *reinterpret_cast<float4*>(c.master + ooff + col) = t4[q];
-          const uint32_t h0 = sr_key(f, col >> 2, sv.sr_step, (uint32_t)c.seed);
-          const uint32_t h1 = hash32(h0 ^ 0x27D4EB2Fu);
-          uint2 pk2;
-          pk2.x = (uint32_t)f32_to_bf16_sr(t4[q].x, h0 & 0xffffu) | ((uint32_t)f32_to_bf16_sr(t4[q].y, h0 >> 16) << 16);
-          pk2.y = (uint32_t)f32_to_bf16_sr(t4[q].z, h1 & 0xffffu) | ((uint32_t)f32_to_bf16_sr(t4[q].w, h1 >> 16) << 16);
-          *reinterpret_cast<uint2*>((uint16_t*)c.table + roff + col) = pk2;
-        } else {
-          *reinterpret_cast<float4*>((float*)c.table + roff + col) = t4[q];
+        m4[q] = v4[q] = t4[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (has && !c.grad_out) {
+          m4[q] = *reinterpret_cast<const float4*>(c.mom + ooff + col);
+          v4[q] = *reinterpret_cast<const float4*>(c.var + ooff + col);
+          if (c.master) {
+            t4[q] = *reinterpret_cast<const float4*>(c.master + ooff + col);
+          } else if (c.bf16) {
+            const uint2 b = *reinterpret_cast<const uint2*>((const uint16_t*)c.table + roff + col);
+            t4[q] = make_float4(__uint_as_float(b.x << 16), __uint_as_float(b.x & 0xffff0000u),
+                                __uint_as_float(b.y << 16), __uint_as_float(b.y & 0xffff0000u));
+          } else {
+            t4[q] = *reinterpret_cast<const float4*>((const float*)c.table + roff + col);
+          }
+        }
+      }
+      rec_next = rec_after;
+      load_slots(rec_next, sl_next);
+      rec_after = load_rec(j0 + (uint32_t)(2 * G) + (uint32_t)grp);
+      const uint32_t nruns = has ? (uint32_t)__popc(rmask) : 0u;
+      uint32_t maxruns = nruns;
+      if constexpr (G > 1) {
+  #pragma unroll
+        for (int off = 32; off > 0; off >>= 1) maxruns = max(maxruns, (uint32_t)__shfl_xor(maxruns, off, EP_WAVE));
+      }
+      bool upd = false;
+      uint32_t rest = rmask;  // run starts not yet consumed
+      for (uint32_t rq = 0; rq < maxruns; ++rq) {
+        const bool hr = rq < nruns;
+        const uint32_t a = hr ? (uint32_t)__builtin_ctz(rest) : 0u;
+        rest = hr ? (rest & (rest - 1u)) : rest;
+        const uint32_t z = hr ? (rest ? (uint32_t)__builtin_ctz(rest) : len) : 0u;
+        const uint32_t bq = (rec.w >> (6u + 3u * rq)) & 7u;
+        uint32_t maxlen = z - a;
+        if constexpr (G > 1) {
+  #pragma unroll
+          for (int off = 32; off > 0; off >>= 1) maxlen = max(maxlen, (uint32_t)__shfl_xor(maxlen, off, EP_WAVE));
+        }
+        float4 acc[NQ];
+  #pragma unroll
+        for (int q = 0; q < NQ; ++q) acc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (uint32_t u0 = 0; u0 < maxlen; u0 += U) {
+          float4 r[U][NQ];
+  #pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const uint32_t e = a + u0 + (uint32_t)u;
+            const bool valid = hr && e < z;
+            uint32_t slot;
+            if constexpr (G == 1 && NS == 1) {
+              slot = __builtin_amdgcn_readlane(sl[0], (int)(e & 63u));  // entry e sits in lane e
+            } else {
+              const uint32_t ei = e / (uint32_t)L;
+              uint32_t pick = sl[0];
+  #pragma unroll
+              for (int i = 1; i < NS; ++i) pick = (ei == (uint32_t)i) ? sl[i] : pick;
+              slot = __shfl(pick, grp * L + (int)(e % (uint32_t)L), EP_WAVE);
+            }
+            if constexpr (H) {
+              const float sc = valid ? c.ascale[slot] : 0.f;
+  #pragma unroll
+              for (int q = 0; q < NQ; ++q) {
+                const uint32_t col = (uint32_t)(q * L + lg) * 4u;
+                const uint2 t = valid ? *reinterpret_cast<const uint2*>(c.adjh + (size_t)slot * D + col) : make_uint2(0u, 0u);
+                r[u][q] = make_float4(sc * f16_to_f32((uint16_t)(t.x & 0xffffu)), sc * f16_to_f32((uint16_t)(t.x >> 16)),
+                                      sc * f16_to_f32((uint16_t)(t.y & 0xffffu)), sc * f16_to_f32((uint16_t)(t.y >> 16)));
+              }
+            } else {
+  #pragma unroll
+              for (int q = 0; q < NQ; ++q) {
+                const uint32_t col = (uint32_t)(q * L + lg) * 4u;
+                r[u][q] = valid ? *reinterpret_cast<const float4*>(c.adj + (size_t)slot * D + col)
+                                : make_float4(0.f, 0.f, 0.f, 0.f);
+              }
+            }
+          }
+  #pragma unroll
+          for (int u = 0; u < U; ++u) {
+  #pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+              acc[q].x += r[u][q].x; acc[q].y += r[u][q].y; acc[q].z += r[u][q].z; acc[q].w += r[u][q].w;
+            }
+          }
+        }
+        if (noisy && hr) {  // mod.rs:238-247: noise on the batch's merged gradient row
+          const float sd = s_sd[bq];
+  #pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            float z[4];
+            noise4(c, f, (uint32_t)(q * L + lg), sv.step0 + bq, z);
+            acc[q].x += sd * z[0]; acc[q].y += sd * z[1]; acc[q].z += sd * z[2]; acc[q].w += sd * z[3];
+          }
+        }
+        if (c.grad_out) {
+          uint32_t ub = 0;
+          const uint64_t heads = __ballot(hr && lg == 0);
+          if (lane == 0 && heads) ub = atomicAdd(&st->n_unique, (uint32_t)__popcll(heads));
+          ub = __shfl(ub, 0, EP_WAVE);
+          const uint32_t u = ub + (uint32_t)__popcll(heads & ((grp * L == 0) ? 0ull : (~0ull >> (64 - grp * L))));
+          if (hr) {
+  #pragma unroll
+            for (int q = 0; q < NQ; ++q)
+              *reinterpret_cast<float4*>(c.G + (size_t)u * D + (q * L + lg) * 4) = acc[q];
+            if (lg == 0) c.ufeat[u] = f | (bq << 27);
+          }
+          continue;
+        }
+        bool bad = false;
+  #pragma unroll
+        for (int q = 0; q < NQ; ++q) bad |= isnan(acc[q].x) || isnan(acc[q].y) || isnan(acc[q].z) || isnan(acc[q].w);
+        const bool skip = (__ballot(hr && bad) & gmask) != 0ull;  // optimizer.rs:56, per batch
+        if (hr && !skip) {
+          const float lr = s_lr[bq];
+  #pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            adam_math(t4[q].x, m4[q].x, v4[q].x, acc[q].x, lr, sv.ibc1, sv.ibc2);
+            adam_math(t4[q].y, m4[q].y, v4[q].y, acc[q].y, lr, sv.ibc1, sv.ibc2);
+            adam_math(t4[q].z, m4[q].z, v4[q].z, acc[q].z, lr, sv.ibc1, sv.ibc2);
+            adam_math(t4[q].w, m4[q].w, v4[q].w, acc[q].w, lr, sv.ibc1, sv.ibc2);
+          }
+          upd = true;
+        }
+      }
+      if (upd) {
+        if (lg == 0) ++done;
+  #pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          const uint32_t col = (uint32_t)(q * L + lg) * 4u;
+          *reinterpret_cast<float4*>(c.mom + ooff + col) = m4[q];
+          *reinterpret_cast<float4*>(c.var + ooff + col) = v4[q];
+          if (c.bf16) {
+            if (c.master) *reinterpret_cast<float4*>(c.master + ooff + col) = t4[q];
+            const uint32_t h0 = sr_key(f, col >> 2, sv.sr_step, (uint32_t)c.seed);
+            const uint32_t h1 = hash32(h0 ^ 0x27D4EB2Fu);
+            uint2 pk2;
+            pk2.x = (uint32_t)f32_to_bf16_sr(t4[q].x, h0 & 0xffffu) | ((uint32_t)f32_to_bf16_sr(t4[q].y, h0 >> 16) << 16);
+            pk2.y = (uint32_t)f32_to_bf16_sr(t4[q].z, h1 & 0xffffu) | ((uint32_t)f32_to_bf16_sr(t4[q].w, h1 >> 16) << 16);
+            *reinterpret_cast<uint2*>((uint16_t*)c.table + roff + col) = pk2;
+          } else {
+            *reinterpret_cast<float4*>((float*)c.table + roff + col) = t4[q];
+          }
         }
       }
     }
@@ -2238,9 +2276,15 @@ __global__ __launch_bounds__(256) void k_pieces(StepConfig c, StepState* st, con
   }
   __syncthreads();
   const uint32_t np = st->n_pieces < c.piece_cap ? st->n_pieces : c.piece_cap;
+  // Batch by batch: a piece reads adjoint rows of its batch only, so every wave working
+  // on the same batch keeps the rows being read within one batch's share of the adjoint
+  // buffer (which the memory-side cache can hold) instead of all of them.
+  for (uint32_t bq = 0; bq <= c.pmask; ++bq)
   for (uint32_t gp = gw; gp < np; gp += nwaves) {
+    const uint32_t info = c.piece_info[gp];
+    if ((info >> 29) != bq) continue;
     const uint32_t s0 = c.piece_st[gp];
-    const uint32_t m = c.piece_info[gp];  // a multiple of PIECE entries at most
+    const uint32_t m = info & 0x1fffffffu;  // entries: a multiple of PIECE at most
     float acc[PL];
 #pragma unroll
     for (int q = 0; q < PL; ++q) acc[q] = 0.f;
@@ -2260,59 +2304,62 @@ __global__ __launch_bounds__(256) void k_pieces(StepConfig c, StepState* st, con
     }
     store_slice<D>(c.part + (size_t)gp * D, lane, acc);
   }
-  // medium rows of plan region gw
+  // medium rows of the plan regions gw, gw + nwaves, ...
   const uint32_t n = st->n_keys <= c.key_cap ? st->n_real : 0u;
   const uint32_t nchunks = (n + 63u) / 64u;
-  const uint32_t cpw = (nchunks + nwaves - 1u) / nwaves;
-  const uint32_t nmed = gw * cpw < nchunks ? c.mcount[gw] : 0u;
-  const uint2* recs = c.mrec + (size_t)gw * cpw * 2u;
+  const uint32_t cpr = (nchunks + c.n_regions - 1u) / c.n_regions;
+  const uint32_t nreg = cpr ? (nchunks + cpr - 1u) / cpr : 0u;
   const StepScalars sv = step_scalars(st);
   uint32_t done = 0;
-  for (uint32_t k = 0; k < nmed; ++k) {
-    const uint2 rec = recs[k];
-    const uint32_t start = rec.x, len = rec.y;  // 32 < len <= 128
-    const uint32_t e0 = (uint32_t)lane, e1 = 64u + (uint32_t)lane;
-    const uint32_t kk0 = e0 < len ? Ss[start + e0] >> kSlotBatchShift : 0u;  // batches
-    const uint32_t kk1 = e1 < len ? Ss[start + e1] >> kSlotBatchShift : 0u;
-    const uint32_t ss0 = e0 < len ? (Ss[start + e0] & kSlotMask) : 0u;
-    const uint32_t ss1 = e1 < len ? (Ss[start + e1] & kSlotMask) : 0u;
-    const uint32_t f = row_of_key(c, Ks[start]);
-    float th[PL], mo[PL], va[PL];
-    if (!c.grad_out) load_row_state<D>(c, f, lane, th, mo, va);
-    const uint32_t last0 = __shfl(kk0, 63, EP_WAVE);
-    uint32_t p0 = __shfl_up(kk0, 1, EP_WAVE);
-    uint32_t p1 = __shfl_up(kk1, 1, EP_WAVE);
-    if (lane == 0) { p0 = ~kk0; p1 = last0; }
-    uint64_t rs0 = __ballot(e0 < len && kk0 != p0);  // run starts
-    uint64_t rs1 = __ballot(e1 < len && kk1 != p1);
-    bool upd = false;
-    while (rs0 | rs1) {
-      const uint32_t a = rs0 ? (uint32_t)(__ffsll((unsigned long long)rs0) - 1) : 64u + (uint32_t)(__ffsll((unsigned long long)rs1) - 1);
-      if (rs0) rs0 &= rs0 - 1ull; else rs1 &= rs1 - 1ull;
-      const uint32_t z = rs0 ? (uint32_t)(__ffsll((unsigned long long)rs0) - 1)
-                             : (rs1 ? 64u + (uint32_t)(__ffsll((unsigned long long)rs1) - 1) : len);
-      const uint32_t ka = __shfl(kk0, (int)(a & 63u), EP_WAVE);
-      const uint32_t kb = __shfl(kk1, (int)(a & 63u), EP_WAVE);
-      const uint32_t bq = a < 64u ? ka : kb;
-      float acc[PL];
-#pragma unroll
-      for (int q = 0; q < PL; ++q) acc[q] = 0.f;
-      for (uint32_t t0 = a; t0 < z; t0 += U) {
-        float r[U][PL];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const uint32_t t = (t0 + u < z) ? (t0 + u) : (z - 1u);
-          const uint32_t sa = __shfl(ss0, (int)(t & 63u), EP_WAVE);
-          const uint32_t sb = __shfl(ss1, (int)(t & 63u), EP_WAVE);
-          load_adj_row<D, H>(c, t < 64u ? sa : sb, lane, r[u]);
+  for (uint32_t reg = gw; reg < nreg; reg += nwaves) {
+    const uint32_t nmed = c.mcount[reg];
+    const uint2* recs = c.mrec + (size_t)reg * cpr * 2u;
+    for (uint32_t k = 0; k < nmed; ++k) {
+      const uint2 rec = recs[k];
+      const uint32_t start = rec.x, len = rec.y;  // 32 < len <= 128
+      const uint32_t e0 = (uint32_t)lane, e1 = 64u + (uint32_t)lane;
+      const uint32_t kk0 = e0 < len ? Ss[start + e0] >> kSlotBatchShift : 0u;  // batches
+      const uint32_t kk1 = e1 < len ? Ss[start + e1] >> kSlotBatchShift : 0u;
+      const uint32_t ss0 = e0 < len ? (Ss[start + e0] & kSlotMask) : 0u;
+      const uint32_t ss1 = e1 < len ? (Ss[start + e1] & kSlotMask) : 0u;
+      const uint32_t f = row_of_key(c, Ks[start]);
+      float th[PL], mo[PL], va[PL];
+      if (!c.grad_out) load_row_state<D>(c, f, lane, th, mo, va);
+      const uint32_t last0 = __shfl(kk0, 63, EP_WAVE);
+      uint32_t p0 = __shfl_up(kk0, 1, EP_WAVE);
+      uint32_t p1 = __shfl_up(kk1, 1, EP_WAVE);
+      if (lane == 0) { p0 = ~kk0; p1 = last0; }
+      uint64_t rs0 = __ballot(e0 < len && kk0 != p0);  // run starts
+      uint64_t rs1 = __ballot(e1 < len && kk1 != p1);
+      bool upd = false;
+      while (rs0 | rs1) {
+        const uint32_t a = rs0 ? (uint32_t)(__ffsll((unsigned long long)rs0) - 1) : 64u + (uint32_t)(__ffsll((unsigned long long)rs1) - 1);
+        if (rs0) rs0 &= rs0 - 1ull; else rs1 &= rs1 - 1ull;
+        const uint32_t z = rs0 ? (uint32_t)(__ffsll((unsigned long long)rs0) - 1)
+                               : (rs1 ? 64u + (uint32_t)(__ffsll((unsigned long long)rs1) - 1) : len);
+        const uint32_t ka = __shfl(kk0, (int)(a & 63u), EP_WAVE);
+        const uint32_t kb = __shfl(kk1, (int)(a & 63u), EP_WAVE);
+        const uint32_t bq = a < 64u ? ka : kb;
+        float acc[PL];
+  #pragma unroll
+        for (int q = 0; q < PL; ++q) acc[q] = 0.f;
+        for (uint32_t t0 = a; t0 < z; t0 += U) {
+          float r[U][PL];
+  #pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const uint32_t t = (t0 + u < z) ? (t0 + u) : (z - 1u);
+            const uint32_t sa = __shfl(ss0, (int)(t & 63u), EP_WAVE);
+            const uint32_t sb = __shfl(ss1, (int)(t & 63u), EP_WAVE);
+            load_adj_row<D, H>(c, t < 64u ? sa : sb, lane, r[u]);
+          }
+          add_rows_tree<U, PL>(acc, r, z - t0);
         }
-        add_rows_tree<U, PL>(acc, r, z - t0);
+        upd |= apply_run<D>(c, st, sv, s_lr[bq], s_sd[bq], f, bq, lane, acc, th, mo, va);
       }
-      upd |= apply_run<D>(c, st, sv, s_lr[bq], s_sd[bq], f, bq, lane, acc, th, mo, va);
-    }
-    if (upd) {
-      store_row_state<D>(c, sv, f, lane, th, mo, va);
-      ++done;
+      if (upd) {
+        store_row_state<D>(c, sv, f, lane, th, mo, va);
+        ++done;
+      }
     }
   }
   if (lane == 0 && done) atomicAdd((unsigned long long*)&st->stat_unique, (unsigned long long)done);
@@ -2816,11 +2863,24 @@ static void launch_seg_update_variant(const StepConfig& c, StepState* st, int bu
     hipLaunchKernelGGL((k_seg_update<D, 4, 8, H>), dim3(grid), dim3(256), 0, s, c, st, c.slots[buf]);
 }
 
+// Blocks of 256 threads of kernel k resident on the current device at once, at most cap.
+template <typename Kernel>
+static int resident_blocks(Kernel k, int cap) {
+  int dev = 0, ncu = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return cap;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) return cap;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, 256, 0) != hipSuccess || per <= 0) return cap;
+  return per * ncu < cap ? per * ncu : cap;
+}
+
 template <int D, bool H>
 static void launch_row_kernels(const StepConfig& c, StepState* st, int buf, hipStream_t s, int grid, hipEvent_t* ev) {
   launch_seg_update_variant<D, H>(c, st, buf, s, grid);
   if (ev) (void)hipEventRecord(ev[1], s);
-  hipLaunchKernelGGL((k_pieces<D, H>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
+  // k_pieces sweeps the pieces batch by batch: one round of resident blocks keeps every
+  // wave on the same batch
+  const int pgrid = resident_blocks(k_pieces<D, H>, grid);
+  hipLaunchKernelGGL((k_pieces<D, H>), dim3(pgrid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
   if (ev) (void)hipEventRecord(ev[2], s);
   hipLaunchKernelGGL((k_long_combine<D, H>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
   if (ev) (void)hipEventRecord(ev[3], s);
