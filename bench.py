@@ -126,6 +126,7 @@ def run_form(w, form, steps, warmup, scatter=None, concurrent=1, graph=True, fp1
         "concurrent_batches": int(cfg.concurrent_batches),
         "adjoint_rows": "fp16" if cfg.fp16_adjoints else "fp32",
         "seg_tile": [int(r.seg_vpl), int(r.seg_unroll)],
+        "forward_rows_in_flight_x": int(r.fwd_x),
         "examples_per_s": r.examples / (r.ms_total / 1e3),
         "ms_per_step": r.ms_total / S,
         "examples": int(r.examples),

@@ -67,6 +67,7 @@ class EPConfig(Structure):
         ("max_features_p", c_float),
         ("max_nodes_p", c_float),
         ("fp16_adjoints", c_int32),
+        ("forward_variant", c_int32),
     ]
 
 
@@ -101,6 +102,8 @@ class EPBenchResult(Structure):
         ("last_pieces", c_uint32),
         ("att_items_sq", c_uint64),
         ("att_items", c_uint64),
+        ("fwd_x", c_int32),
+        ("reserved", c_int32),
     ]
 
 

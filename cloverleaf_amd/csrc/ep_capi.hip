@@ -126,7 +126,9 @@ struct ep_ctx {
   // values-per-lane variants (identical results) and time the scatter with events
   int seg_vpl = 0;                 // chosen variant (values per lane, unroll), 0 while tuning
   int seg_unroll = 0;
+  int fwd_x = 0;                   // chosen k_forward variant (rows-in-flight multiplier), 0 while tuning
   std::vector<std::pair<int, int>> tune_cands;
+  std::vector<int> tune_fwd;       // k_forward candidates, timed in the same steps
   std::vector<hipEvent_t> tune_ev;  // two per tuning step
   uint32_t tune_i = 0;
 
@@ -514,9 +516,11 @@ void refresh_base(ep_ctx* c) {
 // ev (nullable): 9 events recorded at the step start and after the scan, forward, emit,
 // sort, plan, short-row, pieces and long-row phases (atomic path: after its scatter and
 // its update, the remaining ones at the same point).
-// The k_seg_update variant of a launch that names none: the configured or tuned one,
-// (4, 8) while the tuner is choosing (every variant gives the same results).
+// The k_seg_update and k_forward variants of a launch that names none: the configured or
+// tuned ones, (4, 8) and 1 while the tuner is choosing (every variant gives the same
+// results).
 void pick_variant(const ep_ctx* c, ep::StepConfig& sc) {
+  if (sc.fwd_x == 0) sc.fwd_x = c->fwd_x ? c->fwd_x : 1;
   if (sc.seg_vpl != 0) return;
   sc.seg_vpl = c->seg_vpl ? c->seg_vpl : 4;
   sc.seg_unroll = c->seg_vpl ? c->seg_unroll : 8;
@@ -648,49 +652,77 @@ constexpr uint32_t kTuneRounds = 4;
 
 bool tuning(const ep_ctx* c) { return c->cfg.scatter == EP_SCATTER_SORT && c->seg_vpl == 0; }
 
-// One eager training step with the next candidate variant; after the last tuning step
-// the fastest variant (median scatter time over the scored rounds) is fixed.
+// One eager training step with the next candidate variants; after the last tuning step
+// the fastest variant of each kernel (median time over the scored rounds) is fixed.  The
+// k_seg_update candidates rotate with the step index modulo their count, the k_forward
+// candidates modulo theirs; a round is as many steps as the larger count.
 int tune_step(ep_ctx* c, const ep::StepConfig& sc) {
   if (c->tune_cands.empty()) {
     for (int v : {16, 12, 8, 4})
       for (int u : {v == 12 ? 2 : 8 / (v / 4), v == 12 ? 4 : 16 / (v / 4)})
         if (ep::seg_variant_supported((int)row_width(c->cfg), v, u)) c->tune_cands.push_back({v, u});
-    if (c->tune_cands.size() <= 1) {
-      const std::pair<int, int> only = c->tune_cands.empty() ? std::make_pair(4, 8) : c->tune_cands[0];
-      c->seg_vpl = only.first;
-      c->seg_unroll = only.second;
+    if (c->tune_cands.empty()) c->tune_cands.push_back({4, 8});
+    c->tune_fwd.clear();
+    if (c->cfg.forward_variant) {
+      c->tune_fwd.push_back(c->cfg.forward_variant);
+    } else {
+      for (int x : {1, 2})
+        if (!c->cfg.attention_dk && ep::fwd_variant_supported((int)row_width(c->cfg), c->cfg.precision != EP_PREC_FP32, x))
+          c->tune_fwd.push_back(x);
+    }
+    if (c->tune_fwd.empty()) c->tune_fwd.push_back(1);
+    if (c->tune_cands.size() <= 1 && c->tune_fwd.size() <= 1) {
+      c->seg_vpl = c->tune_cands[0].first;
+      c->seg_unroll = c->tune_cands[0].second;
+      c->fwd_x = c->tune_fwd[0];
       return launch_training_step(c, sc, true, -1.f);
     }
-    const size_t ne = 9 * c->tune_cands.size() * kTuneRounds;
+    const size_t ne = 9 * std::max(c->tune_cands.size(), c->tune_fwd.size()) * kTuneRounds;
     c->tune_ev.resize(ne, nullptr);
     for (auto& e : c->tune_ev) EP_HIP(hipEventCreate(&e));
   }
-  const uint32_t n = (uint32_t)c->tune_cands.size();
+  const uint32_t n = (uint32_t)c->tune_cands.size(), nf = (uint32_t)c->tune_fwd.size();
+  const uint32_t per_round = std::max(n, nf);
   const uint32_t i = c->tune_i++;
   ep::StepConfig t = sc;
   t.seg_vpl = c->tune_cands[i % n].first;
   t.seg_unroll = c->tune_cands[i % n].second;
+  t.fwd_x = c->tune_fwd[i % nf];
   hipEvent_t evs[9];
   for (int k = 0; k < 9; ++k) evs[k] = c->tune_ev[9 * i + k];
   EP_TRY(launch_training_step(c, t, true, -1.f, evs));
-  if (c->tune_i < n * kTuneRounds) return EP_OK;
+  if (c->tune_i < per_round * kTuneRounds) return EP_OK;
   EP_HIP(hipEventSynchronize(c->tune_ev.back()));
-  float best = INFINITY;
-  std::pair<int, int> pick = c->tune_cands.back();
-  for (uint32_t k = 0; k < n; ++k) {
+  // median over the scored steps (rounds 1..) of a candidate's phase time: events a -> b
+  auto median = [&](uint32_t ncand, uint32_t k, int a, int b, float* out) -> int {
     std::vector<float> ms;
-    for (uint32_t r = 1; r < kTuneRounds; ++r) {
-      const uint32_t j = r * n + k;
+    for (uint32_t j = per_round; j < per_round * kTuneRounds; ++j) {
+      if (j % ncand != k) continue;
       float x = 0.f;
-      EP_HIP(hipEventElapsedTime(&x, c->tune_ev[9 * j + 5], c->tune_ev[9 * j + 6]));  // short-row kernel
+      EP_HIP(hipEventElapsedTime(&x, c->tune_ev[9 * j + a], c->tune_ev[9 * j + b]));
       ms.push_back(x);
     }
     std::sort(ms.begin(), ms.end());
-    const float med = ms[ms.size() / 2];
+    *out = ms.empty() ? INFINITY : ms[ms.size() / 2];
+    return EP_OK;
+  };
+  float best = INFINITY;
+  std::pair<int, int> pick = c->tune_cands.back();
+  for (uint32_t k = 0; k < n; ++k) {
+    float med = 0.f;
+    EP_TRY(median(n, k, 5, 6, &med));  // short-row kernel
     if (med < best) { best = med; pick = c->tune_cands[k]; }
+  }
+  float bestf = INFINITY;
+  int pickf = c->tune_fwd[0];
+  for (uint32_t k = 0; k < nf; ++k) {
+    float med = 0.f;
+    EP_TRY(median(nf, k, 1, 2, &med));  // forward
+    if (med < bestf) { bestf = med; pickf = c->tune_fwd[k]; }
   }
   c->seg_vpl = pick.first;
   c->seg_unroll = pick.second;
+  c->fwd_x = pickf;
   return EP_OK;
 }
 
@@ -763,6 +795,11 @@ int validate_config(const ep_config* cfg) {
   if (cfg->fp16_adjoints && (cfg->precision == EP_PREC_FP32 || cfg->attention_dk || cfg->scatter != EP_SCATTER_SORT))
     return fail(EP_EINVAL, "fp16_adjoints needs a BF16 table (precision bf16 or bf16-master), the mean model and the "
                            "sort scatter");
+  if (cfg->forward_variant != 0 && cfg->forward_variant != 1 &&
+      !(cfg->forward_variant == 2 && !cfg->attention_dk &&
+        ep::fwd_variant_supported((int)row_width(*cfg), cfg->precision != EP_PREC_FP32, 2)))
+    return fail(EP_EINVAL, "forward_variant must be 0 (tuned), 1, or 2 for the widths the 16-byte gather serves "
+                           "(mean model)");
   if ((cfg->seg_vpl == 0) != (cfg->seg_unroll == 0) ||
       (cfg->seg_vpl != 0 && !ep::seg_variant_supported((int)row_width(*cfg), cfg->seg_vpl, cfg->seg_unroll)))
     return fail(EP_EINVAL, "seg_vpl/seg_unroll must both be 0 (tuned) or a built variant: vpl in {4,8,12,16} "
@@ -825,6 +862,7 @@ int ep_create(const ep_config* cfg, ep_ctx** out) {
   c->cfg = *cfg;
   c->seg_vpl = cfg->seg_vpl;  // 0: tuned during the first training steps
   c->seg_unroll = cfg->seg_unroll;
+  c->fwd_x = cfg->forward_variant;  // 0: tuned with the segment tile
   c->device = cfg->device;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete c; return fail(EP_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
@@ -1434,6 +1472,7 @@ int ep_bench_steps(ep_ctx* c, uint32_t warmup, uint32_t steps, int instrument, e
   out->att_items = h1.stat_att_n - h0.stat_att_n;
   out->seg_vpl = c->seg_vpl;
   out->seg_unroll = c->seg_unroll;
+  out->fwd_x = c->fwd_x ? c->fwd_x : 1;
   if (instrument) {
     // per-phase timing: the same number of steps launched one by one with events
     std::vector<hipEvent_t> ev(9 * (size_t)steps);

@@ -206,6 +206,7 @@ struct StepConfig {
   int unique_grid;         // grid size for the persistent row kernels
   int seg_vpl;             // k_seg_adam values per lane (4, 8, 16; chosen by the tuner)
   int seg_unroll;          // k_seg_adam adjoint rows in flight per lane group (tuned)
+  int fwd_x;               // k_forward rows-in-flight multiplier of the wide gathers: 1 or 2 (tuned)
   // Attention model (model = 1): rows are [Q_1..Q_H | K_1..K_H | V_1..V_H], dims =
   // att_heads * (2 att_dk + att_dm).  An example's attention blocks (h(v), one per
   // sampled neighbour, one per negative) start at blkoff[b]; bout/badj hold each
@@ -284,5 +285,7 @@ constexpr uint32_t kPieceEntries = 32;
 bool seg_vpl_supported(int d, int vpl);
 // Whether k_seg_adam<D, VPL, U> is built (U in {8, 16} / (VPL/4)).
 bool seg_variant_supported(int d, int vpl, int unroll);
+// Forward variants: x = 1, and x = 2 (twice the rows in flight) where the wide gather runs.
+bool fwd_variant_supported(int d, int bf16, int x);
 
 }  // namespace ep

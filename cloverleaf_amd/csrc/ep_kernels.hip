@@ -961,12 +961,15 @@ __host__ __device__ inline uint32_t forward_float_words(uint32_t NC, uint32_t K,
 // Neighbour features of ~h(v) are split over the 4 waves (fixed-order combine);
 // the 1+K single-node constructions are dealt round-robin to the waves.
 // ------------------------------------------------------------------------------------
-template <int D, typename T>
+// WX multiplies the rows in flight of the wide gathers (more bytes in flight per wave,
+// more VGPRs, so fewer waves); the partial sums take the rows in the same order for
+// every WX, so the variants give identical bits and the tuner keeps the faster one.
+template <int D, typename T, int WX>
 __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
   constexpr int PL = RowShape<D>::PL;
   constexpr int U = 8;
   constexpr int GU = PL <= 2 ? 16 : 8;  // rows in flight per wave on the staged path
-  constexpr int WU = WideShape<D, T>::RPW >= 4 ? 4 : 8;  // rows in flight per lane group (wide)
+  constexpr int WU = (WideShape<D, T>::RPW >= 4 ? 4 : 8) * WX;  // rows in flight per lane group (wide)
   const uint32_t b = blockIdx.x;
   if (b >= st->batch_n) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -1552,64 +1555,118 @@ __global__ __launch_bounds__(256) void k_rs_scan(uint32_t* ghist, StepState* st,
   }
 }
 
+// Scatter of one digit pass, a tile of 256 x RS_SCATTER_ITEMS elements at a time: wave w
+// takes the tile's w-th quarter (coalesced loads), ranks its elements by digit in
+// element order with its own LDS counters, the tile's digit starts come from a block
+// scan, the tile is reordered by digit in LDS, and the threads then write it out in
+// order, so each digit's elements leave as one contiguous run instead of one scattered
+// store each.  Stable: the tile order is (wave, item, lane) = element order, and tiles
+// and blocks follow each other through the per-block digit bases.
+static constexpr int RS_SCATTER_ITEMS = 16;
+
 template <int RB>
 __global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                     uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                     StepState* st, const uint32_t* __restrict__ ghist, int pass,
                                                     int shift, int G, uint32_t sent) {
   constexpr uint32_t BINS = 1u << RB, MASK = BINS - 1u;
-  __shared__ uint32_t run[BINS];
-  __shared__ uint32_t wc[NW][BINS];  // (round << 7) | count of wave w for digit d in that round
+  constexpr int IT = RS_SCATTER_ITEMS;
+  constexpr uint32_t TILE = 256u * IT;
+  constexpr uint32_t WSPAN = 64u * IT;                          // elements of one wave in a tile
+  constexpr uint32_t DPT = BINS >= 256u ? BINS / 256u : 1u;     // digits per thread in the scans
+  __shared__ uint32_t run[BINS];       // output position of the block's next element of each digit
+  __shared__ uint32_t wcnt[NW][BINS];  // per-wave digit counts, then their prefix over the waves
+  __shared__ uint32_t toff[BINS];      // start of each digit in the reordered tile
+  __shared__ uint32_t s_wave[NW];
+  __shared__ uint32_t lk[TILE], lv[TILE];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   uint32_t lo, hi;
   block_range(pass == 0 ? st->n_keys : st->n_real, G, blockIdx.x, &lo, &hi);
   for (uint32_t d = tid; d < BINS; d += 256u) {
     run[d] = ghist[(size_t)blockIdx.x * BINS + d];
 #pragma unroll
-    for (int q = 0; q < NW; ++q) wc[q][d] = 0xFFFFFFFFu;
+    for (int q = 0; q < NW; ++q) wcnt[q][d] = 0u;
   }
   __syncthreads();
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  uint32_t r = 0;
-  for (uint32_t t0 = lo; t0 < hi; t0 += 256u * RS_ITEMS) {
-    uint32_t k[RS_ITEMS], val[RS_ITEMS];
+  for (uint32_t t0 = lo; t0 < hi; t0 += TILE) {
+    uint32_t k[IT], val[IT], rk[IT];
 #pragma unroll
-    for (int q = 0; q < RS_ITEMS; ++q) {
-      const uint32_t i = t0 + q * 256u + tid;
-      k[q] = i < hi ? kin[i] : 0u;
+    for (int q = 0; q < IT; ++q) {
+      const uint32_t i = t0 + (uint32_t)w * WSPAN + (uint32_t)q * 64u + (uint32_t)lane;
+      k[q] = i < hi ? kin[i] : sent;  // sent: inactive (pass 0 drops the sentinels)
       val[q] = i < hi ? vin[i] : 0u;
     }
+    // ranks within the wave's quarter: count of earlier elements with the same digit
 #pragma unroll
-    for (int q = 0; q < RS_ITEMS; ++q, ++r) {
-      const uint32_t i = t0 + q * 256u + tid;
-      const bool act = i < hi && k[q] != sent;
+    for (int q = 0; q < IT; ++q) {
+      const bool act = k[q] != sent;
       const uint32_t d = (k[q] >> shift) & MASK;
       const uint64_t peers = digit_peers<RB>(act, d);
-      const uint32_t rank = (uint32_t)__popcll(peers & lt);
-      const uint32_t cw = (uint32_t)__popcll(peers);
-      if (act && rank == cw - 1u) wc[w][d] = (r << 7) | cw;
-      __syncthreads();
-      if (act) {
-        uint32_t off = run[d];
-        for (int q2 = 0; q2 < w; ++q2) {
-          const uint32_t x = wc[q2][d];
-          if ((x >> 7) == r) off += x & 0x7Fu;
-        }
-        kout[off + rank] = k[q];
-        vout[off + rank] = val[q];
-      }
-      __syncthreads();
-      for (uint32_t dd = tid; dd < BINS; dd += 256u) {
-        uint32_t tot = 0;
-#pragma unroll
-        for (int q2 = 0; q2 < NW; ++q2) {
-          const uint32_t x = wc[q2][dd];
-          if ((x >> 7) == r) tot += x & 0x7Fu;
-        }
-        run[dd] += tot;
-      }
-      __syncthreads();
+      const uint32_t before = act ? wcnt[w][d] : 0u;
+      rk[q] = before + (uint32_t)__popcll(peers & lt);
+      if (act && (peers >> lane) == 1ull) wcnt[w][d] = before + (uint32_t)__popcll(peers);  // last lane of the group
+      __builtin_amdgcn_wave_barrier();
     }
+    __syncthreads();
+    // per digit: prefix of the wave counts (in place) and the tile count; tile starts by a
+    // block scan over the digits (thread tid owns digits [tid*DPT, tid*DPT + DPT))
+    uint32_t tt[DPT];
+    uint32_t tsum = 0u;
+#pragma unroll
+    for (uint32_t j = 0; j < DPT; ++j) {
+      const uint32_t d = (uint32_t)tid * DPT + j;
+      uint32_t acc = 0u;
+      if (d < BINS) {
+#pragma unroll
+        for (int q = 0; q < NW; ++q) {
+          const uint32_t x = wcnt[q][d];
+          wcnt[q][d] = acc;
+          acc += x;
+        }
+      }
+      tt[j] = acc;
+      tsum += acc;
+    }
+    uint32_t ntile = 0u;
+    uint32_t excl = block_exclusive_scan(tsum, s_wave, &ntile);
+#pragma unroll
+    for (uint32_t j = 0; j < DPT; ++j) {
+      const uint32_t d = (uint32_t)tid * DPT + j;
+      if (d < BINS) toff[d] = excl;
+      excl += tt[j];
+    }
+    __syncthreads();
+    // reorder the tile by digit in LDS
+#pragma unroll
+    for (int q = 0; q < IT; ++q) {
+      if (k[q] != sent) {
+        const uint32_t d = (k[q] >> shift) & MASK;
+        const uint32_t pos = toff[d] + wcnt[w][d] + rk[q];
+        lk[pos] = k[q];
+        lv[pos] = val[q];
+      }
+    }
+    __syncthreads();
+    // write it out in order: consecutive threads, consecutive positions of a digit's run
+    for (uint32_t j = (uint32_t)tid; j < ntile; j += 256u) {
+      const uint32_t kk = lk[j];
+      const uint32_t d = (kk >> shift) & MASK;
+      const uint32_t o = run[d] + (j - toff[d]);
+      kout[o] = kk;
+      vout[o] = lv[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < DPT; ++j) {
+      const uint32_t d = (uint32_t)tid * DPT + j;
+      if (d < BINS) {
+        run[d] += tt[j];
+#pragma unroll
+        for (int q = 0; q < NW; ++q) wcnt[q][d] = 0u;
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -2772,8 +2829,22 @@ size_t forward_lds_bytes(const StepConfig& c) {
 hipError_t launch_forward(const StepConfig& c, StepState* st, hipStream_t s) {
   const size_t lds = forward_lds_bytes(c);
   EP_DISPATCH(c.dims, {
-    if (c.bf16) hipLaunchKernelGGL((k_forward<D, uint16_t>), dim3(c.batch), dim3(256), lds, s, c, st);
-    else hipLaunchKernelGGL((k_forward<D, float>), dim3(c.batch), dim3(256), lds, s, c, st);
+    const bool x2 = c.fwd_x == 2;
+    if (c.bf16) {
+      if constexpr (WideShape<D, uint16_t>::OK) {
+        if (x2) hipLaunchKernelGGL((k_forward<D, uint16_t, 2>), dim3(c.batch), dim3(256), lds, s, c, st);
+        else hipLaunchKernelGGL((k_forward<D, uint16_t, 1>), dim3(c.batch), dim3(256), lds, s, c, st);
+      } else {
+        hipLaunchKernelGGL((k_forward<D, uint16_t, 1>), dim3(c.batch), dim3(256), lds, s, c, st);
+      }
+    } else {
+      if constexpr (WideShape<D, float>::OK) {
+        if (x2) hipLaunchKernelGGL((k_forward<D, float, 2>), dim3(c.batch), dim3(256), lds, s, c, st);
+        else hipLaunchKernelGGL((k_forward<D, float, 1>), dim3(c.batch), dim3(256), lds, s, c, st);
+      } else {
+        hipLaunchKernelGGL((k_forward<D, float, 1>), dim3(c.batch), dim3(256), lds, s, c, st);
+      }
+    }
   });
   return hipGetLastError();
 }
@@ -2829,6 +2900,14 @@ hipError_t launch_sort(const StepConfig& c, StepState* st, hipStream_t s, int* o
 bool seg_vpl_supported(int d, int vpl) {
   if (vpl != 4 && vpl != 8 && vpl != 12 && vpl != 16) return false;
   return d % vpl == 0 && d / vpl <= 64 && (d % 4 == 0);
+}
+
+bool fwd_variant_supported(int d, int bf16, int x) {
+  if (x == 1) return true;
+  if (x != 2) return false;
+  bool ok = false;
+  EP_DISPATCH(d, { ok = bf16 ? (WideShape<D, uint16_t>::OK) : (WideShape<D, float>::OK); });
+  return ok;
 }
 
 bool seg_variant_supported(int d, int vpl, int unroll) {

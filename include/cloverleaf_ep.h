@@ -128,6 +128,10 @@ typedef struct ep_config {
                                /* gathers are held in FP16 with a power-of-two scale */
                                /* per row (2^-11 relative to the row's largest      */
                                /* value); 0: FP32 adjoint rows (default)            */
+  int32_t forward_variant;     /* forward gather rows in flight: 1 (x1), 2 (x2, for */
+                               /* the widths the 16-byte gather serves); 0 = tuned  */
+                               /* with seg_vpl/seg_unroll (default); every variant  */
+                               /* gives identical results                           */
 } ep_config;
 
 typedef struct ep_ctx ep_ctx;
@@ -159,6 +163,8 @@ typedef struct ep_bench_result {
   uint32_t last_pieces;
   uint64_t att_items_sq;       /* attention model, timed steps: sum over attention    */
   uint64_t att_items;          /* blocks of items^2 and of items (FLOP model inputs)  */
+  int32_t fwd_x;               /* forward variant in use: rows in flight x1 or x2     */
+  int32_t reserved;
 } ep_bench_result;
 
 /* Fills the reference defaults (EmbeddingPropagator::new, src/lib.rs:1262-1274). */

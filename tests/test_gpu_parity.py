@@ -20,7 +20,7 @@ GRAD_TOL = 1e-5
 
 def make_ctx(prob, S, K, gamma=0.5, B=32, grad_mode=N.EP_GRAD_FIRST_LEAF, scatter=N.EP_SCATTER_SORT,
              precision=N.EP_PREC_FP32, lw=N.EP_LW_NONE, lw_exp=0.0, use_graph=0, seed=20222022, seg_tile=(0, 0),
-             concurrent=1, noise=0.0):
+             concurrent=1, noise=0.0, forward_variant=0):
     cfg = N.default_config()
     cfg.dims, cfg.batch_size, cfg.negatives, cfg.gamma = prob.D, B, K, gamma
     cfg.max_nodes_kind, cfg.max_nodes = N.EP_SAMPLE_FIXED, S
@@ -31,6 +31,7 @@ def make_ctx(prob, S, K, gamma=0.5, B=32, grad_mode=N.EP_GRAD_FIRST_LEAF, scatte
     cfg.seg_vpl, cfg.seg_unroll = seg_tile
     cfg.concurrent_batches = concurrent
     cfg.noise = noise
+    cfg.forward_variant = forward_variant
     cfg.valid_pct = 0.0
     ctx = N.Context(cfg)
     ctx.set_graph(prob.indptr, prob.indices)
@@ -309,6 +310,29 @@ def test_seg_tile_variants_bit_identical(precision):
         ctx.bench_steps(warmup=0 if tile == (0, 0) else 24, steps=14, instrument=False)
         t = ctx.get_table_bf16() if precision == N.EP_PREC_BF16 else ctx.get_table()
         outs.append((t, *ctx.get_adam()))
+        ctx.close()
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision,D", [(N.EP_PREC_FP32, 128), (N.EP_PREC_BF16, 256), (N.EP_PREC_BF16_MASTER, 64)])
+def test_forward_variants_bit_identical(precision, D):
+    # the forward's rows-in-flight variants add each construction's rows in the same
+    # order: equal losses, tables and moments bit for bit; 0 (tuned) included
+    prob = make_problem(seed=43, n=400, V=300, D=D, avg_deg=30, zipf=1.3)
+    outs = []
+    for fv in (1, 2, 0):
+        ctx = make_ctx(prob, 8, 5, 0.5, B=64, precision=precision, forward_variant=fv)
+        ctx.set_split(np.arange(400, dtype=np.uint32), np.zeros(0, dtype=np.uint32))
+        ctx.bench_steps(warmup=0 if fv == 0 else 24, steps=14, instrument=False)
+        t = ctx.get_table_bf16() if precision != N.EP_PREC_FP32 else ctx.get_table()
+        anchors = np.arange(0, 400, 7, dtype=np.uint32)
+        cnt, nbr, neg = ctx.sample_batch(anchors, 1)
+        out = ctx.step_explicit(anchors, cnt, nbr, neg, 1, 0.01, apply_update=False, want_cons=True)
+        G = np.zeros((prob.V, D), dtype=np.float32)  # records come in scheduling order
+        G[out["feat"]] = out["grad"]
+        outs.append((t, *ctx.get_adam(), out["loss"], out["cons"], G))
         ctx.close()
     for o in outs[1:]:
         for a, b in zip(outs[0], o):
