@@ -2012,7 +2012,9 @@ __global__ __launch_bounds__(256) void k_seg_plan(StepConfig c, StepState* st, c
 // columns strided by L), G = 64/L rows per wave, U adjoint rows in flight per group.
 // Every (VPL, U) adds each run's rows in entry order: identical bits, so the tuner picks
 // the fastest.
-template <int D, int VPL, int U, bool H>
+// NZ: the gradient-noise path (noise > 0), built for one tile per width only: its
+// Philox and transcendental code would raise every tile's VGPRs (and cut its waves).
+template <int D, int VPL, int U, bool H, bool NZ = false>
 __global__ __launch_bounds__(256) void k_seg_update(StepConfig c, StepState* st, const uint32_t* __restrict__ Ss) {
   static_assert(D % VPL == 0 && VPL % 4 == 0 && D / VPL <= 64, "float4 lane slices");
   constexpr int L = D / VPL;      // lanes per row
@@ -2054,11 +2056,15 @@ __global__ __launch_bounds__(256) void k_seg_update(StepConfig c, StepState* st,
       return (gact && jj < nrec) ? recs[jj] : make_uint4(0u, 0u, 0u, 0u);
     };
     // the row's slots, spread over the group's lanes (entry e at lane e % L, register e / L)
+    // (raw sorted values, batch bits included), loaded unconditionally at an in-bounds
+    // index (entry 0 of the record past its length) and masked where used: a load inside
+    // a branch whose value is modified at the join makes the compiler wait there for
+    // every load in flight, the row's theta/m/v included, which undoes the pipeline.
     auto load_slots = [&](const uint4& r, uint32_t (&out)[NS]) {
   #pragma unroll
       for (int i = 0; i < NS; ++i) {
         const uint32_t e = (uint32_t)(lg + i * L);
-        out[i] = (e < (r.w & 63u)) ? (Ss[r.x + e] & kSlotMask) : 0u;
+        out[i] = Ss[r.x + (e < (r.w & 63u) ? e : 0u)];
       }
     };
     uint4 rec_next = load_rec((uint32_t)grp);
@@ -2079,26 +2085,24 @@ __global__ __launch_bounds__(256) void k_seg_update(StepConfig c, StepState* st,
       for (int i = 0; i < NS; ++i) sl[i] = sl_next[i];
       const uint32_t f = rec.y, rmask = rec.z;
       const uint32_t len = rec.w & 63u;
+      // theta/m/v, the same loads in every precision (row 0 stands in for an empty record
+      // slot and is never stored; grad_out mode loads and ignores them): the sources are
+      // chosen by wave-uniform selects, and the load a precision does not need reads the
+      // m row again from cache, so no branch join waits for these loads.
       float4 m4[NQ], v4[NQ], t4[NQ];
+      uint2 tb[NQ];  // BF16 table without master: the raw row, widened after the first run's sum
+      const bool widen = c.bf16 && !c.master;
       const size_t roff = (size_t)f * D;
       const size_t ooff = (size_t)f * c.ostride;  // optimizer record [master |] m | v
+      const float* tsrc = c.master ? c.master + ooff : (c.bf16 ? c.mom + ooff : (const float*)c.table + roff);
+      const uint16_t* bsrc = widen ? (const uint16_t*)c.table + roff : (const uint16_t*)(c.mom + ooff);
   #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         const uint32_t col = (uint32_t)(q * L + lg) * 4u;
-        m4[q] = v4[q] = t4[q] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (has && !c.grad_out) {
-          m4[q] = *reinterpret_cast<const float4*>(c.mom + ooff + col);
-          v4[q] = *reinterpret_cast<const float4*>(c.var + ooff + col);
-          if (c.master) {
-            t4[q] = *reinterpret_cast<const float4*>(c.master + ooff + col);
-          } else if (c.bf16) {
-            const uint2 b = *reinterpret_cast<const uint2*>((const uint16_t*)c.table + roff + col);
-            t4[q] = make_float4(__uint_as_float(b.x << 16), __uint_as_float(b.x & 0xffff0000u),
-                                __uint_as_float(b.y << 16), __uint_as_float(b.y & 0xffff0000u));
-          } else {
-            t4[q] = *reinterpret_cast<const float4*>((const float*)c.table + roff + col);
-          }
-        }
+        m4[q] = *reinterpret_cast<const float4*>(c.mom + ooff + col);
+        v4[q] = *reinterpret_cast<const float4*>(c.var + ooff + col);
+        t4[q] = *reinterpret_cast<const float4*>(tsrc + col);
+        tb[q] = *reinterpret_cast<const uint2*>(bsrc + (widen ? col : 2u * col));
       }
       rec_next = rec_after;
       load_slots(rec_next, sl_next);
@@ -2126,6 +2130,8 @@ __global__ __launch_bounds__(256) void k_seg_update(StepConfig c, StepState* st,
   #pragma unroll
         for (int q = 0; q < NQ; ++q) acc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
         for (uint32_t u0 = 0; u0 < maxlen; u0 += U) {
+          // entries past the run skip their loads (a wave-uniform branch when G == 1: most
+          // runs of short rows are 1-3 entries long, whose loads all issue before a wait)
           float4 r[U][NQ];
   #pragma unroll
           for (int u = 0; u < U; ++u) {
@@ -2141,6 +2147,7 @@ __global__ __launch_bounds__(256) void k_seg_update(StepConfig c, StepState* st,
               for (int i = 1; i < NS; ++i) pick = (ei == (uint32_t)i) ? sl[i] : pick;
               slot = __shfl(pick, grp * L + (int)(e % (uint32_t)L), EP_WAVE);
             }
+            slot &= kSlotMask;
             if constexpr (H) {
               const float sc = valid ? c.ascale[slot] : 0.f;
   #pragma unroll
@@ -2167,7 +2174,18 @@ __global__ __launch_bounds__(256) void k_seg_update(StepConfig c, StepState* st,
             }
           }
         }
-        if (noisy && hr) {  // mod.rs:238-247: noise on the batch's merged gradient row
+        if (rq == 0u && widen) {  // the raw BF16 row has arrived with the first run's rows
+  #pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            uint32_t bx = tb[q].x, by = tb[q].y;
+            // an empty register fence: keeps the widening (and the wait for the row) here,
+            // after the first run's adjoint loads, instead of hoisted in front of them
+            asm volatile("" : "+v"(bx), "+v"(by));
+            t4[q] = make_float4(__uint_as_float(bx << 16), __uint_as_float(bx & 0xffff0000u),
+                                __uint_as_float(by << 16), __uint_as_float(by & 0xffff0000u));
+          }
+        }
+        if (NZ && noisy && hr) {  // mod.rs:238-247: noise on the batch's merged gradient row
           const float sd = s_sd[bq];
   #pragma unroll
           for (int q = 0; q < NQ; ++q) {
@@ -2914,7 +2932,7 @@ bool seg_variant_supported(int d, int vpl, int unroll) {
   if (!seg_vpl_supported(d, vpl)) return false;
   if (vpl == 12) return unroll == 2 || unroll == 4;
   const int nq = vpl / 4;
-  return unroll == 8 / nq || unroll == 16 / nq;
+  return unroll == 8 / nq || unroll == 16 / nq || (nq <= 2 && unroll == 4 / nq);
 }
 
 template <int D, int VPL, int U, bool H>
@@ -2930,10 +2948,18 @@ static bool try_seg_update(const StepConfig& c, StepState* st, int buf, hipStrea
 
 template <int D, bool H>
 static void launch_seg_update_variant(const StepConfig& c, StepState* st, int buf, hipStream_t s, int grid) {
+  if (c.noise > 1e-9f) {  // the noise path: one tile (every tile gives the same results)
+    if constexpr (D % 4 == 0 && D / 4 <= 64)
+      hipLaunchKernelGGL((k_seg_update<D, 4, 8, H, true>), dim3(grid), dim3(256), 0, s, c, st, c.slots[buf]);
+    else
+      hipLaunchKernelGGL((k_seg_update<D, 16, 2, H, true>), dim3(grid), dim3(256), 0, s, c, st, c.slots[buf]);
+    return;
+  }
   if (try_seg_update<D, 16, 2, H>(c, st, buf, s, grid) || try_seg_update<D, 16, 4, H>(c, st, buf, s, grid) ||
       try_seg_update<D, 12, 2, H>(c, st, buf, s, grid) || try_seg_update<D, 12, 4, H>(c, st, buf, s, grid) ||
       try_seg_update<D, 8, 4, H>(c, st, buf, s, grid) || try_seg_update<D, 8, 8, H>(c, st, buf, s, grid) ||
-      try_seg_update<D, 4, 16, H>(c, st, buf, s, grid) || try_seg_update<D, 4, 8, H>(c, st, buf, s, grid))
+      try_seg_update<D, 4, 16, H>(c, st, buf, s, grid) || try_seg_update<D, 4, 8, H>(c, st, buf, s, grid) ||
+      try_seg_update<D, 4, 4, H>(c, st, buf, s, grid) || try_seg_update<D, 8, 2, H>(c, st, buf, s, grid))
     return;
   // a width no variant divides: the widest built form
   if constexpr (D % 16 == 0 && D / 16 <= 64)
