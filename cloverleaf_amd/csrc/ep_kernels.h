@@ -171,8 +171,9 @@ struct StepConfig {
   // count scount[r], one record {first entry, row id, run-start mask (bit i: entry i
   // starts a (row, batch) run), len | batch of run j << (6 + 3j)}, and its medium rows
   // (more entries, inside the 128-entry window, at most 2 per chunk) at mrec[r*cpr*2 ...]
-  // with count mcount[r], {first entry, len}.  k_seg_update's waves take regions from
-  // the StepState queue reg_next, so its blocks need not all be resident at once.
+  // with count mcount[r], {first entry, len}.  k_seg_update's wave w takes region w and,
+  // when there are more regions than waves, further ones from the StepState queue
+  // reg_next, so its blocks need not all be resident at once.
   uint4* srec;             // [srec_cap]
   uint32_t srec_cap;
   uint2* mrec;             // [mrec_cap]

@@ -2026,7 +2026,7 @@ __global__ __launch_bounds__(256) void k_seg_update(StepConfig c, StepState* st,
   const int grp = lane / L, lg = lane % L;
   const bool gact = grp < G;
   const uint64_t gmask = (L == 64) ? ~0ull : (((1ull << L) - 1ull) << ((grp * L) & 63));
-  // plan regions holding chunks, taken one at a time from the queue st->reg_next
+  // plan regions holding chunks
   const uint32_t n = st->n_keys <= c.key_cap ? st->n_real : 0u;
   const uint32_t nchunks = (n + 63u) / 64u;
   const uint32_t cpr = (nchunks + c.n_regions - 1u) / c.n_regions;
@@ -2040,10 +2040,11 @@ __global__ __launch_bounds__(256) void k_seg_update(StepConfig c, StepState* st,
   __syncthreads();
   const bool noisy = c.noise > 1e-9f;
   uint32_t done = 0;
+  // wave gw takes region gw, then (only when there are more regions than waves) regions
+  // waves, waves + 1, ... from the queue: small steps make no atomics at all
+  const uint32_t nwv = gridDim.x * NW;
+  uint32_t reg = blockIdx.x * NW + (uint32_t)(tid >> 6);
   for (;;) {
-    uint32_t reg = 0u;
-    if (lane == 0) reg = atomicAdd(&st->reg_next, 1u);
-    reg = (uint32_t)__shfl((int)reg, 0, EP_WAVE);
     if (reg >= nreg) break;
     const uint32_t nrec = c.scount[reg];
     const uint4* recs = c.srec + (size_t)reg * cpr * 64u;
@@ -2245,6 +2246,10 @@ __global__ __launch_bounds__(256) void k_seg_update(StepConfig c, StepState* st,
         }
       }
     }
+    if (nreg <= nwv) break;
+    uint32_t nx = 0u;
+    if (lane == 0) nx = atomicAdd(&st->reg_next, 1u);
+    reg = nwv + (uint32_t)__shfl((int)nx, 0, EP_WAVE);
   }
   done = wave_sum_u32(done);
   if (lane == 0 && done) {

@@ -2,6 +2,11 @@
 
 python -m cloverleaf_amd.profiling stats <results.db>
     per-kernel dispatch count and duration statistics (markdown table)
+python -m cloverleaf_amd.profiling sq <run.db> [<run.db> ...]
+    per-kernel shader-sequencer counters (SQ_*) averaged per dispatch, merged over
+    several --pmc runs: the share of wave time waiting on memory / LDS counters
+    (SQ_WAIT_ANY), waiting to issue (SQ_WAIT_INST_ANY) and issuing (SQ_ACTIVE_INST_*),
+    and instructions per wave (SQ_INSTS_* / SQ_WAVES)
 python -m cloverleaf_amd.profiling traffic <fetch.db> <write.db> --probe-mib N --out file.json --form F
     calibrated HBM bytes per dispatch of every kernel, from one FETCH_SIZE and one
     WRITE_SIZE run of the same command; the counters of the copy probe kernel (an
@@ -81,6 +86,23 @@ def traffic(fetch_db: str, write_db: str, probe_mib: int):
     return out, dict(probe_kernel=short_name(pk), fetch_factor=f_fac, write_factor=w_fac, probe_mib=probe_mib)
 
 
+def sq_summary(dbs):
+    """Per kernel: SQ counters averaged per dispatch, merged over the given runs."""
+    acc = defaultdict(lambda: defaultdict(list))
+    for path in dbs:
+        db = sqlite3.connect(path)
+        rows = db.execute("select dispatch_id, kernel_name, counter_name, sum(value) from counters_collection "
+                          "where counter_name like 'SQ_%' group by dispatch_id, counter_name").fetchall()
+        for _, kn, cn, v in rows:
+            acc[short_name(kn)][cn].append(v)
+    out = []
+    for k, cs in acc.items():
+        d = {cn: sum(v) / len(v) for cn, v in cs.items()}
+        out.append((k, d))
+    out.sort(key=lambda kd: -(kd[1].get("SQ_WAIT_ANY", 0) + kd[1].get("SQ_ACTIVE_INST_ANY", 0)))
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -89,6 +111,9 @@ def main(argv=None):
     s1.add_argument("--top", type=int, default=25)
     s1.add_argument("--split-at", default="", help="start a new table at each dispatch of this kernel")
     s1.add_argument("--labels", default="", help="comma-separated titles of the split tables")
+    s3 = sub.add_parser("sq")
+    s3.add_argument("dbs", nargs="+")
+    s3.add_argument("--top", type=int, default=12)
     s2 = sub.add_parser("traffic")
     s2.add_argument("fetch_db")
     s2.add_argument("write_db")
@@ -108,6 +133,20 @@ def main(argv=None):
             for r in rows[:a.top]:
                 print(f"| `{short_name(r['name'])}` | {r['count']} | {r['avg_us']:.2f} | {r['min_us']:.2f} | "
                       f"{r['max_us']:.2f} | {r['total_us']:.0f} |")
+        return 0
+    if a.cmd == "sq":
+        print("| kernel | waves | wait on memory/LDS | wait to issue | issuing | VALU | LDS | VALU / VMEM rd / VMEM wr / LDS instr per wave |")
+        print("|---|---:|---:|---:|---:|---:|---:|---|")
+        for k, d in sq_summary(a.dbs)[:a.top]:
+            t = d.get("SQ_WAIT_ANY", 0) + d.get("SQ_WAIT_INST_ANY", 0) + d.get("SQ_ACTIVE_INST_ANY", 0)
+            w = d.get("SQ_WAVES", 0)
+            if not t or not w:
+                continue
+            pct = lambda x: f"{100 * d.get(x, 0) / t:.0f}%"
+            per = " / ".join(f"{d.get(x, 0) / w:.0f}" for x in ("SQ_INSTS_VALU", "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR",
+                                                               "SQ_INSTS_LDS"))
+            print(f"| `{k}` | {w:.0f} | {pct('SQ_WAIT_ANY')} | {pct('SQ_WAIT_INST_ANY')} | {pct('SQ_ACTIVE_INST_ANY')} | "
+                  f"{pct('SQ_ACTIVE_INST_VALU')} | {pct('SQ_ACTIVE_INST_LDS')} | {per} |")
         return 0
     ker, cal = traffic(a.fetch_db, a.write_db, a.probe_mib)
     try:
