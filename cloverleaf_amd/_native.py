@@ -139,6 +139,8 @@ SIGNATURES = {
     "ep_embedding_file_sizes": (c_int, [c_void_p, _u64p, _u64p, _u64p]),
     "ep_embedding_file_copy": (c_int, [c_void_p, _f32p, c_char_p, _u64p]),
     "ep_embedding_file_free": (None, [c_void_p]),
+    "ep_embeddings_write_tsv": (c_int, [c_char_p, c_int, c_uint64, c_uint64, _f32p, c_char_p, _u64p]),
+    "ep_format_f32": (c_int, [c_float, c_char_p]),
     "ep_sample_weights": (c_int, [c_void_p, c_uint32, _f32p]),
     "ep_set_explicit_weights": (c_int, [c_void_p, c_uint32, _f32p]),
     "ep_set_features": (c_int, [c_void_p, c_uint64, c_uint32, _u64p, _u32p]),
@@ -560,6 +562,33 @@ def load_embeddings_tsv(path: str, filter_types=None, skip_rows: int = 0):
         return _split_names(blob.raw[:nb.value], off, n.value), table
     finally:
         lib.ep_embedding_file_free(h)
+
+
+def write_embeddings_tsv(path: str, names, table: np.ndarray, comp_level=None):
+    """The native embedding file writer (NodeEmbeddings.save): names[i] = (type, name) of
+    row i of the FP32 table."""
+    lib = load()
+    table = np.ascontiguousarray(table, dtype=np.float32)
+    n = table.shape[0]
+    d = table.shape[1] if table.ndim == 2 else 0
+    parts = [f"{t}\t{nm}".encode("utf-8", errors="surrogateescape") for t, nm in names]
+    if len(parts) != n:
+        raise ValueError("one (type, name) per table row")
+    off = np.zeros(n + 1, dtype=np.uint64)
+    if n:
+        np.cumsum(np.fromiter((len(x) for x in parts), dtype=np.uint64, count=n), out=off[1:])
+    rc = lib.ep_embeddings_write_tsv(os.fsencode(path), -1 if comp_level is None else int(comp_level), n, d,
+                                     ptr(table, c_float) if table.size else None, b"".join(parts),
+                                     ptr(off, c_uint64))
+    if rc != 0:
+        raise ValueError(lib.ep_graph_file_error().decode(errors="replace"))
+
+
+def format_f32(x: float) -> str:
+    """One f32 as the embedding writer prints it (ryu's layout)."""
+    buf = ctypes.create_string_buffer(32)
+    n = load().ep_format_f32(float(np.float32(x)), buf)
+    return buf.raw[:n].decode()
 
 
 def philox4x32(ctr, key):

@@ -449,13 +449,10 @@ class NodeEmbeddings:
 
     def save(self, path: str, comp_level: Optional[int] = None):
         """NodeEmbeddings.save (src/lib.rs:2828-2840, EmbeddingWriter io.rs:24-78): one line per
-        node, node_type<TAB>name<TAB>[v1,v2,...] with the shortest round-trip f32 digits;
-        `.gz` paths are gzipped."""
-        with _open_text_write(path, comp_level) as fh:
-            for i in range(self.len()):
-                t, n = self.vocab_.name(i)
-                vals = ",".join(str(x) for x in self.embeddings[i])  # np.float32: shortest digits
-                fh.write(f"{t}\t{n}\t[{vals}]\n")
+        node, node_type<TAB>name<TAB>[v1,v2,...], each value as ryu prints an f32; `.gz`
+        paths are gzipped (level comp_level, default 1).  Written by the library's native
+        writer (ep_embeddings_write_tsv)."""
+        N.write_embeddings_tsv(path, [self.vocab_.name(i) for i in range(self.len())], self.embeddings, comp_level)
 
     @staticmethod
     def load(path: str, distance: Optional[str] = None, filter_type=None, chunk_size: Optional[int] = None,

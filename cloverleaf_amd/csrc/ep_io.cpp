@@ -68,6 +68,72 @@ struct LineReader {
   }
 };
 
+// ryu::Buffer::format for an f32 (EmbeddingWriter::format_embedding, io.rs:69-76).  The
+// digits are the shortest decimal that reads back to the same f32, the one closest to it
+// when several are as short (std::to_chars gives exactly these).  With d the digits
+// (n of them) and the value d x 10^k, kk = n + k, the layout is:
+//   k >= 0 and kk <= 13   the integer and ".0"               1e12   -> 1000000000000.0
+//   0 < kk <= 13          a point after kk digits            123.45
+//   -6 < kk <= 0          "0.", -kk zeros, the digits        1e-6   -> 0.000001
+//   otherwise             d[.ddd] "e" exponent kk - 1        1e13, 1e-7, 3.4028235e38
+// zero as "0.0" (sign kept), NaN as "NaN", infinities as "inf" / "-inf".  Writes at most
+// 17 characters to out (no terminator) and returns their count.
+size_t format_f32_ryu(float x, char* out) {
+  if (std::isnan(x)) { std::memcpy(out, "NaN", 3); return 3; }
+  if (std::isinf(x)) {
+    if (x < 0.f) { std::memcpy(out, "-inf", 4); return 4; }
+    std::memcpy(out, "inf", 3);
+    return 3;
+  }
+  char* p = out;
+  if (std::signbit(x)) *p++ = '-';
+  const float ax = std::fabs(x);
+  if (ax == 0.f) {
+    std::memcpy(p, "0.0", 3);
+    return (size_t)(p - out) + 3;
+  }
+  char sci[32];
+  const auto r = std::to_chars(sci, sci + sizeof(sci), ax, std::chars_format::scientific);  // d[.ddd]e±XX
+  char dig[16];
+  int nd = 0;
+  const char* q = sci;
+  for (; q < r.ptr && *q != 'e'; ++q)
+    if (*q != '.') dig[nd++] = *q;
+  int e10 = 0;
+  std::from_chars(q + 1 + (q[1] == '+' ? 1 : 0), r.ptr, e10);
+  const int k = e10 - (nd - 1), kk = e10 + 1;
+  if (k >= 0 && kk <= 13) {
+    std::memcpy(p, dig, nd);
+    p += nd;
+    for (int i = 0; i < k; ++i) *p++ = '0';
+    *p++ = '.';
+    *p++ = '0';
+  } else if (kk > 0 && kk <= 13) {
+    std::memcpy(p, dig, kk);
+    p += kk;
+    *p++ = '.';
+    std::memcpy(p, dig + kk, nd - kk);
+    p += nd - kk;
+  } else if (kk > -6 && kk <= 0) {
+    *p++ = '0';
+    *p++ = '.';
+    for (int i = 0; i < -kk; ++i) *p++ = '0';
+    std::memcpy(p, dig, nd);
+    p += nd;
+  } else {
+    *p++ = dig[0];
+    if (nd > 1) {
+      *p++ = '.';
+      std::memcpy(p, dig + 1, nd - 1);
+      p += nd - 1;
+    }
+    *p++ = 'e';
+    const auto w = std::to_chars(p, p + 8, kk - 1);
+    p = w.ptr;
+  }
+  return (size_t)(p - out);
+}
+
 // str::parse::<f32> (Rust): an optional sign, decimal or exponent notation, inf / infinity /
 // nan in any case; nothing else on the field.
 bool parse_f32(std::string_view s, float* out) {
@@ -564,5 +630,50 @@ int ep_embedding_file_copy(const ep_embedding_file* f, float* table, char* name_
 }
 
 void ep_embedding_file_free(ep_embedding_file* f) { delete f; }
+
+int ep_embeddings_write_tsv(const char* path, int comp_level, uint64_t n_rows, uint64_t dims, const float* table,
+                            const char* name_blob, const uint64_t* name_off) {
+  if (!path || (n_rows && (!table || !name_blob || !name_off))) return io_fail(EP_EINVAL, "NULL argument");
+  if (comp_level > 9) return io_fail(EP_EINVAL, "comp_level must be in [0, 9] (or negative for the default)");
+  const size_t pl = std::strlen(path);
+  const bool gzip = pl >= 3 && std::strcmp(path + pl - 3, ".gz") == 0;
+  FILE* f = nullptr;
+  gzFile gz = nullptr;
+  if (gzip) {
+    // GzEncoder with Compression::new(level), Compression::fast() (1) by default
+    const std::string mode = "wb" + std::to_string(comp_level < 0 ? 1 : comp_level);
+    gz = gzopen(path, mode.c_str());
+    if (gz) gzbuffer(gz, 1 << 20);
+  } else {
+    f = std::fopen(path, "wb");
+  }
+  if (!f && !gz) return io_fail(EP_EINVAL, std::string("cannot open ") + path + " for writing");
+  std::string line;
+  char num[48];
+  int rc = EP_OK;
+  for (uint64_t i = 0; i < n_rows && rc == EP_OK; ++i) {
+    line.assign(name_blob + name_off[i], name_blob + name_off[i + 1]);  // "node_type\tname"
+    line += "\t[";
+    const float* row = table + i * dims;
+    for (uint64_t j = 0; j < dims; ++j) {
+      if (j) line += ',';
+      line.append(num, format_f32_ryu(row[j], num));
+    }
+    line += "]\n";
+    const bool ok = gz ? gzwrite(gz, line.data(), (unsigned)line.size()) == (int)line.size()
+                       : std::fwrite(line.data(), 1, line.size(), f) == line.size();
+    if (!ok) rc = io_fail(EP_EINVAL, std::string("write failed: ") + path);
+  }
+  if (gz && gzclose(gz) != Z_OK && rc == EP_OK) rc = io_fail(EP_EINVAL, std::string("gzip close failed: ") + path);
+  if (f && std::fclose(f) != 0 && rc == EP_OK) rc = io_fail(EP_EINVAL, std::string("close failed: ") + path);
+  return rc;
+}
+
+int ep_format_f32(float x, char* out) {
+  if (!out) return io_fail(EP_EINVAL, "NULL argument");
+  const size_t n = format_f32_ryu(x, out);
+  out[n] = '\0';
+  return (int)n;
+}
 
 }  // extern "C"

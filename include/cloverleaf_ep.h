@@ -359,6 +359,20 @@ int ep_embedding_file_sizes(const ep_embedding_file* f, uint64_t* n_rows, uint64
 int ep_embedding_file_copy(const ep_embedding_file* f, float* table, char* name_blob, uint64_t* name_off);
 void ep_embedding_file_free(ep_embedding_file* f);
 
+/* Embedding file writer (host): NodeEmbeddings.save (src/lib.rs:2828-2840) through
+ * EmbeddingWriter::stream (src/io.rs:24-78) and open_file_for_writing (io.rs:212-222).
+ * One line per row i: the bytes name_blob[name_off[i] .. name_off[i+1]) ("node_type\tname"),
+ * a tab and "[v1,v2,...]" with each value as ryu::Buffer::format prints an f32: the
+ * shortest digits that read back to the same value, "0.000001" down to 1e-6, "1e-7" and
+ * "1e13" in exponent form, integers up to 13 digits as "123.0", "NaN", "inf".  gzip for a
+ * ".gz" path at comp_level (negative: level 1, Compression::fast()).  Errors: a negative
+ * EP_E* code, message in ep_graph_file_error().                                       */
+int ep_embeddings_write_tsv(const char* path, int comp_level, uint64_t n_rows, uint64_t dims, const float* table,
+                            const char* name_blob, const uint64_t* name_off);
+/* Test hook: one value as the writer prints it into out (at least 24 bytes), terminated;
+ * returns its length.                                                                   */
+int ep_format_f32(float x, char* out);
+
 /* Philox4x32-10 on the host (the device RNG), for known-answer tests.           */
 int ep_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 

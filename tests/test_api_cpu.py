@@ -84,7 +84,7 @@ def test_embeddings_save_load_round_trip(tmp_path):
         back = C.NodeEmbeddings.load(str(tmp_path / name))
         assert back.vocab_.names == v.names
         assert np.array_equal(back.embeddings, emb)  # shortest round-trip digits are exact
-    assert (tmp_path / "e.tsv").read_text().splitlines()[0] == "node\t0\t[0.1,-2.5,1e-07]"
+    assert (tmp_path / "e.tsv").read_text().splitlines()[0] == "node\t0\t[0.1,-2.5,1e-7]"  # ryu's layout
     only = C.NodeEmbeddings.load(str(tmp_path / "e.tsv"), filter_type="feat")
     assert only.vocab_.names == [("feat", "1"), ("feat", "3")]
     assert np.array_equal(only.embeddings, emb[[1, 3]])
