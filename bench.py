@@ -276,7 +276,7 @@ def main():
                     help="shrink only the graph (nodes, edges); vocabulary and model shape unchanged")
     ap.add_argument("--forms", default="fp32-plain,bf16-master-batched,bf16-batched")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--scatters", default="sort", help="scatter paths tried for the batched forms (sort,atomic)")
+    ap.add_argument("--scatters", default="sort", help="scatter paths tried per form (sort,atomic; atomic runs one batch per step)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--concurrent", type=int, default=0,
                     help="reference batches per step in the batched forms (the plain forms use 1); default 8, "
@@ -326,9 +326,9 @@ def main():
     results = {}
     for form in args.forms.split(","):
         best = None
-        scatters = ["sort"] if form.endswith("-plain") else [s for s in args.scatters.split(",")]
-        for sc in scatters:
-            if sc == "atomic" and args.concurrent > 1:
+        plain = form.endswith("-plain")
+        for sc in args.scatters.split(","):
+            if sc == "atomic" and not plain and args.concurrent > 1:
                 continue  # the atomic path runs one batch per step
             r = run_form(w, form, args.steps, args.warmup, scatter=sc, concurrent=args.concurrent,
                          graph=not args.no_graph, fp16_adjoints=bool(args.fp16_adjoints))

@@ -638,19 +638,30 @@ __global__ __launch_bounds__(1024) void k_scan(StepConfig c, StepState* st) {
   if (tid == 0) { s_carry = 0u; s_carry2 = 0u; s_carry3 = 0u; s_nodes = 0ull; }
   __syncthreads();
   uint64_t nodes = 0;
-  for (uint32_t base = 0; base < c.batch; base += 1024) {
-    const uint32_t i = base + tid;
-    const uint32_t x = i < c.batch ? c.rcount[i] : 0u;
-    uint32_t y = 0, z = 0;
-    if (i < c.batch && i < st->batch_n) {
-      const uint32_t cn = c.nbr_cnt[i];
-      if (cn > 0u) {
-        nodes += cn + c.K;
-        y = 1u + cn + c.K;
-        z = cn;
+  // rounds of 1024 x IT anchors: thread t takes IT consecutive ones (a running sum in
+  // registers), then one block scan of the thread sums
+  constexpr uint32_t IT = 8;
+  for (uint32_t base = 0; base < c.batch; base += 1024u * IT) {
+    uint32_t x[IT], y[IT], z[IT];
+    uint32_t sx = 0, sy = 0, sz = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < IT; ++q) {
+      const uint32_t i = base + (uint32_t)tid * IT + q;
+      x[q] = i < c.batch ? c.rcount[i] : 0u;
+      y[q] = z[q] = 0u;
+      if (i < c.batch && i < st->batch_n) {
+        const uint32_t cn = c.nbr_cnt[i];
+        if (cn > 0u) {
+          nodes += cn + c.K;
+          y[q] = 1u + cn + c.K;
+          z[q] = cn;
+        }
       }
+      sx += x[q];
+      sy += y[q];
+      sz += z[q];
     }
-    uint32_t inc = x, inc2 = y, inc3 = z;
+    uint32_t inc = sx, inc2 = sy, inc3 = sz;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
       const uint32_t t1 = __shfl_up(inc, off, EP_WAVE);
@@ -667,10 +678,18 @@ __global__ __launch_bounds__(1024) void k_scan(StepConfig c, StepState* st) {
       tot2 += s_w2[q];
       tot3 += s_w3[q];
     }
-    if (i < c.batch) {
-      c.keyoff[i] = pre + inc - x;
-      if (att) c.blkoff[i] = pre2 + inc2 - y;
-      if (nrows) c.nbroff[i] = pre3 + inc3 - z;
+    uint32_t o1 = pre + inc - sx, o2 = pre2 + inc2 - sy, o3 = pre3 + inc3 - sz;  // this thread's first offsets
+#pragma unroll
+    for (uint32_t q = 0; q < IT; ++q) {
+      const uint32_t i = base + (uint32_t)tid * IT + q;
+      if (i < c.batch) {
+        c.keyoff[i] = o1;
+        if (att) c.blkoff[i] = o2;
+        if (nrows) c.nbroff[i] = o3;
+      }
+      o1 += x[q];
+      o2 += y[q];
+      o3 += z[q];
     }
     __syncthreads();
     if (tid == 0) { s_carry += tot; s_carry2 += tot2; s_carry3 += tot3; }

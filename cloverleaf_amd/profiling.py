@@ -7,6 +7,8 @@ python -m cloverleaf_amd.profiling sq <run.db> [<run.db> ...]
     several --pmc runs: the share of wave time waiting on memory / LDS counters
     (SQ_WAIT_ANY), waiting to issue (SQ_WAIT_INST_ANY) and issuing (SQ_ACTIVE_INST_*),
     and instructions per wave (SQ_INSTS_* / SQ_WAVES)
+python -m cloverleaf_amd.profiling tcc <run.db>
+    per-kernel L2 hits and misses (TCC_HIT_sum, TCC_MISS_sum) averaged per dispatch
 python -m cloverleaf_amd.profiling traffic <fetch.db> <write.db> --probe-mib N --out file.json --form F
     calibrated HBM bytes per dispatch of every kernel, from one FETCH_SIZE and one
     WRITE_SIZE run of the same command; the counters of the copy probe kernel (an
@@ -111,6 +113,9 @@ def main(argv=None):
     s1.add_argument("--top", type=int, default=25)
     s1.add_argument("--split-at", default="", help="start a new table at each dispatch of this kernel")
     s1.add_argument("--labels", default="", help="comma-separated titles of the split tables")
+    s4 = sub.add_parser("tcc")
+    s4.add_argument("db")
+    s4.add_argument("--top", type=int, default=16)
     s3 = sub.add_parser("sq")
     s3.add_argument("dbs", nargs="+")
     s3.add_argument("--top", type=int, default=12)
@@ -133,6 +138,20 @@ def main(argv=None):
             for r in rows[:a.top]:
                 print(f"| `{short_name(r['name'])}` | {r['count']} | {r['avg_us']:.2f} | {r['min_us']:.2f} | "
                       f"{r['max_us']:.2f} | {r['total_us']:.0f} |")
+        return 0
+    if a.cmd == "tcc":
+        hits = counter_per_kernel(a.db, "TCC_HIT_sum")
+        miss = counter_per_kernel(a.db, "TCC_MISS_sum")
+        rows = []
+        for k in hits:
+            h, m = hits[k][0], miss.get(k, (0.0, 0, 0))[0]
+            if h + m > 0:
+                rows.append((short_name(k), hits[k][1], h, m))
+        rows.sort(key=lambda r: -r[3])
+        print("| kernel | dispatches | L2 hits (M requests) | L2 misses (M requests) | hit rate |")
+        print("|---|---:|---:|---:|---:|")
+        for k, n, h, m in rows[:a.top]:
+            print(f"| `{k}` | {n} | {h / 1e6:.1f} | {m / 1e6:.1f} | {100 * h / (h + m):.1f}% |")
         return 0
     if a.cmd == "sq":
         print("| kernel | waves | wait on memory/LDS | wait to issue | issuing | VALU | LDS | VALU / VMEM rd / VMEM wr / LDS instr per wave |")
