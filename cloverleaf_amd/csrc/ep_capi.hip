@@ -276,6 +276,9 @@ int place_step_buffers(ep_ctx* c, uint32_t batch, ep::StepConfig& b, Place&& pla
   EP_TRY(place(&b.nbr, (size_t)batch * std::max<uint32_t>(S, 1)));
   EP_TRY(place(&b.neg, (size_t)batch * K));
   EP_TRY(place(&b.rcount, batch));
+  const uint32_t ent_cap = (uint32_t)std::min<uint64_t>(256, 1ull + neighbour_cap(c) + K);
+  EP_TRY(place(&b.ex_fs, (size_t)batch * ent_cap));
+  EP_TRY(place(&b.ex_fn, (size_t)batch * ent_cap));
   EP_TRY(place(&b.keyoff, batch + 1));
   // adjoint rows: mean model one per construction (D wide); attention one gradient row
   // per feature occurrence
@@ -342,7 +345,7 @@ int place_step_buffers(ep_ctx* c, uint32_t batch, ep::StepConfig& b, Place&& pla
   b.long_cap = long_cap;
   b.piece_cap = piece_cap;
   b.hash_entries = (uint32_t)hash;
-  b.ent_cap = (uint32_t)std::min<uint64_t>(256, 1ull + neighbour_cap(c) + K);
+  b.ent_cap = ent_cap;
   b.sel_cap = feature_sampling(cfg) ? b.ent_cap : 0u;
   b.ids_cap = (uint32_t)std::min<uint64_t>(8192, example_rows_max(c));
   // emission fused into the forward (mean model): its first-leaf hash takes the LDS of the
@@ -364,6 +367,7 @@ int place_step_buffers(ep_ctx* c, uint32_t batch, ep::StepConfig& b, Place&& pla
 // gradient-output and atomic-path arrays, the constructions hook).
 void clear_step_buffers(ep::StepConfig& b) {
   b.nbr_cnt = nullptr; b.nbr = nullptr; b.neg = nullptr; b.rcount = nullptr; b.keyoff = nullptr;
+  b.ex_fs = nullptr; b.ex_fn = nullptr;
   b.nbr_w = nullptr; b.nbroff = nullptr; b.nbr_sel = nullptr; b.nbr_selw = nullptr;
   b.adj = nullptr; b.adjh = nullptr; b.ascale = nullptr; b.coef = nullptr; b.elig = nullptr; b.loss = nullptr;
   b.pass_loss = nullptr; b.xout = nullptr;
@@ -390,6 +394,7 @@ int ensure_step_buffers(ep_ctx* c, uint32_t batch) {
   // drop old buffers
   ep::StepConfig& b = c->base;
   c->release(b.nbr_cnt); c->release(b.nbr); c->release(b.neg); c->release(b.rcount); c->release(b.keyoff);
+  c->release(b.ex_fs); c->release(b.ex_fn);
   c->release(b.nbr_w); c->release(b.nbroff); c->release(b.nbr_sel); c->release(b.nbr_selw);
   c->release(b.adj); c->release(b.adjh); c->release(b.ascale); c->release(b.coef); c->release(b.elig);
   c->release(b.loss);

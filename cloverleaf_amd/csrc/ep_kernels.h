@@ -135,6 +135,10 @@ struct StepConfig {
   uint32_t* nbr;           // [B*S]
   uint32_t* neg;           // [B*K]
   uint32_t* rcount;        // [B]
+  // [B*ent_cap] per example, the feature list of each entry e < ent_cap (v, the sampled
+  // neighbours, the negatives): its start in fids and its length, written by k_sample
+  uint64_t* ex_fs;
+  uint32_t* ex_fn;
   uint32_t* keyoff;        // [B+1]
   float* adj;              // [B*NC*D] FP32 adjoint rows (FP32 tables, attention gradient rows)
   // FP16 adjoint rows (BF16 tables, mean model): row r holds g / s in FP16 (round to

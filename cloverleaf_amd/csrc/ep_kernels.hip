@@ -616,7 +616,13 @@ __global__ __launch_bounds__(256) void k_sample(StepConfig c, StepState* st) {
     const uint32_t E = 1u + cnt + c.K;  // v, neighbours, negatives
     for (uint32_t e = lane; e < E; e += 64u) {
       const uint32_t u = e == 0u ? v : (e <= cnt ? pn[e - 1u] : negs[e - 1u - cnt]);
-      R += feature_count(c, (uint32_t)(c.findptr[u + 1] - c.findptr[u]), pos, pass, e, k0, k1);
+      const uint64_t fs = c.findptr[u];
+      const uint32_t n = (uint32_t)(c.findptr[u + 1] - fs);
+      if (e < c.ent_cap) {  // the forward stages these lists without the node lookups
+        c.ex_fs[(size_t)b * c.ent_cap + e] = fs;
+        c.ex_fn[(size_t)b * c.ent_cap + e] = n;
+      }
+      R += feature_count(c, n, pos, pass, e, k0, k1);
     }
     R = wave_sum_u32(R);
   }
@@ -790,6 +796,17 @@ struct WideShape {
   static constexpr int RPW = OK ? 64 / LW : 1;
 };
 
+// Zero ids after the forward's staged occurrence list: a wide gather round reads the ids
+// of up to (U - 1) * RPW rows past its range without a bounds check (the next
+// construction's ids, or these: valid rows either way) and adds only the rows inside
+// it.  U is at most 2x(8 for RPW < 4, else 4) rows in flight per lane group.
+__host__ __device__ constexpr uint32_t ids_pad_entries(uint32_t d, bool bf16) {
+  const uint32_t vw = bf16 ? 8u : 4u, lw = d / vw;
+  if (d % vw != 0u || lw < 1u || lw > 64u || (lw & (lw - 1u)) != 0u) return 0u;
+  const uint32_t rpw = 64u / lw, u = (rpw >= 4u ? 4u : 8u) * 2u;
+  return (u - 1u) * rpw + 1u;
+}
+
 template <typename T>
 __device__ inline void load_wide(const T* __restrict__ p, float (&v)[WideVec<T>::VW]);
 template <>
@@ -813,21 +830,27 @@ __device__ inline void gather_lds_wide(const T* __restrict__ table, const uint32
                                        int lane, float* dst, float scale) {
   using WS = WideShape<D, T>;
   constexpr int VW = WS::VW, LW = WS::LW, RPW = WS::RPW;
+  static_assert((U - 1) * RPW < (int)ids_pad_entries(D, sizeof(T) == 2), "a round reads at most into the zero padding");
   const int g = lane / LW, lg = lane % LW;
   constexpr int A = U < kGatherSums ? U : kGatherSums;
+  // The row stride is hidden from the optimiser, so that each row address is one
+  // 32 x 32 + 64-bit multiply-add of the id (a shift of a zero-extended id ties every id
+  // read to one register pair and serialises the LDS reads of a round).
+  uint32_t rb = (uint32_t)(D * sizeof(T));
+  asm volatile("" : "+s"(rb));
+  const char* tl = reinterpret_cast<const char*>(table + lg * VW);
   float part[A][VW];
 #pragma unroll
   for (int i = 0; i < A; ++i)
 #pragma unroll
     for (int k = 0; k < VW; ++k) part[i][k] = 0.f;
   for (uint32_t j = a + (uint32_t)g; j < b; j += (uint32_t)(RPW * U)) {
+    uint32_t id[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) id[u] = ids[j + (uint32_t)(u * RPW)];
     float r[U][VW];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint32_t t = j + (uint32_t)(u * RPW);
-      const uint32_t tt = t < b ? t : j;  // j < b: a valid row to load (result unused)
-      load_wide<T>(table + (size_t)ids[tt] * D + lg * VW, r[u]);
-    }
+    for (int u = 0; u < U; ++u) load_wide<T>(reinterpret_cast<const T*>(tl + (uint64_t)id[u] * rb), r[u]);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (j + (uint32_t)(u * RPW) < b) {
@@ -999,10 +1022,18 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
   float* dist = part + NW * D;    // K+1
   float* sn = dist + K + 1;       // NC occurrence counts
   __shared__ float s_scale, s_invA;
+  __shared__ float s_r[32];  // 1/D_0, then invA/D_i for the active negatives (0 otherwise)
   __shared__ int s_contrib;
   __shared__ uint32_t s_active;
   __shared__ uint8_t s_el[32];
   const T* table = (const T*)c.table;
+  // the feature lists of the example's entries, read before anything they wait on
+  uint64_t x_fs = 0;
+  uint32_t x_fn = 0;
+  if ((uint32_t)tid < c.ent_cap) {
+    x_fs = c.ex_fs[(size_t)b * c.ent_cap + tid];
+    x_fn = c.ex_fn[(size_t)b * c.ent_cap + tid];
+  }
   const uint32_t v = c.perm[st->anchor_base + b];
   const uint32_t cnt = c.nbr_cnt[b];
   if (cnt == 0u) {  // no neighbourhood: no example
@@ -1019,13 +1050,17 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
   const int base = lane * PL;
   const uint32_t apos = st->anchor_base + b, apass = st->pass;
   const uint32_t k0 = (uint32_t)c.seed, k1 = (uint32_t)(c.seed >> 32);
+  // read now for the key emission at the end, so no global load waits there (a load
+  // after the adjoint and key stores would wait for those stores too)
+  const uint32_t kb = c.keyoff[b];
+  const bool keys_fit = st->n_keys <= c.key_cap;
 
   __shared__ uint32_t s_wave[NW];
   __shared__ uint32_t s_np[NW];
   __shared__ float s_npw[NW];
-  // Staged path: the example's occurrence list is copied to LDS first (node ranges, then
-  // feature ids, one block-wide round trip each), so every wave streams rows from a
-  // contiguous id range with GU loads in flight.  Entries: e = 0 (v), 1..cnt
+  // Staged path: the example's occurrence list is copied to LDS first (the entries'
+  // feature lists from k_sample, then the feature ids, one block-wide round trip each), so
+  // every wave streams rows from a contiguous id range with GU loads in flight.  Entries: e = 0 (v), 1..cnt
   // (neighbours), cnt+1..cnt+K (negatives).
   const uint32_t E = 1u + cnt + K;
   const uint32_t Re = c.rcount[b];
@@ -1038,9 +1073,8 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
   if (staged) {
     uint32_t ne = 0;
     if ((uint32_t)tid < E) {
-      const uint32_t node = tid == 0 ? v : ((uint32_t)tid <= cnt ? pn[tid - 1] : c.neg[(size_t)b * K + tid - 1 - cnt]);
-      const uint64_t fs = c.findptr[node];
-      ne = (uint32_t)(c.findptr[node + 1] - fs);
+      const uint64_t fs = x_fs;
+      ne = x_fn;
       ent_fs[tid] = fs;
       if (c.mf_kind) {
         const FeatSel fsel = feature_select(c, ne, apos, apass, (uint32_t)tid, k0, k1);
@@ -1054,15 +1088,29 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
     if ((uint32_t)tid < E) ent_off[tid] = excl;
     if (tid == 0) ent_off[E] = tot;
     __syncthreads();
-    for (uint32_t o = tid; o < tot; o += blockDim.x) {
-      uint32_t lo = 0, hi = E;  // last entry e with ent_off[e] <= o
-      while (hi - lo > 1u) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (ent_off[mid] <= o) lo = mid; else hi = mid;
+    // IDS_U feature ids per thread in flight: positions past tot repeat the last one
+    constexpr uint32_t IDS_U = 4;
+    for (uint32_t o0 = tid; o0 < tot; o0 += IDS_U * blockDim.x) {
+      uint64_t src[IDS_U];
+#pragma unroll
+      for (uint32_t u = 0; u < IDS_U; ++u) {
+        const uint32_t o = min(o0 + u * blockDim.x, tot - 1u);
+        uint32_t lo = 0, hi = E;  // last entry e with ent_off[e] <= o
+        while (hi - lo > 1u) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (ent_off[mid] <= o) lo = mid; else hi = mid;
+        }
+        const uint32_t q = o - ent_off[lo];
+        src[u] = ent_fs[lo] + (c.mf_kind ? nth_set_bit(ent_mask[lo], q) : q);
       }
-      const uint32_t q = o - ent_off[lo];
-      ids[o] = c.fids[ent_fs[lo] + (c.mf_kind ? nth_set_bit(ent_mask[lo], q) : q)];
+      uint32_t f[IDS_U];
+#pragma unroll
+      for (uint32_t u = 0; u < IDS_U; ++u) f[u] = c.fids[src[u]];
+#pragma unroll
+      for (uint32_t u = 0; u < IDS_U; ++u)
+        if (o0 + u * blockDim.x < tot) ids[o0 + u * blockDim.x] = f[u];
     }
+    for (uint32_t t = tid; t < ids_pad_entries(D, sizeof(T) == 2); t += blockDim.x) ids[tot + t] = 0u;
     __syncthreads();
     // ~h(v): the pooled neighbour occurrences split evenly over the waves
     const uint32_t p0 = ent_off[1], p1 = ent_off[1 + cnt];
@@ -1212,6 +1260,17 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
     if (lane == 0) dist[i] = sqrtf(sq);
   }
   __syncthreads();
+  if (w == 0) {
+    // lane i - 1 tests negative i (K <= 29) and keeps its adjoint coefficient, so the
+    // adjoints below multiply instead of dividing
+    const float d1 = c.gamma + dist[0];
+    const uint32_t ii = (uint32_t)lane + 1u;
+    const bool on = ii <= K && d1 - dist[ii] > 0.f;
+    const uint32_t act_w = (uint32_t)__ballot(on);
+    const uint32_t nA_w = (uint32_t)__popc(act_w);
+    if (ii <= K) s_r[ii] = on ? (1.f / (float)nA_w) / dist[ii] : 0.f;
+    if (lane == 0) s_r[0] = 1.f / dist[0];
+  }
   if (tid == 0) {
     const float d1 = c.gamma + dist[0];
     float sum = 0.f;
@@ -1235,7 +1294,7 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
     s_invA = nA ? 1.f / (float)nA : 0.f;
     s_scale = 1.f / div;
     c.loss[b] = contrib ? Lw : 0.f;
-    if (c.pass_loss) c.pass_loss[st->anchor_base + b] = contrib ? Lw : 0.f;
+    if (c.pass_loss) c.pass_loss[apos] = contrib ? Lw : 0.f;
   }
   __syncthreads();
   if (c.xout) {
@@ -1254,28 +1313,36 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
     if (contrib) {
       if (ci == 0u) {
         e = true;
+        const float r0 = s_r[0];
 #pragma unroll
         for (int q = 0; q < PL; ++q)
-          if (base + q < D) g[q] = (xc[base + q] - xc[D + base + q]) / D0 * scale;
+          if (base + q < D) g[q] = (xc[base + q] - xc[D + base + q]) * r0 * scale;
       } else if (ci == 1u) {
         e = true;
+        const float r0 = s_r[0];
+        float p[PL];
 #pragma unroll
         for (int q = 0; q < PL; ++q) {
-          if (base + q < D) {
-            const float p = xc[D + base + q];
-            float gq = (p - xc[base + q]) / D0;
-            for (uint32_t i = 1; i <= K; ++i)
-              if (act & (1u << (i - 1u))) gq -= (p - xc[(i + 1u) * D + base + q]) / dist[i] * invA;
-            g[q] = gq * scale;
-          }
+          p[q] = base + q < D ? xc[D + base + q] : 0.f;
+          g[q] = base + q < D ? (p[q] - xc[base + q]) * r0 : 0.f;
         }
+        for (uint32_t m = act; m != 0u; m &= m - 1u) {  // the active negatives, in order
+          const uint32_t i = (uint32_t)__ffs(m);        // bit i - 1 set: negative i
+          const float ri = s_r[i];
+#pragma unroll
+          for (int q = 0; q < PL; ++q)
+            if (base + q < D) g[q] -= (p[q] - xc[(i + 1u) * D + base + q]) * ri;
+        }
+#pragma unroll
+        for (int q = 0; q < PL; ++q) g[q] *= scale;
       } else {
         const uint32_t i = ci - 1u;
         if (act & (1u << (i - 1u))) {
           e = true;
+          const float ri = s_r[i];
 #pragma unroll
           for (int q = 0; q < PL; ++q)
-            if (base + q < D) g[q] = (xc[D + base + q] - xc[ci * D + base + q]) / dist[i] * invA * scale;
+            if (base + q < D) g[q] = (xc[D + base + q] - xc[ci * D + base + q]) * ri * scale;
         }
       }
       if (e) {
@@ -1335,10 +1402,9 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
   // k_emit.
   if (!(staged && c.fuse_emit)) return;
   if (c.grad_mode == 0 && Re > c.fuse_max) return;
-  if (st->n_keys > c.key_cap) return;
+  if (!keys_fit) return;
   __syncthreads();
   const uint32_t tot = ent_off[E];
-  const uint32_t kb = c.keyoff[b];
   const bool first_leaf = c.grad_mode == 0;
   uint32_t* hsh = reinterpret_cast<uint32_t*>(sm);
   const uint32_t hmask = c.fuse_hash - 1u;
@@ -1360,15 +1426,28 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
     }
     __syncthreads();
   }
-  for (uint32_t o = tid; o < tot; o += blockDim.x) {
-    const uint32_t e = entry_of(o);
-    const uint32_t ci = cons_of_entry(e);
-    const uint32_t f = ids[o];
-    uint32_t key = c.sentinel;
-    if (s_el[ci] && (!first_leaf || hash_lookup(hsh, hmask, f) == ci)) key = key_of_row(c, f);
-    c.keys[0][kb + o] = key;
-    const uint32_t row = (c.nbr_rows && ci == 1u) ? nbr_row(c, b, e - 1u) : b * NC + ci;
-    c.slots[0][kb + o] = ((b / c.rbatch) << kSlotBatchShift) | row;
+  // two loops, so that the common one holds no global load (nbr_row reads one): its
+  // stores are then never waited for inside the loop
+  const uint32_t sbatch = (b / c.rbatch) << kSlotBatchShift;
+  if (c.nbr_rows) {
+    for (uint32_t o = tid; o < tot; o += blockDim.x) {
+      const uint32_t e = entry_of(o);
+      const uint32_t ci = cons_of_entry(e);
+      const uint32_t f = ids[o];
+      uint32_t key = c.sentinel;
+      if (s_el[ci] && (!first_leaf || hash_lookup(hsh, hmask, f) == ci)) key = key_of_row(c, f);
+      c.keys[0][kb + o] = key;
+      c.slots[0][kb + o] = sbatch | (ci == 1u ? nbr_row(c, b, e - 1u) : b * NC + ci);
+    }
+  } else {
+    for (uint32_t o = tid; o < tot; o += blockDim.x) {
+      const uint32_t ci = cons_of_entry(entry_of(o));
+      const uint32_t f = ids[o];
+      uint32_t key = c.sentinel;
+      if (s_el[ci] && (!first_leaf || hash_lookup(hsh, hmask, f) == ci)) key = key_of_row(c, f);
+      c.keys[0][kb + o] = key;
+      c.slots[0][kb + o] = sbatch | (b * NC + ci);
+    }
   }
 }
 
@@ -2865,7 +2944,7 @@ hipError_t launch_scan(const StepConfig& c, StepState* st, hipStream_t s) {
 size_t forward_lds_bytes(const StepConfig& c) {
   const uint32_t NC = c.K + 2;
   return (size_t)forward_float_words(NC, c.K, c.dims) * 4 + (size_t)c.ent_cap * 8 + (size_t)c.sel_cap * 12 +
-         (size_t)(c.ent_cap + 1) * 4 + (size_t)c.ids_cap * 4;
+         (size_t)(c.ent_cap + 1) * 4 + ((size_t)c.ids_cap + ids_pad_entries(c.dims, c.bf16 != 0)) * 4;
 }
 
 hipError_t launch_forward(const StepConfig& c, StepState* st, hipStream_t s) {
