@@ -286,6 +286,46 @@ __device__ inline float wave_sum(float x) {
   return x;
 }
 
+// DPP reductions: two quad permutes, the half-row and row mirrors (every lane then holds
+// its 16-lane row's total), then the row broadcasts of lanes 15 and 31 (rows 1 and 3,
+// then rows 2 and 3), which leave the wave's total in lane 63.  No LDS round trips, where
+// a shuffle butterfly takes six.  Lanes a DPP step leaves unwritten read `old` = 0.
+namespace dpp {
+constexpr int kQuad1032 = 0xB1;      // quad_perm [1, 0, 3, 2]
+constexpr int kQuad2301 = 0x4E;      // quad_perm [2, 3, 0, 1]
+constexpr int kRowHalfMirror = 0x141;
+constexpr int kRowMirror = 0x140;
+constexpr int kRowBcast15 = 0x142;
+constexpr int kRowBcast31 = 0x143;
+template <int CTRL, int ROW_MASK>
+__device__ inline float mov(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROW_MASK, 0xf, true));
+}
+}  // namespace dpp
+
+// Sum over the wave, returned to every lane (the order differs from wave_sum's).
+__device__ inline float wave_sum_dpp(float x) {
+  x += dpp::mov<dpp::kQuad1032, 0xf>(x);
+  x += dpp::mov<dpp::kQuad2301, 0xf>(x);
+  x += dpp::mov<dpp::kRowHalfMirror, 0xf>(x);
+  x += dpp::mov<dpp::kRowMirror, 0xf>(x);
+  x += dpp::mov<dpp::kRowBcast15, 0xa>(x);
+  x += dpp::mov<dpp::kRowBcast31, 0xc>(x);
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
+}
+
+// Maximum over the wave of values >= 0 (the zero of unwritten lanes is then neutral),
+// returned to every lane; a maximum is exact, so it equals wave_max's.
+__device__ inline float wave_max_nonneg(float x) {
+  x = fmaxf(x, dpp::mov<dpp::kQuad1032, 0xf>(x));
+  x = fmaxf(x, dpp::mov<dpp::kQuad2301, 0xf>(x));
+  x = fmaxf(x, dpp::mov<dpp::kRowHalfMirror, 0xf>(x));
+  x = fmaxf(x, dpp::mov<dpp::kRowMirror, 0xf>(x));
+  x = fmaxf(x, dpp::mov<dpp::kRowBcast15, 0xa>(x));
+  x = fmaxf(x, dpp::mov<dpp::kRowBcast31, 0xc>(x));
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
+}
+
 __device__ inline double wave_sum(double x) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, EP_WAVE);

@@ -59,7 +59,7 @@ __device__ inline void store_adj_row(const StepConfig& c, size_t slot, int lane,
 #pragma unroll
     for (int q = 0; q < PL; ++q)
       if (lane * PL + q < D) mx = fmaxf(mx, fabsf(g[q]));
-    mx = wave_max(mx);
+    mx = wave_max_nonneg(mx);
     int ex = 0;
     (void)frexpf(mx, &ex);  // mx = m 2^ex, m in [0.5, 1) (ex = 0 for mx = 0)
     ex = ex > 127 ? 127 : ex;
@@ -1088,27 +1088,49 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
     if ((uint32_t)tid < E) ent_off[tid] = excl;
     if (tid == 0) ent_off[E] = tot;
     __syncthreads();
-    // IDS_U feature ids per thread in flight: positions past tot repeat the last one
-    constexpr uint32_t IDS_U = 4;
-    for (uint32_t o0 = tid; o0 < tot; o0 += IDS_U * blockDim.x) {
-      uint64_t src[IDS_U];
+    constexpr int EU = 16;  // entries per wave on the entry-by-entry path
+    if (!c.mf_kind && E <= (uint32_t)(NW * EU)) {
+      // entry by entry: wave w loads the lists of entries w, w + NW, ..., one id per lane
+      // (64 at a time), every first load in flight before the first store
+      uint32_t f[EU];
 #pragma unroll
-      for (uint32_t u = 0; u < IDS_U; ++u) {
-        const uint32_t o = min(o0 + u * blockDim.x, tot - 1u);
-        uint32_t lo = 0, hi = E;  // last entry e with ent_off[e] <= o
-        while (hi - lo > 1u) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (ent_off[mid] <= o) lo = mid; else hi = mid;
-        }
-        const uint32_t q = o - ent_off[lo];
-        src[u] = ent_fs[lo] + (c.mf_kind ? nth_set_bit(ent_mask[lo], q) : q);
+      for (int u = 0; u < EU; ++u) {
+        const uint32_t e = (uint32_t)w + (uint32_t)(u * NW);
+        f[u] = 0u;
+        if (e < E && (uint32_t)lane < ent_off[e + 1] - ent_off[e]) f[u] = c.fids[ent_fs[e] + (uint32_t)lane];
       }
-      uint32_t f[IDS_U];
 #pragma unroll
-      for (uint32_t u = 0; u < IDS_U; ++u) f[u] = c.fids[src[u]];
+      for (int u = 0; u < EU; ++u) {
+        const uint32_t e = (uint32_t)w + (uint32_t)(u * NW);
+        if (e < E) {
+          const uint32_t a = ent_off[e], n = ent_off[e + 1] - a;
+          if ((uint32_t)lane < n) ids[a + (uint32_t)lane] = f[u];
+          for (uint32_t q = (uint32_t)lane + 64u; q < n; q += 64u) ids[a + q] = c.fids[ent_fs[e] + q];
+        }
+      }
+    } else {
+      // IDS_U feature ids per thread in flight: positions past tot repeat the last one
+      constexpr uint32_t IDS_U = 8;
+      for (uint32_t o0 = tid; o0 < tot; o0 += IDS_U * blockDim.x) {
+        uint64_t src[IDS_U];
 #pragma unroll
-      for (uint32_t u = 0; u < IDS_U; ++u)
-        if (o0 + u * blockDim.x < tot) ids[o0 + u * blockDim.x] = f[u];
+        for (uint32_t u = 0; u < IDS_U; ++u) {
+          const uint32_t o = min(o0 + u * blockDim.x, tot - 1u);
+          uint32_t lo = 0, hi = E;  // last entry e with ent_off[e] <= o
+          while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (ent_off[mid] <= o) lo = mid; else hi = mid;
+          }
+          const uint32_t q = o - ent_off[lo];
+          src[u] = ent_fs[lo] + (c.mf_kind ? nth_set_bit(ent_mask[lo], q) : q);
+        }
+        uint32_t f[IDS_U];
+#pragma unroll
+        for (uint32_t u = 0; u < IDS_U; ++u) f[u] = c.fids[src[u]];
+#pragma unroll
+        for (uint32_t u = 0; u < IDS_U; ++u)
+          if (o0 + u * blockDim.x < tot) ids[o0 + u * blockDim.x] = f[u];
+      }
     }
     for (uint32_t t = tid; t < ids_pad_entries(D, sizeof(T) == 2); t += blockDim.x) ids[tot + t] = 0u;
     __syncthreads();
@@ -1256,7 +1278,7 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
         sq += d * d;
       }
     }
-    sq = wave_sum(sq);
+    sq = wave_sum_dpp(sq);
     if (lane == 0) dist[i] = sqrtf(sq);
   }
   __syncthreads();
@@ -1408,14 +1430,12 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
   const bool first_leaf = c.grad_mode == 0;
   uint32_t* hsh = reinterpret_cast<uint32_t*>(sm);
   const uint32_t hmask = c.fuse_hash - 1u;
-  auto entry_of = [&](uint32_t o) -> uint32_t {
-    uint32_t lo = 0, hi = E;  // last entry e with ent_off[e] <= o
-    while (hi - lo > 1u) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (ent_off[mid] <= o) lo = mid; else hi = mid;
-    }
-    return lo;
-  };
+  // the entry of each occurrence, one byte each (E <= 256) after the hash (the host keeps
+  // fuse_hash words + ids_cap bytes inside the region), written entry by entry
+  uint8_t* oent = reinterpret_cast<uint8_t*>(hsh + c.fuse_hash);
+  for (uint32_t e = (uint32_t)w; e < E; e += NW)
+    for (uint32_t o = ent_off[e] + (uint32_t)lane; o < ent_off[e + 1]; o += 64u) oent[o] = (uint8_t)e;
+  auto entry_of = [&](uint32_t o) -> uint32_t { return oent[o]; };
   auto cons_of_entry = [&](uint32_t e) -> uint32_t { return e == 0u ? 0u : (e <= cnt ? 1u : 2u + (e - 1u - cnt)); };
   if (first_leaf) {
     for (uint32_t i = tid; i < c.fuse_hash; i += blockDim.x) hsh[i] = HASH_EMPTY;
@@ -1424,8 +1444,8 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
       const uint32_t ci = cons_of_entry(entry_of(o));
       if (s_el[ci]) hash_insert_min(hsh, hmask, ids[o], ci);
     }
-    __syncthreads();
   }
+  __syncthreads();
   // two loops, so that the common one holds no global load (nbr_row reads one): its
   // stores are then never waited for inside the loop
   const uint32_t sbatch = (b / c.rbatch) << kSlotBatchShift;
