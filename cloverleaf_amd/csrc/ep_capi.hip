@@ -348,16 +348,13 @@ int place_step_buffers(ep_ctx* c, uint32_t batch, ep::StepConfig& b, Place&& pla
   b.ent_cap = ent_cap;
   b.sel_cap = feature_sampling(cfg) ? b.ent_cap : 0u;
   b.ids_cap = (uint32_t)std::min<uint64_t>(8192, example_rows_max(c));
-  // emission fused into the forward (mean model): its first-leaf hash and a byte per
-  // occurrence (its entry) take the LDS of the constructions and partial sums,
-  // (NC + waves) * D floats, after the adjoints are out
+  // emission fused into the forward (mean model): its first-leaf hash takes the LDS of the
+  // constructions and partial sums, (NC + waves) * D floats, after the adjoints are out
   b.fuse_emit = att ? 0 : 1;
   {
     const uint64_t words = (uint64_t)(NC + ep::kBlockWaves) * D;
-    const uint64_t ent_words = (b.ids_cap + 3ull) / 4ull;
     uint32_t fh = 1;
-    while ((uint64_t)fh * 2 + ent_words <= words) fh *= 2;
-    if ((uint64_t)fh + ent_words > words) b.fuse_emit = 0;  // too small a region: k_emit emits
+    while ((uint64_t)fh * 2 <= words) fh *= 2;
     b.fuse_hash = fh;
     b.fuse_max = fh / 8 * 5;
   }

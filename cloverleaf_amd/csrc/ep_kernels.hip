@@ -1430,12 +1430,22 @@ __global__ __launch_bounds__(256) void k_forward(StepConfig c, StepState* st) {
   const bool first_leaf = c.grad_mode == 0;
   uint32_t* hsh = reinterpret_cast<uint32_t*>(sm);
   const uint32_t hmask = c.fuse_hash - 1u;
-  // the entry of each occurrence, one byte each (E <= 256) after the hash (the host keeps
-  // fuse_hash words + ids_cap bytes inside the region), written entry by entry
+  // the entry of each occurrence: one byte each (E <= 256), written entry by entry after
+  // the hash where the region holds them, else a search of the entry offsets
   uint8_t* oent = reinterpret_cast<uint8_t*>(hsh + c.fuse_hash);
-  for (uint32_t e = (uint32_t)w; e < E; e += NW)
-    for (uint32_t o = ent_off[e] + (uint32_t)lane; o < ent_off[e + 1]; o += 64u) oent[o] = (uint8_t)e;
-  auto entry_of = [&](uint32_t o) -> uint32_t { return oent[o]; };
+  const bool ent_bytes = (size_t)c.fuse_hash * 4u + tot <= (size_t)forward_float_words(NC, K, D) * 4u;
+  if (ent_bytes)
+    for (uint32_t e = (uint32_t)w; e < E; e += NW)
+      for (uint32_t o = ent_off[e] + (uint32_t)lane; o < ent_off[e + 1]; o += 64u) oent[o] = (uint8_t)e;
+  auto entry_of = [&](uint32_t o) -> uint32_t {
+    if (ent_bytes) return oent[o];
+    uint32_t lo = 0, hi = E;  // last entry e with ent_off[e] <= o
+    while (hi - lo > 1u) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (ent_off[mid] <= o) lo = mid; else hi = mid;
+    }
+    return lo;
+  };
   auto cons_of_entry = [&](uint32_t e) -> uint32_t { return e == 0u ? 0u : (e <= cnt ? 1u : 2u + (e - 1u - cnt)); };
   if (first_leaf) {
     for (uint32_t i = tid; i < c.fuse_hash; i += blockDim.x) hsh[i] = HASH_EMPTY;
@@ -1964,6 +1974,39 @@ __device__ inline void write_long_row(const StepConfig& c, uint32_t idx, uint32_
   }
 }
 
+// Batch boundaries of a long row [lo, hi) whose batches rise: lane group g (8 lanes) finds
+// the first index whose batch is >= g + 1 by an 8-ary search, all groups at once (one
+// dependent load per level where one search per batch would take PM in sequence).  Groups
+// g >= pm return lo.
+__device__ inline uint32_t group_batch_bounds(const uint32_t* __restrict__ Ss, uint32_t lo, uint32_t hi, uint32_t pm,
+                                              uint32_t lane) {
+  const uint32_t g = lane >> 3, gl = lane & 7u, q = g + 1u;
+  uint32_t a = lo, z = q <= pm ? hi : lo;
+  while (true) {
+    const bool act = z > a;
+    if (__ballot(act) == 0ull) break;
+    const uint32_t span = z - a;
+    const uint32_t step = act ? (span + 7u) / 8u : 1u;
+    const uint32_t off = gl * step;
+    const bool valid = act && off < span;
+    const uint32_t bt = valid ? Ss[a + off] >> kSlotBatchShift : 0u;
+    const uint32_t gm = (uint32_t)(__ballot(valid && bt >= q) >> (g * 8u)) & 0xffu;
+    if (act) {
+      if (step == 1u) {
+        a = gm ? a + (uint32_t)(__ffs(gm) - 1) : z;
+        z = a;
+      } else if (gm == 0u) {
+        a = a + ((span - 1u) / step) * step + 1u;  // past the last sample (all below)
+      } else {
+        const uint32_t f = (uint32_t)(__ffs(gm) - 1);
+        z = a + f * step;                    // this sample is in batch >= q
+        a = f > 0u ? a + (f - 1u) * step + 1u : a;  // the one before is not
+      }
+    }
+  }
+  return a;
+}
+
 __global__ __launch_bounds__(256) void k_seg_plan(StepConfig c, StepState* st, const uint32_t* __restrict__ Ks,
                                                   const uint32_t* __restrict__ Ss) {
   __shared__ uint32_t s_run[NW][132];  // run starts (chunk-relative); s_run[nrun] = window end
@@ -1988,8 +2031,8 @@ __global__ __launch_bounds__(256) void k_seg_plan(StepConfig c, StepState* st, c
   // second half is loaded while this chunk is classified.  nprev / npb: the entry before
   // the chunk (the previous chunk's last).
   uint32_t nr0 = SENT, nb0 = 0u, nr1 = SENT, nb1 = 0u, nprev = 0u, npb = 0u;
-  if (cbeg < cend) {
-    const uint32_t i0 = cbeg * 64u;
+  auto prime = [&](uint32_t ch) {  // the window of chunk ch and the entry before it
+    const uint32_t i0 = ch * 64u;
     const uint32_t j0 = i0 + lane, j1 = i0 + 64u + lane;
     nr0 = j0 < n ? Ks[j0] : SENT;
     nr1 = j1 < n ? Ks[j1] : SENT;
@@ -1997,13 +2040,18 @@ __global__ __launch_bounds__(256) void k_seg_plan(StepConfig c, StepState* st, c
     nb1 = j1 < n ? Ss[j1] >> kSlotBatchShift : 0u;
     nprev = i0 > 0u ? Ks[i0 - 1u] : ~__shfl(nr0, 0, EP_WAVE);
     npb = i0 > 0u ? Ss[i0 - 1u] >> kSlotBatchShift : 0u;
-  }
+  };
+  if (cbeg < cend) prime(cbeg);
   for (uint32_t chunk = cbeg; chunk < cend; ++chunk) {
     const uint32_t i0 = chunk * 64u;
-    if (chunk / cpr != reg) {  // the next region: close this one
+    if (chunk / cpr != reg) {  // the next region: close this one (and any jumped over)
       if (lane == 0) {
         c.scount[reg] = ns;
         c.mcount[reg] = nm;
+        for (uint32_t r = reg + 1u; r < chunk / cpr; ++r) {
+          c.scount[r] = 0u;
+          c.mcount[r] = 0u;
+        }
       }
       reg = chunk / cpr;
       sbase = (size_t)reg * cpr * 64u;
@@ -2033,96 +2081,122 @@ __global__ __launch_bounds__(256) void k_seg_plan(StepConfig c, StepState* st, c
     }
     const bool is_start = (r0 != SENT) && (r0 != prev);
     const uint64_t starts = __ballot(is_start);
-    if (starts == 0ull) continue;
-    const int first = __ffsll((unsigned long long)starts) - 1;
-    const int ql = 63 - __clzll((unsigned long long)starts);
-    const uint32_t fl = __shfl(r0, ql, EP_WAVE);
-    const uint64_t m0 = __ballot(lane > ql && r0 != fl);
-    const uint64_t m1 = __ballot(r1 != fl);
-    const bool deferred = (m0 == 0ull) && (m1 == 0ull);
-    const uint32_t wend = m0 ? (uint32_t)(__ffsll((unsigned long long)m0) - 1)
-                             : (m1 ? 64u + (uint32_t)(__ffsll((unsigned long long)m1) - 1) : (uint32_t)ql);
-    // runs of the rows handled in the window: [first, wend)
-    const uint64_t fs0 = __ballot(lane >= first && (uint32_t)lane < wend && (r0 != prev || b0 != pb0));
-    const uint64_t fs1 = __ballot(64u + (uint32_t)lane < wend && (r1 != prev1 || b1 != pb1));
-    const uint32_t n0 = (uint32_t)__popcll(fs0);
-    const uint32_t nrun = n0 + (uint32_t)__popcll(fs1);
-    if ((fs0 >> lane) & 1ull) {
-      const uint32_t i = (uint32_t)__popcll(fs0 & lt);
-      s_run[w][i] = (uint32_t)lane;
-      s_bat[w][i] = (uint8_t)b0;
-    }
-    if ((fs1 >> lane) & 1ull) {
-      const uint32_t i = n0 + (uint32_t)__popcll(fs1 & lt);
-      s_run[w][i] = 64u + (uint32_t)lane;
-      s_bat[w][i] = (uint8_t)b1;
-    }
-    const uint64_t rowm = deferred ? (starts & ~(1ull << ql)) : starts;
-    const uint32_t nrow = (uint32_t)__popcll(rowm);
-    if ((rowm >> lane) & 1ull) s_row[w][__popcll(rowm & lt)] = (uint32_t)__popcll(fs0 & lt);
-    if (lane == 0) {
-      s_run[w][nrun] = wend;
-      s_row[w][nrow] = nrun;
-    }
-    if (deferred) {
-      // the last row runs past the window: find its runs by search, register it as long
-      const uint32_t lo = i0 + (uint32_t)ql;
-      const uint32_t hi = wave_lower_bound(Ks, lo + 128u - (uint32_t)ql, n, fl + 1u);  // row end
-      uint32_t bnd[kMaxGroup + 1];  // batch boundaries: the row's values rise batch by batch
-      bnd[0] = lo;
-      for (uint32_t b = 0; b < PM; ++b) bnd[b + 1] = wave_lower_bound(Ss, bnd[b], hi, (b + 1u) << kSlotBatchShift);
-      bnd[PM + 1] = hi;
-      uint32_t pcs = 0, nr = 0;
-      for (uint32_t b = 0; b <= PM; ++b) {
-        if (bnd[b + 1] > bnd[b]) ++nr;
-        pcs += long_run_pieces(bnd[b + 1] - bnd[b]);
+    // a row end to find (key of the row + 1, 0: none) and where to go on from
+    uint32_t find = 0u, jump = 0u;
+    bool deferred = false;
+    int ql = 0;
+    uint32_t fl = 0u;
+    if (starts == 0ull) {
+      // inside a row that started before this chunk (another wave's, or a deferred one):
+      // when it also fills the next chunk, find its end and go on from there
+      const uint32_t key = (uint32_t)__builtin_amdgcn_readfirstlane((int)r0);
+      if (key != SENT && (uint32_t)__builtin_amdgcn_readlane((int)r1, 63) == key) find = key + 1u;
+    } else {
+      const int first = __ffsll((unsigned long long)starts) - 1;
+      ql = 63 - __clzll((unsigned long long)starts);
+      fl = (uint32_t)__builtin_amdgcn_readlane((int)r0, ql);
+      const uint64_t m0 = __ballot(lane > ql && r0 != fl);
+      const uint64_t m1 = __ballot(r1 != fl);
+      deferred = (m0 == 0ull) && (m1 == 0ull);
+      if (deferred) find = fl + 1u;
+      const uint32_t wend = m0 ? (uint32_t)(__ffsll((unsigned long long)m0) - 1)
+                               : (m1 ? 64u + (uint32_t)(__ffsll((unsigned long long)m1) - 1) : (uint32_t)ql);
+      // runs of the rows handled in the window: [first, wend)
+      const uint64_t fs0 = __ballot(lane >= first && (uint32_t)lane < wend && (r0 != prev || b0 != pb0));
+      const uint64_t fs1 = __ballot(64u + (uint32_t)lane < wend && (r1 != prev1 || b1 != pb1));
+      const uint32_t n0 = (uint32_t)__popcll(fs0);
+      const uint32_t nrun = n0 + (uint32_t)__popcll(fs1);
+      if ((fs0 >> lane) & 1ull) {
+        const uint32_t i = (uint32_t)__popcll(fs0 & lt);
+        s_run[w][i] = (uint32_t)lane;
+        s_bat[w][i] = (uint8_t)b0;
       }
-      uint32_t idx = 0, pb = 0;
+      if ((fs1 >> lane) & 1ull) {
+        const uint32_t i = n0 + (uint32_t)__popcll(fs1 & lt);
+        s_run[w][i] = 64u + (uint32_t)lane;
+        s_bat[w][i] = (uint8_t)b1;
+      }
+      const uint64_t rowm = deferred ? (starts & ~(1ull << ql)) : starts;
+      const uint32_t nrow = (uint32_t)__popcll(rowm);
+      if ((rowm >> lane) & 1ull) s_row[w][__popcll(rowm & lt)] = (uint32_t)__popcll(fs0 & lt);
       if (lane == 0) {
-        idx = atomicAdd(&st->n_long, 1u);
-        pb = atomicAdd(&st->n_pieces, pcs);
+        s_run[w][nrun] = wend;
+        s_row[w][nrow] = nrun;
       }
-      idx = __shfl(idx, 0, EP_WAVE);
-      pb = __shfl(pb, 0, EP_WAVE);
-      if (idx < c.long_cap) {
-        auto run_at = [&](uint32_t j, uint32_t* a, uint32_t* z, uint32_t* bq) {
-          uint32_t k = 0, seen = 0;  // j-th nonempty batch run
-          for (uint32_t b = 0; b <= PM; ++b)
-            if (bnd[b + 1] > bnd[b]) { if (seen == j) k = b; ++seen; }
-          *a = bnd[k];
-          *z = bnd[k + 1];
-          *bq = k;
+      __builtin_amdgcn_wave_barrier();
+      // classify the owned rows of the window: short records or medium records
+      const bool own = (uint32_t)lane < nrow;
+      const uint32_t ra = own ? s_row[w][lane] : 0u;
+      const uint32_t rz = own ? s_row[w][lane + 1] : 0u;
+      const uint32_t ea = own ? s_run[w][ra] : 0u;
+      const uint32_t ez = own ? s_run[w][rz] : 0u;
+      const bool med = own && (ez - ea) > kSegShort;
+      const uint64_t mmask = __ballot(med);
+      const uint64_t smask = __ballot(own && !med);
+      const uint32_t f = row_of_key(c, __shfl(r0, (int)ea, EP_WAVE));  // row starts lie in [0, 64)
+      if (own && !med) {
+        uint32_t mask = 0u, meta = ez - ea;
+        for (uint32_t j = ra; j < rz; ++j) {
+          mask |= 1u << (s_run[w][j] - ea);
+          meta |= (uint32_t)s_bat[w][j] << (6u + 3u * (j - ra));
+        }
+        c.srec[sbase + ns + (uint32_t)__popcll(smask & lt)] = make_uint4(i0 + ea, f, mask, meta);
+      }
+      if (med) c.mrec[mbase + nm + (uint32_t)__popcll(mmask & lt)] = make_uint2(i0 + ea, ez - ea);
+      ns += (uint32_t)__popcll(smask);
+      nm += (uint32_t)__popcll(mmask);
+      __builtin_amdgcn_wave_barrier();
+    }
+    if (find) {
+      // the end of the row that fills the rest of the window (both search paths share this
+      // one call); the chunks before the one holding it start no row
+      const uint32_t hi = wave_lower_bound(Ks, i0 + 128u, n, find);
+      if (hi / 64u > chunk + 1u) jump = min(hi / 64u, cend);
+      if (deferred) {
+        // the chunk's last row runs past the window: register it as long, its runs found by
+        // the batch boundaries (its values rise batch by batch)
+        const uint32_t lo = i0 + (uint32_t)ql;
+        const uint32_t gb = group_batch_bounds(Ss, lo, hi, PM, (uint32_t)lane);
+        auto bnd = [&](uint32_t b) -> uint32_t {  // b = 0..PM + 1
+          return b == 0u ? lo : (b > PM ? hi : (uint32_t)__builtin_amdgcn_readlane((int)gb, (int)((b - 1u) * 8u)));
         };
-        write_long_row(c, idx, lo, pb, nr, run_at, (uint32_t)lane, 64u);
+        uint32_t pcs = 0, nr = 0;
+        for (uint32_t b = 0; b <= PM; ++b) {
+          if (bnd(b + 1) > bnd(b)) ++nr;
+          pcs += long_run_pieces(bnd(b + 1) - bnd(b));
+        }
+        uint32_t idx = 0, pb = 0;
+        if (lane == 0) {
+          idx = atomicAdd(&st->n_long, 1u);
+          pb = atomicAdd(&st->n_pieces, pcs);
+        }
+        idx = (uint32_t)__builtin_amdgcn_readfirstlane((int)idx);
+        pb = (uint32_t)__builtin_amdgcn_readfirstlane((int)pb);
+        if (idx < c.long_cap) {
+          auto run_at = [&](uint32_t j, uint32_t* a, uint32_t* z, uint32_t* bq) {
+            uint32_t k = 0, seen = 0;  // j-th nonempty batch run
+            for (uint32_t b = 0; b <= PM; ++b)
+              if (bnd(b + 1) > bnd(b)) { if (seen == j) k = b; ++seen; }
+            *a = bnd(k);
+            *z = bnd(k + 1);
+            *bq = k;
+          };
+          write_long_row(c, idx, lo, pb, nr, run_at, (uint32_t)lane, 64u);
+        }
       }
     }
-    __builtin_amdgcn_wave_barrier();
-    // classify the owned rows of the window: short records or medium records
-    const bool own = (uint32_t)lane < nrow;
-    const uint32_t ra = own ? s_row[w][lane] : 0u;
-    const uint32_t rz = own ? s_row[w][lane + 1] : 0u;
-    const uint32_t ea = own ? s_run[w][ra] : 0u;
-    const uint32_t ez = own ? s_run[w][rz] : 0u;
-    const bool med = own && (ez - ea) > kSegShort;
-    const uint64_t mmask = __ballot(med);
-    const uint64_t smask = __ballot(own && !med);
-    const uint32_t f = row_of_key(c, __shfl(r0, (int)ea, EP_WAVE));  // row starts lie in [0, 64)
-    if (own && !med) {
-      uint32_t mask = 0u, meta = ez - ea;
-      for (uint32_t j = ra; j < rz; ++j) {
-        mask |= 1u << (s_run[w][j] - ea);
-        meta |= (uint32_t)s_bat[w][j] << (6u + 3u * (j - ra));
-      }
-      c.srec[sbase + ns + (uint32_t)__popcll(smask & lt)] = make_uint4(i0 + ea, f, mask, meta);
+    if (jump) {
+      if (jump < cend) prime(jump);
+      chunk = jump - 1u;
     }
-    if (med) c.mrec[mbase + nm + (uint32_t)__popcll(mmask & lt)] = make_uint2(i0 + ea, ez - ea);
-    ns += (uint32_t)__popcll(smask);
-    nm += (uint32_t)__popcll(mmask);
-    __builtin_amdgcn_wave_barrier();
   }
   if (lane == 0 && cbeg < cend) {
     c.scount[reg] = ns;
     c.mcount[reg] = nm;
+    for (uint32_t r = reg + 1u; r <= (cend - 1u) / cpr; ++r) {  // regions jumped over at the end
+      c.scount[r] = 0u;
+      c.mcount[r] = 0u;
+    }
   }
 }
 
