@@ -1623,7 +1623,6 @@ __global__ __launch_bounds__(256) void k_rs_hist(const uint32_t* __restrict__ ki
   constexpr uint32_t BINS = 1u << RB, MASK = BINS - 1u;
   __shared__ uint32_t h[BINS];
   const int tid = threadIdx.x, lane = tid & 63;
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint32_t lo, hi;
   block_range(pass == 0 ? st->n_keys : st->n_real, G, blockIdx.x, &lo, &hi);
   for (uint32_t d = tid; d < BINS; d += 256u) h[d] = 0u;
@@ -1640,8 +1639,15 @@ __global__ __launch_bounds__(256) void k_rs_hist(const uint32_t* __restrict__ ki
       const uint32_t i = t0 + r * 256u + tid;
       const bool act = i < hi && k[r] != sent;
       const uint32_t d = (k[r] >> shift) & MASK;
-      const uint64_t peers = digit_peers<RB>(act, d);
-      if (act && (peers & lt) == 0ull) atomicAdd(&h[d], (uint32_t)__popcll(peers));  // lowest lane of the group
+      // one digit for the whole wave (runs of a hot row): one add of the count; else an
+      // LDS add per key (the LDS serialises only the lanes that share an address)
+      const uint64_t am = __ballot(act);
+      const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
+      if (__ballot(act && d == d0) == am) {
+        if (am && lane == __ffsll((unsigned long long)am) - 1) atomicAdd(&h[d0], (uint32_t)__popcll(am));
+      } else if (act) {
+        atomicAdd(&h[d], 1u);
+      }
     }
   }
   __syncthreads();
