@@ -1736,7 +1736,11 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t* __restrict__
     for (int q = 0; q < IT; ++q) {
       const bool act = k[q] != sent;
       const uint32_t d = (k[q] >> shift) & MASK;
-      const uint64_t peers = digit_peers<RB>(act, d);
+      // the lanes sharing this lane's digit: all active lanes when the wave holds one digit
+      // (runs of a hot row), else matched bit by bit
+      const uint64_t am = __ballot(act);
+      const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
+      const uint64_t peers = __ballot(act && d == d0) == am ? (act ? am : 0ull) : digit_peers<RB>(act, d);
       const uint32_t before = act ? wcnt[w][d] : 0u;
       rk[q] = before + (uint32_t)__popcll(peers & lt);
       if (act && (peers >> lane) == 1ull) wcnt[w][d] = before + (uint32_t)__popcll(peers);  // last lane of the group
