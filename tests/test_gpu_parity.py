@@ -100,6 +100,24 @@ def test_dims_100_and_tiny_dims():
         ctx.close()
 
 
+def test_large_examples_unstaged_mean_model():
+    # hub anchors with 300 sampled neighbours: more entries than the forward's LDS stage
+    # holds, so their examples gather unstaged, beside staged ones in the same launch; with
+    # K = 5 the pooled partial sums share the negatives' LDS slots on both paths
+    prob = make_problem(seed=52, n=400, V=500, D=64, fmax=16)
+    rng = np.random.default_rng(52)
+    rows = [set(prob.indices[int(prob.indptr[i]):int(prob.indptr[i + 1])].tolist()) for i in range(prob.n)]
+    for h in range(4):
+        rows[h] |= set(rng.choice(np.arange(4, prob.n), size=320, replace=False).tolist())
+    prob.indptr = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.uint64)
+    prob.indices = np.array([j for r in rows for j in sorted(r)], dtype=np.uint32)
+    anchors = np.concatenate([[0, 1, 2, 3], rng.choice(np.arange(4, prob.n), 12, replace=False)]).astype(np.uint32)
+    for grad_mode in (N.EP_GRAD_FIRST_LEAF, N.EP_GRAD_EXACT):
+        ctx = make_ctx(prob, 300, 5, 0.5, B=16, grad_mode=grad_mode)
+        check_step(prob, ctx, anchors, 0.5, grad_mode)
+        ctx.close()
+
+
 def test_hub_sampling_and_long_segments():
     # few features, heavy Zipf -> very long sorted segments (long-segment path), hubs with deg >> S
     prob = make_problem(seed=11, n=400, V=20, D=64, avg_deg=40, fmax=6, zipf=1.5)
