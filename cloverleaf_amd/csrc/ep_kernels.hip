@@ -2653,7 +2653,9 @@ template <int D, bool H>
 __global__ __launch_bounds__(256) void k_long_combine(StepConfig c, StepState* st, const uint32_t* __restrict__ Ks,
                                                       const uint32_t* __restrict__ Ss) {
   constexpr int PL = RowShape<D>::PL;
-  constexpr int U = SR_UNROLL;
+  // 8 rows in flight where the other row kernels keep 16: at d = 256 the kernel then runs
+  // 4 waves per SIMD instead of 3 (its runs hold a few pieces each)
+  constexpr int U = PL >= 4 ? 8 : SR_UNROLL;
   const int lane = threadIdx.x & 63;
   const uint32_t gw = blockIdx.x * NW + (threadIdx.x >> 6);
   const uint32_t nwaves = gridDim.x * NW;
@@ -3194,7 +3196,9 @@ static void launch_row_kernels(const StepConfig& c, StepState* st, int buf, hipS
   const int pgrid = resident_blocks(k_pieces<D, H>, grid);
   hipLaunchKernelGGL((k_pieces<D, H>), dim3(pgrid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
   if (ev) (void)hipEventRecord(ev[2], s);
-  hipLaunchKernelGGL((k_long_combine<D, H>), dim3(grid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
+  // hot rows dealt to one round of resident blocks: no late blocks holding the last rows
+  const int lgrid = resident_blocks(k_long_combine<D, H>, grid);
+  hipLaunchKernelGGL((k_long_combine<D, H>), dim3(lgrid), dim3(256), 0, s, c, st, c.keys[buf], c.slots[buf]);
   if (ev) (void)hipEventRecord(ev[3], s);
 }
 
