@@ -3088,9 +3088,23 @@ static void radix_plan(int bits, int* passes, int* rb) {
   *rb = (bits + *passes - 1) / *passes;
 }
 
+// Blocks of 256 threads of kernel k resident on the current device at once, at most cap.
+template <typename Kernel>
+static int resident_blocks(Kernel k, int cap) {
+  int dev = 0, ncu = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return cap;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) return cap;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, 256, 0) != hipSuccess || per <= 0) return cap;
+  return per * ncu < cap ? per * ncu : cap;
+}
+
 template <int RB>
 static void launch_sort_rb(const StepConfig& c, StepState* st, hipStream_t s, int passes, int* cur) {
-  const int G = c.sort_grid;
+  // whole rounds of resident scatter blocks (at 9-bit digits its tile and counters take
+  // 44 KB of LDS, three blocks per CU): a last, partial round would leave part of the
+  // device idle; the three kernels of a pass share the block partition
+  const int res = resident_blocks(k_rs_scatter<RB>, c.sort_grid);
+  const int G = res < c.sort_grid ? (c.sort_grid / res) * res : c.sort_grid;
   for (int p = 0; p < passes; ++p) {
     const int shift = p * RB;
     hipLaunchKernelGGL((k_rs_hist<RB>), dim3(G), dim3(256), 0, s, c.keys[*cur], st, c.ghist, p, shift, G,
@@ -3177,15 +3191,6 @@ static void launch_seg_update_variant(const StepConfig& c, StepState* st, int bu
     hipLaunchKernelGGL((k_seg_update<D, 4, 8, H>), dim3(grid), dim3(256), 0, s, c, st, c.slots[buf]);
 }
 
-// Blocks of 256 threads of kernel k resident on the current device at once, at most cap.
-template <typename Kernel>
-static int resident_blocks(Kernel k, int cap) {
-  int dev = 0, ncu = 0, per = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return cap;
-  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) return cap;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, 256, 0) != hipSuccess || per <= 0) return cap;
-  return per * ncu < cap ? per * ncu : cap;
-}
 
 template <int D, bool H>
 static void launch_row_kernels(const StepConfig& c, StepState* st, int buf, hipStream_t s, int grid, hipEvent_t* ev) {
