@@ -231,6 +231,37 @@ def test_sampler_uniformity():
     ctx.close()
 
 
+def test_fixed_neighbour_picks_match_floyd():
+    # Fixed(S) neighbours (Sample::Fixed through model.rs:363-377): deg <= S takes the row in
+    # order; deg > S, Floyd's S-subset over positions deg - S .. deg - 1, pick j drawn from
+    # the Philox stream (batch position, pass, purpose 1) at index j.  Restated on the host.
+    from tests.feature_rules import StreamDraws, uniform_below
+    prob = make_problem(seed=7, n=120, V=100, D=16, avg_deg=30)
+    S, K, pass_, seed = 8, 3, 3, 20222022
+    ctx = make_ctx(prob, S, K, B=120, seed=seed)
+    anchors = np.arange(120, dtype=np.uint32)
+    cnt, nbr, neg = ctx.sample_batch(anchors, pass_)
+    ip = prob.indptr.astype(np.int64)
+    checked = 0
+    for i, v in enumerate(anchors):
+        row = prob.indices[ip[v]:ip[v + 1]]
+        deg = len(row)
+        assert cnt[i] == min(deg, S)
+        if deg <= S:
+            assert np.array_equal(nbr[i, :deg], row)
+            continue
+        d = StreamDraws(seed, i, pass_, 1)
+        picks = []
+        for j in range(S):
+            hi = deg - S + j
+            t = uniform_below(d.word(j), hi + 1)
+            picks.append(hi if t in picks else t)
+        assert np.array_equal(nbr[i, :S], row[picks]), (i, deg)
+        checked += 1
+    assert checked > 50
+    ctx.close()
+
+
 def bf16_round_nearest(x):
     u = np.asarray(x, dtype=np.float32).view(np.uint32)
     return ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).view(np.float32)
