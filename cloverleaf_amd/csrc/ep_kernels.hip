@@ -586,23 +586,42 @@ __global__ __launch_bounds__(256) void k_sample(StepConfig c, StepState* st) {
     } else {
       cnt = c.S;
       // Floyd: for i = 0..S-1 draw t in [0, deg-S+i]; take deg-S+i instead when t was
-      // already picked.  Picks live in LDS; the membership test is one ballot.
-      extern __shared__ uint32_t s_pick[];
-      uint32_t* pk = s_pick + (threadIdx.x >> 6) * c.S;
-      for (uint32_t i = 0; i < c.S; ++i) {
-        const uint32_t hi = deg - c.S + i;
-        const uint32_t t = uniform_below(draw_word(pos, pass, RNG_NEIGHBOURS, i, k0, k1), hi + 1u);
-        bool dup = false;
-        for (uint32_t q = lane; q < i; q += 64u) dup |= (pk[q] == t);
-        const bool any = wave_any(dup);
-        if (lane == 0) pk[i] = any ? hi : t;
-        __builtin_amdgcn_wave_barrier();
-      }
-      for (uint32_t j = lane; j < c.S; j += 64u) out[j] = c.indices[s0 + pk[j]];
-      if (c.wavg) {
-        const float scale = (float)deg / (float)c.S;
-        for (uint32_t j = lane; j < c.S; j += 64u)
-          c.nbr_w[(size_t)b * c.S + j] = scale * edge_prob(c, s0, pk[j]) * (float)deg;
+      // already picked.  Up to 64 picks: lane i draws pick i's t (all draws at once), then
+      // the picks are settled in order, each membership test one ballot over the lanes
+      // already settled.  More picks: one at a time, kept in LDS.
+      if (c.S <= 64u) {
+        const uint32_t hi = deg - c.S + (uint32_t)lane;
+        const uint32_t t = (uint32_t)lane < c.S
+                               ? uniform_below(draw_word(pos, pass, RNG_NEIGHBOURS, (uint32_t)lane, k0, k1), hi + 1u)
+                               : 0u;
+        uint32_t pick = t;
+        for (uint32_t i = 1; i < c.S; ++i) {
+          const uint32_t ti = (uint32_t)__builtin_amdgcn_readlane((int)t, (int)i);
+          const bool dup = __ballot((uint32_t)lane < i && pick == ti) != 0ull;
+          if ((uint32_t)lane == i && dup) pick = deg - c.S + i;
+        }
+        if ((uint32_t)lane < c.S) {
+          out[lane] = c.indices[s0 + pick];
+          if (c.wavg) c.nbr_w[(size_t)b * c.S + lane] = (float)deg / (float)c.S * edge_prob(c, s0, pick) * (float)deg;
+        }
+      } else {
+        extern __shared__ uint32_t s_pick[];
+        uint32_t* pk = s_pick + (threadIdx.x >> 6) * c.S;
+        for (uint32_t i = 0; i < c.S; ++i) {
+          const uint32_t hi = deg - c.S + i;
+          const uint32_t t = uniform_below(draw_word(pos, pass, RNG_NEIGHBOURS, i, k0, k1), hi + 1u);
+          bool dup = false;
+          for (uint32_t q = lane; q < i; q += 64u) dup |= (pk[q] == t);
+          const bool any = wave_any(dup);
+          if (lane == 0) pk[i] = any ? hi : t;
+          __builtin_amdgcn_wave_barrier();
+        }
+        for (uint32_t j = lane; j < c.S; j += 64u) out[j] = c.indices[s0 + pk[j]];
+        if (c.wavg) {
+          const float scale = (float)deg / (float)c.S;
+          for (uint32_t j = lane; j < c.S; j += 64u)
+            c.nbr_w[(size_t)b * c.S + j] = scale * edge_prob(c, s0, pk[j]) * (float)deg;
+        }
       }
     }
     sample_negatives(c, v, s0, deg, pos, pass, k0, k1, negs, lane);
