@@ -301,7 +301,25 @@ template <int CTRL, int ROW_MASK>
 __device__ inline float mov(float x) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROW_MASK, 0xf, true));
 }
+template <int CTRL, int ROW_MASK>
+__device__ inline uint32_t movu(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROW_MASK, 0xf, true);
+}
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
 }  // namespace dpp
+
+// Inclusive prefix sum over the wave: shifts by 1, 2, 4 and 8 within each 16-lane row
+// (lanes shifted in from outside the row read 0), then the row broadcasts of lanes 15 and
+// 31 carry the rows' totals up.  Lane 63 ends with the wave's total.
+__device__ inline uint32_t wave_inclusive_scan_u32(uint32_t x) {
+  x += dpp::movu<dpp::kRowShr1, 0xf>(x);
+  x += dpp::movu<dpp::kRowShr2, 0xf>(x);
+  x += dpp::movu<dpp::kRowShr4, 0xf>(x);
+  x += dpp::movu<dpp::kRowShr8, 0xf>(x);
+  x += dpp::movu<dpp::kRowBcast15, 0xa>(x);
+  x += dpp::movu<dpp::kRowBcast31, 0xc>(x);
+  return x;
+}
 
 // Sum over the wave, returned to every lane (the order differs from wave_sum's).
 __device__ inline float wave_sum_dpp(float x) {

@@ -141,12 +141,7 @@ __device__ inline void gather_rows(const T* __restrict__ table, const uint32_t* 
 // 256-thread block exclusive scan of one value per thread; returns the block total in *total.
 __device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t* s_wave, uint32_t* total) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  uint32_t inc = x;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t y = __shfl_up(inc, off, EP_WAVE);
-    if (lane >= off) inc += y;
-  }
+  const uint32_t inc = wave_inclusive_scan_u32(x);
   if (lane == 63) s_wave[w] = inc;
   __syncthreads();
   uint32_t pre = 0, tot = 0;
